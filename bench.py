@@ -1,0 +1,123 @@
+"""Benchmark of the MI355X time-advance hot path.
+
+Workload (default): the single curvilinear block 512x256x128, FP64, with
+3-D GCL metrics, an isentropic vortex (strength 5, radius 0.5) advected at
+Mach 0.5, 4-stage low-storage RK, 5-pt viscous terms in the last stage,
+shock-capturing filter with Ducros sensor (quiescent), periodic faces.
+
+One "step" = one full RK time step (4 stages + filter) of that block, all
+inputs resident in HBM.  Metric: Mcell-RK-stages/s = cells * nrk * steps / t.
+
+Prints ONE JSON line.  Usage:
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def stage_words_per_cell(case) -> float:
+    """Algorithmic FP64 words per interior cell of one stage-kernel launch,
+    averaged over the nrk stages: stage 1 reads U^n and writes U1 (10 W),
+    later stages also read U^n (15 W); curvilinear metrics add 10 W per stage."""
+    w = [10.0] + [15.0] * (case.nrk - 1)
+    if case.curvilinear:
+        w = [x + 10.0 for x in w]
+    return sum(w) / len(w)
+
+
+def cpu_baseline(case, nsteps: int):
+    """Time the CPU oracle on the host cores for a bounded sample."""
+    from oracle.oracle import Oracle, num_threads
+    o = Oracle(case)
+    U = case.copy_state()
+    o.step(U, 1)  # first-touch of its work arrays
+    U = case.copy_state()
+    t0 = time.perf_counter()
+    o.step(U, nsteps)
+    t = time.perf_counter() - t0
+    val = case.ncells * case.nrk * nsteps / t / 1e6
+    return {"value": round(val, 3), "unit": "Mcell-RK-stages/s", "cores": num_threads(),
+            "kind": "port",
+            "sample": f"{nsteps} full RK steps of the same 512x256x128 block, oracle/musica_oracle.c, "
+                      f"OpenMP over k, {t:.2f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--nx", type=int, default=512)
+    ap.add_argument("--ny", type=int, default=256)
+    ap.add_argument("--nz", type=int, default=128)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        return 2
+
+    from musica2_newgrid_amd import _lib, cases
+    from musica2_newgrid_amd.solver import GpuSolver
+
+    case = cases.vortex_wavy(args.nx, args.ny, args.nz, filter=2)
+    solver = GpuSolver(case)
+    solver.step(args.warmup)
+    solver.sync()
+
+    solver.profile(True)
+    solver.sync()
+    t0 = time.perf_counter()
+    solver.step(args.steps)
+    solver.sync()
+    t = time.perf_counter() - t0
+
+    nst, ms_stage = solver.kernel_time(_lib.K_STAGE)
+    times = {_lib.K_NAMES[k]: solver.kernel_time(k) for k in _lib.K_NAMES}
+    solver.profile(False)
+    solver.close()
+
+    value = case.ncells * case.nrk * args.steps / t / 1e6
+    avg_stage_s = ms_stage / nst / 1e3
+    bytes_launch = stage_words_per_cell(case) * 8.0 * case.ncells
+    achieved = bytes_launch / avg_stage_s / 1e9
+    out = {
+        "metric": "Mcell-RK-stages/s",
+        "value": round(value, 2),
+        "unit": "Mcell-RK-stages/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"curvilinear block {args.nx}x{args.ny}x{args.nz}, isentropic vortex "
+                               "M=0.5, viscous, shock filter + Ducros, RK4, FP64",
+                   "cells": case.ncells, "nrk": case.nrk, "precision": "fp64"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_stage", "avg_launch_ms": round(avg_stage_s * 1e3, 4),
+                     "algorithmic_bytes_per_launch": bytes_launch},
+        "kernel_ms": {k: {"launches": n, "ms": round(ms, 3)} for k, (n, ms) in times.items()},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

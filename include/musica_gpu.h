@@ -1,0 +1,156 @@
+/*
+ * musica_gpu.h -- C-ABI of the MI355X (gfx950) time-advance library
+ * libmusica_gpu.so.
+ *
+ * The reference (LouenasZm/musica2_newGrid, Fortran) advances one explicit
+ * low-storage Runge-Kutta step through a table of no-argument procedure
+ * pointers bound at start-up:
+ *   - pointer table   src/Main/mod_interface.f90:10-13,63-166
+ *   - binding         src/Main/assign_procedures.f90:109-411
+ *   - the step        src/Compute_flow/runge_kutta.f90:2-152, called from
+ *                     src/Compute_flow/solver.f90:415-417
+ * All data flows through module globals (src/Compute_flow/mod_flow.f90:13-41,
+ * src/Grid/mod_grid.f90:131-145).  This header replaces that seam with one
+ * handle that owns the device copy of those globals and advances whole steps,
+ * so that the Euler, viscous, filter and halo passes can be fused on the GPU.
+ *
+ * Conventions
+ *   - plain pointers and sizes only; every host array is borrowed for the
+ *     duration of the call;
+ *   - host arrays are the reference's Fortran module arrays: one contiguous
+ *     array per variable, i fastest, extents (1-5:nx+5, 1-5:ny+5, 1-5:nz+5)
+ *     (src/Main/alloc.f90:116-120, ngh=5 from src/File_IO/read_param.f90:495);
+ *   - return 0 on success, a negative MGPU_E* code on error; the message is
+ *     available from mgpu_last_error().  The Fortran host maps an error to
+ *     mpistop(msg,1) (src/Utils/warnstop.f90:20-90);
+ *   - one host thread per handle; handles are independent.
+ */
+#ifndef MUSICA_GPU_H
+#define MUSICA_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MGPU_NGH 5 /* ghost layers of state and metric arrays */
+
+/* error codes */
+#define MGPU_OK 0
+#define MGPU_EINVAL -1   /* invalid argument or case description        */
+#define MGPU_EDEVICE -2  /* HIP runtime error                           */
+#define MGPU_ENOMEM -3   /* device allocation failed                    */
+#define MGPU_EUNSUP -4   /* feature combination not supported           */
+#define MGPU_ESTATE -5   /* call out of order (e.g. step before metrics) */
+
+/* Face kinds: restate BC_face(dir,side)%sort (src/Boundary/mod_bc.f90:7-61)
+ * for the kinds this library advances.  A periodic face is the reference's
+ * neighbour-is-itself interface (src/Parallel/mod_mpi_part.f90:333-). */
+#define MGPU_FACE_PERIODIC 0
+#define MGPU_FACE_INTERFACE 1
+
+/* Filter selection: is_SF / is_shock (src/Main/assign_procedures.f90:161-219) */
+#define MGPU_FILTER_NONE 0
+#define MGPU_FILTER_SF 1    /* filtering_11pts, mod_filtering.f90:894      */
+#define MGPU_FILTER_SHOCK 2 /* filter_o10_shock, mod_filtering_shock.f90:204 */
+
+/* Arithmetic: FP64 everywhere, or FP32 flux/derivative/filter arithmetic with
+ * FP64 RK accumulation, metrics and shock sensor. */
+#define MGPU_PREC_FP64 0
+#define MGPU_PREC_MIXED 1
+
+/* Faces are numbered imin,imax,jmin,jmax,kmin,kmax = 0..5. */
+typedef struct mgpu_block {
+  int32_t nx, ny, nz;     /* interior cells of the block                     */
+  int32_t face_kind[6];   /* MGPU_FACE_*                                     */
+  int32_t nbr_block[6];   /* neighbour block of an interface face            */
+  int32_t nbr_face[6];    /* neighbour face (0..5) of an interface face       */
+  int32_t orient[6];      /* interface orientation code (0 = aligned)         */
+} mgpu_block;
+
+/* Case description: the param.ini flags of the hot path plus the block table
+ * (param_blocks.ini, src/Main/mod_block.f90:109-229). */
+typedef struct mgpu_case {
+  int32_t nblocks;
+  int32_t curvilinear;    /* 0: Cartesian 1-D metrics (idx,idy,idz); 1: full 3-D (is_curv3) */
+  int32_t iorder_visc;    /* 0 (Euler) or 4 (5-pt viscous terms)               */
+  int32_t filter;         /* MGPU_FILTER_*                                     */
+  int32_t is_ducros;      /* Ducros sensor in the shock filter                 */
+  int32_t nrk;            /* RK stages (mod_time.f90:121)                       */
+  int32_t precision;      /* MGPU_PREC_*                                       */
+  int32_t reserved;
+  const mgpu_block *blocks;
+  double crk[8];          /* RK stage coefficients crk(1..nrk)                 */
+  double deltat;          /* time step                                         */
+  double gam, rg;         /* gamma, gas constant                               */
+  double c0, S0;          /* Sutherland: mu = c0*T^1.5/(T+S0)*diffscale        */
+  double diffscale;       /* viscosity rescaling (setupref.f90:357)            */
+  double cok_ratio;       /* cp/Pr: conductivity = cok_ratio*mu                */
+  double dissip_coeff;    /* filter amplitude chi                              */
+  double dissip_shock;    /* shock-capturing amplitude                         */
+  double csens;           /* pressure-sensor blend                             */
+} mgpu_case;
+
+typedef struct mgpu_handle mgpu_handle;
+
+/* Create a handle for one case on the current HIP device: allocates the
+ * device state (3 state sets: U^n and two stage buffers), metrics and
+ * scratch.  Replaces alloc_tab + assign_procedures for the hot path
+ * (src/Main/alloc.f90:2-295, src/Main/assign_procedures.f90:2-979). */
+int mgpu_create(const mgpu_case *c, mgpu_handle **out);
+
+/* Cartesian metrics of a block: idx, idy, idz (Euler, 11-pt) and idx_v,
+ * idy_v, idz_v (viscous, 5-pt); each of length n+2*MGPU_NGH, ghosts
+ * included (src/Grid/mod_grid.f90:60-64, grid_metrics_cart.f90:2-). */
+int mgpu_set_metrics_cart(mgpu_handle *h, int block, const double *idx, const double *idy,
+                          const double *idz, const double *idx_v, const double *idy_v,
+                          const double *idz_v);
+
+/* Curvilinear metrics of a block: m[0..9] = ksi_x,ksi_y,ksi_z,eta_x,eta_y,
+ * eta_z,phi_x,phi_y,phi_z,ijacob3 and mv[0..9] the same for the viscous set
+ * (*_v), each a padded 3-D array with ghosts
+ * (src/Grid/mod_grid.f90:131-145, smod_grid_metrics1_c3.f90:8-484). */
+int mgpu_set_metrics_c3(mgpu_handle *h, int block, const double *const *m, const double *const *mv);
+
+/* Upload the conserved variables rho,rhou,rhov,rhow,rhoe of a block (padded
+ * arrays; ghosts are refreshed on the device). */
+int mgpu_put_state(mgpu_handle *h, int block, const double *rho, const double *rhou,
+                   const double *rhov, const double *rhow, const double *rhoe);
+
+/* Download the conserved variables of a block (padded arrays, ghosts valid). */
+int mgpu_get_state(mgpu_handle *h, int block, double *rho, double *rhou, double *rhov,
+                   double *rhow, double *rhoe);
+
+/* Advance nsteps full time steps: nrk stages (viscous terms in the last),
+ * then the filter, with device-side halo/interface copies in between
+ * (runge_kutta, src/Compute_flow/runge_kutta.f90:2-152).  Asynchronous on
+ * the handle's stream. */
+int mgpu_step(mgpu_handle *h, int nsteps);
+
+/* Wait for all work queued on the handle. */
+int mgpu_sync(mgpu_handle *h);
+
+/* Per-kernel timing with HIP events on the handle's stream.  enable=1 starts
+ * recording (and clears the totals); mgpu_kernel_time returns, for kernel
+ * class kind (MGPU_K_*), the number of launches and their summed duration
+ * in milliseconds. */
+#define MGPU_K_STAGE 0      /* Euler (+viscous divergence) stage + RK update */
+#define MGPU_K_VISC 1       /* viscous fluxes of U^n                         */
+#define MGPU_K_FILTER 2     /* filter + update_varn                          */
+#define MGPU_K_HALO 3       /* halo / interface copies                       */
+#define MGPU_K_COUNT 4
+int mgpu_profile(mgpu_handle *h, int enable);
+int mgpu_kernel_time(mgpu_handle *h, int kind, int64_t *launches, double *ms);
+
+/* Message of the last error on h (or of the last failed mgpu_create when h
+ * is NULL). */
+const char *mgpu_last_error(const mgpu_handle *h);
+
+/* Release every device resource of the handle. */
+void mgpu_destroy(mgpu_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

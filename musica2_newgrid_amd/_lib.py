@@ -1,0 +1,119 @@
+"""ctypes binding of libmusica_gpu.so (the C-ABI declared in include/musica_gpu.h).
+
+The library is the product: every call here goes to the HIP kernels.  There is
+no CPU fallback -- if the shared library is missing or fails to load, the
+import of :func:`lib` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmusica_gpu.so")
+
+MGPU_NGH = 5
+FACE_PERIODIC = 0
+FACE_INTERFACE = 1
+FILTER_NONE, FILTER_SF, FILTER_SHOCK = 0, 1, 2
+PREC_FP64, PREC_MIXED = 0, 1
+K_STAGE, K_VISC, K_FILTER, K_HALO = 0, 1, 2, 3
+K_NAMES = {K_STAGE: "stage", K_VISC: "visc", K_FILTER: "filter", K_HALO: "halo"}
+
+# every symbol include/musica_gpu.h declares
+EXPORTS = (
+    "mgpu_create",
+    "mgpu_set_metrics_cart",
+    "mgpu_set_metrics_c3",
+    "mgpu_put_state",
+    "mgpu_get_state",
+    "mgpu_step",
+    "mgpu_sync",
+    "mgpu_profile",
+    "mgpu_kernel_time",
+    "mgpu_last_error",
+    "mgpu_destroy",
+)
+
+
+class MgpuBlock(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("face_kind", C.c_int32 * 6),
+        ("nbr_block", C.c_int32 * 6),
+        ("nbr_face", C.c_int32 * 6),
+        ("orient", C.c_int32 * 6),
+    ]
+
+
+class MgpuCase(C.Structure):
+    _fields_ = [
+        ("nblocks", C.c_int32),
+        ("curvilinear", C.c_int32),
+        ("iorder_visc", C.c_int32),
+        ("filter", C.c_int32),
+        ("is_ducros", C.c_int32),
+        ("nrk", C.c_int32),
+        ("precision", C.c_int32),
+        ("reserved", C.c_int32),
+        ("blocks", C.POINTER(MgpuBlock)),
+        ("crk", C.c_double * 8),
+        ("deltat", C.c_double),
+        ("gam", C.c_double), ("rg", C.c_double),
+        ("c0", C.c_double), ("S0", C.c_double),
+        ("diffscale", C.c_double),
+        ("cok_ratio", C.c_double),
+        ("dissip_coeff", C.c_double),
+        ("dissip_shock", C.c_double),
+        ("csens", C.c_double),
+    ]
+
+
+class MgpuError(RuntimeError):
+    pass
+
+
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+
+
+def lib() -> C.CDLL:
+    """Load libmusica_gpu.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MgpuError(
+            f"{LIB_PATH} is not built: run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    L = C.CDLL(LIB_PATH)
+    L.mgpu_create.argtypes = [C.POINTER(MgpuCase), C.POINTER(C.c_void_p)]
+    L.mgpu_set_metrics_cart.argtypes = [C.c_void_p, C.c_int] + [_dp] * 6
+    L.mgpu_set_metrics_c3.argtypes = [C.c_void_p, C.c_int, C.POINTER(_dp), C.POINTER(_dp)]
+    L.mgpu_put_state.argtypes = [C.c_void_p, C.c_int] + [_dp] * 5
+    L.mgpu_get_state.argtypes = [C.c_void_p, C.c_int] + [_dp] * 5
+    L.mgpu_step.argtypes = [C.c_void_p, C.c_int]
+    L.mgpu_sync.argtypes = [C.c_void_p]
+    L.mgpu_profile.argtypes = [C.c_void_p, C.c_int]
+    L.mgpu_kernel_time.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    L.mgpu_last_error.argtypes = [C.c_void_p]
+    L.mgpu_last_error.restype = C.c_char_p
+    L.mgpu_destroy.argtypes = [C.c_void_p]
+    L.mgpu_destroy.restype = None
+    for name in ("mgpu_create", "mgpu_set_metrics_cart", "mgpu_set_metrics_c3", "mgpu_put_state",
+                 "mgpu_get_state", "mgpu_step", "mgpu_sync", "mgpu_profile", "mgpu_kernel_time"):
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(rc: int, handle=None, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().mgpu_last_error(handle).decode(errors="replace")
+        raise MgpuError(f"{what} failed (rc={rc}): {msg}")
+
+
+def dptr(a):
+    """double* of a C-contiguous float64 numpy array."""
+    assert a.dtype.name == "float64" and a.flags["C_CONTIGUOUS"], "need contiguous float64"
+    return a.ctypes.data_as(_dp)

@@ -1,0 +1,207 @@
+"""Case descriptions and synthetic workloads (set-up, host side).
+
+A :class:`Case` carries what the reference reads from param.ini /
+param_blocks.ini / feos_air.ini for the hot path, the grid metrics, the time
+step and the initial conserved state.  Builders:
+
+* :func:`tgv`            -- Taylor-Green vortex on a periodic Cartesian cube
+                            (IC: add_tgv, src/Initial_condition/mod_init_flow.f90:577-620;
+                            reference quantities: src/Main/setupref.f90:39-69,258-266,357)
+* :func:`vortex_wavy`    -- isentropic vortex advected over a curvilinear
+                            (wavy, 3-D GCL metrics) block, periodic faces
+* :func:`entropy_wave`   -- small-amplitude entropy/shear waves for closed-form tests
+
+The time step follows timestep_new (src/Time/timestep_new.f90:270-288):
+deltat = CFL * delta_min / max(|u| + c).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional
+
+import numpy as np
+
+from . import grid as gr
+from .grid import G
+
+# air (harness-authored feos_air.ini) and Sutherland constants of
+# init_viscosity (src/Fluid/smod_tranprop_init.f90:33-40)
+GAM = 1.4
+RG = 287.0
+PR = 0.71
+C0 = 1.4579326545176254e-06
+S0 = 110.4
+# low-storage RK4 (init_RK nrk=4, src/Time/mod_time.f90:165-169)
+CRK4 = (0.25, 1.0 / 3.0, 0.5, 1.0)
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    nx: int
+    ny: int
+    nz: int
+    curvilinear: bool
+    iorder_visc: int = 4
+    filter: int = 1            # 0 none, 1 selective filter, 2 shock filter
+    ducros: bool = True
+    nrk: int = 4
+    crk: tuple = CRK4
+    dt: float = 0.0
+    gam: float = GAM
+    rg: float = RG
+    c0: float = C0
+    S0: float = S0
+    diffscale: float = 1.0
+    cok_ratio: float = GAM * RG / (GAM - 1.0) / PR
+    chi: float = 0.2           # dissip_coeff
+    dissip_shock: float = 0.1
+    csens: float = 0.5
+    # Cartesian metrics: idx, idy, idz, idx_v, idy_v, idz_v (1-D, length n+2G)
+    cart: Optional[List[np.ndarray]] = None
+    # curvilinear metrics: 10 padded arrays each (ksi_x..phi_z, ijacob3)
+    m: Optional[List[np.ndarray]] = None
+    mv: Optional[List[np.ndarray]] = None
+    # initial conserved state: rho, rhou, rhov, rhow, rhoe (padded)
+    U: Optional[List[np.ndarray]] = None
+    info: dict = dataclasses.field(default_factory=dict)
+
+    @property
+    def ncells(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    def copy_state(self) -> List[np.ndarray]:
+        return [a.copy() for a in self.U]
+
+
+def sutherland(T):
+    return C0 * np.sqrt(T * T * T) / (T + S0)
+
+
+def conserved(rho, u, v, w, p, gam=GAM):
+    return [rho, rho * u, rho * v, rho * w, p / (gam - 1.0) + 0.5 * rho * (u * u + v * v + w * w)]
+
+
+def _to_padded(fields, nx, ny, nz):
+    out = []
+    for f in fields:
+        a = gr.padded(nx, ny, nz)
+        a[G:-G, G:-G, G:-G] = f
+        gr.fill_periodic(a)
+        out.append(a)
+    return out
+
+
+def timestep(U, delta_min, cfl, gam=GAM, rg=RG):
+    """deltat = CFL*delta_min/max(|u|+c) over the interior (timestep_new.f90:256-288)."""
+    rho = gr.interior(U[0])
+    u, v, w = (gr.interior(U[q]) / rho for q in (1, 2, 3))
+    p = (gam - 1.0) * (gr.interior(U[4]) - 0.5 * rho * (u * u + v * v + w * w))
+    c = np.sqrt(gam * p / rho)
+    umax = float(np.max(np.sqrt(u * u + v * v + w * w) + c))
+    return cfl * delta_min / umax
+
+
+def tgv(n: int = 64, mach: float = 0.1, re: float = 1600.0, cfl: float = 0.8, chi: float = 0.2,
+        filter: int = 1, iorder_visc: int = 4, T_ref: float = 300.0, rho_ref: float = 1.177) -> Case:
+    """3-D Taylor-Green vortex on [0,2pi)^3, Cartesian, periodic (config: 64^3, M=0.1, Re=1600)."""
+    L = 2.0 * math.pi
+    dx = L / n
+    p_ref = rho_ref * RG * T_ref
+    c_ref = math.sqrt(GAM * RG * T_ref)
+    u_ref = mach * c_ref
+    mu_ref = rho_ref * u_ref * 1.0 / re           # setupref.f90:265-266 (L=1)
+    muw_ref = float(sutherland(np.float64(T_ref)))
+    x = np.arange(n) * dx
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    u = u_ref * np.sin(X) * np.cos(Y) * np.cos(Z)
+    v = -u_ref * np.cos(X) * np.sin(Y) * np.cos(Z)
+    w = np.zeros_like(u)
+    p = p_ref + rho_ref * u_ref ** 2 / 16.0 * (np.cos(2.0 * Z) + 2.0) * (np.cos(2.0 * X) + np.cos(2.0 * Y))
+    T = p / p_ref * T_ref
+    rho = p / (RG * T)
+    U = _to_padded(conserved(rho, u, v, w, p), n, n, n)
+    idx = np.full(n + 2 * G, 1.0 / dx)
+    c = Case("tgv", n, n, n, curvilinear=False, iorder_visc=iorder_visc, filter=filter, ducros=False,
+             diffscale=mu_ref / muw_ref, chi=chi,
+             cart=[idx.copy() for _ in range(6)], U=U)
+    c.dt = timestep(U, dx, cfl)
+    c.info = dict(mach=mach, re=re, cfl=cfl, u_ref=u_ref, p_ref=p_ref, rho_ref=rho_ref, dx=dx, L=L)
+    return c
+
+
+def wavy_mapping(lengths, amp):
+    """x = xi + ax sin(2pi eta/Ly) sin(2pi zeta/Lz), and cyclic: a smooth,
+    fully periodic curvilinear distortion of a box."""
+    Lx, Ly, Lz = lengths
+    ax, ay, az = amp
+    tp = 2.0 * math.pi
+
+    def f(XI, ET, ZE):
+        x = XI + ax * np.sin(tp * ET / Ly) * np.sin(tp * ZE / Lz)
+        y = ET + ay * np.sin(tp * XI / Lx) * np.sin(tp * ZE / Lz)
+        z = ZE + az * np.sin(tp * XI / Lx) * np.sin(tp * ET / Ly)
+        return x, y, z
+
+    return f
+
+
+def vortex_wavy(nx: int = 512, ny: int = 256, nz: int = 128, mach: float = 0.5, beta: float = 5.0,
+                radius: float = 0.5, re: float = 1.0e4, cfl: float = 0.8, filter: int = 2,
+                lengths=None, amp=None, T_inf: float = 300.0, rho_inf: float = 1.177,
+                iorder_visc: int = 4) -> Case:
+    """Isentropic vortex (strength beta, radius R) advected at Mach `mach` along x
+    over a curvilinear block with 3-D GCL metrics and periodic faces."""
+    if lengths is None:
+        h = 10.0 / nx
+        lengths = (nx * h, ny * h, nz * h)
+    Lx, Ly, Lz = lengths
+    if amp is None:
+        amp = (0.02 * Lx / 2, 0.05 * Ly / 2, 0.02 * Lz / 2)
+    mapping = wavy_mapping(lengths, amp)
+    m, mv, (xc, yc, zc) = gr.curvilinear_metrics_periodic(mapping, nx, ny, nz, lengths,
+                                                            viscous=iorder_visc != 0)
+    c_inf = math.sqrt(GAM * RG * T_inf)
+    u_inf = mach * c_inf
+    xv, yv = Lx / 2.0, Ly / 2.0
+    r2 = ((xc - xv) ** 2 + (yc - yv) ** 2) / radius ** 2
+    e = np.exp(0.5 * (1.0 - r2))
+    u = u_inf * (1.0 - beta / (2.0 * math.pi) * (yc - yv) / radius * e)
+    v = u_inf * (beta / (2.0 * math.pi) * (xc - xv) / radius * e)
+    w = np.zeros_like(u)
+    T = T_inf * (1.0 - (GAM - 1.0) * beta ** 2 * mach ** 2 / (8.0 * math.pi ** 2) * e * e)
+    rho = rho_inf * (T / T_inf) ** (1.0 / (GAM - 1.0))
+    p = rho * RG * T
+    U = _to_padded(conserved(rho, u, v, w, p), nx, ny, nz)
+    mu_ref = rho_inf * u_inf * radius / re
+    diffscale = mu_ref / float(sutherland(np.float64(T_inf)))
+    # minimum edge length between neighbouring nodes (timestep_new.f90:40-200)
+    dmin = min(float(np.min(np.sqrt(np.diff(xc, axis=a) ** 2 + np.diff(yc, axis=a) ** 2 +
+                                    np.diff(zc, axis=a) ** 2))) for a in (0, 1, 2))
+    cs = Case("vortex_wavy", nx, ny, nz, curvilinear=True, iorder_visc=iorder_visc, filter=filter,
+              ducros=True, diffscale=diffscale, m=m, mv=mv, U=U)
+    cs.dt = timestep(U, dmin, cfl)
+    cs.info = dict(mach=mach, beta=beta, radius=radius, re=re, cfl=cfl, lengths=lengths, amp=amp,
+                   dmin=dmin)
+    return cs
+
+
+def cartesian_box(nx, ny, nz, lengths, fields, iorder_visc=0, filter=0, diffscale=1.0, dt=1e-6,
+                  chi=0.2) -> Case:
+    """Generic periodic Cartesian case from primitive fields (rho,u,v,w,p) on the interior."""
+    Lx, Ly, Lz = lengths
+    cart = [np.full(nx + 2 * G, nx / Lx), np.full(ny + 2 * G, ny / Ly), np.full(nz + 2 * G, nz / Lz)]
+    cart = cart + [a.copy() for a in cart]
+    U = _to_padded(conserved(*fields), nx, ny, nz)
+    return Case("box", nx, ny, nz, curvilinear=False, iorder_visc=iorder_visc, filter=filter,
+                ducros=False, diffscale=diffscale, cart=cart, U=U, dt=dt, chi=chi)
+
+
+def affine_c3_metrics(nx, ny, nz, lengths, viscous=True):
+    """c3 metrics of an undistorted box: ksi_x = dy*dz etc., 1/J = 1/(dx dy dz)."""
+    dx, dy, dz = (L / n for L, n in zip(lengths, (nx, ny, nz)))
+    vals = [dy * dz, 0, 0, 0, dx * dz, 0, 0, 0, dx * dy, 1.0 / (dx * dy * dz)]
+    m = [gr.padded(nx, ny, nz, float(v)) for v in vals]
+    mv = [gr.padded(nx, ny, nz, float(v)) for v in vals] if viscous else None
+    return m, mv

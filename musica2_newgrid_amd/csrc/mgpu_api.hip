@@ -1,0 +1,474 @@
+// C-ABI of libmusica_gpu.so: handle, device memory, and the RK step
+// choreography of runge_kutta (src/Compute_flow/runge_kutta.f90:2-152).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mgpu_internal.h"
+
+using namespace mgpu;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct BlockData {
+  Geom g{};
+  double *st[3][5] = {};  // three state sets: st[cur] = U^n, the others stage buffers
+  double *idx[6] = {};    // Cartesian metrics idx,idy,idz,idx_v,idy_v,idz_v
+  double *m[10] = {};
+  double *mv[10] = {};
+  double *fv[12] = {};    // viscous fluxes (last stage)
+  double *prs = nullptr, *sens = nullptr;
+  bool have_metrics = false;
+  bool have_state = false;
+};
+
+struct ProfRec {
+  int kind;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct mgpu_handle {
+  mgpu_case c{};
+  std::vector<mgpu_block> blocks;
+  std::vector<BlockData> b;
+  int cur = 0;  // index of the set holding U^n
+  hipStream_t stream = nullptr;
+  int device = 0;
+  std::string err;
+  bool profiling = false;
+  std::vector<ProfRec> recs;
+  std::vector<hipEvent_t> ev_pool;
+  int64_t kcount[MGPU_K_COUNT] = {};
+  double kms[MGPU_K_COUNT] = {};
+};
+
+namespace {
+
+int fail(mgpu_handle *h, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (h) h->err = buf;
+  else g_create_error = buf;
+  return code;
+}
+
+#define HIPCK(h, call)                                                                        \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail((h), MGPU_EDEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),   \
+                  __FILE__, __LINE__);                                                        \
+  } while (0)
+
+hipEvent_t get_event(mgpu_handle *h) {
+  if (!h->ev_pool.empty()) {
+    hipEvent_t e = h->ev_pool.back();
+    h->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one launch with events when profiling is on.
+struct ProfScope {
+  mgpu_handle *h;
+  int kind;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(mgpu_handle *h_, int k) : h(h_), kind(k) {
+    if (h->profiling) {
+      a = get_event(h);
+      b = get_event(h);
+      if (a) (void)hipEventRecord(a, h->stream);
+    }
+  }
+  ~ProfScope() {
+    if (h->profiling && a && b) {
+      (void)hipEventRecord(b, h->stream);
+      h->recs.push_back({kind, a, b});
+    }
+  }
+};
+
+int drain_profile(mgpu_handle *h) {
+  if (h->recs.empty()) return 0;
+  HIPCK(h, hipStreamSynchronize(h->stream));
+  for (auto &r : h->recs) {
+    float ms = 0.f;
+    HIPCK(h, hipEventElapsedTime(&ms, r.a, r.b));
+    h->kcount[r.kind] += 1;
+    h->kms[r.kind] += ms;
+    h->ev_pool.push_back(r.a);
+    h->ev_pool.push_back(r.b);
+  }
+  h->recs.clear();
+  return 0;
+}
+
+Gas gas_of(const mgpu_case &c) {
+  Gas gs;
+  gs.gam = c.gam;
+  gs.gam1 = c.gam - 1.0;
+  gs.rg = c.rg;
+  gs.c0 = c.c0;
+  gs.S0 = c.S0;
+  gs.diffscale = c.diffscale;
+  gs.cok_ratio = c.cok_ratio;
+  return gs;
+}
+
+Metrics metrics_of(const BlockData &bd) {
+  Metrics m{};
+  m.idx = bd.idx[0];
+  m.idy = bd.idx[1];
+  m.idz = bd.idx[2];
+  m.idx_v = bd.idx[3];
+  m.idy_v = bd.idx[4];
+  m.idz_v = bd.idx[5];
+  for (int q = 0; q < 10; q++) {
+    m.m[q] = bd.m[q];
+    m.mv[q] = bd.mv[q];
+  }
+  return m;
+}
+
+int halo(mgpu_handle *h, BlockData &bd, double *const *set) {
+  ProfScope ps(h, MGPU_K_HALO);
+  HaloArgs ha{};
+  ha.g = bd.g;
+  for (int v = 0; v < 5; v++) ha.u.v[v] = set[v];
+  HIPCK(h, launch_halo_periodic(ha, 5, h->stream));
+  return 0;
+}
+
+int kchunk_for(const Geom &g) {
+  const long nxy = (long)((g.nx + 63) / 64) * ((g.ny + 3) / 4);
+  long want = (4096 + nxy - 1) / nxy;  // aim at >= 4096 workgroups
+  if (want < 1) want = 1;
+  long kc = (g.nz + want - 1) / want;
+  if (kc < 8) kc = 8;
+  if (kc > g.nz) kc = g.nz;
+  return (int)kc;
+}
+
+void free_block(BlockData &bd) {
+  for (auto &s : bd.st)
+    for (auto &p : s) { if (p) (void)hipFree(p); p = nullptr; }
+  for (auto &p : bd.idx) { if (p) (void)hipFree(p); p = nullptr; }
+  for (auto &p : bd.m) { if (p) (void)hipFree(p); p = nullptr; }
+  for (auto &p : bd.mv) { if (p) (void)hipFree(p); p = nullptr; }
+  for (auto &p : bd.fv) { if (p) (void)hipFree(p); p = nullptr; }
+  if (bd.prs) (void)hipFree(bd.prs);
+  if (bd.sens) (void)hipFree(bd.sens);
+  bd.prs = bd.sens = nullptr;
+}
+
+int dalloc(mgpu_handle *h, double **p, size_t count) {
+  hipError_t e = hipMalloc((void **)p, count * sizeof(double));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(h, MGPU_ENOMEM, "hipMalloc of %zu doubles failed: %s", count, hipGetErrorString(e));
+  }
+  e = hipMemset(*p, 0, count * sizeof(double));
+  if (e != hipSuccess) return fail(h, MGPU_EDEVICE, "hipMemset failed: %s", hipGetErrorString(e));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
+  if (!c || !out) return fail(nullptr, MGPU_EINVAL, "null argument");
+  *out = nullptr;
+  if (c->nblocks < 1 || !c->blocks) return fail(nullptr, MGPU_EINVAL, "nblocks must be >= 1");
+  if (c->nrk < 1 || c->nrk > 8) return fail(nullptr, MGPU_EINVAL, "nrk=%d out of range 1..8", c->nrk);
+  if (c->iorder_visc != 0 && c->iorder_visc != 4)
+    return fail(nullptr, MGPU_EUNSUP, "iorder_visc=%d: only 0 and 4 are supported", c->iorder_visc);
+  if (c->filter < 0 || c->filter > 2) return fail(nullptr, MGPU_EINVAL, "filter=%d", c->filter);
+  if (c->filter == MGPU_FILTER_SHOCK && c->iorder_visc == 0)
+    return fail(nullptr, MGPU_EUNSUP,
+                "shock filter needs iorder_visc=4 (its sensor reads the viscous gradients of U^n)");
+  if (c->precision != MGPU_PREC_FP64)
+    return fail(nullptr, MGPU_EUNSUP, "precision=%d not supported yet", c->precision);
+  if (c->nblocks != 1) return fail(nullptr, MGPU_EUNSUP, "multiblock cases not supported yet");
+  for (int bI = 0; bI < c->nblocks; bI++) {
+    const mgpu_block &bl = c->blocks[bI];
+    if (bl.nx < G || bl.ny < G || bl.nz < G)
+      return fail(nullptr, MGPU_EINVAL, "block %d: every extent must be >= %d", bI, G);
+    for (int f = 0; f < 6; f++)
+      if (bl.face_kind[f] != MGPU_FACE_PERIODIC)
+        return fail(nullptr, MGPU_EUNSUP, "block %d face %d: only periodic faces supported yet", bI, f);
+  }
+
+  mgpu_handle *h = new mgpu_handle();
+  h->c = *c;
+  h->blocks.assign(c->blocks, c->blocks + c->nblocks);
+  h->c.blocks = h->blocks.data();
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    delete h;
+    return fail(nullptr, MGPU_EDEVICE, "no HIP device");
+  }
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return fail(nullptr, MGPU_EDEVICE, "hipStreamCreate failed");
+  }
+  h->b.resize(c->nblocks);
+  int rc = 0;
+  for (int bI = 0; bI < c->nblocks && !rc; bI++) {
+    BlockData &bd = h->b[bI];
+    const mgpu_block &bl = h->blocks[bI];
+    bd.g.nx = bl.nx;
+    bd.g.ny = bl.ny;
+    bd.g.nz = bl.nz;
+    bd.g.sx = bl.nx + 2 * G;
+    bd.g.sy = (long)bd.g.sx * (bl.ny + 2 * G);
+    bd.g.n = bd.g.sy * (bl.nz + 2 * G);
+    const size_t n = (size_t)bd.g.n;
+    for (int s = 0; s < 3 && !rc; s++)
+      for (int v = 0; v < 5 && !rc; v++) rc = dalloc(h, &bd.st[s][v], n);
+    if (c->curvilinear) {
+      for (int q = 0; q < 10 && !rc; q++) rc = dalloc(h, &bd.m[q], n);
+      if (c->iorder_visc)
+        for (int q = 0; q < 10 && !rc; q++) rc = dalloc(h, &bd.mv[q], n);
+    } else {
+      const int len[3] = {bl.nx + 2 * G, bl.ny + 2 * G, bl.nz + 2 * G};
+      for (int q = 0; q < 6 && !rc; q++) rc = dalloc(h, &bd.idx[q], (size_t)len[q % 3]);
+    }
+    if (c->iorder_visc) {
+      for (int q = 0; q < 12 && !rc; q++) rc = dalloc(h, &bd.fv[q], n);
+      if (c->filter == MGPU_FILTER_SHOCK) {
+        if (!rc) rc = dalloc(h, &bd.prs, n);
+        if (!rc) rc = dalloc(h, &bd.sens, n);
+      }
+    }
+  }
+  if (rc) {
+    g_create_error = h->err;
+    mgpu_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+int mgpu_set_metrics_cart(mgpu_handle *h, int block, const double *idx, const double *idy,
+                          const double *idz, const double *idx_v, const double *idy_v,
+                          const double *idz_v) {
+  if (!h) return MGPU_EINVAL;
+  if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
+  if (h->c.curvilinear) return fail(h, MGPU_ESTATE, "case is curvilinear: use mgpu_set_metrics_c3");
+  BlockData &bd = h->b[block];
+  const double *src[6] = {idx, idy, idz, idx_v, idy_v, idz_v};
+  const int len[3] = {bd.g.nx + 2 * G, bd.g.ny + 2 * G, bd.g.nz + 2 * G};
+  for (int q = 0; q < 6; q++) {
+    if (!src[q]) {
+      if (q >= 3 && h->c.iorder_visc == 0) continue;
+      return fail(h, MGPU_EINVAL, "metric array %d is null", q);
+    }
+    HIPCK(h, hipMemcpy(bd.idx[q], src[q], sizeof(double) * len[q % 3], hipMemcpyHostToDevice));
+  }
+  bd.have_metrics = true;
+  return 0;
+}
+
+int mgpu_set_metrics_c3(mgpu_handle *h, int block, const double *const *m, const double *const *mv) {
+  if (!h) return MGPU_EINVAL;
+  if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
+  if (!h->c.curvilinear) return fail(h, MGPU_ESTATE, "case is Cartesian: use mgpu_set_metrics_cart");
+  if (!m) return fail(h, MGPU_EINVAL, "null metric table");
+  BlockData &bd = h->b[block];
+  const size_t bytes = sizeof(double) * (size_t)bd.g.n;
+  for (int q = 0; q < 10; q++) {
+    if (!m[q]) return fail(h, MGPU_EINVAL, "metric %d is null", q);
+    HIPCK(h, hipMemcpy(bd.m[q], m[q], bytes, hipMemcpyHostToDevice));
+  }
+  if (h->c.iorder_visc) {
+    if (!mv) return fail(h, MGPU_EINVAL, "viscous metric table is null");
+    for (int q = 0; q < 10; q++) {
+      if (!mv[q]) return fail(h, MGPU_EINVAL, "viscous metric %d is null", q);
+      HIPCK(h, hipMemcpy(bd.mv[q], mv[q], bytes, hipMemcpyHostToDevice));
+    }
+  }
+  bd.have_metrics = true;
+  return 0;
+}
+
+int mgpu_put_state(mgpu_handle *h, int block, const double *rho, const double *rhou, const double *rhov,
+                   const double *rhow, const double *rhoe) {
+  if (!h) return MGPU_EINVAL;
+  if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
+  const double *src[5] = {rho, rhou, rhov, rhow, rhoe};
+  BlockData &bd = h->b[block];
+  const size_t bytes = sizeof(double) * (size_t)bd.g.n;
+  for (int v = 0; v < 5; v++) {
+    if (!src[v]) return fail(h, MGPU_EINVAL, "state array %d is null", v);
+    HIPCK(h, hipMemcpyAsync(bd.st[h->cur][v], src[v], bytes, hipMemcpyHostToDevice, h->stream));
+  }
+  int rc = halo(h, bd, bd.st[h->cur]);
+  if (rc) return rc;
+  HIPCK(h, hipStreamSynchronize(h->stream));
+  bd.have_state = true;
+  return 0;
+}
+
+int mgpu_get_state(mgpu_handle *h, int block, double *rho, double *rhou, double *rhov, double *rhow,
+                   double *rhoe) {
+  if (!h) return MGPU_EINVAL;
+  if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
+  double *dst[5] = {rho, rhou, rhov, rhow, rhoe};
+  BlockData &bd = h->b[block];
+  const size_t bytes = sizeof(double) * (size_t)bd.g.n;
+  HIPCK(h, hipStreamSynchronize(h->stream));
+  for (int v = 0; v < 5; v++) {
+    if (!dst[v]) return fail(h, MGPU_EINVAL, "state array %d is null", v);
+    HIPCK(h, hipMemcpy(dst[v], bd.st[h->cur][v], bytes, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int mgpu_step(mgpu_handle *h, int nsteps) {
+  if (!h) return MGPU_EINVAL;
+  if (nsteps < 0) return fail(h, MGPU_EINVAL, "nsteps=%d", nsteps);
+  for (size_t bI = 0; bI < h->b.size(); bI++) {
+    if (!h->b[bI].have_metrics) return fail(h, MGPU_ESTATE, "block %zu: metrics not set", bI);
+    if (!h->b[bI].have_state) return fail(h, MGPU_ESTATE, "block %zu: state not set", bI);
+  }
+  const mgpu_case &c = h->c;
+  const bool curv = c.curvilinear != 0, visc = c.iorder_visc != 0;
+  const Gas gs = gas_of(c);
+  for (int step = 0; step < nsteps; step++) {
+    const int s0 = h->cur, sa = (h->cur + 1) % 3, sb = (h->cur + 2) % 3;
+    for (auto &bd : h->b) {
+      const Metrics met = metrics_of(bd);
+      int in = s0;
+      for (int irk = 1; irk <= c.nrk; irk++) {
+        const bool last = irk == c.nrk;
+        const int out = (irk % 2 == 1) ? sa : sb;
+        if (last && visc) {
+          ProfScope ps(h, MGPU_K_VISC);
+          ViscArgs va{};
+          va.g = bd.g;
+          va.gas = gs;
+          for (int v = 0; v < 5; v++) va.u0.v[v] = bd.st[s0][v];
+          va.met = met;
+          for (int q = 0; q < 12; q++) va.fv[q] = bd.fv[q];
+          va.prs = bd.prs;
+          va.sens = bd.sens;
+          va.want_sensor = c.filter == MGPU_FILTER_SHOCK;
+          va.ducros = c.is_ducros;
+          HIPCK(h, launch_visc(va, curv, h->stream));
+        }
+        {
+          ProfScope ps(h, MGPU_K_STAGE);
+          StageArgs sa_{};
+          sa_.g = bd.g;
+          sa_.gas = gs;
+          for (int v = 0; v < 5; v++) {
+            sa_.uin.v[v] = bd.st[in][v];
+            sa_.u0.v[v] = bd.st[s0][v];
+            sa_.uout.v[v] = bd.st[out][v];
+          }
+          sa_.met = met;
+          for (int q = 0; q < 12; q++) sa_.fv[q] = bd.fv[q];
+          sa_.alpha = c.deltat * c.crk[irk - 1];
+          sa_.kchunk = kchunk_for(bd.g);
+          HIPCK(h, launch_stage(sa_, curv, last && visc, h->stream));
+        }
+        int rc = halo(h, bd, bd.st[out]);
+        if (rc) return rc;
+        in = out;
+      }
+      if (c.filter != MGPU_FILTER_NONE) {
+        ProfScope ps(h, MGPU_K_FILTER);
+        FilterArgs fa{};
+        fa.g = bd.g;
+        for (int v = 0; v < 5; v++) {
+          fa.uin.v[v] = bd.st[in][v];
+          fa.uout.v[v] = bd.st[s0][v];
+        }
+        for (int l = 0; l < 6; l++) {
+          const double d11[6] = {D11_0, D11_1, D11_2, D11_3, D11_4, D11_5};
+          fa.dd[l] = -d11[l] * c.dissip_coeff;
+        }
+        fa.prs = bd.prs;
+        fa.sens = c.is_ducros ? bd.sens : nullptr;
+        fa.xnu0 = -c.dissip_shock;
+        fa.xnu10 = c.dissip_coeff / 1260.0;
+        fa.csens = c.csens;
+        if (c.filter == MGPU_FILTER_SF) HIPCK(h, launch_filter_sf(fa, h->stream));
+        else HIPCK(h, launch_filter_shock(fa, h->stream));
+      }
+    }
+    if (c.filter != MGPU_FILTER_NONE) {
+      for (auto &bd : h->b) {
+        int rc = halo(h, bd, bd.st[s0]);
+        if (rc) return rc;
+      }
+    } else {
+      // U^{n+1} is the last stage output: make that set current
+      h->cur = (c.nrk % 2 == 1) ? sa : sb;
+    }
+  }
+  return 0;
+}
+
+int mgpu_sync(mgpu_handle *h) {
+  if (!h) return MGPU_EINVAL;
+  HIPCK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int mgpu_profile(mgpu_handle *h, int enable) {
+  if (!h) return MGPU_EINVAL;
+  int rc = drain_profile(h);
+  if (rc) return rc;
+  for (int k = 0; k < MGPU_K_COUNT; k++) {
+    h->kcount[k] = 0;
+    h->kms[k] = 0.0;
+  }
+  h->profiling = enable != 0;
+  return 0;
+}
+
+int mgpu_kernel_time(mgpu_handle *h, int kind, int64_t *launches, double *ms) {
+  if (!h) return MGPU_EINVAL;
+  if (kind < 0 || kind >= MGPU_K_COUNT) return fail(h, MGPU_EINVAL, "kernel kind %d", kind);
+  int rc = drain_profile(h);
+  if (rc) return rc;
+  if (launches) *launches = h->kcount[kind];
+  if (ms) *ms = h->kms[kind];
+  return 0;
+}
+
+const char *mgpu_last_error(const mgpu_handle *h) {
+  if (!h) return g_create_error.c_str();
+  return h->err.c_str();
+}
+
+void mgpu_destroy(mgpu_handle *h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (auto &r : h->recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : h->ev_pool) (void)hipEventDestroy(e);
+  for (auto &bd : h->b) free_block(bd);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+}  // extern "C"

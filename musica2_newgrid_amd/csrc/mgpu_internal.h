@@ -1,0 +1,111 @@
+// Internal definitions shared by the HIP kernels and the C-ABI of
+// libmusica_gpu.so.  gfx950 only (64-wide wavefronts).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/musica_gpu.h"
+
+namespace mgpu {
+
+constexpr int G = MGPU_NGH;  // ghost layers (ngh=5, read_param.f90:495)
+
+// 11-pt centred 10th-order first derivative (mod_coeff_deriv.f90:316-325).
+constexpr double A11_1 = 5.0 / 6.0, A11_2 = -5.0 / 21.0, A11_3 = 5.0 / 84.0,
+                 A11_4 = -5.0 / 504.0, A11_5 = 1.0 / 1260.0;
+// 5-pt centred 4th-order first derivative (mod_coeff_deriv.f90:121-126).
+constexpr double A5_1 = 2.0 / 3.0, A5_2 = -1.0 / 12.0;
+// 11-pt selective filter (mod_filtering.f90:63-68), scaled by -chi at run time.
+constexpr double D11_0 = 252.0 / 1024.0, D11_1 = -210.0 / 1024.0, D11_2 = 120.0 / 1024.0,
+                 D11_3 = -45.0 / 1024.0, D11_4 = 10.0 / 1024.0, D11_5 = -1.0 / 1024.0;
+// 9th-order conservative dissipation (mod_filtering_shock.f90:95-100).
+constexpr double D10_1 = -126.0, D10_2 = 84.0, D10_3 = -36.0, D10_4 = 9.0, D10_5 = -1.0;
+
+// Geometry of one block's padded arrays (i fastest).
+struct Geom {
+  int nx, ny, nz;
+  int sx;   // nx + 2G
+  long sy;  // sx*(ny+2G)
+  long n;   // total padded points
+};
+
+__host__ __device__ inline long lin(const Geom &g, int i, int j, int k) {
+  return (long)(i + G - 1) + (long)g.sx * (long)(j + G - 1) + g.sy * (long)(k + G - 1);
+}
+
+// Thermodynamics / transport constants of a case.
+struct Gas {
+  double gam1, rg, gam;
+  double c0, S0, diffscale, cok_ratio;
+};
+
+// Five conserved variables as separate arrays.
+struct State {
+  double *v[5];
+};
+struct CState {
+  const double *v[5];
+};
+
+// Metrics of a block on the device.
+struct Metrics {
+  // Cartesian (length n+2G each)
+  const double *idx, *idy, *idz, *idx_v, *idy_v, *idz_v;
+  // c3 padded 3-D arrays
+  const double *m[10];
+  const double *mv[10];
+};
+
+// Arguments of the Euler (+viscous divergence) stage kernel.
+struct StageArgs {
+  Geom g;
+  Gas gas;
+  CState uin;    // stage input U_n
+  CState u0;     // step-start U^n
+  State uout;    // stage output
+  Metrics met;
+  const double *fv[12];  // viscous fluxes F^v,G^v,H^v (4 components each), last stage
+  double alpha;          // deltat*crk(irk)
+  int kchunk;            // planes per block along k
+};
+
+struct ViscArgs {
+  Geom g;
+  Gas gas;
+  CState u0;          // U^n
+  Metrics met;
+  double *fv[12];     // out: viscous fluxes on the extended box [-1, n+2]^3
+  double *prs;        // out: p(U^n) on [-1, n+2]^3 (shock filter)
+  double *sens;       // out: Ducros sensor on [0, n+1]^3 (shock filter)
+  int want_sensor;    // 1: write prs and sens
+  int ducros;
+};
+
+struct FilterArgs {
+  Geom g;
+  CState uin;   // U_n after the last stage
+  State uout;   // U_n + K
+  const double *prs;   // shock filter: p(U^n)
+  const double *sens;  // shock filter: Ducros sensor
+  double dd[6];        // -chi*d11 (selective filter)
+  double xnu0, xnu10, csens;
+};
+
+// Halo copy: one face pair of one block.  For a periodic face the source
+// is the opposite side of the same block.
+struct HaloArgs {
+  Geom g;
+  State u;
+};
+
+}  // namespace mgpu
+
+// Kernel launchers (defined in the .hip files).
+namespace mgpu {
+hipError_t launch_stage(const StageArgs &a, bool curv, bool visc, hipStream_t s);
+hipError_t launch_visc(const ViscArgs &a, bool curv, hipStream_t s);
+hipError_t launch_filter_sf(const FilterArgs &a, hipStream_t s);
+hipError_t launch_filter_shock(const FilterArgs &a, hipStream_t s);
+hipError_t launch_halo_periodic(const HaloArgs &a, int nvar_from_0, hipStream_t s);
+}  // namespace mgpu

@@ -1,0 +1,158 @@
+"""Host-side set-up of block grids: padded array helpers and curvilinear metrics.
+
+Set-up work, done once before the time loop, as in the reference's
+grid_finalize (src/Grid/grid_finalize.f90:2-268).  It is not on the per-step
+path: the library receives the metrics through mgpu_set_metrics_*.
+
+Arrays are numpy arrays of shape (nz+2G, ny+2G, nx+2G), C order -- the same
+bytes as the reference's Fortran arrays (1-G:nx+G, 1-G:ny+G, 1-G:nz+G).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+G = 5
+
+# 11-pt and 5-pt centred first-derivative coefficients
+# (src/Derivatives/mod_coeff_deriv.f90:121-126, 316-325)
+A11 = (5.0 / 6.0, -5.0 / 21.0, 5.0 / 84.0, -5.0 / 504.0, 1.0 / 1260.0)
+A5 = (2.0 / 3.0, -1.0 / 12.0)
+
+
+def padded(nx: int, ny: int, nz: int, fill: float = 0.0) -> np.ndarray:
+    return np.full((nz + 2 * G, ny + 2 * G, nx + 2 * G), fill, dtype=np.float64)
+
+
+def interior(a: np.ndarray) -> np.ndarray:
+    return a[G:-G, G:-G, G:-G]
+
+
+def fill_periodic(a: np.ndarray, axes=(0, 1, 2)) -> None:
+    """Face ghosts from the periodic image: i-faces over interior (j,k), then
+    j-faces over all i, then k-faces over all (i,j) -- mirroring the
+    reference's self-neighbour halo (src/Parallel/mod_comm.f90:249-290)."""
+    nz, ny, nx = (s - 2 * G for s in a.shape)
+    if 2 in axes:  # i
+        a[G:G + nz, G:G + ny, :G] = a[G:G + nz, G:G + ny, nx:nx + G]
+        a[G:G + nz, G:G + ny, nx + G:] = a[G:G + nz, G:G + ny, G:2 * G]
+    if 1 in axes:  # j
+        a[G:G + nz, :G, :] = a[G:G + nz, ny:ny + G, :]
+        a[G:G + nz, ny + G:, :] = a[G:G + nz, G:2 * G, :]
+    if 0 in axes:  # k
+        a[:G, :, :] = a[nz:nz + G, :, :]
+        a[nz + G:, :, :] = a[G:2 * G, :, :]
+
+
+def _deriv(f: np.ndarray, axis: int, coeffs) -> np.ndarray:
+    """Centred derivative along axis; the result is shorter by 2R there."""
+    R = len(coeffs)
+    n = f.shape[axis]
+
+    def sl(a, b):
+        s = [slice(None)] * f.ndim
+        s[axis] = slice(a, b)
+        return f[tuple(s)]
+
+    out = None
+    for l, a in enumerate(coeffs, start=1):
+        t = sl(R + l, n - R + l) - sl(R - l, n - R - l)
+        t *= a
+        if out is None:
+            out = t
+        else:
+            out += t
+    return out
+
+
+def _crop(f: np.ndarray, e_from: int, e_to: int, axes=(0, 1, 2)) -> np.ndarray:
+    """Reduce the extension of the given axes from e_from to e_to layers."""
+    d = e_from - e_to
+    sl = [slice(None)] * 3
+    for ax in axes:
+        sl[ax] = slice(d, f.shape[ax] - d)
+    return f[tuple(sl)]
+
+
+def gcl_metrics(xe: np.ndarray, ye: np.ndarray, ze: np.ndarray, ext: int, coeffs):
+    """Conservative (Thomas & Lombard) metrics J*dksi/dx ... and the inverse
+    Jacobian on the interior, from coordinates given on a box extended by
+    `ext` layers (ext >= 2R).  Follows grid_metrics_3d
+    (src/Grid/smod_grid_metrics1_c3.f90:152-230): ksi_x = (y_eta z)_phi -
+    (y_phi z)_eta and cyclic, and grid_metrics_ijacob_3d
+    (src/Grid/smod_grid_metrics3_c3.f90:29-35,131) for 1/J.
+
+    Returns (m, ijac): m = [ksi_x,ksi_y,ksi_z,eta_x,eta_y,eta_z,phi_x,phi_y,phi_z]
+    on the interior (shape (nz,ny,nx)), ijac the inverse Jacobian there."""
+    R = len(coeffs)
+    if ext > 2 * R:  # keep exactly the 2R layers the two derivative levels need
+        xe, ye, ze = (_crop(a, ext, 2 * R) for a in (xe, ye, ze))
+        ext = 2 * R
+    e1 = ext - R  # extension kept after the first derivatives
+    # axis 2 = i (ksi), 1 = j (eta), 0 = k (phi)
+
+    def d1(f, axis):
+        return _crop(_deriv(f, axis, coeffs), ext, e1, axes=[a for a in (0, 1, 2) if a != axis])
+
+    xk, yk, zk = d1(xe, 2), d1(ye, 2), d1(ze, 2)
+    xh, yh, zh = d1(xe, 1), d1(ye, 1), d1(ze, 1)
+    xp, yp, zp = d1(xe, 0), d1(ye, 0), d1(ze, 0)
+    xc, yc, zc = (_crop(a, ext, e1) for a in (xe, ye, ze))
+
+    def d2(f, axis):
+        # derivative along axis, then crop the other axes to the interior
+        return _crop(_deriv(f, axis, coeffs), e1, 0, axes=[a for a in (0, 1, 2) if a != axis])
+
+    # ksi_x=(y_eta z)_phi-(y_phi z)_eta ; ksi_y=(z_eta x)_phi-(z_phi x)_eta ; ksi_z=(x_eta y)_phi-(x_phi y)_eta
+    ksi_x = d2(yh * zc, 0) - d2(yp * zc, 1)
+    ksi_y = d2(zh * xc, 0) - d2(zp * xc, 1)
+    ksi_z = d2(xh * yc, 0) - d2(xp * yc, 1)
+    # eta_x=(y_phi z)_ksi-(y_ksi z)_phi ...
+    eta_x = d2(yp * zc, 2) - d2(yk * zc, 0)
+    eta_y = d2(zp * xc, 2) - d2(zk * xc, 0)
+    eta_z = d2(xp * yc, 2) - d2(xk * yc, 0)
+    # phi_x=(y_ksi z)_eta-(y_eta z)_ksi ...
+    phi_x = d2(yk * zc, 1) - d2(yh * zc, 2)
+    phi_y = d2(zk * xc, 1) - d2(zh * xc, 2)
+    phi_z = d2(xk * yc, 1) - d2(xh * yc, 2)
+
+    c = lambda f: _crop(f, e1, 0)  # noqa: E731
+    XK, YK, ZK, XH, YH, ZH, XP, YP, ZP = (c(a) for a in (xk, yk, zk, xh, yh, zh, xp, yp, zp))
+    jac = XK * (YH * ZP - YP * ZH) - XH * (YK * ZP - YP * ZK) + XP * (YK * ZH - YH * ZK)
+    return [ksi_x, ksi_y, ksi_z, eta_x, eta_y, eta_z, phi_x, phi_y, phi_z], 1.0 / jac
+
+
+def curvilinear_metrics_periodic(mapping, nx, ny, nz, lengths, viscous=True):
+    """Metrics of a block periodic in all three directions.
+
+    mapping(xi, eta, zeta) -> (x, y, z) maps computational coordinates
+    (xi = (i-1)*Lx/nx, ...) to physical ones, with x - xi, y - eta, z - zeta
+    periodic.  Coordinates are evaluated on a box extended by 10 layers without
+    wrapping, metrics are computed on the interior and their ghosts are filled
+    from the periodic image (as the reference's metric halo exchange does).
+
+    Returns (m, mv): two lists of 10 padded arrays
+    (ksi_x..phi_z, ijacob3), the Euler (11-pt) and viscous (5-pt) sets."""
+    ext = 10
+    Lx, Ly, Lz = lengths
+    i = np.arange(1 - ext, nx + ext + 1, dtype=np.float64)
+    j = np.arange(1 - ext, ny + ext + 1, dtype=np.float64)
+    k = np.arange(1 - ext, nz + ext + 1, dtype=np.float64)
+    XI = (i - 1) * (Lx / nx)
+    ET = (j - 1) * (Ly / ny)
+    ZE = (k - 1) * (Lz / nz)
+    Z3, Y3, X3 = np.meshgrid(ZE, ET, XI, indexing="ij")
+    xe, ye, ze = mapping(X3, Y3, Z3)
+    out = []
+    for coeffs in ((A11, A5) if viscous else (A11,)):
+        m9, ij = gcl_metrics(xe, ye, ze, ext, coeffs)
+        arrs = []
+        for f in m9 + [ij]:
+            a = padded(nx, ny, nz)
+            a[G:-G, G:-G, G:-G] = f
+            fill_periodic(a)
+            arrs.append(a)
+        out.append(arrs)
+    if not viscous:
+        out.append(None)
+    coords = tuple(_crop(a, ext, 0) for a in (xe, ye, ze))
+    return out[0], out[1], coords

@@ -1,0 +1,599 @@
+/*
+ * musica_oracle.c -- CPU restatement of MUSICAA's explicit RK time step.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker of the HIP product path; see the
+ * header).  Plain C99 + OpenMP over k-planes, FP64 throughout, no FMA
+ * contraction (built with -ffp-contract=off), evaluation order of every sum
+ * as in the reference's Fortran so that results track the reference to
+ * round-off.
+ *
+ * Reference files followed (LouenasZm/musica2_newGrid):
+ *   runge_kutta.f90:2-152        step choreography            orc_step
+ *   update.f90:32-44, 538-565    update_var / update_varn      orc_step
+ *   mod_time.f90:165-190         crk for nrk=4                 (caller)
+ *   smod_primitive_pfg.f90:12-136 primitives / primitives_visc prim_*
+ *   smod_flux_euler_11pts.f90    Cartesian Euler divergence    euler_cart
+ *   smod_flux_euler_11pts_c3.f90 curvilinear Euler divergence  euler_c3
+ *   mod_coeff_deriv.f90:108-325  a3,a5,a7,a9,a11 coefficients  A11,A5
+ *   smod_grad_vel*.f90, smod_grad_T*.f90   5-pt gradients      grad_*
+ *   smod_flux_visc_5pts*.f90     viscous flux divergence       visc_*
+ *   mod_filtering.f90:18-205,894-1492  selective filter        filter_sf
+ *   mod_filtering_shock.f90:22-1512    shock filter            filter_shock
+ *   mod_comm.f90:249-290         halo exchange (periodic self) orc_fill_periodic
+ *   mpi_index_bounds.f90         loop bounds for periodic faces
+ */
+#include "musica_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NG ORC_NG
+
+typedef struct {
+  int nx, ny, nz;
+  long sx, sy, sz;
+  size_t n;
+} dims_t;
+
+static dims_t mkdims(const orc_cfg *c) {
+  dims_t d;
+  d.nx = c->nx; d.ny = c->ny; d.nz = c->nz;
+  d.sx = c->nx + 2 * NG; d.sy = c->ny + 2 * NG; d.sz = c->nz + 2 * NG;
+  d.n = (size_t)d.sx * d.sy * d.sz;
+  return d;
+}
+
+/* Fortran index (i,j,k), interior 1..n, to linear offset. */
+#define ID(d, i, j, k) \
+  ((size_t)((i) + NG - 1) + (size_t)(d).sx * ((size_t)((j) + NG - 1) + (size_t)(d).sy * (size_t)((k) + NG - 1)))
+
+/* 11-pt centred 10th-order first derivative, mod_coeff_deriv.f90:316-325. */
+static const double A11[6] = {0.0, 5.0 / 6.0, -5.0 / 21.0, 5.0 / 84.0, -5.0 / 504.0, 1.0 / 1260.0};
+/* 5-pt centred 4th-order first derivative, mod_coeff_deriv.f90:121-126. */
+static const double A5[3] = {0.0, 2.0 / 3.0, -1.0 / 12.0};
+/* 11-pt standard selective filter, mod_filtering.f90:63-68 (before -chi scaling). */
+static const double D11[6] = {252.0 / 1024.0, -210.0 / 1024.0, 120.0 / 1024.0,
+                              -45.0 / 1024.0, 10.0 / 1024.0, -1.0 / 1024.0};
+/* 9th-order conservative dissipation, mod_filtering_shock.f90:95-100. */
+static const double D10[6] = {0.0, -126.0, 84.0, -36.0, 9.0, -1.0};
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+static double *alloc_arr(const dims_t *d) {
+  double *p = (double *)calloc(d->n, sizeof(double));
+  return p;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Halo: periodic self-neighbour faces (mod_comm.f90:249-290).  The x pass   */
+/* covers interior j,k; the y pass all i; the z pass all i,j -- so edge and  */
+/* corner ghosts also hold the periodic image (the cross stencils never read */
+/* them, but primitives are evaluated on the whole padded box).              */
+/* ------------------------------------------------------------------------ */
+void orc_fill_periodic(const orc_cfg *c, double *a) {
+  dims_t d = mkdims(c);
+  if (c->face[0][0] == ORC_FACE_PERIODIC) {
+#pragma omp parallel for collapse(2)
+    for (int k = 1; k <= d.nz; k++)
+      for (int j = 1; j <= d.ny; j++)
+        for (int l = 1; l <= NG; l++) {
+          a[ID(d, 1 - l, j, k)] = a[ID(d, d.nx + 1 - l, j, k)];
+          a[ID(d, d.nx + l, j, k)] = a[ID(d, l, j, k)];
+        }
+  }
+  if (c->face[1][0] == ORC_FACE_PERIODIC) {
+#pragma omp parallel for
+    for (int k = 1; k <= d.nz; k++)
+      for (int l = 1; l <= NG; l++)
+        for (int i = 1 - NG; i <= d.nx + NG; i++) {
+          a[ID(d, i, 1 - l, k)] = a[ID(d, i, d.ny + 1 - l, k)];
+          a[ID(d, i, d.ny + l, k)] = a[ID(d, i, l, k)];
+        }
+  }
+  if (c->face[2][0] == ORC_FACE_PERIODIC) {
+#pragma omp parallel for
+    for (int l = 1; l <= NG; l++)
+      for (int j = 1 - NG; j <= d.ny + NG; j++)
+        for (int i = 1 - NG; i <= d.nx + NG; i++) {
+          a[ID(d, i, j, 1 - l)] = a[ID(d, i, j, d.nz + 1 - l)];
+          a[ID(d, i, j, d.nz + l)] = a[ID(d, i, j, l)];
+        }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Primitive variables (smod_primitive_pfg.f90:12-59, 62-136), whole box.    */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  double *uu, *vv, *ww, *prs, *tmp; /* primitives                          */
+  double *cs, *visc, *cok;          /* sound speed, viscosity, conductivity */
+  double *grad[12];                 /* dux,duy,duz,dvx,dvy,dvz,dwx,dwy,dwz,dTx,dTy,dTz */
+} work_t;
+
+enum { DUX, DUY, DUZ, DVX, DVY, DVZ, DWX, DWY, DWZ, DTX, DTY, DTZ };
+
+static void prim_euler(const orc_cfg *c, const dims_t *d, double *const U[5], work_t *w) {
+  const double gam1 = c->gam - 1.0, rg = c->rg;
+  const long n = (long)d->n;
+#pragma omp parallel for
+  for (long p = 0; p < n; p++) {
+    double rro = U[0][p];
+    if (rro == 0.0) { w->uu[p] = w->vv[p] = w->ww[p] = w->prs[p] = w->tmp[p] = 0.0; continue; }
+    double v = 1.0 / rro;
+    double u1 = U[1][p] * v, u2 = U[2][p] * v, u3 = U[3][p] * v;
+    w->uu[p] = u1; w->vv[p] = u2; w->ww[p] = u3;
+    double pr = gam1 * (U[4][p] - 0.5 * rro * (u1 * u1 + u2 * u2 + u3 * u3));
+    w->prs[p] = pr;
+    w->tmp[p] = pr * v / rg;
+  }
+}
+
+static void prim_visc(const orc_cfg *c, const dims_t *d, double *const U[5], work_t *w) {
+  const double gam1 = c->gam - 1.0, rg = c->rg, gam = c->gam;
+  const long n = (long)d->n;
+#pragma omp parallel for
+  for (long p = 0; p < n; p++) {
+    double rro = U[0][p];
+    if (rro == 0.0) {
+      w->uu[p] = w->vv[p] = w->ww[p] = w->prs[p] = w->tmp[p] = 0.0;
+      w->cs[p] = w->visc[p] = w->cok[p] = 0.0;
+      continue;
+    }
+    double v = 1.0 / rro;
+    double u1 = U[1][p] * v, u2 = U[2][p] * v, u3 = U[3][p] * v;
+    w->uu[p] = u1; w->vv[p] = u2; w->ww[p] = u3;
+    double pr = gam1 * (U[4][p] - 0.5 * rro * (u1 * u1 + u2 * u2 + u3 * u3));
+    double t = pr * v / rg;
+    w->prs[p] = pr;
+    w->tmp[p] = t;
+    w->cs[p] = sqrt(gam * rg * t);
+    double mu = c->c0 * sqrt(t * t * t) / (t + c->S0) * c->diffscale;
+    w->visc[p] = mu;
+    w->cok[p] = c->cok_ratio * mu;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Euler flux divergence.                                                    */
+/* Flux along one direction on the whole line (incl. 5 ghosts), then the     */
+/* interior 11-pt difference.  Periodic faces only: loop bounds ndxt..nfxt = */
+/* 1-ng..n+ng and ndx..nfx = 1..n (mpi_index_bounds.f90:82-107).             */
+/* ------------------------------------------------------------------------ */
+static inline double d11(const double *f, long p, long s) {
+  return A11[1] * (f[p + s] - f[p - s]) + A11[2] * (f[p + 2 * s] - f[p - 2 * s]) +
+         A11[3] * (f[p + 3 * s] - f[p - 3 * s]) + A11[4] * (f[p + 4 * s] - f[p - 4 * s]) +
+         A11[5] * (f[p + 5 * s] - f[p - 5 * s]);
+}
+
+/* Physical (Cartesian) flux along direction dir at linear offset p. */
+static inline void flux_cart(int dir, double *const U[5], const work_t *w, long p, double F[5]) {
+  double pr = w->prs[p];
+  double m = U[1 + dir][p]; /* rhou, rhov or rhow */
+  double vel = dir == 0 ? w->uu[p] : (dir == 1 ? w->vv[p] : w->ww[p]);
+  F[0] = m;
+  F[1] = m * w->uu[p] + (dir == 0 ? pr : 0.0);
+  F[2] = m * w->vv[p] + (dir == 1 ? pr : 0.0);
+  F[3] = m * w->ww[p] + (dir == 2 ? pr : 0.0);
+  F[4] = (U[4][p] + pr) * vel;
+}
+
+/* Contravariant flux along direction dir (smod_flux_euler_11pts_c3.f90:25-38). */
+static inline void flux_c3(int dir, double *const U[5], const work_t *w, const orc_grid *g,
+                           long p, double F[5]) {
+  const double mx = g->m[3 * dir][p], my = g->m[3 * dir + 1][p], mz = g->m[3 * dir + 2][p];
+  double u = w->uu[p], v = w->vv[p], ww = w->ww[p], pr = w->prs[p];
+  double vc = u * mx + v * my + ww * mz;
+  double rovc = U[0][p] * vc;
+  F[0] = rovc;
+  F[1] = rovc * u + pr * mx;
+  F[2] = rovc * v + pr * my;
+  F[3] = rovc * ww + pr * mz;
+  F[4] = (U[4][p] + pr) * vc;
+}
+
+static void euler_rhs_w(const orc_cfg *c, const orc_grid *g, const dims_t *d, double *const U[5],
+                        const work_t *w, double *K[5]) {
+  const long sx = 1, sy = d->sx, sz = d->sx * d->sy;
+  const long strides[3] = {sx, sy, sz};
+  const int nn[3] = {d->nx, d->ny, d->nz};
+  /* line buffers: fluxes along a line with ghosts */
+  for (int dir = 0; dir < 3; dir++) {
+    const long s = strides[dir];
+    const int n = nn[dir];
+    /* iterate over the 2 transverse interior indices */
+    int o1 = dir == 0 ? 1 : 0, o2 = dir == 2 ? 1 : 2;
+    int n1 = nn[o1], n2 = nn[o2];
+#pragma omp parallel
+    {
+      double *fl = (double *)malloc(sizeof(double) * 5 * (size_t)(n + 2 * NG));
+#pragma omp for collapse(2)
+      for (int b = 1; b <= n2; b++)
+        for (int a = 1; a <= n1; a++) {
+          int idx3[3];
+          idx3[o1] = a; idx3[o2] = b; idx3[dir] = 1 - NG;
+          long p0 = (long)ID(*d, idx3[0], idx3[1], idx3[2]);
+          /* fluxes on 1-ng..n+ng */
+          for (int l = 0; l < n + 2 * NG; l++) {
+            double F[5];
+            long p = p0 + l * s;
+            if (c->curv) flux_c3(dir, U, w, g, p, F);
+            else flux_cart(dir, U, w, p, F);
+            for (int v = 0; v < 5; v++) fl[v * (n + 2 * NG) + l] = F[v];
+          }
+          for (int i = 1; i <= n; i++) {
+            long p = p0 + (long)(i - 1 + NG) * s;
+            long q = i - 1 + NG; /* position in line buffer */
+            double metric = 1.0;
+            if (!c->curv) metric = dir == 0 ? g->idx[i - 1 + NG] : (dir == 1 ? g->idy[i - 1 + NG] : g->idz[i - 1 + NG]);
+            for (int v = 0; v < 5; v++) {
+              double dv = d11(fl + v * (n + 2 * NG), q, 1);
+              if (!c->curv) dv = dv * metric;
+              if (dir == 0) K[v][p] = dv;
+              else K[v][p] = dv + K[v][p];
+            }
+          }
+        }
+      free(fl);
+    }
+  }
+  if (c->curv) {
+    /* K *= ijacob3 (smod_flux_euler_11pts_c3.f90:763-773) */
+#pragma omp parallel for collapse(2)
+    for (int k = 1; k <= d->nz; k++)
+      for (int j = 1; j <= d->ny; j++)
+        for (int i = 1; i <= d->nx; i++) {
+          long p = (long)ID(*d, i, j, k);
+          double ij = g->m[9][p];
+          for (int v = 0; v < 5; v++) K[v][p] = K[v][p] * ij;
+        }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Viscous terms: 5-pt gradients of u,v,w,T on the interior                  */
+/* (smod_grad_vel.f90:12-, smod_grad_T.f90:12-; c3: smod_grad_vel_c3.f90:11- */
+/* with the Cartesian mapping at :260-300), halo of the 12 gradients         */
+/* (communication_3dv, mod_comm.f90:376-402), then the viscous flux and its  */
+/* 5-pt divergence (smod_flux_visc_5pts.f90:27-, smod_flux_visc_5pts_c3.f90).*/
+/* ------------------------------------------------------------------------ */
+static inline double d5(const double *f, long p, long s) {
+  return A5[1] * (f[p + s] - f[p - s]) + A5[2] * (f[p + 2 * s] - f[p - 2 * s]);
+}
+
+static void gradients(const orc_cfg *c, const orc_grid *g, const dims_t *d, work_t *w) {
+  const long sy = d->sx, sz = d->sx * d->sy;
+  double *src[4] = {w->uu, w->vv, w->ww, w->tmp};
+#pragma omp parallel for collapse(2)
+  for (int k = 1; k <= d->nz; k++)
+    for (int j = 1; j <= d->ny; j++)
+      for (int i = 1; i <= d->nx; i++) {
+        long p = (long)ID(*d, i, j, k);
+        for (int q = 0; q < 4; q++) {
+          double gx = d5(src[q], p, 1), gy = d5(src[q], p, sy), gz = d5(src[q], p, sz);
+          if (!c->curv) {
+            gx = gx * g->idx_v[i - 1 + NG];
+            gy = gy * g->idy_v[j - 1 + NG];
+            gz = gz * g->idz_v[k - 1 + NG];
+          } else {
+            const double *const *mv = g->mv;
+            double dvx = gx * mv[0][p] + gy * mv[3][p] + gz * mv[6][p];
+            double dvy = gx * mv[1][p] + gy * mv[4][p] + gz * mv[7][p];
+            double dvz = gx * mv[2][p] + gy * mv[5][p] + gz * mv[8][p];
+            gx = dvx * mv[9][p];
+            gy = dvy * mv[9][p];
+            gz = dvz * mv[9][p];
+          }
+          w->grad[3 * q][p] = gx;
+          w->grad[3 * q + 1][p] = gy;
+          w->grad[3 * q + 2][p] = gz;
+        }
+      }
+  for (int q = 0; q < 12; q++) orc_fill_periodic(c, w->grad[q]);
+}
+
+static void visc_rhs_w(const orc_cfg *c, const orc_grid *g, const dims_t *d, work_t *w, double *K[5]) {
+  const long sy = d->sx, sz = d->sx * d->sy;
+  /* viscous fluxes on ndxt_v..nfxt_v = -1..n+2 (periodic: ndx_vi=1, ngh_v=2) */
+  double *F[12];
+  for (int q = 0; q < 12; q++) F[q] = alloc_arr(d); /* Frhou..Frhoe, Grhou.., Hrhou.. */
+  const double one_third = 1.0 / 3.0;
+#pragma omp parallel for collapse(2)
+  for (int k = -1; k <= d->nz + 2; k++)
+    for (int j = -1; j <= d->ny + 2; j++)
+      for (int i = -1; i <= d->nx + 2; i++) {
+        long p = (long)ID(*d, i, j, k);
+        double *const *gr = w->grad;
+        double t11 = gr[DUX][p], t22 = gr[DVY][p], t33 = gr[DWZ][p];
+        double t12 = 0.5 * (gr[DUY][p] + gr[DVX][p]);
+        double t13 = 0.5 * (gr[DUZ][p] + gr[DWX][p]);
+        double t23 = 0.5 * (gr[DVZ][p] + gr[DWY][p]);
+        double tr = one_third * (t11 + t22 + t33);
+        double mu = -2.0 * w->visc[p];
+        t11 = mu * (t11 - tr);
+        t22 = mu * (t22 - tr);
+        t33 = mu * (t33 - tr);
+        t12 = mu * t12;
+        t13 = mu * t13;
+        t23 = mu * t23;
+        double u = w->uu[p], v = w->vv[p], ww = w->ww[p], ck = w->cok[p];
+        if (!c->curv) {
+          F[0][p] = t11; F[1][p] = t12; F[2][p] = t13;
+          F[3][p] = (u * F[0][p] + v * F[1][p] + ww * F[2][p]) - ck * gr[DTX][p];
+          F[4][p] = t12; F[5][p] = t22; F[6][p] = t23;
+          F[7][p] = (u * F[4][p] + v * F[5][p] + ww * F[6][p]) - ck * gr[DTY][p];
+          F[8][p] = t13; F[9][p] = t23; F[10][p] = t33;
+          F[11][p] = (u * F[8][p] + v * F[9][p] + ww * F[10][p]) - ck * gr[DTZ][p];
+        } else {
+          for (int dir = 0; dir < 3; dir++) {
+            double mx = g->mv[3 * dir][p], my = g->mv[3 * dir + 1][p], mz = g->mv[3 * dir + 2][p];
+            double fu = t11 * mx + t12 * my + t13 * mz;
+            double fv = t12 * mx + t22 * my + t23 * mz;
+            double fw = t13 * mx + t23 * my + t33 * mz;
+            F[4 * dir][p] = fu;
+            F[4 * dir + 1][p] = fv;
+            F[4 * dir + 2][p] = fw;
+            F[4 * dir + 3][p] = (u * fu + v * fv + ww * fw) -
+                                ck * (gr[DTX][p] * mx + gr[DTY][p] * my + gr[DTZ][p] * mz);
+          }
+        }
+      }
+  /* divergence, x then y then z, each ( ... )*metric + K */
+#pragma omp parallel for collapse(2)
+  for (int k = 1; k <= d->nz; k++)
+    for (int j = 1; j <= d->ny; j++)
+      for (int i = 1; i <= d->nx; i++) {
+        long p = (long)ID(*d, i, j, k);
+        const long st[3] = {1, sy, sz};
+        for (int dir = 0; dir < 3; dir++) {
+          double metric;
+          if (c->curv) metric = g->mv[9][p];
+          else metric = dir == 0 ? g->idx_v[i - 1 + NG] : (dir == 1 ? g->idy_v[j - 1 + NG] : g->idz_v[k - 1 + NG]);
+          for (int v = 0; v < 4; v++) K[1 + v][p] = d5(F[4 * dir + v], p, st[dir]) * metric + K[1 + v][p];
+        }
+      }
+  for (int q = 0; q < 12; q++) free(F[q]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Selective filter, interior 11-pt stencil (mod_filtering.f90:992-1034,     */
+/* 1192-1233, 1382-1423); coefficients d = -chi*d11 (init_filtering :189-203)*/
+/* ------------------------------------------------------------------------ */
+static void filter_sf(const orc_cfg *c, const dims_t *d, double *const Un[5], double *K[5]) {
+  double dd[6];
+  for (int l = 0; l < 6; l++) dd[l] = -D11[l] * c->chi;
+  const long st[3] = {1, d->sx, d->sx * d->sy};
+#pragma omp parallel for collapse(2)
+  for (int k = 1; k <= d->nz; k++)
+    for (int j = 1; j <= d->ny; j++)
+      for (int i = 1; i <= d->nx; i++) {
+        long p = (long)ID(*d, i, j, k);
+        for (int v = 0; v < 5; v++) {
+          const double *u = Un[v];
+          /* x sets K (:993), y and z accumulate as K + d0*u + ... (:1192, :1382) */
+          double acc = dd[0] * u[p] + dd[1] * (u[p + 1] + u[p - 1]) + dd[2] * (u[p + 2] + u[p - 2]) +
+                       dd[3] * (u[p + 3] + u[p - 3]) + dd[4] * (u[p + 4] + u[p - 4]) +
+                       dd[5] * (u[p + 5] + u[p - 5]);
+          for (int dir = 1; dir < 3; dir++) {
+            long s = st[dir];
+            acc = acc + dd[0] * u[p] + dd[1] * (u[p + s] + u[p - s]) + dd[2] * (u[p + 2 * s] + u[p - 2 * s]) +
+                  dd[3] * (u[p + 3 * s] + u[p - 3 * s]) + dd[4] * (u[p + 4 * s] + u[p - 4 * s]) +
+                  dd[5] * (u[p + 5 * s] + u[p - 5 * s]);
+          }
+          K[v][p] = acc;
+        }
+      }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Shock filter (filter_o10_shock, mod_filtering_shock.f90:204-1512), for    */
+/* periodic faces: Ducros sensor on 0..n+1 (:223-239), pressure sensor per   */
+/* direction (:243-260), face dissipation D_{i+1/2} on 0..n (:419-440) and   */
+/* K_i = D_{i-1/2} - D_{i+1/2} accumulated over directions (:690-700).       */
+/* ------------------------------------------------------------------------ */
+static void filter_shock(const orc_cfg *c, const dims_t *d, double *const Un[5], const work_t *w,
+                         double *K[5]) {
+  const double xnu10 = c->chi / 1260.0, xnu0 = -c->dissip_shock, cs = c->csens;
+  double *sens = alloc_arr(d);
+  /* Ducros sensor on 0..n+1 in all directions */
+#pragma omp parallel for collapse(2)
+  for (int k = 0; k <= d->nz + 1; k++)
+    for (int j = 0; j <= d->ny + 1; j++)
+      for (int i = 0; i <= d->nx + 1; i++) {
+        long p = (long)ID(*d, i, j, k);
+        if (c->ducros) {
+          double *const *gr = w->grad;
+          double dv = gr[DUX][p] + gr[DVY][p] + gr[DWZ][p];
+          double div2 = dv * dv;
+          double o1 = gr[DWY][p] - gr[DVZ][p];
+          double o2 = gr[DUZ][p] - gr[DWX][p];
+          double o3 = gr[DVX][p] - gr[DUY][p];
+          sens[p] = div2 / (div2 + o1 * o1 + o2 * o2 + o3 * o3 + 1.0e-16);
+        } else {
+          sens[p] = 1.0;
+        }
+      }
+  const long st[3] = {1, d->sx, d->sx * d->sy};
+  const int nn[3] = {d->nx, d->ny, d->nz};
+  const double *prs = w->prs;
+  for (int dir = 0; dir < 3; dir++) {
+    const long s = st[dir];
+    const int n = nn[dir];
+    int o1 = dir == 0 ? 1 : 0, o2 = dir == 2 ? 1 : 2;
+    int n1 = nn[o1], n2 = nn[o2];
+#pragma omp parallel
+    {
+      double *ps = (double *)malloc(sizeof(double) * (size_t)(n + 2));
+      double *D = (double *)malloc(sizeof(double) * 5 * (size_t)(n + 1));
+#pragma omp for collapse(2)
+      for (int b = 1; b <= n2; b++)
+        for (int a = 1; a <= n1; a++) {
+          int idx3[3];
+          idx3[o1] = a; idx3[o2] = b; idx3[dir] = 0;
+          long p0 = (long)ID(*d, idx3[0], idx3[1], idx3[2]); /* offset of index 0 */
+          for (int i = 0; i <= n + 1; i++) {
+            long p = p0 + i * s;
+            double ptrad = prs[p - s] + 2.0 * prs[p] + prs[p + s];
+            double ptvd = fabs(prs[p + s] - prs[p]) + fabs(prs[p] - prs[p - s]);
+            ps[i] = sens[p] * fabs(prs[p - s] - 2.0 * prs[p] + prs[p + s]) / (cs * ptrad + (1.0 - cs) * ptvd);
+          }
+          for (int i = 0; i <= n; i++) {
+            long p = p0 + i * s;
+            double xk0 = xnu0 * fmax(ps[i + 1], ps[i]);
+            double xk10 = fmax(0.0, xnu10 + xk0 / 105.0);
+            for (int v = 0; v < 5; v++) {
+              const double *u = Un[v];
+              D[v * (n + 1) + i] = (u[p + s] - u[p]) * xk0 +
+                                   (D10[1] * (u[p + s] - u[p]) + D10[2] * (u[p + 2 * s] - u[p - s]) +
+                                    D10[3] * (u[p + 3 * s] - u[p - 2 * s]) + D10[4] * (u[p + 4 * s] - u[p - 3 * s]) +
+                                    D10[5] * (u[p + 5 * s] - u[p - 4 * s])) *
+                                       xk10;
+            }
+          }
+          for (int i = 1; i <= n; i++) {
+            long p = p0 + i * s;
+            for (int v = 0; v < 5; v++) {
+              const double *Dv = D + v * (n + 1);
+              if (dir == 0) K[v][p] = Dv[i - 1] - Dv[i];
+              else K[v][p] = K[v][p] + Dv[i - 1] - Dv[i];
+            }
+          }
+        }
+      free(ps);
+      free(D);
+    }
+  }
+  free(sens);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Public entry points                                                       */
+/* ------------------------------------------------------------------------ */
+static int check_cfg(const orc_cfg *c) {
+  if (c->nx < NG || c->ny < NG || c->nz < NG) return -1;
+  for (int dir = 0; dir < 3; dir++)
+    for (int s = 0; s < 2; s++)
+      if (c->face[dir][s] != ORC_FACE_PERIODIC) return -2; /* only periodic faces so far */
+  if (c->filter == 2 && !c->visc) return -3;              /* sensor needs the viscous gradients */
+  if (c->nrk < 1 || c->nrk > 8) return -4;
+  return 0;
+}
+
+static int work_alloc(const dims_t *d, work_t *w, int visc) {
+  memset(w, 0, sizeof(*w));
+  w->uu = alloc_arr(d); w->vv = alloc_arr(d); w->ww = alloc_arr(d);
+  w->prs = alloc_arr(d); w->tmp = alloc_arr(d);
+  if (!w->uu || !w->vv || !w->ww || !w->prs || !w->tmp) return -1;
+  if (visc) {
+    w->cs = alloc_arr(d); w->visc = alloc_arr(d); w->cok = alloc_arr(d);
+    if (!w->cs || !w->visc || !w->cok) return -1;
+    for (int q = 0; q < 12; q++) {
+      w->grad[q] = alloc_arr(d);
+      if (!w->grad[q]) return -1;
+    }
+  }
+  return 0;
+}
+
+static void work_free(work_t *w) {
+  free(w->uu); free(w->vv); free(w->ww); free(w->prs); free(w->tmp);
+  free(w->cs); free(w->visc); free(w->cok);
+  for (int q = 0; q < 12; q++) free(w->grad[q]);
+}
+
+void orc_euler_rhs(const orc_cfg *c, const orc_grid *g, double *const U[5], double *K[5]) {
+  dims_t d = mkdims(c);
+  work_t w;
+  if (work_alloc(&d, &w, 0)) { work_free(&w); return; }
+  prim_euler(c, &d, U, &w);
+  euler_rhs_w(c, g, &d, U, &w, K);
+  work_free(&w);
+}
+
+void orc_visc_rhs(const orc_cfg *c, const orc_grid *g, double *const Uref[5], double *K[5]) {
+  dims_t d = mkdims(c);
+  work_t w;
+  if (work_alloc(&d, &w, 1)) { work_free(&w); return; }
+  prim_visc(c, &d, Uref, &w);
+  gradients(c, g, &d, &w);
+  visc_rhs_w(c, g, &d, &w, K);
+  work_free(&w);
+}
+
+void orc_filter_rhs(const orc_cfg *c, const orc_grid *g, double *const Un[5], double *const Uref[5],
+                    double *K[5]) {
+  (void)g;
+  dims_t d = mkdims(c);
+  if (c->filter == 1) {
+    filter_sf(c, &d, Un, K);
+  } else if (c->filter == 2) {
+    work_t w;
+    if (work_alloc(&d, &w, 1)) { work_free(&w); return; }
+    prim_visc(c, &d, Uref, &w);
+    gradients(c, g, &d, &w);
+    filter_shock(c, &d, Un, &w, K);
+    work_free(&w);
+  }
+}
+
+int orc_step(const orc_cfg *c, const orc_grid *g, double *U[5], int nsteps) {
+  int rc = check_cfg(c);
+  if (rc) return rc;
+  dims_t d = mkdims(c);
+  work_t w;
+  double *Un[5], *K[5];
+  for (int v = 0; v < 5; v++) { Un[v] = alloc_arr(&d); K[v] = alloc_arr(&d); }
+  if (work_alloc(&d, &w, c->visc)) rc = -10;
+  for (int v = 0; v < 5; v++) if (!Un[v] || !K[v]) rc = -10;
+  if (rc) goto out;
+
+  for (int step = 0; step < nsteps; step++) {
+    /* U_n = U (solver.f90:199-203 / end of previous step runge_kutta.f90:146-150) */
+    for (int v = 0; v < 5; v++) memcpy(Un[v], U[v], d.n * sizeof(double));
+    for (int irk = 1; irk <= c->nrk; irk++) {
+      prim_euler(c, &d, Un, &w);                      /* primitives(U_n) */
+      euler_rhs_w(c, g, &d, Un, &w, K);               /* flux_euler       */
+      if (irk == c->nrk && c->visc) {
+        prim_visc(c, &d, U, &w);                      /* primitives_visc(U^n) */
+        gradients(c, g, &d, &w);                      /* grad_vel, grad_T, communication_v */
+        visc_rhs_w(c, g, &d, &w, K);                  /* flux_visc        */
+      }
+      const double alpha = c->dt * c->crk[irk - 1];   /* update_var, update.f90:32-44 */
+#pragma omp parallel for collapse(2)
+      for (int k = 1; k <= d.nz; k++)
+        for (int j = 1; j <= d.ny; j++)
+          for (int i = 1; i <= d.nx; i++) {
+            long p = (long)ID(d, i, j, k);
+            for (int v = 0; v < 5; v++) Un[v][p] = U[v][p] - alpha * K[v][p];
+          }
+      for (int v = 0; v < 5; v++) orc_fill_periodic(c, Un[v]); /* communication */
+    }
+    if (c->filter) {
+      /* num_dissip then update_varn (runge_kutta.f90:113-126; update.f90:538-565) */
+      if (c->filter == 1) filter_sf(c, &d, Un, K);
+      else filter_shock(c, &d, Un, &w, K);           /* sensor on p, grads of U^n */
+#pragma omp parallel for collapse(2)
+      for (int k = 1; k <= d.nz; k++)
+        for (int j = 1; j <= d.ny; j++)
+          for (int i = 1; i <= d.nx; i++) {
+            long p = (long)ID(d, i, j, k);
+            for (int v = 0; v < 5; v++) Un[v][p] = Un[v][p] + K[v][p];
+          }
+      for (int v = 0; v < 5; v++) orc_fill_periodic(c, Un[v]);
+    }
+    for (int v = 0; v < 5; v++) memcpy(U[v], Un[v], d.n * sizeof(double)); /* U = U_n */
+  }
+out:
+  for (int v = 0; v < 5; v++) { free(Un[v]); free(K[v]); }
+  work_free(&w);
+  return rc;
+}

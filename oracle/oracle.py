@@ -1,0 +1,148 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product path never imports
+this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+_dp = C.POINTER(C.c_double)
+
+
+class OrcCfg(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("curv", C.c_int), ("visc", C.c_int), ("filter", C.c_int), ("ducros", C.c_int),
+        ("nrk", C.c_int),
+        ("face", (C.c_int * 2) * 3),
+        ("pad_", C.c_int),
+        ("crk", C.c_double * 8),
+        ("dt", C.c_double),
+        ("gam", C.c_double), ("rg", C.c_double),
+        ("c0", C.c_double), ("S0", C.c_double), ("diffscale", C.c_double),
+        ("cok_ratio", C.c_double),
+        ("chi", C.c_double), ("dissip_shock", C.c_double), ("csens", C.c_double),
+    ]
+
+
+class OrcGrid(C.Structure):
+    _fields_ = [
+        ("idx", _dp), ("idy", _dp), ("idz", _dp),
+        ("idx_v", _dp), ("idy_v", _dp), ("idz_v", _dp),
+        ("m", _dp * 10),
+        ("mv", _dp * 10),
+    ]
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_step.argtypes = [C.POINTER(OrcCfg), C.POINTER(OrcGrid), C.POINTER(_dp), C.c_int]
+        L.orc_step.restype = C.c_int
+        for f in ("orc_euler_rhs", "orc_visc_rhs"):
+            getattr(L, f).argtypes = [C.POINTER(OrcCfg), C.POINTER(OrcGrid), C.POINTER(_dp), C.POINTER(_dp)]
+            getattr(L, f).restype = None
+        L.orc_filter_rhs.argtypes = [C.POINTER(OrcCfg), C.POINTER(OrcGrid), C.POINTER(_dp),
+                                     C.POINTER(_dp), C.POINTER(_dp)]
+        L.orc_filter_rhs.restype = None
+        L.orc_fill_periodic.argtypes = [C.POINTER(OrcCfg), _dp]
+        L.orc_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+class Oracle:
+    """CPU restatement of the reference step for one periodic block."""
+
+    def __init__(self, case):
+        self.case = case
+        c = OrcCfg()
+        c.nx, c.ny, c.nz = case.nx, case.ny, case.nz
+        c.curv = int(case.curvilinear)
+        c.visc = int(case.iorder_visc != 0)
+        c.filter = int(case.filter)
+        c.ducros = int(case.ducros)
+        c.nrk = int(case.nrk)
+        for d in range(3):
+            for s in range(2):
+                c.face[d][s] = 0
+        for q, v in enumerate(case.crk):
+            c.crk[q] = v
+        c.dt = case.dt
+        c.gam, c.rg, c.c0, c.S0 = case.gam, case.rg, case.c0, case.S0
+        c.diffscale, c.cok_ratio = case.diffscale, case.cok_ratio
+        c.chi, c.dissip_shock, c.csens = case.chi, case.dissip_shock, case.csens
+        self.cfg = c
+        g = OrcGrid()
+        self._keep = []
+        if case.curvilinear:
+            for q in range(10):
+                a = np.ascontiguousarray(case.m[q])
+                self._keep.append(a)
+                g.m[q] = _p(a)
+                if case.mv is not None:
+                    b = np.ascontiguousarray(case.mv[q])
+                    self._keep.append(b)
+                    g.mv[q] = _p(b)
+        else:
+            arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in case.cart]
+            self._keep += arrs
+            g.idx, g.idy, g.idz, g.idx_v, g.idy_v, g.idz_v = [_p(a) for a in arrs]
+        self.grid = g
+
+    @staticmethod
+    def _tab(arrs):
+        return (_dp * 5)(*[_p(a) for a in arrs])
+
+    def step(self, U, nsteps: int = 1):
+        """Advance the list of 5 padded arrays U in place."""
+        rc = lib().orc_step(C.byref(self.cfg), C.byref(self.grid), self._tab(U), int(nsteps))
+        if rc != 0:
+            raise RuntimeError(f"orc_step failed rc={rc}")
+        return U
+
+    def euler_rhs(self, U):
+        K = [np.zeros_like(U[0]) for _ in range(5)]
+        lib().orc_euler_rhs(C.byref(self.cfg), C.byref(self.grid), self._tab(U), self._tab(K))
+        return K
+
+    def visc_rhs(self, U, K=None):
+        if K is None:
+            K = [np.zeros_like(U[0]) for _ in range(5)]
+        lib().orc_visc_rhs(C.byref(self.cfg), C.byref(self.grid), self._tab(U), self._tab(K))
+        return K
+
+    def filter_rhs(self, Un, Uref=None):
+        K = [np.zeros_like(Un[0]) for _ in range(5)]
+        Uref = Un if Uref is None else Uref
+        lib().orc_filter_rhs(C.byref(self.cfg), C.byref(self.grid), self._tab(Un), self._tab(Uref),
+                             self._tab(K))
+        return K
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())

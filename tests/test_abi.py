@@ -1,0 +1,31 @@
+"""The C-ABI library loads and exports every symbol include/musica_gpu.h
+declares (no compute calls: runs without a GPU)."""
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "musica_gpu.h")).read()
+    return sorted(set(re.findall(r"\b(mgpu_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_the_binding_list():
+    from musica2_newgrid_amd import _lib
+    assert sorted(_lib.EXPORTS) == declared_symbols()
+
+
+def test_library_exports_all_symbols():
+    from musica2_newgrid_amd import _lib
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_struct_layout_matches_header():
+    from musica2_newgrid_amd._lib import MgpuBlock, MgpuCase
+    assert ctypes.sizeof(MgpuBlock) == 4 * (3 + 24)
+    # 8 int32, one pointer, 8 crk, deltat and 9 gas/filter doubles
+    assert ctypes.sizeof(MgpuCase) == 32 + 8 + 8 * (8 + 1 + 9)
