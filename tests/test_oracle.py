@@ -1,0 +1,261 @@
+"""Pins of the CPU oracle against what the mathematics fixes.
+
+The reference ships no tests, fixtures or golden vectors for this path, and
+its Fortran subset cannot be built in this image (it needs an `mpi` module for
+flang), so each oracle routine is pinned here by a closed form, an invariant
+or a limit chosen so that a plausible slip (a wrong coefficient, sign, index,
+stage weight or operand) fails:
+
+* 11-pt derivative: observed order of accuracy 10 on a Fourier mode, per axis;
+* RK + Euler + update choreography: one step on a linear entropy wave equals
+  the RK4 amplification polynomial G(z) of the semi-discrete eigenvalue;
+* selective filter: transfer function 1 - chi*sin^10(theta/2) (0.4 at Nyquist
+  for chi = 0.2) and zero response to constants;
+* viscous terms: a shear wave decays with the 5-pt double-derivative symbol,
+  Sutherland viscosity, once per step and lagged to U^n (the exact
+  amplitude is G(z) - dt*nu*k5^2, not the stage-3 value);
+* curvilinear (c3) metrics: GCL free-stream preservation on a distorted grid,
+  and an undistorted c3 block equals the Cartesian block;
+* conservation: the periodic sums of every increment vanish;
+* shock filter: with a flat pressure it is the 10th-order dissipation whose
+  symbol is -(chi/1260)*2^10*sin^10(theta/2); the Ducros sensor switches the
+  low-order term off in a purely rotational flow.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from musica2_newgrid_amd import cases
+from musica2_newgrid_amd import grid as gr
+from oracle.oracle import Oracle
+from tests._util import relerr
+
+G = gr.G
+RHO0, P0 = 1.177, 101325.0
+
+
+def wave_case(n, axis, field, amp, u0=(0.0, 0.0, 0.0), visc=0, filt=0, chi=0.2, dt=None, L=2 * math.pi,
+              m=1):
+    """Periodic Cartesian n^3 box; `field` ('rho','v','p') carries
+    amp*sin(m*2pi*x_axis/L)."""
+    h = L / n
+    x = np.arange(n) * h
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    ph = m * 2 * math.pi * (X, Y, Z)[axis] / L
+    s = np.cos(ph) if 2 * m == n else np.sin(ph)  # the Nyquist sine vanishes on the grid
+    rho = np.full_like(X, RHO0)
+    u, v, w = (np.full_like(X, c) for c in u0)
+    p = np.full_like(X, P0)
+    if field == "rho":
+        rho = rho + amp * s
+    elif field == "v":
+        v = v + amp * s
+    elif field == "p":
+        p = p + amp * s
+    c = cases.cartesian_box(n, n, n, (L, L, L), (rho, u, v, w, p), iorder_visc=visc, filter=filt,
+                            dt=dt or 1e-5, chi=chi)
+    c.diffscale = 50.0
+    return c, h
+
+
+def k11(theta):
+    a = (5.0 / 6.0, -5.0 / 21.0, 5.0 / 84.0, -5.0 / 504.0, 1.0 / 1260.0)
+    return 2.0 * sum(al * math.sin((l + 1) * theta) for l, al in enumerate(a))
+
+
+def k5(theta):
+    return (4.0 / 3.0) * math.sin(theta) - (1.0 / 6.0) * math.sin(2.0 * theta)
+
+
+def mode(field_int, n, axis, m=1):
+    """Complex amplitude of the sin/cos(m*2pi*i/n) mode of an interior field."""
+    idx = np.arange(n)
+    ph = np.exp(1j * 2 * math.pi * m * idx / n)
+    f = np.moveaxis(field_int, 2 - axis, -1)  # the chosen axis last
+    return complex(np.mean(f * ph[None, None, :].conj()) * 2.0)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_euler_derivative_is_tenth_order(axis):
+    """K_rho = u0 d(rho)/dx for an entropy wave in uniform flow: the 11-pt
+    operator's error drops by ~2^10 when the mode resolution doubles."""
+    errs = []
+    for n in (16, 32):
+        u0 = [0.0, 0.0, 0.0]
+        u0[axis] = 30.0
+        c, h = wave_case(n, axis, "rho", 1e-3, u0=tuple(u0))
+        K = Oracle(c).euler_rhs(c.U)
+        x = np.arange(n) * h
+        exact = 30.0 * 1e-3 * np.cos(x)  # d/dx of amp*sin(x)
+        Ki = np.moveaxis(gr.interior(K[0]), 2 - axis, -1)
+        errs.append(float(np.max(np.abs(Ki - exact[None, None, :]))))
+    order = math.log2(errs[0] / errs[1])
+    assert 9.5 < order < 10.5, (errs, order)
+
+
+@pytest.mark.parametrize("m", [1, 5, 11])
+def test_rk_step_entropy_wave_matches_amplification_polynomial(m):
+    n = 32
+    c, h = wave_case(n, 0, "rho", 1e-3, u0=(40.0, 0.0, 0.0), m=m)
+    c.dt = 0.7 * h / 40.0
+    U = c.copy_state()
+    Oracle(c).step(U, 1)
+    theta = 2 * math.pi * m / n
+    z = -1j * 40.0 * k11(theta) / h * c.dt
+    Gz = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
+    a0 = mode(gr.interior(c.U[0]) - RHO0, n, 0, m)
+    a1 = mode(gr.interior(U[0]) - RHO0, n, 0, m)
+    assert abs(a1 / a0 - Gz) < 1e-9, (a1 / a0, Gz)
+
+
+@pytest.mark.parametrize("m", [3, 8, 16])
+def test_selective_filter_transfer_function(m):
+    n = 32
+    chi = 0.2
+    c, h = wave_case(n, 1, "rho", 1e-3, u0=(0.0, 0.0, 0.0), filt=1, chi=chi, m=m)
+    U = c.copy_state()
+    o = Oracle(c)
+    o.step(U, 1)  # no flow: Euler K = 0 on a density wave at rest with flat pressure
+    theta = 2 * math.pi * m / n
+    a0 = mode(gr.interior(c.U[0]) - RHO0, n, 1, m)
+    a1 = mode(gr.interior(U[0]) - RHO0, n, 1, m)
+    expect = 1.0 - chi * math.sin(theta / 2) ** 10
+    assert abs(a1 / a0 - expect) < 1e-9, (a1 / a0, expect)
+    if m == 16:
+        assert abs(expect - 0.8) < 1e-15  # Nyquist: 1 - chi
+
+
+def test_filters_leave_constants_untouched():
+    n = 16
+    c, _ = wave_case(n, 0, "rho", 0.0, filt=1)
+    K = Oracle(c).filter_rhs(c.U)
+    assert max(float(np.max(np.abs(gr.interior(k)))) for k in K) < 1e-9
+
+
+def test_viscous_shear_wave_lagged_to_step_start():
+    """v = eps sin(x) advected by u0 along x with viscosity: after one step the
+    mode amplitude is G(z) - dt*nu*k5^2 (viscous term once, on U^n)."""
+    n = 32
+    c, h = wave_case(n, 0, "v", 1e-4, u0=(40.0, 0.0, 0.0), visc=4, m=2)
+    c.dt = 0.5 * h / 40.0
+    T0 = P0 / (RHO0 * c.rg)
+    mu = 1.4579326545176254e-06 * math.sqrt(T0 ** 3) / (T0 + 110.4) * c.diffscale
+    nu = mu / RHO0
+    U = c.copy_state()
+    Oracle(c).step(U, 1)
+    theta = 2 * math.pi * 2 / n
+    z = -1j * 40.0 * k11(theta) / h * c.dt
+    Gz = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
+    expect = Gz - c.dt * nu * (k5(theta) / h) ** 2
+    a0 = mode(gr.interior(c.U[2]), n, 0, 2)
+    a1 = mode(gr.interior(U[2]), n, 0, 2)
+    ratio = a1 / a0
+    assert abs(ratio - expect) < 1e-9, (ratio, expect)
+    # the stage-3 lagging alternative differs measurably
+    G3 = 1 + z / 2 + z ** 2 / 6 + z ** 3 / 24
+    assert abs(ratio - (Gz - c.dt * nu * (k5(theta) / h) ** 2 * G3)) > 1e-7
+
+
+def test_curvilinear_free_stream_preservation():
+    c = cases.vortex_wavy(24, 20, 16, filter=2)
+    U = [gr.padded(24, 20, 16, v) for v in cases.conserved(1.177, 150.0, 20.0, -10.0, 101325.0)]
+    o = Oracle(c)
+    K = o.euler_rhs(U)
+    scale = [1.177 * 150 / c.info["dmin"], 1.0, 1.0, 1.0, 3.6e5 * 150 / c.info["dmin"]]
+    assert float(np.max(np.abs(gr.interior(K[0])))) < 1e-12 * scale[0]
+    assert float(np.max(np.abs(gr.interior(K[4])))) < 1e-12 * scale[4]
+    U1 = [a.copy() for a in U]
+    c.U = U
+    o.step(U1, 2)
+    for a, b in zip(U1, U):
+        assert float(np.max(np.abs(gr.interior(a) - gr.interior(b)))) <= 1e-12 * float(np.max(np.abs(b)))
+
+
+def test_affine_curvilinear_equals_cartesian():
+    n = 16
+    cc = cases.tgv(n)
+    c3 = cases.tgv(n)
+    c3.curvilinear = True
+    L = cc.info["L"]
+    c3.m, c3.mv = cases.affine_c3_metrics(n, n, n, (L, L, L))
+    A, B = cc.copy_state(), c3.copy_state()
+    Oracle(cc).step(A, 2)
+    Oracle(c3).step(B, 2)
+    assert relerr(B, A) < 1e-13
+
+
+def test_periodic_increments_are_conservative():
+    """Telescoping: on a periodic block every increment (Euler, viscous,
+    selective filter, shock filter) sums to zero up to round-off: 1e-13 of
+    the sum of its magnitudes plus the rounding of the state it is built from
+    (64 ulp of max|U| per cell)."""
+    n = 16
+    L = 2 * math.pi
+    x = np.arange(n) * (L / n)
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    rho = RHO0 * (1.0 + 0.05 * np.sin(X + 2 * Y) * np.cos(Z))
+    u = 30.0 * np.sin(X) * np.cos(Y)
+    v = -20.0 * np.cos(X + Z)
+    w = 10.0 * np.sin(Y - Z)
+    p = P0 * (1.0 + 0.02 * np.cos(2 * X) * np.sin(Y + Z)) + 3e3 * (np.cos(X) > 0.6)
+    c = cases.cartesian_box(n, n, n, (L, L, L), (rho, u, v, w, p), iorder_visc=4, filter=1)
+    c.diffscale = 100.0
+    o = Oracle(c)
+    incs = {"euler": o.euler_rhs(c.U), "visc": o.visc_rhs(c.U), "sf": o.filter_rhs(c.U)}
+    c.filter, c.dissip_shock, c.ducros = 2, 1.0, True
+    incs["shock"] = Oracle(c).filter_rhs(c.U)
+    for name, K in incs.items():
+        for q in range(5):
+            Ki = gr.interior(K[q])
+            tot = float(np.sum(np.abs(Ki)))
+            if tot == 0.0:
+                continue
+            ulp = 64 * np.finfo(np.float64).eps * float(np.max(np.abs(gr.interior(c.U[q])))) * n ** 3
+            assert abs(float(np.sum(Ki))) < 1e-13 * tot + ulp, (name, q)
+
+
+@pytest.mark.parametrize("m", [4, 9, 16])
+def test_shock_filter_flat_pressure_symbol(m):
+    """Flat pressure: sensor psi = 0, so D = xnu10 * sum d10 (...) and the
+    increment of a mode is -(chi/1260)*2^10*sin^10(theta/2)."""
+    n = 32
+    chi = 0.2
+    c, h = wave_case(n, 2, "rho", 1e-3, filt=2, chi=chi, m=m, visc=4)
+    c.ducros = False
+    K = Oracle(c).filter_rhs(c.U)
+    theta = 2 * math.pi * m / n
+    a0 = mode(gr.interior(c.U[0]) - RHO0, n, 2, m)
+    ak = mode(gr.interior(K[0]), n, 2, m)
+    expect = -(chi / 1260.0) * 2 ** 10 * math.sin(theta / 2) ** 10
+    assert abs(ak / a0 - expect) < 1e-9, (ak / a0, expect)
+
+
+def test_ducros_sensor_switches_off_low_order_term_in_rotation():
+    """Solid-body rotation (zero dilatation) over a pressure bump: with the
+    Ducros sensor the increment equals the pure 10th-order one (xnu0 = 0);
+    without it the low-order term acts."""
+    n = 24
+    L = 2 * math.pi
+    x = np.arange(n) * (L / n)
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    om = 5.0
+    u = -om * np.sin(Y)
+    v = om * np.sin(X)
+    w = np.zeros_like(X)
+    p = P0 * (1.0 + 0.3 * (np.cos(X) > 0.5))
+    rho = np.full_like(X, RHO0)
+    base = cases.cartesian_box(n, n, n, (L, L, L), (rho, u, v, w, p), iorder_visc=4, filter=2)
+    base.diffscale = 1.0
+    base.ducros = True
+    base.dissip_shock = 1.0
+    K_duc = Oracle(base).filter_rhs(base.U)
+    base.dissip_shock = 0.0
+    K_lin = Oracle(base).filter_rhs(base.U)
+    base.ducros = False
+    base.dissip_shock = 1.0
+    K_noduc = Oracle(base).filter_rhs(base.U)
+    d_duc = max(float(np.max(np.abs(gr.interior(a) - gr.interior(b)))) for a, b in zip(K_duc, K_lin))
+    d_noduc = max(float(np.max(np.abs(gr.interior(a) - gr.interior(b)))) for a, b in zip(K_noduc, K_lin))
+    assert d_noduc > 1.0          # the pressure jump triggers the low-order term
+    assert d_duc < 1e-6 * d_noduc  # theta ~ div^2/(div^2+|w|^2) ~ 0 in rotation
