@@ -58,6 +58,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--ny", type=int, default=256)
@@ -72,6 +73,9 @@ def main() -> int:
 
     case = cases.vortex_wavy(args.nx, args.ny, args.nz, filter=2)
     solver = GpuSolver(case)
+    if not args.no_tune:
+        solver.autotune()
+    tuned = {"euler_stages": solver.stage_config(0), "last_stage": solver.stage_config(1)}
     solver.step(args.warmup)
     solver.sync()
 
@@ -106,7 +110,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"curvilinear block {args.nx}x{args.ny}x{args.nz}, isentropic vortex "
                                "M=0.5, viscous, shock filter + Ducros, RK4, FP64",
-                   "cells": case.ncells, "nrk": case.nrk, "precision": "fp64"},
+                   "cells": case.ncells, "nrk": case.nrk, "precision": "fp64",
+                   "stage_tiles": {k: {"tile": f"{v[0]}x{v[1]}", "kchunk": v[2]} for k, v in tuned.items()}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_stage", "avg_launch_ms": round(avg_stage_s * 1e3, 4),

@@ -128,6 +128,18 @@ int mgpu_get_state(mgpu_handle *h, int block, double *rho, double *rhou, double 
  * the handle's stream. */
 int mgpu_step(mgpu_handle *h, int nsteps);
 
+/* Tile autotuner of the stage kernel.  For every block (and kernel variant:
+ * Euler stages, last stage with the viscous divergence) it times each tile
+ * shape (i x j workgroup tile) and k-chunk length on the block's own data
+ * and keeps the fastest; results are cached per process by block geometry.
+ * flags bit 0: re-tune even when cached.  Synchronous.  Needs metrics and
+ * state set.  No counterpart in the reference (its loops have no tiles). */
+int mgpu_autotune(mgpu_handle *h, int flags);
+
+/* The stage-kernel launch parameters in use for a block: variant 0 = Euler
+ * stages, 1 = last stage with viscous divergence. */
+int mgpu_stage_config(mgpu_handle *h, int block, int variant, int *shape_tx, int *shape_ty, int *kchunk);
+
 /* Wait for all work queued on the handle. */
 int mgpu_sync(mgpu_handle *h);
 

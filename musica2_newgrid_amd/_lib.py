@@ -28,6 +28,8 @@ EXPORTS = (
     "mgpu_put_state",
     "mgpu_get_state",
     "mgpu_step",
+    "mgpu_autotune",
+    "mgpu_stage_config",
     "mgpu_sync",
     "mgpu_profile",
     "mgpu_kernel_time",
@@ -93,6 +95,8 @@ def lib() -> C.CDLL:
     L.mgpu_put_state.argtypes = [C.c_void_p, C.c_int] + [_dp] * 5
     L.mgpu_get_state.argtypes = [C.c_void_p, C.c_int] + [_dp] * 5
     L.mgpu_step.argtypes = [C.c_void_p, C.c_int]
+    L.mgpu_autotune.argtypes = [C.c_void_p, C.c_int]
+    L.mgpu_stage_config.argtypes = [C.c_void_p, C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 3
     L.mgpu_sync.argtypes = [C.c_void_p]
     L.mgpu_profile.argtypes = [C.c_void_p, C.c_int]
     L.mgpu_kernel_time.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
@@ -101,7 +105,7 @@ def lib() -> C.CDLL:
     L.mgpu_destroy.argtypes = [C.c_void_p]
     L.mgpu_destroy.restype = None
     for name in ("mgpu_create", "mgpu_set_metrics_cart", "mgpu_set_metrics_c3", "mgpu_put_state",
-                 "mgpu_get_state", "mgpu_step", "mgpu_sync", "mgpu_profile", "mgpu_kernel_time"):
+                 "mgpu_get_state", "mgpu_step", "mgpu_autotune", "mgpu_stage_config", "mgpu_sync", "mgpu_profile", "mgpu_kernel_time"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

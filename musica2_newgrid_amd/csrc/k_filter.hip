@@ -6,98 +6,386 @@
 //                      pressure sensor :243-260, face dissipation :419-440,
 //                      increment :690-700 (and the j, k repeats)
 // Interior rows only (periodic faces and 1-to-1 interfaces).
+//
+// 2.5-D layout: the state of the tile + 5-cell halos of the current k plane
+// sits in LDS for the i/j stencils, the own column keeps an 11-deep register
+// queue for k.  The shock filter also stages p(U^n) and the Ducros sensor
+// and computes each pressure sensor once per plane in LDS.  Halo cells are
+// statically assigned to lanes so that all global loads of a plane are
+// issued in one batch.
 #include "mgpu_internal.h"
 
 namespace mgpu {
 
 namespace {
 
-constexpr int FX = 64, FY = 4;
+constexpr int FX = 32, FY = 16, FT = FX * FY;
+constexpr int NHX = 10 * FY, NH = NHX + 10 * FX, NS = (NH + FT - 1) / FT;  // state halo cells
+constexpr int NPX = FY * (FX + 4), NPY = (FY + 4) * FX;                  // p cells along i / j
+constexpr int NSP = (NPX + NPY + FT - 1) / FT;
 
-__global__ __launch_bounds__(FX *FY) void k_filter_sf(FilterArgs a) {
-  const Geom &g = a.g;
-  const int i = blockIdx.x * FX + threadIdx.x + 1;
-  const int j = blockIdx.y * FY + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
-  if (i > g.nx || j > g.ny) return;
-  const long p = lin(g, i, j, k);
-  const long sy = g.sx, sz = g.sy;
-  const double *dd = a.dd;
+struct Tiling {
+  int tiles_x, tiles_y, ntiles, kchunk;
+};
+
+__device__ __forceinline__ void tile_of(const Tiling &t, int &i0, int &j0, int &kbeg, int &kend, int nz) {
+  const int logical = xcd_remap(blockIdx.x, t.ntiles);
+  const int tile_x = logical % t.tiles_x;
+  const int rest = logical / t.tiles_x;
+  const int tile_y = rest % t.tiles_y;
+  const int chunk = rest / t.tiles_y;
+  i0 = tile_x * FX + 1;
+  j0 = tile_y * FY + 1;
+  kbeg = chunk * t.kchunk + 1;
+  kend = min(kbeg + t.kchunk - 1, nz);
+}
+
+// Static assignment of the 5-cell state halo of a plane to the lanes.
+struct Halo {
+  idx_t off[NS];
+  int r[NS], c[NS];
+  bool on[NS], xdir[NS];
+};
+
+__device__ __forceinline__ void halo_setup(const Geom &g, int i0, int j0, Halo &h) {
+  const int tid = threadIdx.x;
 #pragma unroll
-  for (int v = 0; v < 5; v++) {
-    const double *u = a.uin.v[v];
-    double K = dd[0] * u[p] + dd[1] * (u[p + 1] + u[p - 1]) + dd[2] * (u[p + 2] + u[p - 2]) +
-               dd[3] * (u[p + 3] + u[p - 3]) + dd[4] * (u[p + 4] + u[p - 4]) + dd[5] * (u[p + 5] + u[p - 5]);
-    K = K + dd[0] * u[p] + dd[1] * (u[p + sy] + u[p - sy]) + dd[2] * (u[p + 2 * sy] + u[p - 2 * sy]) +
-        dd[3] * (u[p + 3 * sy] + u[p - 3 * sy]) + dd[4] * (u[p + 4 * sy] + u[p - 4 * sy]) +
-        dd[5] * (u[p + 5 * sy] + u[p - 5 * sy]);
-    K = K + dd[0] * u[p] + dd[1] * (u[p + sz] + u[p - sz]) + dd[2] * (u[p + 2 * sz] + u[p - 2 * sz]) +
-        dd[3] * (u[p + 3 * sz] + u[p - 3 * sz]) + dd[4] * (u[p + 4 * sz] + u[p - 4 * sz]) +
-        dd[5] * (u[p + 5 * sz] + u[p - 5 * sz]);
-    a.uout.v[v][p] = u[p] + K;
+  for (int s = 0; s < NS; s++) {
+    const int t = tid + s * FT;
+    h.on[s] = false;
+    h.xdir[s] = true;
+    h.r[s] = h.c[s] = 0;
+    h.off[s] = 0;
+    if (t < NHX) {
+      const int r = t / 10, c = t % 10;
+      const int li = c < 5 ? c : FX + c;
+      const int gi = i0 - 5 + li, gj = j0 + r;
+      h.on[s] = gj <= g.ny && gi <= g.nx + 5;
+      h.r[s] = r;
+      h.c[s] = li;
+      h.off[s] = lin(g, gi, gj, 0);
+    } else if (t < NH) {
+      const int tt = t - NHX;
+      const int r = tt / FX, c = tt % FX;
+      const int lj = r < 5 ? r : FY + r;
+      const int gi = i0 + c, gj = j0 - 5 + lj;
+      h.on[s] = gi <= g.nx && gj <= g.ny + 5;
+      h.xdir[s] = false;
+      h.r[s] = lj;
+      h.c[s] = c;
+      h.off[s] = lin(g, gi, gj, 0);
+    }
   }
 }
 
-// pressure sensor psens at padded offset m along stride s
-__device__ __forceinline__ double psens(const FilterArgs &a, long m, long s) {
-  const double *pr = a.prs;
-  const double ptrad = pr[m - s] + 2.0 * pr[m] + pr[m + s];
-  const double ptvd = fabs(pr[m + s] - pr[m]) + fabs(pr[m] - pr[m - s]);
-  const double sn = a.sens ? a.sens[m] : 1.0;
-  return sn * fabs(pr[m - s] - 2.0 * pr[m] + pr[m + s]) / (a.csens * ptrad + (1.0 - a.csens) * ptvd);
+__device__ __forceinline__ void ld5(const CState &U, idx_t p, double o[5]) {
+#pragma unroll
+  for (int v = 0; v < 5; v++) o[v] = ldg(U.v[v], p);
 }
 
-// face dissipation D_{m+1/2} of variable u along stride s
-__device__ __forceinline__ double dface(const double *u, long m, long s, double xk0, double xk10) {
-  return (u[m + s] - u[m]) * xk0 +
-         (D10_1 * (u[m + s] - u[m]) + D10_2 * (u[m + 2 * s] - u[m - s]) + D10_3 * (u[m + 3 * s] - u[m - 2 * s]) +
-          D10_4 * (u[m + 4 * s] - u[m - 3 * s]) + D10_5 * (u[m + 5 * s] - u[m - 4 * s])) *
+__global__ __launch_bounds__(FT, 2) void k_filter_sf(FilterArgs a, Tiling tl) {
+  __shared__ double sux[5][FY][FX + 10];
+  __shared__ double suy[5][FY + 10][FX];
+  const Geom &g = a.g;
+  int i0, j0, kbeg, kend;
+  tile_of(tl, i0, j0, kbeg, kend, g.nz);
+  const int tx = threadIdx.x % FX, ty = threadIdx.x / FX;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool own = i <= g.nx && j <= g.ny;
+  const bool needx = i <= g.nx + 5 && j <= g.ny, needy = i <= g.nx && j <= g.ny + 5;
+  const idx_t sz = g.sy, own_off = lin(g, i, j, 0);
+  Halo hl;
+  halo_setup(g, i0, j0, hl);
+  const double *dd = a.dd;
+
+  double q[11][5];
+  if (own) {
+#pragma unroll
+    for (int l = 0; l < 10; l++) ld5(a.uin, own_off + (idx_t)(kbeg - 5 + l) * sz, q[l]);
+  }
+  for (int k = kbeg; k <= kend; k++) {
+    const idx_t pk = own_off + (idx_t)k * sz;
+    double uc[5], uh[NS][5];
+    if (own) ld5(a.uin, pk + 5 * sz, q[10]);
+    if (!own && (needx || needy)) ld5(a.uin, pk, uc);
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+      if (hl.on[s]) ld5(a.uin, hl.off[s] + (idx_t)k * sz, uh[s]);
+    if (needx || needy) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) {
+        const double x = own ? q[5][v] : uc[v];
+        if (needx) sux[v][ty][tx + 5] = x;
+        if (needy) suy[v][ty + 5][tx] = x;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      if (hl.on[s]) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+          if (hl.xdir[s]) sux[v][hl.r[s]][hl.c[s]] = uh[s][v];
+          else suy[v][hl.r[s]][hl.c[s]] = uh[s][v];
+        }
+      }
+    }
+    __syncthreads();
+    if (own) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) {
+        const double *x = &sux[v][ty][tx + 5];
+        const double *y = &suy[v][ty + 5][tx];
+        double K = dd[0] * x[0] + dd[1] * (x[1] + x[-1]) + dd[2] * (x[2] + x[-2]) + dd[3] * (x[3] + x[-3]) +
+                   dd[4] * (x[4] + x[-4]) + dd[5] * (x[5] + x[-5]);
+        K = K + dd[0] * y[0] + dd[1] * (y[FX] + y[-FX]) + dd[2] * (y[2 * FX] + y[-2 * FX]) +
+            dd[3] * (y[3 * FX] + y[-3 * FX]) + dd[4] * (y[4 * FX] + y[-4 * FX]) +
+            dd[5] * (y[5 * FX] + y[-5 * FX]);
+        K = K + dd[0] * q[5][v] + dd[1] * (q[6][v] + q[4][v]) + dd[2] * (q[7][v] + q[3][v]) +
+            dd[3] * (q[8][v] + q[2][v]) + dd[4] * (q[9][v] + q[1][v]) + dd[5] * (q[10][v] + q[0][v]);
+        stg_nt(a.uout.v[v], pk, q[5][v] + K);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < 10; l++)
+#pragma unroll
+      for (int v = 0; v < 5; v++) q[l][v] = q[l + 1][v];
+  }
+}
+
+// pressure sensor (mod_filtering_shock.f90:252-257) from p at m-1, m, m+1
+__device__ __forceinline__ double psens3(double pm, double p0, double pp, double sn, double cs) {
+  const double ptrad = pm + 2.0 * p0 + pp;
+  const double ptvd = fabs(pp - p0) + fabs(p0 - pm);
+  return sn * fabs(pm - 2.0 * p0 + pp) / (cs * ptrad + (1.0 - cs) * ptvd);
+}
+
+// D_{m+1/2} from u at m-4..m+5 given as x[-4..5] with stride s
+__device__ __forceinline__ double dface(const double *x, int s, double xk0, double xk10) {
+  return (x[s] - x[0]) * xk0 +
+         (D10_1 * (x[s] - x[0]) + D10_2 * (x[2 * s] - x[-s]) + D10_3 * (x[3 * s] - x[-2 * s]) +
+          D10_4 * (x[4 * s] - x[-3 * s]) + D10_5 * (x[5 * s] - x[-4 * s])) *
              xk10;
 }
 
-__global__ __launch_bounds__(FX *FY) void k_filter_shock(FilterArgs a) {
+__global__ __launch_bounds__(FT, 2) void k_filter_shock(FilterArgs a, Tiling tl) {
+  __shared__ double sux[5][FY][FX + 10];
+  __shared__ double suy[5][FY + 10][FX];
+  __shared__ double spx[FY][FX + 4];    // p at i0-2 .. i0+FX+1
+  __shared__ double spy[FY + 4][FX];    // p at j0-2 .. j0+FY+1
+  __shared__ double ssx[FY][FX + 4];    // sensor at i0-2 .. i0+FX+1 (ends unused)
+  __shared__ double ssy[FY + 4][FX];    // sensor at j0-2 .. j0+FY+1 (ends unused)
+  __shared__ double psx[FY][FX + 2];    // psens_x at i0-1 .. i0+FX
+  __shared__ double psy[FY + 2][FX];    // psens_y at j0-1 .. j0+FY
+
   const Geom &g = a.g;
-  const int i = blockIdx.x * FX + threadIdx.x + 1;
-  const int j = blockIdx.y * FY + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
-  if (i > g.nx || j > g.ny) return;
-  const long p = lin(g, i, j, k);
-  const long st[3] = {1, (long)g.sx, g.sy};
-  double K[5];
+  int i0, j0, kbeg, kend;
+  tile_of(tl, i0, j0, kbeg, kend, g.nz);
+  const int tid = threadIdx.x, tx = tid % FX, ty = tid / FX;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool own = i <= g.nx && j <= g.ny;
+  const bool needx = i <= g.nx + 5 && j <= g.ny, needy = i <= g.nx && j <= g.ny + 5;
+  const bool duc = a.sens != nullptr;
+  const double cs = a.csens;
+  const idx_t sz = g.sy, own_off = lin(g, i, j, 0);
+  Halo hl;
+  halo_setup(g, i0, j0, hl);
+  // p / sensor cells of this thread: rows of spx (i-direction) then of spy
+  idx_t pp_off[NSP];
+  int pp_r[NSP], pp_c[NSP];
+  bool pp_on[NSP], pp_x[NSP], pp_s[NSP];
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
-    const long s = st[d];
-    const double psm = psens(a, p - s, s), ps0 = psens(a, p, s), psp = psens(a, p + s, s);
-    const double xk0m = a.xnu0 * fmax(ps0, psm);
-    const double xk10m = fmax(0.0, a.xnu10 + xk0m / 105.0);
-    const double xk0p = a.xnu0 * fmax(psp, ps0);
-    const double xk10p = fmax(0.0, a.xnu10 + xk0p / 105.0);
-#pragma unroll
-    for (int v = 0; v < 5; v++) {
-      const double *u = a.uin.v[v];
-      const double dlo = dface(u, p - s, s, xk0m, xk10m);
-      const double dhi = dface(u, p, s, xk0p, xk10p);
-      if (d == 0) K[v] = dlo - dhi;
-      else K[v] = K[v] + dlo - dhi;
+  for (int s = 0; s < NSP; s++) {
+    const int t = tid + s * FT;
+    pp_on[s] = false;
+    pp_x[s] = true;
+    pp_s[s] = false;
+    pp_r[s] = pp_c[s] = 0;
+    pp_off[s] = 0;
+    if (t < NPX) {
+      const int r = t / (FX + 4), c = t % (FX + 4);
+      const int gi = i0 - 2 + c, gj = j0 + r;
+      pp_on[s] = gj <= g.ny && gi <= g.nx + 2;
+      pp_s[s] = c >= 1 && c <= FX + 2 && gi <= g.nx + 1;  // sensor needed at i0-1..i0+FX
+      pp_r[s] = r;
+      pp_c[s] = c;
+      pp_off[s] = lin(g, gi, gj, 0);
+    } else if (t < NPX + NPY) {
+      const int tt = t - NPX;
+      const int r = tt / FX, c = tt % FX;
+      const int gi = i0 + c, gj = j0 - 2 + r;
+      pp_on[s] = gi <= g.nx && gj <= g.ny + 2;
+      pp_s[s] = r >= 1 && r <= FY + 2 && gj <= g.ny + 1;
+      pp_x[s] = false;
+      pp_r[s] = r;
+      pp_c[s] = c;
+      pp_off[s] = lin(g, gi, gj, 0);
     }
   }
+
+  double q[11][5];   // U at k-5..k+5
+  double qp[5];      // p at k-2..k+2
+  double qs[3];      // sensor at k-1..k+1
+  if (own) {
 #pragma unroll
-  for (int v = 0; v < 5; v++) a.uout.v[v][p] = a.uin.v[v][p] + K[v];
+    for (int l = 0; l < 10; l++) ld5(a.uin, own_off + (idx_t)(kbeg - 5 + l) * sz, q[l]);
+#pragma unroll
+    for (int l = 0; l < 4; l++) qp[l] = ldg(a.prs, own_off + (idx_t)(kbeg - 2 + l) * sz);
+#pragma unroll
+    for (int l = 0; l < 2; l++) qs[l] = duc ? ldg(a.sens, own_off + (idx_t)(kbeg - 1 + l) * sz) : 1.0;
+  }
+
+  for (int k = kbeg; k <= kend; k++) {
+    const idx_t pk = own_off + (idx_t)k * sz;
+    // ---- every global load of the plane ----
+    double uc[5], uh[NS][5], pv[NSP], sv[NSP];
+    if (own) {
+      ld5(a.uin, pk + 5 * sz, q[10]);
+      qp[4] = ldg(a.prs, pk + 2 * sz);
+      qs[2] = duc ? ldg(a.sens, pk + sz) : 1.0;
+    }
+    if (!own && (needx || needy)) ld5(a.uin, pk, uc);
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+      if (hl.on[s]) ld5(a.uin, hl.off[s] + (idx_t)k * sz, uh[s]);
+#pragma unroll
+    for (int s = 0; s < NSP; s++) {
+      if (pp_on[s]) {
+        pv[s] = ldg(a.prs, pp_off[s] + (idx_t)k * sz);
+        sv[s] = (duc && pp_s[s]) ? ldg(a.sens, pp_off[s] + (idx_t)k * sz) : 1.0;
+      }
+    }
+    // ---- stage the plane in LDS ----
+    if (needx || needy) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) {
+        const double x = own ? q[5][v] : uc[v];
+        if (needx) sux[v][ty][tx + 5] = x;
+        if (needy) suy[v][ty + 5][tx] = x;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      if (hl.on[s]) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+          if (hl.xdir[s]) sux[v][hl.r[s]][hl.c[s]] = uh[s][v];
+          else suy[v][hl.r[s]][hl.c[s]] = uh[s][v];
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NSP; s++) {
+      if (pp_on[s]) {
+        if (pp_x[s]) {
+          spx[pp_r[s]][pp_c[s]] = pv[s];
+          ssx[pp_r[s]][pp_c[s]] = sv[s];
+        } else {
+          spy[pp_r[s]][pp_c[s]] = pv[s];
+          ssy[pp_r[s]][pp_c[s]] = sv[s];
+        }
+      }
+    }
+    __syncthreads();
+    // pressure sensors of the plane
+    for (int t = tid; t < FY * (FX + 2); t += FT) {
+      const int r = t / (FX + 2), c = t % (FX + 2);
+      const int gi = i0 - 1 + c, gj = j0 + r;
+      if (gj <= g.ny && gi <= g.nx + 1)
+        psx[r][c] = psens3(spx[r][c], spx[r][c + 1], spx[r][c + 2], ssx[r][c + 1], cs);
+    }
+    for (int t = tid; t < (FY + 2) * FX; t += FT) {
+      const int r = t / FX, c = t % FX;
+      const int gi = i0 + c, gj = j0 - 1 + r;
+      if (gi <= g.nx && gj <= g.ny + 1)
+        psy[r][c] = psens3(spy[r][c], spy[r + 1][c], spy[r + 2][c], ssy[r + 1][c], cs);
+    }
+    __syncthreads();
+
+    if (own) {
+      double K[5];
+      // i faces i-1/2 and i+1/2
+      {
+        const double pm = psx[ty][tx], p0 = psx[ty][tx + 1], pp = psx[ty][tx + 2];
+        const double xk0m = a.xnu0 * fmax(p0, pm), xk10m = fmax(0.0, a.xnu10 + xk0m / 105.0);
+        const double xk0p = a.xnu0 * fmax(pp, p0), xk10p = fmax(0.0, a.xnu10 + xk0p / 105.0);
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+          const double *x = &sux[v][ty][tx + 5];
+          K[v] = dface(x - 1, 1, xk0m, xk10m) - dface(x, 1, xk0p, xk10p);
+        }
+      }
+      // j faces
+      {
+        const double pm = psy[ty][tx], p0 = psy[ty + 1][tx], pp = psy[ty + 2][tx];
+        const double xk0m = a.xnu0 * fmax(p0, pm), xk10m = fmax(0.0, a.xnu10 + xk0m / 105.0);
+        const double xk0p = a.xnu0 * fmax(pp, p0), xk10p = fmax(0.0, a.xnu10 + xk0p / 105.0);
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+          const double *y = &suy[v][ty + 5][tx];
+          K[v] = K[v] + dface(y - FX, FX, xk0m, xk10m) - dface(y, FX, xk0p, xk10p);
+        }
+      }
+      // k faces from the register queues
+      {
+        const double pm = psens3(qp[0], qp[1], qp[2], qs[0], cs);
+        const double p0 = psens3(qp[1], qp[2], qp[3], qs[1], cs);
+        const double pp = psens3(qp[2], qp[3], qp[4], qs[2], cs);
+        const double xk0m = a.xnu0 * fmax(p0, pm), xk10m = fmax(0.0, a.xnu10 + xk0m / 105.0);
+        const double xk0p = a.xnu0 * fmax(pp, p0), xk10p = fmax(0.0, a.xnu10 + xk0p / 105.0);
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+          const double dlo = (q[5][v] - q[4][v]) * xk0m +
+                             (D10_1 * (q[5][v] - q[4][v]) + D10_2 * (q[6][v] - q[3][v]) +
+                              D10_3 * (q[7][v] - q[2][v]) + D10_4 * (q[8][v] - q[1][v]) +
+                              D10_5 * (q[9][v] - q[0][v])) * xk10m;
+          const double dhi = (q[6][v] - q[5][v]) * xk0p +
+                             (D10_1 * (q[6][v] - q[5][v]) + D10_2 * (q[7][v] - q[4][v]) +
+                              D10_3 * (q[8][v] - q[3][v]) + D10_4 * (q[9][v] - q[2][v]) +
+                              D10_5 * (q[10][v] - q[1][v])) * xk10p;
+          K[v] = K[v] + dlo - dhi;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 5; v++) stg_nt(a.uout.v[v], pk, q[5][v] + K[v]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < 10; l++)
+#pragma unroll
+      for (int v = 0; v < 5; v++) q[l][v] = q[l + 1][v];
+#pragma unroll
+    for (int l = 0; l < 4; l++) qp[l] = qp[l + 1];
+    qs[0] = qs[1];
+    qs[1] = qs[2];
+  }
+}
+
+Tiling tiling_for(const Geom &g) {
+  Tiling t;
+  t.tiles_x = (g.nx + FX - 1) / FX;
+  t.tiles_y = (g.ny + FY - 1) / FY;
+  const long nxy = (long)t.tiles_x * t.tiles_y;
+  long want = (1024 + nxy - 1) / nxy;
+  int kc = (int)((g.nz + want - 1) / want);
+  if (kc < 16) kc = 16;
+  if (kc > g.nz) kc = g.nz;
+  t.kchunk = kc;
+  t.ntiles = t.tiles_x * t.tiles_y * ((g.nz + kc - 1) / kc);
+  return t;
 }
 
 }  // namespace
 
 hipError_t launch_filter_sf(const FilterArgs &a, hipStream_t s) {
-  dim3 blk(FX, FY, 1);
-  dim3 grd((a.g.nx + FX - 1) / FX, (a.g.ny + FY - 1) / FY, a.g.nz);
-  hipLaunchKernelGGL(k_filter_sf, grd, blk, 0, s, a);
+  const Tiling t = tiling_for(a.g);
+  hipLaunchKernelGGL(k_filter_sf, dim3(t.ntiles), dim3(FT), 0, s, a, t);
   return hipGetLastError();
 }
 
 hipError_t launch_filter_shock(const FilterArgs &a, hipStream_t s) {
-  dim3 blk(FX, FY, 1);
-  dim3 grd((a.g.nx + FX - 1) / FX, (a.g.ny + FY - 1) / FY, a.g.nz);
-  hipLaunchKernelGGL(k_filter_shock, grd, blk, 0, s, a);
+  const Tiling t = tiling_for(a.g);
+  hipLaunchKernelGGL(k_filter_shock, dim3(t.ntiles), dim3(FT), 0, s, a, t);
   return hipGetLastError();
 }
 

@@ -19,7 +19,7 @@ __global__ __launch_bounds__(HB) void k_halo_periodic(HaloArgs a, int dir, int n
   const Geom &g = a.g;
   long t = (long)blockIdx.x * HB + threadIdx.x;
   int i, j, k, n;
-  long s;
+  idx_t s;
   if (dir == 0) {
     const long cnt = (long)G * g.ny * g.nz;
     if (t >= cnt) return;
@@ -50,14 +50,14 @@ __global__ __launch_bounds__(HB) void k_halo_periodic(HaloArgs a, int dir, int n
   // (i,j,k) is interior layer l of the low face along dir.  Ghost n+l takes
   // interior l; ghost 1-l takes interior n+1-l.
   const int l = dir == 0 ? i : (dir == 1 ? j : k);
-  const long p_lo_src = lin(g, i, j, k);                     // interior index l
-  const long p_hi_dst = p_lo_src + (long)n * s;              // ghost index n+l
-  const long p_src2 = p_lo_src + (long)(n + 1 - 2 * l) * s;  // interior index n+1-l
-  const long p_dst2 = p_lo_src + (long)(1 - 2 * l) * s;      // ghost index 1-l
+  const idx_t p_lo_src = lin(g, i, j, k);                      // interior index l
+  const idx_t p_hi_dst = p_lo_src + (idx_t)n * s;              // ghost index n+l
+  const idx_t p_src2 = p_lo_src + (idx_t)(n + 1 - 2 * l) * s;  // interior index n+1-l
+  const idx_t p_dst2 = p_lo_src - (idx_t)(2 * l - 1) * s;      // ghost index 1-l
   for (int v = 0; v < nvar; v++) {
     double *u = a.u.v[v];
-    u[p_hi_dst] = u[p_lo_src];
-    u[p_dst2] = u[p_src2];
+    stg(u, p_hi_dst, ldg(u, p_lo_src));
+    stg(u, p_dst2, ldg(u, p_src2));
   }
 }
 

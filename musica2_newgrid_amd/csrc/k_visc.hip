@@ -12,147 +12,439 @@
 // is the same arithmetic on the same values.
 // It also emits p(U^n) and the Ducros sensor that the shock filter reads
 // (mod_filtering_shock.f90:223-239).
+//
+// 2.5-D layout as in the stage kernel: per k plane the primitives (u,v,w,T)
+// of the tile + 2-cell halos sit in LDS for the i/j derivatives; the
+// thread's own column keeps a 5-deep register queue for the k derivative.
+// All global loads of a plane are issued in one batch.
 #include "mgpu_internal.h"
 
 namespace mgpu {
 
 namespace {
 
-constexpr int VX = 64, VY = 4;
+struct P5 {
+  double u, v, w, T, p;
+};
 
-__device__ __forceinline__ void prim4(const CState &U, long p, double gam1, double rg, double &u,
-                                      double &v, double &w, double &T) {
-  const double r = U.v[0][p];
+__device__ __forceinline__ P5 prim5r(const double U[5], double gam1, double rg) {
+  P5 q;
+  const double r = U[0];
   const double vi = 1.0 / r;
-  u = U.v[1][p] * vi;
-  v = U.v[2][p] * vi;
-  w = U.v[3][p] * vi;
-  const double pr = gam1 * (U.v[4][p] - 0.5 * r * (u * u + v * v + w * w));
-  T = pr * vi / rg;
+  q.u = U[1] * vi;
+  q.v = U[2] * vi;
+  q.w = U[3] * vi;
+  q.p = gam1 * (U[4] - 0.5 * r * (q.u * q.u + q.v * q.v + q.w * q.w));
+  q.T = q.p * vi / rg;
+  return q;
 }
 
-template <bool CURV>
-__global__ __launch_bounds__(VX *VY) void k_visc(ViscArgs a) {
+__device__ __forceinline__ void ld5(const CState &U, idx_t p, double o[5]) {
+#pragma unroll
+  for (int v = 0; v < 5; v++) o[v] = ldg(U.v[v], p);
+}
+
+template <int TX, int TY, bool CURV>
+__global__ __launch_bounds__(TX *TY, 2) void k_visc(ViscArgs a, int tiles_x, int tiles_y, int ntiles, int kchunk) {
+  constexpr int NT = TX * TY;
+  constexpr int NHX = 4 * TY;             // i-halo cells per plane
+  constexpr int NH = NHX + 4 * TX;        // all halo cells per plane
+  constexpr int NS = (NH + NT - 1) / NT;  // halo cells per thread
+  __shared__ double spx[4][TY][TX + 4];
+  __shared__ double spy[4][TY + 4][TX];
+
   const Geom &g = a.g;
-  const int i = blockIdx.x * VX + threadIdx.x - 1;
-  const int j = blockIdx.y * VY + threadIdx.y - 1;
-  const int k = blockIdx.z - 1;
-  if (i > g.nx + 2 || j > g.ny + 2) return;
+  const int logical = xcd_remap(blockIdx.x, ntiles);
+  const int tile_x = logical % tiles_x;
+  const int rest = logical / tiles_x;
+  const int tile_y = rest % tiles_y;
+  const int chunk = rest / tiles_y;
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX, tid = threadIdx.x;
+  const int i0 = tile_x * TX - 1, j0 = tile_y * TY - 1;  // box starts at -1
+  const int i = i0 + tx, j = j0 + ty;
+  const int kbeg = chunk * kchunk - 1;
+  const int kend = min(kbeg + kchunk - 1, g.nz + 2);
+  const bool own = (i <= g.nx + 2) && (j <= g.ny + 2);
+  const bool needx = (i <= g.nx + 4) && (j <= g.ny + 2);  // point feeds the i-pencil
+  const bool needy = (i <= g.nx + 2) && (j <= g.ny + 4);  // point feeds the j-pencil
   const double gam1 = a.gas.gam1, rg = a.gas.rg;
-  const long p = lin(g, i, j, k);
-  const long st[3] = {1, (long)g.sx, g.sy};
+  const idx_t sz = g.sy;
 
-  // gradient of (u,v,w,T) along each computational direction
-  double gd[3][4];
+  // halo cells of this thread (LDS slot, offset in plane k=0)
+  int hs_dir[NS], hs_r[NS], hs_c[NS];
+  idx_t hs_off[NS];
+  bool hs_on[NS];
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
-    double pm2[4], pm1[4], pp1[4], pp2[4];
-    prim4(a.u0, p - 2 * st[d], gam1, rg, pm2[0], pm2[1], pm2[2], pm2[3]);
-    prim4(a.u0, p - st[d], gam1, rg, pm1[0], pm1[1], pm1[2], pm1[3]);
-    prim4(a.u0, p + st[d], gam1, rg, pp1[0], pp1[1], pp1[2], pp1[3]);
-    prim4(a.u0, p + 2 * st[d], gam1, rg, pp2[0], pp2[1], pp2[2], pp2[3]);
-#pragma unroll
-    for (int q = 0; q < 4; q++) gd[d][q] = A5_1 * (pp1[q] - pm1[q]) + A5_2 * (pp2[q] - pm2[q]);
-  }
-  // physical gradients g[q][x/y/z]
-  double gr[4][3];
-  if constexpr (CURV) {
-    const double *const *mv = a.met.mv;
-    const double ij = mv[9][p];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const double dvx = gd[0][q] * mv[0][p] + gd[1][q] * mv[3][p] + gd[2][q] * mv[6][p];
-      const double dvy = gd[0][q] * mv[1][p] + gd[1][q] * mv[4][p] + gd[2][q] * mv[7][p];
-      const double dvz = gd[0][q] * mv[2][p] + gd[1][q] * mv[5][p] + gd[2][q] * mv[8][p];
-      gr[q][0] = dvx * ij;
-      gr[q][1] = dvy * ij;
-      gr[q][2] = dvz * ij;
-    }
-  } else {
-    const double ix = a.met.idx_v[i - 1 + G], iy = a.met.idy_v[j - 1 + G], iz = a.met.idz_v[k - 1 + G];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      gr[q][0] = gd[0][q] * ix;
-      gr[q][1] = gd[1][q] * iy;
-      gr[q][2] = gd[2][q] * iz;
+  for (int s = 0; s < NS; s++) {
+    const int h = tid + s * NT;
+    hs_on[s] = false;
+    hs_dir[s] = hs_r[s] = hs_c[s] = 0;
+    hs_off[s] = 0;
+    if (h < NHX) {
+      const int r = h / 4, c = h % 4;
+      const int li = c < 2 ? c : TX + c;
+      const int gi = i0 - 2 + li, gj = j0 + r;
+      hs_on[s] = gj <= g.ny + 2 && gi <= g.nx + 4;
+      hs_r[s] = r;
+      hs_c[s] = li;
+      hs_off[s] = lin(g, gi, gj, 0);
+    } else if (h < NH) {
+      const int hh = h - NHX;
+      const int r = hh / TX, c = hh % TX;
+      const int lj = r < 2 ? r : TY + r;
+      const int gi = i0 + c, gj = j0 - 2 + lj;
+      hs_on[s] = gi <= g.nx + 2 && gj <= g.ny + 4;
+      hs_dir[s] = 1;
+      hs_r[s] = lj;
+      hs_c[s] = c;
+      hs_off[s] = lin(g, gi, gj, 0);
     }
   }
+  const idx_t own_off = lin(g, i, j, 0);
 
-  // primitives, viscosity and conductivity at the point (primitives_visc_pfg)
-  const double r = a.u0.v[0][p];
-  const double vi = 1.0 / r;
-  const double u = a.u0.v[1][p] * vi, v = a.u0.v[2][p] * vi, w = a.u0.v[3][p] * vi;
-  const double pr = gam1 * (a.u0.v[4][p] - 0.5 * r * (u * u + v * v + w * w));
-  const double T = pr * vi / rg;
-  const double mu_l = a.gas.c0 * sqrt(T * T * T) / (T + a.gas.S0) * a.gas.diffscale;
-  const double ck = a.gas.cok_ratio * mu_l;
-
-  // stress tensor (flux_visc_5pts: S_ij, then -tau_ij)
-  double t11 = gr[0][0], t22 = gr[1][1], t33 = gr[2][2];
-  double t12 = 0.5 * (gr[0][1] + gr[1][0]);
-  double t13 = 0.5 * (gr[0][2] + gr[2][0]);
-  double t23 = 0.5 * (gr[1][2] + gr[2][1]);
-  const double tr = (1.0 / 3.0) * (t11 + t22 + t33);
-  const double mu = -2.0 * mu_l;
-  t11 = mu * (t11 - tr);
-  t22 = mu * (t22 - tr);
-  t33 = mu * (t33 - tr);
-  t12 = mu * t12;
-  t13 = mu * t13;
-  t23 = mu * t23;
-
-  if constexpr (CURV) {
+  P5 q[5];  // own column, planes k-2..k+2
+  if (own) {
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-      const double mx = a.met.mv[3 * d][p], my = a.met.mv[3 * d + 1][p], mz = a.met.mv[3 * d + 2][p];
-      const double fu = t11 * mx + t12 * my + t13 * mz;
-      const double fv = t12 * mx + t22 * my + t23 * mz;
-      const double fw = t13 * mx + t23 * my + t33 * mz;
-      a.fv[4 * d][p] = fu;
-      a.fv[4 * d + 1][p] = fv;
-      a.fv[4 * d + 2][p] = fw;
-      a.fv[4 * d + 3][p] = (u * fu + v * fv + w * fw) - ck * (gr[3][0] * mx + gr[3][1] * my + gr[3][2] * mz);
+    for (int l = 0; l < 4; l++) {
+      double U[5];
+      ld5(a.u0, own_off + (idx_t)(kbeg - 2 + l) * sz, U);
+      q[l] = prim5r(U, gam1, rg);
     }
-  } else {
-    a.fv[0][p] = t11;
-    a.fv[1][p] = t12;
-    a.fv[2][p] = t13;
-    a.fv[3][p] = (u * t11 + v * t12 + w * t13) - ck * gr[3][0];
-    a.fv[4][p] = t12;
-    a.fv[5][p] = t22;
-    a.fv[6][p] = t23;
-    a.fv[7][p] = (u * t12 + v * t22 + w * t23) - ck * gr[3][1];
-    a.fv[8][p] = t13;
-    a.fv[9][p] = t23;
-    a.fv[10][p] = t33;
-    a.fv[11][p] = (u * t13 + v * t23 + w * t33) - ck * gr[3][2];
   }
 
-  if (a.want_sensor) {
-    a.prs[p] = pr;
-    if (i >= 0 && i <= g.nx + 1 && j >= 0 && j <= g.ny + 1 && k >= 0 && k <= g.nz + 1) {
-      double s = 1.0;
-      if (a.ducros) {
-        const double dv = gr[0][0] + gr[1][1] + gr[2][2];
-        const double div2 = dv * dv;
-        const double o1 = gr[2][1] - gr[1][2];
-        const double o2 = gr[0][2] - gr[2][0];
-        const double o3 = gr[1][0] - gr[0][1];
-        s = div2 / (div2 + o1 * o1 + o2 * o2 + o3 * o3 + 1.0e-16);
+  for (int k = kbeg; k <= kend; k++) {
+    const idx_t pk = own_off + (idx_t)k * sz;
+    // ---- every global load of the plane ----
+    double uq[5], uc[5], uh[NS][5], mvv[10];
+    if (own) ld5(a.u0, pk + 2 * sz, uq);
+    if (!own && (needx || needy)) ld5(a.u0, pk, uc);
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+      if (hs_on[s]) ld5(a.u0, hs_off[s] + (idx_t)k * sz, uh[s]);
+    if (CURV && own) {
+#pragma unroll
+      for (int c = 0; c < 10; c++) mvv[c] = ldg(a.met.mv[c], pk);
+    }
+
+    // ---- primitives into the queue and the LDS pencils ----
+    if (own) q[4] = prim5r(uq, gam1, rg);
+    if (needx || needy) {
+      // own column: from the queue; ghost cells just past a partial tile: loaded
+      const P5 c = own ? q[2] : prim5r(uc, gam1, rg);
+      if (needx) {
+        spx[0][ty][tx + 2] = c.u;
+        spx[1][ty][tx + 2] = c.v;
+        spx[2][ty][tx + 2] = c.w;
+        spx[3][ty][tx + 2] = c.T;
       }
-      a.sens[p] = s;
+      if (needy) {
+        spy[0][ty + 2][tx] = c.u;
+        spy[1][ty + 2][tx] = c.v;
+        spy[2][ty + 2][tx] = c.w;
+        spy[3][ty + 2][tx] = c.T;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      if (hs_on[s]) {
+        const P5 h = prim5r(uh[s], gam1, rg);
+        if (hs_dir[s] == 0) {
+          spx[0][hs_r[s]][hs_c[s]] = h.u;
+          spx[1][hs_r[s]][hs_c[s]] = h.v;
+          spx[2][hs_r[s]][hs_c[s]] = h.w;
+          spx[3][hs_r[s]][hs_c[s]] = h.T;
+        } else {
+          spy[0][hs_r[s]][hs_c[s]] = h.u;
+          spy[1][hs_r[s]][hs_c[s]] = h.v;
+          spy[2][hs_r[s]][hs_c[s]] = h.w;
+          spy[3][hs_r[s]][hs_c[s]] = h.T;
+        }
+      }
+    }
+    __syncthreads();
+
+    if (own) {
+      const idx_t p = pk;
+      // gradients along the computational directions: gd[dir][u,v,w,T]
+      double gd[3][4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const double *fx = &spx[c][ty][tx + 2];
+        gd[0][c] = A5_1 * (fx[1] - fx[-1]) + A5_2 * (fx[2] - fx[-2]);
+        const double *fy = &spy[c][ty + 2][tx];
+        gd[1][c] = A5_1 * (fy[TX] - fy[-TX]) + A5_2 * (fy[2 * TX] - fy[-2 * TX]);
+      }
+      gd[2][0] = A5_1 * (q[3].u - q[1].u) + A5_2 * (q[4].u - q[0].u);
+      gd[2][1] = A5_1 * (q[3].v - q[1].v) + A5_2 * (q[4].v - q[0].v);
+      gd[2][2] = A5_1 * (q[3].w - q[1].w) + A5_2 * (q[4].w - q[0].w);
+      gd[2][3] = A5_1 * (q[3].T - q[1].T) + A5_2 * (q[4].T - q[0].T);
+
+      // physical gradients gr[u,v,w,T][x,y,z]
+      double gr[4][3];
+      if constexpr (CURV) {
+        const double ij = mvv[9];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const double dvx = gd[0][c] * mvv[0] + gd[1][c] * mvv[3] + gd[2][c] * mvv[6];
+          const double dvy = gd[0][c] * mvv[1] + gd[1][c] * mvv[4] + gd[2][c] * mvv[7];
+          const double dvz = gd[0][c] * mvv[2] + gd[1][c] * mvv[5] + gd[2][c] * mvv[8];
+          gr[c][0] = dvx * ij;
+          gr[c][1] = dvy * ij;
+          gr[c][2] = dvz * ij;
+        }
+      } else {
+        const double ix = a.met.idx_v[i - 1 + G], iy = a.met.idy_v[j - 1 + G], iz = a.met.idz_v[k - 1 + G];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          gr[c][0] = gd[0][c] * ix;
+          gr[c][1] = gd[1][c] * iy;
+          gr[c][2] = gd[2][c] * iz;
+        }
+      }
+
+      const double u = q[2].u, v = q[2].v, w = q[2].w, T = q[2].T;
+      const double mu_l = a.gas.c0 * sqrt(T * T * T) / (T + a.gas.S0) * a.gas.diffscale;
+      const double ck = a.gas.cok_ratio * mu_l;
+
+      double t11 = gr[0][0], t22 = gr[1][1], t33 = gr[2][2];
+      double t12 = 0.5 * (gr[0][1] + gr[1][0]);
+      double t13 = 0.5 * (gr[0][2] + gr[2][0]);
+      double t23 = 0.5 * (gr[1][2] + gr[2][1]);
+      const double tr = (1.0 / 3.0) * (t11 + t22 + t33);
+      const double mu = -2.0 * mu_l;
+      t11 = mu * (t11 - tr);
+      t22 = mu * (t22 - tr);
+      t33 = mu * (t33 - tr);
+      t12 = mu * t12;
+      t13 = mu * t13;
+      t23 = mu * t23;
+
+      if constexpr (CURV) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          const double mx = mvv[3 * d], my = mvv[3 * d + 1], mz = mvv[3 * d + 2];
+          const double fu = t11 * mx + t12 * my + t13 * mz;
+          const double fv = t12 * mx + t22 * my + t23 * mz;
+          const double fw = t13 * mx + t23 * my + t33 * mz;
+          stg(a.fv[4 * d], p, fu);
+          stg(a.fv[4 * d + 1], p, fv);
+          stg(a.fv[4 * d + 2], p, fw);
+          stg(a.fv[4 * d + 3], p,
+              (u * fu + v * fv + w * fw) - ck * (gr[3][0] * mx + gr[3][1] * my + gr[3][2] * mz));
+        }
+      } else {
+        stg(a.fv[0], p, t11);
+        stg(a.fv[1], p, t12);
+        stg(a.fv[2], p, t13);
+        stg(a.fv[3], p, (u * t11 + v * t12 + w * t13) - ck * gr[3][0]);
+        stg(a.fv[4], p, t12);
+        stg(a.fv[5], p, t22);
+        stg(a.fv[6], p, t23);
+        stg(a.fv[7], p, (u * t12 + v * t22 + w * t23) - ck * gr[3][1]);
+        stg(a.fv[8], p, t13);
+        stg(a.fv[9], p, t23);
+        stg(a.fv[10], p, t33);
+        stg(a.fv[11], p, (u * t13 + v * t23 + w * t33) - ck * gr[3][2]);
+      }
+
+      if (a.want_sensor) {
+        stg(a.prs, p, q[2].p);
+        if (i >= 0 && i <= g.nx + 1 && j >= 0 && j <= g.ny + 1 && k >= 0 && k <= g.nz + 1) {
+          double s = 1.0;
+          if (a.ducros) {
+            const double dv = gr[0][0] + gr[1][1] + gr[2][2];
+            const double div2 = dv * dv;
+            const double o1 = gr[2][1] - gr[1][2];
+            const double o2 = gr[0][2] - gr[2][0];
+            const double o3 = gr[1][0] - gr[0][1];
+            s = div2 / (div2 + o1 * o1 + o2 * o2 + o3 * o3 + 1.0e-16);
+          }
+          stg(a.sens, p, s);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < 4; l++) q[l] = q[l + 1];
+  }
+}
+
+template <int TX, int TY>
+hipError_t launch_t(const ViscArgs &a, bool curv, hipStream_t s) {
+  const int ex = a.g.nx + 4, ey = a.g.ny + 4, ez = a.g.nz + 4;
+  const int tiles_x = (ex + TX - 1) / TX, tiles_y = (ey + TY - 1) / TY;
+  const long nxy = (long)tiles_x * tiles_y;
+  // about 4 workgroups per CU-slot in flight (256 CUs x 2), chunks of >= 16 planes
+  long want = (2048 + nxy - 1) / nxy;
+  int kchunk = (int)((ez + want - 1) / want);
+  if (kchunk < 16) kchunk = 16;
+  if (kchunk > ez) kchunk = ez;
+  const int nchunk = (ez + kchunk - 1) / kchunk;
+  const int nt = tiles_x * tiles_y * nchunk;
+  if (curv)
+    hipLaunchKernelGGL((k_visc<TX, TY, true>), dim3(nt), dim3(TX * TY), 0, s, a, tiles_x, tiles_y, nt, kchunk);
+  else
+    hipLaunchKernelGGL((k_visc<TX, TY, false>), dim3(nt), dim3(TX * TY), 0, s, a, tiles_x, tiles_y, nt, kchunk);
+  return hipGetLastError();
+}
+
+// Viscous divergence (flux_visc_5pts(_c3) divergence part,
+// smod_flux_visc_5pts.f90:119-, smod_flux_visc_5pts_c3.f90:221-): 5-pt
+// differences of F^v along i and G^v along j from LDS pencils (tile + 2-cell
+// halos), of H^v along k from a 5-deep register queue; times idx_v,idy_v,
+// idz_v (Cartesian) or ijacob3_v (c3).  Interior points only.
+template <int TX, int TY, bool CURV>
+__global__ __launch_bounds__(TX *TY, 2) void k_vdiv(VdivArgs a, int tiles_x, int tiles_y, int ntiles, int kchunk) {
+  constexpr int NT = TX * TY;
+  constexpr int NHX = 4 * TY;
+  constexpr int NH = NHX + 4 * TX;
+  constexpr int NS = (NH + NT - 1) / NT;
+  __shared__ double sfx[4][TY][TX + 4];
+  __shared__ double sfy[4][TY + 4][TX];
+
+  const Geom &g = a.g;
+  const int logical = xcd_remap(blockIdx.x, ntiles);
+  const int tile_x = logical % tiles_x;
+  const int rest = logical / tiles_x;
+  const int tile_y = rest % tiles_y;
+  const int chunk = rest / tiles_y;
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX, tid = threadIdx.x;
+  const int i0 = tile_x * TX + 1, j0 = tile_y * TY + 1;
+  const int i = i0 + tx, j = j0 + ty;
+  const int kbeg = chunk * kchunk + 1;
+  const int kend = min(kbeg + kchunk - 1, g.nz);
+  const bool own = i <= g.nx && j <= g.ny;
+  const bool needx = i <= g.nx + 2 && j <= g.ny, needy = i <= g.nx && j <= g.ny + 2;
+  const idx_t sz = g.sy, own_off = lin(g, i, j, 0);
+
+  int hs_r[NS], hs_c[NS];
+  idx_t hs_off[NS];
+  bool hs_on[NS], hs_x[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int h = tid + s * NT;
+    hs_on[s] = false;
+    hs_x[s] = true;
+    hs_r[s] = hs_c[s] = 0;
+    hs_off[s] = 0;
+    if (h < NHX) {
+      const int r = h / 4, c = h % 4;
+      const int li = c < 2 ? c : TX + c;
+      const int gi = i0 - 2 + li, gj = j0 + r;
+      hs_on[s] = gj <= g.ny && gi <= g.nx + 2;
+      hs_r[s] = r;
+      hs_c[s] = li;
+      hs_off[s] = lin(g, gi, gj, 0);
+    } else if (h < NH) {
+      const int hh = h - NHX;
+      const int r = hh / TX, c = hh % TX;
+      const int lj = r < 2 ? r : TY + r;
+      const int gi = i0 + c, gj = j0 - 2 + lj;
+      hs_on[s] = gi <= g.nx && gj <= g.ny + 2;
+      hs_x[s] = false;
+      hs_r[s] = lj;
+      hs_c[s] = c;
+      hs_off[s] = lin(g, gi, gj, 0);
     }
   }
+
+  double q[5][4];  // H^v at k-2..k+2
+  if (own) {
+#pragma unroll
+    for (int l = 0; l < 4; l++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) q[l][c] = ldg(a.fv[8 + c], own_off + (idx_t)(kbeg - 2 + l) * sz);
+  }
+  for (int k = kbeg; k <= kend; k++) {
+    const idx_t pk = own_off + (idx_t)k * sz;
+    double fx[4], fy[4], fh[NS][4];
+    double ijv = 0.0;
+    if (own) {
+#pragma unroll
+      for (int c = 0; c < 4; c++) q[4][c] = ldg(a.fv[8 + c], pk + 2 * sz);
+      if constexpr (CURV) ijv = ldg(a.met.mv[9], pk);
+    }
+    if (needx) {
+#pragma unroll
+      for (int c = 0; c < 4; c++) fx[c] = ldg(a.fv[c], pk);
+    }
+    if (needy) {
+#pragma unroll
+      for (int c = 0; c < 4; c++) fy[c] = ldg(a.fv[4 + c], pk);
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      if (hs_on[s]) {
+        const idx_t ph = hs_off[s] + (idx_t)k * sz;
+#pragma unroll
+        for (int c = 0; c < 4; c++) fh[s][c] = ldg(a.fv[(hs_x[s] ? 0 : 4) + c], ph);
+      }
+    }
+    if (needx) {
+#pragma unroll
+      for (int c = 0; c < 4; c++) sfx[c][ty][tx + 2] = fx[c];
+    }
+    if (needy) {
+#pragma unroll
+      for (int c = 0; c < 4; c++) sfy[c][ty + 2][tx] = fy[c];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      if (hs_on[s]) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          if (hs_x[s]) sfx[c][hs_r[s]][hs_c[s]] = fh[s][c];
+          else sfy[c][hs_r[s]][hs_c[s]] = fh[s][c];
+        }
+      }
+    }
+    __syncthreads();
+    if (own) {
+      double mx, my, mz;
+      if constexpr (CURV) {
+        mx = my = mz = ijv;
+      } else {
+        mx = a.met.idx_v[i - 1 + G];
+        my = a.met.idy_v[j - 1 + G];
+        mz = a.met.idz_v[k - 1 + G];
+      }
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const double *x = &sfx[c][ty][tx + 2];
+        const double *y = &sfy[c][ty + 2][tx];
+        double kv = (A5_1 * (x[1] - x[-1]) + A5_2 * (x[2] - x[-2])) * mx;
+        kv = (A5_1 * (y[TX] - y[-TX]) + A5_2 * (y[2 * TX] - y[-2 * TX])) * my + kv;
+        kv = (A5_1 * (q[3][c] - q[1][c]) + A5_2 * (q[4][c] - q[0][c])) * mz + kv;
+        stg(a.kv[c], pk, kv);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < 4; l++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) q[l][c] = q[l + 1][c];
+  }
+}
+
+template <int TX, int TY>
+hipError_t launch_vdiv_t(const VdivArgs &a, bool curv, hipStream_t s) {
+  const int tiles_x = (a.g.nx + TX - 1) / TX, tiles_y = (a.g.ny + TY - 1) / TY;
+  const long nxy = (long)tiles_x * tiles_y;
+  long want = (2048 + nxy - 1) / nxy;
+  int kchunk = (int)((a.g.nz + want - 1) / want);
+  if (kchunk < 16) kchunk = 16;
+  if (kchunk > a.g.nz) kchunk = a.g.nz;
+  const int nt = tiles_x * tiles_y * ((a.g.nz + kchunk - 1) / kchunk);
+  if (curv)
+    hipLaunchKernelGGL((k_vdiv<TX, TY, true>), dim3(nt), dim3(TX * TY), 0, s, a, tiles_x, tiles_y, nt, kchunk);
+  else
+    hipLaunchKernelGGL((k_vdiv<TX, TY, false>), dim3(nt), dim3(TX * TY), 0, s, a, tiles_x, tiles_y, nt, kchunk);
+  return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_visc(const ViscArgs &a, bool curv, hipStream_t s) {
-  dim3 blk(VX, VY, 1);
-  dim3 grd((a.g.nx + 4 + VX - 1) / VX, (a.g.ny + 4 + VY - 1) / VY, a.g.nz + 4);
-  if (curv) hipLaunchKernelGGL((k_visc<true>), grd, blk, 0, s, a);
-  else hipLaunchKernelGGL((k_visc<false>), grd, blk, 0, s, a);
-  return hipGetLastError();
-}
+hipError_t launch_visc(const ViscArgs &a, bool curv, hipStream_t s) { return launch_t<32, 16>(a, curv, s); }
+hipError_t launch_vdiv(const VdivArgs &a, bool curv, hipStream_t s) { return launch_vdiv_t<32, 16>(a, curv, s); }
 
 }  // namespace mgpu

@@ -4,7 +4,11 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -23,9 +27,14 @@ struct BlockData {
   double *m[10] = {};
   double *mv[10] = {};
   double *fv[12] = {};    // viscous fluxes (last stage)
+  double *kv[4] = {};     // viscous increment of rhou..rhoe (last stage)
   double *prs = nullptr, *sens = nullptr;
   bool have_metrics = false;
   bool have_state = false;
+  // stage kernel launch parameters: [0] Euler stages, [1] last stage with the
+  // viscous divergence (chosen by mgpu_autotune, else a default)
+  int shape[2] = {2, 2};
+  int kchunk[2] = {0, 0};
 };
 
 struct ProfRec {
@@ -153,15 +162,29 @@ int halo(mgpu_handle *h, BlockData &bd, double *const *set) {
   return 0;
 }
 
-int kchunk_for(const Geom &g) {
-  const long nxy = (long)((g.nx + 63) / 64) * ((g.ny + 3) / 4);
-  long want = (4096 + nxy - 1) / nxy;  // aim at >= 4096 workgroups
+// Default k-chunk: aim at >= 2048 workgroups, at least 32 planes per chunk
+// (the 10-plane queue warm-up is paid once per chunk).
+int kchunk_for(const Geom &g, int shape) {
+  const int TX = kStageShapes[shape][0], TY = kStageShapes[shape][1];
+  const long nxy = (long)((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY);
+  long want = (2048 + nxy - 1) / nxy;
   if (want < 1) want = 1;
   long kc = (g.nz + want - 1) / want;
-  if (kc < 8) kc = 8;
+  if (kc < 32) kc = 32;
   if (kc > g.nz) kc = g.nz;
   return (int)kc;
 }
+
+// Process-wide cache of tuned stage parameters, keyed by block geometry and
+// kernel variant, so that equal blocks (and later handles) are tuned once.
+struct TuneKey {
+  int dev, nx, ny, nz, curv, visc;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(dev, nx, ny, nz, curv, visc) < std::tie(o.dev, o.nx, o.ny, o.nz, o.curv, o.visc);
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::pair<int, int>> g_tune_cache;
 
 void free_block(BlockData &bd) {
   for (auto &s : bd.st)
@@ -170,6 +193,7 @@ void free_block(BlockData &bd) {
   for (auto &p : bd.m) { if (p) (void)hipFree(p); p = nullptr; }
   for (auto &p : bd.mv) { if (p) (void)hipFree(p); p = nullptr; }
   for (auto &p : bd.fv) { if (p) (void)hipFree(p); p = nullptr; }
+  for (auto &p : bd.kv) { if (p) (void)hipFree(p); p = nullptr; }
   if (bd.prs) (void)hipFree(bd.prs);
   if (bd.sens) (void)hipFree(bd.sens);
   bd.prs = bd.sens = nullptr;
@@ -184,6 +208,28 @@ int dalloc(mgpu_handle *h, double **p, size_t count) {
   e = hipMemset(*p, 0, count * sizeof(double));
   if (e != hipSuccess) return fail(h, MGPU_EDEVICE, "hipMemset failed: %s", hipGetErrorString(e));
   return 0;
+}
+
+StageArgs stage_args(mgpu_handle *h, BlockData &bd, int in, int s0, int out, int irk) {
+  StageArgs sa{};
+  sa.g = bd.g;
+  sa.gas = gas_of(h->c);
+  for (int v = 0; v < 5; v++) {
+    sa.uin.v[v] = bd.st[in][v];
+    sa.u0.v[v] = bd.st[s0][v];
+    sa.uout.v[v] = bd.st[out][v];
+  }
+  sa.met = metrics_of(bd);
+  for (int q = 0; q < 4; q++) sa.kv[q] = bd.kv[q];
+  sa.alpha = h->c.deltat * h->c.crk[irk - 1];
+  return sa;
+}
+
+hipError_t launch_block_stage(mgpu_handle *h, BlockData &bd, int in, int s0, int out, int irk, bool visc_last) {
+  StageArgs sa = stage_args(h, bd, in, s0, out, irk);
+  const int v = visc_last ? 1 : 0;
+  sa.kchunk = bd.kchunk[v] > 0 ? bd.kchunk[v] : kchunk_for(bd.g, bd.shape[v]);
+  return launch_stage(sa, h->c.curvilinear != 0, visc_last, h->stream, bd.shape[v]);
 }
 
 }  // namespace
@@ -208,6 +254,10 @@ int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
     const mgpu_block &bl = c->blocks[bI];
     if (bl.nx < G || bl.ny < G || bl.nz < G)
       return fail(nullptr, MGPU_EINVAL, "block %d: every extent must be >= %d", bI, G);
+    const size_t padded = (size_t)(bl.nx + 2 * G) * (bl.ny + 2 * G) * (bl.nz + 2 * G);
+    if (padded > kMaxPadded)
+      return fail(nullptr, MGPU_EINVAL, "block %d: %zu padded points exceed the 2^28 limit of one block", bI,
+                  padded);
     for (int f = 0; f < 6; f++)
       if (bl.face_kind[f] != MGPU_FACE_PERIODIC)
         return fail(nullptr, MGPU_EUNSUP, "block %d face %d: only periodic faces supported yet", bI, f);
@@ -234,8 +284,8 @@ int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
     bd.g.ny = bl.ny;
     bd.g.nz = bl.nz;
     bd.g.sx = bl.nx + 2 * G;
-    bd.g.sy = (long)bd.g.sx * (bl.ny + 2 * G);
-    bd.g.n = bd.g.sy * (bl.nz + 2 * G);
+    bd.g.sy = (idx_t)bd.g.sx * (idx_t)(bl.ny + 2 * G);
+    bd.g.n = (long)bd.g.sy * (bl.nz + 2 * G);
     const size_t n = (size_t)bd.g.n;
     for (int s = 0; s < 3 && !rc; s++)
       for (int v = 0; v < 5 && !rc; v++) rc = dalloc(h, &bd.st[s][v], n);
@@ -249,6 +299,7 @@ int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
     }
     if (c->iorder_visc) {
       for (int q = 0; q < 12 && !rc; q++) rc = dalloc(h, &bd.fv[q], n);
+      for (int q = 0; q < 4 && !rc; q++) rc = dalloc(h, &bd.kv[q], n);
       if (c->filter == MGPU_FILTER_SHOCK) {
         if (!rc) rc = dalloc(h, &bd.prs, n);
         if (!rc) rc = dalloc(h, &bd.sens, n);
@@ -370,22 +421,16 @@ int mgpu_step(mgpu_handle *h, int nsteps) {
           va.want_sensor = c.filter == MGPU_FILTER_SHOCK;
           va.ducros = c.is_ducros;
           HIPCK(h, launch_visc(va, curv, h->stream));
+          VdivArgs vd{};
+          vd.g = bd.g;
+          vd.met = met;
+          for (int q = 0; q < 12; q++) vd.fv[q] = bd.fv[q];
+          for (int q = 0; q < 4; q++) vd.kv[q] = bd.kv[q];
+          HIPCK(h, launch_vdiv(vd, curv, h->stream));
         }
         {
           ProfScope ps(h, MGPU_K_STAGE);
-          StageArgs sa_{};
-          sa_.g = bd.g;
-          sa_.gas = gs;
-          for (int v = 0; v < 5; v++) {
-            sa_.uin.v[v] = bd.st[in][v];
-            sa_.u0.v[v] = bd.st[s0][v];
-            sa_.uout.v[v] = bd.st[out][v];
-          }
-          sa_.met = met;
-          for (int q = 0; q < 12; q++) sa_.fv[q] = bd.fv[q];
-          sa_.alpha = c.deltat * c.crk[irk - 1];
-          sa_.kchunk = kchunk_for(bd.g);
-          HIPCK(h, launch_stage(sa_, curv, last && visc, h->stream));
+          HIPCK(h, launch_block_stage(h, bd, in, s0, out, irk, last && visc));
         }
         int rc = halo(h, bd, bd.st[out]);
         if (rc) return rc;
@@ -422,6 +467,82 @@ int mgpu_step(mgpu_handle *h, int nsteps) {
       h->cur = (c.nrk % 2 == 1) ? sa : sb;
     }
   }
+  return 0;
+}
+
+int mgpu_autotune(mgpu_handle *h, int flags) {
+  if (!h) return MGPU_EINVAL;
+  for (size_t bI = 0; bI < h->b.size(); bI++) {
+    if (!h->b[bI].have_metrics) return fail(h, MGPU_ESTATE, "block %zu: metrics not set", bI);
+    if (!h->b[bI].have_state) return fail(h, MGPU_ESTATE, "block %zu: state not set", bI);
+  }
+  const bool force = (flags & 1) != 0;
+  const int s0 = h->cur, sa = (h->cur + 1) % 3;
+  hipEvent_t e0 = get_event(h), e1 = get_event(h);
+  if (!e0 || !e1) return fail(h, MGPU_EDEVICE, "hipEventCreate failed");
+  int rc = 0;
+  for (auto &bd : h->b) {
+    for (int v = 0; v < (h->c.iorder_visc ? 2 : 1) && !rc; v++) {
+      const TuneKey key{h->device, bd.g.nx, bd.g.ny, bd.g.nz, h->c.curvilinear, v};
+      if (!force) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+          bd.shape[v] = it->second.first;
+          bd.kchunk[v] = it->second.second;
+          continue;
+        }
+      }
+      // candidates: every tile shape x k-chunks from 16 planes to the whole block
+      float best = 1e30f;
+      int best_shape = bd.shape[v], best_kc = bd.kchunk[v];
+      const int kcs[] = {16, 32, 64, 128, 256};
+      for (int sh = 0; sh < kNumStageShapes && !rc; sh++) {
+        int last_kc = -1;
+        for (int kc0 : kcs) {
+          const int kc = kc0 < bd.g.nz ? kc0 : bd.g.nz;
+          if (kc == last_kc) break;
+          last_kc = kc;
+          bd.shape[v] = sh;
+          bd.kchunk[v] = kc;
+          // one warm-up launch, then three timed ones (U^n in, scratch set out)
+          hipError_t e = launch_block_stage(h, bd, s0, s0, sa, 2, v == 1);
+          if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
+          for (int r = 0; r < 3 && e == hipSuccess; r++) e = launch_block_stage(h, bd, s0, s0, sa, 2, v == 1);
+          if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+          if (e == hipSuccess) e = hipEventSynchronize(e1);
+          float ms = 0.f;
+          if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+          if (e != hipSuccess) {
+            rc = fail(h, MGPU_EDEVICE, "autotune launch failed: %s", hipGetErrorString(e));
+            break;
+          }
+          if (ms < best) {
+            best = ms;
+            best_shape = sh;
+            best_kc = kc;
+          }
+        }
+      }
+      bd.shape[v] = best_shape;
+      bd.kchunk[v] = best_kc;
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tune_cache[key] = {best_shape, best_kc};
+    }
+  }
+  h->ev_pool.push_back(e0);
+  h->ev_pool.push_back(e1);
+  return rc;
+}
+
+int mgpu_stage_config(mgpu_handle *h, int block, int variant, int *shape_tx, int *shape_ty, int *kchunk) {
+  if (!h) return MGPU_EINVAL;
+  if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
+  if (variant < 0 || variant > 1) return fail(h, MGPU_EINVAL, "variant %d", variant);
+  BlockData &bd = h->b[block];
+  if (shape_tx) *shape_tx = kStageShapes[bd.shape[variant]][0];
+  if (shape_ty) *shape_ty = kStageShapes[bd.shape[variant]][1];
+  if (kchunk) *kchunk = bd.kchunk[variant] > 0 ? bd.kchunk[variant] : kchunk_for(bd.g, bd.shape[variant]);
   return 0;
 }
 

@@ -22,16 +22,45 @@ constexpr double D11_0 = 252.0 / 1024.0, D11_1 = -210.0 / 1024.0, D11_2 = 120.0 
 // 9th-order conservative dissipation (mod_filtering_shock.f90:95-100).
 constexpr double D10_1 = -126.0, D10_2 = 84.0, D10_3 = -36.0, D10_4 = 9.0, D10_5 = -1.0;
 
+// Element offset inside one padded array.  32-bit: a block's padded array
+// holds at most 2^28 points (checked at mgpu_create), so byte offsets fit in
+// 32 bits and loads use the SGPR-base + 32-bit VGPR-offset form.
+using idx_t = uint32_t;
+constexpr size_t kMaxPadded = size_t(1) << 28;
+
 // Geometry of one block's padded arrays (i fastest).
 struct Geom {
   int nx, ny, nz;
-  int sx;   // nx + 2G
-  long sy;  // sx*(ny+2G)
-  long n;   // total padded points
+  int sx;    // nx + 2G
+  idx_t sy;  // sx*(ny+2G)
+  long n;    // total padded points
 };
 
-__host__ __device__ inline long lin(const Geom &g, int i, int j, int k) {
-  return (long)(i + G - 1) + (long)g.sx * (long)(j + G - 1) + g.sy * (long)(k + G - 1);
+__host__ __device__ inline idx_t lin(const Geom &g, int i, int j, int k) {
+  return (idx_t)(i + G - 1) + (idx_t)g.sx * (idx_t)(j + G - 1) + g.sy * (idx_t)(k + G - 1);
+}
+
+// Load/store at element offset i through a 32-bit byte offset.
+__device__ __forceinline__ double ldg(const double *b, idx_t i) {
+  return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(b) + (uint32_t)(i << 3));
+}
+__device__ __forceinline__ void stg(double *b, idx_t i, double v) {
+  *reinterpret_cast<double *>(reinterpret_cast<char *>(b) + (uint32_t)(i << 3)) = v;
+}
+// Streaming store for results no kernel of the same launch reads back.
+__device__ __forceinline__ void stg_nt(double *b, idx_t i, double v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<double *>(reinterpret_cast<char *>(b) + (uint32_t)(i << 3)));
+}
+
+// XCD-aware workgroup order.  gfx950 dispatches workgroup w to XCD w % 8;
+// each XCD has its own L2.  Renumber so that XCD x owns one contiguous range
+// of logical tiles: neighbouring tiles (which re-read each other's halos)
+// then share an L2.  Bijective for any count n.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_remap(int w, int n) {
+  const int x = w % NXCD, l = w / NXCD;
+  const int q = n / NXCD, r = n % NXCD;
+  return x * q + (x < r ? x : r) + l;
 }
 
 // Thermodynamics / transport constants of a case.
@@ -65,7 +94,7 @@ struct StageArgs {
   CState u0;     // step-start U^n
   State uout;    // stage output
   Metrics met;
-  const double *fv[12];  // viscous fluxes F^v,G^v,H^v (4 components each), last stage
+  const double *kv[4];   // viscous increment of rhou,rhov,rhow,rhoe (last stage)
   double alpha;          // deltat*crk(irk)
   int kchunk;            // planes per block along k
 };
@@ -80,6 +109,14 @@ struct ViscArgs {
   double *sens;       // out: Ducros sensor on [0, n+1]^3 (shock filter)
   int want_sensor;    // 1: write prs and sens
   int ducros;
+};
+
+// Divergence of the viscous fluxes: K_v = sum_d D5_d(F^v_d) * metric.
+struct VdivArgs {
+  Geom g;
+  Metrics met;
+  const double *fv[12];  // F^v (x), G^v (y), H^v (z), 4 components each
+  double *kv[4];         // out, interior points
 };
 
 struct FilterArgs {
@@ -103,8 +140,11 @@ struct HaloArgs {
 
 // Kernel launchers (defined in the .hip files).
 namespace mgpu {
-hipError_t launch_stage(const StageArgs &a, bool curv, bool visc, hipStream_t s);
+hipError_t launch_stage(const StageArgs &a, bool curv, bool visc, hipStream_t s, int shape);
+extern const int kStageShapes[][2];
+extern const int kNumStageShapes;
 hipError_t launch_visc(const ViscArgs &a, bool curv, hipStream_t s);
+hipError_t launch_vdiv(const VdivArgs &a, bool curv, hipStream_t s);
 hipError_t launch_filter_sf(const FilterArgs &a, hipStream_t s);
 hipError_t launch_filter_shock(const FilterArgs &a, hipStream_t s);
 hipError_t launch_halo_periodic(const HaloArgs &a, int nvar_from_0, hipStream_t s);

@@ -90,6 +90,15 @@ class GpuSolver:
     def step(self, nsteps: int = 1) -> None:
         check(lib().mgpu_step(self.h, int(nsteps)), self.h, "mgpu_step")
 
+    def autotune(self, force: bool = False) -> None:
+        check(lib().mgpu_autotune(self.h, int(force)), self.h, "mgpu_autotune")
+
+    def stage_config(self, variant: int = 0):
+        tx, ty, kc = C.c_int(), C.c_int(), C.c_int()
+        check(lib().mgpu_stage_config(self.h, 0, variant, C.byref(tx), C.byref(ty), C.byref(kc)), self.h,
+              "mgpu_stage_config")
+        return tx.value, ty.value, kc.value
+
     def sync(self) -> None:
         check(lib().mgpu_sync(self.h), self.h, "mgpu_sync")
 
