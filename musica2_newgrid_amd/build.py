@@ -30,6 +30,35 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+BIN = os.path.join(HERE, "bin")
+
+
+def flang() -> str:
+    for cand in (os.environ.get("FC"), "/opt/rocm/bin/amdflang", shutil.which("amdflang"), shutil.which("flang")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("no Fortran compiler (amdflang) found")
+
+
+def build_fortran(verbose: bool = False) -> None:
+    """The Fortran host: binding module, TGV driver and param.ini checker."""
+    os.makedirs(BIN, exist_ok=True)
+    mod_dir = os.path.join(HERE, "_build", "fmod")
+    os.makedirs(mod_dir, exist_ok=True)
+    fc = flang()
+    mod = os.path.join(FORTRAN, "mod_musica_gpu.f90")
+    progs = {"musica_gpu_tgv": True, "param_check": True}
+    for name, needs_lib in progs.items():
+        cmd = [fc, "-O2", "-module-dir", mod_dir, mod, os.path.join(FORTRAN, name + ".f90"),
+               "-o", os.path.join(BIN, name)]
+        if needs_lib:
+            cmd += ["-L" + HERE, "-lmusica_gpu", "-Wl,-rpath,$ORIGIN/.."]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+
+
 def build(verbose: bool = False, extra=()) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = OUT + ".tmp"
@@ -38,6 +67,7 @@ def build(verbose: bool = False, extra=()) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, OUT)
+    build_fortran(verbose)
     return OUT
 
 

@@ -298,7 +298,6 @@ static void gradients(const orc_cfg *c, const orc_grid *g, const dims_t *d, work
           w->grad[3 * q + 2][p] = gz;
         }
       }
-  for (int q = 0; q < 12; q++) orc_fill_periodic(c, w->grad[q]);
 }
 
 static void visc_rhs_w(const orc_cfg *c, const orc_grid *g, const dims_t *d, work_t *w, double *K[5]) {
@@ -525,6 +524,7 @@ void orc_visc_rhs(const orc_cfg *c, const orc_grid *g, double *const Uref[5], do
   if (work_alloc(&d, &w, 1)) { work_free(&w); return; }
   prim_visc(c, &d, Uref, &w);
   gradients(c, g, &d, &w);
+  for (int q = 0; q < 12; q++) orc_fill_periodic(c, w.grad[q]);
   visc_rhs_w(c, g, &d, &w, K);
   work_free(&w);
 }
@@ -540,6 +540,7 @@ void orc_filter_rhs(const orc_cfg *c, const orc_grid *g, double *const Un[5], do
     if (work_alloc(&d, &w, 1)) { work_free(&w); return; }
     prim_visc(c, &d, Uref, &w);
     gradients(c, g, &d, &w);
+    for (int q = 0; q < 12; q++) orc_fill_periodic(c, w.grad[q]);
     filter_shock(c, &d, Un, &w, K);
     work_free(&w);
   }
@@ -564,7 +565,8 @@ int orc_step(const orc_cfg *c, const orc_grid *g, double *U[5], int nsteps) {
       euler_rhs_w(c, g, &d, Un, &w, K);               /* flux_euler       */
       if (irk == c->nrk && c->visc) {
         prim_visc(c, &d, U, &w);                      /* primitives_visc(U^n) */
-        gradients(c, g, &d, &w);                      /* grad_vel, grad_T, communication_v */
+        gradients(c, g, &d, &w);                      /* grad_vel, grad_T */
+        for (int q = 0; q < 12; q++) orc_fill_periodic(c, w.grad[q]); /* communication_v */
         visc_rhs_w(c, g, &d, &w, K);                  /* flux_visc        */
       }
       const double alpha = c->dt * c->crk[irk - 1];   /* update_var, update.f90:32-44 */
@@ -595,5 +597,148 @@ int orc_step(const orc_cfg *c, const orc_grid *g, double *U[5], int nsteps) {
 out:
   for (int v = 0; v < 5; v++) { free(Un[v]); free(K[v]); }
   work_free(&w);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multiblock: interface copies with orientation, and the step over blocks.  */
+/* ------------------------------------------------------------------------ */
+static void face_axes(int f, int *a, int *t1, int *t2) {
+  *a = f / 2;
+  *t1 = *a == 0 ? 1 : 0;
+  *t2 = *a == 2 ? 1 : 2;
+}
+
+void orc_fill_interfaces(int nblocks, const orc_block *bl, double *const *arrs) {
+  for (int b = 0; b < nblocks; b++) {
+    const orc_block *A = &bl[b];
+    const int na[3] = {A->nx, A->ny, A->nz};
+    const long sA[3] = {1, A->nx + 2 * NG, (long)(A->nx + 2 * NG) * (A->ny + 2 * NG)};
+    for (int f = 0; f < 6; f++) {
+      const orc_block *B = &bl[A->nbr_block[f]];
+      const int fb = A->nbr_face[f], o = A->orient[f];
+      const int nb[3] = {B->nx, B->ny, B->nz};
+      const long sB[3] = {1, B->nx + 2 * NG, (long)(B->nx + 2 * NG) * (B->ny + 2 * NG)};
+      int a, t1, t2, bb, u1, u2;
+      face_axes(f, &a, &t1, &t2);
+      face_axes(fb, &bb, &u1, &u2);
+      const int side = f % 2, sideb = fb % 2;
+      double *dst = arrs[b];
+      const double *src = arrs[A->nbr_block[f]];
+#pragma omp parallel for collapse(2)
+      for (int qq = 1; qq <= na[t2]; qq++)
+        for (int pp = 1; pp <= na[t1]; pp++) {
+          int x1 = (o & 1) ? qq : pp, x2 = (o & 1) ? pp : qq;
+          if (o & 2) x1 = nb[u1] + 1 - x1;
+          if (o & 4) x2 = nb[u2] + 1 - x2;
+          for (int l = 1; l <= NG; l++) {
+            const int ia = side == 0 ? 1 - l : na[a] + l;
+            const int ib = sideb == 0 ? l : nb[bb] + 1 - l;
+            const long pa = (long)(ia + NG - 1) * sA[a] + (long)(pp + NG - 1) * sA[t1] + (long)(qq + NG - 1) * sA[t2];
+            const long pb = (long)(ib + NG - 1) * sB[bb] + (long)(x1 + NG - 1) * sB[u1] + (long)(x2 + NG - 1) * sB[u2];
+            dst[pa] = src[pb];
+          }
+        }
+    }
+  }
+}
+
+int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_grid *g, double *const *U, int nsteps) {
+  if (nb < 1 || nb > 64) return -1;
+  if (c0->filter == 2 && !c0->visc) return -3;
+  orc_cfg cb[64];
+  dims_t d[64];
+  work_t w[64];
+  double *Un[64][5], *K[64][5];
+  int rc = 0;
+  for (int b = 0; b < nb; b++) {
+    cb[b] = *c0;
+    cb[b].nx = bl[b].nx;
+    cb[b].ny = bl[b].ny;
+    cb[b].nz = bl[b].nz;
+    if (bl[b].nx < NG || bl[b].ny < NG || bl[b].nz < NG) return -1;
+    d[b] = mkdims(&cb[b]);
+    for (int v = 0; v < 5; v++) {
+      Un[b][v] = alloc_arr(&d[b]);
+      K[b][v] = alloc_arr(&d[b]);
+      if (!Un[b][v] || !K[b][v]) rc = -10;
+    }
+    if (work_alloc(&d[b], &w[b], c0->visc)) rc = -10;
+  }
+  double *tab[64];
+  if (rc) goto out;
+  for (int step = 0; step < nsteps; step++) {
+    for (int b = 0; b < nb; b++)
+      for (int v = 0; v < 5; v++) memcpy(Un[b][v], U[5 * b + v], d[b].n * sizeof(double));
+    for (int irk = 1; irk <= c0->nrk; irk++) {
+      const int last = irk == c0->nrk;
+      if (last && c0->visc) {
+        for (int b = 0; b < nb; b++) {
+          prim_visc(&cb[b], &d[b], &U[5 * b], &w[b]);
+          gradients(&cb[b], &g[b], &d[b], &w[b]);
+        }
+        for (int q = 0; q < 12; q++) {
+          for (int b = 0; b < nb; b++) tab[b] = w[b].grad[q];
+          orc_fill_interfaces(nb, bl, tab);
+        }
+      }
+      for (int b = 0; b < nb; b++) {
+        if (last && c0->visc) {
+          /* primitives of U_n for the Euler part, primitives_visc of U^n kept for the sensor */
+          work_t we = w[b];
+          double *save[5] = {w[b].uu, w[b].vv, w[b].ww, w[b].prs, w[b].tmp};
+          double *tmp5[5];
+          for (int q = 0; q < 5; q++) tmp5[q] = alloc_arr(&d[b]);
+          we.uu = tmp5[0]; we.vv = tmp5[1]; we.ww = tmp5[2]; we.prs = tmp5[3]; we.tmp = tmp5[4];
+          prim_euler(&cb[b], &d[b], Un[b], &we);
+          euler_rhs_w(&cb[b], &g[b], &d[b], Un[b], &we, K[b]);
+          for (int q = 0; q < 5; q++) free(tmp5[q]);
+          (void)save;
+          visc_rhs_w(&cb[b], &g[b], &d[b], &w[b], K[b]);
+        } else {
+          prim_euler(&cb[b], &d[b], Un[b], &w[b]);
+          euler_rhs_w(&cb[b], &g[b], &d[b], Un[b], &w[b], K[b]);
+        }
+        const double alpha = c0->dt * c0->crk[irk - 1];
+        const dims_t dd = d[b];
+#pragma omp parallel for collapse(2)
+        for (int k = 1; k <= dd.nz; k++)
+          for (int j = 1; j <= dd.ny; j++)
+            for (int i = 1; i <= dd.nx; i++) {
+              long p = (long)ID(dd, i, j, k);
+              for (int v = 0; v < 5; v++) Un[b][v][p] = U[5 * b + v][p] - alpha * K[b][v][p];
+            }
+      }
+      for (int v = 0; v < 5; v++) {
+        for (int b = 0; b < nb; b++) tab[b] = Un[b][v];
+        orc_fill_interfaces(nb, bl, tab);
+      }
+    }
+    if (c0->filter) {
+      for (int b = 0; b < nb; b++) {
+        if (c0->filter == 1) filter_sf(&cb[b], &d[b], Un[b], K[b]);
+        else filter_shock(&cb[b], &d[b], Un[b], &w[b], K[b]);
+        const dims_t dd = d[b];
+#pragma omp parallel for collapse(2)
+        for (int k = 1; k <= dd.nz; k++)
+          for (int j = 1; j <= dd.ny; j++)
+            for (int i = 1; i <= dd.nx; i++) {
+              long p = (long)ID(dd, i, j, k);
+              for (int v = 0; v < 5; v++) Un[b][v][p] = Un[b][v][p] + K[b][v][p];
+            }
+      }
+      for (int v = 0; v < 5; v++) {
+        for (int b = 0; b < nb; b++) tab[b] = Un[b][v];
+        orc_fill_interfaces(nb, bl, tab);
+      }
+    }
+    for (int b = 0; b < nb; b++)
+      for (int v = 0; v < 5; v++) memcpy(U[5 * b + v], Un[b][v], d[b].n * sizeof(double));
+  }
+out:
+  for (int b = 0; b < nb; b++) {
+    for (int v = 0; v < 5; v++) { free(Un[b][v]); free(K[b][v]); }
+    work_free(&w[b]);
+  }
   return rc;
 }

@@ -61,6 +61,34 @@ typedef struct orc_grid {
   const double *mv[10];    /* the viscous set (*_v arrays)                 */
 } orc_grid;
 
+/* One block of a multiblock case.  Face f (imin,imax,jmin,jmax,kmin,kmax =
+ * 0..5) is a 1-to-1 interface to face nbr_face[f] of block nbr_block[f]
+ * (a periodic face is an interface of the block with itself).  orient[f]:
+ * bit 0 swaps the two tangential axes, bits 1 and 2 reverse the first and
+ * second tangential axis of the neighbour face (tangential axes of a face
+ * are the two other axes in increasing order).  Restates the face exchange
+ * with swapped/reversed neighbour axes of mpi_types_comm and
+ * determine_mpi_face (src/Parallel/mod_mpi_types_two_sided.f90:47-197,
+ * 331-512; src/Grid/mod_grid_directions.f90:16-40). */
+typedef struct orc_block {
+  int nx, ny, nz;
+  int nbr_block[6];
+  int nbr_face[6];
+  int orient[6];
+} orc_block;
+
+/* Fill the face ghosts of every block from its neighbours' interior
+ * (ngh layers, tangential extent of the face, edges and corners untouched).
+ * arrs[b] is the padded array of block b. */
+void orc_fill_interfaces(int nblocks, const orc_block *blocks, double *const *arrs);
+
+/* Multiblock step: blocks advance together, interfaces exchanged after every
+ * stage and after the filter (runge_kutta.f90:48,109,126).  c gives the
+ * flags; its nx,ny,nz and face[][] are ignored.  U[5*b+v] is variable v of
+ * block b, grids[b] its metrics. */
+int orc_step_multi(const orc_cfg *c, int nblocks, const orc_block *blocks, const orc_grid *grids,
+                   double *const *U, int nsteps);
+
 /* Fill the face ghost layers of a padded array from the periodic image.
  * Restates communication3d for self-neighbour faces
  * (src/Parallel/mod_comm.f90:249-290). */

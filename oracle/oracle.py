@@ -34,6 +34,13 @@ class OrcCfg(C.Structure):
     ]
 
 
+class OrcBlock(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("nbr_block", C.c_int * 6), ("nbr_face", C.c_int * 6), ("orient", C.c_int * 6),
+    ]
+
+
 class OrcGrid(C.Structure):
     _fields_ = [
         ("idx", _dp), ("idy", _dp), ("idz", _dp),
@@ -66,6 +73,11 @@ def lib():
         L.orc_filter_rhs.restype = None
         L.orc_fill_periodic.argtypes = [C.POINTER(OrcCfg), _dp]
         L.orc_num_threads.restype = C.c_int
+        L.orc_step_multi.argtypes = [C.POINTER(OrcCfg), C.c_int, C.POINTER(OrcBlock), C.POINTER(OrcGrid),
+                                     C.POINTER(_dp), C.c_int]
+        L.orc_step_multi.restype = C.c_int
+        L.orc_fill_interfaces.argtypes = [C.c_int, C.POINTER(OrcBlock), C.POINTER(_dp)]
+        L.orc_fill_interfaces.restype = None
         _lib = L
     return _lib
 
@@ -142,6 +154,36 @@ class Oracle:
         lib().orc_filter_rhs(C.byref(self.cfg), C.byref(self.grid), self._tab(Un), self._tab(Uref),
                              self._tab(K))
         return K
+
+
+def make_blocks(blocks):
+    """blocks: list of dicts with nx, ny, nz, nbr_block[6], nbr_face[6], orient[6]."""
+    arr = (OrcBlock * len(blocks))()
+    for q, b in enumerate(blocks):
+        arr[q].nx, arr[q].ny, arr[q].nz = b["nx"], b["ny"], b["nz"]
+        for f in range(6):
+            arr[q].nbr_block[f] = b["nbr_block"][f]
+            arr[q].nbr_face[f] = b["nbr_face"][f]
+            arr[q].orient[f] = b["orient"][f]
+    return arr
+
+
+def step_multi(case, blocks, block_cases, Us, nsteps: int = 1):
+    """Advance the multiblock state Us (list per block of 5 padded arrays) in
+    place.  case gives the flags; block_cases[b] carries block b's metrics."""
+    orcs = [Oracle(bc) for bc in block_cases]
+    cfg = Oracle(case).cfg
+    grids = (OrcGrid * len(blocks))(*[o.grid for o in orcs])
+    tab = (_dp * (5 * len(blocks)))(*[_p(a) for U in Us for a in U])
+    rc = lib().orc_step_multi(C.byref(cfg), len(blocks), make_blocks(blocks), grids, tab, int(nsteps))
+    if rc != 0:
+        raise RuntimeError(f"orc_step_multi failed rc={rc}")
+    return Us
+
+
+def fill_interfaces(blocks, arrs):
+    tab = (_dp * len(arrs))(*[_p(a) for a in arrs])
+    lib().orc_fill_interfaces(len(blocks), make_blocks(blocks), tab)
 
 
 def num_threads() -> int:
