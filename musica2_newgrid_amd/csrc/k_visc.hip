@@ -1,22 +1,28 @@
-// Viscous fluxes of U^n for the last RK stage, on the box [-1, n+2]^3 that
-// the 5-pt divergence reads.  One pass replaces, for the reference:
+// Viscous terms of U^n for the last RK stage, in two kernels.
+//
+// k_visc: the physical viscous flux tensor on the interior points.  One pass
+// replaces, for the reference:
 //   primitives_visc_pfg  src/Fluid/smod_primitive_pfg.f90:62-136
 //   grad_vel_5pts / grad_T_5pts   smod_grad_vel.f90:12-, smod_grad_T.f90:12-
 //   grad_*_5pts_c3 (+ mapping to x,y,z)  smod_grad_vel_c3.f90:11-,260-300;
 //                                        smod_grad_T_c3.f90:200-213
-//   communication_3dv (gradient halo)    src/Parallel/mod_comm.f90:376-402
-//   flux_visc_5pts(_c3) flux part        smod_flux_visc_5pts.f90:27-67;
-//                                        smod_flux_visc_5pts_c3.f90:27-75
-// Gradients in the two ghost layers are evaluated from the state ghosts
-// instead of being exchanged: for periodic faces and 1-to-1 interfaces this
-// is the same arithmetic on the same values.
+//   flux_visc_5pts(_c3) stress and heat flux  smod_flux_visc_5pts.f90:27-67;
+//                                             smod_flux_visc_5pts_c3.f90:27-75
+// It stores the tensor in the physical frame (tau, see mgpu_internal.h), so
+// that the 2-layer interface exchange of tau that follows stands for the
+// reference's exchange of the gradients (communication_3dv,
+// src/Parallel/mod_comm.f90:376-402) and the flux evaluation at the ghosts.
 // It also emits p(U^n) and the Ducros sensor that the shock filter reads
-// (mod_filtering_shock.f90:223-239).
+// (mod_filtering_shock.f90:223-239), exchanged with tau.
 //
-// 2.5-D layout as in the stage kernel: per k plane the primitives (u,v,w,T)
-// of the tile + 2-cell halos sit in LDS for the i/j derivatives; the
-// thread's own column keeps a 5-deep register queue for the k derivative.
-// All global loads of a plane are issued in one batch.
+// k_vdiv: the 5-pt divergence (flux_visc_5pts(_c3) divergence part,
+// smod_flux_visc_5pts.f90:119-, smod_flux_visc_5pts_c3.f90:221-), with the
+// projection of tau on the viscous metrics of each stencil point.
+//
+// 2.5-D layout as in the stage kernel: per k plane the tile + 2-cell halos
+// sit in LDS for the i/j derivatives; the thread's own column keeps a 5-deep
+// register queue for the k derivative.  All global loads of a plane are
+// issued in one batch.
 #include "mgpu_internal.h"
 
 namespace mgpu {
@@ -60,13 +66,13 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(ViscArgs a, int tiles_x, int
   const int tile_y = rest % tiles_y;
   const int chunk = rest / tiles_y;
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX, tid = threadIdx.x;
-  const int i0 = tile_x * TX - 1, j0 = tile_y * TY - 1;  // box starts at -1
+  const int i0 = tile_x * TX + 1, j0 = tile_y * TY + 1;
   const int i = i0 + tx, j = j0 + ty;
-  const int kbeg = chunk * kchunk - 1;
-  const int kend = min(kbeg + kchunk - 1, g.nz + 2);
-  const bool own = (i <= g.nx + 2) && (j <= g.ny + 2);
-  const bool needx = (i <= g.nx + 4) && (j <= g.ny + 2);  // point feeds the i-pencil
-  const bool needy = (i <= g.nx + 2) && (j <= g.ny + 4);  // point feeds the j-pencil
+  const int kbeg = chunk * kchunk + 1;
+  const int kend = min(kbeg + kchunk - 1, g.nz);
+  const bool own = (i <= g.nx) && (j <= g.ny);
+  const bool needx = (i <= g.nx + 2) && (j <= g.ny);  // point feeds the i-pencil
+  const bool needy = (i <= g.nx) && (j <= g.ny + 2);  // point feeds the j-pencil
   const double gam1 = a.gas.gam1, rg = a.gas.rg;
   const idx_t sz = g.sy;
 
@@ -84,7 +90,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(ViscArgs a, int tiles_x, int
       const int r = h / 4, c = h % 4;
       const int li = c < 2 ? c : TX + c;
       const int gi = i0 - 2 + li, gj = j0 + r;
-      hs_on[s] = gj <= g.ny + 2 && gi <= g.nx + 4;
+      hs_on[s] = gj <= g.ny && gi <= g.nx + 2;
       hs_r[s] = r;
       hs_c[s] = li;
       hs_off[s] = lin(g, gi, gj, 0);
@@ -93,7 +99,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(ViscArgs a, int tiles_x, int
       const int r = hh / TX, c = hh % TX;
       const int lj = r < 2 ? r : TY + r;
       const int gi = i0 + c, gj = j0 - 2 + lj;
-      hs_on[s] = gi <= g.nx + 2 && gj <= g.ny + 4;
+      hs_on[s] = gi <= g.nx && gj <= g.ny + 2;
       hs_dir[s] = 1;
       hs_r[s] = lj;
       hs_c[s] = c;
@@ -219,48 +225,28 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(ViscArgs a, int tiles_x, int
       t13 = mu * t13;
       t23 = mu * t23;
 
-      if constexpr (CURV) {
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-          const double mx = mvv[3 * d], my = mvv[3 * d + 1], mz = mvv[3 * d + 2];
-          const double fu = t11 * mx + t12 * my + t13 * mz;
-          const double fv = t12 * mx + t22 * my + t23 * mz;
-          const double fw = t13 * mx + t23 * my + t33 * mz;
-          stg(a.fv[4 * d], p, fu);
-          stg(a.fv[4 * d + 1], p, fv);
-          stg(a.fv[4 * d + 2], p, fw);
-          stg(a.fv[4 * d + 3], p,
-              (u * fu + v * fv + w * fw) - ck * (gr[3][0] * mx + gr[3][1] * my + gr[3][2] * mz));
-        }
-      } else {
-        stg(a.fv[0], p, t11);
-        stg(a.fv[1], p, t12);
-        stg(a.fv[2], p, t13);
-        stg(a.fv[3], p, (u * t11 + v * t12 + w * t13) - ck * gr[3][0]);
-        stg(a.fv[4], p, t12);
-        stg(a.fv[5], p, t22);
-        stg(a.fv[6], p, t23);
-        stg(a.fv[7], p, (u * t12 + v * t22 + w * t23) - ck * gr[3][1]);
-        stg(a.fv[8], p, t13);
-        stg(a.fv[9], p, t23);
-        stg(a.fv[10], p, t33);
-        stg(a.fv[11], p, (u * t13 + v * t23 + w * t33) - ck * gr[3][2]);
-      }
+      stg(a.tau[0], p, t11);
+      stg(a.tau[1], p, t12);
+      stg(a.tau[2], p, t13);
+      stg(a.tau[3], p, t22);
+      stg(a.tau[4], p, t23);
+      stg(a.tau[5], p, t33);
+      stg(a.tau[6], p, (u * t11 + v * t12 + w * t13) - ck * gr[3][0]);
+      stg(a.tau[7], p, (u * t12 + v * t22 + w * t23) - ck * gr[3][1]);
+      stg(a.tau[8], p, (u * t13 + v * t23 + w * t33) - ck * gr[3][2]);
 
       if (a.want_sensor) {
         stg(a.prs, p, q[2].p);
-        if (i >= 0 && i <= g.nx + 1 && j >= 0 && j <= g.ny + 1 && k >= 0 && k <= g.nz + 1) {
-          double s = 1.0;
-          if (a.ducros) {
-            const double dv = gr[0][0] + gr[1][1] + gr[2][2];
-            const double div2 = dv * dv;
-            const double o1 = gr[2][1] - gr[1][2];
-            const double o2 = gr[0][2] - gr[2][0];
-            const double o3 = gr[1][0] - gr[0][1];
-            s = div2 / (div2 + o1 * o1 + o2 * o2 + o3 * o3 + 1.0e-16);
-          }
-          stg(a.sens, p, s);
+        double s = 1.0;
+        if (a.ducros) {
+          const double dv = gr[0][0] + gr[1][1] + gr[2][2];
+          const double div2 = dv * dv;
+          const double o1 = gr[2][1] - gr[1][2];
+          const double o2 = gr[0][2] - gr[2][0];
+          const double o3 = gr[1][0] - gr[0][1];
+          s = div2 / (div2 + o1 * o1 + o2 * o2 + o3 * o3 + 1.0e-16);
         }
+        stg(a.sens, p, s);
       }
     }
     __syncthreads();
@@ -271,7 +257,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(ViscArgs a, int tiles_x, int
 
 template <int TX, int TY>
 hipError_t launch_t(const ViscArgs &a, bool curv, hipStream_t s) {
-  const int ex = a.g.nx + 4, ey = a.g.ny + 4, ez = a.g.nz + 4;
+  const int ex = a.g.nx, ey = a.g.ny, ez = a.g.nz;
   const int tiles_x = (ex + TX - 1) / TX, tiles_y = (ey + TY - 1) / TY;
   const long nxy = (long)tiles_x * tiles_y;
   // about 4 workgroups per CU-slot in flight (256 CUs x 2), chunks of >= 16 planes
@@ -288,17 +274,38 @@ hipError_t launch_t(const ViscArgs &a, bool curv, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Viscous divergence (flux_visc_5pts(_c3) divergence part,
-// smod_flux_visc_5pts.f90:119-, smod_flux_visc_5pts_c3.f90:221-): 5-pt
-// differences of F^v along i and G^v along j from LDS pencils (tile + 2-cell
-// halos), of H^v along k from a 5-deep register queue; times idx_v,idy_v,
-// idz_v (Cartesian) or ijacob3_v (c3).  Interior points only.
+// Components of tau forming the flux along direction d in a Cartesian block
+// (row d of the stress tensor and the energy flux e_d).
+__constant__ const int kRow[3][4] = {{0, 1, 2, 6}, {1, 3, 4, 7}, {2, 4, 5, 8}};
+
+// Flux along computational direction d at one point: Cartesian, row d of
+// tau (t holds those four values); c3, tau projected on the viscous metric
+// row (mx,my,mz) of d (t holds all nine values).
+template <bool CURV>
+__device__ __forceinline__ void vflux(const double *t, double mx, double my, double mz, double f[4]) {
+  if constexpr (CURV) {
+    f[0] = t[0] * mx + t[1] * my + t[2] * mz;
+    f[1] = t[1] * mx + t[3] * my + t[4] * mz;
+    f[2] = t[2] * mx + t[4] * my + t[5] * mz;
+    f[3] = t[6] * mx + t[7] * my + t[8] * mz;
+  } else {
+    f[0] = t[0];
+    f[1] = t[1];
+    f[2] = t[2];
+    f[3] = t[3];
+  }
+}
+
+// 5-pt differences of F^v along i and G^v along j from LDS pencils (tile +
+// 2-cell halos), of H^v along k from a 5-deep register queue; times
+// idx_v,idy_v,idz_v (Cartesian) or ijacob3_v (c3).  Interior points only.
 template <int TX, int TY, bool CURV>
 __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(VdivArgs a, int tiles_x, int tiles_y, int ntiles, int kchunk) {
   constexpr int NT = TX * TY;
   constexpr int NHX = 4 * TY;
   constexpr int NH = NHX + 4 * TX;
   constexpr int NS = (NH + NT - 1) / NT;
+  constexpr int NL = CURV ? NTAU : 4;  // tau values loaded per point and direction
   __shared__ double sfx[4][TY][TX + 4];
   __shared__ double sfy[4][TY + 4][TX];
 
@@ -316,6 +323,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(VdivArgs a, int tiles_x, int
   const bool own = i <= g.nx && j <= g.ny;
   const bool needx = i <= g.nx + 2 && j <= g.ny, needy = i <= g.nx && j <= g.ny + 2;
   const idx_t sz = g.sy, own_off = lin(g, i, j, 0);
+  const double *const *mv = a.met.mv;
 
   int hs_r[NS], hs_c[NS];
   idx_t hs_off[NS];
@@ -348,38 +356,38 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(VdivArgs a, int tiles_x, int
     }
   }
 
+  // tau values of direction d at element p, projected into f[4]
+  auto flux_at = [&](int d, idx_t p, double f[4]) {
+    double t[NL];
+#pragma unroll
+    for (int c = 0; c < NL; c++) t[c] = ldg(a.tau[CURV ? c : kRow[d][c]], p);
+    double mx = 0.0, my = 0.0, mz = 0.0;
+    if constexpr (CURV) {
+      mx = ldg(mv[3 * d], p);
+      my = ldg(mv[3 * d + 1], p);
+      mz = ldg(mv[3 * d + 2], p);
+    }
+    vflux<CURV>(t, mx, my, mz, f);
+  };
+
   double q[5][4];  // H^v at k-2..k+2
   if (own) {
 #pragma unroll
-    for (int l = 0; l < 4; l++)
-#pragma unroll
-      for (int c = 0; c < 4; c++) q[l][c] = ldg(a.fv[8 + c], own_off + (idx_t)(kbeg - 2 + l) * sz);
+    for (int l = 0; l < 4; l++) flux_at(2, own_off + (idx_t)(kbeg - 2 + l) * sz, q[l]);
   }
   for (int k = kbeg; k <= kend; k++) {
     const idx_t pk = own_off + (idx_t)k * sz;
     double fx[4], fy[4], fh[NS][4];
     double ijv = 0.0;
     if (own) {
-#pragma unroll
-      for (int c = 0; c < 4; c++) q[4][c] = ldg(a.fv[8 + c], pk + 2 * sz);
-      if constexpr (CURV) ijv = ldg(a.met.mv[9], pk);
+      flux_at(2, pk + 2 * sz, q[4]);
+      if constexpr (CURV) ijv = ldg(mv[9], pk);
     }
-    if (needx) {
+    if (needx) flux_at(0, pk, fx);
+    if (needy) flux_at(1, pk, fy);
 #pragma unroll
-      for (int c = 0; c < 4; c++) fx[c] = ldg(a.fv[c], pk);
-    }
-    if (needy) {
-#pragma unroll
-      for (int c = 0; c < 4; c++) fy[c] = ldg(a.fv[4 + c], pk);
-    }
-#pragma unroll
-    for (int s = 0; s < NS; s++) {
-      if (hs_on[s]) {
-        const idx_t ph = hs_off[s] + (idx_t)k * sz;
-#pragma unroll
-        for (int c = 0; c < 4; c++) fh[s][c] = ldg(a.fv[(hs_x[s] ? 0 : 4) + c], ph);
-      }
-    }
+    for (int s = 0; s < NS; s++)
+      if (hs_on[s]) flux_at(hs_x[s] ? 0 : 1, hs_off[s] + (idx_t)k * sz, fh[s]);
     if (needx) {
 #pragma unroll
       for (int c = 0; c < 4; c++) sfx[c][ty][tx + 2] = fx[c];

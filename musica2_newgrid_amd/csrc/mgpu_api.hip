@@ -26,7 +26,7 @@ struct BlockData {
   double *idx[6] = {};    // Cartesian metrics idx,idy,idz,idx_v,idy_v,idz_v
   double *m[10] = {};
   double *mv[10] = {};
-  double *fv[12] = {};    // viscous fluxes (last stage)
+  double *tau[NTAU] = {}; // physical viscous flux tensor of U^n (last stage)
   double *kv[4] = {};     // viscous increment of rhou..rhoe (last stage)
   double *prs = nullptr, *sens = nullptr;
   bool have_metrics = false;
@@ -57,6 +57,13 @@ struct mgpu_handle {
   std::vector<hipEvent_t> ev_pool;
   int64_t kcount[MGPU_K_COUNT] = {};
   double kms[MGPU_K_COUNT] = {};
+  // interface exchange: task and chunk tables (5 state layers, 2 viscous
+  // layers) and per-array-set pointer tables, all in device memory
+  mgpu::IfaceArgs xs[3] = {};  // state set s
+  mgpu::IfaceArgs xv = {};     // tau (+ prs, sens)
+  int nchunks_s = 0, nchunks_v = 0;
+  std::vector<void *> dev_tables;
+  bool ghosts_dirty = false;
 };
 
 namespace {
@@ -153,12 +160,228 @@ Metrics metrics_of(const BlockData &bd) {
   return m;
 }
 
-int halo(mgpu_handle *h, BlockData &bd, double *const *set) {
+// Ghost layers of state set `set` of every block from the neighbours.
+int exchange_state(mgpu_handle *h, int set) {
   ProfScope ps(h, MGPU_K_HALO);
-  HaloArgs ha{};
-  ha.g = bd.g;
-  for (int v = 0; v < 5; v++) ha.u.v[v] = set[v];
-  HIPCK(h, launch_halo_periodic(ha, 5, h->stream));
+  HIPCK(h, launch_iface(h->xs[set], h->nchunks_s, h->stream));
+  return 0;
+}
+
+// Neighbour of face f of block b: (block, face, orientation code).
+void face_nbr(const mgpu_block &bl, int b, int f, int &nb, int &nf, int &o) {
+  if (bl.face_kind[f] == MGPU_FACE_PERIODIC) {
+    nb = b;
+    nf = f ^ 1;
+    o = 0;
+  } else {
+    nb = bl.nbr_block[f];
+    nf = bl.nbr_face[f];
+    o = bl.orient[f];
+  }
+}
+
+// Axes of face f: normal axis, then the two tangential axes in increasing order.
+void face_axes(int f, int &a, int &t1, int &t2) {
+  a = f / 2;
+  t1 = a == 0 ? 1 : 0;
+  t2 = a == 2 ? 1 : 2;
+}
+
+// Checks the block table: neighbours exist, faces match in extent under the
+// orientation, and every interface is declared from both sides with the
+// inverse orientation.  Returns an error message or an empty string.
+std::string check_topology(const mgpu_case &c) {
+  char buf[256];
+  for (int b = 0; b < c.nblocks; b++) {
+    const mgpu_block &A = c.blocks[b];
+    const int na[3] = {A.nx, A.ny, A.nz};
+    for (int f = 0; f < 6; f++) {
+      const int kind = A.face_kind[f];
+      if (kind != MGPU_FACE_PERIODIC && kind != MGPU_FACE_INTERFACE) {
+        snprintf(buf, sizeof buf, "block %d face %d: face kind %d not supported", b, f, kind);
+        return buf;
+      }
+      if (kind == MGPU_FACE_PERIODIC) {
+        if (A.face_kind[f ^ 1] != MGPU_FACE_PERIODIC) {
+          snprintf(buf, sizeof buf, "block %d face %d: periodic face without a periodic opposite face", b, f);
+          return buf;
+        }
+        continue;
+      }
+      int nb, nf, o;
+      face_nbr(A, b, f, nb, nf, o);
+      if (nb < 0 || nb >= c.nblocks || nf < 0 || nf > 5 || o < 0 || o > 7) {
+        snprintf(buf, sizeof buf, "block %d face %d: neighbour (%d, face %d, orient %d) out of range", b, f, nb,
+                 nf, o);
+        return buf;
+      }
+      const mgpu_block &B = c.blocks[nb];
+      const int nbx[3] = {B.nx, B.ny, B.nz};
+      int a, t1, t2, bb, u1, u2;
+      face_axes(f, a, t1, t2);
+      face_axes(nf, bb, u1, u2);
+      const int m1 = (o & 1) ? u2 : u1, m2 = (o & 1) ? u1 : u2;
+      if (na[t1] != nbx[m1] || na[t2] != nbx[m2]) {
+        snprintf(buf, sizeof buf, "block %d face %d: extents %dx%d do not match face %d of block %d under orient %d",
+                 b, f, na[t1], na[t2], nf, nb, o);
+        return buf;
+      }
+      if (B.face_kind[nf] != MGPU_FACE_INTERFACE || B.nbr_block[nf] != b || B.nbr_face[nf] != f) {
+        snprintf(buf, sizeof buf, "block %d face %d: face %d of block %d does not point back", b, f, nf, nb);
+        return buf;
+      }
+      const int swap = o & 1, r1 = (o >> 1) & 1, r2 = (o >> 2) & 1;
+      const int inv = swap | ((swap ? r2 : r1) << 1) | ((swap ? r1 : r2) << 2);
+      if (B.orient[nf] != inv) {
+        snprintf(buf, sizeof buf, "block %d face %d: orient %d, but the neighbour declares %d (expected %d)", b, f,
+                 o, B.orient[nf], inv);
+        return buf;
+      }
+    }
+  }
+  return std::string();
+}
+
+// One affine coordinate: c0 + cl*l + cp*p + cq*q (l, p, q from 0).
+struct Aff {
+  long c0, cl, cp, cq;
+};
+
+// Every element a task touches lies inside the padded arrays: the offsets
+// are affine in the box indices, so their extremes sit at the box corners.
+bool task_in_bounds(const FaceTask &t, long ndst, long nsrc) {
+  for (int c = 0; c < 8; c++) {
+    long pd = t.dst_base, ps = t.src_base;
+    for (int x = 0; x < 3; x++) {
+      const long a = ((c >> x) & 1) ? t.n[x] - 1 : 0;
+      pd += a * t.dst_s[x];
+      ps += a * t.src_s[x];
+    }
+    if (pd < 0 || pd >= ndst || ps < 0 || ps >= nsrc) return false;
+  }
+  return true;
+}
+
+// Face tasks of every face of every block for `ng` ghost layers, and the
+// chunk table that maps workgroups onto them.  False if a task would reach
+// outside its arrays (a block table that passed check_topology never does).
+bool build_tasks(const mgpu_case &c, const std::vector<BlockData> &bds, int ng, std::vector<FaceTask> &tasks,
+                 std::vector<IfaceChunk> &chunks) {
+  tasks.clear();
+  chunks.clear();
+  for (int b = 0; b < c.nblocks; b++) {
+    const mgpu_block &A = c.blocks[b];
+    const Geom &ga = bds[b].g;
+    const int na[3] = {ga.nx, ga.ny, ga.nz};
+    for (int f = 0; f < 6; f++) {
+      int nb, nf, o;
+      face_nbr(A, b, f, nb, nf, o);
+      const Geom &gb = bds[nb].g;
+      const int nbx[3] = {gb.nx, gb.ny, gb.nz};
+      int a, t1, t2, bb, u1, u2;
+      face_axes(f, a, t1, t2);
+      face_axes(nf, bb, u1, u2);
+      Aff d[3] = {}, sr[3] = {};
+      d[a] = (f & 1) ? Aff{na[a] + 1, 1, 0, 0} : Aff{0, -1, 0, 0};
+      d[t1] = Aff{1, 0, 1, 0};
+      d[t2] = Aff{1, 0, 0, 1};
+      sr[bb] = (nf & 1) ? Aff{nbx[bb], -1, 0, 0} : Aff{1, 1, 0, 0};
+      // neighbour tangential coordinates: s1 = swap ? q : p, s2 = swap ? p : q
+      Aff s1 = (o & 1) ? Aff{0, 0, 0, 1} : Aff{0, 0, 1, 0};
+      Aff s2 = (o & 1) ? Aff{0, 0, 1, 0} : Aff{0, 0, 0, 1};
+      sr[u1] = (o & 2) ? Aff{nbx[u1] - s1.c0, -s1.cl, -s1.cp, -s1.cq} : Aff{1 + s1.c0, s1.cl, s1.cp, s1.cq};
+      sr[u2] = (o & 4) ? Aff{nbx[u2] - s2.c0, -s2.cl, -s2.cp, -s2.cq} : Aff{1 + s2.c0, s2.cl, s2.cp, s2.cq};
+      const long sta[3] = {1, ga.sx, (long)ga.sy}, stb[3] = {1, gb.sx, (long)gb.sy};
+      long db = 0, sb = 0, dv[3] = {0, 0, 0}, sv[3] = {0, 0, 0};
+      for (int x = 0; x < 3; x++) {
+        db += (d[x].c0 - 1 + G) * sta[x];
+        sb += (sr[x].c0 - 1 + G) * stb[x];
+        dv[0] += d[x].cl * sta[x];
+        dv[1] += d[x].cp * sta[x];
+        dv[2] += d[x].cq * sta[x];
+        sv[0] += sr[x].cl * stb[x];
+        sv[1] += sr[x].cp * stb[x];
+        sv[2] += sr[x].cq * stb[x];
+      }
+      const int ext[3] = {ng, na[t1], na[t2]};
+      // box axis 0 = the variable with the smallest destination stride
+      int ord[3] = {0, 1, 2};
+      for (int x = 0; x < 3; x++)
+        for (int y = x + 1; y < 3; y++)
+          if (labs(dv[ord[y]]) < labs(dv[ord[x]])) std::swap(ord[x], ord[y]);
+      FaceTask t{};
+      t.dst = b;
+      t.src = nb;
+      t.dst_base = (int)db;
+      t.src_base = (int)sb;
+      for (int x = 0; x < 3; x++) {
+        t.n[x] = ext[ord[x]];
+        t.dst_s[x] = (int)dv[ord[x]];
+        t.src_s[x] = (int)sv[ord[x]];
+      }
+      if (!task_in_bounds(t, ga.n, gb.n)) return false;
+      const int cnt = t.n[0] * t.n[1] * t.n[2];
+      const int id = (int)tasks.size();
+      tasks.push_back(t);
+      for (int st = 0; st < cnt; st += kIfaceThreads) chunks.push_back({id, st});
+    }
+  }
+  return true;
+}
+
+template <class T>
+int upload(mgpu_handle *h, const std::vector<T> &v, T **out) {
+  *out = nullptr;
+  if (v.empty()) return 0;
+  void *p = nullptr;
+  HIPCK(h, hipMalloc(&p, v.size() * sizeof(T)));
+  h->dev_tables.push_back(p);
+  HIPCK(h, hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  *out = (T *)p;
+  return 0;
+}
+
+// Task, chunk and pointer tables of the interface exchange.
+int setup_exchange(mgpu_handle *h) {
+  std::vector<FaceTask> tasks;
+  std::vector<IfaceChunk> chunks;
+  const int nb = (int)h->b.size();
+  int rc;
+  // state: 5 layers, 5 variables, one pointer table per state set
+  if (!build_tasks(h->c, h->b, G, tasks, chunks))
+    return fail(h, MGPU_EINVAL, "interface exchange would reach outside a block's arrays");
+  FaceTask *dt = nullptr;
+  IfaceChunk *dc = nullptr;
+  if ((rc = upload(h, tasks, &dt)) || (rc = upload(h, chunks, &dc))) return rc;
+  h->nchunks_s = (int)chunks.size();
+  for (int s = 0; s < 3; s++) {
+    std::vector<double *> ptrs;
+    for (int b = 0; b < nb; b++)
+      for (int v = 0; v < 5; v++) ptrs.push_back(h->b[b].st[s][v]);
+    double **dp = nullptr;
+    if ((rc = upload(h, ptrs, &dp))) return rc;
+    h->xs[s] = IfaceArgs{dt, dc, dp, 5};
+  }
+  // viscous: 2 layers of tau (+ p and the sensor for the shock filter)
+  if (h->c.iorder_visc) {
+    if (!build_tasks(h->c, h->b, 2, tasks, chunks))
+      return fail(h, MGPU_EINVAL, "viscous interface exchange would reach outside a block's arrays");
+    if ((rc = upload(h, tasks, &dt)) || (rc = upload(h, chunks, &dc))) return rc;
+    h->nchunks_v = (int)chunks.size();
+    const bool shock = h->c.filter == MGPU_FILTER_SHOCK;
+    const int nv = NTAU + (shock ? 2 : 0);
+    std::vector<double *> ptrs;
+    for (int b = 0; b < nb; b++) {
+      for (int q = 0; q < NTAU; q++) ptrs.push_back(h->b[b].tau[q]);
+      if (shock) {
+        ptrs.push_back(h->b[b].prs);
+        ptrs.push_back(h->b[b].sens);
+      }
+    }
+    double **dp = nullptr;
+    if ((rc = upload(h, ptrs, &dp))) return rc;
+    h->xv = IfaceArgs{dt, dc, dp, nv};
+  }
   return 0;
 }
 
@@ -192,7 +415,7 @@ void free_block(BlockData &bd) {
   for (auto &p : bd.idx) { if (p) (void)hipFree(p); p = nullptr; }
   for (auto &p : bd.m) { if (p) (void)hipFree(p); p = nullptr; }
   for (auto &p : bd.mv) { if (p) (void)hipFree(p); p = nullptr; }
-  for (auto &p : bd.fv) { if (p) (void)hipFree(p); p = nullptr; }
+  for (auto &p : bd.tau) { if (p) (void)hipFree(p); p = nullptr; }
   for (auto &p : bd.kv) { if (p) (void)hipFree(p); p = nullptr; }
   if (bd.prs) (void)hipFree(bd.prs);
   if (bd.sens) (void)hipFree(bd.sens);
@@ -232,6 +455,16 @@ hipError_t launch_block_stage(mgpu_handle *h, BlockData &bd, int in, int s0, int
   return launch_stage(sa, h->c.curvilinear != 0, visc_last, h->stream, bd.shape[v]);
 }
 
+// Exchange the ghosts of U^n after mgpu_put_state, once every block that
+// has state is in place.
+int refresh_ghosts(mgpu_handle *h) {
+  if (!h->ghosts_dirty) return 0;
+  int rc = exchange_state(h, h->cur);
+  if (rc) return rc;
+  h->ghosts_dirty = false;
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -249,7 +482,6 @@ int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
                 "shock filter needs iorder_visc=4 (its sensor reads the viscous gradients of U^n)");
   if (c->precision != MGPU_PREC_FP64)
     return fail(nullptr, MGPU_EUNSUP, "precision=%d not supported yet", c->precision);
-  if (c->nblocks != 1) return fail(nullptr, MGPU_EUNSUP, "multiblock cases not supported yet");
   for (int bI = 0; bI < c->nblocks; bI++) {
     const mgpu_block &bl = c->blocks[bI];
     if (bl.nx < G || bl.ny < G || bl.nz < G)
@@ -258,9 +490,10 @@ int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
     if (padded > kMaxPadded)
       return fail(nullptr, MGPU_EINVAL, "block %d: %zu padded points exceed the 2^28 limit of one block", bI,
                   padded);
-    for (int f = 0; f < 6; f++)
-      if (bl.face_kind[f] != MGPU_FACE_PERIODIC)
-        return fail(nullptr, MGPU_EUNSUP, "block %d face %d: only periodic faces supported yet", bI, f);
+  }
+  {
+    const std::string msg = check_topology(*c);
+    if (!msg.empty()) return fail(nullptr, MGPU_EINVAL, "%s", msg.c_str());
   }
 
   mgpu_handle *h = new mgpu_handle();
@@ -298,7 +531,7 @@ int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
       for (int q = 0; q < 6 && !rc; q++) rc = dalloc(h, &bd.idx[q], (size_t)len[q % 3]);
     }
     if (c->iorder_visc) {
-      for (int q = 0; q < 12 && !rc; q++) rc = dalloc(h, &bd.fv[q], n);
+      for (int q = 0; q < NTAU && !rc; q++) rc = dalloc(h, &bd.tau[q], n);
       for (int q = 0; q < 4 && !rc; q++) rc = dalloc(h, &bd.kv[q], n);
       if (c->filter == MGPU_FILTER_SHOCK) {
         if (!rc) rc = dalloc(h, &bd.prs, n);
@@ -306,6 +539,7 @@ int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
       }
     }
   }
+  if (!rc) rc = setup_exchange(h);
   if (rc) {
     g_create_error = h->err;
     mgpu_destroy(h);
@@ -368,10 +602,9 @@ int mgpu_put_state(mgpu_handle *h, int block, const double *rho, const double *r
     if (!src[v]) return fail(h, MGPU_EINVAL, "state array %d is null", v);
     HIPCK(h, hipMemcpyAsync(bd.st[h->cur][v], src[v], bytes, hipMemcpyHostToDevice, h->stream));
   }
-  int rc = halo(h, bd, bd.st[h->cur]);
-  if (rc) return rc;
   HIPCK(h, hipStreamSynchronize(h->stream));
   bd.have_state = true;
+  h->ghosts_dirty = true;  // the exchange runs before the next use of the ghosts
   return 0;
 }
 
@@ -382,6 +615,8 @@ int mgpu_get_state(mgpu_handle *h, int block, double *rho, double *rhou, double 
   double *dst[5] = {rho, rhou, rhov, rhow, rhoe};
   BlockData &bd = h->b[block];
   const size_t bytes = sizeof(double) * (size_t)bd.g.n;
+  int rc = refresh_ghosts(h);
+  if (rc) return rc;
   HIPCK(h, hipStreamSynchronize(h->stream));
   for (int v = 0; v < 5; v++) {
     if (!dst[v]) return fail(h, MGPU_EINVAL, "state array %d is null", v);
@@ -400,68 +635,68 @@ int mgpu_step(mgpu_handle *h, int nsteps) {
   const mgpu_case &c = h->c;
   const bool curv = c.curvilinear != 0, visc = c.iorder_visc != 0;
   const Gas gs = gas_of(c);
+  int rc = refresh_ghosts(h);
+  if (rc) return rc;
+  const double d11[6] = {D11_0, D11_1, D11_2, D11_3, D11_4, D11_5};
   for (int step = 0; step < nsteps; step++) {
     const int s0 = h->cur, sa = (h->cur + 1) % 3, sb = (h->cur + 2) % 3;
-    for (auto &bd : h->b) {
-      const Metrics met = metrics_of(bd);
-      int in = s0;
-      for (int irk = 1; irk <= c.nrk; irk++) {
-        const bool last = irk == c.nrk;
-        const int out = (irk % 2 == 1) ? sa : sb;
-        if (last && visc) {
-          ProfScope ps(h, MGPU_K_VISC);
+    int in = s0;
+    for (int irk = 1; irk <= c.nrk; irk++) {
+      const bool last = irk == c.nrk;
+      const int out = (irk % 2 == 1) ? sa : sb;
+      if (last && visc) {
+        ProfScope ps(h, MGPU_K_VISC);
+        for (auto &bd : h->b) {
           ViscArgs va{};
           va.g = bd.g;
           va.gas = gs;
           for (int v = 0; v < 5; v++) va.u0.v[v] = bd.st[s0][v];
-          va.met = met;
-          for (int q = 0; q < 12; q++) va.fv[q] = bd.fv[q];
+          va.met = metrics_of(bd);
+          for (int q = 0; q < NTAU; q++) va.tau[q] = bd.tau[q];
           va.prs = bd.prs;
           va.sens = bd.sens;
           va.want_sensor = c.filter == MGPU_FILTER_SHOCK;
           va.ducros = c.is_ducros;
           HIPCK(h, launch_visc(va, curv, h->stream));
+        }
+        HIPCK(h, launch_iface(h->xv, h->nchunks_v, h->stream));
+        for (auto &bd : h->b) {
           VdivArgs vd{};
           vd.g = bd.g;
-          vd.met = met;
-          for (int q = 0; q < 12; q++) vd.fv[q] = bd.fv[q];
+          vd.met = metrics_of(bd);
+          for (int q = 0; q < NTAU; q++) vd.tau[q] = bd.tau[q];
           for (int q = 0; q < 4; q++) vd.kv[q] = bd.kv[q];
           HIPCK(h, launch_vdiv(vd, curv, h->stream));
         }
-        {
-          ProfScope ps(h, MGPU_K_STAGE);
-          HIPCK(h, launch_block_stage(h, bd, in, s0, out, irk, last && visc));
-        }
-        int rc = halo(h, bd, bd.st[out]);
-        if (rc) return rc;
-        in = out;
       }
-      if (c.filter != MGPU_FILTER_NONE) {
-        ProfScope ps(h, MGPU_K_FILTER);
-        FilterArgs fa{};
-        fa.g = bd.g;
-        for (int v = 0; v < 5; v++) {
-          fa.uin.v[v] = bd.st[in][v];
-          fa.uout.v[v] = bd.st[s0][v];
-        }
-        for (int l = 0; l < 6; l++) {
-          const double d11[6] = {D11_0, D11_1, D11_2, D11_3, D11_4, D11_5};
-          fa.dd[l] = -d11[l] * c.dissip_coeff;
-        }
-        fa.prs = bd.prs;
-        fa.sens = c.is_ducros ? bd.sens : nullptr;
-        fa.xnu0 = -c.dissip_shock;
-        fa.xnu10 = c.dissip_coeff / 1260.0;
-        fa.csens = c.csens;
-        if (c.filter == MGPU_FILTER_SF) HIPCK(h, launch_filter_sf(fa, h->stream));
-        else HIPCK(h, launch_filter_shock(fa, h->stream));
+      {
+        ProfScope ps(h, MGPU_K_STAGE);
+        for (auto &bd : h->b) HIPCK(h, launch_block_stage(h, bd, in, s0, out, irk, last && visc));
       }
+      if ((rc = exchange_state(h, out))) return rc;
+      in = out;
     }
     if (c.filter != MGPU_FILTER_NONE) {
-      for (auto &bd : h->b) {
-        int rc = halo(h, bd, bd.st[s0]);
-        if (rc) return rc;
+      {
+        ProfScope ps(h, MGPU_K_FILTER);
+        for (auto &bd : h->b) {
+          FilterArgs fa{};
+          fa.g = bd.g;
+          for (int v = 0; v < 5; v++) {
+            fa.uin.v[v] = bd.st[in][v];
+            fa.uout.v[v] = bd.st[s0][v];
+          }
+          for (int l = 0; l < 6; l++) fa.dd[l] = -d11[l] * c.dissip_coeff;
+          fa.prs = bd.prs;
+          fa.sens = c.is_ducros ? bd.sens : nullptr;
+          fa.xnu0 = -c.dissip_shock;
+          fa.xnu10 = c.dissip_coeff / 1260.0;
+          fa.csens = c.csens;
+          if (c.filter == MGPU_FILTER_SF) HIPCK(h, launch_filter_sf(fa, h->stream));
+          else HIPCK(h, launch_filter_shock(fa, h->stream));
+        }
       }
+      if ((rc = exchange_state(h, s0))) return rc;
     } else {
       // U^{n+1} is the last stage output: make that set current
       h->cur = (c.nrk % 2 == 1) ? sa : sb;
@@ -477,6 +712,7 @@ int mgpu_autotune(mgpu_handle *h, int flags) {
     if (!h->b[bI].have_state) return fail(h, MGPU_ESTATE, "block %zu: state not set", bI);
   }
   const bool force = (flags & 1) != 0;
+  if (int rc0 = refresh_ghosts(h)) return rc0;
   const int s0 = h->cur, sa = (h->cur + 1) % 3;
   hipEvent_t e0 = get_event(h), e1 = get_event(h);
   if (!e0 || !e1) return fail(h, MGPU_EDEVICE, "hipEventCreate failed");
@@ -588,6 +824,7 @@ void mgpu_destroy(mgpu_handle *h) {
   }
   for (auto e : h->ev_pool) (void)hipEventDestroy(e);
   for (auto &bd : h->b) free_block(bd);
+  for (void *p : h->dev_tables) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }

@@ -99,24 +99,33 @@ struct StageArgs {
   int kchunk;            // planes per block along k
 };
 
+// Viscous flux tensor of U^n in the physical frame, per interior point:
+// tau[0..5] = -tau_xx,-tau_xy,-tau_xz,-tau_yy,-tau_yz,-tau_zz (the
+// reference's t11..t33 after scaling by -2mu), tau[6..8] = energy fluxes
+// e_d = u t_d1 + v t_d2 + w t_d3 - kappa dT/dx_d.  Frame-independent, so
+// that an interface copy of them equals the reference's copy of the
+// gradients followed by the flux evaluation at the ghost point.
+constexpr int NTAU = 9;
 struct ViscArgs {
   Geom g;
   Gas gas;
   CState u0;          // U^n
   Metrics met;
-  double *fv[12];     // out: viscous fluxes on the extended box [-1, n+2]^3
-  double *prs;        // out: p(U^n) on [-1, n+2]^3 (shock filter)
-  double *sens;       // out: Ducros sensor on [0, n+1]^3 (shock filter)
+  double *tau[NTAU];  // out: interior points
+  double *prs;        // out: p(U^n), interior points (shock filter)
+  double *sens;       // out: Ducros sensor, interior points (shock filter)
   int want_sensor;    // 1: write prs and sens
   int ducros;
 };
 
-// Divergence of the viscous fluxes: K_v = sum_d D5_d(F^v_d) * metric.
+// Divergence of the viscous fluxes: K_v = sum_d D5_d(F^v_d) * metric, with
+// F^v_d the projection of tau on the viscous metrics of direction d (c3) or
+// its row d (Cartesian), at every stencil point.
 struct VdivArgs {
   Geom g;
   Metrics met;
-  const double *fv[12];  // F^v (x), G^v (y), H^v (z), 4 components each
-  double *kv[4];         // out, interior points
+  const double *tau[NTAU];  // interior + 2 face-ghost layers
+  double *kv[4];            // out, interior points
 };
 
 struct FilterArgs {
@@ -129,11 +138,26 @@ struct FilterArgs {
   double xnu0, xnu10, csens;
 };
 
-// Halo copy: one face pair of one block.  For a periodic face the source
-// is the opposite side of the same block.
-struct HaloArgs {
-  Geom g;
-  State u;
+// One face of one destination block in the interface exchange: the ghost
+// layers of block `dst` on that face take the interior layers of block
+// `src` on the neighbour face.  Point (a0,a1,a2) of the box n[0]xn[1]xn[2]
+// writes element dst_base + sum a_d*dst_s[d] of each of the dst block's
+// arrays and reads src_base + sum a_d*src_s[d] of the src block's.
+struct FaceTask {
+  int dst, src;
+  int n[3];
+  int dst_base, src_base;
+  int dst_s[3], src_s[3];
+};
+struct IfaceChunk {
+  int task, start;
+};
+constexpr int kIfaceThreads = 256;
+struct IfaceArgs {
+  const FaceTask *tasks;
+  const IfaceChunk *chunks;
+  double *const *ptrs;  // ptrs[block*nvar + v]: array v of block
+  int nvar;
 };
 
 }  // namespace mgpu
@@ -147,5 +171,5 @@ hipError_t launch_visc(const ViscArgs &a, bool curv, hipStream_t s);
 hipError_t launch_vdiv(const VdivArgs &a, bool curv, hipStream_t s);
 hipError_t launch_filter_sf(const FilterArgs &a, hipStream_t s);
 hipError_t launch_filter_shock(const FilterArgs &a, hipStream_t s);
-hipError_t launch_halo_periodic(const HaloArgs &a, int nvar_from_0, hipStream_t s);
+hipError_t launch_iface(const IfaceArgs &a, int nchunks, hipStream_t s);
 }  // namespace mgpu

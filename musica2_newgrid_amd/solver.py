@@ -16,17 +16,24 @@ from . import _lib
 from ._lib import MgpuBlock, MgpuCase, check, dptr, lib
 
 
-def make_case_struct(case, precision: int = _lib.PREC_FP64):
-    blk = MgpuBlock()
-    blk.nx, blk.ny, blk.nz = case.nx, case.ny, case.nz
-    for f in range(6):
-        blk.face_kind[f] = _lib.FACE_PERIODIC
-        blk.nbr_block[f] = 0
-        blk.nbr_face[f] = f ^ 1
-        blk.orient[f] = 0
-    blocks = (MgpuBlock * 1)(blk)
+def make_case_struct(case, precision: int = _lib.PREC_FP64, topology=None):
+    """C-ABI case description.  `topology` is a block table (see
+    multiblock.py); None means one block with periodic faces."""
+    if topology is None:
+        topology = [{"nx": case.nx, "ny": case.ny, "nz": case.nz, "kind": [_lib.FACE_PERIODIC] * 6,
+                     "nbr_block": [0] * 6, "nbr_face": [1, 0, 3, 2, 5, 4], "orient": [0] * 6}]
+    blocks = (MgpuBlock * len(topology))()
+    for b, d in enumerate(topology):
+        blk = blocks[b]
+        blk.nx, blk.ny, blk.nz = d["nx"], d["ny"], d["nz"]
+        kinds = d.get("kind", [_lib.FACE_INTERFACE] * 6)
+        for f in range(6):
+            blk.face_kind[f] = kinds[f]
+            blk.nbr_block[f] = d["nbr_block"][f]
+            blk.nbr_face[f] = d["nbr_face"][f]
+            blk.orient[f] = d["orient"][f]
     cs = MgpuCase()
-    cs.nblocks = 1
+    cs.nblocks = len(topology)
     cs.curvilinear = int(case.curvilinear)
     cs.iorder_visc = int(case.iorder_visc)
     cs.filter = int(case.filter)
@@ -48,44 +55,58 @@ def make_case_struct(case, precision: int = _lib.PREC_FP64):
 
 
 class GpuSolver:
-    """Owns one mgpu handle (device state, metrics, scratch) for a case."""
+    """Owns one mgpu handle (device state, metrics, scratch) for a case.
 
-    def __init__(self, case, precision: int = _lib.PREC_FP64):
+    One block: ``GpuSolver(case)``.  Several blocks: ``GpuSolver(case,
+    topology=blocks, block_cases=cases)``, where `case` carries the flags and
+    block_cases[b] the extents, metrics and initial state of block b."""
+
+    def __init__(self, case, precision: int = _lib.PREC_FP64, topology=None, block_cases=None):
         L = lib()
         self.case = case
-        self._cs, self._blocks = make_case_struct(case, precision)
+        self.block_cases = [case] if block_cases is None else list(block_cases)
+        if topology is not None and len(topology) != len(self.block_cases):
+            raise ValueError("topology and block_cases differ in length")
+        self._cs, self._blocks = make_case_struct(case, precision, topology)
         h = C.c_void_p()
         rc = L.mgpu_create(C.byref(self._cs), C.byref(h))
         check(rc, None, "mgpu_create")
         self.h = h
-        self._set_metrics()
-        self.put_state(case.U)
+        for b, bc in enumerate(self.block_cases):
+            self._set_metrics(b, bc)
+            self.put_state(bc.U, b)
 
-    def _set_metrics(self):
-        L, c = lib(), self.case
+    @property
+    def nblocks(self) -> int:
+        return len(self.block_cases)
+
+    def _set_metrics(self, b, c):
+        L = lib()
         if c.curvilinear:
-            self._m = [np.ascontiguousarray(a) for a in c.m]
-            tab = (_lib._dp * 10)(*[dptr(a) for a in self._m])
+            m = [np.ascontiguousarray(a) for a in c.m]
+            tab = (_lib._dp * 10)(*[dptr(a) for a in m])
+            tabv = None
             if c.mv is not None:
-                self._mv = [np.ascontiguousarray(a) for a in c.mv]
-                tabv = (_lib._dp * 10)(*[dptr(a) for a in self._mv])
-            else:
-                tabv = None
-            check(L.mgpu_set_metrics_c3(self.h, 0, tab, tabv), self.h, "mgpu_set_metrics_c3")
+                mv = [np.ascontiguousarray(a) for a in c.mv]
+                tabv = (_lib._dp * 10)(*[dptr(a) for a in mv])
+            check(L.mgpu_set_metrics_c3(self.h, b, tab, tabv), self.h, "mgpu_set_metrics_c3")
         else:
             arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in c.cart]
-            check(L.mgpu_set_metrics_cart(self.h, 0, *[dptr(a) for a in arrs]), self.h,
+            check(L.mgpu_set_metrics_cart(self.h, b, *[dptr(a) for a in arrs]), self.h,
                   "mgpu_set_metrics_cart")
 
-    def put_state(self, U: Sequence[np.ndarray]) -> None:
+    def put_state(self, U: Sequence[np.ndarray], block: int = 0) -> None:
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in U]
-        check(lib().mgpu_put_state(self.h, 0, *[dptr(a) for a in arrs]), self.h, "mgpu_put_state")
+        check(lib().mgpu_put_state(self.h, block, *[dptr(a) for a in arrs]), self.h, "mgpu_put_state")
 
-    def get_state(self) -> List[np.ndarray]:
-        c = self.case
+    def get_state(self, block: int = 0) -> List[np.ndarray]:
+        c = self.block_cases[block]
         out = [np.empty((c.nz + 10, c.ny + 10, c.nx + 10)) for _ in range(5)]
-        check(lib().mgpu_get_state(self.h, 0, *[dptr(a) for a in out]), self.h, "mgpu_get_state")
+        check(lib().mgpu_get_state(self.h, block, *[dptr(a) for a in out]), self.h, "mgpu_get_state")
         return out
+
+    def get_states(self) -> List[List[np.ndarray]]:
+        return [self.get_state(b) for b in range(self.nblocks)]
 
     def step(self, nsteps: int = 1) -> None:
         check(lib().mgpu_step(self.h, int(nsteps)), self.h, "mgpu_step")
@@ -93,9 +114,9 @@ class GpuSolver:
     def autotune(self, force: bool = False) -> None:
         check(lib().mgpu_autotune(self.h, int(force)), self.h, "mgpu_autotune")
 
-    def stage_config(self, variant: int = 0):
+    def stage_config(self, variant: int = 0, block: int = 0):
         tx, ty, kc = C.c_int(), C.c_int(), C.c_int()
-        check(lib().mgpu_stage_config(self.h, 0, variant, C.byref(tx), C.byref(ty), C.byref(kc)), self.h,
+        check(lib().mgpu_stage_config(self.h, block, variant, C.byref(tx), C.byref(ty), C.byref(kc)), self.h,
               "mgpu_stage_config")
         return tx.value, ty.value, kc.value
 

@@ -1,10 +1,26 @@
-"""Multiblock interface copies of the oracle: a periodic block split into
-blocks (aligned and reversed) advances exactly like the single block."""
+"""Multiblock interface exchange of the oracle, pinned by invariance: a
+periodic block cut into sub-blocks whose axes are signed permutations of the
+parent's (every orientation code of the face exchange) advances like the
+uncut parent.  A wrong ghost index, a missed reversal or a swapped tangential
+axis puts a wrong neighbour value into the stencils and fails these."""
 import numpy as np
+import pytest
 
 from musica2_newgrid_amd import cases, multiblock as mb
 from musica2_newgrid_amd import grid as gr
 from oracle import oracle as orc
+
+# (perm, sign) per sub-block; the c3 sets produce all eight orientation codes
+CART_ORIENTS = [((0, 1, 2), (1, 1, 1)), ((0, 1, 2), (-1, 1, 1)), ((0, 1, 2), (1, -1, -1))]
+C3_ORIENTS = [((0, 1, 2), (1, 1, 1)), ((2, 1, 0), (-1, 1, 1)), ((2, 1, 0), (1, 1, -1)),
+              ((1, 0, 2), (-1, 1, -1)), ((2, 0, 1), (-1, 1, -1)), ((0, 2, 1), (-1, 1, 1)),
+              ((1, 2, 0), (1, -1, 1))]
+C3_PARENT = (35, 10, 10)
+
+
+def _rel(A, B):
+    return max(float(np.max(np.abs(gr.interior(a) - gr.interior(b)))) / float(np.max(np.abs(gr.interior(a))))
+               for a, b in zip(A, B))
 
 
 def test_single_block_multi_path_equals_single_path():
@@ -17,38 +33,34 @@ def test_single_block_multi_path_equals_single_path():
         assert np.array_equal(gr.interior(a), gr.interior(b))
 
 
-def _split_case(c, m, reverse):
-    bc = cases.tgv(m)  # carries the metric layout of an m x n x n Cartesian block
-    bc.nx, bc.ny, bc.nz = m, c.ny, c.nz
-    idx = c.cart[0][0]
-    sign = -1.0 if reverse else 1.0
-    bc.cart = [np.full(m + 10, sign * idx), np.full(c.ny + 10, idx), np.full(c.nz + 10, idx),
-               np.full(m + 10, sign * idx), np.full(c.ny + 10, idx), np.full(c.nz + 10, idx)]
-    return bc
+def test_orientation_table_is_consistent():
+    c = cases.vortex_wavy(*C3_PARENT, filter=2)
+    blocks, _ = mb.split_oriented(c, len(C3_ORIENTS), C3_ORIENTS)
+    mb.check_table(blocks)
+    codes = {blk["orient"][f] for blk in blocks for f in range(6)}
+    assert codes == set(range(8)), codes
 
 
-def test_split_blocks_equal_single_block():
-    n, parts, rev = 24, 3, (1,)
+def test_cartesian_split_with_reversed_axes_equals_parent():
+    n, parts = 24, 3
     c = cases.tgv(n)
     A = c.copy_state()
-    orc.Oracle(c).step(A, 2)
-    blocks = mb.split_x(n, n, n, parts, reverse=rev)
-    Us = mb.split_x_state(c.U, parts, reverse=rev)
-    for v in range(5):
-        orc.fill_interfaces(blocks, [U[v] for U in Us])
-    bcs = [_split_case(c, n // parts, b in rev) for b in range(parts)]
-    orc.step_multi(c, blocks, bcs, Us, 2)
-    B = mb.join_x_state(Us, reverse=rev)
-    for q, (a, b) in enumerate(zip(A, B)):
-        ai, bi = gr.interior(a), gr.interior(b)
-        # reversed blocks sum their i-stencils in mirrored order: round-off only
-        assert np.max(np.abs(ai - bi)) <= 1e-12 * np.max(np.abs(ai)) + 1e-12, q
-    # aligned split only: bitwise
-    blocks = mb.split_x(n, n, n, parts)
-    Us = mb.split_x_state(c.U, parts)
-    for v in range(5):
-        orc.fill_interfaces(blocks, [U[v] for U in Us])
-    orc.step_multi(c, blocks, [_split_case(c, n // parts, False) for _ in range(parts)], Us, 2)
-    B = mb.join_x_state(Us)
-    for a, b in zip(A, B):
-        assert np.array_equal(gr.interior(a), gr.interior(b))
+    orc.Oracle(c).step(A, 3)
+    blocks, bcs = mb.split_oriented(c, parts, CART_ORIENTS)
+    Us = [bc.copy_state() for bc in bcs]
+    orc.step_multi(c, blocks, bcs, Us, 3)
+    B = mb.join_oriented(c, Us, CART_ORIENTS)
+    assert _rel(A, B) < 1e-13
+
+
+@pytest.mark.parametrize("filt", [1, 2])
+def test_c3_split_with_permuted_axes_equals_parent(filt):
+    c = cases.vortex_wavy(*C3_PARENT, filter=filt)
+    A = c.copy_state()
+    orc.Oracle(c).step(A, 3)
+    blocks, bcs = mb.split_oriented(c, len(C3_ORIENTS), C3_ORIENTS)
+    Us = [bc.copy_state() for bc in bcs]
+    orc.step_multi(c, blocks, bcs, Us, 3)
+    B = mb.join_oriented(c, Us, C3_ORIENTS)
+    # permuted axes sum the three directions in another order: round-off only
+    assert _rel(A, B) < 1e-12

@@ -1,0 +1,53 @@
+"""mgpu_create's checks of the block table (they run before any device
+call, so on any host): neighbours in range, matching face extents under the
+orientation, interfaces declared from both sides with the inverse
+orientation."""
+import ctypes as C
+
+import pytest
+
+from musica2_newgrid_amd import _lib, cases, multiblock as mb
+from musica2_newgrid_amd.solver import make_case_struct
+
+
+def _create(parent, blocks):
+    cs, keep = make_case_struct(parent, topology=blocks)
+    h = C.c_void_p()
+    rc = _lib.lib().mgpu_create(C.byref(cs), C.byref(h))
+    msg = _lib.lib().mgpu_last_error(None)
+    return rc, (msg.decode() if isinstance(msg, bytes) else msg)
+
+
+def _table():
+    parent = cases.tgv(30)
+    blocks, _ = mb.split_oriented(parent, 3, [((0, 1, 2), (1, 1, 1)), ((0, 1, 2), (-1, 1, 1)),
+                                              ((0, 1, 2), (1, -1, -1))])
+    return parent, blocks
+
+
+def test_back_link_is_checked():
+    parent, blocks = _table()
+    blocks[0]["nbr_face"][1] = 0 if blocks[0]["nbr_face"][1] == 1 else 1
+    rc, msg = _create(parent, blocks)
+    assert rc == _lib.EINVAL and "point back" in msg
+
+
+def test_orientation_must_be_inverse_on_both_sides():
+    parent, blocks = _table()
+    blocks[1]["orient"][2] = 2
+    rc, msg = _create(parent, blocks)
+    assert rc == _lib.EINVAL and "orient" in msg
+
+
+def test_face_extents_must_match():
+    parent, blocks = _table()
+    blocks[2]["ny"] = 29
+    rc, msg = _create(parent, blocks)
+    assert rc == _lib.EINVAL and "extents" in msg
+
+
+def test_neighbour_out_of_range():
+    parent, blocks = _table()
+    blocks[0]["nbr_block"][0] = 7
+    rc, msg = _create(parent, blocks)
+    assert rc == _lib.EINVAL and "out of range" in msg
