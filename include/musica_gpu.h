@@ -60,7 +60,18 @@ extern "C" {
 #define MGPU_PREC_FP64 0
 #define MGPU_PREC_MIXED 1
 
-/* Faces are numbered imin,imax,jmin,jmax,kmin,kmax = 0..5. */
+/* Launch flags.  By default every step is replayed from a HIP graph captured
+ * on first use (one per U^n buffer), with one batched launch per kernel for
+ * all blocks; EAGER launches the same kernels one by one instead. */
+#define MGPU_LAUNCH_EAGER 1
+
+/* Faces are numbered imin,imax,jmin,jmax,kmin,kmax = 0..5.  An interface
+ * face f of block b takes its ghosts from face nbr_face[f] of block
+ * nbr_block[f].  The tangential axes of a face are its two other axes in
+ * increasing order; orient bit 0 swaps them, bit 1 reverses the neighbour's
+ * first tangential axis, bit 2 its second.  Both sides must declare the
+ * interface, with inverse orientations (swap'=swap, rev1'=swap?rev2:rev1,
+ * rev2'=swap?rev1:rev2). */
 typedef struct mgpu_block {
   int32_t nx, ny, nz;     /* interior cells of the block                     */
   int32_t face_kind[6];   /* MGPU_FACE_*                                     */
@@ -79,7 +90,7 @@ typedef struct mgpu_case {
   int32_t is_ducros;      /* Ducros sensor in the shock filter                 */
   int32_t nrk;            /* RK stages (mod_time.f90:121)                       */
   int32_t precision;      /* MGPU_PREC_*                                       */
-  int32_t reserved;
+  int32_t launch_flags;   /* MGPU_LAUNCH_* (0: default)                        */
   const mgpu_block *blocks;
   double crk[8];          /* RK stage coefficients crk(1..nrk)                 */
   double deltat;          /* time step                                         */

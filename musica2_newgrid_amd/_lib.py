@@ -17,6 +17,7 @@ FACE_PERIODIC = 0
 FACE_INTERFACE = 1
 FILTER_NONE, FILTER_SF, FILTER_SHOCK = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
+LAUNCH_EAGER = 1
 OK, EINVAL, EDEVICE, ENOMEM, EUNSUP, ESTATE = 0, -1, -2, -3, -4, -5
 K_STAGE, K_VISC, K_FILTER, K_HALO = 0, 1, 2, 3
 K_NAMES = {K_STAGE: "stage", K_VISC: "visc", K_FILTER: "filter", K_HALO: "halo"}
@@ -58,7 +59,7 @@ class MgpuCase(C.Structure):
         ("is_ducros", C.c_int32),
         ("nrk", C.c_int32),
         ("precision", C.c_int32),
-        ("reserved", C.c_int32),
+        ("launch_flags", C.c_int32),
         ("blocks", C.POINTER(MgpuBlock)),
         ("crk", C.c_double * 8),
         ("deltat", C.c_double),

@@ -24,20 +24,18 @@ constexpr int NHX = 10 * FY, NH = NHX + 10 * FX, NS = (NH + FT - 1) / FT;  // st
 constexpr int NPX = FY * (FX + 4), NPY = (FY + 4) * FX;                  // p cells along i / j
 constexpr int NSP = (NPX + NPY + FT - 1) / FT;
 
-struct Tiling {
-  int tiles_x, tiles_y, ntiles, kchunk;
-};
-
-__device__ __forceinline__ void tile_of(const Tiling &t, int &i0, int &j0, int &kbeg, int &kend, int nz) {
-  const int logical = xcd_remap(blockIdx.x, t.ntiles);
-  const int tile_x = logical % t.tiles_x;
-  const int rest = logical / t.tiles_x;
-  const int tile_y = rest % t.tiles_y;
-  const int chunk = rest / t.tiles_y;
+// Block and tile of this workgroup in a batched launch.
+__device__ __forceinline__ const FilterArgs &tile_of(const Batch<FilterArgs> *__restrict__ tab, int nb, int ntiles,
+                                                     int &i0, int &j0, int &kbeg, int &kend) {
+  const int w = xcd_remap(blockIdx.x, ntiles);
+  const Batch<FilterArgs> &e = batch_find(tab, nb, w);
+  int tile_x, tile_y, chunk;
+  tile_coords(e.t, w, tile_x, tile_y, chunk);
   i0 = tile_x * FX + 1;
   j0 = tile_y * FY + 1;
-  kbeg = chunk * t.kchunk + 1;
-  kend = min(kbeg + t.kchunk - 1, nz);
+  kbeg = chunk * e.t.kchunk + 1;
+  kend = min(kbeg + e.t.kchunk - 1, e.a.g.nz);
+  return e.a;
 }
 
 // Static assignment of the 5-cell state halo of a plane to the lanes.
@@ -83,12 +81,12 @@ __device__ __forceinline__ void ld5(const CState &U, idx_t p, double o[5]) {
   for (int v = 0; v < 5; v++) o[v] = ldg(U.v[v], p);
 }
 
-__global__ __launch_bounds__(FT, 2) void k_filter_sf(FilterArgs a, Tiling tl) {
+__global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__restrict__ tab, int nb, int ntiles) {
   __shared__ double sux[5][FY][FX + 10];
   __shared__ double suy[5][FY + 10][FX];
-  const Geom &g = a.g;
   int i0, j0, kbeg, kend;
-  tile_of(tl, i0, j0, kbeg, kend, g.nz);
+  const FilterArgs &a = tile_of(tab, nb, ntiles, i0, j0, kbeg, kend);
+  const Geom &g = a.g;
   const int tx = threadIdx.x % FX, ty = threadIdx.x / FX;
   const int i = i0 + tx, j = j0 + ty;
   const bool own = i <= g.nx && j <= g.ny;
@@ -168,7 +166,8 @@ __device__ __forceinline__ double dface(const double *x, int s, double xk0, doub
              xk10;
 }
 
-__global__ __launch_bounds__(FT, 2) void k_filter_shock(FilterArgs a, Tiling tl) {
+__global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> *__restrict__ tab, int nb,
+                                                        int ntiles) {
   __shared__ double sux[5][FY][FX + 10];
   __shared__ double suy[5][FY + 10][FX];
   __shared__ double spx[FY][FX + 4];    // p at i0-2 .. i0+FX+1
@@ -178,9 +177,9 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(FilterArgs a, Tiling tl)
   __shared__ double psx[FY][FX + 2];    // psens_x at i0-1 .. i0+FX
   __shared__ double psy[FY + 2][FX];    // psens_y at j0-1 .. j0+FY
 
-  const Geom &g = a.g;
   int i0, j0, kbeg, kend;
-  tile_of(tl, i0, j0, kbeg, kend, g.nz);
+  const FilterArgs &a = tile_of(tab, nb, ntiles, i0, j0, kbeg, kend);
+  const Geom &g = a.g;
   const int tid = threadIdx.x, tx = tid % FX, ty = tid / FX;
   const int i = i0 + tx, j = j0 + ty;
   const bool own = i <= g.nx && j <= g.ny;
@@ -361,31 +360,31 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(FilterArgs a, Tiling tl)
   }
 }
 
-Tiling tiling_for(const Geom &g) {
-  Tiling t;
+}  // namespace
+
+// Tiles of the filters: chunks of >= 16 planes, about 1024 workgroups per block.
+TileRange filter_tiles(const Geom &g) {
+  TileRange t{};
   t.tiles_x = (g.nx + FX - 1) / FX;
   t.tiles_y = (g.ny + FY - 1) / FY;
   const long nxy = (long)t.tiles_x * t.tiles_y;
-  long want = (1024 + nxy - 1) / nxy;
+  const long want = (1024 + nxy - 1) / nxy;
   int kc = (int)((g.nz + want - 1) / want);
   if (kc < 16) kc = 16;
   if (kc > g.nz) kc = g.nz;
   t.kchunk = kc;
-  t.ntiles = t.tiles_x * t.tiles_y * ((g.nz + kc - 1) / kc);
   return t;
 }
 
-}  // namespace
-
-hipError_t launch_filter_sf(const FilterArgs &a, hipStream_t s) {
-  const Tiling t = tiling_for(a.g);
-  hipLaunchKernelGGL(k_filter_sf, dim3(t.ntiles), dim3(FT), 0, s, a, t);
+hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int nt, hipStream_t s) {
+  if (nt <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_filter_sf, dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   return hipGetLastError();
 }
 
-hipError_t launch_filter_shock(const FilterArgs &a, hipStream_t s) {
-  const Tiling t = tiling_for(a.g);
-  hipLaunchKernelGGL(k_filter_shock, dim3(t.ntiles), dim3(FT), 0, s, a, t);
+hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int nt, hipStream_t s) {
+  if (nt <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_filter_shock, dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   return hipGetLastError();
 }
 

@@ -89,7 +89,7 @@ __device__ __forceinline__ void flux_m(const Prim &q, int dir, const double m[3]
 }
 
 template <int TX, int TY, bool CURV, bool VISC>
-__global__ __launch_bounds__(TX *TY, 2) void k_stage(StageArgs a, int tiles_x, int tiles_y, int ntiles) {
+__global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__restrict__ tab, int nb, int ntiles) {
   constexpr int NT = TX * TY;
   constexpr int NHX = 10 * TY;            // i-halo cells per plane
   constexpr int NH = NHX + 10 * TX;       // all halo cells per plane
@@ -99,18 +99,19 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(StageArgs a, int tiles_x, i
   __shared__ double sfx2[2][5][TY][TX + 10];
   __shared__ double sfy2[2][5][TY + 10][TX];
 
+  const int w = xcd_remap(blockIdx.x, ntiles);
+  const Batch<StageArgs> &e = batch_find(tab, nb, w);
+  const StageArgs &a = e.a;
   const Geom &g = a.g;
-  const int logical = xcd_remap(blockIdx.x, ntiles);
-  const int tile_x = logical % tiles_x;
-  const int rest = logical / tiles_x;
-  const int tile_y = rest % tiles_y;
-  const int chunk = rest / tiles_y;
+  int tile_x, tile_y, chunk;
+  tile_coords(e.t, w, tile_x, tile_y, chunk);
+  const int kchunk = e.t.kchunk;
 
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX, tid = threadIdx.x;
   const int i0 = tile_x * TX + 1, j0 = tile_y * TY + 1;
   const int i = i0 + tx, j = j0 + ty;
-  const int kbeg = chunk * a.kchunk + 1;
-  const int kend = min(kbeg + a.kchunk - 1, g.nz);
+  const int kbeg = chunk * kchunk + 1;
+  const int kend = min(kbeg + kchunk - 1, g.nz);
   const bool own = (i <= g.nx) && (j <= g.ny);
   const bool needx = (i <= g.nx + 5) && (j <= g.ny);  // own point feeds the i-pencil
   const bool needy = (i <= g.nx) && (j <= g.ny + 5);  // own point feeds the j-pencil
@@ -275,17 +276,14 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(StageArgs a, int tiles_x, i
 }
 
 template <int TX, int TY>
-hipError_t launch_t(const StageArgs &a, bool curv, bool visc, hipStream_t s) {
-  const int tiles_x = (a.g.nx + TX - 1) / TX, tiles_y = (a.g.ny + TY - 1) / TY;
-  const int nchunk = (a.g.nz + a.kchunk - 1) / a.kchunk;
-  const int nt = tiles_x * tiles_y * nchunk;
+hipError_t launch_t(const Batch<StageArgs> *tab, int nb, int nt, bool curv, bool visc, hipStream_t s) {
   dim3 blk(TX * TY), grd(nt);
   if (curv) {
-    if (visc) hipLaunchKernelGGL((k_stage<TX, TY, true, true>), grd, blk, 0, s, a, tiles_x, tiles_y, nt);
-    else hipLaunchKernelGGL((k_stage<TX, TY, true, false>), grd, blk, 0, s, a, tiles_x, tiles_y, nt);
+    if (visc) hipLaunchKernelGGL((k_stage<TX, TY, true, true>), grd, blk, 0, s, tab, nb, nt);
+    else hipLaunchKernelGGL((k_stage<TX, TY, true, false>), grd, blk, 0, s, tab, nb, nt);
   } else {
-    if (visc) hipLaunchKernelGGL((k_stage<TX, TY, false, true>), grd, blk, 0, s, a, tiles_x, tiles_y, nt);
-    else hipLaunchKernelGGL((k_stage<TX, TY, false, false>), grd, blk, 0, s, a, tiles_x, tiles_y, nt);
+    if (visc) hipLaunchKernelGGL((k_stage<TX, TY, false, true>), grd, blk, 0, s, tab, nb, nt);
+    else hipLaunchKernelGGL((k_stage<TX, TY, false, false>), grd, blk, 0, s, tab, nb, nt);
   }
   return hipGetLastError();
 }
@@ -296,13 +294,24 @@ hipError_t launch_t(const StageArgs &a, bool curv, bool visc, hipStream_t s) {
 const int kStageShapes[][2] = {{64, 4}, {32, 8}, {64, 8}, {16, 16}, {32, 16}};
 const int kNumStageShapes = 5;
 
-hipError_t launch_stage(const StageArgs &a, bool curv, bool visc, hipStream_t s, int shape) {
+TileRange stage_tiles(const Geom &g, int shape, int kchunk) {
+  const int TX = kStageShapes[shape][0], TY = kStageShapes[shape][1];
+  TileRange t{};
+  t.tiles_x = (g.nx + TX - 1) / TX;
+  t.tiles_y = (g.ny + TY - 1) / TY;
+  t.kchunk = kchunk < 1 ? 1 : (kchunk > g.nz ? g.nz : kchunk);
+  return t;
+}
+
+hipError_t launch_stage(const Batch<StageArgs> *tab, int nb, int nt, bool curv, bool visc, int shape,
+                        hipStream_t s) {
+  if (nt <= 0) return hipSuccess;
   switch (shape) {
-    case 1: return launch_t<32, 8>(a, curv, visc, s);
-    case 2: return launch_t<64, 8>(a, curv, visc, s);
-    case 3: return launch_t<16, 16>(a, curv, visc, s);
-    case 4: return launch_t<32, 16>(a, curv, visc, s);
-    default: return launch_t<64, 4>(a, curv, visc, s);
+    case 1: return launch_t<32, 8>(tab, nb, nt, curv, visc, s);
+    case 2: return launch_t<64, 8>(tab, nb, nt, curv, visc, s);
+    case 3: return launch_t<16, 16>(tab, nb, nt, curv, visc, s);
+    case 4: return launch_t<32, 16>(tab, nb, nt, curv, visc, s);
+    default: return launch_t<64, 4>(tab, nb, nt, curv, visc, s);
   }
 }
 

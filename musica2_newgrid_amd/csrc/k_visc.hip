@@ -51,7 +51,7 @@ __device__ __forceinline__ void ld5(const CState &U, idx_t p, double o[5]) {
 }
 
 template <int TX, int TY, bool CURV>
-__global__ __launch_bounds__(TX *TY, 2) void k_visc(ViscArgs a, int tiles_x, int tiles_y, int ntiles, int kchunk) {
+__global__ __launch_bounds__(TX *TY, 2) void k_visc(const Batch<ViscArgs> *__restrict__ tab, int nb, int ntiles) {
   constexpr int NT = TX * TY;
   constexpr int NHX = 4 * TY;             // i-halo cells per plane
   constexpr int NH = NHX + 4 * TX;        // all halo cells per plane
@@ -59,12 +59,13 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(ViscArgs a, int tiles_x, int
   __shared__ double spx[4][TY][TX + 4];
   __shared__ double spy[4][TY + 4][TX];
 
+  const int w = xcd_remap(blockIdx.x, ntiles);
+  const auto &e = batch_find(tab, nb, w);
+  const auto &a = e.a;
   const Geom &g = a.g;
-  const int logical = xcd_remap(blockIdx.x, ntiles);
-  const int tile_x = logical % tiles_x;
-  const int rest = logical / tiles_x;
-  const int tile_y = rest % tiles_y;
-  const int chunk = rest / tiles_y;
+  int tile_x, tile_y, chunk;
+  tile_coords(e.t, w, tile_x, tile_y, chunk);
+  const int kchunk = e.t.kchunk;
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX, tid = threadIdx.x;
   const int i0 = tile_x * TX + 1, j0 = tile_y * TY + 1;
   const int i = i0 + tx, j = j0 + ty;
@@ -255,23 +256,20 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(ViscArgs a, int tiles_x, int
   }
 }
 
-template <int TX, int TY>
-hipError_t launch_t(const ViscArgs &a, bool curv, hipStream_t s) {
-  const int ex = a.g.nx, ey = a.g.ny, ez = a.g.nz;
-  const int tiles_x = (ex + TX - 1) / TX, tiles_y = (ey + TY - 1) / TY;
-  const long nxy = (long)tiles_x * tiles_y;
-  // about 4 workgroups per CU-slot in flight (256 CUs x 2), chunks of >= 16 planes
-  long want = (2048 + nxy - 1) / nxy;
-  int kchunk = (int)((ez + want - 1) / want);
-  if (kchunk < 16) kchunk = 16;
-  if (kchunk > ez) kchunk = ez;
-  const int nchunk = (ez + kchunk - 1) / kchunk;
-  const int nt = tiles_x * tiles_y * nchunk;
-  if (curv)
-    hipLaunchKernelGGL((k_visc<TX, TY, true>), dim3(nt), dim3(TX * TY), 0, s, a, tiles_x, tiles_y, nt, kchunk);
-  else
-    hipLaunchKernelGGL((k_visc<TX, TY, false>), dim3(nt), dim3(TX * TY), 0, s, a, tiles_x, tiles_y, nt, kchunk);
-  return hipGetLastError();
+// Tiles of the 32x16 viscous kernels: chunks of >= 16 planes, about 2048
+// workgroups per block (4 per CU slot of 256 CUs x 2).
+constexpr int VX = 32, VY = 16;
+TileRange vtiles(const Geom &g) {
+  TileRange t{};
+  t.tiles_x = (g.nx + VX - 1) / VX;
+  t.tiles_y = (g.ny + VY - 1) / VY;
+  const long nxy = (long)t.tiles_x * t.tiles_y;
+  const long want = (2048 + nxy - 1) / nxy;
+  int kc = (int)((g.nz + want - 1) / want);
+  if (kc < 16) kc = 16;
+  if (kc > g.nz) kc = g.nz;
+  t.kchunk = kc;
+  return t;
 }
 
 // Components of tau forming the flux along direction d in a Cartesian block
@@ -300,7 +298,7 @@ __device__ __forceinline__ void vflux(const double *t, double mx, double my, dou
 // 2-cell halos), of H^v along k from a 5-deep register queue; times
 // idx_v,idy_v,idz_v (Cartesian) or ijacob3_v (c3).  Interior points only.
 template <int TX, int TY, bool CURV>
-__global__ __launch_bounds__(TX *TY, 2) void k_vdiv(VdivArgs a, int tiles_x, int tiles_y, int ntiles, int kchunk) {
+__global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__restrict__ tab, int nb, int ntiles) {
   constexpr int NT = TX * TY;
   constexpr int NHX = 4 * TY;
   constexpr int NH = NHX + 4 * TX;
@@ -309,12 +307,13 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(VdivArgs a, int tiles_x, int
   __shared__ double sfx[4][TY][TX + 4];
   __shared__ double sfy[4][TY + 4][TX];
 
+  const int w = xcd_remap(blockIdx.x, ntiles);
+  const auto &e = batch_find(tab, nb, w);
+  const auto &a = e.a;
   const Geom &g = a.g;
-  const int logical = xcd_remap(blockIdx.x, ntiles);
-  const int tile_x = logical % tiles_x;
-  const int rest = logical / tiles_x;
-  const int tile_y = rest % tiles_y;
-  const int chunk = rest / tiles_y;
+  int tile_x, tile_y, chunk;
+  tile_coords(e.t, w, tile_x, tile_y, chunk);
+  const int kchunk = e.t.kchunk;
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX, tid = threadIdx.x;
   const int i0 = tile_x * TX + 1, j0 = tile_y * TY + 1;
   const int i = i0 + tx, j = j0 + ty;
@@ -434,25 +433,23 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(VdivArgs a, int tiles_x, int
   }
 }
 
-template <int TX, int TY>
-hipError_t launch_vdiv_t(const VdivArgs &a, bool curv, hipStream_t s) {
-  const int tiles_x = (a.g.nx + TX - 1) / TX, tiles_y = (a.g.ny + TY - 1) / TY;
-  const long nxy = (long)tiles_x * tiles_y;
-  long want = (2048 + nxy - 1) / nxy;
-  int kchunk = (int)((a.g.nz + want - 1) / want);
-  if (kchunk < 16) kchunk = 16;
-  if (kchunk > a.g.nz) kchunk = a.g.nz;
-  const int nt = tiles_x * tiles_y * ((a.g.nz + kchunk - 1) / kchunk);
-  if (curv)
-    hipLaunchKernelGGL((k_vdiv<TX, TY, true>), dim3(nt), dim3(TX * TY), 0, s, a, tiles_x, tiles_y, nt, kchunk);
-  else
-    hipLaunchKernelGGL((k_vdiv<TX, TY, false>), dim3(nt), dim3(TX * TY), 0, s, a, tiles_x, tiles_y, nt, kchunk);
+}  // namespace
+
+TileRange visc_tiles(const Geom &g) { return vtiles(g); }
+TileRange vdiv_tiles(const Geom &g) { return vtiles(g); }
+
+hipError_t launch_visc(const Batch<ViscArgs> *tab, int nb, int nt, bool curv, hipStream_t s) {
+  if (nt <= 0) return hipSuccess;
+  if (curv) hipLaunchKernelGGL((k_visc<VX, VY, true>), dim3(nt), dim3(VX * VY), 0, s, tab, nb, nt);
+  else hipLaunchKernelGGL((k_visc<VX, VY, false>), dim3(nt), dim3(VX * VY), 0, s, tab, nb, nt);
   return hipGetLastError();
 }
 
-}  // namespace
-
-hipError_t launch_visc(const ViscArgs &a, bool curv, hipStream_t s) { return launch_t<32, 16>(a, curv, s); }
-hipError_t launch_vdiv(const VdivArgs &a, bool curv, hipStream_t s) { return launch_vdiv_t<32, 16>(a, curv, s); }
+hipError_t launch_vdiv(const Batch<VdivArgs> *tab, int nb, int nt, bool curv, hipStream_t s) {
+  if (nt <= 0) return hipSuccess;
+  if (curv) hipLaunchKernelGGL((k_vdiv<VX, VY, true>), dim3(nt), dim3(VX * VY), 0, s, tab, nb, nt);
+  else hipLaunchKernelGGL((k_vdiv<VX, VY, false>), dim3(nt), dim3(VX * VY), 0, s, tab, nb, nt);
+  return hipGetLastError();
+}
 
 }  // namespace mgpu

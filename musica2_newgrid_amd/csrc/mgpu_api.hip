@@ -31,10 +31,6 @@ struct BlockData {
   double *prs = nullptr, *sens = nullptr;
   bool have_metrics = false;
   bool have_state = false;
-  // stage kernel launch parameters: [0] Euler stages, [1] last stage with the
-  // viscous divergence (chosen by mgpu_autotune, else a default)
-  int shape[2] = {2, 2};
-  int kchunk[2] = {0, 0};
 };
 
 struct ProfRec {
@@ -64,6 +60,22 @@ struct mgpu_handle {
   int nchunks_s = 0, nchunks_v = 0;
   std::vector<void *> dev_tables;
   bool ghosts_dirty = false;
+  // stage kernel launch parameters: [0] Euler stages, [1] last stage with the
+  // viscous divergence (chosen by mgpu_autotune, else a default); kchunk 0
+  // means the per-block default
+  int shape[2] = {2, 2};
+  int kchunk[2] = {0, 0};
+  // batched launch tables in device memory, one entry per block
+  mgpu::Batch<mgpu::StageArgs> *d_stage = nullptr;  // [cur][irk-1][block]
+  mgpu::Batch<mgpu::ViscArgs> *d_visc = nullptr;    // [cur][block]
+  mgpu::Batch<mgpu::VdivArgs> *d_vdiv = nullptr;    // [block]
+  mgpu::Batch<mgpu::FilterArgs> *d_filt = nullptr;  // [cur][block]
+  mgpu::Batch<mgpu::StageArgs> *d_tune = nullptr;   // [block], autotuner
+  int nt_stage[2] = {0, 0}, nt_visc = 0, nt_vdiv = 0, nt_filt = 0;
+  bool tables_ready = false;
+  // one captured step per U^n set (a case without filter rotates the sets)
+  bool use_graph = true;
+  hipGraphExec_t gexec[3] = {};
 };
 
 namespace {
@@ -401,9 +413,10 @@ int kchunk_for(const Geom &g, int shape) {
 // Process-wide cache of tuned stage parameters, keyed by block geometry and
 // kernel variant, so that equal blocks (and later handles) are tuned once.
 struct TuneKey {
-  int dev, nx, ny, nz, curv, visc;
+  int dev, curv, visc;
+  std::vector<int> dims;  // nx,ny,nz of every block of the batch
   bool operator<(const TuneKey &o) const {
-    return std::tie(dev, nx, ny, nz, curv, visc) < std::tie(o.dev, o.nx, o.ny, o.nz, o.curv, o.visc);
+    return std::tie(dev, curv, visc, dims) < std::tie(o.dev, o.curv, o.visc, o.dims);
   }
 };
 std::mutex g_tune_mu;
@@ -433,7 +446,7 @@ int dalloc(mgpu_handle *h, double **p, size_t count) {
   return 0;
 }
 
-StageArgs stage_args(mgpu_handle *h, BlockData &bd, int in, int s0, int out, int irk) {
+StageArgs stage_args(const mgpu_handle *h, const BlockData &bd, int in, int s0, int out, int irk) {
   StageArgs sa{};
   sa.g = bd.g;
   sa.gas = gas_of(h->c);
@@ -448,11 +461,188 @@ StageArgs stage_args(mgpu_handle *h, BlockData &bd, int in, int s0, int out, int
   return sa;
 }
 
-hipError_t launch_block_stage(mgpu_handle *h, BlockData &bd, int in, int s0, int out, int irk, bool visc_last) {
-  StageArgs sa = stage_args(h, bd, in, s0, out, irk);
-  const int v = visc_last ? 1 : 0;
-  sa.kchunk = bd.kchunk[v] > 0 ? bd.kchunk[v] : kchunk_for(bd.g, bd.shape[v]);
-  return launch_stage(sa, h->c.curvilinear != 0, visc_last, h->stream, bd.shape[v]);
+// Stage tiles of every block for variant v, numbered consecutively; returns
+// the launch's tile count.
+int stage_ranges(const mgpu_handle *h, int v, std::vector<TileRange> &tr) {
+  tr.clear();
+  int first = 0;
+  for (const auto &bd : h->b) {
+    const int kc = h->kchunk[v] > 0 ? h->kchunk[v] : kchunk_for(bd.g, h->shape[v]);
+    TileRange t = stage_tiles(bd.g, h->shape[v], kc);
+    t.first = first;
+    first += tile_count(t, bd.g.nz);
+    tr.push_back(t);
+  }
+  return first;
+}
+
+template <class T>
+int dalloc_table(mgpu_handle *h, T **p, size_t count) {
+  if (*p) return 0;
+  void *q = nullptr;
+  HIPCK(h, hipMalloc(&q, count * sizeof(T)));
+  h->dev_tables.push_back(q);
+  *p = (T *)q;
+  return 0;
+}
+
+void drop_graphs(mgpu_handle *h) {
+  for (auto &g : h->gexec) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+}
+
+// Fills the batched launch tables (after create, and after the autotuner
+// changed the tiles).  The stream is idle when they are rewritten.
+int build_tables(mgpu_handle *h) {
+  if (h->tables_ready) return 0;
+  const int nb = (int)h->b.size(), nrk = h->c.nrk;
+  const mgpu_case &c = h->c;
+  int rc;
+  if ((rc = dalloc_table(h, &h->d_stage, (size_t)3 * nrk * nb)) || (rc = dalloc_table(h, &h->d_visc, 3 * nb)) ||
+      (rc = dalloc_table(h, &h->d_vdiv, nb)) || (rc = dalloc_table(h, &h->d_filt, 3 * nb)) ||
+      (rc = dalloc_table(h, &h->d_tune, nb)))
+    return rc;
+  HIPCK(h, hipStreamSynchronize(h->stream));
+  drop_graphs(h);
+  std::vector<TileRange> tr[2];
+  h->nt_stage[0] = stage_ranges(h, 0, tr[0]);
+  h->nt_stage[1] = stage_ranges(h, 1, tr[1]);
+  std::vector<Batch<StageArgs>> st((size_t)3 * nrk * nb);
+  std::vector<Batch<ViscArgs>> vi(3 * nb);
+  std::vector<Batch<VdivArgs>> vd(nb);
+  std::vector<Batch<FilterArgs>> fi(3 * nb);
+  const bool visc = c.iorder_visc != 0;
+  const double d11[6] = {D11_0, D11_1, D11_2, D11_3, D11_4, D11_5};
+  int fv = 0, fd = 0, ff = 0;
+  for (int b = 0; b < nb; b++) {
+    const BlockData &bd = h->b[b];
+    TileRange tv = visc_tiles(bd.g), td = vdiv_tiles(bd.g), tf = filter_tiles(bd.g);
+    tv.first = fv;
+    td.first = fd;
+    tf.first = ff;
+    fv += tile_count(tv, bd.g.nz);
+    fd += tile_count(td, bd.g.nz);
+    ff += tile_count(tf, bd.g.nz);
+    for (int s0 = 0; s0 < 3; s0++) {
+      const int sa = (s0 + 1) % 3, sb = (s0 + 2) % 3;
+      int in = s0;
+      for (int irk = 1; irk <= nrk; irk++) {
+        const int out = (irk % 2 == 1) ? sa : sb;
+        const bool vl = visc && irk == nrk;
+        Batch<StageArgs> &e = st[((size_t)s0 * nrk + (irk - 1)) * nb + b];
+        e.a = stage_args(h, bd, in, s0, out, irk);
+        e.t = tr[vl ? 1 : 0][b];
+        in = out;
+      }
+      ViscArgs &va = vi[s0 * nb + b].a;
+      va.g = bd.g;
+      va.gas = gas_of(c);
+      for (int v = 0; v < 5; v++) va.u0.v[v] = bd.st[s0][v];
+      va.met = metrics_of(bd);
+      for (int q = 0; q < NTAU; q++) va.tau[q] = bd.tau[q];
+      va.prs = bd.prs;
+      va.sens = bd.sens;
+      va.want_sensor = c.filter == MGPU_FILTER_SHOCK;
+      va.ducros = c.is_ducros;
+      vi[s0 * nb + b].t = tv;
+      FilterArgs &fa = fi[s0 * nb + b].a;
+      fa.g = bd.g;
+      for (int v = 0; v < 5; v++) {
+        fa.uin.v[v] = bd.st[in][v];  // output of the last stage
+        fa.uout.v[v] = bd.st[s0][v];
+      }
+      for (int l = 0; l < 6; l++) fa.dd[l] = -d11[l] * c.dissip_coeff;
+      fa.prs = bd.prs;
+      fa.sens = c.is_ducros ? bd.sens : nullptr;
+      fa.xnu0 = -c.dissip_shock;
+      fa.xnu10 = c.dissip_coeff / 1260.0;
+      fa.csens = c.csens;
+      fi[s0 * nb + b].t = tf;
+    }
+    VdivArgs &va = vd[b].a;
+    va.g = bd.g;
+    va.met = metrics_of(bd);
+    for (int q = 0; q < NTAU; q++) va.tau[q] = bd.tau[q];
+    for (int q = 0; q < 4; q++) va.kv[q] = bd.kv[q];
+    vd[b].t = td;
+  }
+  h->nt_visc = fv;
+  h->nt_vdiv = fd;
+  h->nt_filt = ff;
+  HIPCK(h, hipMemcpy(h->d_stage, st.data(), st.size() * sizeof(st[0]), hipMemcpyHostToDevice));
+  HIPCK(h, hipMemcpy(h->d_visc, vi.data(), vi.size() * sizeof(vi[0]), hipMemcpyHostToDevice));
+  HIPCK(h, hipMemcpy(h->d_vdiv, vd.data(), vd.size() * sizeof(vd[0]), hipMemcpyHostToDevice));
+  HIPCK(h, hipMemcpy(h->d_filt, fi.data(), fi.size() * sizeof(fi[0]), hipMemcpyHostToDevice));
+  h->tables_ready = true;
+  return 0;
+}
+
+// Enqueues one step with U^n in set s0 (no host synchronisation: the same
+// sequence is captured into a graph).  runge_kutta, runge_kutta.f90:2-152.
+int enqueue_step(mgpu_handle *h, int s0) {
+  const mgpu_case &c = h->c;
+  const bool curv = c.curvilinear != 0, visc = c.iorder_visc != 0;
+  const int nb = (int)h->b.size(), nrk = c.nrk;
+  const int sa = (s0 + 1) % 3, sb = (s0 + 2) % 3;
+  int rc, in = s0;
+  for (int irk = 1; irk <= nrk; irk++) {
+    const bool last = irk == nrk;
+    const int out = (irk % 2 == 1) ? sa : sb;
+    if (last && visc) {
+      ProfScope ps(h, MGPU_K_VISC);
+      HIPCK(h, launch_visc(h->d_visc + s0 * nb, nb, h->nt_visc, curv, h->stream));
+      HIPCK(h, launch_iface(h->xv, h->nchunks_v, h->stream));
+      HIPCK(h, launch_vdiv(h->d_vdiv, nb, h->nt_vdiv, curv, h->stream));
+    }
+    {
+      ProfScope ps(h, MGPU_K_STAGE);
+      const int v = (last && visc) ? 1 : 0;
+      HIPCK(h, launch_stage(h->d_stage + ((size_t)s0 * nrk + (irk - 1)) * nb, nb, h->nt_stage[v], curv,
+                            last && visc, h->shape[v], h->stream));
+    }
+    if ((rc = exchange_state(h, out))) return rc;
+    in = out;
+  }
+  (void)in;
+  if (c.filter != MGPU_FILTER_NONE) {
+    {
+      ProfScope ps(h, MGPU_K_FILTER);
+      if (c.filter == MGPU_FILTER_SF) HIPCK(h, launch_filter_sf(h->d_filt + s0 * nb, nb, h->nt_filt, h->stream));
+      else HIPCK(h, launch_filter_shock(h->d_filt + s0 * nb, nb, h->nt_filt, h->stream));
+    }
+    if ((rc = exchange_state(h, s0))) return rc;
+  }
+  return 0;
+}
+
+// Set holding U^{n+1} after a step from set s0.
+int next_set(const mgpu_case &c, int s0) {
+  if (c.filter != MGPU_FILTER_NONE) return s0;
+  return (c.nrk % 2 == 1) ? (s0 + 1) % 3 : (s0 + 2) % 3;
+}
+
+// One step through the captured graph of set h->cur (captured on first use).
+int graph_step(mgpu_handle *h) {
+  const int s0 = h->cur;
+  if (!h->gexec[s0]) {
+    hipGraph_t g = nullptr;
+    HIPCK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_step(h, s0);
+    const hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (rc) {
+      if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    HIPCK(h, e);
+    const hipError_t ei = hipGraphInstantiate(&h->gexec[s0], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) h->gexec[s0] = nullptr;
+    HIPCK(h, ei);
+  }
+  HIPCK(h, hipGraphLaunch(h->gexec[s0], h->stream));
+  return 0;
 }
 
 // Exchange the ghosts of U^n after mgpu_put_state, once every block that
@@ -498,6 +688,7 @@ int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
 
   mgpu_handle *h = new mgpu_handle();
   h->c = *c;
+  h->use_graph = (c->launch_flags & MGPU_LAUNCH_EAGER) == 0;
   h->blocks.assign(c->blocks, c->blocks + c->nblocks);
   h->c.blocks = h->blocks.data();
   if (hipGetDevice(&h->device) != hipSuccess) {
@@ -632,75 +823,14 @@ int mgpu_step(mgpu_handle *h, int nsteps) {
     if (!h->b[bI].have_metrics) return fail(h, MGPU_ESTATE, "block %zu: metrics not set", bI);
     if (!h->b[bI].have_state) return fail(h, MGPU_ESTATE, "block %zu: state not set", bI);
   }
-  const mgpu_case &c = h->c;
-  const bool curv = c.curvilinear != 0, visc = c.iorder_visc != 0;
-  const Gas gs = gas_of(c);
-  int rc = refresh_ghosts(h);
+  int rc = build_tables(h);
+  if (!rc) rc = refresh_ghosts(h);
   if (rc) return rc;
-  const double d11[6] = {D11_0, D11_1, D11_2, D11_3, D11_4, D11_5};
+  const bool graph = h->use_graph && !h->profiling;
   for (int step = 0; step < nsteps; step++) {
-    const int s0 = h->cur, sa = (h->cur + 1) % 3, sb = (h->cur + 2) % 3;
-    int in = s0;
-    for (int irk = 1; irk <= c.nrk; irk++) {
-      const bool last = irk == c.nrk;
-      const int out = (irk % 2 == 1) ? sa : sb;
-      if (last && visc) {
-        ProfScope ps(h, MGPU_K_VISC);
-        for (auto &bd : h->b) {
-          ViscArgs va{};
-          va.g = bd.g;
-          va.gas = gs;
-          for (int v = 0; v < 5; v++) va.u0.v[v] = bd.st[s0][v];
-          va.met = metrics_of(bd);
-          for (int q = 0; q < NTAU; q++) va.tau[q] = bd.tau[q];
-          va.prs = bd.prs;
-          va.sens = bd.sens;
-          va.want_sensor = c.filter == MGPU_FILTER_SHOCK;
-          va.ducros = c.is_ducros;
-          HIPCK(h, launch_visc(va, curv, h->stream));
-        }
-        HIPCK(h, launch_iface(h->xv, h->nchunks_v, h->stream));
-        for (auto &bd : h->b) {
-          VdivArgs vd{};
-          vd.g = bd.g;
-          vd.met = metrics_of(bd);
-          for (int q = 0; q < NTAU; q++) vd.tau[q] = bd.tau[q];
-          for (int q = 0; q < 4; q++) vd.kv[q] = bd.kv[q];
-          HIPCK(h, launch_vdiv(vd, curv, h->stream));
-        }
-      }
-      {
-        ProfScope ps(h, MGPU_K_STAGE);
-        for (auto &bd : h->b) HIPCK(h, launch_block_stage(h, bd, in, s0, out, irk, last && visc));
-      }
-      if ((rc = exchange_state(h, out))) return rc;
-      in = out;
-    }
-    if (c.filter != MGPU_FILTER_NONE) {
-      {
-        ProfScope ps(h, MGPU_K_FILTER);
-        for (auto &bd : h->b) {
-          FilterArgs fa{};
-          fa.g = bd.g;
-          for (int v = 0; v < 5; v++) {
-            fa.uin.v[v] = bd.st[in][v];
-            fa.uout.v[v] = bd.st[s0][v];
-          }
-          for (int l = 0; l < 6; l++) fa.dd[l] = -d11[l] * c.dissip_coeff;
-          fa.prs = bd.prs;
-          fa.sens = c.is_ducros ? bd.sens : nullptr;
-          fa.xnu0 = -c.dissip_shock;
-          fa.xnu10 = c.dissip_coeff / 1260.0;
-          fa.csens = c.csens;
-          if (c.filter == MGPU_FILTER_SF) HIPCK(h, launch_filter_sf(fa, h->stream));
-          else HIPCK(h, launch_filter_shock(fa, h->stream));
-        }
-      }
-      if ((rc = exchange_state(h, s0))) return rc;
-    } else {
-      // U^{n+1} is the last stage output: make that set current
-      h->cur = (c.nrk % 2 == 1) ? sa : sb;
-    }
+    rc = graph ? graph_step(h) : enqueue_step(h, h->cur);
+    if (rc) return rc;
+    h->cur = next_set(h->c, h->cur);
   }
   return 0;
 }
@@ -712,62 +842,74 @@ int mgpu_autotune(mgpu_handle *h, int flags) {
     if (!h->b[bI].have_state) return fail(h, MGPU_ESTATE, "block %zu: state not set", bI);
   }
   const bool force = (flags & 1) != 0;
-  if (int rc0 = refresh_ghosts(h)) return rc0;
+  int rc = build_tables(h);
+  if (!rc) rc = refresh_ghosts(h);
+  if (rc) return rc;
+  const int nb = (int)h->b.size();
   const int s0 = h->cur, sa = (h->cur + 1) % 3;
+  const bool curv = h->c.curvilinear != 0;
   hipEvent_t e0 = get_event(h), e1 = get_event(h);
   if (!e0 || !e1) return fail(h, MGPU_EDEVICE, "hipEventCreate failed");
-  int rc = 0;
-  for (auto &bd : h->b) {
-    for (int v = 0; v < (h->c.iorder_visc ? 2 : 1) && !rc; v++) {
-      const TuneKey key{h->device, bd.g.nx, bd.g.ny, bd.g.nz, h->c.curvilinear, v};
-      if (!force) {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        auto it = g_tune_cache.find(key);
-        if (it != g_tune_cache.end()) {
-          bd.shape[v] = it->second.first;
-          bd.kchunk[v] = it->second.second;
-          continue;
-        }
-      }
-      // candidates: every tile shape x k-chunks from 16 planes to the whole block
-      float best = 1e30f;
-      int best_shape = bd.shape[v], best_kc = bd.kchunk[v];
-      const int kcs[] = {16, 32, 64, 128, 256};
-      for (int sh = 0; sh < kNumStageShapes && !rc; sh++) {
-        int last_kc = -1;
-        for (int kc0 : kcs) {
-          const int kc = kc0 < bd.g.nz ? kc0 : bd.g.nz;
-          if (kc == last_kc) break;
-          last_kc = kc;
-          bd.shape[v] = sh;
-          bd.kchunk[v] = kc;
-          // one warm-up launch, then three timed ones (U^n in, scratch set out)
-          hipError_t e = launch_block_stage(h, bd, s0, s0, sa, 2, v == 1);
-          if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
-          for (int r = 0; r < 3 && e == hipSuccess; r++) e = launch_block_stage(h, bd, s0, s0, sa, 2, v == 1);
-          if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
-          if (e == hipSuccess) e = hipEventSynchronize(e1);
-          float ms = 0.f;
-          if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-          if (e != hipSuccess) {
-            rc = fail(h, MGPU_EDEVICE, "autotune launch failed: %s", hipGetErrorString(e));
-            break;
-          }
-          if (ms < best) {
-            best = ms;
-            best_shape = sh;
-            best_kc = kc;
-          }
-        }
-      }
-      bd.shape[v] = best_shape;
-      bd.kchunk[v] = best_kc;
+  for (int v = 0; v < (h->c.iorder_visc ? 2 : 1) && !rc; v++) {
+    TuneKey key{h->device, h->c.curvilinear, v, {}};
+    for (const auto &bd : h->b) key.dims.insert(key.dims.end(), {bd.g.nx, bd.g.ny, bd.g.nz});
+    if (!force) {
       std::lock_guard<std::mutex> lk(g_tune_mu);
-      g_tune_cache[key] = {best_shape, best_kc};
+      auto it = g_tune_cache.find(key);
+      if (it != g_tune_cache.end()) {
+        h->shape[v] = it->second.first;
+        h->kchunk[v] = it->second.second;
+        continue;
+      }
     }
+    // candidates: every tile shape x k-chunks from 16 planes to the whole
+    // block (0 = the per-block default), timed on one batched launch of
+    // the stage (U^n in, a stage buffer out)
+    float best = 1e30f;
+    int best_shape = h->shape[v], best_kc = h->kchunk[v];
+    const int kcs[] = {0, 16, 32, 64, 128, 256};
+    int maxnz = 0;
+    for (const auto &bd : h->b) maxnz = bd.g.nz > maxnz ? bd.g.nz : maxnz;
+    std::vector<Batch<StageArgs>> tab(nb);
+    for (int sh = 0; sh < kNumStageShapes && !rc; sh++) {
+      for (int kc : kcs) {
+        if (kc > 16 && kc / 2 >= maxnz) break;  // same as the previous candidate for every block
+        h->shape[v] = sh;
+        h->kchunk[v] = kc;
+        std::vector<TileRange> tr;
+        const int nt = stage_ranges(h, v, tr);
+        for (int b = 0; b < nb; b++) {
+          tab[b].a = stage_args(h, h->b[b], s0, s0, sa, v == 1 ? h->c.nrk : 1);
+          tab[b].t = tr[b];
+        }
+        hipError_t e = hipMemcpy(h->d_tune, tab.data(), nb * sizeof(tab[0]), hipMemcpyHostToDevice);
+        // one warm-up launch, then three timed ones
+        if (e == hipSuccess) e = launch_stage(h->d_tune, nb, nt, curv, v == 1, sh, h->stream);
+        if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
+        for (int r = 0; r < 3 && e == hipSuccess; r++) e = launch_stage(h->d_tune, nb, nt, curv, v == 1, sh, h->stream);
+        if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+          rc = fail(h, MGPU_EDEVICE, "autotune launch failed: %s", hipGetErrorString(e));
+          break;
+        }
+        if (ms < best) {
+          best = ms;
+          best_shape = sh;
+          best_kc = kc;
+        }
+      }
+    }
+    h->shape[v] = best_shape;
+    h->kchunk[v] = best_kc;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = {best_shape, best_kc};
   }
   h->ev_pool.push_back(e0);
   h->ev_pool.push_back(e1);
+  h->tables_ready = false;  // rebuilt with the chosen tiles before the next step
   return rc;
 }
 
@@ -775,10 +917,13 @@ int mgpu_stage_config(mgpu_handle *h, int block, int variant, int *shape_tx, int
   if (!h) return MGPU_EINVAL;
   if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
   if (variant < 0 || variant > 1) return fail(h, MGPU_EINVAL, "variant %d", variant);
-  BlockData &bd = h->b[block];
-  if (shape_tx) *shape_tx = kStageShapes[bd.shape[variant]][0];
-  if (shape_ty) *shape_ty = kStageShapes[bd.shape[variant]][1];
-  if (kchunk) *kchunk = bd.kchunk[variant] > 0 ? bd.kchunk[variant] : kchunk_for(bd.g, bd.shape[variant]);
+  const BlockData &bd = h->b[block];
+  if (shape_tx) *shape_tx = kStageShapes[h->shape[variant]][0];
+  if (shape_ty) *shape_ty = kStageShapes[h->shape[variant]][1];
+  if (kchunk) {
+    const int kc = h->kchunk[variant] > 0 ? h->kchunk[variant] : kchunk_for(bd.g, h->shape[variant]);
+    *kchunk = stage_tiles(bd.g, h->shape[variant], kc).kchunk;
+  }
   return 0;
 }
 
@@ -823,6 +968,7 @@ void mgpu_destroy(mgpu_handle *h) {
     (void)hipEventDestroy(r.b);
   }
   for (auto e : h->ev_pool) (void)hipEventDestroy(e);
+  drop_graphs(h);
   for (auto &bd : h->b) free_block(bd);
   for (void *p : h->dev_tables) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);

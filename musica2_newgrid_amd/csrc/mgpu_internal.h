@@ -63,6 +63,44 @@ __device__ __forceinline__ int xcd_remap(int w, int n) {
   return x * q + (x < r ? x : r) + l;
 }
 
+// Tiles of one block in a batched launch: logical tiles [first, first +
+// tiles_x*tiles_y*ceil(nz/kchunk)) belong to it.  One launch covers every
+// block of a case; each workgroup finds its block by binary search over
+// `first` and reads that block's arguments from a device table.
+struct TileRange {
+  int first, tiles_x, tiles_y, kchunk;
+};
+template <class A>
+struct Batch {
+  A a;
+  TileRange t;
+};
+
+__host__ __device__ inline int tile_count(const TileRange &t, int nz) {
+  return t.tiles_x * t.tiles_y * ((nz + t.kchunk - 1) / t.kchunk);
+}
+
+// Entry of the block that owns logical tile w (entries sorted by first).
+template <class E>
+__device__ __forceinline__ const E &batch_find(const E *__restrict__ tab, int nb, int w) {
+  int lo = 0, hi = nb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].t.first <= w) lo = mid;
+    else hi = mid - 1;
+  }
+  return tab[lo];
+}
+
+// Tile coordinates of logical tile w inside its block.
+__device__ __forceinline__ void tile_coords(const TileRange &t, int w, int &tile_x, int &tile_y, int &chunk) {
+  const int l = w - t.first;
+  tile_x = l % t.tiles_x;
+  const int r = l / t.tiles_x;
+  tile_y = r % t.tiles_y;
+  chunk = r / t.tiles_y;
+}
+
 // Thermodynamics / transport constants of a case.
 struct Gas {
   double gam1, rg, gam;
@@ -96,7 +134,6 @@ struct StageArgs {
   Metrics met;
   const double *kv[4];   // viscous increment of rhou,rhov,rhow,rhoe (last stage)
   double alpha;          // deltat*crk(irk)
-  int kchunk;            // planes per block along k
 };
 
 // Viscous flux tensor of U^n in the physical frame, per interior point:
@@ -162,14 +199,20 @@ struct IfaceArgs {
 
 }  // namespace mgpu
 
-// Kernel launchers (defined in the .hip files).
+// Kernel launchers (defined in the .hip files).  Each takes a device table
+// of nb Batch entries (sorted by first tile) and the launch's tile count.
 namespace mgpu {
-hipError_t launch_stage(const StageArgs &a, bool curv, bool visc, hipStream_t s, int shape);
 extern const int kStageShapes[][2];
 extern const int kNumStageShapes;
-hipError_t launch_visc(const ViscArgs &a, bool curv, hipStream_t s);
-hipError_t launch_vdiv(const VdivArgs &a, bool curv, hipStream_t s);
-hipError_t launch_filter_sf(const FilterArgs &a, hipStream_t s);
-hipError_t launch_filter_shock(const FilterArgs &a, hipStream_t s);
+TileRange stage_tiles(const Geom &g, int shape, int kchunk);
+hipError_t launch_stage(const Batch<StageArgs> *tab, int nb, int ntiles, bool curv, bool visc, int shape,
+                        hipStream_t s);
+TileRange visc_tiles(const Geom &g);
+hipError_t launch_visc(const Batch<ViscArgs> *tab, int nb, int ntiles, bool curv, hipStream_t s);
+TileRange vdiv_tiles(const Geom &g);
+hipError_t launch_vdiv(const Batch<VdivArgs> *tab, int nb, int ntiles, bool curv, hipStream_t s);
+TileRange filter_tiles(const Geom &g);
+hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int ntiles, hipStream_t s);
+hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int ntiles, hipStream_t s);
 hipError_t launch_iface(const IfaceArgs &a, int nchunks, hipStream_t s);
 }  // namespace mgpu

@@ -16,6 +16,7 @@ module mod_musica_gpu
   integer(c_int32_t), parameter :: MGPU_FACE_PERIODIC = 0, MGPU_FACE_INTERFACE = 1
   integer(c_int32_t), parameter :: MGPU_FILTER_NONE = 0, MGPU_FILTER_SF = 1, MGPU_FILTER_SHOCK = 2
   integer(c_int32_t), parameter :: MGPU_PREC_FP64 = 0, MGPU_PREC_MIXED = 1
+  integer(c_int32_t), parameter :: MGPU_LAUNCH_EAGER = 1
   integer, parameter :: MGPU_NGH = 5
 
   type, bind(C) :: mgpu_block_t
@@ -24,7 +25,7 @@ module mod_musica_gpu
   end type mgpu_block_t
 
   type, bind(C) :: mgpu_case_t
-     integer(c_int32_t) :: nblocks, curvilinear, iorder_visc, filter, is_ducros, nrk, precision, reserved
+     integer(c_int32_t) :: nblocks, curvilinear, iorder_visc, filter, is_ducros, nrk, precision, launch_flags
      type(c_ptr) :: blocks
      real(c_double) :: crk(8), deltat, gam, rg, c0, S0, diffscale, cok_ratio
      real(c_double) :: dissip_coeff, dissip_shock, csens

@@ -89,7 +89,7 @@ program musica_gpu_tgv
   cs%is_ducros = merge(1, 0, prm%is_ducros)
   cs%nrk = prm%nrk
   cs%precision = MGPU_PREC_FP64
-  cs%reserved = 0
+  cs%launch_flags = 0
   cs%blocks = c_loc(blk(1))
   cs%crk = 0.0d0
   if (prm%nrk /= 4) then

@@ -16,7 +16,7 @@ from . import _lib
 from ._lib import MgpuBlock, MgpuCase, check, dptr, lib
 
 
-def make_case_struct(case, precision: int = _lib.PREC_FP64, topology=None):
+def make_case_struct(case, precision: int = _lib.PREC_FP64, topology=None, launch_flags: int = 0):
     """C-ABI case description.  `topology` is a block table (see
     multiblock.py); None means one block with periodic faces."""
     if topology is None:
@@ -40,6 +40,7 @@ def make_case_struct(case, precision: int = _lib.PREC_FP64, topology=None):
     cs.is_ducros = int(case.ducros)
     cs.nrk = int(case.nrk)
     cs.precision = int(precision)
+    cs.launch_flags = int(launch_flags)
     cs.blocks = C.cast(blocks, C.POINTER(MgpuBlock))
     for q, v in enumerate(case.crk):
         cs.crk[q] = v
@@ -61,13 +62,15 @@ class GpuSolver:
     topology=blocks, block_cases=cases)``, where `case` carries the flags and
     block_cases[b] the extents, metrics and initial state of block b."""
 
-    def __init__(self, case, precision: int = _lib.PREC_FP64, topology=None, block_cases=None):
+    def __init__(self, case, precision: int = _lib.PREC_FP64, topology=None, block_cases=None,
+                 eager: bool = False):
         L = lib()
         self.case = case
         self.block_cases = [case] if block_cases is None else list(block_cases)
         if topology is not None and len(topology) != len(self.block_cases):
             raise ValueError("topology and block_cases differ in length")
-        self._cs, self._blocks = make_case_struct(case, precision, topology)
+        self._cs, self._blocks = make_case_struct(case, precision, topology,
+                                                  _lib.LAUNCH_EAGER if eager else 0)
         h = C.c_void_p()
         rc = L.mgpu_create(C.byref(self._cs), C.byref(h))
         check(rc, None, "mgpu_create")
