@@ -28,10 +28,14 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 def stage_words_per_cell(case) -> float:
     """Algorithmic FP64 words per interior cell of one stage-kernel launch,
     averaged over the nrk stages: stage 1 reads U^n and writes U1 (10 W),
-    later stages also read U^n (15 W); curvilinear metrics add 10 W per stage."""
+    later stages also read U^n (15 W); curvilinear metrics add 10 W per stage
+    (9 metric terms and 1/J); the viscous last stage reads the 4 viscous
+    increments (4 W)."""
     w = [10.0] + [15.0] * (case.nrk - 1)
     if case.curvilinear:
         w = [x + 10.0 for x in w]
+    if case.iorder_visc:
+        w[-1] += 4.0
     return sum(w) / len(w)
 
 
@@ -52,12 +56,40 @@ def cpu_baseline(case, nsteps: int):
                       f"OpenMP over k, {t:.2f} s"}
 
 
+def run_leg(case, precision: int, steps: int, warmup: int, tune: bool):
+    """Advance `case` on the GPU: autotune, warm up, then time `steps` steps
+    with per-kernel HIP events on the library's stream."""
+    from musica2_newgrid_amd import _lib
+    from musica2_newgrid_amd.solver import GpuSolver
+
+    solver = GpuSolver(case, precision=precision)
+    if tune:
+        solver.autotune()
+    tuned = {"euler_stages": solver.stage_config(0), "last_stage": solver.stage_config(1)}
+    solver.step(warmup)
+    solver.sync()
+
+    solver.profile(True)
+    solver.sync()
+    t0 = time.perf_counter()
+    solver.step(steps)
+    solver.sync()
+    t = time.perf_counter() - t0
+
+    times = {_lib.K_NAMES[k]: solver.kernel_time(k) for k in _lib.K_NAMES}
+    solver.profile(False)
+    rho = solver.get_state()[0]
+    solver.close()
+    return {"t": t, "times": times, "tuned": tuned, "rho": rho}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--nx", type=int, default=512)
@@ -68,28 +100,15 @@ def main() -> int:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         return 2
 
+    import numpy as np
+
     from musica2_newgrid_amd import _lib, cases
-    from musica2_newgrid_amd.solver import GpuSolver
 
     case = cases.vortex_wavy(args.nx, args.ny, args.nz, filter=2)
-    solver = GpuSolver(case)
-    if not args.no_tune:
-        solver.autotune()
-    tuned = {"euler_stages": solver.stage_config(0), "last_stage": solver.stage_config(1)}
-    solver.step(args.warmup)
-    solver.sync()
-
-    solver.profile(True)
-    solver.sync()
-    t0 = time.perf_counter()
-    solver.step(args.steps)
-    solver.sync()
-    t = time.perf_counter() - t0
-
-    nst, ms_stage = solver.kernel_time(_lib.K_STAGE)
-    times = {_lib.K_NAMES[k]: solver.kernel_time(k) for k in _lib.K_NAMES}
-    solver.profile(False)
-    solver.close()
+    print(f"bench: {args.nx}x{args.ny}x{args.nz} FP64 leg, {args.steps} steps", file=sys.stderr, flush=True)
+    fp = run_leg(case, _lib.PREC_FP64, args.steps, args.warmup, not args.no_tune)
+    t = fp["t"]
+    nst, ms_stage = fp["times"]["stage"]
 
     value = case.ncells * case.nrk * args.steps / t / 1e6
     avg_stage_s = ms_stage / nst / 1e3
@@ -111,14 +130,30 @@ def main() -> int:
         "config": {"workload": f"curvilinear block {args.nx}x{args.ny}x{args.nz}, isentropic vortex "
                                "M=0.5, viscous, shock filter + Ducros, RK4, FP64",
                    "cells": case.ncells, "nrk": case.nrk, "precision": "fp64",
-                   "stage_tiles": {k: {"tile": f"{v[0]}x{v[1]}", "kchunk": v[2]} for k, v in tuned.items()}},
+                   "stage_tiles": {k: {"tile": f"{v[0]}x{v[1]}", "kchunk": v[2]} for k, v in fp["tuned"].items()}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_stage", "avg_launch_ms": round(avg_stage_s * 1e3, 4),
                      "algorithmic_bytes_per_launch": bytes_launch},
-        "kernel_ms": {k: {"launches": n, "ms": round(ms, 3)} for k, (n, ms) in times.items()},
+        "kernel_ms": {k: {"launches": n, "ms": round(ms, 3)} for k, (n, ms) in fp["times"].items()},
     }
+    if not args.no_mixed:
+        print("bench: mixed-precision leg", file=sys.stderr, flush=True)
+        mx = run_leg(case, _lib.PREC_MIXED, args.steps, args.warmup, not args.no_tune)
+        a = mx["rho"][5:-5, 5:-5, 5:-5]
+        b = fp["rho"][5:-5, 5:-5, 5:-5]
+        mval = case.ncells * case.nrk * args.steps / mx["t"] / 1e6
+        out["mixed"] = {
+            "value": round(mval, 2), "unit": "Mcell-RK-stages/s",
+            "ms_per_step": round(mx["t"] / args.steps * 1e3, 4),
+            "speedup_vs_fp64": round(mval / value, 3),
+            "rho_linf_vs_fp64": float(np.max(np.abs(a - b)) / np.max(np.abs(b))),
+            "rho_l2_vs_fp64": float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2))),
+            "after_steps": args.warmup + args.steps,
+            "kernel_ms": {k: {"launches": n, "ms": round(ms, 3)} for k, (n, ms) in mx["times"].items()},
+        }
     if not args.no_cpu:
+        print("bench: CPU baseline leg", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
     print(json.dumps(out), flush=True)
     return 0

@@ -13,6 +13,8 @@
 // and computes each pressure sensor once per plane in LDS.  Halo cells are
 // statically assigned to lanes so that all global loads of a plane are
 // issued in one batch.
+#include <type_traits>
+
 #include "mgpu_internal.h"
 
 namespace mgpu {
@@ -81,9 +83,26 @@ __device__ __forceinline__ void ld5(const CState &U, idx_t p, double o[5]) {
   for (int v = 0; v < 5; v++) o[v] = ldg(U.v[v], p);
 }
 
+// State at element p as the filter arithmetic sees it: FP64, or (mixed
+// precision) the FP32 difference with the reference state.  The filter
+// stencils sum to zero, so the constant drops out of K.
+template <bool MIX, class R>
+__device__ __forceinline__ void ldr(const CState &U, idx_t p, const RefState &rf, R o[5]) {
+  if constexpr (MIX) {
+    const double ref[5] = {rf.r, rf.ru, rf.rv, rf.rw, rf.re};
+#pragma unroll
+    for (int v = 0; v < 5; v++) o[v] = (R)(ldg(U.v[v], p) - ref[v]);
+  } else {
+#pragma unroll
+    for (int v = 0; v < 5; v++) o[v] = ldg(U.v[v], p);
+  }
+}
+
+template <bool MIX>
 __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__restrict__ tab, int nb, int ntiles) {
-  __shared__ double sux[5][FY][FX + 10];
-  __shared__ double suy[5][FY + 10][FX];
+  using R = typename std::conditional<MIX, float, double>::type;
+  __shared__ R sux[5][FY][FX + 10];
+  __shared__ R suy[5][FY + 10][FX];
   int i0, j0, kbeg, kend;
   const FilterArgs &a = tile_of(tab, nb, ntiles, i0, j0, kbeg, kend);
   const Geom &g = a.g;
@@ -94,25 +113,30 @@ __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__
   const idx_t sz = g.sy, own_off = lin(g, i, j, 0);
   Halo hl;
   halo_setup(g, i0, j0, hl);
-  const double *dd = a.dd;
+  const RefState &rf = a.ref;
+  R dd[6];
+#pragma unroll
+  for (int l = 0; l < 6; l++) dd[l] = (R)a.dd[l];
 
-  double q[11][5];
+  R q[11][5];
   if (own) {
 #pragma unroll
-    for (int l = 0; l < 10; l++) ld5(a.uin, own_off + (idx_t)(kbeg - 5 + l) * sz, q[l]);
+    for (int l = 0; l < 10; l++) ldr<MIX>(a.uin, own_off + (idx_t)(kbeg - 5 + l) * sz, rf, q[l]);
   }
   for (int k = kbeg; k <= kend; k++) {
     const idx_t pk = own_off + (idx_t)k * sz;
-    double uc[5], uh[NS][5];
-    if (own) ld5(a.uin, pk + 5 * sz, q[10]);
-    if (!own && (needx || needy)) ld5(a.uin, pk, uc);
+    R uc[5], uh[NS][5];
+    double u64[5];  // mixed: the FP64 state of the own point, to which K is added
+    if (own) ldr<MIX>(a.uin, pk + 5 * sz, rf, q[10]);
+    if (!own && (needx || needy)) ldr<MIX>(a.uin, pk, rf, uc);
+    if (MIX && own) ld5(a.uin, pk, u64);
 #pragma unroll
     for (int s = 0; s < NS; s++)
-      if (hl.on[s]) ld5(a.uin, hl.off[s] + (idx_t)k * sz, uh[s]);
+      if (hl.on[s]) ldr<MIX>(a.uin, hl.off[s] + (idx_t)k * sz, rf, uh[s]);
     if (needx || needy) {
 #pragma unroll
       for (int v = 0; v < 5; v++) {
-        const double x = own ? q[5][v] : uc[v];
+        const R x = own ? q[5][v] : uc[v];
         if (needx) sux[v][ty][tx + 5] = x;
         if (needy) suy[v][ty + 5][tx] = x;
       }
@@ -131,16 +155,17 @@ __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__
     if (own) {
 #pragma unroll
       for (int v = 0; v < 5; v++) {
-        const double *x = &sux[v][ty][tx + 5];
-        const double *y = &suy[v][ty + 5][tx];
-        double K = dd[0] * x[0] + dd[1] * (x[1] + x[-1]) + dd[2] * (x[2] + x[-2]) + dd[3] * (x[3] + x[-3]) +
+        const R *x = &sux[v][ty][tx + 5];
+        const R *y = &suy[v][ty + 5][tx];
+        R K = dd[0] * x[0] + dd[1] * (x[1] + x[-1]) + dd[2] * (x[2] + x[-2]) + dd[3] * (x[3] + x[-3]) +
                    dd[4] * (x[4] + x[-4]) + dd[5] * (x[5] + x[-5]);
         K = K + dd[0] * y[0] + dd[1] * (y[FX] + y[-FX]) + dd[2] * (y[2 * FX] + y[-2 * FX]) +
             dd[3] * (y[3 * FX] + y[-3 * FX]) + dd[4] * (y[4 * FX] + y[-4 * FX]) +
             dd[5] * (y[5 * FX] + y[-5 * FX]);
         K = K + dd[0] * q[5][v] + dd[1] * (q[6][v] + q[4][v]) + dd[2] * (q[7][v] + q[3][v]) +
             dd[3] * (q[8][v] + q[2][v]) + dd[4] * (q[9][v] + q[1][v]) + dd[5] * (q[10][v] + q[0][v]);
-        stg_nt(a.uout.v[v], pk, q[5][v] + K);
+        if constexpr (MIX) stg_nt(a.uout.v[v], pk, u64[v] + (double)K);
+        else stg_nt(a.uout.v[v], pk, q[5][v] + K);
       }
     }
     __syncthreads();
@@ -159,17 +184,20 @@ __device__ __forceinline__ double psens3(double pm, double p0, double pp, double
 }
 
 // D_{m+1/2} from u at m-4..m+5 given as x[-4..5] with stride s
-__device__ __forceinline__ double dface(const double *x, int s, double xk0, double xk10) {
+template <class R>
+__device__ __forceinline__ R dface(const R *x, int s, R xk0, R xk10) {
   return (x[s] - x[0]) * xk0 +
-         (D10_1 * (x[s] - x[0]) + D10_2 * (x[2 * s] - x[-s]) + D10_3 * (x[3 * s] - x[-2 * s]) +
-          D10_4 * (x[4 * s] - x[-3 * s]) + D10_5 * (x[5 * s] - x[-4 * s])) *
+         (R(D10_1) * (x[s] - x[0]) + R(D10_2) * (x[2 * s] - x[-s]) + R(D10_3) * (x[3 * s] - x[-2 * s]) +
+          R(D10_4) * (x[4 * s] - x[-3 * s]) + R(D10_5) * (x[5 * s] - x[-4 * s])) *
              xk10;
 }
 
+template <bool MIX>
 __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> *__restrict__ tab, int nb,
                                                         int ntiles) {
-  __shared__ double sux[5][FY][FX + 10];
-  __shared__ double suy[5][FY + 10][FX];
+  using R = typename std::conditional<MIX, float, double>::type;
+  __shared__ R sux[5][FY][FX + 10];
+  __shared__ R suy[5][FY + 10][FX];
   __shared__ double spx[FY][FX + 4];    // p at i0-2 .. i0+FX+1
   __shared__ double spy[FY + 4][FX];    // p at j0-2 .. j0+FY+1
   __shared__ double ssx[FY][FX + 4];    // sensor at i0-2 .. i0+FX+1 (ends unused)
@@ -222,12 +250,13 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
     }
   }
 
-  double q[11][5];   // U at k-5..k+5
+  const RefState &rf = a.ref;
+  R q[11][5];        // U at k-5..k+5 (mixed: minus the reference state)
   double qp[5];      // p at k-2..k+2
   double qs[3];      // sensor at k-1..k+1
   if (own) {
 #pragma unroll
-    for (int l = 0; l < 10; l++) ld5(a.uin, own_off + (idx_t)(kbeg - 5 + l) * sz, q[l]);
+    for (int l = 0; l < 10; l++) ldr<MIX>(a.uin, own_off + (idx_t)(kbeg - 5 + l) * sz, rf, q[l]);
 #pragma unroll
     for (int l = 0; l < 4; l++) qp[l] = ldg(a.prs, own_off + (idx_t)(kbeg - 2 + l) * sz);
 #pragma unroll
@@ -237,16 +266,19 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
   for (int k = kbeg; k <= kend; k++) {
     const idx_t pk = own_off + (idx_t)k * sz;
     // ---- every global load of the plane ----
-    double uc[5], uh[NS][5], pv[NSP], sv[NSP];
+    R uc[5], uh[NS][5];
+    double pv[NSP], sv[NSP];
+    double u64[5];  // mixed: the FP64 state of the own point, to which K is added
     if (own) {
-      ld5(a.uin, pk + 5 * sz, q[10]);
+      ldr<MIX>(a.uin, pk + 5 * sz, rf, q[10]);
+      if constexpr (MIX) ld5(a.uin, pk, u64);
       qp[4] = ldg(a.prs, pk + 2 * sz);
       qs[2] = duc ? ldg(a.sens, pk + sz) : 1.0;
     }
-    if (!own && (needx || needy)) ld5(a.uin, pk, uc);
+    if (!own && (needx || needy)) ldr<MIX>(a.uin, pk, rf, uc);
 #pragma unroll
     for (int s = 0; s < NS; s++)
-      if (hl.on[s]) ld5(a.uin, hl.off[s] + (idx_t)k * sz, uh[s]);
+      if (hl.on[s]) ldr<MIX>(a.uin, hl.off[s] + (idx_t)k * sz, rf, uh[s]);
 #pragma unroll
     for (int s = 0; s < NSP; s++) {
       if (pp_on[s]) {
@@ -258,7 +290,7 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
     if (needx || needy) {
 #pragma unroll
       for (int v = 0; v < 5; v++) {
-        const double x = own ? q[5][v] : uc[v];
+        const R x = own ? q[5][v] : uc[v];
         if (needx) sux[v][ty][tx + 5] = x;
         if (needy) suy[v][ty + 5][tx] = x;
       }
@@ -302,7 +334,7 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
     __syncthreads();
 
     if (own) {
-      double K[5];
+      R K[5];
       // i faces i-1/2 and i+1/2
       {
         const double pm = psx[ty][tx], p0 = psx[ty][tx + 1], pp = psx[ty][tx + 2];
@@ -310,8 +342,8 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
         const double xk0p = a.xnu0 * fmax(pp, p0), xk10p = fmax(0.0, a.xnu10 + xk0p / 105.0);
 #pragma unroll
         for (int v = 0; v < 5; v++) {
-          const double *x = &sux[v][ty][tx + 5];
-          K[v] = dface(x - 1, 1, xk0m, xk10m) - dface(x, 1, xk0p, xk10p);
+          const R *x = &sux[v][ty][tx + 5];
+          K[v] = dface<R>(x - 1, 1, (R)xk0m, (R)xk10m) - dface<R>(x, 1, (R)xk0p, (R)xk10p);
         }
       }
       // j faces
@@ -321,8 +353,8 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
         const double xk0p = a.xnu0 * fmax(pp, p0), xk10p = fmax(0.0, a.xnu10 + xk0p / 105.0);
 #pragma unroll
         for (int v = 0; v < 5; v++) {
-          const double *y = &suy[v][ty + 5][tx];
-          K[v] = K[v] + dface(y - FX, FX, xk0m, xk10m) - dface(y, FX, xk0p, xk10p);
+          const R *y = &suy[v][ty + 5][tx];
+          K[v] = K[v] + dface<R>(y - FX, FX, (R)xk0m, (R)xk10m) - dface<R>(y, FX, (R)xk0p, (R)xk10p);
         }
       }
       // k faces from the register queues
@@ -334,19 +366,22 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
         const double xk0p = a.xnu0 * fmax(pp, p0), xk10p = fmax(0.0, a.xnu10 + xk0p / 105.0);
 #pragma unroll
         for (int v = 0; v < 5; v++) {
-          const double dlo = (q[5][v] - q[4][v]) * xk0m +
-                             (D10_1 * (q[5][v] - q[4][v]) + D10_2 * (q[6][v] - q[3][v]) +
-                              D10_3 * (q[7][v] - q[2][v]) + D10_4 * (q[8][v] - q[1][v]) +
-                              D10_5 * (q[9][v] - q[0][v])) * xk10m;
-          const double dhi = (q[6][v] - q[5][v]) * xk0p +
-                             (D10_1 * (q[6][v] - q[5][v]) + D10_2 * (q[7][v] - q[4][v]) +
-                              D10_3 * (q[8][v] - q[3][v]) + D10_4 * (q[9][v] - q[2][v]) +
-                              D10_5 * (q[10][v] - q[1][v])) * xk10p;
+          const R dlo = (q[5][v] - q[4][v]) * (R)xk0m +
+                        (R(D10_1) * (q[5][v] - q[4][v]) + R(D10_2) * (q[6][v] - q[3][v]) +
+                         R(D10_3) * (q[7][v] - q[2][v]) + R(D10_4) * (q[8][v] - q[1][v]) +
+                         R(D10_5) * (q[9][v] - q[0][v])) * (R)xk10m;
+          const R dhi = (q[6][v] - q[5][v]) * (R)xk0p +
+                        (R(D10_1) * (q[6][v] - q[5][v]) + R(D10_2) * (q[7][v] - q[4][v]) +
+                         R(D10_3) * (q[8][v] - q[3][v]) + R(D10_4) * (q[9][v] - q[2][v]) +
+                         R(D10_5) * (q[10][v] - q[1][v])) * (R)xk10p;
           K[v] = K[v] + dlo - dhi;
         }
       }
 #pragma unroll
-      for (int v = 0; v < 5; v++) stg_nt(a.uout.v[v], pk, q[5][v] + K[v]);
+      for (int v = 0; v < 5; v++) {
+        if constexpr (MIX) stg_nt(a.uout.v[v], pk, u64[v] + (double)K[v]);
+        else stg_nt(a.uout.v[v], pk, q[5][v] + K[v]);
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -376,15 +411,17 @@ TileRange filter_tiles(const Geom &g) {
   return t;
 }
 
-hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int nt, hipStream_t s) {
+hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int nt, bool mixed, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_filter_sf, dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  if (mixed) hipLaunchKernelGGL(k_filter_sf<true>, dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  else hipLaunchKernelGGL(k_filter_sf<false>, dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   return hipGetLastError();
 }
 
-hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int nt, hipStream_t s) {
+hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int nt, bool mixed, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_filter_shock, dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  if (mixed) hipLaunchKernelGGL(k_filter_shock<true>, dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  else hipLaunchKernelGGL(k_filter_shock<false>, dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   return hipGetLastError();
 }
 

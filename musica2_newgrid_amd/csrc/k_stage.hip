@@ -19,6 +19,8 @@
 // streamed from HBM once and the k-derivative needs no second pass.  i is the
 // fastest index of every array: a wavefront reads whole rows.  Tiles are
 // numbered XCD-aware so that tiles sharing halos share an L2.
+#include <type_traits>
+
 #include "mgpu_internal.h"
 
 namespace mgpu {
@@ -29,9 +31,10 @@ struct Prim {
   double r, ru, rv, rw, re, u, v, w, pr;
 };
 
-__device__ __forceinline__ double d11c(const double *f, int s) {
-  return A11_1 * (f[s] - f[-s]) + A11_2 * (f[2 * s] - f[-2 * s]) + A11_3 * (f[3 * s] - f[-3 * s]) +
-         A11_4 * (f[4 * s] - f[-4 * s]) + A11_5 * (f[5 * s] - f[-5 * s]);
+template <class R>
+__device__ __forceinline__ R d11c(const R *f, int s) {
+  return R(A11_1) * (f[s] - f[-s]) + R(A11_2) * (f[2 * s] - f[-2 * s]) + R(A11_3) * (f[3 * s] - f[-3 * s]) +
+         R(A11_4) * (f[4 * s] - f[-4 * s]) + R(A11_5) * (f[5 * s] - f[-5 * s]);
 }
 
 // Raw inputs of one flux evaluation: the 5 conserved variables and the 3
@@ -66,6 +69,39 @@ __device__ __forceinline__ Prim prim_raw(const double u[5], double gam1) {
   return q;
 }
 
+// Flux along dir minus the flux of the reference state, in FP32 (mixed
+// precision).  With W = (rho u).m, vc = u.m and primes for differences with
+// the reference state (taken in FP64, then rounded):
+//   F'0 = W',  F'(1+a) = W' u_a + W_r u'_a + p' m_a,  F'4 = H' vc + H_r vc'
+// which equals F(U) - F(U_ref) exactly in real arithmetic.
+template <bool CURV>
+__device__ __forceinline__ void flux_mix(const Prim &q, int dir, const double m[3], const RefState &rf, float F[5]) {
+  const float uu[3] = {(float)q.u, (float)q.v, (float)q.w};
+  const float up[3] = {(float)(q.u - rf.u), (float)(q.v - rf.v), (float)(q.w - rf.w)};
+  const float wp[3] = {(float)(q.ru - rf.ru), (float)(q.rv - rf.rv), (float)(q.rw - rf.rw)};
+  const double pd = q.pr - rf.p;
+  const float pp = (float)pd;
+  const float hp = (float)((q.re - rf.re) + pd);
+  float mm[3];
+  if constexpr (CURV) {
+    mm[0] = (float)m[0];
+    mm[1] = (float)m[1];
+    mm[2] = (float)m[2];
+  } else {
+    mm[0] = dir == 0 ? 1.f : 0.f;
+    mm[1] = dir == 1 ? 1.f : 0.f;
+    mm[2] = dir == 2 ? 1.f : 0.f;
+  }
+  const float W1 = wp[0] * mm[0] + wp[1] * mm[1] + wp[2] * mm[2];
+  const float Wr = (float)rf.ru * mm[0] + (float)rf.rv * mm[1] + (float)rf.rw * mm[2];
+  const float vc = uu[0] * mm[0] + uu[1] * mm[1] + uu[2] * mm[2];
+  const float vcp = up[0] * mm[0] + up[1] * mm[1] + up[2] * mm[2];
+  F[0] = W1;
+#pragma unroll
+  for (int c = 0; c < 3; c++) F[1 + c] = W1 * uu[c] + Wr * up[c] + pp * mm[c];
+  F[4] = hp * vc + (float)rf.H * vcp;
+}
+
 // Flux along dir from primitives and the metric terms of dir (c3).
 template <bool CURV>
 __device__ __forceinline__ void flux_m(const Prim &q, int dir, const double m[3], double F[5]) {
@@ -88,16 +124,24 @@ __device__ __forceinline__ void flux_m(const Prim &q, int dir, const double m[3]
   }
 }
 
-template <int TX, int TY, bool CURV, bool VISC>
+// FP64 flux, or the FP32 flux perturbation of the mixed path.
+template <bool CURV, bool MIX, class R>
+__device__ __forceinline__ void flux_r(const Prim &q, int dir, const double m[3], const RefState &rf, R F[5]) {
+  if constexpr (MIX) flux_mix<CURV>(q, dir, m, rf, F);
+  else flux_m<CURV>(q, dir, m, F);
+}
+
+template <int TX, int TY, bool CURV, bool VISC, bool MIX>
 __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__restrict__ tab, int nb, int ntiles) {
+  using R = typename std::conditional<MIX, float, double>::type;
   constexpr int NT = TX * TY;
   constexpr int NHX = 10 * TY;            // i-halo cells per plane
   constexpr int NH = NHX + 10 * TX;       // all halo cells per plane
   constexpr int NS = (NH + NT - 1) / NT;  // halo cells per thread
   // flux pencils, double-buffered over k planes so that one barrier per
   // plane suffices (plane k+1 is written while plane k may still be read)
-  __shared__ double sfx2[2][5][TY][TX + 10];
-  __shared__ double sfy2[2][5][TY + 10][TX];
+  __shared__ R sfx2[2][5][TY][TX + 10];
+  __shared__ R sfy2[2][5][TY + 10][TX];
 
   const int w = xcd_remap(blockIdx.x, ntiles);
   const Batch<StageArgs> &e = batch_find(tab, nb, w);
@@ -152,20 +196,21 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
   const idx_t own_off = lin(g, i, j, 0);
   const idx_t sz = g.sy;
 
-  double q[11][5];
+  const RefState &rf = a.ref;
+  R q[11][5];
   if (own) {
 #pragma unroll
     for (int l = 0; l < 10; l++) {
       Raw r;
       load_raw<CURV>(a.uin, a.met, own_off + (idx_t)(kbeg - 5 + l) * sz, 2, r);
-      flux_m<CURV>(prim_raw(r.u, gam1), 2, r.m, q[l]);
+      flux_r<CURV, MIX>(prim_raw(r.u, gam1), 2, r.m, rf, q[l]);
     }
   }
 
   for (int k = kbeg; k <= kend; k++) {
     const idx_t pk = own_off + (idx_t)k * sz;
-    double (*sfx)[TY][TX + 10] = sfx2[(k - kbeg) & 1];
-    double (*sfy)[TY + 10][TX] = sfy2[(k - kbeg) & 1];
+    R (*sfx)[TY][TX + 10] = sfx2[(k - kbeg) & 1];
+    R (*sfy)[TY + 10][TX] = sfy2[(k - kbeg) & 1];
     // ---- batch 1: queue point k+5, own point k, U^n and 1/J at k, halo cell 0 ----
     Raw rq, rh0;
     double rxm[3] = {0, 0, 0}, rym[3] = {0, 0, 0}, ru[5] = {0, 0, 0, 0, 0}, u0[5] = {0, 0, 0, 0, 0};
@@ -193,24 +238,24 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
         for (int c = 0; c < 4; c++) kvis[c] = ldg(a.kv[c], pk);
       }
     }
-    if (own) flux_m<CURV>(prim_raw(rq.u, gam1), 2, rq.m, q[10]);
+    if (own) flux_r<CURV, MIX>(prim_raw(rq.u, gam1), 2, rq.m, rf, q[10]);
     if (needx || needy) {
       const Prim pq = prim_raw(ru, gam1);
-      double F[5];
+      R F[5];
       if (needx) {
-        flux_m<CURV>(pq, 0, rxm, F);
+        flux_r<CURV, MIX>(pq, 0, rxm, rf, F);
 #pragma unroll
         for (int v = 0; v < 5; v++) sfx[v][ty][tx + 5] = F[v];
       }
       if (needy) {
-        flux_m<CURV>(pq, 1, rym, F);
+        flux_r<CURV, MIX>(pq, 1, rym, rf, F);
 #pragma unroll
         for (int v = 0; v < 5; v++) sfy[v][ty + 5][tx] = F[v];
       }
     }
     if (hs_on[0]) {
-      double F[5];
-      flux_m<CURV>(prim_raw(rh0.u, gam1), hs_dir[0], rh0.m, F);
+      R F[5];
+      flux_r<CURV, MIX>(prim_raw(rh0.u, gam1), hs_dir[0], rh0.m, rf, F);
       if (hs_dir[0] == 0) {
 #pragma unroll
         for (int v = 0; v < 5; v++) sfx[v][hs_r[0]][hs_c[0]] = F[v];
@@ -229,8 +274,8 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
 #pragma unroll
       for (int s = 1; s < NS; s++) {
         if (hs_on[s]) {
-          double F[5];
-          flux_m<CURV>(prim_raw(rh[s].u, gam1), hs_dir[s], rh[s].m, F);
+          R F[5];
+          flux_r<CURV, MIX>(prim_raw(rh[s].u, gam1), hs_dir[s], rh[s].m, rf, F);
           if (hs_dir[s] == 0) {
 #pragma unroll
             for (int v = 0; v < 5; v++) sfx[v][hs_r[s]][hs_c[s]] = F[v];
@@ -247,17 +292,18 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
       double K[5];
 #pragma unroll
       for (int v = 0; v < 5; v++) {
-        const double dx = d11c(&sfx[v][ty][tx + 5], 1);
-        const double dy = d11c(&sfy[v][ty + 5][tx], TX);
-        const double dz = A11_1 * (q[6][v] - q[4][v]) + A11_2 * (q[7][v] - q[3][v]) +
-                          A11_3 * (q[8][v] - q[2][v]) + A11_4 * (q[9][v] - q[1][v]) +
-                          A11_5 * (q[10][v] - q[0][v]);
+        const R dx = d11c(&sfx[v][ty][tx + 5], 1);
+        const R dy = d11c(&sfy[v][ty + 5][tx], TX);
+        const R dz = R(A11_1) * (q[6][v] - q[4][v]) + R(A11_2) * (q[7][v] - q[3][v]) +
+                     R(A11_3) * (q[8][v] - q[2][v]) + R(A11_4) * (q[9][v] - q[1][v]) +
+                     R(A11_5) * (q[10][v] - q[0][v]);
+        // metrics and Jacobian multiply in FP64
         if constexpr (CURV) {
-          K[v] = (dz + (dy + dx)) * ijac;
+          K[v] = (double)(dz + (dy + dx)) * ijac;
         } else {
-          const double kx = dx * a.met.idx[i - 1 + G];
-          const double ky = dy * a.met.idy[j - 1 + G] + kx;
-          K[v] = dz * a.met.idz[k - 1 + G] + ky;
+          const double kx = (double)dx * a.met.idx[i - 1 + G];
+          const double ky = (double)dy * a.met.idy[j - 1 + G] + kx;
+          K[v] = (double)dz * a.met.idz[k - 1 + G] + ky;
         }
       }
       if constexpr (VISC) {
@@ -275,17 +321,28 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
   }
 }
 
-template <int TX, int TY>
+template <int TX, int TY, bool MIX>
 hipError_t launch_t(const Batch<StageArgs> *tab, int nb, int nt, bool curv, bool visc, hipStream_t s) {
   dim3 blk(TX * TY), grd(nt);
   if (curv) {
-    if (visc) hipLaunchKernelGGL((k_stage<TX, TY, true, true>), grd, blk, 0, s, tab, nb, nt);
-    else hipLaunchKernelGGL((k_stage<TX, TY, true, false>), grd, blk, 0, s, tab, nb, nt);
+    if (visc) hipLaunchKernelGGL((k_stage<TX, TY, true, true, MIX>), grd, blk, 0, s, tab, nb, nt);
+    else hipLaunchKernelGGL((k_stage<TX, TY, true, false, MIX>), grd, blk, 0, s, tab, nb, nt);
   } else {
-    if (visc) hipLaunchKernelGGL((k_stage<TX, TY, false, true>), grd, blk, 0, s, tab, nb, nt);
-    else hipLaunchKernelGGL((k_stage<TX, TY, false, false>), grd, blk, 0, s, tab, nb, nt);
+    if (visc) hipLaunchKernelGGL((k_stage<TX, TY, false, true, MIX>), grd, blk, 0, s, tab, nb, nt);
+    else hipLaunchKernelGGL((k_stage<TX, TY, false, false, MIX>), grd, blk, 0, s, tab, nb, nt);
   }
   return hipGetLastError();
+}
+
+template <bool MIX>
+hipError_t launch_shape(const Batch<StageArgs> *tab, int nb, int nt, bool curv, bool visc, int shape, hipStream_t s) {
+  switch (shape) {
+    case 1: return launch_t<32, 8, MIX>(tab, nb, nt, curv, visc, s);
+    case 2: return launch_t<64, 8, MIX>(tab, nb, nt, curv, visc, s);
+    case 3: return launch_t<16, 16, MIX>(tab, nb, nt, curv, visc, s);
+    case 4: return launch_t<32, 16, MIX>(tab, nb, nt, curv, visc, s);
+    default: return launch_t<64, 4, MIX>(tab, nb, nt, curv, visc, s);
+  }
 }
 
 }  // namespace
@@ -303,16 +360,11 @@ TileRange stage_tiles(const Geom &g, int shape, int kchunk) {
   return t;
 }
 
-hipError_t launch_stage(const Batch<StageArgs> *tab, int nb, int nt, bool curv, bool visc, int shape,
+hipError_t launch_stage(const Batch<StageArgs> *tab, int nb, int nt, bool curv, bool visc, int shape, bool mixed,
                         hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  switch (shape) {
-    case 1: return launch_t<32, 8>(tab, nb, nt, curv, visc, s);
-    case 2: return launch_t<64, 8>(tab, nb, nt, curv, visc, s);
-    case 3: return launch_t<16, 16>(tab, nb, nt, curv, visc, s);
-    case 4: return launch_t<32, 16>(tab, nb, nt, curv, visc, s);
-    default: return launch_t<64, 4>(tab, nb, nt, curv, visc, s);
-  }
+  return mixed ? launch_shape<true>(tab, nb, nt, curv, visc, shape, s)
+               : launch_shape<false>(tab, nb, nt, curv, visc, shape, s);
 }
 
 }  // namespace mgpu

@@ -76,6 +76,10 @@ struct mgpu_handle {
   // one captured step per U^n set (a case without filter rotates the sets)
   bool use_graph = true;
   hipGraphExec_t gexec[3] = {};
+  // mixed precision: FP32 flux/derivative/filter arithmetic about a constant
+  // reference state (the interior mean of block 0's state at mgpu_put_state)
+  bool mixed = false;
+  mgpu::RefState ref{};
 };
 
 namespace {
@@ -458,6 +462,7 @@ StageArgs stage_args(const mgpu_handle *h, const BlockData &bd, int in, int s0, 
   sa.met = metrics_of(bd);
   for (int q = 0; q < 4; q++) sa.kv[q] = bd.kv[q];
   sa.alpha = h->c.deltat * h->c.crk[irk - 1];
+  sa.ref = h->ref;
   return sa;
 }
 
@@ -559,6 +564,7 @@ int build_tables(mgpu_handle *h) {
       fa.xnu0 = -c.dissip_shock;
       fa.xnu10 = c.dissip_coeff / 1260.0;
       fa.csens = c.csens;
+      fa.ref = h->ref;
       fi[s0 * nb + b].t = tf;
     }
     VdivArgs &va = vd[b].a;
@@ -600,7 +606,7 @@ int enqueue_step(mgpu_handle *h, int s0) {
       ProfScope ps(h, MGPU_K_STAGE);
       const int v = (last && visc) ? 1 : 0;
       HIPCK(h, launch_stage(h->d_stage + ((size_t)s0 * nrk + (irk - 1)) * nb, nb, h->nt_stage[v], curv,
-                            last && visc, h->shape[v], h->stream));
+                            last && visc, h->shape[v], h->mixed, h->stream));
     }
     if ((rc = exchange_state(h, out))) return rc;
     in = out;
@@ -609,8 +615,9 @@ int enqueue_step(mgpu_handle *h, int s0) {
   if (c.filter != MGPU_FILTER_NONE) {
     {
       ProfScope ps(h, MGPU_K_FILTER);
-      if (c.filter == MGPU_FILTER_SF) HIPCK(h, launch_filter_sf(h->d_filt + s0 * nb, nb, h->nt_filt, h->stream));
-      else HIPCK(h, launch_filter_shock(h->d_filt + s0 * nb, nb, h->nt_filt, h->stream));
+      if (c.filter == MGPU_FILTER_SF)
+        HIPCK(h, launch_filter_sf(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, h->stream));
+      else HIPCK(h, launch_filter_shock(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, h->stream));
     }
     if ((rc = exchange_state(h, s0))) return rc;
   }
@@ -670,8 +677,8 @@ int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
   if (c->filter == MGPU_FILTER_SHOCK && c->iorder_visc == 0)
     return fail(nullptr, MGPU_EUNSUP,
                 "shock filter needs iorder_visc=4 (its sensor reads the viscous gradients of U^n)");
-  if (c->precision != MGPU_PREC_FP64)
-    return fail(nullptr, MGPU_EUNSUP, "precision=%d not supported yet", c->precision);
+  if (c->precision != MGPU_PREC_FP64 && c->precision != MGPU_PREC_MIXED)
+    return fail(nullptr, MGPU_EINVAL, "precision=%d", c->precision);
   for (int bI = 0; bI < c->nblocks; bI++) {
     const mgpu_block &bl = c->blocks[bI];
     if (bl.nx < G || bl.ny < G || bl.nz < G)
@@ -689,6 +696,7 @@ int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
   mgpu_handle *h = new mgpu_handle();
   h->c = *c;
   h->use_graph = (c->launch_flags & MGPU_LAUNCH_EAGER) == 0;
+  h->mixed = c->precision == MGPU_PREC_MIXED;
   h->blocks.assign(c->blocks, c->blocks + c->nblocks);
   h->c.blocks = h->blocks.data();
   if (hipGetDevice(&h->device) != hipSuccess) {
@@ -796,6 +804,32 @@ int mgpu_put_state(mgpu_handle *h, int block, const double *rho, const double *r
   HIPCK(h, hipStreamSynchronize(h->stream));
   bd.have_state = true;
   h->ghosts_dirty = true;  // the exchange runs before the next use of the ghosts
+  if (h->mixed && block == 0) {
+    // reference state of the mixed path: interior mean of block 0
+    double mean[5] = {0, 0, 0, 0, 0};
+    const Geom &g = bd.g;
+    for (int v = 0; v < 5; v++) {
+      double acc = 0.0;
+      for (int k = 1; k <= g.nz; k++)
+        for (int j = 1; j <= g.ny; j++) {
+          const double *row = src[v] + lin(g, 1, j, k);
+          for (int i = 0; i < g.nx; i++) acc += row[i];
+        }
+      mean[v] = acc / ((double)g.nx * g.ny * g.nz);
+    }
+    RefState &r = h->ref;
+    r.r = mean[0];
+    r.ru = mean[1];
+    r.rv = mean[2];
+    r.rw = mean[3];
+    r.re = mean[4];
+    r.u = r.ru / r.r;
+    r.v = r.rv / r.r;
+    r.w = r.rw / r.r;
+    r.p = (h->c.gam - 1.0) * (r.re - 0.5 * r.r * (r.u * r.u + r.v * r.v + r.w * r.w));
+    r.H = r.re + r.p;
+    h->tables_ready = false;
+  }
   return 0;
 }
 
@@ -851,7 +885,7 @@ int mgpu_autotune(mgpu_handle *h, int flags) {
   hipEvent_t e0 = get_event(h), e1 = get_event(h);
   if (!e0 || !e1) return fail(h, MGPU_EDEVICE, "hipEventCreate failed");
   for (int v = 0; v < (h->c.iorder_visc ? 2 : 1) && !rc; v++) {
-    TuneKey key{h->device, h->c.curvilinear, v, {}};
+    TuneKey key{h->device, h->c.curvilinear, v + (h->mixed ? 2 : 0), {}};
     for (const auto &bd : h->b) key.dims.insert(key.dims.end(), {bd.g.nx, bd.g.ny, bd.g.nz});
     if (!force) {
       std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -884,9 +918,10 @@ int mgpu_autotune(mgpu_handle *h, int flags) {
         }
         hipError_t e = hipMemcpy(h->d_tune, tab.data(), nb * sizeof(tab[0]), hipMemcpyHostToDevice);
         // one warm-up launch, then three timed ones
-        if (e == hipSuccess) e = launch_stage(h->d_tune, nb, nt, curv, v == 1, sh, h->stream);
+        if (e == hipSuccess) e = launch_stage(h->d_tune, nb, nt, curv, v == 1, sh, h->mixed, h->stream);
         if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
-        for (int r = 0; r < 3 && e == hipSuccess; r++) e = launch_stage(h->d_tune, nb, nt, curv, v == 1, sh, h->stream);
+        for (int r = 0; r < 3 && e == hipSuccess; r++)
+          e = launch_stage(h->d_tune, nb, nt, curv, v == 1, sh, h->mixed, h->stream);
         if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.f;

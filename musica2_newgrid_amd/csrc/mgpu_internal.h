@@ -124,6 +124,17 @@ struct Metrics {
   const double *mv[10];
 };
 
+// Reference state of the mixed-precision path: the FP32 arithmetic works on
+// fluxes and states minus their values at this constant state.  The 11-pt
+// divergence and the filters annihilate constants (and, on c3 metrics, the
+// constant-state contravariant flux by the discrete GCL), so the results are
+// those of the full quantities, with FP32 rounding relative to the
+// perturbation instead of the mean.
+struct RefState {
+  double r, ru, rv, rw, re;  // conserved
+  double u, v, w, p, H;      // primitives, H = rho E + p
+};
+
 // Arguments of the Euler (+viscous divergence) stage kernel.
 struct StageArgs {
   Geom g;
@@ -134,6 +145,7 @@ struct StageArgs {
   Metrics met;
   const double *kv[4];   // viscous increment of rhou,rhov,rhow,rhoe (last stage)
   double alpha;          // deltat*crk(irk)
+  RefState ref;          // mixed precision only
 };
 
 // Viscous flux tensor of U^n in the physical frame, per interior point:
@@ -173,6 +185,7 @@ struct FilterArgs {
   const double *sens;  // shock filter: Ducros sensor
   double dd[6];        // -chi*d11 (selective filter)
   double xnu0, xnu10, csens;
+  RefState ref;        // mixed precision only
 };
 
 // One face of one destination block in the interface exchange: the ghost
@@ -206,13 +219,13 @@ extern const int kStageShapes[][2];
 extern const int kNumStageShapes;
 TileRange stage_tiles(const Geom &g, int shape, int kchunk);
 hipError_t launch_stage(const Batch<StageArgs> *tab, int nb, int ntiles, bool curv, bool visc, int shape,
-                        hipStream_t s);
+                        bool mixed, hipStream_t s);
 TileRange visc_tiles(const Geom &g);
 hipError_t launch_visc(const Batch<ViscArgs> *tab, int nb, int ntiles, bool curv, hipStream_t s);
 TileRange vdiv_tiles(const Geom &g);
 hipError_t launch_vdiv(const Batch<VdivArgs> *tab, int nb, int ntiles, bool curv, hipStream_t s);
 TileRange filter_tiles(const Geom &g);
-hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int ntiles, hipStream_t s);
-hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int ntiles, hipStream_t s);
+hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, hipStream_t s);
+hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, hipStream_t s);
 hipError_t launch_iface(const IfaceArgs &a, int nchunks, hipStream_t s);
 }  // namespace mgpu

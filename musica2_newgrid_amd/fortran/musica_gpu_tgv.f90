@@ -46,7 +46,8 @@ program musica_gpu_tgv
   mu_ref = rho_ref*u_ref*1.0d0/Re
   muw_ref = c0*sqrt(T_ref**3)/(T_ref+S0)
 
-  allocate(rho(1-ngh:n+ngh,1-ngh:n+ngh,1-ngh:n+ngh), rhou, rhov, rhow, rhoe, mold=rho)
+  allocate(rho(1-ngh:n+ngh,1-ngh:n+ngh,1-ngh:n+ngh))
+  allocate(rhou, rhov, rhow, rhoe, mold=rho)
   rho = 0; rhou = 0; rhov = 0; rhow = 0; rhoe = 0
   dx = 2.0d0*pi/n
   umax = 0.0d0
