@@ -173,6 +173,47 @@ const char *mgpu_last_error(const mgpu_handle *h);
 /* Release every device resource of the handle. */
 void mgpu_destroy(mgpu_handle *h);
 
+/* ------------------------------------------------------------------------
+ * Serving loop.  No counterpart in the reference, which runs one case per
+ * MPI job (src/Main/musica_main.f90): this is for callers that advance many
+ * independent cases of different sizes on one GPU.  Each request is a
+ * one-block case with periodic faces; requests are admitted in submission
+ * order while their device memory fits the budget, stay resident, and
+ * advance together -- the RK stages of all resident requests of the same
+ * kernel variant run as one batched launch per kernel.  Their steps are
+ * exactly those of mgpu_step.
+ * ------------------------------------------------------------------------ */
+typedef struct mgpu_server mgpu_server;
+
+int mgpu_server_create(int64_t hbm_budget_bytes, mgpu_server **out);
+
+/* Queue a request.  c->nblocks must be 1 with periodic faces; metrics: the 6
+ * Cartesian arrays idx,idy,idz,idx_v,idy_v,idz_v, or (curvilinear) the 10
+ * c3 arrays followed by the 10 viscous ones; state: rho..rhoe padded.  The
+ * inputs are copied.  *id receives the request id. */
+int mgpu_server_submit(mgpu_server *s, const mgpu_case *c, const double *const *metrics,
+                       const double *const *state, int nsteps, int64_t *id);
+
+/* One scheduling round: retire finished requests (result copied to host
+ * memory held by the server, device memory released), admit queued
+ * requests while the budget allows, then advance every group of resident
+ * requests by up to max_steps steps (never past a member's last step).
+ * Reports the resident and queued counts. */
+int mgpu_server_tick(mgpu_server *s, int max_steps, int *n_resident, int *n_queued);
+
+/* state: 0 queued, 1 resident, 2 done (result waiting in mgpu_server_take). */
+int mgpu_server_status(mgpu_server *s, int64_t id, int *state, int *steps_done);
+
+/* Copy a finished request's state (padded arrays) and forget the request. */
+int mgpu_server_take(mgpu_server *s, int64_t id, double *rho, double *rhou, double *rhov, double *rhow,
+                     double *rhoe);
+
+/* Device bytes held by resident requests. */
+int64_t mgpu_server_memory(const mgpu_server *s);
+
+const char *mgpu_server_last_error(const mgpu_server *s);
+void mgpu_server_destroy(mgpu_server *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,14 @@ EXPORTS = (
     "mgpu_kernel_time",
     "mgpu_last_error",
     "mgpu_destroy",
+    "mgpu_server_create",
+    "mgpu_server_submit",
+    "mgpu_server_tick",
+    "mgpu_server_status",
+    "mgpu_server_take",
+    "mgpu_server_memory",
+    "mgpu_server_last_error",
+    "mgpu_server_destroy",
 )
 
 
@@ -106,6 +114,21 @@ def lib() -> C.CDLL:
     L.mgpu_last_error.restype = C.c_char_p
     L.mgpu_destroy.argtypes = [C.c_void_p]
     L.mgpu_destroy.restype = None
+    L.mgpu_server_create.argtypes = [C.c_int64, C.POINTER(C.c_void_p)]
+    L.mgpu_server_submit.argtypes = [C.c_void_p, C.POINTER(MgpuCase), C.POINTER(_dp), C.POINTER(_dp), C.c_int,
+                                     C.POINTER(C.c_int64)]
+    L.mgpu_server_tick.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.mgpu_server_status.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.mgpu_server_take.argtypes = [C.c_void_p, C.c_int64] + [_dp] * 5
+    L.mgpu_server_memory.argtypes = [C.c_void_p]
+    L.mgpu_server_memory.restype = C.c_int64
+    L.mgpu_server_last_error.argtypes = [C.c_void_p]
+    L.mgpu_server_last_error.restype = C.c_char_p
+    L.mgpu_server_destroy.argtypes = [C.c_void_p]
+    L.mgpu_server_destroy.restype = None
+    for name in ("mgpu_server_create", "mgpu_server_submit", "mgpu_server_tick", "mgpu_server_status",
+                 "mgpu_server_take"):
+        getattr(L, name).restype = C.c_int
     for name in ("mgpu_create", "mgpu_set_metrics_cart", "mgpu_set_metrics_c3", "mgpu_put_state",
                  "mgpu_get_state", "mgpu_step", "mgpu_autotune", "mgpu_stage_config", "mgpu_sync", "mgpu_profile", "mgpu_kernel_time"):
         getattr(L, name).restype = C.c_int

@@ -187,6 +187,37 @@ def vortex_wavy(nx: int = 512, ny: int = 256, nz: int = 128, mach: float = 0.5, 
     return cs
 
 
+def vortex_periodic(n: int = 64, mach: float = 0.5, beta: float = 5.0, radius: float = 0.5,
+                    re: float = 1.0e4, cfl: float = 0.8, chi: float = 0.2, filter: int = 1,
+                    iorder_visc: int = 4, T_inf: float = 300.0, rho_inf: float = 1.177) -> Case:
+    """Isentropic vortex (Shu, strength beta, radius R) advected at Mach `mach`
+    along x in a periodic Cartesian cube of edge 10 R."""
+    L = 20.0 * radius
+    dx = L / n
+    x = np.arange(n) * dx
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    c_inf = math.sqrt(GAM * RG * T_inf)
+    u_inf = mach * c_inf
+    xv = yv = L / 2.0
+    r2 = ((X - xv) ** 2 + (Y - yv) ** 2) / radius ** 2
+    e = np.exp(0.5 * (1.0 - r2))
+    u = u_inf * (1.0 - beta / (2.0 * math.pi) * (Y - yv) / radius * e)
+    v = u_inf * (beta / (2.0 * math.pi) * (X - xv) / radius * e)
+    w = np.zeros_like(u)
+    T = T_inf * (1.0 - (GAM - 1.0) * beta ** 2 * mach ** 2 / (8.0 * math.pi ** 2) * e * e)
+    rho = rho_inf * (T / T_inf) ** (1.0 / (GAM - 1.0))
+    p = rho * RG * T
+    U = _to_padded(conserved(rho, u, v, w, p), n, n, n)
+    mu_ref = rho_inf * u_inf * radius / re
+    idx = np.full(n + 2 * G, 1.0 / dx)
+    c = Case("vortex", n, n, n, curvilinear=False, iorder_visc=iorder_visc, filter=filter, ducros=False,
+             diffscale=mu_ref / float(sutherland(np.float64(T_inf))), chi=chi,
+             cart=[idx.copy() for _ in range(6)], U=U)
+    c.dt = timestep(U, dx, cfl)
+    c.info = dict(mach=mach, beta=beta, radius=radius, re=re, cfl=cfl, L=L)
+    return c
+
+
 def cartesian_box(nx, ny, nz, lengths, fields, iorder_visc=0, filter=0, diffscale=1.0, dt=1e-6,
                   chi=0.2) -> Case:
     """Generic periodic Cartesian case from primitive fields (rho,u,v,w,p) on the interior."""

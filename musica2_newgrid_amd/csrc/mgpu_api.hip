@@ -12,77 +12,13 @@
 #include <string>
 #include <vector>
 
-#include "mgpu_internal.h"
+#include "mgpu_handle.h"
 
 using namespace mgpu;
 
-namespace {
+namespace mgpu_host {
 
 thread_local std::string g_create_error;
-
-struct BlockData {
-  Geom g{};
-  double *st[3][5] = {};  // three state sets: st[cur] = U^n, the others stage buffers
-  double *idx[6] = {};    // Cartesian metrics idx,idy,idz,idx_v,idy_v,idz_v
-  double *m[10] = {};
-  double *mv[10] = {};
-  double *tau[NTAU] = {}; // physical viscous flux tensor of U^n (last stage)
-  double *kv[4] = {};     // viscous increment of rhou..rhoe (last stage)
-  double *prs = nullptr, *sens = nullptr;
-  bool have_metrics = false;
-  bool have_state = false;
-};
-
-struct ProfRec {
-  int kind;
-  hipEvent_t a, b;
-};
-
-}  // namespace
-
-struct mgpu_handle {
-  mgpu_case c{};
-  std::vector<mgpu_block> blocks;
-  std::vector<BlockData> b;
-  int cur = 0;  // index of the set holding U^n
-  hipStream_t stream = nullptr;
-  int device = 0;
-  std::string err;
-  bool profiling = false;
-  std::vector<ProfRec> recs;
-  std::vector<hipEvent_t> ev_pool;
-  int64_t kcount[MGPU_K_COUNT] = {};
-  double kms[MGPU_K_COUNT] = {};
-  // interface exchange: task and chunk tables (5 state layers, 2 viscous
-  // layers) and per-array-set pointer tables, all in device memory
-  mgpu::IfaceArgs xs[3] = {};  // state set s
-  mgpu::IfaceArgs xv = {};     // tau (+ prs, sens)
-  int nchunks_s = 0, nchunks_v = 0;
-  std::vector<void *> dev_tables;
-  bool ghosts_dirty = false;
-  // stage kernel launch parameters: [0] Euler stages, [1] last stage with the
-  // viscous divergence (chosen by mgpu_autotune, else a default); kchunk 0
-  // means the per-block default
-  int shape[2] = {2, 2};
-  int kchunk[2] = {0, 0};
-  // batched launch tables in device memory, one entry per block
-  mgpu::Batch<mgpu::StageArgs> *d_stage = nullptr;  // [cur][irk-1][block]
-  mgpu::Batch<mgpu::ViscArgs> *d_visc = nullptr;    // [cur][block]
-  mgpu::Batch<mgpu::VdivArgs> *d_vdiv = nullptr;    // [block]
-  mgpu::Batch<mgpu::FilterArgs> *d_filt = nullptr;  // [cur][block]
-  mgpu::Batch<mgpu::StageArgs> *d_tune = nullptr;   // [block], autotuner
-  int nt_stage[2] = {0, 0}, nt_visc = 0, nt_vdiv = 0, nt_filt = 0;
-  bool tables_ready = false;
-  // one captured step per U^n set (a case without filter rotates the sets)
-  bool use_graph = true;
-  hipGraphExec_t gexec[3] = {};
-  // mixed precision: FP32 flux/derivative/filter arithmetic about a constant
-  // reference state (the interior mean of block 0's state at mgpu_put_state)
-  bool mixed = false;
-  mgpu::RefState ref{};
-};
-
-namespace {
 
 int fail(mgpu_handle *h, int code, const char *fmt, ...) {
   char buf[512];
@@ -351,14 +287,33 @@ int upload(mgpu_handle *h, const std::vector<T> &v, T **out) {
   if (v.empty()) return 0;
   void *p = nullptr;
   HIPCK(h, hipMalloc(&p, v.size() * sizeof(T)));
-  h->dev_tables.push_back(p);
+  h->xtables.push_back(p);
   HIPCK(h, hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
   *out = (T *)p;
   return 0;
 }
 
-// Task, chunk and pointer tables of the interface exchange.
+void free_exchange(mgpu_handle *h) {
+  for (void *p : h->xtables) (void)hipFree(p);
+  h->xtables.clear();
+  for (auto &x : h->xs) x = IfaceArgs{};
+  h->xv = IfaceArgs{};
+  h->nchunks_s = h->nchunks_v = 0;
+  h->exchange_ready = false;
+}
+
+// Task, chunk and pointer tables of the interface exchange (rebuilt when the
+// set of blocks changes; the stream is idle then).
 int setup_exchange(mgpu_handle *h) {
+  if (h->exchange_ready) return 0;
+  HIPCK(h, hipStreamSynchronize(h->stream));
+  free_exchange(h);
+  h->c.blocks = h->blocks.data();
+  h->c.nblocks = (int)h->blocks.size();
+  if (h->b.empty()) {
+    h->exchange_ready = true;
+    return 0;
+  }
   std::vector<FaceTask> tasks;
   std::vector<IfaceChunk> chunks;
   const int nb = (int)h->b.size();
@@ -398,6 +353,7 @@ int setup_exchange(mgpu_handle *h) {
     if ((rc = upload(h, ptrs, &dp))) return rc;
     h->xv = IfaceArgs{dt, dc, dp, nv};
   }
+  h->exchange_ready = true;
   return 0;
 }
 
@@ -451,9 +407,10 @@ int dalloc(mgpu_handle *h, double **p, size_t count) {
 }
 
 StageArgs stage_args(const mgpu_handle *h, const BlockData &bd, int in, int s0, int out, int irk) {
+  (void)h;
   StageArgs sa{};
   sa.g = bd.g;
-  sa.gas = gas_of(h->c);
+  sa.gas = gas_of(bd.p);
   for (int v = 0; v < 5; v++) {
     sa.uin.v[v] = bd.st[in][v];
     sa.u0.v[v] = bd.st[s0][v];
@@ -461,8 +418,8 @@ StageArgs stage_args(const mgpu_handle *h, const BlockData &bd, int in, int s0, 
   }
   sa.met = metrics_of(bd);
   for (int q = 0; q < 4; q++) sa.kv[q] = bd.kv[q];
-  sa.alpha = h->c.deltat * h->c.crk[irk - 1];
-  sa.ref = h->ref;
+  sa.alpha = bd.p.deltat * bd.p.crk[irk - 1];
+  sa.ref = bd.ref;
   return sa;
 }
 
@@ -483,12 +440,26 @@ int stage_ranges(const mgpu_handle *h, int v, std::vector<TileRange> &tr) {
 
 template <class T>
 int dalloc_table(mgpu_handle *h, T **p, size_t count) {
-  if (*p) return 0;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
   void *q = nullptr;
   HIPCK(h, hipMalloc(&q, count * sizeof(T)));
-  h->dev_tables.push_back(q);
   *p = (T *)q;
   return 0;
+}
+
+void free_tables(mgpu_handle *h) {
+  if (h->d_stage) (void)hipFree(h->d_stage);
+  if (h->d_visc) (void)hipFree(h->d_visc);
+  if (h->d_vdiv) (void)hipFree(h->d_vdiv);
+  if (h->d_filt) (void)hipFree(h->d_filt);
+  if (h->d_tune) (void)hipFree(h->d_tune);
+  h->d_stage = nullptr;
+  h->d_visc = nullptr;
+  h->d_vdiv = nullptr;
+  h->d_filt = nullptr;
+  h->d_tune = nullptr;
+  h->tab_cap = 0;
 }
 
 void drop_graphs(mgpu_handle *h) {
@@ -505,12 +476,21 @@ int build_tables(mgpu_handle *h) {
   const int nb = (int)h->b.size(), nrk = h->c.nrk;
   const mgpu_case &c = h->c;
   int rc;
-  if ((rc = dalloc_table(h, &h->d_stage, (size_t)3 * nrk * nb)) || (rc = dalloc_table(h, &h->d_visc, 3 * nb)) ||
-      (rc = dalloc_table(h, &h->d_vdiv, nb)) || (rc = dalloc_table(h, &h->d_filt, 3 * nb)) ||
-      (rc = dalloc_table(h, &h->d_tune, nb)))
-    return rc;
   HIPCK(h, hipStreamSynchronize(h->stream));
   drop_graphs(h);
+  if (nb > h->tab_cap) {
+    const int cap = nb < 8 ? 8 : nb;
+    if ((rc = dalloc_table(h, &h->d_stage, (size_t)3 * nrk * cap)) || (rc = dalloc_table(h, &h->d_visc, 3 * cap)) ||
+        (rc = dalloc_table(h, &h->d_vdiv, cap)) || (rc = dalloc_table(h, &h->d_filt, 3 * cap)) ||
+        (rc = dalloc_table(h, &h->d_tune, cap)))
+      return rc;
+    h->tab_cap = cap;
+  }
+  if (nb == 0) {
+    h->nt_stage[0] = h->nt_stage[1] = h->nt_visc = h->nt_vdiv = h->nt_filt = 0;
+    h->tables_ready = true;
+    return 0;
+  }
   std::vector<TileRange> tr[2];
   h->nt_stage[0] = stage_ranges(h, 0, tr[0]);
   h->nt_stage[1] = stage_ranges(h, 1, tr[1]);
@@ -543,7 +523,7 @@ int build_tables(mgpu_handle *h) {
       }
       ViscArgs &va = vi[s0 * nb + b].a;
       va.g = bd.g;
-      va.gas = gas_of(c);
+      va.gas = gas_of(bd.p);
       for (int v = 0; v < 5; v++) va.u0.v[v] = bd.st[s0][v];
       va.met = metrics_of(bd);
       for (int q = 0; q < NTAU; q++) va.tau[q] = bd.tau[q];
@@ -558,13 +538,13 @@ int build_tables(mgpu_handle *h) {
         fa.uin.v[v] = bd.st[in][v];  // output of the last stage
         fa.uout.v[v] = bd.st[s0][v];
       }
-      for (int l = 0; l < 6; l++) fa.dd[l] = -d11[l] * c.dissip_coeff;
+      for (int l = 0; l < 6; l++) fa.dd[l] = -d11[l] * bd.p.dissip_coeff;
       fa.prs = bd.prs;
       fa.sens = c.is_ducros ? bd.sens : nullptr;
-      fa.xnu0 = -c.dissip_shock;
-      fa.xnu10 = c.dissip_coeff / 1260.0;
-      fa.csens = c.csens;
-      fa.ref = h->ref;
+      fa.xnu0 = -bd.p.dissip_shock;
+      fa.xnu10 = bd.p.dissip_coeff / 1260.0;
+      fa.csens = bd.p.csens;
+      fa.ref = bd.ref;
       fi[s0 * nb + b].t = tf;
     }
     VdivArgs &va = vd[b].a;
@@ -652,53 +632,38 @@ int graph_step(mgpu_handle *h) {
   return 0;
 }
 
-// Exchange the ghosts of U^n after mgpu_put_state, once every block that
-// has state is in place.
+// Exchange the ghosts of U^n after put_state, once every block that has
+// state is in place.
 int refresh_ghosts(mgpu_handle *h) {
-  if (!h->ghosts_dirty) return 0;
-  int rc = exchange_state(h, h->cur);
-  if (rc) return rc;
+  int rc = setup_exchange(h);
+  if (rc || !h->ghosts_dirty) return rc;
+  if ((rc = exchange_state(h, h->cur))) return rc;
   h->ghosts_dirty = false;
   return 0;
 }
 
-}  // namespace
+std::string check_flags(const mgpu_case &c) {
+  char buf[256];
+  buf[0] = 0;
+  if (c.nrk < 1 || c.nrk > 8) snprintf(buf, sizeof buf, "nrk=%d out of range 1..8", c.nrk);
+  else if (c.iorder_visc != 0 && c.iorder_visc != 4)
+    snprintf(buf, sizeof buf, "iorder_visc=%d: only 0 and 4 are supported", c.iorder_visc);
+  else if (c.filter < 0 || c.filter > 2) snprintf(buf, sizeof buf, "filter=%d", c.filter);
+  else if (c.filter == MGPU_FILTER_SHOCK && c.iorder_visc == 0)
+    snprintf(buf, sizeof buf, "shock filter needs iorder_visc=4 (its sensor reads the viscous gradients of U^n)");
+  else if (c.precision != MGPU_PREC_FP64 && c.precision != MGPU_PREC_MIXED)
+    snprintf(buf, sizeof buf, "precision=%d", c.precision);
+  return std::string(buf);
+}
 
-extern "C" {
-
-int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
-  if (!c || !out) return fail(nullptr, MGPU_EINVAL, "null argument");
+int handle_new(const mgpu_case &c, mgpu_handle **out) {
   *out = nullptr;
-  if (c->nblocks < 1 || !c->blocks) return fail(nullptr, MGPU_EINVAL, "nblocks must be >= 1");
-  if (c->nrk < 1 || c->nrk > 8) return fail(nullptr, MGPU_EINVAL, "nrk=%d out of range 1..8", c->nrk);
-  if (c->iorder_visc != 0 && c->iorder_visc != 4)
-    return fail(nullptr, MGPU_EUNSUP, "iorder_visc=%d: only 0 and 4 are supported", c->iorder_visc);
-  if (c->filter < 0 || c->filter > 2) return fail(nullptr, MGPU_EINVAL, "filter=%d", c->filter);
-  if (c->filter == MGPU_FILTER_SHOCK && c->iorder_visc == 0)
-    return fail(nullptr, MGPU_EUNSUP,
-                "shock filter needs iorder_visc=4 (its sensor reads the viscous gradients of U^n)");
-  if (c->precision != MGPU_PREC_FP64 && c->precision != MGPU_PREC_MIXED)
-    return fail(nullptr, MGPU_EINVAL, "precision=%d", c->precision);
-  for (int bI = 0; bI < c->nblocks; bI++) {
-    const mgpu_block &bl = c->blocks[bI];
-    if (bl.nx < G || bl.ny < G || bl.nz < G)
-      return fail(nullptr, MGPU_EINVAL, "block %d: every extent must be >= %d", bI, G);
-    const size_t padded = (size_t)(bl.nx + 2 * G) * (bl.ny + 2 * G) * (bl.nz + 2 * G);
-    if (padded > kMaxPadded)
-      return fail(nullptr, MGPU_EINVAL, "block %d: %zu padded points exceed the 2^28 limit of one block", bI,
-                  padded);
-  }
-  {
-    const std::string msg = check_topology(*c);
-    if (!msg.empty()) return fail(nullptr, MGPU_EINVAL, "%s", msg.c_str());
-  }
-
   mgpu_handle *h = new mgpu_handle();
-  h->c = *c;
-  h->use_graph = (c->launch_flags & MGPU_LAUNCH_EAGER) == 0;
-  h->mixed = c->precision == MGPU_PREC_MIXED;
-  h->blocks.assign(c->blocks, c->blocks + c->nblocks);
-  h->c.blocks = h->blocks.data();
+  h->c = c;
+  h->c.nblocks = 0;
+  h->c.blocks = nullptr;
+  h->use_graph = (c.launch_flags & MGPU_LAUNCH_EAGER) == 0;
+  h->mixed = c.precision == MGPU_PREC_MIXED;
   if (hipGetDevice(&h->device) != hipSuccess) {
     delete h;
     return fail(nullptr, MGPU_EDEVICE, "no HIP device");
@@ -707,55 +672,86 @@ int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
     delete h;
     return fail(nullptr, MGPU_EDEVICE, "hipStreamCreate failed");
   }
-  h->b.resize(c->nblocks);
-  int rc = 0;
-  for (int bI = 0; bI < c->nblocks && !rc; bI++) {
-    BlockData &bd = h->b[bI];
-    const mgpu_block &bl = h->blocks[bI];
-    bd.g.nx = bl.nx;
-    bd.g.ny = bl.ny;
-    bd.g.nz = bl.nz;
-    bd.g.sx = bl.nx + 2 * G;
-    bd.g.sy = (idx_t)bd.g.sx * (idx_t)(bl.ny + 2 * G);
-    bd.g.n = (long)bd.g.sy * (bl.nz + 2 * G);
-    const size_t n = (size_t)bd.g.n;
-    for (int s = 0; s < 3 && !rc; s++)
-      for (int v = 0; v < 5 && !rc; v++) rc = dalloc(h, &bd.st[s][v], n);
-    if (c->curvilinear) {
-      for (int q = 0; q < 10 && !rc; q++) rc = dalloc(h, &bd.m[q], n);
-      if (c->iorder_visc)
-        for (int q = 0; q < 10 && !rc; q++) rc = dalloc(h, &bd.mv[q], n);
-    } else {
-      const int len[3] = {bl.nx + 2 * G, bl.ny + 2 * G, bl.nz + 2 * G};
-      for (int q = 0; q < 6 && !rc; q++) rc = dalloc(h, &bd.idx[q], (size_t)len[q % 3]);
-    }
-    if (c->iorder_visc) {
-      for (int q = 0; q < NTAU && !rc; q++) rc = dalloc(h, &bd.tau[q], n);
-      for (int q = 0; q < 4 && !rc; q++) rc = dalloc(h, &bd.kv[q], n);
-      if (c->filter == MGPU_FILTER_SHOCK) {
-        if (!rc) rc = dalloc(h, &bd.prs, n);
-        if (!rc) rc = dalloc(h, &bd.sens, n);
-      }
-    }
-  }
-  if (!rc) rc = setup_exchange(h);
-  if (rc) {
-    g_create_error = h->err;
-    mgpu_destroy(h);
-    return rc;
-  }
   *out = h;
   return 0;
 }
 
-int mgpu_set_metrics_cart(mgpu_handle *h, int block, const double *idx, const double *idy,
-                          const double *idz, const double *idx_v, const double *idy_v,
-                          const double *idz_v) {
-  if (!h) return MGPU_EINVAL;
+size_t block_bytes(const mgpu_case &c, int nx, int ny, int nz) {
+  const size_t n = (size_t)(nx + 2 * G) * (ny + 2 * G) * (nz + 2 * G);
+  size_t arrays = 15;  // three state sets
+  if (c.curvilinear) arrays += c.iorder_visc ? 20 : 10;
+  if (c.iorder_visc) arrays += NTAU + 4 + (c.filter == MGPU_FILTER_SHOCK ? 2 : 0);
+  size_t bytes = arrays * n * sizeof(double);
+  if (!c.curvilinear) bytes += 6 * (size_t)(nx + ny + nz + 6 * G) * sizeof(double);
+  return bytes;
+}
+
+int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
+  const mgpu_case &c = h->c;
+  if (bl.nx < G || bl.ny < G || bl.nz < G)
+    return fail(h, MGPU_EINVAL, "block extents %dx%dx%d: every extent must be >= %d", bl.nx, bl.ny, bl.nz, G);
+  const size_t padded = (size_t)(bl.nx + 2 * G) * (bl.ny + 2 * G) * (bl.nz + 2 * G);
+  if (padded > kMaxPadded)
+    return fail(h, MGPU_EINVAL, "%zu padded points exceed the 2^28 limit of one block", padded);
+  HIPCK(h, hipStreamSynchronize(h->stream));
+  h->b.emplace_back();
+  h->blocks.push_back(bl);
+  BlockData &bd = h->b.back();
+  bd.p = p;
+  bd.g.nx = bl.nx;
+  bd.g.ny = bl.ny;
+  bd.g.nz = bl.nz;
+  bd.g.sx = bl.nx + 2 * G;
+  bd.g.sy = (idx_t)bd.g.sx * (idx_t)(bl.ny + 2 * G);
+  bd.g.n = (long)bd.g.sy * (bl.nz + 2 * G);
+  bd.bytes = block_bytes(c, bl.nx, bl.ny, bl.nz);
+  const size_t n = (size_t)bd.g.n;
+  int rc = 0;
+  for (int s = 0; s < 3 && !rc; s++)
+    for (int v = 0; v < 5 && !rc; v++) rc = dalloc(h, &bd.st[s][v], n);
+  if (c.curvilinear) {
+    for (int q = 0; q < 10 && !rc; q++) rc = dalloc(h, &bd.m[q], n);
+    if (c.iorder_visc)
+      for (int q = 0; q < 10 && !rc; q++) rc = dalloc(h, &bd.mv[q], n);
+  } else {
+    const int len[3] = {bl.nx + 2 * G, bl.ny + 2 * G, bl.nz + 2 * G};
+    for (int q = 0; q < 6 && !rc; q++) rc = dalloc(h, &bd.idx[q], (size_t)len[q % 3]);
+  }
+  if (c.iorder_visc) {
+    for (int q = 0; q < NTAU && !rc; q++) rc = dalloc(h, &bd.tau[q], n);
+    for (int q = 0; q < 4 && !rc; q++) rc = dalloc(h, &bd.kv[q], n);
+    if (c.filter == MGPU_FILTER_SHOCK) {
+      if (!rc) rc = dalloc(h, &bd.prs, n);
+      if (!rc) rc = dalloc(h, &bd.sens, n);
+    }
+  }
+  if (rc) {
+    free_block(bd);
+    h->b.pop_back();
+    h->blocks.pop_back();
+    return rc;
+  }
+  h->exchange_ready = false;
+  h->tables_ready = false;
+  return 0;
+}
+
+int handle_remove_block(mgpu_handle *h, int i) {
+  if (i < 0 || i >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", i);
+  HIPCK(h, hipStreamSynchronize(h->stream));
+  drop_graphs(h);
+  free_block(h->b[i]);
+  h->b.erase(h->b.begin() + i);
+  h->blocks.erase(h->blocks.begin() + i);
+  h->exchange_ready = false;
+  h->tables_ready = false;
+  return 0;
+}
+
+int put_metrics_cart(mgpu_handle *h, int block, const double *const *src) {
   if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
   if (h->c.curvilinear) return fail(h, MGPU_ESTATE, "case is curvilinear: use mgpu_set_metrics_c3");
   BlockData &bd = h->b[block];
-  const double *src[6] = {idx, idy, idz, idx_v, idy_v, idz_v};
   const int len[3] = {bd.g.nx + 2 * G, bd.g.ny + 2 * G, bd.g.nz + 2 * G};
   for (int q = 0; q < 6; q++) {
     if (!src[q]) {
@@ -768,8 +764,7 @@ int mgpu_set_metrics_cart(mgpu_handle *h, int block, const double *idx, const do
   return 0;
 }
 
-int mgpu_set_metrics_c3(mgpu_handle *h, int block, const double *const *m, const double *const *mv) {
-  if (!h) return MGPU_EINVAL;
+int put_metrics_c3(mgpu_handle *h, int block, const double *const *m, const double *const *mv) {
   if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
   if (!h->c.curvilinear) return fail(h, MGPU_ESTATE, "case is Cartesian: use mgpu_set_metrics_cart");
   if (!m) return fail(h, MGPU_EINVAL, "null metric table");
@@ -790,22 +785,18 @@ int mgpu_set_metrics_c3(mgpu_handle *h, int block, const double *const *m, const
   return 0;
 }
 
-int mgpu_put_state(mgpu_handle *h, int block, const double *rho, const double *rhou, const double *rhov,
-                   const double *rhow, const double *rhoe) {
-  if (!h) return MGPU_EINVAL;
+int put_state(mgpu_handle *h, int block, const double *const *src) {
   if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
-  const double *src[5] = {rho, rhou, rhov, rhow, rhoe};
   BlockData &bd = h->b[block];
   const size_t bytes = sizeof(double) * (size_t)bd.g.n;
-  for (int v = 0; v < 5; v++) {
+  for (int v = 0; v < 5; v++)
     if (!src[v]) return fail(h, MGPU_EINVAL, "state array %d is null", v);
-    HIPCK(h, hipMemcpyAsync(bd.st[h->cur][v], src[v], bytes, hipMemcpyHostToDevice, h->stream));
-  }
   HIPCK(h, hipStreamSynchronize(h->stream));
+  for (int v = 0; v < 5; v++) HIPCK(h, hipMemcpy(bd.st[h->cur][v], src[v], bytes, hipMemcpyHostToDevice));
   bd.have_state = true;
   h->ghosts_dirty = true;  // the exchange runs before the next use of the ghosts
-  if (h->mixed && block == 0) {
-    // reference state of the mixed path: interior mean of block 0
+  if (h->mixed) {
+    // reference state of the mixed path: interior mean of this block's state
     double mean[5] = {0, 0, 0, 0, 0};
     const Geom &g = bd.g;
     for (int v = 0; v < 5; v++) {
@@ -817,7 +808,7 @@ int mgpu_put_state(mgpu_handle *h, int block, const double *rho, const double *r
         }
       mean[v] = acc / ((double)g.nx * g.ny * g.nz);
     }
-    RefState &r = h->ref;
+    RefState &r = bd.ref;
     r.r = mean[0];
     r.ru = mean[1];
     r.rv = mean[2];
@@ -826,47 +817,132 @@ int mgpu_put_state(mgpu_handle *h, int block, const double *rho, const double *r
     r.u = r.ru / r.r;
     r.v = r.rv / r.r;
     r.w = r.rw / r.r;
-    r.p = (h->c.gam - 1.0) * (r.re - 0.5 * r.r * (r.u * r.u + r.v * r.v + r.w * r.w));
+    r.p = (bd.p.gam - 1.0) * (r.re - 0.5 * r.r * (r.u * r.u + r.v * r.v + r.w * r.w));
     r.H = r.re + r.p;
     h->tables_ready = false;
   }
   return 0;
 }
 
-int mgpu_get_state(mgpu_handle *h, int block, double *rho, double *rhou, double *rhov, double *rhow,
-                   double *rhoe) {
-  if (!h) return MGPU_EINVAL;
+int get_state(mgpu_handle *h, int block, double *const *dst) {
   if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
-  double *dst[5] = {rho, rhou, rhov, rhow, rhoe};
   BlockData &bd = h->b[block];
   const size_t bytes = sizeof(double) * (size_t)bd.g.n;
+  for (int v = 0; v < 5; v++)
+    if (!dst[v]) return fail(h, MGPU_EINVAL, "state array %d is null", v);
   int rc = refresh_ghosts(h);
   if (rc) return rc;
   HIPCK(h, hipStreamSynchronize(h->stream));
-  for (int v = 0; v < 5; v++) {
-    if (!dst[v]) return fail(h, MGPU_EINVAL, "state array %d is null", v);
-    HIPCK(h, hipMemcpy(dst[v], bd.st[h->cur][v], bytes, hipMemcpyDeviceToHost));
-  }
+  for (int v = 0; v < 5; v++) HIPCK(h, hipMemcpy(dst[v], bd.st[h->cur][v], bytes, hipMemcpyDeviceToHost));
   return 0;
 }
 
-int mgpu_step(mgpu_handle *h, int nsteps) {
-  if (!h) return MGPU_EINVAL;
+int step(mgpu_handle *h, int nsteps) {
   if (nsteps < 0) return fail(h, MGPU_EINVAL, "nsteps=%d", nsteps);
   for (size_t bI = 0; bI < h->b.size(); bI++) {
     if (!h->b[bI].have_metrics) return fail(h, MGPU_ESTATE, "block %zu: metrics not set", bI);
     if (!h->b[bI].have_state) return fail(h, MGPU_ESTATE, "block %zu: state not set", bI);
   }
-  int rc = build_tables(h);
-  if (!rc) rc = refresh_ghosts(h);
+  if (h->b.empty()) return 0;
+  int rc = refresh_ghosts(h);
+  if (!rc) rc = build_tables(h);
   if (rc) return rc;
   const bool graph = h->use_graph && !h->profiling;
-  for (int step = 0; step < nsteps; step++) {
+  for (int s = 0; s < nsteps; s++) {
     rc = graph ? graph_step(h) : enqueue_step(h, h->cur);
     if (rc) return rc;
     h->cur = next_set(h->c, h->cur);
   }
   return 0;
+}
+
+void destroy(mgpu_handle *h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (auto &r : h->recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : h->ev_pool) (void)hipEventDestroy(e);
+  drop_graphs(h);
+  for (auto &bd : h->b) free_block(bd);
+  free_exchange(h);
+  free_tables(h);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+}  // namespace mgpu_host
+
+using namespace mgpu_host;
+
+extern "C" {
+
+int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
+  if (!c || !out) return fail(nullptr, MGPU_EINVAL, "null argument");
+  *out = nullptr;
+  if (c->nblocks < 1 || !c->blocks) return fail(nullptr, MGPU_EINVAL, "nblocks must be >= 1");
+  {
+    const std::string msg = check_flags(*c);
+    if (!msg.empty()) return fail(nullptr, MGPU_EINVAL, "%s", msg.c_str());
+  }
+  for (int bI = 0; bI < c->nblocks; bI++) {
+    const mgpu_block &bl = c->blocks[bI];
+    if (bl.nx < G || bl.ny < G || bl.nz < G)
+      return fail(nullptr, MGPU_EINVAL, "block %d: every extent must be >= %d", bI, G);
+    const size_t padded = (size_t)(bl.nx + 2 * G) * (bl.ny + 2 * G) * (bl.nz + 2 * G);
+    if (padded > kMaxPadded)
+      return fail(nullptr, MGPU_EINVAL, "block %d: %zu padded points exceed the 2^28 limit of one block", bI,
+                  padded);
+  }
+  {
+    const std::string msg = check_topology(*c);
+    if (!msg.empty()) return fail(nullptr, MGPU_EINVAL, "%s", msg.c_str());
+  }
+  mgpu_handle *h = nullptr;
+  int rc = handle_new(*c, &h);
+  if (rc) return rc;
+  for (int bI = 0; bI < c->nblocks && !rc; bI++) rc = handle_add_block(h, c->blocks[bI], *c);
+  if (!rc) rc = setup_exchange(h);
+  if (rc) {
+    g_create_error = h->err;
+    destroy(h);
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+int mgpu_set_metrics_cart(mgpu_handle *h, int block, const double *idx, const double *idy,
+                          const double *idz, const double *idx_v, const double *idy_v,
+                          const double *idz_v) {
+  if (!h) return MGPU_EINVAL;
+  const double *src[6] = {idx, idy, idz, idx_v, idy_v, idz_v};
+  return put_metrics_cart(h, block, src);
+}
+
+int mgpu_set_metrics_c3(mgpu_handle *h, int block, const double *const *m, const double *const *mv) {
+  if (!h) return MGPU_EINVAL;
+  return put_metrics_c3(h, block, m, mv);
+}
+
+int mgpu_put_state(mgpu_handle *h, int block, const double *rho, const double *rhou, const double *rhov,
+                   const double *rhow, const double *rhoe) {
+  if (!h) return MGPU_EINVAL;
+  const double *src[5] = {rho, rhou, rhov, rhow, rhoe};
+  return put_state(h, block, src);
+}
+
+int mgpu_get_state(mgpu_handle *h, int block, double *rho, double *rhou, double *rhov, double *rhow,
+                   double *rhoe) {
+  if (!h) return MGPU_EINVAL;
+  double *dst[5] = {rho, rhou, rhov, rhow, rhoe};
+  return get_state(h, block, dst);
+}
+
+int mgpu_step(mgpu_handle *h, int nsteps) {
+  if (!h) return MGPU_EINVAL;
+  return step(h, nsteps);
 }
 
 int mgpu_autotune(mgpu_handle *h, int flags) {
@@ -995,19 +1071,6 @@ const char *mgpu_last_error(const mgpu_handle *h) {
   return h->err.c_str();
 }
 
-void mgpu_destroy(mgpu_handle *h) {
-  if (!h) return;
-  if (h->stream) (void)hipStreamSynchronize(h->stream);
-  for (auto &r : h->recs) {
-    (void)hipEventDestroy(r.a);
-    (void)hipEventDestroy(r.b);
-  }
-  for (auto e : h->ev_pool) (void)hipEventDestroy(e);
-  drop_graphs(h);
-  for (auto &bd : h->b) free_block(bd);
-  for (void *p : h->dev_tables) (void)hipFree(p);
-  if (h->stream) (void)hipStreamDestroy(h->stream);
-  delete h;
-}
+void mgpu_destroy(mgpu_handle *h) { destroy(h); }
 
 }  // extern "C"

@@ -1,0 +1,107 @@
+// Host-side state of a libmusica_gpu.so handle, shared by the C-ABI
+// (mgpu_api.hip) and the serving loop (mgpu_serve.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "mgpu_internal.h"
+
+namespace mgpu_host {
+
+using namespace mgpu;
+
+struct BlockData {
+  Geom g{};
+  mgpu_case p{};          // scalars of this block's case (time step, gas, filter amplitudes)
+  double *st[3][5] = {};  // three state sets: st[cur] = U^n, the others stage buffers
+  double *idx[6] = {};    // Cartesian metrics idx,idy,idz,idx_v,idy_v,idz_v
+  double *m[10] = {};
+  double *mv[10] = {};
+  double *tau[NTAU] = {}; // physical viscous flux tensor of U^n (last stage)
+  double *kv[4] = {};     // viscous increment of rhou..rhoe (last stage)
+  double *prs = nullptr, *sens = nullptr;
+  RefState ref{};         // mixed precision: interior mean of the state at put_state
+  size_t bytes = 0;       // device memory held by the block
+  bool have_metrics = false;
+  bool have_state = false;
+};
+
+struct ProfRec {
+  int kind;
+  hipEvent_t a, b;
+};
+
+}  // namespace mgpu_host
+
+struct mgpu_handle {
+  mgpu_case c{};                    // flags (uniform over the blocks) and defaults
+  std::vector<mgpu_block> blocks;   // topology, c.blocks points here
+  std::vector<mgpu_host::BlockData> b;
+  int cur = 0;  // index of the set holding U^n
+  hipStream_t stream = nullptr;
+  int device = 0;
+  std::string err;
+  bool profiling = false;
+  std::vector<mgpu_host::ProfRec> recs;
+  std::vector<hipEvent_t> ev_pool;
+  int64_t kcount[MGPU_K_COUNT] = {};
+  double kms[MGPU_K_COUNT] = {};
+  // interface exchange: task and chunk tables (5 state layers, 2 viscous
+  // layers) and per-array-set pointer tables, all in device memory
+  mgpu::IfaceArgs xs[3] = {};  // state set s
+  mgpu::IfaceArgs xv = {};     // tau (+ prs, sens)
+  int nchunks_s = 0, nchunks_v = 0;
+  std::vector<void *> xtables;  // device memory of the exchange tables
+  bool exchange_ready = false;
+  bool ghosts_dirty = false;
+  // stage kernel launch parameters: [0] Euler stages, [1] last stage with the
+  // viscous divergence (chosen by mgpu_autotune, else a default); kchunk 0
+  // means the per-block default
+  int shape[2] = {2, 2};
+  int kchunk[2] = {0, 0};
+  // batched launch tables in device memory, one entry per block
+  mgpu::Batch<mgpu::StageArgs> *d_stage = nullptr;  // [cur][irk-1][block]
+  mgpu::Batch<mgpu::ViscArgs> *d_visc = nullptr;    // [cur][block]
+  mgpu::Batch<mgpu::VdivArgs> *d_vdiv = nullptr;    // [block]
+  mgpu::Batch<mgpu::FilterArgs> *d_filt = nullptr;  // [cur][block]
+  mgpu::Batch<mgpu::StageArgs> *d_tune = nullptr;   // [block], autotuner
+  int tab_cap = 0;                                  // blocks the tables hold
+  int nt_stage[2] = {0, 0}, nt_visc = 0, nt_vdiv = 0, nt_filt = 0;
+  bool tables_ready = false;
+  // one captured step per U^n set (a case without filter rotates the sets)
+  bool use_graph = true;
+  hipGraphExec_t gexec[3] = {};
+  bool mixed = false;  // FP32 flux/derivative/filter arithmetic
+};
+
+namespace mgpu_host {
+
+extern thread_local std::string g_create_error;
+
+// Records an error message on the handle (or the thread's create error) and
+// returns `code`.
+int fail(mgpu_handle *h, int code, const char *fmt, ...);
+
+// Validates the flags of a case (not its blocks).  Empty string if valid.
+std::string check_flags(const mgpu_case &c);
+// Empty handle with the flags of c on the current device.
+int handle_new(const mgpu_case &c, mgpu_handle **out);
+// Appends a block (extents of bl, scalars of p) and allocates its arrays.
+int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p);
+// Frees block i and removes it from the handle (the other blocks keep their order).
+int handle_remove_block(mgpu_handle *h, int i);
+// Device bytes a block of extents nx,ny,nz needs under the flags of c.
+size_t block_bytes(const mgpu_case &c, int nx, int ny, int nz);
+// Uploads metrics / state of block i (host arrays in the padded layout).
+int put_metrics_cart(mgpu_handle *h, int i, const double *const *a6);
+int put_metrics_c3(mgpu_handle *h, int i, const double *const *m, const double *const *mv);
+int put_state(mgpu_handle *h, int i, const double *const *u5);
+int get_state(mgpu_handle *h, int i, double *const *u5);
+// Advances every block of the handle by nsteps (asynchronous on h->stream).
+int step(mgpu_handle *h, int nsteps);
+void destroy(mgpu_handle *h);
+
+}  // namespace mgpu_host
