@@ -24,6 +24,14 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
+# HBM bytes per k_stage launch on the 512x256x128 curvilinear block (tuned
+# 64x8 tile, k-chunk 128), averaged over the 4 stages (3 Euler launches + the
+# viscous last stage): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch,
+# FETCH_SIZE scaled by 2.000 and WRITE_SIZE by 1.000 (their readings for a
+# 1 GiB device copy).  Summary: profiles/step_kernels_c3_512x256x128_fp64.md.
+STAGE_TRAFFIC_BYTES = (3 * 4.497e9 + 5.222e9) / 4
+STAGE_TRAFFIC_WORKLOAD = (512, 256, 128)
+
 
 def stage_words_per_cell(case) -> float:
     """Algorithmic FP64 words per interior cell of one stage-kernel launch,
@@ -54,6 +62,20 @@ def cpu_baseline(case, nsteps: int):
             "kind": "port",
             "sample": f"{nsteps} full RK steps of the same 512x256x128 block, oracle/musica_oracle.c, "
                       f"OpenMP over k, {t:.2f} s"}
+
+
+def copy_calibration(nbytes: int = 1 << 30, reps: int = 5) -> int:
+    """A plain device-to-device copy of nbytes, reps times: under rocprofv3
+    --pmc FETCH_SIZE / WRITE_SIZE it gives the counter's reading for a known
+    number of bytes read and written per dispatch."""
+    import torch
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "dispatches": reps}), flush=True)
+    return 0
 
 
 def run_leg(case, precision: int, steps: int, warmup: int, tune: bool):
@@ -95,7 +117,11 @@ def main() -> int:
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--ny", type=int, default=256)
     ap.add_argument("--nz", type=int, default=128)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of known size (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         return 2
@@ -132,7 +158,11 @@ def main() -> int:
                    "cells": case.ncells, "nrk": case.nrk, "precision": "fp64",
                    "stage_tiles": {k: {"tile": f"{v[0]}x{v[1]}", "kchunk": v[2]} for k, v in fp["tuned"].items()}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": (STAGE_TRAFFIC_BYTES if (args.nx, args.ny, args.nz) == STAGE_TRAFFIC_WORKLOAD
+                                 else None),
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, "
+                                       "profiles/step_kernels_c3_512x256x128_fp64.md",
                      "kernel": "k_stage", "avg_launch_ms": round(avg_stage_s * 1e3, 4),
                      "algorithmic_bytes_per_launch": bytes_launch},
         "kernel_ms": {k: {"launches": n, "ms": round(ms, 3)} for k, (n, ms) in fp["times"].items()},
