@@ -105,6 +105,58 @@ def run_leg(case, precision: int, steps: int, warmup: int, tune: bool):
     return {"t": t, "times": times, "tuned": tuned, "rho": rho}
 
 
+def multiblock_leg(steps: int, warmup: int):
+    """The uneven 7-block workload (384x192x96, 2x256x128x96, 3x128x128x64,
+    96^3): one batched launch per kernel and stage for all blocks, steps
+    replayed from a captured HIP graph, autotuned tiles."""
+    from musica2_newgrid_amd import _lib, multiblock as mb
+    from musica2_newgrid_amd.solver import GpuSolver
+
+    flags, blocks, bcs = mb.uneven_islands()
+    cells = sum(bc.ncells for bc in bcs)
+    with GpuSolver(flags, topology=blocks, block_cases=bcs) as s:
+        s.autotune()
+        s.step(warmup)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        t = time.perf_counter() - t0
+        # per-kernel split from a separate, eagerly launched and event-timed pass
+        s.profile(True)
+        s.step(2)
+        times = {_lib.K_NAMES[k]: s.kernel_time(k) for k in _lib.K_NAMES}
+        s.profile(False)
+    return {"value": round(cells * flags.nrk * steps / t / 1e6, 2), "unit": "Mcell-RK-stages/s",
+            "ms_per_step": round(t / steps * 1e3, 4), "steps": steps, "blocks": len(blocks), "cells": cells,
+            "workload": "7 curvilinear blocks 384x192x96, 2x256x128x96, 3x128x128x64, 96^3 in 4 periodic "
+                        "islands of 1-to-1 interfaces (all orientation kinds), M=0.3 + 1e-3 pressure pulse, "
+                        "viscous, selective filter, RK4, FP64, graph replay",
+            "kernel_ms_per_step_eager": {k: round(ms / 2, 4) for k, (n, ms) in times.items()}}
+
+
+def serve_leg(nreq: int, nsteps: int, seed: int = 5):
+    """Serving loop: a Poisson stream of TGV / isentropic-vortex requests on
+    cubes of edge 64..256, Re log-uniform in [400, 4000], FP64 or mixed,
+    admitted by an HBM budget and co-batched."""
+    import numpy as np
+
+    from musica2_newgrid_amd import serving
+    edges = (64, 96, 128, 192, 256)
+    mean_cells = sum(e ** 3 for e in edges) / len(edges)
+    # arrival rate for about 80 % of the device: service time of the mean
+    # request at ~9e9 cell-stages/s
+    rate = 0.8 / (mean_cells * 4 * nsteps / 9.0e9)
+    reqs = serving.make_requests(nreq, seed=seed, edges=edges, nsteps=nsteps, rate=rate)
+    budget = 64 << 30
+    lat, wall, work = serving.run_stream(reqs, budget_bytes=budget, quantum=10)
+    p50, p99 = np.percentile(lat, [50, 99])
+    return {"requests": nreq, "steps_per_request": nsteps, "arrival_rate_per_s": round(rate, 3),
+            "hbm_budget_GiB": budget >> 30, "wall_s": round(wall, 3),
+            "throughput": round(work / wall / 1e6, 2), "unit": "Mcell-RK-stages/s",
+            "p50_ms": round(p50 * 1e3, 2), "p99_ms": round(p99 * 1e3, 2), "seed": seed}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +169,10 @@ def main() -> int:
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--ny", type=int, default=256)
     ap.add_argument("--nz", type=int, default=128)
+    ap.add_argument("--no-multiblock", action="store_true", help="skip the uneven 7-block leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-steps", type=int, default=100)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (HBM counter calibration)")
     args = ap.parse_args()
@@ -182,6 +238,12 @@ def main() -> int:
             "after_steps": args.warmup + args.steps,
             "kernel_ms": {k: {"launches": n, "ms": round(ms, 3)} for k, (n, ms) in mx["times"].items()},
         }
+    if not args.no_multiblock:
+        print("bench: uneven 7-block leg", file=sys.stderr, flush=True)
+        out["multiblock"] = multiblock_leg(args.steps, args.warmup)
+    if not args.no_serve:
+        print("bench: serving leg", file=sys.stderr, flush=True)
+        out["serving"] = serve_leg(args.serve_requests, args.serve_steps)
     if not args.no_cpu:
         print("bench: CPU baseline leg", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)

@@ -368,22 +368,51 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
     }
     vflux<CURV>(t, mx, my, mz, f);
   };
+  // the same from tau values already in registers (all nine components)
+  auto flux_of = [&](int d, const double *t9, idx_t p, double f[4]) {
+    double t[NL];
+#pragma unroll
+    for (int c = 0; c < NL; c++) t[c] = t9[CURV ? c : kRow[d][c]];
+    double mx = 0.0, my = 0.0, mz = 0.0;
+    if constexpr (CURV) {
+      mx = ldg(mv[3 * d], p);
+      my = ldg(mv[3 * d + 1], p);
+      mz = ldg(mv[3 * d + 2], p);
+    }
+    vflux<CURV>(t, mx, my, mz, f);
+  };
 
-  double q[5][4];  // H^v at k-2..k+2
+  // own column: H^v at k-2..k+2, and tau of planes k, k+1, k+2 (each plane's
+  // tau is loaded once, at k+2, and reused for the i/j pencils two planes on)
+  double q[5][4];
+  double tw[3][NTAU];
   if (own) {
 #pragma unroll
-    for (int l = 0; l < 4; l++) flux_at(2, own_off + (idx_t)(kbeg - 2 + l) * sz, q[l]);
+    for (int l = 0; l < 2; l++) flux_at(2, own_off + (idx_t)(kbeg - 2 + l) * sz, q[l]);
+#pragma unroll
+    for (int l = 0; l < 2; l++) {
+      const idx_t p = own_off + (idx_t)(kbeg + l) * sz;
+#pragma unroll
+      for (int c = 0; c < NTAU; c++) tw[l][c] = ldg(a.tau[c], p);
+      flux_of(2, tw[l], p, q[2 + l]);
+    }
   }
   for (int k = kbeg; k <= kend; k++) {
     const idx_t pk = own_off + (idx_t)k * sz;
     double fx[4], fy[4], fh[NS][4];
     double ijv = 0.0;
     if (own) {
-      flux_at(2, pk + 2 * sz, q[4]);
+      const idx_t p2 = pk + 2 * sz;
+#pragma unroll
+      for (int c = 0; c < NTAU; c++) tw[2][c] = ldg(a.tau[c], p2);
+      flux_of(2, tw[2], p2, q[4]);
       if constexpr (CURV) ijv = ldg(mv[9], pk);
+      flux_of(0, tw[0], pk, fx);
+      flux_of(1, tw[0], pk, fy);
+    } else {
+      if (needx) flux_at(0, pk, fx);
+      if (needy) flux_at(1, pk, fy);
     }
-    if (needx) flux_at(0, pk, fx);
-    if (needy) flux_at(1, pk, fy);
 #pragma unroll
     for (int s = 0; s < NS; s++)
       if (hs_on[s]) flux_at(hs_x[s] ? 0 : 1, hs_off[s] + (idx_t)k * sz, fh[s]);
@@ -430,6 +459,10 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
     for (int l = 0; l < 4; l++)
 #pragma unroll
       for (int c = 0; c < 4; c++) q[l][c] = q[l + 1][c];
+#pragma unroll
+    for (int l = 0; l < 2; l++)
+#pragma unroll
+      for (int c = 0; c < NTAU; c++) tw[l][c] = tw[l + 1][c];
   }
 }
 

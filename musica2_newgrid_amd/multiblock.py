@@ -182,3 +182,103 @@ def check_table(blocks) -> None:
             other = blocks[nb]
             assert other["nbr_block"][nf] == b and other["nbr_face"][nf] == f, (b, f)
             assert other["orient"][nf] == inverse_orient(o), (b, f, o, other["orient"][nf])
+
+
+def _random_mapping(lengths, h, seed, amp_cells=0.02, modes=3):
+    """Smooth periodic distortion: each coordinate displaced by a sum of
+    low-mode sines of the other two, amplitude amp_cells*h, random phases."""
+    rng = np.random.default_rng(seed)
+    Lx, Ly, Lz = lengths
+    tp = 2.0 * np.pi
+    terms = []
+    for _ in range(3):
+        t = []
+        for _ in range(modes):
+            m1, m2 = rng.integers(1, 3, size=2)
+            ph1, ph2 = rng.uniform(0, tp, size=2)
+            t.append((m1, m2, ph1, ph2, rng.uniform(0.5, 1.0) / modes))
+        terms.append(t)
+    a = amp_cells * h
+
+    def f(XI, ET, ZE):
+        out = []
+        base = (XI, ET, ZE)
+        others = ((ET / Ly, ZE / Lz), (XI / Lx, ZE / Lz), (XI / Lx, ET / Ly))
+        for c in range(3):
+            u, v = others[c]
+            d = 0.0
+            for m1, m2, ph1, ph2, w in terms[c]:
+                d = d + w * np.sin(tp * m1 * u + ph1) * np.sin(tp * m2 * v + ph2)
+            out.append(base[c] + a * d)
+        return tuple(out)
+
+    return f
+
+
+def pulse_case(nx, ny, nz, h, seed, mach=0.3, amp=1e-3, width_cells=4.0, filter=1, iorder_visc=4,
+               T_inf=300.0, rho_inf=1.177, re_cell=50.0, cfl=0.8):
+    """Periodic curvilinear block (random smooth distortion of 2 % of a
+    cell) with a freestream at `mach` along x and a Gaussian pressure pulse
+    of relative amplitude `amp` and half-width `width_cells` cells at its
+    centre (ar = -ln2/(b dx)^2, the pulse of mod_init_flow.f90:2270)."""
+    import math
+
+    from . import cases
+    from . import grid as gr
+    lengths = (nx * h, ny * h, nz * h)
+    m, mv, (xc, yc, zc) = gr.curvilinear_metrics_periodic(_random_mapping(lengths, h, seed), nx, ny, nz, lengths,
+                                                            viscous=iorder_visc != 0)
+    c_inf = math.sqrt(cases.GAM * cases.RG * T_inf)
+    u_inf = mach * c_inf
+    p_inf = rho_inf * cases.RG * T_inf
+    xm, ym, zm = (L / 2.0 for L in lengths)
+    ar = -math.log(2.0) / (width_cells * h) ** 2
+    p = p_inf * (1.0 + amp * np.exp(ar * ((xc - xm) ** 2 + (yc - ym) ** 2 + (zc - zm) ** 2)))
+    rho = p / (cases.RG * T_inf)
+    u = np.full_like(p, u_inf)
+    v = np.zeros_like(p)
+    w = np.zeros_like(p)
+    U = cases._to_padded(cases.conserved(rho, u, v, w, p), nx, ny, nz)
+    # viscosity from a cell Reynolds number rho u h / mu = re_cell
+    diffscale = (rho_inf * u_inf * h / re_cell) / float(cases.sutherland(np.float64(T_inf)))
+    cs = cases.Case("pulse", nx, ny, nz, curvilinear=True, iorder_visc=iorder_visc, filter=filter, ducros=True,
+                    diffscale=diffscale, m=m, mv=mv, U=U)
+    cs.dt = cases.timestep(U, h, cfl)
+    cs.info = dict(mach=mach, amp=amp, width_cells=width_cells, seed=seed)
+    return cs
+
+
+# Uneven multiblock workload: 7 blocks of the sizes 384x192x96,
+# 2x(256x128x96), 3x(128x128x64), 96^3.  A 1-to-1 interface joins faces of
+# equal extent, and only blocks of equal size have such faces here, so the
+# blocks form four periodic islands: A alone, B1-B2 (a ring along i, B2
+# reversed), C1-C2-C3 (a ring whose blocks' axes are permuted and reversed,
+# joining i-faces to j-faces), D alone (axes permuted).
+ISLANDS = [
+    ((384, 192, 96), 1, [((0, 1, 2), (1, 1, 1))]),
+    ((512, 128, 96), 2, [((0, 1, 2), (1, 1, 1)), ((0, 1, 2), (-1, 1, -1))]),
+    ((384, 128, 64), 3, [((0, 1, 2), (1, 1, 1)), ((1, 0, 2), (1, -1, 1)), ((1, 0, 2), (-1, 1, -1))]),
+    ((96, 96, 96), 1, [((2, 0, 1), (1, -1, 1))]),
+]
+
+
+def uneven_islands(seed: int = 20261202, scale: int = 1, filter: int = 1):
+    """The 7-block workload (sizes divided by `scale`), every block with the
+    same nominal spacing h and one time step.  Returns (flags case, block
+    table, block cases)."""
+    h = 0.01 * scale
+    blocks, bcases = [], []
+    for q, (ext, parts, orients) in enumerate(ISLANDS):
+        nx, ny, nz = (e // scale for e in ext)
+        parent = pulse_case(nx, ny, nz, h, seed + q, filter=filter)
+        tab, bcs = split_oriented(parent, parts, orients)
+        off = len(blocks)
+        for blk in tab:
+            blk["nbr_block"] = [b + off for b in blk["nbr_block"]]
+        blocks += tab
+        bcases += bcs
+    flags = copy.copy(bcases[0])
+    flags.dt = min(bc.dt for bc in bcases)
+    for bc in bcases:
+        bc.dt = flags.dt
+    return flags, blocks, bcases

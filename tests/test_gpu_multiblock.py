@@ -84,3 +84,20 @@ def test_graph_replays_match_eager_launches():
                 for a, b in zip(ue, ug):
                     for x, y in zip(a, b):
                         assert np.array_equal(x[5:-5, 5:-5, 5:-5], y[5:-5, 5:-5, 5:-5]), rep
+
+
+def test_uneven_seven_blocks_match_oracle():
+    """The uneven 7-block workload, scaled down 8x (48x24x12 ... 12^3)."""
+    from musica2_newgrid_amd.solver import GpuSolver
+    from oracle import oracle as orc
+    flags, blocks, bcs = mb.uneven_islands(scale=8)
+    print(f"uneven islands: {len(blocks)} blocks, 10 steps", flush=True)
+    with GpuSolver(flags, topology=blocks, block_cases=bcs) as s:
+        s.autotune()
+        s.step(10)
+        Ug = s.get_states()
+    Uc = [bc.copy_state() for bc in bcs]
+    orc.step_multi(flags, blocks, bcs, Uc, 10)
+    e = max(relerr(a, b) for a, b in zip(Ug, Uc))
+    print("relerr", e)
+    assert e < 1e-12

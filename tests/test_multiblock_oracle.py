@@ -64,3 +64,24 @@ def test_c3_split_with_permuted_axes_equals_parent(filt):
     B = mb.join_oriented(c, Us, C3_ORIENTS)
     # permuted axes sum the three directions in another order: round-off only
     assert _rel(A, B) < 1e-12
+
+
+def test_uneven_islands_equal_their_parents():
+    """The 7-block workload (scaled down 8x): each island advanced as split
+    blocks equals its parent advanced as one block."""
+    flags, blocks, bcs = mb.uneven_islands(scale=8)
+    Us = [bc.copy_state() for bc in bcs]
+    orc.step_multi(flags, blocks, bcs, Us, 2)
+    first = 0
+    for q, (ext, parts, orients) in enumerate(mb.ISLANDS):
+        nx, ny, nz = (e // 8 for e in ext)
+        parent = mb.pulse_case(nx, ny, nz, 0.08, 20261202 + q, filter=1)
+        parent.dt = flags.dt
+        A = parent.copy_state()
+        orc.Oracle(parent).step(A, 2)
+        B = mb.join_oriented(parent, Us[first:first + parts], orients)
+        # compare the change over the two steps (the state is a 1e-3 pulse on a uniform field)
+        for a, b, a0 in zip(A, B, parent.U):
+            da, db = gr.interior(a) - gr.interior(a0), gr.interior(b) - gr.interior(a0)
+            assert np.max(np.abs(da - db)) <= 1e-11 * np.max(np.abs(da)) + 1e-14 * np.max(np.abs(gr.interior(a0)))
+        first += parts
