@@ -36,6 +36,7 @@ EXPORTS = (
     "mgpu_profile",
     "mgpu_kernel_time",
     "mgpu_last_error",
+    "mgpu_diagnostics",
     "mgpu_destroy",
     "mgpu_server_create",
     "mgpu_server_submit",
@@ -81,6 +82,10 @@ class MgpuCase(C.Structure):
     ]
 
 
+class MgpuDiag(C.Structure):
+    _fields_ = [("mass", C.c_double), ("kinetic", C.c_double), ("enstrophy", C.c_double), ("cells", C.c_int64)]
+
+
 class MgpuError(RuntimeError):
     pass
 
@@ -114,6 +119,8 @@ def lib() -> C.CDLL:
     L.mgpu_last_error.restype = C.c_char_p
     L.mgpu_destroy.argtypes = [C.c_void_p]
     L.mgpu_destroy.restype = None
+    L.mgpu_diagnostics.argtypes = [C.c_void_p, C.POINTER(MgpuDiag)]
+    L.mgpu_diagnostics.restype = C.c_int
     L.mgpu_server_create.argtypes = [C.c_int64, C.POINTER(C.c_void_p)]
     L.mgpu_server_submit.argtypes = [C.c_void_p, C.POINTER(MgpuCase), C.POINTER(_dp), C.POINTER(_dp), C.c_int,
                                      C.POINTER(C.c_int64)]

@@ -1,0 +1,103 @@
+// Diagnostics of the state: sums over the interior of rho, 1/2 rho |u|^2 and
+// 1/2 rho |curl u|^2, the quantities of stats_tgv
+// (src/File_IO/stats.f90:37-112), with the velocity gradients by the 5-pt
+// scheme of grad_vel (smod_grad_vel.f90, c3 mapping smod_grad_vel_c3.f90:
+// 260-300).  Not on the time-step path: one thread per point, neighbours
+// read straight from global memory; each workgroup writes its three partial
+// sums, which the host adds in a fixed order (deterministic results).
+#include "mgpu_internal.h"
+
+namespace mgpu {
+
+namespace {
+
+__device__ __forceinline__ void vel(const CState &U, idx_t p, double o[3]) {
+  const double r = ldg(U.v[0], p);
+  const double vi = 1.0 / r;
+  o[0] = ldg(U.v[1], p) * vi;
+  o[1] = ldg(U.v[2], p) * vi;
+  o[2] = ldg(U.v[3], p) * vi;
+}
+
+template <bool CURV>
+__global__ __launch_bounds__(kDiagThreads) void k_diag(DiagArgs a) {
+  __shared__ double red[3][kDiagThreads];
+  const Geom &g = a.g;
+  const long t = (long)blockIdx.x * kDiagThreads + threadIdx.x;
+  const long cnt = (long)g.nx * g.ny * g.nz;
+  double mg = 0.0, ek = 0.0, wr = 0.0;
+  if (t < cnt) {
+    const int i = (int)(t % g.nx) + 1;
+    const long r = t / g.nx;
+    const int j = (int)(r % g.ny) + 1;
+    const int k = (int)(r / g.ny) + 1;
+    const idx_t p = lin(g, i, j, k);
+    const idx_t st[3] = {1, (idx_t)g.sx, g.sy};
+    double gd[3][3];  // gd[dir][u,v,w]
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      double a1[3], b1[3], a2[3], b2[3];
+      vel(a.u, p + st[d], a1);
+      vel(a.u, p - st[d], b1);
+      vel(a.u, p + 2 * st[d], a2);
+      vel(a.u, p - 2 * st[d], b2);
+#pragma unroll
+      for (int c = 0; c < 3; c++) gd[d][c] = A5_1 * (a1[c] - b1[c]) + A5_2 * (a2[c] - b2[c]);
+    }
+    double gr[3][3];  // gr[u,v,w][x,y,z]
+    if constexpr (CURV) {
+      double m[10];
+#pragma unroll
+      for (int q = 0; q < 10; q++) m[q] = ldg(a.met.mv[q], p);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double dvx = gd[0][c] * m[0] + gd[1][c] * m[3] + gd[2][c] * m[6];
+        const double dvy = gd[0][c] * m[1] + gd[1][c] * m[4] + gd[2][c] * m[7];
+        const double dvz = gd[0][c] * m[2] + gd[1][c] * m[5] + gd[2][c] * m[8];
+        gr[c][0] = dvx * m[9];
+        gr[c][1] = dvy * m[9];
+        gr[c][2] = dvz * m[9];
+      }
+    } else {
+      const double ix = a.met.idx_v[i - 1 + G], iy = a.met.idy_v[j - 1 + G], iz = a.met.idz_v[k - 1 + G];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        gr[c][0] = gd[0][c] * ix;
+        gr[c][1] = gd[1][c] * iy;
+        gr[c][2] = gd[2][c] * iz;
+      }
+    }
+    double u[3];
+    vel(a.u, p, u);
+    const double rho = ldg(a.u.v[0], p);
+    mg = rho;
+    ek = 0.5 * rho * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const double o1 = gr[2][1] - gr[1][2], o2 = gr[0][2] - gr[2][0], o3 = gr[1][0] - gr[0][1];
+    wr = 0.5 * rho * (o1 * o1 + o2 * o2 + o3 * o3);
+  }
+  red[0][threadIdx.x] = mg;
+  red[1][threadIdx.x] = ek;
+  red[2][threadIdx.x] = wr;
+  __syncthreads();
+  for (int s = kDiagThreads / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) a.partial[3 * (long)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+}  // namespace
+
+long diag_groups(const Geom &g) { return ((long)g.nx * g.ny * g.nz + kDiagThreads - 1) / kDiagThreads; }
+
+hipError_t launch_diag(const DiagArgs &a, bool curv, hipStream_t s) {
+  const long nb = diag_groups(a.g);
+  if (curv) hipLaunchKernelGGL(k_diag<true>, dim3((unsigned)nb), dim3(kDiagThreads), 0, s, a);
+  else hipLaunchKernelGGL(k_diag<false>, dim3((unsigned)nb), dim3(kDiagThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace mgpu

@@ -868,8 +868,49 @@ void destroy(mgpu_handle *h) {
   for (auto &bd : h->b) free_block(bd);
   free_exchange(h);
   free_tables(h);
+  if (h->diag_buf) (void)hipFree(h->diag_buf);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
+}
+
+int diagnostics(mgpu_handle *h, mgpu_diag *out) {
+  for (size_t bI = 0; bI < h->b.size(); bI++) {
+    if (!h->b[bI].have_metrics) return fail(h, MGPU_ESTATE, "block %zu: metrics not set", bI);
+    if (!h->b[bI].have_state) return fail(h, MGPU_ESTATE, "block %zu: state not set", bI);
+  }
+  int rc = refresh_ghosts(h);
+  if (rc) return rc;
+  double sum[3] = {0.0, 0.0, 0.0};
+  int64_t cells = 0;
+  std::vector<double> part;
+  for (auto &bd : h->b) {
+    const long ng = diag_groups(bd.g);
+    if (3 * ng > h->diag_cap) {
+      HIPCK(h, hipStreamSynchronize(h->stream));
+      if (h->diag_buf) (void)hipFree(h->diag_buf);
+      h->diag_buf = nullptr;
+      h->diag_cap = 0;
+      HIPCK(h, hipMalloc((void **)&h->diag_buf, sizeof(double) * 3 * ng));
+      h->diag_cap = 3 * ng;
+    }
+    DiagArgs da{};
+    da.g = bd.g;
+    for (int v = 0; v < 5; v++) da.u.v[v] = bd.st[h->cur][v];
+    da.met = metrics_of(bd);
+    da.partial = h->diag_buf;
+    HIPCK(h, launch_diag(da, h->c.curvilinear != 0, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    part.resize(3 * ng);
+    HIPCK(h, hipMemcpy(part.data(), h->diag_buf, sizeof(double) * 3 * ng, hipMemcpyDeviceToHost));
+    for (long w = 0; w < ng; w++)
+      for (int q = 0; q < 3; q++) sum[q] += part[3 * w + q];
+    cells += (int64_t)bd.g.nx * bd.g.ny * bd.g.nz;
+  }
+  out->mass = sum[0];
+  out->kinetic = sum[1];
+  out->enstrophy = sum[2];
+  out->cells = cells;
+  return 0;
 }
 
 }  // namespace mgpu_host
@@ -1069,6 +1110,12 @@ int mgpu_kernel_time(mgpu_handle *h, int kind, int64_t *launches, double *ms) {
 const char *mgpu_last_error(const mgpu_handle *h) {
   if (!h) return g_create_error.c_str();
   return h->err.c_str();
+}
+
+int mgpu_diagnostics(mgpu_handle *h, mgpu_diag *out) {
+  if (!h) return MGPU_EINVAL;
+  if (!out) return fail(h, MGPU_EINVAL, "null argument");
+  return diagnostics(h, out);
 }
 
 void mgpu_destroy(mgpu_handle *h) { destroy(h); }

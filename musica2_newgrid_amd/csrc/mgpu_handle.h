@@ -75,6 +75,8 @@ struct mgpu_handle {
   bool use_graph = true;
   hipGraphExec_t gexec[3] = {};
   bool mixed = false;  // FP32 flux/derivative/filter arithmetic
+  double *diag_buf = nullptr;  // per-workgroup partial sums of mgpu_diagnostics
+  long diag_cap = 0;
 };
 
 namespace mgpu_host {

@@ -210,6 +210,15 @@ struct IfaceArgs {
   int nvar;
 };
 
+// Diagnostics of one block: per-workgroup partial sums (3 per group).
+constexpr int kDiagThreads = 256;
+struct DiagArgs {
+  Geom g;
+  CState u;
+  Metrics met;
+  double *partial;
+};
+
 }  // namespace mgpu
 
 // Kernel launchers (defined in the .hip files).  Each takes a device table
@@ -228,4 +237,6 @@ TileRange filter_tiles(const Geom &g);
 hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, hipStream_t s);
 hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, hipStream_t s);
 hipError_t launch_iface(const IfaceArgs &a, int nchunks, hipStream_t s);
+long diag_groups(const Geom &g);
+hipError_t launch_diag(const DiagArgs &a, bool curv, hipStream_t s);
 }  // namespace mgpu

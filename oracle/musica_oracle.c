@@ -742,3 +742,38 @@ out:
   }
   return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* TGV statistics of the state U (stats_tgv, src/File_IO/stats.f90:37-112): */
+/* sums over the interior of rho, 1/2 rho |u|^2 and 1/2 rho |curl u|^2, the */
+/* velocity gradients by the 5-pt scheme of grad_vel (with the viscous      */
+/* metrics).  The reference sums gradients its last stage computed; here    */
+/* they are those of U itself, whose ghosts must be valid.                  */
+/* ------------------------------------------------------------------------ */
+int orc_diag(const orc_cfg *c, const orc_grid *g, double *const U[5], double out[3]) {
+  int rc = check_cfg(c);
+  if (rc) return rc;
+  dims_t d = mkdims(c);
+  work_t w;
+  memset(&w, 0, sizeof w);
+  if (work_alloc(&d, &w, 1)) return -10;
+  prim_euler(c, &d, U, &w);
+  gradients(c, g, &d, &w);
+  double mg = 0.0, ek = 0.0, wr = 0.0;
+  for (int k = 1; k <= d.nz; k++)
+    for (int j = 1; j <= d.ny; j++)
+      for (int i = 1; i <= d.nx; i++) {
+        long p = (long)ID(d, i, j, k);
+        double r = U[0][p];
+        double *const *gr = w.grad;
+        mg += r;
+        ek += 0.5 * r * (w.uu[p] * w.uu[p] + w.vv[p] * w.vv[p] + w.ww[p] * w.ww[p]);
+        double o1 = gr[DWY][p] - gr[DVZ][p], o2 = gr[DUZ][p] - gr[DWX][p], o3 = gr[DVX][p] - gr[DUY][p];
+        wr += 0.5 * r * (o1 * o1 + o2 * o2 + o3 * o3);
+      }
+  out[0] = mg;
+  out[1] = ek;
+  out[2] = wr;
+  work_free(&w);
+  return 0;
+}

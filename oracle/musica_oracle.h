@@ -111,6 +111,11 @@ void orc_filter_rhs(const orc_cfg *c, const orc_grid *g, double *const Un[5],
  * (src/Compute_flow/runge_kutta.f90:2-152).  Returns 0 on success. */
 int orc_step(const orc_cfg *c, const orc_grid *g, double *U[5], int nsteps);
 
+/* Sums over the interior of rho, 1/2 rho|u|^2 and 1/2 rho|curl u|^2 (5-pt
+ * velocity gradients) of U, whose ghosts must be valid: the quantities of
+ * stats_tgv (src/File_IO/stats.f90:37-112).  Returns 0 on success. */
+int orc_diag(const orc_cfg *c, const orc_grid *g, double *const U[5], double out[3]);
+
 /* Number of OpenMP threads the oracle uses. */
 int orc_num_threads(void);
 

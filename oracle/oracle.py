@@ -78,6 +78,8 @@ def lib():
         L.orc_step_multi.restype = C.c_int
         L.orc_fill_interfaces.argtypes = [C.c_int, C.POINTER(OrcBlock), C.POINTER(_dp)]
         L.orc_fill_interfaces.restype = None
+        L.orc_diag.argtypes = [C.POINTER(OrcCfg), C.POINTER(OrcGrid), C.POINTER(_dp), _dp]
+        L.orc_diag.restype = C.c_int
         _lib = L
     return _lib
 
@@ -147,6 +149,14 @@ class Oracle:
             K = [np.zeros_like(U[0]) for _ in range(5)]
         lib().orc_visc_rhs(C.byref(self.cfg), C.byref(self.grid), self._tab(U), self._tab(K))
         return K
+
+    def diag(self, U):
+        """(sum rho, sum 1/2 rho|u|^2, sum 1/2 rho|curl u|^2) over the interior."""
+        out = np.zeros(3)
+        rc = lib().orc_diag(C.byref(self.cfg), C.byref(self.grid), self._tab(U), _p(out))
+        if rc != 0:
+            raise RuntimeError(f"orc_diag failed rc={rc}")
+        return out
 
     def filter_rhs(self, Un, Uref=None):
         K = [np.zeros_like(Un[0]) for _ in range(5)]

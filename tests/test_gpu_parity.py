@@ -122,3 +122,24 @@ def test_full_size_parity_and_freestream():
     e = relerr(U, U0)
     print("free-stream relative change", e)
     assert e < 1e-12
+
+
+@pytest.mark.parametrize("which", ["tgv", "vortex_c3"])
+def test_diagnostics_match_oracle(which):
+    """mgpu_diagnostics (stats_tgv sums) against the oracle, before and after steps."""
+    from musica2_newgrid_amd.solver import GpuSolver
+    case = cases.tgv(32) if which == "tgv" else cases.vortex_wavy(40, 28, 20, filter=2)
+    print(f"diagnostics {which}", flush=True)
+    with GpuSolver(case) as s:
+        d0 = s.diagnostics()
+        s.step(5)
+        d1 = s.diagnostics()
+        U1 = s.get_state()
+    o = _oracle(case)
+    for d, U in ((d0, case.U), (d1, U1)):
+        ref = o.diag(U)
+        got = np.array([d["mass"], d["kinetic"], d["enstrophy"]])
+        e = np.max(np.abs(got - ref) / np.abs(ref))
+        print("diag relerr", e, got)
+        assert e < 1e-12
+        assert d["cells"] == case.ncells

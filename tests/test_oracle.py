@@ -259,3 +259,22 @@ def test_ducros_sensor_switches_off_low_order_term_in_rotation():
     d_noduc = max(float(np.max(np.abs(gr.interior(a) - gr.interior(b)))) for a, b in zip(K_noduc, K_lin))
     assert d_noduc > 1.0          # the pressure jump triggers the low-order term
     assert d_duc < 1e-6 * d_noduc  # theta ~ div^2/(div^2+|w|^2) ~ 0 in rotation
+
+
+def test_tgv_statistics_closed_forms():
+    """stats_tgv sums on the initial TGV field (uniform rho0): mass = rho0 N,
+    Ek = rho0 U0^2 N / 8, enstrophy = (3/8) rho0 U0^2 kappa^2 N with kappa the
+    5-pt scheme's symbol of d/dx on the fundamental mode, (4/3 sin h -
+    1/6 sin 2h)/h.  A wrong curl component, sign or coefficient fails."""
+    import math
+    n = 24
+    c = cases.tgv(n)
+    mg, ek, wr = Oracle(c).diag(c.U)
+    N = n ** 3
+    rho0 = float(gr.interior(c.U[0])[0, 0, 0])
+    U0 = c.info["u_ref"]
+    h = c.info["dx"]
+    kappa = (4.0 / 3.0 * math.sin(h) - 1.0 / 6.0 * math.sin(2 * h)) / h
+    assert abs(mg - rho0 * N) <= 1e-12 * rho0 * N  # rho is uniform to a few ulp
+    assert abs(ek - rho0 * U0 ** 2 * N / 8) <= 1e-12 * ek
+    assert abs(wr - 3.0 / 8.0 * rho0 * U0 ** 2 * kappa ** 2 * N) <= 1e-11 * wr
