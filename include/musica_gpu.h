@@ -171,14 +171,16 @@ int mgpu_kernel_time(mgpu_handle *h, int kind, int64_t *launches, double *ms);
 const char *mgpu_last_error(const mgpu_handle *h);
 
 /* Release every device resource of the handle. */
-/* Statistics of the current state U^n summed over the interior cells of all
+/* Statistics of the current state U^n over the interior cells of all
  * blocks: mass sum(rho), kinetic energy sum(1/2 rho |u|^2) and enstrophy
  * sum(1/2 rho |curl u|^2) with the 5-pt velocity gradients -- the sums of
  * stats_tgv (src/File_IO/stats.f90:37-112), before its normalisation and
- * MPI_REDUCE.  (stats_tgv takes the gradients its last stage computed; these
- * are the current state's.)  Deterministic: fixed summation order. */
+ * MPI_REDUCE (stats_tgv takes the gradients its last stage computed; these
+ * are the current state's) -- and umax = max(|u| + c), the velocity scale
+ * of timestep_new (src/Time/timestep_new.f90:270-288: deltat =
+ * CFL*delta_min/umax).  Deterministic: fixed summation order. */
 typedef struct mgpu_diag {
-  double mass, kinetic, enstrophy;
+  double mass, kinetic, enstrophy, umax;
   int64_t cells;
 } mgpu_diag;
 int mgpu_diagnostics(mgpu_handle *h, mgpu_diag *out);

@@ -83,7 +83,8 @@ class MgpuCase(C.Structure):
 
 
 class MgpuDiag(C.Structure):
-    _fields_ = [("mass", C.c_double), ("kinetic", C.c_double), ("enstrophy", C.c_double), ("cells", C.c_int64)]
+    _fields_ = [("mass", C.c_double), ("kinetic", C.c_double), ("enstrophy", C.c_double), ("umax", C.c_double),
+                ("cells", C.c_int64)]
 
 
 class MgpuError(RuntimeError):

@@ -34,6 +34,25 @@ C0 = 1.4579326545176254e-06
 S0 = 110.4
 # low-storage RK4 (init_RK nrk=4, src/Time/mod_time.f90:165-169)
 CRK4 = (0.25, 1.0 / 3.0, 0.5, 1.0)
+# 6-stage low-storage DRP RK (init_RK nrk=6, src/Time/mod_time.f90:253-294; Bogey & Bailly, JCP 2004):
+# the amplification polynomial is sum_m g_m z^m with these g_m; crk(6) = g1, crk(5) = g2,
+# crk(4) = g3/crk(5), ... crk(1) = g6/(crk(2) crk(3) crk(4) crk(5))
+G_RK6_DRP = (1.0, 0.5, 0.165919771368, 0.040919732041, 0.007555704391, 0.000891421261)
+
+
+def crk_from_gammas(g):
+    """Low-storage stage coefficients whose amplification polynomial is
+    1 + sum_m g[m-1] z^m: crk(nrk) = g1 and prod_{j > nrk-m} crk(j) = g_m."""
+    n = len(g)
+    c = [0.0] * n
+    prod = 1.0
+    for m in range(1, n + 1):
+        c[n - m] = g[m - 1] / prod
+        prod *= c[n - m]
+    return tuple(c)
+
+
+CRK6_DRP = crk_from_gammas(G_RK6_DRP)
 
 
 @dataclasses.dataclass

@@ -2,7 +2,8 @@
 // 1/2 rho |curl u|^2, the quantities of stats_tgv
 // (src/File_IO/stats.f90:37-112), with the velocity gradients by the 5-pt
 // scheme of grad_vel (smod_grad_vel.f90, c3 mapping smod_grad_vel_c3.f90:
-// 260-300).  Not on the time-step path: one thread per point, neighbours
+// 260-300); and the maximum of |u| + c, the velocity scale of timestep_new
+// (src/Time/timestep_new.f90:270-288).  Not on the time-step path: one thread per point, neighbours
 // read straight from global memory; each workgroup writes its three partial
 // sums, which the host adds in a fixed order (deterministic results).
 #include "mgpu_internal.h"
@@ -21,11 +22,11 @@ __device__ __forceinline__ void vel(const CState &U, idx_t p, double o[3]) {
 
 template <bool CURV>
 __global__ __launch_bounds__(kDiagThreads) void k_diag(DiagArgs a) {
-  __shared__ double red[3][kDiagThreads];
+  __shared__ double red[4][kDiagThreads];
   const Geom &g = a.g;
   const long t = (long)blockIdx.x * kDiagThreads + threadIdx.x;
   const long cnt = (long)g.nx * g.ny * g.nz;
-  double mg = 0.0, ek = 0.0, wr = 0.0;
+  double mg = 0.0, ek = 0.0, wr = 0.0, um = 0.0;
   if (t < cnt) {
     const int i = (int)(t % g.nx) + 1;
     const long r = t / g.nx;
@@ -74,19 +75,23 @@ __global__ __launch_bounds__(kDiagThreads) void k_diag(DiagArgs a) {
     ek = 0.5 * rho * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
     const double o1 = gr[2][1] - gr[1][2], o2 = gr[0][2] - gr[2][0], o3 = gr[1][0] - gr[0][1];
     wr = 0.5 * rho * (o1 * o1 + o2 * o2 + o3 * o3);
+    const double pr = a.gam1 * (ldg(a.u.v[4], p) - 0.5 * rho * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+    um = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) + sqrt(a.gam * pr / rho);
   }
   red[0][threadIdx.x] = mg;
   red[1][threadIdx.x] = ek;
   red[2][threadIdx.x] = wr;
+  red[3][threadIdx.x] = um;
   __syncthreads();
   for (int s = kDiagThreads / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) {
 #pragma unroll
       for (int q = 0; q < 3; q++) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+      red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][threadIdx.x + s]);
     }
     __syncthreads();
   }
-  if (threadIdx.x < 3) a.partial[3 * (long)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < 4) a.partial[4 * (long)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
 }  // namespace

@@ -880,35 +880,40 @@ int diagnostics(mgpu_handle *h, mgpu_diag *out) {
   }
   int rc = refresh_ghosts(h);
   if (rc) return rc;
-  double sum[3] = {0.0, 0.0, 0.0};
+  double sum[3] = {0.0, 0.0, 0.0}, umax = 0.0;
   int64_t cells = 0;
   std::vector<double> part;
   for (auto &bd : h->b) {
     const long ng = diag_groups(bd.g);
-    if (3 * ng > h->diag_cap) {
+    if (4 * ng > h->diag_cap) {
       HIPCK(h, hipStreamSynchronize(h->stream));
       if (h->diag_buf) (void)hipFree(h->diag_buf);
       h->diag_buf = nullptr;
       h->diag_cap = 0;
-      HIPCK(h, hipMalloc((void **)&h->diag_buf, sizeof(double) * 3 * ng));
-      h->diag_cap = 3 * ng;
+      HIPCK(h, hipMalloc((void **)&h->diag_buf, sizeof(double) * 4 * ng));
+      h->diag_cap = 4 * ng;
     }
     DiagArgs da{};
     da.g = bd.g;
     for (int v = 0; v < 5; v++) da.u.v[v] = bd.st[h->cur][v];
     da.met = metrics_of(bd);
+    da.gam = bd.p.gam;
+    da.gam1 = bd.p.gam - 1.0;
     da.partial = h->diag_buf;
     HIPCK(h, launch_diag(da, h->c.curvilinear != 0, h->stream));
     HIPCK(h, hipStreamSynchronize(h->stream));
-    part.resize(3 * ng);
-    HIPCK(h, hipMemcpy(part.data(), h->diag_buf, sizeof(double) * 3 * ng, hipMemcpyDeviceToHost));
-    for (long w = 0; w < ng; w++)
-      for (int q = 0; q < 3; q++) sum[q] += part[3 * w + q];
+    part.resize(4 * ng);
+    HIPCK(h, hipMemcpy(part.data(), h->diag_buf, sizeof(double) * 4 * ng, hipMemcpyDeviceToHost));
+    for (long w = 0; w < ng; w++) {
+      for (int q = 0; q < 3; q++) sum[q] += part[4 * w + q];
+      umax = part[4 * w + 3] > umax ? part[4 * w + 3] : umax;
+    }
     cells += (int64_t)bd.g.nx * bd.g.ny * bd.g.nz;
   }
   out->mass = sum[0];
   out->kinetic = sum[1];
   out->enstrophy = sum[2];
+  out->umax = umax;
   out->cells = cells;
   return 0;
 }

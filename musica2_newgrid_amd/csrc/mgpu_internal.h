@@ -210,12 +210,13 @@ struct IfaceArgs {
   int nvar;
 };
 
-// Diagnostics of one block: per-workgroup partial sums (3 per group).
+// Diagnostics of one block: per-workgroup partial results (3 sums and a max).
 constexpr int kDiagThreads = 256;
 struct DiagArgs {
   Geom g;
   CState u;
   Metrics met;
+  double gam, gam1;
   double *partial;
 };
 

@@ -27,7 +27,7 @@ program musica_gpu_tgv
   real(8), allocatable :: idx(:)
   real(8) :: T_ref, rho_ref, Mach, Re, gam, rg, Pr, p_ref, c_ref, u_ref, mu_ref, muw_ref
   real(8) :: c0, S0, dx, x, y, z, uu, vv, ww, pp, TT, rr, umax, dt, ek, mass
-  real(8) :: t0, t1
+  real(8) :: t0, t1, g(6)
 
   call get_command_argument(1, pfile)
   call get_command_argument(2, arg); read(arg,*) n
@@ -93,11 +93,21 @@ program musica_gpu_tgv
   cs%launch_flags = 0
   cs%blocks = c_loc(blk(1))
   cs%crk = 0.0d0
-  if (prm%nrk /= 4) then
-     write(*,*) 'only the 4-stage low-storage RK (init_RK, mod_time.f90:165-169) is wired'
+  select case (prm%nrk)
+  case (4)   ! init_RK, mod_time.f90:165-169
+     cs%crk(1:4) = [1.0d0/4.0d0, 1.0d0/3.0d0, 1.0d0/2.0d0, 1.0d0]
+  case (6)   ! DRP scheme, mod_time.f90:253-294: amplification polynomial sum g_m z^m
+     g = [1.0d0, 0.5d0, 0.165919771368d0, 0.040919732041d0, 0.007555704391d0, 0.000891421261d0]
+     cs%crk(6) = g(1)
+     cs%crk(5) = g(2)
+     cs%crk(4) = g(3)/cs%crk(5)
+     cs%crk(3) = g(4)/(cs%crk(4)*cs%crk(5))
+     cs%crk(2) = g(5)/(cs%crk(4)*cs%crk(3)*cs%crk(5))
+     cs%crk(1) = g(6)/(cs%crk(4)*cs%crk(3)*cs%crk(5)*cs%crk(2))
+  case default
+     write(*,*) 'nrk must be 4 or 6 (init_RK, mod_time.f90:121-331)'
      error stop 1
-  endif
-  cs%crk(1:4) = [1.0d0/4.0d0, 1.0d0/3.0d0, 1.0d0/2.0d0, 1.0d0]
+  end select
   cs%deltat = dt
   cs%gam = gam; cs%rg = rg; cs%c0 = c0; cs%S0 = S0
   cs%diffscale = mu_ref/muw_ref

@@ -124,10 +124,12 @@ class GpuSolver:
         return tx.value, ty.value, kc.value
 
     def diagnostics(self) -> dict:
-        """Sums of rho, 1/2 rho|u|^2, 1/2 rho|curl u|^2 over all interior cells (stats_tgv)."""
+        """Sums of rho, 1/2 rho|u|^2, 1/2 rho|curl u|^2 over all interior cells
+        (stats_tgv) and max(|u| + c) (timestep_new)."""
         d = _lib.MgpuDiag()
         check(lib().mgpu_diagnostics(self.h, C.byref(d)), self.h, "mgpu_diagnostics")
-        return {"mass": d.mass, "kinetic": d.kinetic, "enstrophy": d.enstrophy, "cells": d.cells}
+        return {"mass": d.mass, "kinetic": d.kinetic, "enstrophy": d.enstrophy, "umax": d.umax,
+                "cells": d.cells}
 
     def sync(self) -> None:
         check(lib().mgpu_sync(self.h), self.h, "mgpu_sync")

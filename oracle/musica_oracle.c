@@ -750,7 +750,7 @@ out:
 /* metrics).  The reference sums gradients its last stage computed; here    */
 /* they are those of U itself, whose ghosts must be valid.                  */
 /* ------------------------------------------------------------------------ */
-int orc_diag(const orc_cfg *c, const orc_grid *g, double *const U[5], double out[3]) {
+int orc_diag(const orc_cfg *c, const orc_grid *g, double *const U[5], double out[4]) {
   int rc = check_cfg(c);
   if (rc) return rc;
   dims_t d = mkdims(c);
@@ -759,7 +759,7 @@ int orc_diag(const orc_cfg *c, const orc_grid *g, double *const U[5], double out
   if (work_alloc(&d, &w, 1)) return -10;
   prim_euler(c, &d, U, &w);
   gradients(c, g, &d, &w);
-  double mg = 0.0, ek = 0.0, wr = 0.0;
+  double mg = 0.0, ek = 0.0, wr = 0.0, um = 0.0;
   for (int k = 1; k <= d.nz; k++)
     for (int j = 1; j <= d.ny; j++)
       for (int i = 1; i <= d.nx; i++) {
@@ -770,10 +770,16 @@ int orc_diag(const orc_cfg *c, const orc_grid *g, double *const U[5], double out
         ek += 0.5 * r * (w.uu[p] * w.uu[p] + w.vv[p] * w.vv[p] + w.ww[p] * w.ww[p]);
         double o1 = gr[DWY][p] - gr[DVZ][p], o2 = gr[DUZ][p] - gr[DWX][p], o3 = gr[DVX][p] - gr[DUY][p];
         wr += 0.5 * r * (o1 * o1 + o2 * o2 + o3 * o3);
+        /* u_max of timestep_new.f90:270-288: |u| + c */
+        double q2 = w.uu[p] * w.uu[p] + w.vv[p] * w.vv[p] + w.ww[p] * w.ww[p];
+        double cs = sqrt(c->gam * w.prs[p] / r);
+        double s = sqrt(q2) + cs;
+        if (s > um) um = s;
       }
   out[0] = mg;
   out[1] = ek;
   out[2] = wr;
+  out[3] = um;
   work_free(&w);
   return 0;
 }

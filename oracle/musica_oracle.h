@@ -113,8 +113,9 @@ int orc_step(const orc_cfg *c, const orc_grid *g, double *U[5], int nsteps);
 
 /* Sums over the interior of rho, 1/2 rho|u|^2 and 1/2 rho|curl u|^2 (5-pt
  * velocity gradients) of U, whose ghosts must be valid: the quantities of
- * stats_tgv (src/File_IO/stats.f90:37-112).  Returns 0 on success. */
-int orc_diag(const orc_cfg *c, const orc_grid *g, double *const U[5], double out[3]);
+ * stats_tgv (src/File_IO/stats.f90:37-112); out[3] = max(|u| + c) over the
+ * interior (timestep_new.f90:270-288).  Returns 0 on success. */
+int orc_diag(const orc_cfg *c, const orc_grid *g, double *const U[5], double out[4]);
 
 /* Number of OpenMP threads the oracle uses. */
 int orc_num_threads(void);

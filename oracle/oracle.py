@@ -151,8 +151,8 @@ class Oracle:
         return K
 
     def diag(self, U):
-        """(sum rho, sum 1/2 rho|u|^2, sum 1/2 rho|curl u|^2) over the interior."""
-        out = np.zeros(3)
+        """(sum rho, sum 1/2 rho|u|^2, sum 1/2 rho|curl u|^2, max |u|+c) over the interior."""
+        out = np.zeros(4)
         rc = lib().orc_diag(C.byref(self.cfg), C.byref(self.grid), self._tab(U), _p(out))
         if rc != 0:
             raise RuntimeError(f"orc_diag failed rc={rc}")

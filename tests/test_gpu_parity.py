@@ -138,8 +138,19 @@ def test_diagnostics_match_oracle(which):
     o = _oracle(case)
     for d, U in ((d0, case.U), (d1, U1)):
         ref = o.diag(U)
-        got = np.array([d["mass"], d["kinetic"], d["enstrophy"]])
+        got = np.array([d["mass"], d["kinetic"], d["enstrophy"], d["umax"]])
         e = np.max(np.abs(got - ref) / np.abs(ref))
         print("diag relerr", e, got)
         assert e < 1e-12
         assert d["cells"] == case.ncells
+
+
+def test_rk6_drp_matches_oracle():
+    """nrk = 6 (init_RK's DRP scheme) through the whole path: c3, viscous, shock filter."""
+    case = cases.vortex_wavy(40, 28, 20, filter=2)
+    case.nrk, case.crk = 6, cases.CRK6_DRP
+    case.dt *= 1.5
+    Ug, Uc = run_both(case, 10)
+    e = relerr(Ug, Uc)
+    print("relerr", e)
+    assert e < 1e-12

@@ -109,6 +109,32 @@ def test_rk_step_entropy_wave_matches_amplification_polynomial(m):
     assert abs(a1 / a0 - Gz) < 1e-9, (a1 / a0, Gz)
 
 
+@pytest.mark.parametrize("m", [1, 5, 11])
+def test_rk6_drp_step_matches_its_gamma_polynomial(m):
+    """nrk = 6 (the DRP scheme of init_RK): one step multiplies the mode by
+    1 + sum g_m z^m with the scheme's published g_m."""
+    n = 32
+    c, h = wave_case(n, 0, "rho", 1e-3, u0=(40.0, 0.0, 0.0), m=m)
+    c.nrk, c.crk = 6, cases.CRK6_DRP
+    c.dt = 1.2 * h / 40.0
+    U = c.copy_state()
+    Oracle(c).step(U, 1)
+    theta = 2 * math.pi * m / n
+    z = -1j * 40.0 * k11(theta) / h * c.dt
+    Gz = 1 + sum(g * z ** (q + 1) for q, g in enumerate(cases.G_RK6_DRP))
+    a0 = mode(gr.interior(c.U[0]) - RHO0, n, 0, m)
+    a1 = mode(gr.interior(U[0]) - RHO0, n, 0, m)
+    assert abs(a1 / a0 - Gz) < 1e-9, (a1 / a0, Gz)
+    # the stage coefficients restate mod_time.f90:285-290
+    g = cases.G_RK6_DRP
+    c6, c5 = g[0], g[1]
+    c4 = g[2] / c5
+    c3 = g[3] / (c4 * c5)
+    c2 = g[4] / (c4 * c3 * c5)
+    c1 = g[5] / (c4 * c3 * c5 * c2)
+    assert np.allclose(cases.CRK6_DRP, (c1, c2, c3, c4, c5, c6), rtol=1e-15, atol=0)
+
+
 @pytest.mark.parametrize("m", [3, 8, 16])
 def test_selective_filter_transfer_function(m):
     n = 32
@@ -269,7 +295,7 @@ def test_tgv_statistics_closed_forms():
     import math
     n = 24
     c = cases.tgv(n)
-    mg, ek, wr = Oracle(c).diag(c.U)
+    mg, ek, wr, um = Oracle(c).diag(c.U)
     N = n ** 3
     rho0 = float(gr.interior(c.U[0])[0, 0, 0])
     U0 = c.info["u_ref"]
@@ -278,3 +304,16 @@ def test_tgv_statistics_closed_forms():
     assert abs(mg - rho0 * N) <= 1e-12 * rho0 * N  # rho is uniform to a few ulp
     assert abs(ek - rho0 * U0 ** 2 * N / 8) <= 1e-12 * ek
     assert abs(wr - 3.0 / 8.0 * rho0 * U0 ** 2 * kappa ** 2 * N) <= 1e-11 * wr
+
+
+def test_umax_of_a_uniform_stream():
+    """max(|u| + c) of timestep_new on a uniform stream is |u| + sqrt(gamma p / rho)."""
+    import math
+    n = 12
+    shape = (n, n, n)
+    rho, u, v, w, p = 1.2, 70.0, -30.0, 20.0, 1.0e5
+    fields = [np.full(shape, x) for x in (rho, u, v, w, p)]
+    c = cases.cartesian_box(n, n, n, (1.0, 1.0, 1.0), fields)
+    um = Oracle(c).diag(c.U)[3]
+    ref = math.sqrt(u * u + v * v + w * w) + math.sqrt(1.4 * p / rho)
+    assert abs(um - ref) <= 1e-13 * ref
