@@ -22,7 +22,8 @@ namespace mgpu {
 
 namespace {
 
-__global__ __launch_bounds__(kIfaceThreads) void k_iface(IfaceArgs a) {
+template <class T>
+__global__ __launch_bounds__(kIfaceThreads) void k_iface(IfaceArgsT<T> a) {
   const IfaceChunk ch = a.chunks[blockIdx.x];
   const FaceTask &t = a.tasks[ch.task];
   const int e = ch.start + (int)threadIdx.x;
@@ -34,8 +35,8 @@ __global__ __launch_bounds__(kIfaceThreads) void k_iface(IfaceArgs a) {
   const int a2 = r / t.n[1];
   const idx_t pd = (idx_t)(t.dst_base + a0 * t.dst_s[0] + a1 * t.dst_s[1] + a2 * t.dst_s[2]);
   const idx_t ps = (idx_t)(t.src_base + a0 * t.src_s[0] + a1 * t.src_s[1] + a2 * t.src_s[2]);
-  double *const *dst = a.ptrs + (size_t)t.dst * a.nvar;
-  double *const *src = a.ptrs + (size_t)t.src * a.nvar;
+  T *const *dst = a.ptrs + (size_t)t.dst * a.nvar;
+  T *const *src = a.ptrs + (size_t)t.src * a.nvar;
   for (int v = 0; v < a.nvar; v++) stg(dst[v], pd, ldg(src[v], ps));
 }
 
@@ -43,7 +44,13 @@ __global__ __launch_bounds__(kIfaceThreads) void k_iface(IfaceArgs a) {
 
 hipError_t launch_iface(const IfaceArgs &a, int nchunks, hipStream_t s) {
   if (nchunks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_iface, dim3(nchunks), dim3(kIfaceThreads), 0, s, a);
+  hipLaunchKernelGGL(k_iface<double>, dim3(nchunks), dim3(kIfaceThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_iface(const IfaceArgs32 &a, int nchunks, hipStream_t s) {
+  if (nchunks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_iface<float>, dim3(nchunks), dim3(kIfaceThreads), 0, s, a);
   return hipGetLastError();
 }
 

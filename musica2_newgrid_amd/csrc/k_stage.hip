@@ -44,10 +44,23 @@ struct Raw {
   double m[3];
 };
 
-template <bool CURV>
-__device__ __forceinline__ void load_raw(const CState &U, const Metrics &M, idx_t p, int dir, Raw &r) {
+// Stage state at element p: FP64, or (IN32) the reference state plus the
+// stored FP32 difference.
+template <bool IN32>
+__device__ __forceinline__ void load_u(const StageArgs &a, idx_t p, double u[5]) {
+  if constexpr (IN32) {
+    const double ref[5] = {a.ref.r, a.ref.ru, a.ref.rv, a.ref.rw, a.ref.re};
 #pragma unroll
-  for (int v = 0; v < 5; v++) r.u[v] = ldg(U.v[v], p);
+    for (int v = 0; v < 5; v++) u[v] = ref[v] + (double)ldg(a.uin32[v], p);
+  } else {
+#pragma unroll
+    for (int v = 0; v < 5; v++) u[v] = ldg(a.uin.v[v], p);
+  }
+}
+
+template <bool CURV, bool IN32>
+__device__ __forceinline__ void load_raw(const StageArgs &a, const Metrics &M, idx_t p, int dir, Raw &r) {
+  load_u<IN32>(a, p, r.u);
   if constexpr (CURV) {
 #pragma unroll
     for (int c = 0; c < 3; c++) r.m[c] = ldg(M.m[3 * dir + c], p);
@@ -131,8 +144,19 @@ __device__ __forceinline__ void flux_r(const Prim &q, int dir, const double m[3]
   else flux_m<CURV>(q, dir, m, F);
 }
 
-template <int TX, int TY, bool CURV, bool VISC, bool MIX>
+// Flux of a halo cell, whose direction (i or j) is only known at run time:
+// branch to a constant direction so that the primitives stay in registers.
+template <bool CURV, bool MIX, class R>
+__device__ __forceinline__ void flux_halo(const Prim &q, int dir, const double m[3], const RefState &rf, R F[5]) {
+  if (dir == 0) flux_r<CURV, MIX>(q, 0, m, rf, F);
+  else flux_r<CURV, MIX>(q, 1, m, rf, F);
+}
+
+template <int TX, int TY, bool CURV, bool VISC, int MODE>
 __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__restrict__ tab, int nb, int ntiles) {
+  constexpr bool MIX = MODE != kFP64;
+  constexpr bool IN32 = MODE == kMixMiddle || MODE == kMixLast;
+  constexpr bool OUT32 = MODE == kMixFirst || MODE == kMixMiddle;
   using R = typename std::conditional<MIX, float, double>::type;
   constexpr int NT = TX * TY;
   constexpr int NHX = 10 * TY;            // i-halo cells per plane
@@ -202,7 +226,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
 #pragma unroll
     for (int l = 0; l < 10; l++) {
       Raw r;
-      load_raw<CURV>(a.uin, a.met, own_off + (idx_t)(kbeg - 5 + l) * sz, 2, r);
+      load_raw<CURV, IN32>(a, a.met, own_off + (idx_t)(kbeg - 5 + l) * sz, 2, r);
       flux_r<CURV, MIX>(prim_raw(r.u, gam1), 2, r.m, rf, q[l]);
     }
   }
@@ -215,10 +239,9 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
     Raw rq, rh0;
     double rxm[3] = {0, 0, 0}, rym[3] = {0, 0, 0}, ru[5] = {0, 0, 0, 0, 0}, u0[5] = {0, 0, 0, 0, 0};
     double ijac = 0.0;
-    if (own) load_raw<CURV>(a.uin, a.met, pk + 5 * sz, 2, rq);
+    if (own) load_raw<CURV, IN32>(a, a.met, pk + 5 * sz, 2, rq);
     if (needx || needy) {
-#pragma unroll
-      for (int v = 0; v < 5; v++) ru[v] = ldg(a.uin.v[v], pk);
+      load_u<IN32>(a, pk, ru);
       if constexpr (CURV) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -227,7 +250,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
         }
       }
     }
-    if (hs_on[0]) load_raw<CURV>(a.uin, a.met, hs_off[0] + (idx_t)k * sz, hs_dir[0], rh0);
+    if (hs_on[0]) load_raw<CURV, IN32>(a, a.met, hs_off[0] + (idx_t)k * sz, hs_dir[0], rh0);
     double kvis[4] = {0, 0, 0, 0};
     if (own) {
 #pragma unroll
@@ -255,7 +278,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
     }
     if (hs_on[0]) {
       R F[5];
-      flux_r<CURV, MIX>(prim_raw(rh0.u, gam1), hs_dir[0], rh0.m, rf, F);
+      flux_halo<CURV, MIX>(prim_raw(rh0.u, gam1), hs_dir[0], rh0.m, rf, F);
       if (hs_dir[0] == 0) {
 #pragma unroll
         for (int v = 0; v < 5; v++) sfx[v][hs_r[0]][hs_c[0]] = F[v];
@@ -270,12 +293,12 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
       Raw rh[NS];
 #pragma unroll
       for (int s = 1; s < NS; s++)
-        if (hs_on[s]) load_raw<CURV>(a.uin, a.met, hs_off[s] + (idx_t)k * sz, hs_dir[s], rh[s]);
+        if (hs_on[s]) load_raw<CURV, IN32>(a, a.met, hs_off[s] + (idx_t)k * sz, hs_dir[s], rh[s]);
 #pragma unroll
       for (int s = 1; s < NS; s++) {
         if (hs_on[s]) {
           R F[5];
-          flux_r<CURV, MIX>(prim_raw(rh[s].u, gam1), hs_dir[s], rh[s].m, rf, F);
+          flux_halo<CURV, MIX>(prim_raw(rh[s].u, gam1), hs_dir[s], rh[s].m, rf, F);
           if (hs_dir[s] == 0) {
 #pragma unroll
             for (int v = 0; v < 5; v++) sfx[v][hs_r[s]][hs_c[s]] = F[v];
@@ -310,8 +333,14 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
 #pragma unroll
         for (int c = 0; c < 4; c++) K[1 + c] = K[1 + c] + kvis[c];
       }
+      if constexpr (OUT32) {
+        const double ref[5] = {rf.r, rf.ru, rf.rv, rf.rw, rf.re};
 #pragma unroll
-      for (int v = 0; v < 5; v++) stg_nt(a.uout.v[v], pk, u0[v] - a.alpha * K[v]);
+        for (int v = 0; v < 5; v++) stg_nt(a.uout32[v], pk, (float)((u0[v] - a.alpha * K[v]) - ref[v]));
+      } else {
+#pragma unroll
+        for (int v = 0; v < 5; v++) stg_nt(a.uout.v[v], pk, u0[v] - a.alpha * K[v]);
+      }
     }
 
 #pragma unroll
@@ -321,27 +350,40 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
   }
 }
 
-template <int TX, int TY, bool MIX>
+template <int TX, int TY, int MODE>
 hipError_t launch_t(const Batch<StageArgs> *tab, int nb, int nt, bool curv, bool visc, hipStream_t s) {
   dim3 blk(TX * TY), grd(nt);
+  // the viscous increment is added in the last stage only
+  constexpr bool VOK = MODE == kFP64 || MODE == kMixLast;
+  if (visc && !VOK) return hipErrorInvalidValue;
   if (curv) {
-    if (visc) hipLaunchKernelGGL((k_stage<TX, TY, true, true, MIX>), grd, blk, 0, s, tab, nb, nt);
-    else hipLaunchKernelGGL((k_stage<TX, TY, true, false, MIX>), grd, blk, 0, s, tab, nb, nt);
+    if constexpr (VOK) {
+      if (visc) {
+        hipLaunchKernelGGL((k_stage<TX, TY, true, true, MODE>), grd, blk, 0, s, tab, nb, nt);
+        return hipGetLastError();
+      }
+    }
+    hipLaunchKernelGGL((k_stage<TX, TY, true, false, MODE>), grd, blk, 0, s, tab, nb, nt);
   } else {
-    if (visc) hipLaunchKernelGGL((k_stage<TX, TY, false, true, MIX>), grd, blk, 0, s, tab, nb, nt);
-    else hipLaunchKernelGGL((k_stage<TX, TY, false, false, MIX>), grd, blk, 0, s, tab, nb, nt);
+    if constexpr (VOK) {
+      if (visc) {
+        hipLaunchKernelGGL((k_stage<TX, TY, false, true, MODE>), grd, blk, 0, s, tab, nb, nt);
+        return hipGetLastError();
+      }
+    }
+    hipLaunchKernelGGL((k_stage<TX, TY, false, false, MODE>), grd, blk, 0, s, tab, nb, nt);
   }
   return hipGetLastError();
 }
 
-template <bool MIX>
+template <int MODE>
 hipError_t launch_shape(const Batch<StageArgs> *tab, int nb, int nt, bool curv, bool visc, int shape, hipStream_t s) {
   switch (shape) {
-    case 1: return launch_t<32, 8, MIX>(tab, nb, nt, curv, visc, s);
-    case 2: return launch_t<64, 8, MIX>(tab, nb, nt, curv, visc, s);
-    case 3: return launch_t<16, 16, MIX>(tab, nb, nt, curv, visc, s);
-    case 4: return launch_t<32, 16, MIX>(tab, nb, nt, curv, visc, s);
-    default: return launch_t<64, 4, MIX>(tab, nb, nt, curv, visc, s);
+    case 1: return launch_t<32, 8, MODE>(tab, nb, nt, curv, visc, s);
+    case 2: return launch_t<64, 8, MODE>(tab, nb, nt, curv, visc, s);
+    case 3: return launch_t<16, 16, MODE>(tab, nb, nt, curv, visc, s);
+    case 4: return launch_t<32, 16, MODE>(tab, nb, nt, curv, visc, s);
+    default: return launch_t<64, 4, MODE>(tab, nb, nt, curv, visc, s);
   }
 }
 
@@ -360,11 +402,15 @@ TileRange stage_tiles(const Geom &g, int shape, int kchunk) {
   return t;
 }
 
-hipError_t launch_stage(const Batch<StageArgs> *tab, int nb, int nt, bool curv, bool visc, int shape, bool mixed,
+hipError_t launch_stage(const Batch<StageArgs> *tab, int nb, int nt, bool curv, bool visc, int shape, int mode,
                         hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  return mixed ? launch_shape<true>(tab, nb, nt, curv, visc, shape, s)
-               : launch_shape<false>(tab, nb, nt, curv, visc, shape, s);
+  switch (mode) {
+    case kMixFirst: return launch_shape<kMixFirst>(tab, nb, nt, curv, visc, shape, s);
+    case kMixMiddle: return launch_shape<kMixMiddle>(tab, nb, nt, curv, visc, shape, s);
+    case kMixLast: return launch_shape<kMixLast>(tab, nb, nt, curv, visc, shape, s);
+    default: return launch_shape<kFP64>(tab, nb, nt, curv, visc, shape, s);
+  }
 }
 
 }  // namespace mgpu

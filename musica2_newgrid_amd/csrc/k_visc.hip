@@ -272,9 +272,12 @@ TileRange vtiles(const Geom &g) {
   return t;
 }
 
-// Components of tau forming the flux along direction d in a Cartesian block
-// (row d of the stress tensor and the energy flux e_d).
-__constant__ const int kRow[3][4] = {{0, 1, 2, 6}, {1, 3, 4, 7}, {2, 4, 5, 8}};
+// Component c of tau in the flux along direction d of a Cartesian block (row
+// d of the stress tensor, then the energy flux e_d); constant-folds for
+// constant d and c.
+__host__ __device__ constexpr int krow(int d, int c) {
+  return c == 3 ? 6 + d : (d == 0 ? c : (d == 1 ? (c == 0 ? 1 : 2 + c) : (c == 0 ? 2 : (c == 1 ? 4 : 5))));
+}
 
 // Flux along computational direction d at one point: Cartesian, row d of
 // tau (t holds those four values); c3, tau projected on the viscous metric
@@ -359,7 +362,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
   auto flux_at = [&](int d, idx_t p, double f[4]) {
     double t[NL];
 #pragma unroll
-    for (int c = 0; c < NL; c++) t[c] = ldg(a.tau[CURV ? c : kRow[d][c]], p);
+    for (int c = 0; c < NL; c++) t[c] = ldg(a.tau[CURV ? c : krow(d, c)], p);
     double mx = 0.0, my = 0.0, mz = 0.0;
     if constexpr (CURV) {
       mx = ldg(mv[3 * d], p);
@@ -372,7 +375,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
   auto flux_of = [&](int d, const double *t9, idx_t p, double f[4]) {
     double t[NL];
 #pragma unroll
-    for (int c = 0; c < NL; c++) t[c] = t9[CURV ? c : kRow[d][c]];
+    for (int c = 0; c < NL; c++) t[c] = t9[CURV ? c : krow(d, c)];
     double mx = 0.0, my = 0.0, mz = 0.0;
     if constexpr (CURV) {
       mx = ldg(mv[3 * d], p);
@@ -415,7 +418,10 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
     }
 #pragma unroll
     for (int s = 0; s < NS; s++)
-      if (hs_on[s]) flux_at(hs_x[s] ? 0 : 1, hs_off[s] + (idx_t)k * sz, fh[s]);
+      if (hs_on[s]) {
+        if (hs_x[s]) flux_at(0, hs_off[s] + (idx_t)k * sz, fh[s]);
+        else flux_at(1, hs_off[s] + (idx_t)k * sz, fh[s]);
+      }
     if (needx) {
 #pragma unroll
       for (int c = 0; c < 4; c++) sfx[c][ty][tx + 2] = fx[c];

@@ -297,6 +297,7 @@ void free_exchange(mgpu_handle *h) {
   for (void *p : h->xtables) (void)hipFree(p);
   h->xtables.clear();
   for (auto &x : h->xs) x = IfaceArgs{};
+  for (auto &x : h->xf) x = IfaceArgs32{};
   h->xv = IfaceArgs{};
   h->nchunks_s = h->nchunks_v = 0;
   h->exchange_ready = false;
@@ -332,6 +333,16 @@ int setup_exchange(mgpu_handle *h) {
     double **dp = nullptr;
     if ((rc = upload(h, ptrs, &dp))) return rc;
     h->xs[s] = IfaceArgs{dt, dc, dp, 5};
+  }
+  if (h->mixed) {
+    for (int s = 0; s < 2; s++) {
+      std::vector<float *> ptrs;
+      for (int b = 0; b < nb; b++)
+        for (int v = 0; v < 5; v++) ptrs.push_back(h->b[b].sf[s][v]);
+      float **dp = nullptr;
+      if ((rc = upload(h, ptrs, &dp))) return rc;
+      h->xf[s] = IfaceArgs32{dt, dc, dp, 5};
+    }
   }
   // viscous: 2 layers of tau (+ p and the sensor for the shock filter)
   if (h->c.iorder_visc) {
@@ -385,6 +396,8 @@ std::map<TuneKey, std::pair<int, int>> g_tune_cache;
 void free_block(BlockData &bd) {
   for (auto &s : bd.st)
     for (auto &p : s) { if (p) (void)hipFree(p); p = nullptr; }
+  for (auto &s : bd.sf)
+    for (auto &p : s) { if (p) (void)hipFree(p); p = nullptr; }
   for (auto &p : bd.idx) { if (p) (void)hipFree(p); p = nullptr; }
   for (auto &p : bd.m) { if (p) (void)hipFree(p); p = nullptr; }
   for (auto &p : bd.mv) { if (p) (void)hipFree(p); p = nullptr; }
@@ -406,6 +419,26 @@ int dalloc(mgpu_handle *h, double **p, size_t count) {
   return 0;
 }
 
+int dalloc32(mgpu_handle *h, float **p, size_t count) {
+  hipError_t e = hipMalloc((void **)p, count * sizeof(float));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(h, MGPU_ENOMEM, "hipMalloc of %zu floats failed: %s", count, hipGetErrorString(e));
+  }
+  e = hipMemset(*p, 0, count * sizeof(float));
+  if (e != hipSuccess) return fail(h, MGPU_EDEVICE, "hipMemset failed: %s", hipGetErrorString(e));
+  return 0;
+}
+
+// Stage kernel variant of stage irk.
+int stage_mode(const mgpu_handle *h, int irk) {
+  if (!h->mixed) return kFP64;
+  return irk == 1 ? kMixFirst : (irk == h->c.nrk ? kMixLast : kMixMiddle);
+}
+
+// FP32 set written by intermediate stage irk (mixed precision).
+int fset_of(int irk) { return (irk % 2 == 1) ? 0 : 1; }
+
 StageArgs stage_args(const mgpu_handle *h, const BlockData &bd, int in, int s0, int out, int irk) {
   (void)h;
   StageArgs sa{};
@@ -420,6 +453,13 @@ StageArgs stage_args(const mgpu_handle *h, const BlockData &bd, int in, int s0, 
   for (int q = 0; q < 4; q++) sa.kv[q] = bd.kv[q];
   sa.alpha = bd.p.deltat * bd.p.crk[irk - 1];
   sa.ref = bd.ref;
+  if (h->mixed) {
+    const int mode = stage_mode(h, irk);
+    for (int v = 0; v < 5; v++) {
+      if (mode == kMixMiddle || mode == kMixLast) sa.uin32[v] = bd.sf[fset_of(irk - 1)][v];
+      if (mode == kMixFirst || mode == kMixMiddle) sa.uout32[v] = bd.sf[fset_of(irk)][v];
+    }
+  }
   return sa;
 }
 
@@ -586,9 +626,14 @@ int enqueue_step(mgpu_handle *h, int s0) {
       ProfScope ps(h, MGPU_K_STAGE);
       const int v = (last && visc) ? 1 : 0;
       HIPCK(h, launch_stage(h->d_stage + ((size_t)s0 * nrk + (irk - 1)) * nb, nb, h->nt_stage[v], curv,
-                            last && visc, h->shape[v], h->mixed, h->stream));
+                            last && visc, h->shape[v], stage_mode(h, irk), h->stream));
     }
-    if ((rc = exchange_state(h, out))) return rc;
+    if (h->mixed && !last) {
+      ProfScope ps(h, MGPU_K_HALO);
+      HIPCK(h, launch_iface(h->xf[fset_of(irk)], h->nchunks_s, h->stream));
+    } else if ((rc = exchange_state(h, out))) {
+      return rc;
+    }
     in = out;
   }
   (void)in;
@@ -653,6 +698,8 @@ std::string check_flags(const mgpu_case &c) {
     snprintf(buf, sizeof buf, "shock filter needs iorder_visc=4 (its sensor reads the viscous gradients of U^n)");
   else if (c.precision != MGPU_PREC_FP64 && c.precision != MGPU_PREC_MIXED)
     snprintf(buf, sizeof buf, "precision=%d", c.precision);
+  else if (c.precision == MGPU_PREC_MIXED && c.nrk < 2)
+    snprintf(buf, sizeof buf, "mixed precision needs nrk >= 2");
   return std::string(buf);
 }
 
@@ -682,6 +729,7 @@ size_t block_bytes(const mgpu_case &c, int nx, int ny, int nz) {
   if (c.curvilinear) arrays += c.iorder_visc ? 20 : 10;
   if (c.iorder_visc) arrays += NTAU + 4 + (c.filter == MGPU_FILTER_SHOCK ? 2 : 0);
   size_t bytes = arrays * n * sizeof(double);
+  if (c.precision == MGPU_PREC_MIXED) bytes += 10 * n * sizeof(float);
   if (!c.curvilinear) bytes += 6 * (size_t)(nx + ny + nz + 6 * G) * sizeof(double);
   return bytes;
 }
@@ -709,6 +757,9 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
   int rc = 0;
   for (int s = 0; s < 3 && !rc; s++)
     for (int v = 0; v < 5 && !rc; v++) rc = dalloc(h, &bd.st[s][v], n);
+  if (h->mixed)
+    for (int s = 0; s < 2 && !rc; s++)
+      for (int v = 0; v < 5 && !rc; v++) rc = dalloc32(h, &bd.sf[s][v], n);
   if (c.curvilinear) {
     for (int q = 0; q < 10 && !rc; q++) rc = dalloc(h, &bd.m[q], n);
     if (c.iorder_visc)
@@ -1034,16 +1085,19 @@ int mgpu_autotune(mgpu_handle *h, int flags) {
         h->kchunk[v] = kc;
         std::vector<TileRange> tr;
         const int nt = stage_ranges(h, v, tr);
+        // Euler variant timed as an intermediate stage, viscous variant as the last one
+        const int irk = v == 1 ? h->c.nrk : (h->c.nrk > 2 ? 2 : 1);
         for (int b = 0; b < nb; b++) {
-          tab[b].a = stage_args(h, h->b[b], s0, s0, sa, v == 1 ? h->c.nrk : 1);
+          tab[b].a = stage_args(h, h->b[b], s0, s0, sa, irk);
           tab[b].t = tr[b];
         }
+        const int mode = stage_mode(h, irk);
         hipError_t e = hipMemcpy(h->d_tune, tab.data(), nb * sizeof(tab[0]), hipMemcpyHostToDevice);
         // one warm-up launch, then three timed ones
-        if (e == hipSuccess) e = launch_stage(h->d_tune, nb, nt, curv, v == 1, sh, h->mixed, h->stream);
+        if (e == hipSuccess) e = launch_stage(h->d_tune, nb, nt, curv, v == 1, sh, mode, h->stream);
         if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
         for (int r = 0; r < 3 && e == hipSuccess; r++)
-          e = launch_stage(h->d_tune, nb, nt, curv, v == 1, sh, h->mixed, h->stream);
+          e = launch_stage(h->d_tune, nb, nt, curv, v == 1, sh, mode, h->stream);
         if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.f;

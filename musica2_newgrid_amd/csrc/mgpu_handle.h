@@ -17,6 +17,7 @@ struct BlockData {
   Geom g{};
   mgpu_case p{};          // scalars of this block's case (time step, gas, filter amplitudes)
   double *st[3][5] = {};  // three state sets: st[cur] = U^n, the others stage buffers
+  float *sf[2][5] = {};   // mixed precision: intermediate stage states, FP32 differences with ref
   double *idx[6] = {};    // Cartesian metrics idx,idy,idz,idx_v,idy_v,idz_v
   double *m[10] = {};
   double *mv[10] = {};
@@ -53,6 +54,7 @@ struct mgpu_handle {
   // layers) and per-array-set pointer tables, all in device memory
   mgpu::IfaceArgs xs[3] = {};  // state set s
   mgpu::IfaceArgs xv = {};     // tau (+ prs, sens)
+  mgpu::IfaceArgs32 xf[2] = {};  // mixed precision: FP32 stage sets
   int nchunks_s = 0, nchunks_v = 0;
   std::vector<void *> xtables;  // device memory of the exchange tables
   bool exchange_ready = false;

@@ -51,6 +51,16 @@ __device__ __forceinline__ void stg(double *b, idx_t i, double v) {
 __device__ __forceinline__ void stg_nt(double *b, idx_t i, double v) {
   __builtin_nontemporal_store(v, reinterpret_cast<double *>(reinterpret_cast<char *>(b) + (uint32_t)(i << 3)));
 }
+// The same for FP32 arrays.
+__device__ __forceinline__ float ldg(const float *b, idx_t i) {
+  return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(b) + (uint32_t)(i << 2));
+}
+__device__ __forceinline__ void stg(float *b, idx_t i, float v) {
+  *reinterpret_cast<float *>(reinterpret_cast<char *>(b) + (uint32_t)(i << 2)) = v;
+}
+__device__ __forceinline__ void stg_nt(float *b, idx_t i, float v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<float *>(reinterpret_cast<char *>(b) + (uint32_t)(i << 2)));
+}
 
 // XCD-aware workgroup order.  gfx950 dispatches workgroup w to XCD w % 8;
 // each XCD has its own L2.  Renumber so that XCD x owns one contiguous range
@@ -146,7 +156,15 @@ struct StageArgs {
   const double *kv[4];   // viscous increment of rhou,rhov,rhow,rhoe (last stage)
   double alpha;          // deltat*crk(irk)
   RefState ref;          // mixed precision only
+  // mixed precision: intermediate stage states as FP32 differences with ref
+  const float *uin32[5];
+  float *uout32[5];
 };
+
+// Stage kernel variants: FP64, or mixed precision with the stage state read
+// and written as FP64 (U^n, the last stage's output) or as FP32 differences
+// with the reference state (the intermediate stages).
+enum StageMode { kFP64 = 0, kMixFirst = 1, kMixMiddle = 2, kMixLast = 3 };
 
 // Viscous flux tensor of U^n in the physical frame, per interior point:
 // tau[0..5] = -tau_xx,-tau_xy,-tau_xz,-tau_yy,-tau_yz,-tau_zz (the
@@ -203,12 +221,15 @@ struct IfaceChunk {
   int task, start;
 };
 constexpr int kIfaceThreads = 256;
-struct IfaceArgs {
+template <class T>
+struct IfaceArgsT {
   const FaceTask *tasks;
   const IfaceChunk *chunks;
-  double *const *ptrs;  // ptrs[block*nvar + v]: array v of block
+  T *const *ptrs;  // ptrs[block*nvar + v]: array v of block
   int nvar;
 };
+using IfaceArgs = IfaceArgsT<double>;
+using IfaceArgs32 = IfaceArgsT<float>;
 
 // Diagnostics of one block: per-workgroup partial results (3 sums and a max).
 constexpr int kDiagThreads = 256;
@@ -229,7 +250,7 @@ extern const int kStageShapes[][2];
 extern const int kNumStageShapes;
 TileRange stage_tiles(const Geom &g, int shape, int kchunk);
 hipError_t launch_stage(const Batch<StageArgs> *tab, int nb, int ntiles, bool curv, bool visc, int shape,
-                        bool mixed, hipStream_t s);
+                        int mode, hipStream_t s);
 TileRange visc_tiles(const Geom &g);
 hipError_t launch_visc(const Batch<ViscArgs> *tab, int nb, int ntiles, bool curv, hipStream_t s);
 TileRange vdiv_tiles(const Geom &g);
@@ -238,6 +259,7 @@ TileRange filter_tiles(const Geom &g);
 hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, hipStream_t s);
 hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, hipStream_t s);
 hipError_t launch_iface(const IfaceArgs &a, int nchunks, hipStream_t s);
+hipError_t launch_iface(const IfaceArgs32 &a, int nchunks, hipStream_t s);
 long diag_groups(const Geom &g);
 hipError_t launch_diag(const DiagArgs &a, bool curv, hipStream_t s);
 }  // namespace mgpu
