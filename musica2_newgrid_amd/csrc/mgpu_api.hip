@@ -846,8 +846,10 @@ int put_state(mgpu_handle *h, int block, const double *const *src) {
   for (int v = 0; v < 5; v++) HIPCK(h, hipMemcpy(bd.st[h->cur][v], src[v], bytes, hipMemcpyHostToDevice));
   bd.have_state = true;
   h->ghosts_dirty = true;  // the exchange runs before the next use of the ghosts
-  if (h->mixed) {
-    // reference state of the mixed path: interior mean of this block's state
+  if (h->mixed && (block == 0 || !h->shared_ref)) {
+    // reference state of the mixed path: interior mean of this block's state;
+    // blocks joined by interfaces exchange FP32 differences with it, so they
+    // all take block 0's (requests of the serving loop keep their own)
     double mean[5] = {0, 0, 0, 0, 0};
     const Geom &g = bd.g;
     for (int v = 0; v < 5; v++) {
@@ -859,7 +861,7 @@ int put_state(mgpu_handle *h, int block, const double *const *src) {
         }
       mean[v] = acc / ((double)g.nx * g.ny * g.nz);
     }
-    RefState &r = bd.ref;
+    RefState r{};
     r.r = mean[0];
     r.ru = mean[1];
     r.rv = mean[2];
@@ -870,6 +872,10 @@ int put_state(mgpu_handle *h, int block, const double *const *src) {
     r.w = r.rw / r.r;
     r.p = (bd.p.gam - 1.0) * (r.re - 0.5 * r.r * (r.u * r.u + r.v * r.v + r.w * r.w));
     r.H = r.re + r.p;
+    if (h->shared_ref)
+      for (auto &b : h->b) b.ref = r;
+    else
+      bd.ref = r;
     h->tables_ready = false;
   }
   return 0;
@@ -1006,6 +1012,7 @@ int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
     destroy(h);
     return rc;
   }
+  h->shared_ref = true;
   *out = h;
   return 0;
 }

@@ -77,6 +77,7 @@ struct mgpu_handle {
   bool use_graph = true;
   hipGraphExec_t gexec[3] = {};
   bool mixed = false;  // FP32 flux/derivative/filter arithmetic
+  bool shared_ref = false;  // one mixed-precision reference state for all blocks (set from block 0)
   double *diag_buf = nullptr;  // per-workgroup partial sums of mgpu_diagnostics
   long diag_cap = 0;
 };
