@@ -87,7 +87,8 @@ def run_leg(case, precision: int, steps: int, warmup: int, tune: bool):
     solver = GpuSolver(case, precision=precision)
     if tune:
         solver.autotune()
-    tuned = {"euler_stages": solver.stage_config(0), "last_stage": solver.stage_config(1)}
+    tuned = {"euler_stages": solver.stage_config(0), "last_stage": solver.stage_config(1),
+             "visc": solver.stage_config(2), "vdiv": solver.stage_config(3), "filter": solver.stage_config(4)}
     solver.step(warmup)
     solver.sync()
 

@@ -147,8 +147,9 @@ int mgpu_step(mgpu_handle *h, int nsteps);
  * state set.  No counterpart in the reference (its loops have no tiles). */
 int mgpu_autotune(mgpu_handle *h, int flags);
 
-/* The stage-kernel launch parameters in use for a block: variant 0 = Euler
- * stages, 1 = last stage with viscous divergence. */
+/* Launch parameters in use for a block: variant 0 = Euler stages, 1 = last
+ * stage with viscous divergence, 2 = viscous fluxes, 3 = viscous divergence,
+ * 4 = filter (tile extents and planes per workgroup). */
 int mgpu_stage_config(mgpu_handle *h, int block, int variant, int *shape_tx, int *shape_ty, int *kchunk);
 
 /* Wait for all work queued on the handle. */

@@ -398,14 +398,14 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
 }  // namespace
 
 // Tiles of the filters: chunks of >= 16 planes, about 1024 workgroups per block.
-TileRange filter_tiles(const Geom &g) {
+TileRange filter_tiles(const Geom &g, int kc_req) {
   TileRange t{};
   t.tiles_x = (g.nx + FX - 1) / FX;
   t.tiles_y = (g.ny + FY - 1) / FY;
   const long nxy = (long)t.tiles_x * t.tiles_y;
   const long want = (1024 + nxy - 1) / nxy;
-  int kc = (int)((g.nz + want - 1) / want);
-  if (kc < 16) kc = 16;
+  int kc = kc_req > 0 ? kc_req : (int)((g.nz + want - 1) / want);
+  if (kc_req <= 0 && kc < 16) kc = 16;
   if (kc > g.nz) kc = g.nz;
   t.kchunk = kc;
   return t;

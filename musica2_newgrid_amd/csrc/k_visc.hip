@@ -259,14 +259,14 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(const Batch<ViscArgs> *__res
 // Tiles of the 32x16 viscous kernels: chunks of >= 16 planes, about 2048
 // workgroups per block (4 per CU slot of 256 CUs x 2).
 constexpr int VX = 32, VY = 16;
-TileRange vtiles(const Geom &g) {
+TileRange vtiles(const Geom &g, int kc_req) {
   TileRange t{};
   t.tiles_x = (g.nx + VX - 1) / VX;
   t.tiles_y = (g.ny + VY - 1) / VY;
   const long nxy = (long)t.tiles_x * t.tiles_y;
   const long want = (2048 + nxy - 1) / nxy;
-  int kc = (int)((g.nz + want - 1) / want);
-  if (kc < 16) kc = 16;
+  int kc = kc_req > 0 ? kc_req : (int)((g.nz + want - 1) / want);
+  if (kc_req <= 0 && kc < 16) kc = 16;
   if (kc > g.nz) kc = g.nz;
   t.kchunk = kc;
   return t;
@@ -474,8 +474,8 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
 
 }  // namespace
 
-TileRange visc_tiles(const Geom &g) { return vtiles(g); }
-TileRange vdiv_tiles(const Geom &g) { return vtiles(g); }
+TileRange visc_tiles(const Geom &g, int kchunk) { return vtiles(g, kchunk); }
+TileRange vdiv_tiles(const Geom &g, int kchunk) { return vtiles(g, kchunk); }
 
 hipError_t launch_visc(const Batch<ViscArgs> *tab, int nb, int nt, bool curv, hipStream_t s) {
   if (nt <= 0) return hipSuccess;

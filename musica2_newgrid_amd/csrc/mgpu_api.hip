@@ -494,6 +494,8 @@ void free_tables(mgpu_handle *h) {
   if (h->d_vdiv) (void)hipFree(h->d_vdiv);
   if (h->d_filt) (void)hipFree(h->d_filt);
   if (h->d_tune) (void)hipFree(h->d_tune);
+  if (h->d_tune_f) (void)hipFree(h->d_tune_f);
+  h->d_tune_f = nullptr;
   h->d_stage = nullptr;
   h->d_visc = nullptr;
   h->d_vdiv = nullptr;
@@ -509,6 +511,24 @@ void drop_graphs(mgpu_handle *h) {
   }
 }
 
+FilterArgs filter_args(const mgpu_handle *h, const BlockData &bd, int in, int out) {
+  const double d11[6] = {D11_0, D11_1, D11_2, D11_3, D11_4, D11_5};
+  FilterArgs fa{};
+  fa.g = bd.g;
+  for (int v = 0; v < 5; v++) {
+    fa.uin.v[v] = bd.st[in][v];
+    fa.uout.v[v] = bd.st[out][v];
+  }
+  for (int l = 0; l < 6; l++) fa.dd[l] = -d11[l] * bd.p.dissip_coeff;
+  fa.prs = bd.prs;
+  fa.sens = h->c.is_ducros ? bd.sens : nullptr;
+  fa.xnu0 = -bd.p.dissip_shock;
+  fa.xnu10 = bd.p.dissip_coeff / 1260.0;
+  fa.csens = bd.p.csens;
+  fa.ref = bd.ref;
+  return fa;
+}
+
 // Fills the batched launch tables (after create, and after the autotuner
 // changed the tiles).  The stream is idle when they are rewritten.
 int build_tables(mgpu_handle *h) {
@@ -522,7 +542,7 @@ int build_tables(mgpu_handle *h) {
     const int cap = nb < 8 ? 8 : nb;
     if ((rc = dalloc_table(h, &h->d_stage, (size_t)3 * nrk * cap)) || (rc = dalloc_table(h, &h->d_visc, 3 * cap)) ||
         (rc = dalloc_table(h, &h->d_vdiv, cap)) || (rc = dalloc_table(h, &h->d_filt, 3 * cap)) ||
-        (rc = dalloc_table(h, &h->d_tune, cap)))
+        (rc = dalloc_table(h, &h->d_tune, cap)) || (rc = dalloc_table(h, &h->d_tune_f, cap)))
       return rc;
     h->tab_cap = cap;
   }
@@ -539,11 +559,11 @@ int build_tables(mgpu_handle *h) {
   std::vector<Batch<VdivArgs>> vd(nb);
   std::vector<Batch<FilterArgs>> fi(3 * nb);
   const bool visc = c.iorder_visc != 0;
-  const double d11[6] = {D11_0, D11_1, D11_2, D11_3, D11_4, D11_5};
   int fv = 0, fd = 0, ff = 0;
   for (int b = 0; b < nb; b++) {
     const BlockData &bd = h->b[b];
-    TileRange tv = visc_tiles(bd.g), td = vdiv_tiles(bd.g), tf = filter_tiles(bd.g);
+    TileRange tv = visc_tiles(bd.g, h->kc_aux[0]), td = vdiv_tiles(bd.g, h->kc_aux[1]),
+              tf = filter_tiles(bd.g, h->kc_aux[2]);
     tv.first = fv;
     td.first = fd;
     tf.first = ff;
@@ -572,19 +592,7 @@ int build_tables(mgpu_handle *h) {
       va.want_sensor = c.filter == MGPU_FILTER_SHOCK;
       va.ducros = c.is_ducros;
       vi[s0 * nb + b].t = tv;
-      FilterArgs &fa = fi[s0 * nb + b].a;
-      fa.g = bd.g;
-      for (int v = 0; v < 5; v++) {
-        fa.uin.v[v] = bd.st[in][v];  // output of the last stage
-        fa.uout.v[v] = bd.st[s0][v];
-      }
-      for (int l = 0; l < 6; l++) fa.dd[l] = -d11[l] * bd.p.dissip_coeff;
-      fa.prs = bd.prs;
-      fa.sens = c.is_ducros ? bd.sens : nullptr;
-      fa.xnu0 = -bd.p.dissip_shock;
-      fa.xnu10 = bd.p.dissip_coeff / 1260.0;
-      fa.csens = bd.p.csens;
-      fa.ref = bd.ref;
+      fi[s0 * nb + b].a = filter_args(h, bd, in, s0);  // reads the last stage's output
       fi[s0 * nb + b].t = tf;
     }
     VdivArgs &va = vd[b].a;
@@ -975,6 +983,79 @@ int diagnostics(mgpu_handle *h, mgpu_diag *out) {
   return 0;
 }
 
+// k-chunks of k_visc, k_vdiv and the filter: every candidate timed on three
+// launches of the kernel on the case's own data (the filter writes into a
+// stage buffer, never into U^n).
+int tune_aux(mgpu_handle *h, bool force, hipEvent_t e0, hipEvent_t e1) {
+  const int nb = (int)h->b.size();
+  const bool curv = h->c.curvilinear != 0;
+  const int s0 = h->cur, sa = (h->cur + 1) % 3;
+  int maxnz = 0;
+  for (const auto &bd : h->b) maxnz = bd.g.nz > maxnz ? bd.g.nz : maxnz;
+  const int kcs[] = {0, 16, 32, 64, 128, 256};
+  int rc = 0;
+  for (int kind = 0; kind < 3 && !rc; kind++) {
+    if (kind < 2 && !h->c.iorder_visc) continue;
+    if (kind == 2 && h->c.filter == MGPU_FILTER_NONE) continue;
+    TuneKey key{h->device, h->c.curvilinear, 10 + 3 * h->c.filter + kind + (h->mixed ? 100 : 0), {}};
+    for (const auto &bd : h->b) key.dims.insert(key.dims.end(), {bd.g.nx, bd.g.ny, bd.g.nz});
+    if (!force) {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tune_cache.find(key);
+      if (it != g_tune_cache.end()) {
+        h->kc_aux[kind] = it->second.second;
+        h->tables_ready = false;
+        continue;
+      }
+    }
+    float best = 1e30f;
+    int best_kc = 0;
+    for (int kc : kcs) {
+      if (kc > 16 && kc / 2 >= maxnz) break;
+      h->kc_aux[kind] = kc;
+      h->tables_ready = false;
+      if ((rc = build_tables(h))) break;
+      if (kind == 2) {
+        std::vector<Batch<FilterArgs>> tab(nb);
+        int first = 0;
+        for (int b = 0; b < nb; b++) {
+          tab[b].a = filter_args(h, h->b[b], s0, sa);
+          tab[b].t = filter_tiles(h->b[b].g, kc);
+          tab[b].t.first = first;
+          first += tile_count(tab[b].t, h->b[b].g.nz);
+        }
+        HIPCK(h, hipMemcpy(h->d_tune_f, tab.data(), nb * sizeof(tab[0]), hipMemcpyHostToDevice));
+      }
+      auto launch = [&]() -> hipError_t {
+        if (kind == 0) return launch_visc(h->d_visc + s0 * nb, nb, h->nt_visc, curv, h->stream);
+        if (kind == 1) return launch_vdiv(h->d_vdiv, nb, h->nt_vdiv, curv, h->stream);
+        if (h->c.filter == MGPU_FILTER_SF) return launch_filter_sf(h->d_tune_f, nb, h->nt_filt, h->mixed, h->stream);
+        return launch_filter_shock(h->d_tune_f, nb, h->nt_filt, h->mixed, h->stream);
+      };
+      hipError_t e = launch();
+      if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
+      for (int r = 0; r < 3 && e == hipSuccess; r++) e = launch();
+      if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) {
+        rc = fail(h, MGPU_EDEVICE, "autotune launch failed: %s", hipGetErrorString(e));
+        break;
+      }
+      if (ms < best) {
+        best = ms;
+        best_kc = kc;
+      }
+    }
+    h->kc_aux[kind] = best_kc;
+    h->tables_ready = false;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = {0, best_kc};
+  }
+  return rc;
+}
+
 }  // namespace mgpu_host
 
 using namespace mgpu_host;
@@ -1125,6 +1206,7 @@ int mgpu_autotune(mgpu_handle *h, int flags) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[key] = {best_shape, best_kc};
   }
+  if (!rc) rc = tune_aux(h, force, e0, e1);
   h->ev_pool.push_back(e0);
   h->ev_pool.push_back(e1);
   h->tables_ready = false;  // rebuilt with the chosen tiles before the next step
@@ -1134,8 +1216,17 @@ int mgpu_autotune(mgpu_handle *h, int flags) {
 int mgpu_stage_config(mgpu_handle *h, int block, int variant, int *shape_tx, int *shape_ty, int *kchunk) {
   if (!h) return MGPU_EINVAL;
   if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
-  if (variant < 0 || variant > 1) return fail(h, MGPU_EINVAL, "variant %d", variant);
+  if (variant < 0 || variant > 4) return fail(h, MGPU_EINVAL, "variant %d", variant);
   const BlockData &bd = h->b[block];
+  if (variant >= 2) {
+    const int kind = variant - 2;
+    const TileRange t = kind == 0 ? visc_tiles(bd.g, h->kc_aux[0])
+                                  : (kind == 1 ? vdiv_tiles(bd.g, h->kc_aux[1]) : filter_tiles(bd.g, h->kc_aux[2]));
+    if (shape_tx) *shape_tx = 32;  // the viscous kernels and the filters use 32x16 tiles
+    if (shape_ty) *shape_ty = 16;
+    if (kchunk) *kchunk = t.kchunk;
+    return 0;
+  }
   if (shape_tx) *shape_tx = kStageShapes[h->shape[variant]][0];
   if (shape_ty) *shape_ty = kStageShapes[h->shape[variant]][1];
   if (kchunk) {

@@ -64,12 +64,15 @@ struct mgpu_handle {
   // means the per-block default
   int shape[2] = {2, 2};
   int kchunk[2] = {0, 0};
+  // planes per workgroup of k_visc, k_vdiv and the filter (0: per-block default)
+  int kc_aux[3] = {0, 0, 0};
   // batched launch tables in device memory, one entry per block
   mgpu::Batch<mgpu::StageArgs> *d_stage = nullptr;  // [cur][irk-1][block]
   mgpu::Batch<mgpu::ViscArgs> *d_visc = nullptr;    // [cur][block]
   mgpu::Batch<mgpu::VdivArgs> *d_vdiv = nullptr;    // [block]
   mgpu::Batch<mgpu::FilterArgs> *d_filt = nullptr;  // [cur][block]
   mgpu::Batch<mgpu::StageArgs> *d_tune = nullptr;   // [block], autotuner
+  mgpu::Batch<mgpu::FilterArgs> *d_tune_f = nullptr;  // [block], autotuner (writes a stage buffer)
   int tab_cap = 0;                                  // blocks the tables hold
   int nt_stage[2] = {0, 0}, nt_visc = 0, nt_vdiv = 0, nt_filt = 0;
   bool tables_ready = false;

@@ -251,11 +251,11 @@ extern const int kNumStageShapes;
 TileRange stage_tiles(const Geom &g, int shape, int kchunk);
 hipError_t launch_stage(const Batch<StageArgs> *tab, int nb, int ntiles, bool curv, bool visc, int shape,
                         int mode, hipStream_t s);
-TileRange visc_tiles(const Geom &g);
+TileRange visc_tiles(const Geom &g, int kchunk);  // kchunk 0: default
 hipError_t launch_visc(const Batch<ViscArgs> *tab, int nb, int ntiles, bool curv, hipStream_t s);
-TileRange vdiv_tiles(const Geom &g);
+TileRange vdiv_tiles(const Geom &g, int kchunk);
 hipError_t launch_vdiv(const Batch<VdivArgs> *tab, int nb, int ntiles, bool curv, hipStream_t s);
-TileRange filter_tiles(const Geom &g);
+TileRange filter_tiles(const Geom &g, int kchunk);
 hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, hipStream_t s);
 hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, hipStream_t s);
 hipError_t launch_iface(const IfaceArgs &a, int nchunks, hipStream_t s);
