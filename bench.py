@@ -136,6 +136,26 @@ def multiblock_leg(steps: int, warmup: int):
             "kernel_ms_per_step_eager": {k: round(ms / 2, 4) for k, (n, ms) in times.items()}}
 
 
+def tgv_leg(steps: int = 100):
+    """The TGV 64^3 (Cartesian, viscous, selective filter, RK4, CFL 0.8, 100
+    steps): graph replay, small enough to be launch-bound."""
+    from musica2_newgrid_amd import cases
+    from musica2_newgrid_amd.solver import GpuSolver
+    case = cases.tgv(64)
+    with GpuSolver(case) as s:
+        s.autotune()
+        s.step(5)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        t = time.perf_counter() - t0
+    return {"value": round(case.ncells * case.nrk * steps / t / 1e6, 2), "unit": "Mcell-RK-stages/s",
+            "ms_per_step": round(t / steps * 1e3, 4), "steps": steps,
+            "workload": "TGV 64^3, Cartesian periodic, M=0.1, Re=1600, viscous, selective filter chi=0.2, "
+                        "RK4, CFL 0.8, FP64, graph replay"}
+
+
 def serve_leg(nreq: int, nsteps: int, seed: int = 5):
     """Serving loop: a Poisson stream of TGV / isentropic-vortex requests on
     cubes of edge 64..256, Re log-uniform in [400, 4000], FP64 or mixed,
@@ -172,6 +192,7 @@ def main() -> int:
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--no-multiblock", action="store_true", help="skip the uneven 7-block leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
+    ap.add_argument("--no-tgv", action="store_true", help="skip the TGV 64^3 leg")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-steps", type=int, default=100)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -239,6 +260,9 @@ def main() -> int:
             "after_steps": args.warmup + args.steps,
             "kernel_ms": {k: {"launches": n, "ms": round(ms, 3)} for k, (n, ms) in mx["times"].items()},
         }
+    if not args.no_tgv:
+        print("bench: TGV 64^3 leg", file=sys.stderr, flush=True)
+        out["tgv64"] = tgv_leg()
     if not args.no_multiblock:
         print("bench: uneven 7-block leg", file=sys.stderr, flush=True)
         out["multiblock"] = multiblock_leg(args.steps, args.warmup)
