@@ -154,3 +154,15 @@ def test_rk6_drp_matches_oracle():
     e = relerr(Ug, Uc)
     print("relerr", e)
     assert e < 1e-12
+
+
+@pytest.mark.parametrize("shape", [(5, 6, 7), (33, 5, 9), (7, 17, 5)])
+def test_smallest_and_ragged_blocks(shape):
+    """Extents down to the 5 ghost layers and extents that leave partial tiles in
+    every direction, c3 with viscous terms and the shock filter."""
+    nx, ny, nz = shape
+    case = cases.vortex_wavy(nx, ny, nz, filter=2)
+    Ug, Uc = run_both(case, 3)
+    e = relerr(Ug, Uc)
+    print("relerr", e)
+    assert e < 1e-12

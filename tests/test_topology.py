@@ -51,3 +51,30 @@ def test_neighbour_out_of_range():
     blocks[0]["nbr_block"][0] = 7
     rc, msg = _create(parent, blocks)
     assert rc == _lib.EINVAL and "out of range" in msg
+
+
+def test_block_limits_are_checked():
+    """Extents below the 5 ghost layers and blocks above 2^28 padded points are
+    refused before any device work."""
+    parent, blocks = _table()
+    blocks[0]["nx"] = 4
+    rc, msg = _create(parent, blocks)
+    assert rc == _lib.EINVAL and ">= 5" in msg
+    big = [{"nx": 1024, "ny": 1024, "nz": 300, "kind": [0] * 6, "nbr_block": [0] * 6,
+            "nbr_face": [1, 0, 3, 2, 5, 4], "orient": [0] * 6}]
+    rc, msg = _create(parent, big)
+    assert rc == _lib.EINVAL and "2^28" in msg
+
+
+def test_flag_checks():
+    import copy
+    parent, blocks = _table()
+    c = copy.copy(parent)
+    c.filter = 2
+    c.iorder_visc = 0
+    rc, msg = _create(c, blocks)
+    assert rc == _lib.EINVAL and "shock filter needs" in msg
+    c = copy.copy(parent)
+    c.nrk = 9
+    rc, msg = _create(c, blocks)
+    assert rc == _lib.EINVAL and "nrk" in msg
