@@ -60,7 +60,7 @@ __global__ __launch_bounds__(kDiagThreads) void k_diag(DiagArgs a) {
         gr[c][2] = dvz * m[9];
       }
     } else {
-      const double ix = a.met.idx_v[i - 1 + G], iy = a.met.idy_v[j - 1 + G], iz = a.met.idz_v[k - 1 + G];
+      const double ix = ldg(a.met.idx_v, (idx_t)(i - 1 + G)), iy = ldg(a.met.idy_v, (idx_t)(j - 1 + G)), iz = ldg(a.met.idz_v, (idx_t)(k - 1 + G));
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         gr[c][0] = gd[0][c] * ix;

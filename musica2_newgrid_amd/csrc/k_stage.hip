@@ -324,9 +324,9 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
         if constexpr (CURV) {
           K[v] = (double)(dz + (dy + dx)) * ijac;
         } else {
-          const double kx = (double)dx * a.met.idx[i - 1 + G];
-          const double ky = (double)dy * a.met.idy[j - 1 + G] + kx;
-          K[v] = (double)dz * a.met.idz[k - 1 + G] + ky;
+          const double kx = (double)dx * ldg(a.met.idx, (idx_t)(i - 1 + G));
+          const double ky = (double)dy * ldg(a.met.idy, (idx_t)(j - 1 + G)) + kx;
+          K[v] = (double)dz * ldg(a.met.idz, (idx_t)(k - 1 + G)) + ky;
         }
       }
       if constexpr (VISC) {

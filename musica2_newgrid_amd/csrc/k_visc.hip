@@ -200,7 +200,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(const Batch<ViscArgs> *__res
           gr[c][2] = dvz * ij;
         }
       } else {
-        const double ix = a.met.idx_v[i - 1 + G], iy = a.met.idy_v[j - 1 + G], iz = a.met.idz_v[k - 1 + G];
+        const double ix = ldg(a.met.idx_v, (idx_t)(i - 1 + G)), iy = ldg(a.met.idy_v, (idx_t)(j - 1 + G)), iz = ldg(a.met.idz_v, (idx_t)(k - 1 + G));
 #pragma unroll
         for (int c = 0; c < 4; c++) {
           gr[c][0] = gd[0][c] * ix;
@@ -446,9 +446,9 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
       if constexpr (CURV) {
         mx = my = mz = ijv;
       } else {
-        mx = a.met.idx_v[i - 1 + G];
-        my = a.met.idy_v[j - 1 + G];
-        mz = a.met.idz_v[k - 1 + G];
+        mx = ldg(a.met.idx_v, (idx_t)(i - 1 + G));
+        my = ldg(a.met.idy_v, (idx_t)(j - 1 + G));
+        mz = ldg(a.met.idz_v, (idx_t)(k - 1 + G));
       }
 #pragma unroll
       for (int c = 0; c < 4; c++) {

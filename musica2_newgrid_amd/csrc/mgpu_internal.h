@@ -40,27 +40,32 @@ __host__ __device__ inline idx_t lin(const Geom &g, int i, int j, int k) {
   return (idx_t)(i + G - 1) + (idx_t)g.sx * (idx_t)(j + G - 1) + g.sy * (idx_t)(k + G - 1);
 }
 
-// Load/store at element offset i through a 32-bit byte offset.
-__device__ __forceinline__ double ldg(const double *b, idx_t i) {
-  return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(b) + (uint32_t)(i << 3));
+// Load/store at element offset i through a 32-bit byte offset.  The array
+// pointers reach the kernels through device-memory tables, so the compiler
+// cannot infer their address space: cast them to the global one, so that the
+// accesses are global (SGPR base + 32-bit VGPR offset, counted by vmcnt only)
+// instead of flat ones, which also count against lgkmcnt and make every LDS
+// wait a wait for the outstanding memory loads.
+#define MGPU_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const MGPU_GLOBAL T *gaddr(const T *b, idx_t i) {
+  return reinterpret_cast<const MGPU_GLOBAL T *>(reinterpret_cast<const MGPU_GLOBAL char *>(
+                                                     (const MGPU_GLOBAL T *)b) +
+                                                 (uint32_t)(i * (idx_t)sizeof(T)));
 }
-__device__ __forceinline__ void stg(double *b, idx_t i, double v) {
-  *reinterpret_cast<double *>(reinterpret_cast<char *>(b) + (uint32_t)(i << 3)) = v;
+template <class T>
+__device__ __forceinline__ MGPU_GLOBAL T *gaddr(T *b, idx_t i) {
+  return reinterpret_cast<MGPU_GLOBAL T *>(reinterpret_cast<MGPU_GLOBAL char *>((MGPU_GLOBAL T *)b) +
+                                           (uint32_t)(i * (idx_t)sizeof(T)));
 }
+__device__ __forceinline__ double ldg(const double *b, idx_t i) { return *gaddr(b, i); }
+__device__ __forceinline__ void stg(double *b, idx_t i, double v) { *gaddr(b, i) = v; }
 // Streaming store for results no kernel of the same launch reads back.
-__device__ __forceinline__ void stg_nt(double *b, idx_t i, double v) {
-  __builtin_nontemporal_store(v, reinterpret_cast<double *>(reinterpret_cast<char *>(b) + (uint32_t)(i << 3)));
-}
+__device__ __forceinline__ void stg_nt(double *b, idx_t i, double v) { __builtin_nontemporal_store(v, gaddr(b, i)); }
 // The same for FP32 arrays.
-__device__ __forceinline__ float ldg(const float *b, idx_t i) {
-  return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(b) + (uint32_t)(i << 2));
-}
-__device__ __forceinline__ void stg(float *b, idx_t i, float v) {
-  *reinterpret_cast<float *>(reinterpret_cast<char *>(b) + (uint32_t)(i << 2)) = v;
-}
-__device__ __forceinline__ void stg_nt(float *b, idx_t i, float v) {
-  __builtin_nontemporal_store(v, reinterpret_cast<float *>(reinterpret_cast<char *>(b) + (uint32_t)(i << 2)));
-}
+__device__ __forceinline__ float ldg(const float *b, idx_t i) { return *gaddr(b, i); }
+__device__ __forceinline__ void stg(float *b, idx_t i, float v) { *gaddr(b, i) = v; }
+__device__ __forceinline__ void stg_nt(float *b, idx_t i, float v) { __builtin_nontemporal_store(v, gaddr(b, i)); }
 
 // XCD-aware workgroup order.  gfx950 dispatches workgroup w to XCD w % 8;
 // each XCD has its own L2.  Renumber so that XCD x owns one contiguous range
