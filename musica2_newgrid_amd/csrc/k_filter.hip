@@ -105,6 +105,10 @@ __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__
   __shared__ R suy[5][FY + 10][FX];
   int i0, j0, kbeg, kend;
   const FilterArgs &a = tile_of(tab, nb, ntiles, i0, j0, kbeg, kend);
+  // the block's array pointers, read once from the table into registers
+  const CState uinp = a.uin;
+  const State uoutp = a.uout;
+  const double *const prsp = a.prs, *const sensp = a.sens;
   const Geom &g = a.g;
   const int tx = threadIdx.x % FX, ty = threadIdx.x / FX;
   const int i = i0 + tx, j = j0 + ty;
@@ -121,18 +125,18 @@ __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__
   R q[11][5];
   if (own) {
 #pragma unroll
-    for (int l = 0; l < 10; l++) ldr<MIX>(a.uin, own_off + (idx_t)(kbeg - 5 + l) * sz, rf, q[l]);
+    for (int l = 0; l < 10; l++) ldr<MIX>(uinp, own_off + (idx_t)(kbeg - 5 + l) * sz, rf, q[l]);
   }
   for (int k = kbeg; k <= kend; k++) {
     const idx_t pk = own_off + (idx_t)k * sz;
     R uc[5], uh[NS][5];
     double u64[5];  // mixed: the FP64 state of the own point, to which K is added
-    if (own) ldr<MIX>(a.uin, pk + 5 * sz, rf, q[10]);
-    if (!own && (needx || needy)) ldr<MIX>(a.uin, pk, rf, uc);
-    if (MIX && own) ld5(a.uin, pk, u64);
+    if (own) ldr<MIX>(uinp, pk + 5 * sz, rf, q[10]);
+    if (!own && (needx || needy)) ldr<MIX>(uinp, pk, rf, uc);
+    if (MIX && own) ld5(uinp, pk, u64);
 #pragma unroll
     for (int s = 0; s < NS; s++)
-      if (hl.on[s]) ldr<MIX>(a.uin, hl.off[s] + (idx_t)k * sz, rf, uh[s]);
+      if (hl.on[s]) ldr<MIX>(uinp, hl.off[s] + (idx_t)k * sz, rf, uh[s]);
     if (needx || needy) {
 #pragma unroll
       for (int v = 0; v < 5; v++) {
@@ -164,8 +168,8 @@ __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__
             dd[5] * (y[5 * FX] + y[-5 * FX]);
         K = K + dd[0] * q[5][v] + dd[1] * (q[6][v] + q[4][v]) + dd[2] * (q[7][v] + q[3][v]) +
             dd[3] * (q[8][v] + q[2][v]) + dd[4] * (q[9][v] + q[1][v]) + dd[5] * (q[10][v] + q[0][v]);
-        if constexpr (MIX) stg_nt(a.uout.v[v], pk, u64[v] + (double)K);
-        else stg_nt(a.uout.v[v], pk, q[5][v] + K);
+        if constexpr (MIX) stg_nt(uoutp.v[v], pk, u64[v] + (double)K);
+        else stg_nt(uoutp.v[v], pk, q[5][v] + K);
       }
     }
     __syncthreads();
@@ -207,6 +211,10 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
 
   int i0, j0, kbeg, kend;
   const FilterArgs &a = tile_of(tab, nb, ntiles, i0, j0, kbeg, kend);
+  // the block's array pointers, read once from the table into registers
+  const CState uinp = a.uin;
+  const State uoutp = a.uout;
+  const double *const prsp = a.prs, *const sensp = a.sens;
   const Geom &g = a.g;
   const int tid = threadIdx.x, tx = tid % FX, ty = tid / FX;
   const int i = i0 + tx, j = j0 + ty;
@@ -256,11 +264,11 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
   double qs[3];      // sensor at k-1..k+1
   if (own) {
 #pragma unroll
-    for (int l = 0; l < 10; l++) ldr<MIX>(a.uin, own_off + (idx_t)(kbeg - 5 + l) * sz, rf, q[l]);
+    for (int l = 0; l < 10; l++) ldr<MIX>(uinp, own_off + (idx_t)(kbeg - 5 + l) * sz, rf, q[l]);
 #pragma unroll
-    for (int l = 0; l < 4; l++) qp[l] = ldg(a.prs, own_off + (idx_t)(kbeg - 2 + l) * sz);
+    for (int l = 0; l < 4; l++) qp[l] = ldg(prsp, own_off + (idx_t)(kbeg - 2 + l) * sz);
 #pragma unroll
-    for (int l = 0; l < 2; l++) qs[l] = duc ? ldg(a.sens, own_off + (idx_t)(kbeg - 1 + l) * sz) : 1.0;
+    for (int l = 0; l < 2; l++) qs[l] = duc ? ldg(sensp, own_off + (idx_t)(kbeg - 1 + l) * sz) : 1.0;
   }
 
   for (int k = kbeg; k <= kend; k++) {
@@ -270,20 +278,20 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
     double pv[NSP], sv[NSP];
     double u64[5];  // mixed: the FP64 state of the own point, to which K is added
     if (own) {
-      ldr<MIX>(a.uin, pk + 5 * sz, rf, q[10]);
-      if constexpr (MIX) ld5(a.uin, pk, u64);
-      qp[4] = ldg(a.prs, pk + 2 * sz);
-      qs[2] = duc ? ldg(a.sens, pk + sz) : 1.0;
+      ldr<MIX>(uinp, pk + 5 * sz, rf, q[10]);
+      if constexpr (MIX) ld5(uinp, pk, u64);
+      qp[4] = ldg(prsp, pk + 2 * sz);
+      qs[2] = duc ? ldg(sensp, pk + sz) : 1.0;
     }
-    if (!own && (needx || needy)) ldr<MIX>(a.uin, pk, rf, uc);
+    if (!own && (needx || needy)) ldr<MIX>(uinp, pk, rf, uc);
 #pragma unroll
     for (int s = 0; s < NS; s++)
-      if (hl.on[s]) ldr<MIX>(a.uin, hl.off[s] + (idx_t)k * sz, rf, uh[s]);
+      if (hl.on[s]) ldr<MIX>(uinp, hl.off[s] + (idx_t)k * sz, rf, uh[s]);
 #pragma unroll
     for (int s = 0; s < NSP; s++) {
       if (pp_on[s]) {
-        pv[s] = ldg(a.prs, pp_off[s] + (idx_t)k * sz);
-        sv[s] = (duc && pp_s[s]) ? ldg(a.sens, pp_off[s] + (idx_t)k * sz) : 1.0;
+        pv[s] = ldg(prsp, pp_off[s] + (idx_t)k * sz);
+        sv[s] = (duc && pp_s[s]) ? ldg(sensp, pp_off[s] + (idx_t)k * sz) : 1.0;
       }
     }
     // ---- stage the plane in LDS ----
@@ -379,8 +387,8 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
       }
 #pragma unroll
       for (int v = 0; v < 5; v++) {
-        if constexpr (MIX) stg_nt(a.uout.v[v], pk, u64[v] + (double)K[v]);
-        else stg_nt(a.uout.v[v], pk, q[5][v] + K[v]);
+        if constexpr (MIX) stg_nt(uoutp.v[v], pk, u64[v] + (double)K[v]);
+        else stg_nt(uoutp.v[v], pk, q[5][v] + K[v]);
       }
     }
     __syncthreads();

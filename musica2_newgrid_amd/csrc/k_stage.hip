@@ -44,27 +44,78 @@ struct Raw {
   double m[3];
 };
 
+// Array pointers one stage launch uses for its block.
+struct Ptrs {
+  const double *uin[5];
+  const float *uin32[5];
+  const double *u0[5];
+  double *uout[5];
+  float *uout32[5];
+  const double *m[10];
+  const double *kv[4];
+  const double *idx, *idy, *idz;
+  double ref[5];
+};
+
+template <bool CURV, bool VISC, bool IN32, bool OUT32>
+__device__ __forceinline__ void load_ptrs(const StageArgs &a, Ptrs &P) {
+#pragma unroll
+  for (int v = 0; v < 5; v++) {
+    if constexpr (IN32) P.uin32[v] = a.uin32[v];
+    else P.uin[v] = a.uin.v[v];
+    if constexpr (OUT32) P.uout32[v] = a.uout32[v];
+    else P.uout[v] = a.uout.v[v];
+    P.u0[v] = a.u0.v[v];
+  }
+  if constexpr (CURV) {
+#pragma unroll
+    for (int q = 0; q < 10; q++) P.m[q] = a.met.m[q];
+  } else {
+    P.idx = a.met.idx;
+    P.idy = a.met.idy;
+    P.idz = a.met.idz;
+  }
+  if constexpr (VISC) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) P.kv[c] = a.kv[c];
+  }
+  if constexpr (IN32 || OUT32) {
+    P.ref[0] = a.ref.r;
+    P.ref[1] = a.ref.ru;
+    P.ref[2] = a.ref.rv;
+    P.ref[3] = a.ref.rw;
+    P.ref[4] = a.ref.re;
+  }
+}
+
 // Stage state at element p: FP64, or (IN32) the reference state plus the
 // stored FP32 difference.
 template <bool IN32>
-__device__ __forceinline__ void load_u(const StageArgs &a, idx_t p, double u[5]) {
+__device__ __forceinline__ void load_u(const Ptrs &P, idx_t p, double u[5]) {
   if constexpr (IN32) {
-    const double ref[5] = {a.ref.r, a.ref.ru, a.ref.rv, a.ref.rw, a.ref.re};
 #pragma unroll
-    for (int v = 0; v < 5; v++) u[v] = ref[v] + (double)ldg(a.uin32[v], p);
+    for (int v = 0; v < 5; v++) u[v] = P.ref[v] + (double)ldg(P.uin32[v], p);
   } else {
 #pragma unroll
-    for (int v = 0; v < 5; v++) u[v] = ldg(a.uin.v[v], p);
+    for (int v = 0; v < 5; v++) u[v] = ldg(P.uin[v], p);
   }
 }
 
 template <bool CURV, bool IN32>
-__device__ __forceinline__ void load_raw(const StageArgs &a, const Metrics &M, idx_t p, int dir, Raw &r) {
-  load_u<IN32>(a, p, r.u);
+__device__ __forceinline__ void load_raw(const Ptrs &P, idx_t p, int dir, Raw &r) {
+  load_u<IN32>(P, p, r.u);
   if constexpr (CURV) {
 #pragma unroll
-    for (int c = 0; c < 3; c++) r.m[c] = ldg(M.m[3 * dir + c], p);
+    for (int c = 0; c < 3; c++) r.m[c] = ldg(P.m[3 * dir + c], p);
   }
+}
+
+// Raw inputs of a halo cell, whose direction is known at run time only:
+// branch to a constant direction so that the pointer table stays in registers.
+template <bool CURV, bool IN32>
+__device__ __forceinline__ void load_raw_halo(const Ptrs &P, idx_t p, int dir, Raw &r) {
+  if (dir == 0) load_raw<CURV, IN32>(P, p, 0, r);
+  else load_raw<CURV, IN32>(P, p, 1, r);
 }
 
 __device__ __forceinline__ Prim prim_raw(const double u[5], double gam1) {
@@ -170,6 +221,11 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
   const int w = xcd_remap(blockIdx.x, ntiles);
   const Batch<StageArgs> &e = batch_find(tab, nb, w);
   const StageArgs &a = e.a;
+  // the array pointers of the block, read once: they stay in scalar
+  // registers for the whole march instead of being re-read from the table
+  // in every plane
+  Ptrs P;
+  load_ptrs<CURV, VISC, IN32, OUT32>(a, P);
   const Geom &g = a.g;
   int tile_x, tile_y, chunk;
   tile_coords(e.t, w, tile_x, tile_y, chunk);
@@ -221,12 +277,13 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
   const idx_t sz = g.sy;
 
   const RefState &rf = a.ref;
+  const double alpha = a.alpha;
   R q[11][5];
   if (own) {
 #pragma unroll
     for (int l = 0; l < 10; l++) {
       Raw r;
-      load_raw<CURV, IN32>(a, a.met, own_off + (idx_t)(kbeg - 5 + l) * sz, 2, r);
+      load_raw<CURV, IN32>(P, own_off + (idx_t)(kbeg - 5 + l) * sz, 2, r);
       flux_r<CURV, MIX>(prim_raw(r.u, gam1), 2, r.m, rf, q[l]);
     }
   }
@@ -239,26 +296,26 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
     Raw rq, rh0;
     double rxm[3] = {0, 0, 0}, rym[3] = {0, 0, 0}, ru[5] = {0, 0, 0, 0, 0}, u0[5] = {0, 0, 0, 0, 0};
     double ijac = 0.0;
-    if (own) load_raw<CURV, IN32>(a, a.met, pk + 5 * sz, 2, rq);
+    if (own) load_raw<CURV, IN32>(P, pk + 5 * sz, 2, rq);
     if (needx || needy) {
-      load_u<IN32>(a, pk, ru);
+      load_u<IN32>(P, pk, ru);
       if constexpr (CURV) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          rxm[c] = ldg(a.met.m[c], pk);
-          rym[c] = ldg(a.met.m[3 + c], pk);
+          rxm[c] = ldg(P.m[c], pk);
+          rym[c] = ldg(P.m[3 + c], pk);
         }
       }
     }
-    if (hs_on[0]) load_raw<CURV, IN32>(a, a.met, hs_off[0] + (idx_t)k * sz, hs_dir[0], rh0);
+    if (hs_on[0]) load_raw_halo<CURV, IN32>(P, hs_off[0] + (idx_t)k * sz, hs_dir[0], rh0);
     double kvis[4] = {0, 0, 0, 0};
     if (own) {
 #pragma unroll
-      for (int v = 0; v < 5; v++) u0[v] = ldg(a.u0.v[v], pk);
-      if constexpr (CURV) ijac = ldg(a.met.m[9], pk);
+      for (int v = 0; v < 5; v++) u0[v] = ldg(P.u0[v], pk);
+      if constexpr (CURV) ijac = ldg(P.m[9], pk);
       if constexpr (VISC) {
 #pragma unroll
-        for (int c = 0; c < 4; c++) kvis[c] = ldg(a.kv[c], pk);
+        for (int c = 0; c < 4; c++) kvis[c] = ldg(P.kv[c], pk);
       }
     }
     if (own) flux_r<CURV, MIX>(prim_raw(rq.u, gam1), 2, rq.m, rf, q[10]);
@@ -293,7 +350,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
       Raw rh[NS];
 #pragma unroll
       for (int s = 1; s < NS; s++)
-        if (hs_on[s]) load_raw<CURV, IN32>(a, a.met, hs_off[s] + (idx_t)k * sz, hs_dir[s], rh[s]);
+        if (hs_on[s]) load_raw_halo<CURV, IN32>(P, hs_off[s] + (idx_t)k * sz, hs_dir[s], rh[s]);
 #pragma unroll
       for (int s = 1; s < NS; s++) {
         if (hs_on[s]) {
@@ -324,9 +381,9 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
         if constexpr (CURV) {
           K[v] = (double)(dz + (dy + dx)) * ijac;
         } else {
-          const double kx = (double)dx * ldg(a.met.idx, (idx_t)(i - 1 + G));
-          const double ky = (double)dy * ldg(a.met.idy, (idx_t)(j - 1 + G)) + kx;
-          K[v] = (double)dz * ldg(a.met.idz, (idx_t)(k - 1 + G)) + ky;
+          const double kx = (double)dx * ldg(P.idx, (idx_t)(i - 1 + G));
+          const double ky = (double)dy * ldg(P.idy, (idx_t)(j - 1 + G)) + kx;
+          K[v] = (double)dz * ldg(P.idz, (idx_t)(k - 1 + G)) + ky;
         }
       }
       if constexpr (VISC) {
@@ -336,10 +393,10 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
       if constexpr (OUT32) {
         const double ref[5] = {rf.r, rf.ru, rf.rv, rf.rw, rf.re};
 #pragma unroll
-        for (int v = 0; v < 5; v++) stg_nt(a.uout32[v], pk, (float)((u0[v] - a.alpha * K[v]) - ref[v]));
+        for (int v = 0; v < 5; v++) stg_nt(P.uout32[v], pk, (float)((u0[v] - alpha * K[v]) - ref[v]));
       } else {
 #pragma unroll
-        for (int v = 0; v < 5; v++) stg_nt(a.uout.v[v], pk, u0[v] - a.alpha * K[v]);
+        for (int v = 0; v < 5; v++) stg_nt(P.uout[v], pk, u0[v] - alpha * K[v]);
       }
     }
 

@@ -62,6 +62,16 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(const Batch<ViscArgs> *__res
   const int w = xcd_remap(blockIdx.x, ntiles);
   const auto &e = batch_find(tab, nb, w);
   const auto &a = e.a;
+  // the block's array pointers, read once from the table into registers
+  const CState u0p = a.u0;
+  const double *mvp[10];
+#pragma unroll
+  for (int c = 0; c < 10; c++) mvp[c] = a.met.mv[c];
+  double *taup[NTAU];
+#pragma unroll
+  for (int c = 0; c < NTAU; c++) taup[c] = a.tau[c];
+  double *const prsp = a.prs, *const sensp = a.sens;
+  const double *const idxv = a.met.idx_v, *const idyv = a.met.idy_v, *const idzv = a.met.idz_v;
   const Geom &g = a.g;
   int tile_x, tile_y, chunk;
   tile_coords(e.t, w, tile_x, tile_y, chunk);
@@ -114,7 +124,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(const Batch<ViscArgs> *__res
 #pragma unroll
     for (int l = 0; l < 4; l++) {
       double U[5];
-      ld5(a.u0, own_off + (idx_t)(kbeg - 2 + l) * sz, U);
+      ld5(u0p, own_off + (idx_t)(kbeg - 2 + l) * sz, U);
       q[l] = prim5r(U, gam1, rg);
     }
   }
@@ -123,14 +133,14 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(const Batch<ViscArgs> *__res
     const idx_t pk = own_off + (idx_t)k * sz;
     // ---- every global load of the plane ----
     double uq[5], uc[5], uh[NS][5], mvv[10];
-    if (own) ld5(a.u0, pk + 2 * sz, uq);
-    if (!own && (needx || needy)) ld5(a.u0, pk, uc);
+    if (own) ld5(u0p, pk + 2 * sz, uq);
+    if (!own && (needx || needy)) ld5(u0p, pk, uc);
 #pragma unroll
     for (int s = 0; s < NS; s++)
-      if (hs_on[s]) ld5(a.u0, hs_off[s] + (idx_t)k * sz, uh[s]);
+      if (hs_on[s]) ld5(u0p, hs_off[s] + (idx_t)k * sz, uh[s]);
     if (CURV && own) {
 #pragma unroll
-      for (int c = 0; c < 10; c++) mvv[c] = ldg(a.met.mv[c], pk);
+      for (int c = 0; c < 10; c++) mvv[c] = ldg(mvp[c], pk);
     }
 
     // ---- primitives into the queue and the LDS pencils ----
@@ -200,7 +210,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(const Batch<ViscArgs> *__res
           gr[c][2] = dvz * ij;
         }
       } else {
-        const double ix = ldg(a.met.idx_v, (idx_t)(i - 1 + G)), iy = ldg(a.met.idy_v, (idx_t)(j - 1 + G)), iz = ldg(a.met.idz_v, (idx_t)(k - 1 + G));
+        const double ix = ldg(idxv, (idx_t)(i - 1 + G)), iy = ldg(idyv, (idx_t)(j - 1 + G)), iz = ldg(idzv, (idx_t)(k - 1 + G));
 #pragma unroll
         for (int c = 0; c < 4; c++) {
           gr[c][0] = gd[0][c] * ix;
@@ -226,18 +236,18 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(const Batch<ViscArgs> *__res
       t13 = mu * t13;
       t23 = mu * t23;
 
-      stg(a.tau[0], p, t11);
-      stg(a.tau[1], p, t12);
-      stg(a.tau[2], p, t13);
-      stg(a.tau[3], p, t22);
-      stg(a.tau[4], p, t23);
-      stg(a.tau[5], p, t33);
-      stg(a.tau[6], p, (u * t11 + v * t12 + w * t13) - ck * gr[3][0]);
-      stg(a.tau[7], p, (u * t12 + v * t22 + w * t23) - ck * gr[3][1]);
-      stg(a.tau[8], p, (u * t13 + v * t23 + w * t33) - ck * gr[3][2]);
+      stg(taup[0], p, t11);
+      stg(taup[1], p, t12);
+      stg(taup[2], p, t13);
+      stg(taup[3], p, t22);
+      stg(taup[4], p, t23);
+      stg(taup[5], p, t33);
+      stg(taup[6], p, (u * t11 + v * t12 + w * t13) - ck * gr[3][0]);
+      stg(taup[7], p, (u * t12 + v * t22 + w * t23) - ck * gr[3][1]);
+      stg(taup[8], p, (u * t13 + v * t23 + w * t33) - ck * gr[3][2]);
 
       if (a.want_sensor) {
-        stg(a.prs, p, q[2].p);
+        stg(prsp, p, q[2].p);
         double s = 1.0;
         if (a.ducros) {
           const double dv = gr[0][0] + gr[1][1] + gr[2][2];
@@ -247,7 +257,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(const Batch<ViscArgs> *__res
           const double o3 = gr[1][0] - gr[0][1];
           s = div2 / (div2 + o1 * o1 + o2 * o2 + o3 * o3 + 1.0e-16);
         }
-        stg(a.sens, p, s);
+        stg(sensp, p, s);
       }
     }
     __syncthreads();
@@ -313,6 +323,17 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
   const int w = xcd_remap(blockIdx.x, ntiles);
   const auto &e = batch_find(tab, nb, w);
   const auto &a = e.a;
+  // the block's array pointers, read once from the table into registers
+  const double *mvp[10];
+#pragma unroll
+  for (int c = 0; c < 10; c++) mvp[c] = a.met.mv[c];
+  const double *taup[NTAU];
+#pragma unroll
+  for (int c = 0; c < NTAU; c++) taup[c] = a.tau[c];
+  double *kvp[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) kvp[c] = a.kv[c];
+  const double *const idxv = a.met.idx_v, *const idyv = a.met.idy_v, *const idzv = a.met.idz_v;
   const Geom &g = a.g;
   int tile_x, tile_y, chunk;
   tile_coords(e.t, w, tile_x, tile_y, chunk);
@@ -325,7 +346,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
   const bool own = i <= g.nx && j <= g.ny;
   const bool needx = i <= g.nx + 2 && j <= g.ny, needy = i <= g.nx && j <= g.ny + 2;
   const idx_t sz = g.sy, own_off = lin(g, i, j, 0);
-  const double *const *mv = a.met.mv;
+  const double *const *mv = mvp;
 
   int hs_r[NS], hs_c[NS];
   idx_t hs_off[NS];
@@ -362,7 +383,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
   auto flux_at = [&](int d, idx_t p, double f[4]) {
     double t[NL];
 #pragma unroll
-    for (int c = 0; c < NL; c++) t[c] = ldg(a.tau[CURV ? c : krow(d, c)], p);
+    for (int c = 0; c < NL; c++) t[c] = ldg(taup[CURV ? c : krow(d, c)], p);
     double mx = 0.0, my = 0.0, mz = 0.0;
     if constexpr (CURV) {
       mx = ldg(mv[3 * d], p);
@@ -396,7 +417,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
     for (int l = 0; l < 2; l++) {
       const idx_t p = own_off + (idx_t)(kbeg + l) * sz;
 #pragma unroll
-      for (int c = 0; c < NTAU; c++) tw[l][c] = ldg(a.tau[c], p);
+      for (int c = 0; c < NTAU; c++) tw[l][c] = ldg(taup[c], p);
       flux_of(2, tw[l], p, q[2 + l]);
     }
   }
@@ -407,7 +428,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
     if (own) {
       const idx_t p2 = pk + 2 * sz;
 #pragma unroll
-      for (int c = 0; c < NTAU; c++) tw[2][c] = ldg(a.tau[c], p2);
+      for (int c = 0; c < NTAU; c++) tw[2][c] = ldg(taup[c], p2);
       flux_of(2, tw[2], p2, q[4]);
       if constexpr (CURV) ijv = ldg(mv[9], pk);
       flux_of(0, tw[0], pk, fx);
@@ -446,9 +467,9 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
       if constexpr (CURV) {
         mx = my = mz = ijv;
       } else {
-        mx = ldg(a.met.idx_v, (idx_t)(i - 1 + G));
-        my = ldg(a.met.idy_v, (idx_t)(j - 1 + G));
-        mz = ldg(a.met.idz_v, (idx_t)(k - 1 + G));
+        mx = ldg(idxv, (idx_t)(i - 1 + G));
+        my = ldg(idyv, (idx_t)(j - 1 + G));
+        mz = ldg(idzv, (idx_t)(k - 1 + G));
       }
 #pragma unroll
       for (int c = 0; c < 4; c++) {
@@ -457,7 +478,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
         double kv = (A5_1 * (x[1] - x[-1]) + A5_2 * (x[2] - x[-2])) * mx;
         kv = (A5_1 * (y[TX] - y[-TX]) + A5_2 * (y[2 * TX] - y[-2 * TX])) * my + kv;
         kv = (A5_1 * (q[3][c] - q[1][c]) + A5_2 * (q[4][c] - q[0][c])) * mz + kv;
-        stg(a.kv[c], pk, kv);
+        stg(kvp[c], pk, kv);
       }
     }
     __syncthreads();
