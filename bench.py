@@ -201,7 +201,7 @@ def main() -> int:
     if args.copy_calibration:
         return copy_calibration()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "bench.py runs on one GPU only; --gpus must be 1"}))
         return 2
 
     import numpy as np

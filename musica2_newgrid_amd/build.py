@@ -42,7 +42,7 @@ def flang() -> str:
 
 
 def build_fortran(verbose: bool = False) -> None:
-    """The Fortran host: binding module, TGV driver and param.ini checker."""
+    """The Fortran host: binding module, TGV driver and param.ini reader check."""
     os.makedirs(BIN, exist_ok=True)
     mod_dir = os.path.join(HERE, "_build", "fmod")
     os.makedirs(mod_dir, exist_ok=True)

@@ -1,8 +1,8 @@
 /*
  * musica_oracle.c -- CPU restatement of MUSICAA's explicit RK time step.
  *
- * TEST INFRASTRUCTURE ONLY (the checker of the HIP product path; see the
- * header).  Plain C99 + OpenMP over k-planes, FP64 throughout, no FMA
+ * TEST INFRASTRUCTURE ONLY (what the tests compare the HIP product path
+ * with; see the header).  Plain C99 + OpenMP over k-planes, FP64 throughout, no FMA
  * contraction (built with -ffp-contract=off), evaluation order of every sum
  * as in the reference's Fortran so that results track the reference to
  * round-off.

@@ -1,8 +1,8 @@
 /*
  * musica_oracle.h -- CPU restatement of MUSICAA's explicit RK time step.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
- * product path: only tests/, __graft_entry__.smoke() and bench.py's
+ * TEST INFRASTRUCTURE ONLY.  The tests compare the HIP product path with
+ * this library: only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it.  Nothing in musica2_newgrid_amd/ links or
  * calls it.
  *

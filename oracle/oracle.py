@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product path never imports
+cpu_baseline leg of bench.py, as the reference the results are compared with.  The product path never imports
 this module.
 """
 from __future__ import annotations
