@@ -10,6 +10,8 @@ step and the initial conserved state.  Builders:
 * :func:`vortex_wavy`    -- isentropic vortex advected over a curvilinear
                             (wavy, 3-D GCL metrics) block, periodic faces
 * :func:`entropy_wave`   -- small-amplitude entropy/shear waves for closed-form tests
+* :func:`channel`        -- Cartesian channel: periodic x and z, no-slip walls at
+                            y = 0 and y = Ly on a stretched grid
 
 The time step follows timestep_new (src/Time/timestep_new.f90:270-288):
 deltat = CFL * delta_min / max(|u| + c).
@@ -84,6 +86,13 @@ class Case:
     mv: Optional[List[np.ndarray]] = None
     # initial conserved state: rho, rhou, rhov, rhow, rhoe (padded)
     U: Optional[List[np.ndarray]] = None
+    # face kinds imin..kmax (0 periodic, 2 no-slip wall) and the wall condition
+    faces: tuple = (0, 0, 0, 0, 0, 0)
+    adiabatic: bool = True
+    t_wall: float = 0.0
+    # wall extrapolation coefficients per face (cextrp2, cextrp3)
+    cextrp2: tuple = (0.0,) * 6
+    cextrp3: tuple = (0.0,) * 6
     info: dict = dataclasses.field(default_factory=dict)
 
     @property
@@ -246,6 +255,78 @@ def cartesian_box(nx, ny, nz, lengths, fields, iorder_visc=0, filter=0, diffscal
     U = _to_padded(conserved(*fields), nx, ny, nz)
     return Case("box", nx, ny, nz, curvilinear=False, iorder_visc=iorder_visc, filter=filter,
                 ducros=False, diffscale=diffscale, cart=cart, U=U, dt=dt, chi=chi)
+
+
+def stretched_wall_coords(n: int, L: float, beta: float) -> np.ndarray:
+    """Nodes of rows 1..n on [0, L] clustered at both ends (tanh), padded with
+    G ghost coordinates of the end spacings (unused next to walls)."""
+    s = np.arange(n) / (n - 1)
+    y = 0.5 * L * (1.0 + np.tanh(beta * (2.0 * s - 1.0)) / math.tanh(beta))
+    ye = np.empty(n + 2 * G)
+    ye[G:G + n] = y
+    ye[:G] = y[0] - (y[1] - y[0]) * np.arange(G, 0, -1)
+    ye[G + n:] = y[-1] + (y[-1] - y[-2]) * np.arange(1, G + 1)
+    return ye
+
+
+def channel(nx: int = 64, ny: int = 48, nz: int = 16, lengths=(4.0, 2.0, 1.0), beta: float = 1.5,
+            mach: float = 0.3, re: float = 2000.0, cfl: float = 0.5, chi: float = 0.2, filter: int = 1,
+            iorder_visc: int = 4, adiabatic: bool = True, t_wall: float = 300.0, pulse: float = 1e-2,
+            seed: int = 20261202, T_ref: float = 300.0, rho_ref: float = 1.177,
+            symmetric: bool = False) -> Case:
+    """Channel between no-slip walls at y = 0 and y = Ly (rows 1 and ny), periodic in
+    x and z, y-nodes clustered at the walls.  Flow: a parabolic streamwise profile
+    of Mach mach, 3-D velocity perturbations vanishing at the walls, and a Gaussian
+    pressure pulse; temperature T_ref (the isothermal wall temperature t_wall).
+    symmetric: the flow is mirror-symmetric about y = Ly/2 (v antisymmetric)."""
+    Lx, Ly, Lz = lengths
+    dx, dz = Lx / nx, Lz / nz
+    ye = stretched_wall_coords(ny, Ly, beta)
+    y = ye[G:G + ny]
+    x = np.arange(nx) * dx
+    z = np.arange(nz) * dz
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    c_ref = math.sqrt(GAM * RG * T_ref)
+    u_ref = mach * c_ref
+    p_ref = rho_ref * RG * T_ref
+    eta = Y / Ly
+    env = 4.0 * eta * (1.0 - eta)                      # 0 on both walls
+    rng = np.random.default_rng(seed)
+    ph = rng.uniform(0.0, 2.0 * math.pi, 6)
+    kx, kz = 2.0 * math.pi / Lx, 2.0 * math.pi / Lz
+    u = u_ref * (env + 0.05 * env * np.sin(kx * X + ph[0]) * np.cos(kz * Z + ph[1]))
+    v = 0.05 * u_ref * env * env * np.sin(2.0 * kx * X + ph[2]) * np.sin(kz * Z + ph[3])
+    if symmetric:
+        v = v * (1.0 - 2.0 * eta)
+    w = 0.05 * u_ref * env * np.cos(kx * X + ph[4]) * np.sin(2.0 * kz * Z + ph[5])
+    yc = 0.5 * Ly if symmetric else 0.4 * Ly
+    r2 = ((X - 0.5 * Lx) ** 2 + (Y - yc) ** 2 + (Z - 0.5 * Lz) ** 2) / (0.1 * Ly) ** 2
+    p = p_ref * (1.0 + pulse * np.exp(-math.log(2.0) * r2))
+    T = np.full_like(p, T_ref)
+    rho = p / (RG * T)
+    U = []
+    for f in conserved(rho, u, v, w, p):
+        a = gr.padded(nx, ny, nz)
+        a[G:-G, G:-G, G:-G] = f
+        a[G:-G, :G, G:-G] = f[:, :1, :]                # ghosts beyond the walls: never read
+        a[G:-G, G + ny:, G:-G] = f[:, -1:, :]
+        gr.fill_periodic(a, axes=(0, 2))
+        U.append(a)
+    ide, idv, c2, c3 = gr.wall_metrics(ye, True, True)
+    idx = np.full(nx + 2 * G, 1.0 / dx)
+    idz = np.full(nz + 2 * G, 1.0 / dz)
+    mu_ref = rho_ref * u_ref * (0.5 * Ly) / re
+    muw_ref = float(sutherland(np.float64(T_ref)))
+    ext2 = [0.0] * 6
+    ext3 = [0.0] * 6
+    ext2[2], ext2[3], ext3[2], ext3[3] = c2[0], c2[1], c3[0], c3[1]
+    c = Case("channel", nx, ny, nz, curvilinear=False, iorder_visc=iorder_visc, filter=filter, ducros=False,
+             diffscale=mu_ref / muw_ref, chi=chi, cart=[idx, ide, idz, idx.copy(), idv, idz.copy()], U=U,
+             faces=(0, 0, 2, 2, 0, 0), adiabatic=adiabatic, t_wall=t_wall,
+             cextrp2=tuple(ext2), cextrp3=tuple(ext3))
+    c.dt = timestep(U, min(dx, dz, float(np.min(np.diff(y)))), cfl)
+    c.info = dict(mach=mach, re=re, cfl=cfl, beta=beta, lengths=lengths, y=y, u_ref=u_ref, p_ref=p_ref)
+    return c
 
 
 def affine_c3_metrics(nx, ny, nz, lengths, viscous=True):

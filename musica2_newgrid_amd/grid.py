@@ -156,3 +156,75 @@ def curvilinear_metrics_periodic(mapping, nx, ny, nz, lengths, viscous=True):
         out.append(None)
     coords = tuple(_crop(a, ext, 0) for a in (xe, ye, ze))
     return out[0], out[1], coords
+
+
+# Wall closures of the Cartesian metrics (src/Grid/grid_metrics_cart.f90:2-186,
+# 189-436; src/Derivatives/mod_coeff_deriv.f90:100-160, 232-241, 1099-1160).
+# a3, a5, a7, a9: centred schemes of half-width 1..4 used on rows 2..5 from a wall.
+A_CLOSE = ((0.5,), A5, (3.0 / 4.0, -3.0 / 20.0, 1.0 / 60.0),
+           (4.0 / 5.0, -1.0 / 5.0, 4.0 / 105.0, -1.0 / 280.0), A11)
+A02 = (-3.0 / 2.0, 2.0, -1.0 / 2.0)
+A04 = (-25.0 / 12.0, 4.0, -3.0, 4.0 / 3.0, -1.0 / 4.0)
+A13 = (-1.0 / 4.0, -5.0 / 6.0, 3.0 / 2.0, -1.0 / 2.0, 1.0 / 12.0)
+
+
+def wall_metrics(ye: np.ndarray, lo_wall: bool, hi_wall: bool):
+    """Metrics of one axis whose min and/or max face is a wall, from the node
+    coordinates ye of rows 1-G..n+G (padded; the wall rows are rows 1 and n,
+    ghost coordinates beyond a wall are never used, those beyond an interface
+    are the neighbour's).
+
+    Returns (ide, idv, c2, c3):
+      ide  Euler metric as the flux closures use it (length n+2G): 1/(11-pt
+           derivative of y) inside, the idy2/idy4/idy6/idy8 of
+           grid_metrics_wall_cart (centred a3/a5/a7/a9) on rows 2..5 (n-1..n-4),
+           1/(a02 y) on the wall row (unused: wall rows get no Euler increment);
+      idv  viscous metric (grid_metrics_cart with ngh=2): 1/(5-pt derivative),
+           one-sided a04/a13 (a31/a40) on rows 1,2 (n-1,n);
+      c2, c3  extrapolation coefficients cextrp2, cextrp3 of the min (key 0)
+           and max (key 1) face (init_coeff_wall_cart).
+    Ghost entries of ide/idv repeat the nearest row."""
+    n = len(ye) - 2 * G
+    y = lambda r: float(ye[G + r - 1])  # row r = 1..n (and ghosts)
+    ide = np.zeros(n + 2 * G)
+    idv = np.zeros(n + 2 * G)
+    for r in range(1, n + 1):
+        q = 5
+        if lo_wall:
+            q = min(q, r - 1)
+        if hi_wall:
+            q = min(q, n - r)
+        if q == 0:
+            if r == 1:
+                d = A02[0] * y(1) + A02[1] * y(2) + A02[2] * y(3)
+            else:
+                d = -(A02[0] * y(n) + A02[1] * y(n - 1) + A02[2] * y(n - 2))
+        else:
+            d = 0.0
+            for l, al in enumerate(A_CLOSE[q - 1], start=1):
+                d += al * (y(r + l) - y(r - l))
+        ide[G + r - 1] = 1.0 / d
+        if lo_wall and r == 1:
+            dv = sum(c * y(1 + m) for m, c in enumerate(A04))
+        elif lo_wall and r == 2:
+            dv = sum(c * y(1 + m) for m, c in enumerate(A13))
+        elif hi_wall and r == n - 1:
+            dv = -sum(c * y(n - m) for m, c in enumerate(A13))
+        elif hi_wall and r == n:
+            dv = -sum(c * y(n - m) for m, c in enumerate(A04))
+        else:
+            dv = A5[0] * (y(r + 1) - y(r - 1)) + A5[1] * (y(r + 2) - y(r - 2))
+        idv[G + r - 1] = 1.0 / dv
+    for a in (ide, idv):
+        a[:G] = a[G]
+        a[G + n:] = a[G + n - 1]
+    c2, c3 = {}, {}
+    if lo_wall:
+        ry = (y(3) - y(2)) / (y(2) - y(1))
+        c2[0] = (1.0 + ry) ** 2 / (ry * (2.0 + ry))
+        c3[0] = 1.0 / (ry * (2.0 + ry))
+    if hi_wall:
+        ry = (y(n - 2) - y(n - 1)) / (y(n - 1) - y(n))
+        c2[1] = (1.0 + ry) ** 2 / (ry * (2.0 + ry))
+        c3[1] = 1.0 / (ry * (2.0 + ry))
+    return ide, idv, c2, c3

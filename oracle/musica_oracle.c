@@ -20,7 +20,9 @@
  *   mod_filtering.f90:18-205,894-1492  selective filter        filter_sf
  *   mod_filtering_shock.f90:22-1512    shock filter            filter_shock
  *   mod_comm.f90:249-290         halo exchange (periodic self) orc_fill_periodic
- *   mpi_index_bounds.f90         loop bounds for periodic faces
+ *   mpi_index_bounds.f90         loop bounds for periodic faces and walls
+ *   smod_bc_wall.f90:11-98       no-slip wall, dp/dn = 0         bc_wall
+ *   mod_coeff_deriv.f90:100-160, 1099-1160  closure and extrapolation coefficients
  */
 #include "musica_oracle.h"
 
@@ -60,6 +62,69 @@ static const double D11[6] = {252.0 / 1024.0, -210.0 / 1024.0, 120.0 / 1024.0,
                               -45.0 / 1024.0, 10.0 / 1024.0, -1.0 / 1024.0};
 /* 9th-order conservative dissipation, mod_filtering_shock.f90:95-100. */
 static const double D10[6] = {0.0, -126.0, 84.0, -36.0, 9.0, -1.0};
+/* Centred closures next to a wall by half-width q = 1..4: a3, a5, a7, a9
+ * (mod_coeff_deriv.f90:102-104, 109-113, 125-131, 232-241; stencil 11 keeps
+ * the standard a7/a9). */
+static const double ACL[5][5] = {{0.0, 0.0, 0.0, 0.0, 0.0},
+                                 {0.0, 0.5, 0.0, 0.0, 0.0},
+                                 {0.0, 2.0 / 3.0, -1.0 / 12.0, 0.0, 0.0},
+                                 {0.0, 3.0 / 4.0, -3.0 / 20.0, 1.0 / 60.0, 0.0},
+                                 {0.0, 4.0 / 5.0, -1.0 / 5.0, 4.0 / 105.0, -1.0 / 280.0}};
+/* 5-pt one-sided first derivatives at the first two rows (mod_coeff_deriv.f90:114-124);
+ * a31 = -a13, a40 = -a04 at the last two. */
+static const double A04[6] = {0.0, -25.0 / 12.0, 4.0, -3.0, 4.0 / 3.0, -1.0 / 4.0};
+static const double A13[6] = {0.0, -1.0 / 4.0, -5.0 / 6.0, 3.0 / 2.0, -1.0 / 2.0, 1.0 / 12.0};
+/* Standard filters on 5, 7, 9 points used on rows 3, 4, 5 from a wall
+ * (mod_filtering.f90:24-38), before the -chi scaling. */
+static const double DCL[6][6] = {{0.0}, {0.0},
+                                 {6.0 / 16.0, -4.0 / 16.0, 1.0 / 16.0, 0.0, 0.0, 0.0},
+                                 {20.0 / 64.0, -15.0 / 64.0, 6.0 / 64.0, -1.0 / 64.0, 0.0, 0.0},
+                                 {70.0 / 256.0, -56.0 / 256.0, 28.0 / 256.0, -8.0 / 256.0, 1.0 / 256.0, 0.0},
+                                 {252.0 / 1024.0, -210.0 / 1024.0, 120.0 / 1024.0,
+                                  -45.0 / 1024.0, 10.0 / 1024.0, -1.0 / 1024.0}};
+
+/* Wall faces of a block: lo[a] / hi[a] for the min / max face of axis a. */
+typedef struct {
+  int lo[3], hi[3], any;
+} walls_t;
+
+static walls_t walls_of(const orc_cfg *c) {
+  walls_t W;
+  W.any = 0;
+  for (int a = 0; a < 3; a++) {
+    W.lo[a] = c->face[a][0] == ORC_FACE_WALL;
+    W.hi[a] = c->face[a][1] == ORC_FACE_WALL;
+    W.any |= W.lo[a] | W.hi[a];
+  }
+  return W;
+}
+
+/* Half-width of the centred stencil at row r (1..n) of an axis: 5 inside,
+ * r-1 (n-r) near a min (max) wall, 0 on the wall row itself. */
+static inline int halfw(int r, int n, int lo, int hi) {
+  int q = 5;
+  if (lo && r - 1 < q) q = r - 1;
+  if (hi && n - r < q) q = n - r;
+  return q;
+}
+
+/* 5-pt first derivative at row r with the one-sided rows of a wall axis
+ * (smod_grad_vel.f90:12-100: a04 at 1, a13 at 2, a31 at n-1, a40 at n). */
+static inline double g5w(const double *f, long p, long s, int r, int n, int lo, int hi) {
+  if (lo && r == 1)
+    return A04[1] * f[p] + A04[2] * f[p + s] + A04[3] * f[p + 2 * s] + A04[4] * f[p + 3 * s] +
+           A04[5] * f[p + 4 * s];
+  if (lo && r == 2)
+    return A13[1] * f[p - s] + A13[2] * f[p] + A13[3] * f[p + s] + A13[4] * f[p + 2 * s] +
+           A13[5] * f[p + 3 * s];
+  if (hi && r == n - 1)
+    return (-A13[5]) * f[p - 3 * s] + (-A13[4]) * f[p - 2 * s] + (-A13[3]) * f[p - s] + (-A13[2]) * f[p] +
+           (-A13[1]) * f[p + s];
+  if (hi && r == n)
+    return (-A04[5]) * f[p - 4 * s] + (-A04[4]) * f[p - 3 * s] + (-A04[3]) * f[p - 2 * s] +
+           (-A04[2]) * f[p - s] + (-A04[1]) * f[p];
+  return A5[1] * (f[p + s] - f[p - s]) + A5[2] * (f[p + 2 * s] - f[p - 2 * s]);
+}
 
 int orc_num_threads(void) {
 #ifdef _OPENMP
@@ -206,19 +271,28 @@ static void euler_rhs_w(const orc_cfg *c, const orc_grid *g, const dims_t *d, do
   const long sx = 1, sy = d->sx, sz = d->sx * d->sy;
   const long strides[3] = {sx, sy, sz};
   const int nn[3] = {d->nx, d->ny, d->nz};
+  const walls_t W = walls_of(c);
+  /* rows with an Euler increment along each axis: 2..n-1 next to walls (ndx_e, nfx_e) */
+  int lo_e[3], hi_e[3];
+  for (int a = 0; a < 3; a++) {
+    lo_e[a] = W.lo[a] ? 2 : 1;
+    hi_e[a] = nn[a] - (W.hi[a] ? 1 : 0);
+  }
+  /* the wall rows keep K = 0 (bc_wall zeroes them, flux_euler skips them) */
+  if (W.any)
+    for (int v = 0; v < 5; v++) memset(K[v], 0, d->n * sizeof(double));
   /* line buffers: fluxes along a line with ghosts */
   for (int dir = 0; dir < 3; dir++) {
     const long s = strides[dir];
     const int n = nn[dir];
-    /* iterate over the 2 transverse interior indices */
+    /* iterate over the 2 transverse indices with an increment */
     int o1 = dir == 0 ? 1 : 0, o2 = dir == 2 ? 1 : 2;
-    int n1 = nn[o1], n2 = nn[o2];
 #pragma omp parallel
     {
       double *fl = (double *)malloc(sizeof(double) * 5 * (size_t)(n + 2 * NG));
 #pragma omp for collapse(2)
-      for (int b = 1; b <= n2; b++)
-        for (int a = 1; a <= n1; a++) {
+      for (int b = lo_e[o2]; b <= hi_e[o2]; b++)
+        for (int a = lo_e[o1]; a <= hi_e[o1]; a++) {
           int idx3[3];
           idx3[o1] = a; idx3[o2] = b; idx3[dir] = 1 - NG;
           long p0 = (long)ID(*d, idx3[0], idx3[1], idx3[2]);
@@ -230,13 +304,22 @@ static void euler_rhs_w(const orc_cfg *c, const orc_grid *g, const dims_t *d, do
             else flux_cart(dir, U, w, p, F);
             for (int v = 0; v < 5; v++) fl[v * (n + 2 * NG) + l] = F[v];
           }
-          for (int i = 1; i <= n; i++) {
+          for (int i = lo_e[dir]; i <= hi_e[dir]; i++) {
             long p = p0 + (long)(i - 1 + NG) * s;
             long q = i - 1 + NG; /* position in line buffer */
+            const int hw = halfw(i, n, W.lo[dir], W.hi[dir]);
             double metric = 1.0;
             if (!c->curv) metric = dir == 0 ? g->idx[i - 1 + NG] : (dir == 1 ? g->idy[i - 1 + NG] : g->idz[i - 1 + NG]);
             for (int v = 0; v < 5; v++) {
-              double dv = d11(fl + v * (n + 2 * NG), q, 1);
+              const double *f = fl + v * (n + 2 * NG);
+              double dv;
+              if (hw == 5) {
+                dv = d11(f, q, 1);
+              } else {
+                /* rows 2..5 from a wall: a3, a5, a7, a9 (smod_flux_euler_11pts.f90:36-133) */
+                dv = 0.0;
+                for (int l = 1; l <= hw; l++) dv = dv + ACL[hw][l] * (f[q + l] - f[q - l]);
+              }
               if (!c->curv) dv = dv * metric;
               if (dir == 0) K[v][p] = dv;
               else K[v][p] = dv + K[v][p];
@@ -273,13 +356,16 @@ static inline double d5(const double *f, long p, long s) {
 static void gradients(const orc_cfg *c, const orc_grid *g, const dims_t *d, work_t *w) {
   const long sy = d->sx, sz = d->sx * d->sy;
   double *src[4] = {w->uu, w->vv, w->ww, w->tmp};
+  const walls_t W = walls_of(c);
 #pragma omp parallel for collapse(2)
   for (int k = 1; k <= d->nz; k++)
     for (int j = 1; j <= d->ny; j++)
       for (int i = 1; i <= d->nx; i++) {
         long p = (long)ID(*d, i, j, k);
         for (int q = 0; q < 4; q++) {
-          double gx = d5(src[q], p, 1), gy = d5(src[q], p, sy), gz = d5(src[q], p, sz);
+          double gx = g5w(src[q], p, 1, i, d->nx, W.lo[0], W.hi[0]);
+          double gy = g5w(src[q], p, sy, j, d->ny, W.lo[1], W.hi[1]);
+          double gz = g5w(src[q], p, sz, k, d->nz, W.lo[2], W.hi[2]);
           if (!c->curv) {
             gx = gx * g->idx_v[i - 1 + NG];
             gy = gy * g->idy_v[j - 1 + NG];
@@ -346,18 +432,24 @@ static void visc_rhs_w(const orc_cfg *c, const orc_grid *g, const dims_t *d, wor
           }
         }
       }
-  /* divergence, x then y then z, each ( ... )*metric + K */
+  /* divergence, x then y then z, each ( ... )*metric + K; one-sided rows at walls
+   * (smod_flux_visc_5pts.f90:80-160) */
+  const walls_t W = walls_of(c);
+  const int nn[3] = {d->nx, d->ny, d->nz};
 #pragma omp parallel for collapse(2)
   for (int k = 1; k <= d->nz; k++)
     for (int j = 1; j <= d->ny; j++)
       for (int i = 1; i <= d->nx; i++) {
         long p = (long)ID(*d, i, j, k);
         const long st[3] = {1, sy, sz};
+        const int rr[3] = {i, j, k};
         for (int dir = 0; dir < 3; dir++) {
           double metric;
           if (c->curv) metric = g->mv[9][p];
           else metric = dir == 0 ? g->idx_v[i - 1 + NG] : (dir == 1 ? g->idy_v[j - 1 + NG] : g->idz_v[k - 1 + NG]);
-          for (int v = 0; v < 4; v++) K[1 + v][p] = d5(F[4 * dir + v], p, st[dir]) * metric + K[1 + v][p];
+          for (int v = 0; v < 4; v++)
+            K[1 + v][p] = g5w(F[4 * dir + v], p, st[dir], rr[dir], nn[dir], W.lo[dir], W.hi[dir]) * metric +
+                          K[1 + v][p];
         }
       }
   for (int q = 0; q < 12; q++) free(F[q]);
@@ -371,6 +463,38 @@ static void filter_sf(const orc_cfg *c, const dims_t *d, double *const Un[5], do
   double dd[6];
   for (int l = 0; l < 6; l++) dd[l] = -D11[l] * c->chi;
   const long st[3] = {1, d->sx, d->sx * d->sy};
+  const walls_t W = walls_of(c);
+  if (W.any) {
+    /* rows 1,2 from a wall get nothing along its axis, rows 3,4,5 the 5/7/9-pt
+     * filters (mod_filtering.f90:900-988, 1116-1190) */
+    const int nn[3] = {d->nx, d->ny, d->nz};
+    double dc[6][6];
+    for (int q = 2; q <= 5; q++)
+      for (int l = 0; l < 6; l++) dc[q][l] = -DCL[q][l] * c->chi;
+#pragma omp parallel for collapse(2)
+    for (int k = 1; k <= d->nz; k++)
+      for (int j = 1; j <= d->ny; j++)
+        for (int i = 1; i <= d->nx; i++) {
+          long p = (long)ID(*d, i, j, k);
+          const int rr[3] = {i, j, k};
+          int hw[3];
+          for (int a = 0; a < 3; a++) hw[a] = halfw(rr[a], nn[a], W.lo[a], W.hi[a]);
+          for (int v = 0; v < 5; v++) {
+            const double *u = Un[v];
+            double acc = 0.0;
+            for (int dir = 0; dir < 3; dir++) {
+              const int q = hw[dir];
+              if (q < 2) continue;
+              const long s = st[dir];
+              double t = dc[q][0] * u[p];
+              for (int l = 1; l <= q; l++) t = t + dc[q][l] * (u[p + l * s] + u[p - l * s]);
+              acc = acc + t;
+            }
+            K[v][p] = acc;
+          }
+        }
+    return;
+  }
 #pragma omp parallel for collapse(2)
   for (int k = 1; k <= d->nz; k++)
     for (int j = 1; j <= d->ny; j++)
@@ -472,6 +596,43 @@ static void filter_shock(const orc_cfg *c, const dims_t *d, double *const Un[5],
     }
   }
   free(sens);
+}
+
+/* ------------------------------------------------------------------------ */
+/* No-slip wall with dp/dn = 0 (bc_wall_*_adiabatic_dpdn / _isotherm_dpdn,   */
+/* smod_bc_wall.f90:11-98 and its imax..kmax twins), faces in the order of   */
+/* bc_wall (mod_bc_apply.f90:237-261), so a later face wins on an edge.  T   */
+/* and p are the primitives last computed (w), not those of Un.              */
+/* ------------------------------------------------------------------------ */
+static void bc_wall(const orc_cfg *c, const orc_grid *g, const dims_t *d, double *const Un[5], const work_t *w) {
+  const walls_t W = walls_of(c);
+  if (!W.any) return;
+  const int nn[3] = {d->nx, d->ny, d->nz};
+  const long st[3] = {1, d->sx, d->sx * d->sy};
+  const double cv = 1.0 / (c->gam - 1.0) * c->rg; /* cvfg, mod_eos_pfg.f90:33 */
+  for (int f = 0; f < 6; f++) {
+    const int a = f / 2, side = f % 2;
+    if (!(side ? W.hi[a] : W.lo[a])) continue;
+    const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+    const long s = side ? -st[a] : st[a]; /* from the wall into the block */
+    const double c2 = g->cextrp2[f], c3 = g->cextrp3[f];
+#pragma omp parallel for
+    for (int q2 = 1; q2 <= nn[t2]; q2++)
+      for (int q1 = 1; q1 <= nn[t1]; q1++) {
+        int idx3[3];
+        idx3[t1] = q1; idx3[t2] = q2; idx3[a] = side ? nn[a] : 1;
+        const long p1 = (long)ID(*d, idx3[0], idx3[1], idx3[2]);
+        const long p2 = p1 + s, p3 = p1 + 2 * s;
+        const double t_wall = c->adiabatic ? c2 * w->tmp[p2] - c3 * w->tmp[p3] : c->t_wall;
+        const double p_wall = c2 * w->prs[p2] - c3 * w->prs[p3];
+        const double ro = p_wall / (c->rg * t_wall); /* rocalc_pt_pfg */
+        Un[1][p1] = 0.0;
+        Un[2][p1] = 0.0;
+        Un[3][p1] = 0.0;
+        Un[0][p1] = ro;
+        Un[4][p1] = ro * (cv * t_wall); /* ro*ecalc_tro_pfg */
+      }
+  }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -615,6 +776,7 @@ void orc_fill_interfaces(int nblocks, const orc_block *bl, double *const *arrs) 
     const int na[3] = {A->nx, A->ny, A->nz};
     const long sA[3] = {1, A->nx + 2 * NG, (long)(A->nx + 2 * NG) * (A->ny + 2 * NG)};
     for (int f = 0; f < 6; f++) {
+      if (A->wall[f]) continue;
       const orc_block *B = &bl[A->nbr_block[f]];
       const int fb = A->nbr_face[f], o = A->orient[f];
       const int nb[3] = {B->nx, B->ny, B->nz};
@@ -646,6 +808,14 @@ void orc_fill_interfaces(int nblocks, const orc_block *bl, double *const *arrs) 
 int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_grid *g, double *const *U, int nsteps) {
   if (nb < 1 || nb > 64) return -1;
   if (c0->filter == 2 && !c0->visc) return -3;
+  for (int b = 0; b < nb; b++)
+    for (int f = 0; f < 6; f++)
+      if (bl[b].wall[f]) {
+        const int ext = f < 2 ? bl[b].nx : (f < 4 ? bl[b].ny : bl[b].nz);
+        if (c0->curv) return -5;        /* walls: Cartesian blocks only */
+        if (c0->filter == 2) return -6; /* walls: selective filter or none */
+        if (ext < 10) return -7;        /* closures of the two sides must not overlap */
+      }
   orc_cfg cb[64];
   dims_t d[64];
   work_t w[64];
@@ -656,6 +826,7 @@ int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_gri
     cb[b].nx = bl[b].nx;
     cb[b].ny = bl[b].ny;
     cb[b].nz = bl[b].nz;
+    for (int f = 0; f < 6; f++) cb[b].face[f / 2][f % 2] = bl[b].wall[f] ? ORC_FACE_WALL : ORC_FACE_INTERFACE;
     if (bl[b].nx < NG || bl[b].ny < NG || bl[b].nz < NG) return -1;
     d[b] = mkdims(&cb[b]);
     for (int v = 0; v < 5; v++) {
@@ -672,6 +843,9 @@ int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_gri
       for (int v = 0; v < 5; v++) memcpy(Un[b][v], U[5 * b + v], d[b].n * sizeof(double));
     for (int irk = 1; irk <= c0->nrk; irk++) {
       const int last = irk == c0->nrk;
+      /* bc_wall at the start of stages 2..nrk, from the previous stage's primitives */
+      if (irk > 1)
+        for (int b = 0; b < nb; b++) bc_wall(&cb[b], &g[b], &d[b], Un[b], &w[b]);
       if (last && c0->visc) {
         for (int b = 0; b < nb; b++) {
           prim_visc(&cb[b], &d[b], &U[5 * b], &w[b]);
@@ -732,6 +906,10 @@ int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_gri
         orc_fill_interfaces(nb, bl, tab);
       }
     }
+    /* bc_wall at the end of the step (runge_kutta.f90:130-136): from the primitives
+     * of U^n after a viscous last stage (primitives_visc wrote them), else from
+     * those of the last stage's input */
+    for (int b = 0; b < nb; b++) bc_wall(&cb[b], &g[b], &d[b], Un[b], &w[b]);
     for (int b = 0; b < nb; b++)
       for (int v = 0; v < 5; v++) memcpy(U[5 * b + v], Un[b][v], d[b].n * sizeof(double));
   }

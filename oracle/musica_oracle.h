@@ -30,7 +30,7 @@ extern "C" {
 enum orc_face_kind {
   ORC_FACE_PERIODIC = 0,   /* neighbour block is the block itself          */
   ORC_FACE_INTERFACE = 1,  /* 1-to-1 interface with another block          */
-  ORC_FACE_WALL = 2        /* slip wall, sort=0 with flag 's'              */
+  ORC_FACE_WALL = 2        /* no-slip wall, sort=0 (is_wall2 = F): Cartesian */
 };
 
 typedef struct orc_cfg {
@@ -50,6 +50,9 @@ typedef struct orc_cfg {
   double chi;              /* filter amplitude (dissip_coeff)               */
   double dissip_shock;     /* shock-capturing amplitude                     */
   double csens;            /* pressure sensor blend                         */
+  int adiabatic;           /* walls: 1 adiabatic (dT/dn = 0), 0 isothermal  */
+  int pad2_;
+  double t_wall;           /* wall temperature of isothermal walls          */
 } orc_cfg;
 
 typedef struct orc_grid {
@@ -59,6 +62,10 @@ typedef struct orc_grid {
   /* c3: padded 3-D arrays: ksi_x,ksi_y,ksi_z,eta_x,eta_y,eta_z,phi_x,phi_y,phi_z,ijacob3 */
   const double *m[10];
   const double *mv[10];    /* the viscous set (*_v arrays)                 */
+  /* wall extrapolation coefficients per face (imin..kmax): p_wall =
+   * cextrp2*p(2) - cextrp3*p(3) (init_coeff_wall_cart,
+   * src/Derivatives/mod_coeff_deriv.f90:1099-1160) */
+  double cextrp2[6], cextrp3[6];
 } orc_grid;
 
 /* One block of a multiblock case.  Face f (imin,imax,jmin,jmax,kmin,kmax =
@@ -75,17 +82,35 @@ typedef struct orc_block {
   int nbr_block[6];
   int nbr_face[6];
   int orient[6];
+  int wall[6];             /* 1: the face is a wall (nbr_* ignored)         */
 } orc_block;
 
 /* Fill the face ghosts of every block from its neighbours' interior
- * (ngh layers, tangential extent of the face, edges and corners untouched).
- * arrs[b] is the padded array of block b. */
+ * (ngh layers, tangential extent of the face, edges and corners untouched;
+ * wall faces are skipped).  arrs[b] is the padded array of block b. */
 void orc_fill_interfaces(int nblocks, const orc_block *blocks, double *const *arrs);
 
 /* Multiblock step: blocks advance together, interfaces exchanged after every
  * stage and after the filter (runge_kutta.f90:48,109,126).  c gives the
  * flags; its nx,ny,nz and face[][] are ignored.  U[5*b+v] is variable v of
- * block b, grids[b] its metrics. */
+ * block b, grids[b] its metrics.
+ *
+ * Wall faces (Cartesian blocks, selective or no filter; at least 10 cells
+ * along a wall axis) follow the reference with is_wall2 = F:
+ *   - no Euler increment on the wall rows (ndx_e = 2, mpi_index_bounds.f90:20-70);
+ *     rows 2..5 use the centred a3/a5/a7/a9 closures with the metric row's value
+ *     (smod_flux_euler_11pts.f90:36-133, the idx2..idx8 of grid_metrics_wall_cart);
+ *   - 5-pt one-sided a04/a13 (a31/a40) gradients and viscous divergence on rows
+ *     1,2 (n-1,n) (smod_grad_vel.f90:12-100, smod_flux_visc_5pts.f90:80-160);
+ *   - selective filter: nothing from the wall axis on rows 1,2, d5/d7/d9 on
+ *     rows 3,4,5 (mod_filtering.f90:900-988, 1116-1190);
+ *   - bc_wall (smod_bc_wall.f90:11-98) at the start of stages 2..nrk and at the
+ *     end of the step: zero momentum, p and (adiabatic) T extrapolated from rows
+ *     2,3 of the primitives last computed (those of the previous stage's input;
+ *     at the end of a viscous step those of U^n, which primitives_visc wrote),
+ *     rho = p/(R T), rhoE = rho cv T (mod_eos_pfg.f90:170-184, 303-317);
+ *   - the exchange runs before bc_wall, so face ghosts on a wall row keep the
+ *     neighbour's value from before the wall state was imposed. */
 int orc_step_multi(const orc_cfg *c, int nblocks, const orc_block *blocks, const orc_grid *grids,
                    double *const *U, int nsteps);
 

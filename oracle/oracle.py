@@ -31,6 +31,7 @@ class OrcCfg(C.Structure):
         ("c0", C.c_double), ("S0", C.c_double), ("diffscale", C.c_double),
         ("cok_ratio", C.c_double),
         ("chi", C.c_double), ("dissip_shock", C.c_double), ("csens", C.c_double),
+        ("adiabatic", C.c_int), ("pad2_", C.c_int), ("t_wall", C.c_double),
     ]
 
 
@@ -38,6 +39,7 @@ class OrcBlock(C.Structure):
     _fields_ = [
         ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
         ("nbr_block", C.c_int * 6), ("nbr_face", C.c_int * 6), ("orient", C.c_int * 6),
+        ("wall", C.c_int * 6),
     ]
 
 
@@ -47,6 +49,7 @@ class OrcGrid(C.Structure):
         ("idx_v", _dp), ("idy_v", _dp), ("idz_v", _dp),
         ("m", _dp * 10),
         ("mv", _dp * 10),
+        ("cextrp2", C.c_double * 6), ("cextrp3", C.c_double * 6),
     ]
 
 
@@ -90,7 +93,8 @@ def _p(a):
 
 
 class Oracle:
-    """CPU restatement of the reference step for one periodic block."""
+    """CPU restatement of the reference step for one block (periodic faces, or
+    walls: then the step runs as a one-block multiblock case)."""
 
     def __init__(self, case):
         self.case = case
@@ -101,9 +105,13 @@ class Oracle:
         c.filter = int(case.filter)
         c.ducros = int(case.ducros)
         c.nrk = int(case.nrk)
+        faces = tuple(getattr(case, "faces", (0,) * 6))
         for d in range(3):
             for s in range(2):
-                c.face[d][s] = 0
+                c.face[d][s] = faces[2 * d + s]
+        self.walls = [int(k == 2) for k in faces]
+        c.adiabatic = int(getattr(case, "adiabatic", True))
+        c.t_wall = float(getattr(case, "t_wall", 0.0))
         for q, v in enumerate(case.crk):
             c.crk[q] = v
         c.dt = case.dt
@@ -126,6 +134,9 @@ class Oracle:
             arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in case.cart]
             self._keep += arrs
             g.idx, g.idy, g.idz, g.idx_v, g.idy_v, g.idz_v = [_p(a) for a in arrs]
+        for f in range(6):
+            g.cextrp2[f] = float(getattr(case, "cextrp2", (0.0,) * 6)[f])
+            g.cextrp3[f] = float(getattr(case, "cextrp3", (0.0,) * 6)[f])
         self.grid = g
 
     @staticmethod
@@ -134,6 +145,13 @@ class Oracle:
 
     def step(self, U, nsteps: int = 1):
         """Advance the list of 5 padded arrays U in place."""
+        if any(self.walls):
+            blk = self_block(self.case)
+            rc = lib().orc_step_multi(C.byref(self.cfg), 1, make_blocks([blk]), C.byref(self.grid),
+                                      self._tab(U), int(nsteps))
+            if rc != 0:
+                raise RuntimeError(f"orc_step_multi failed rc={rc}")
+            return U
         rc = lib().orc_step(C.byref(self.cfg), C.byref(self.grid), self._tab(U), int(nsteps))
         if rc != 0:
             raise RuntimeError(f"orc_step failed rc={rc}")
@@ -166,8 +184,17 @@ class Oracle:
         return K
 
 
+def self_block(case):
+    """One-block table: periodic faces are interfaces with the block itself,
+    faces of kind 2 walls."""
+    faces = tuple(getattr(case, "faces", (0,) * 6))
+    return dict(nx=case.nx, ny=case.ny, nz=case.nz, nbr_block=[0] * 6,
+                nbr_face=[f ^ 1 for f in range(6)], orient=[0] * 6, wall=[int(k == 2) for k in faces])
+
+
 def make_blocks(blocks):
-    """blocks: list of dicts with nx, ny, nz, nbr_block[6], nbr_face[6], orient[6]."""
+    """blocks: list of dicts with nx, ny, nz, nbr_block[6], nbr_face[6], orient[6]
+    and optionally wall[6]."""
     arr = (OrcBlock * len(blocks))()
     for q, b in enumerate(blocks):
         arr[q].nx, arr[q].ny, arr[q].nz = b["nx"], b["ny"], b["nz"]
@@ -175,6 +202,7 @@ def make_blocks(blocks):
             arr[q].nbr_block[f] = b["nbr_block"][f]
             arr[q].nbr_face[f] = b["nbr_face"][f]
             arr[q].orient[f] = b["orient"][f]
+            arr[q].wall[f] = int(b.get("wall", [0] * 6)[f])
     return arr
 
 
