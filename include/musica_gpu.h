@@ -49,6 +49,13 @@ extern "C" {
  * neighbour-is-itself interface (src/Parallel/mod_mpi_part.f90:333-). */
 #define MGPU_FACE_PERIODIC 0
 #define MGPU_FACE_INTERFACE 1
+/* No-slip wall, BC_face%sort = 0 with is_wall2 = F (the reference's default,
+ * src/File_IO/read_param.f90:75): bc_wall_*_adiabatic_dpdn or
+ * _isotherm_dpdn (src/Boundary/smod_bc_wall.f90) with the boundary-row
+ * closures of the Euler, viscous and filter operators.  On the j-faces (2, 3)
+ * of Cartesian FP64 cases with the selective filter or none, ny >= 10; the
+ * block's extrapolation coefficients come from mgpu_set_wall. */
+#define MGPU_FACE_WALL 2
 
 /* Filter selection: is_SF / is_shock (src/Main/assign_procedures.f90:161-219) */
 #define MGPU_FILTER_NONE 0
@@ -101,6 +108,9 @@ typedef struct mgpu_case {
   double dissip_coeff;    /* filter amplitude chi                              */
   double dissip_shock;    /* shock-capturing amplitude                         */
   double csens;           /* pressure-sensor blend                             */
+  int32_t is_adiab;       /* walls: 1 adiabatic (dT/dn = 0), 0 isothermal (is_adiab) */
+  int32_t reserved;       /* 0                                                 */
+  double t_wall;          /* walls: temperature of an isothermal wall (T_wall) */
 } mgpu_case;
 
 typedef struct mgpu_handle mgpu_handle;
@@ -123,6 +133,16 @@ int mgpu_set_metrics_cart(mgpu_handle *h, int block, const double *idx, const do
  * (*_v), each a padded 3-D array with ghosts
  * (src/Grid/mod_grid.f90:131-145, smod_grid_metrics1_c3.f90:8-484). */
 int mgpu_set_metrics_c3(mgpu_handle *h, int block, const double *const *m, const double *const *mv);
+
+/* Wall extrapolation coefficients of a block with MGPU_FACE_WALL faces:
+ * cextrp2[f], cextrp3[f] for faces f = 0..5 (entries of faces without a
+ * wall are ignored), as init_coeff_wall_cart computes them from the grid
+ * (src/Derivatives/mod_coeff_deriv.f90:1099-1160): p_wall = cextrp2*p(2) -
+ * cextrp3*p(3) along the wall normal.  Required before the first step of
+ * such a block.  The Euler metric (idy) of a wall block holds, on rows 2..5
+ * (ny-1..ny-4), the idy2/idy4/idy6/idy8 of grid_metrics_wall_cart
+ * (src/Grid/grid_metrics_cart.f90:190-436), which the closure rows use. */
+int mgpu_set_wall(mgpu_handle *h, int block, const double *cextrp2, const double *cextrp3);
 
 /* Upload the conserved variables rho,rhou,rhov,rhow,rhoe of a block (padded
  * arrays; ghosts are refreshed on the device). */
@@ -171,7 +191,6 @@ int mgpu_kernel_time(mgpu_handle *h, int kind, int64_t *launches, double *ms);
  * is NULL). */
 const char *mgpu_last_error(const mgpu_handle *h);
 
-/* Release every device resource of the handle. */
 /* Statistics of the current state U^n over the interior cells of all
  * blocks: mass sum(rho), kinetic energy sum(1/2 rho |u|^2) and enstrophy
  * sum(1/2 rho |curl u|^2) with the 5-pt velocity gradients -- the sums of
@@ -179,13 +198,15 @@ const char *mgpu_last_error(const mgpu_handle *h);
  * MPI_REDUCE (stats_tgv takes the gradients its last stage computed; these
  * are the current state's) -- and umax = max(|u| + c), the velocity scale
  * of timestep_new (src/Time/timestep_new.f90:270-288: deltat =
- * CFL*delta_min/umax).  Deterministic: fixed summation order. */
+ * CFL*delta_min/umax).  Deterministic: fixed summation order.  Blocks with
+ * walls: MGPU_EUNSUP (the gradients read ghosts on every face). */
 typedef struct mgpu_diag {
   double mass, kinetic, enstrophy, umax;
   int64_t cells;
 } mgpu_diag;
 int mgpu_diagnostics(mgpu_handle *h, mgpu_diag *out);
 
+/* Release every device resource of the handle. */
 void mgpu_destroy(mgpu_handle *h);
 
 /* ------------------------------------------------------------------------

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libmusica_gpu.so")
 MGPU_NGH = 5
 FACE_PERIODIC = 0
 FACE_INTERFACE = 1
+FACE_WALL = 2
 FILTER_NONE, FILTER_SF, FILTER_SHOCK = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 LAUNCH_EAGER = 1
@@ -27,6 +28,7 @@ EXPORTS = (
     "mgpu_create",
     "mgpu_set_metrics_cart",
     "mgpu_set_metrics_c3",
+    "mgpu_set_wall",
     "mgpu_put_state",
     "mgpu_get_state",
     "mgpu_step",
@@ -79,6 +81,9 @@ class MgpuCase(C.Structure):
         ("dissip_coeff", C.c_double),
         ("dissip_shock", C.c_double),
         ("csens", C.c_double),
+        ("is_adiab", C.c_int32),
+        ("reserved", C.c_int32),
+        ("t_wall", C.c_double),
     ]
 
 
@@ -108,6 +113,7 @@ def lib() -> C.CDLL:
     L.mgpu_create.argtypes = [C.POINTER(MgpuCase), C.POINTER(C.c_void_p)]
     L.mgpu_set_metrics_cart.argtypes = [C.c_void_p, C.c_int] + [_dp] * 6
     L.mgpu_set_metrics_c3.argtypes = [C.c_void_p, C.c_int, C.POINTER(_dp), C.POINTER(_dp)]
+    L.mgpu_set_wall.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     L.mgpu_put_state.argtypes = [C.c_void_p, C.c_int] + [_dp] * 5
     L.mgpu_get_state.argtypes = [C.c_void_p, C.c_int] + [_dp] * 5
     L.mgpu_step.argtypes = [C.c_void_p, C.c_int]
@@ -137,7 +143,7 @@ def lib() -> C.CDLL:
     for name in ("mgpu_server_create", "mgpu_server_submit", "mgpu_server_tick", "mgpu_server_status",
                  "mgpu_server_take"):
         getattr(L, name).restype = C.c_int
-    for name in ("mgpu_create", "mgpu_set_metrics_cart", "mgpu_set_metrics_c3", "mgpu_put_state",
+    for name in ("mgpu_create", "mgpu_set_metrics_cart", "mgpu_set_metrics_c3", "mgpu_set_wall", "mgpu_put_state",
                  "mgpu_get_state", "mgpu_step", "mgpu_autotune", "mgpu_stage_config", "mgpu_sync", "mgpu_profile", "mgpu_kernel_time"):
         getattr(L, name).restype = C.c_int
     _lib = L

@@ -5,7 +5,10 @@
 //   shock filter       filter_o10_shock src/Num_dissipation/mod_filtering_shock.f90:
 //                      pressure sensor :243-260, face dissipation :419-440,
 //                      increment :690-700 (and the j, k repeats)
-// Interior rows only (periodic faces and 1-to-1 interfaces).
+// Interior rows of periodic faces and 1-to-1 interfaces; with no-slip walls
+// on j-faces the selective filter leaves rows 1,2 (ny-1,ny) unfiltered along
+// j and uses the 5/7/9-pt filters on rows 3,4,5 (mod_filtering.f90:900-988,
+// 1116-1190); along i and k every row is filtered.
 //
 // 2.5-D layout: the state of the tile + 5-cell halos of the current k plane
 // sits in LDS for the i/j stencils, the own column keeps an 11-deep register
@@ -98,8 +101,25 @@ __device__ __forceinline__ void ldr(const CState &U, idx_t p, const RefState &rf
   }
 }
 
-template <bool MIX>
+// Filter increment along j of rows next to a wall: nothing on the first two,
+// the 5/7/9-pt filters (times -chi) on the next three; y points at the row
+// with stride s.  Reads nothing beyond the wall.
+__device__ __forceinline__ double fwall(const double *y, int s, int hw, double chi) {
+  switch (hw) {
+    case 2: return -chi * (D5_0 * y[0] + D5_1 * (y[s] + y[-s]) + D5_2 * (y[2 * s] + y[-2 * s]));
+    case 3:
+      return -chi * (D7_0 * y[0] + D7_1 * (y[s] + y[-s]) + D7_2 * (y[2 * s] + y[-2 * s]) +
+                     D7_3 * (y[3 * s] + y[-3 * s]));
+    case 4:
+      return -chi * (D9_0 * y[0] + D9_1 * (y[s] + y[-s]) + D9_2 * (y[2 * s] + y[-2 * s]) +
+                     D9_3 * (y[3 * s] + y[-3 * s]) + D9_4 * (y[4 * s] + y[-4 * s]));
+    default: return 0.0;
+  }
+}
+
+template <bool MIX, bool WALL>
 __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__restrict__ tab, int nb, int ntiles) {
+  static_assert(!(MIX && WALL), "walls: FP64 only");
   using R = typename std::conditional<MIX, float, double>::type;
   __shared__ R sux[5][FY][FX + 10];
   __shared__ R suy[5][FY + 10][FX];
@@ -157,15 +177,20 @@ __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__
     }
     __syncthreads();
     if (own) {
+      const int hw = WALL ? jhalf(g, j) : 5;
 #pragma unroll
       for (int v = 0; v < 5; v++) {
         const R *x = &sux[v][ty][tx + 5];
         const R *y = &suy[v][ty + 5][tx];
         R K = dd[0] * x[0] + dd[1] * (x[1] + x[-1]) + dd[2] * (x[2] + x[-2]) + dd[3] * (x[3] + x[-3]) +
                    dd[4] * (x[4] + x[-4]) + dd[5] * (x[5] + x[-5]);
-        K = K + dd[0] * y[0] + dd[1] * (y[FX] + y[-FX]) + dd[2] * (y[2 * FX] + y[-2 * FX]) +
-            dd[3] * (y[3 * FX] + y[-3 * FX]) + dd[4] * (y[4 * FX] + y[-4 * FX]) +
-            dd[5] * (y[5 * FX] + y[-5 * FX]);
+        if (!WALL || hw == 5) {
+          K = K + dd[0] * y[0] + dd[1] * (y[FX] + y[-FX]) + dd[2] * (y[2 * FX] + y[-2 * FX]) +
+              dd[3] * (y[3 * FX] + y[-3 * FX]) + dd[4] * (y[4 * FX] + y[-4 * FX]) +
+              dd[5] * (y[5 * FX] + y[-5 * FX]);
+        } else {
+          if constexpr (WALL) K = K + fwall(y, FX, hw, a.chi);
+        }
         K = K + dd[0] * q[5][v] + dd[1] * (q[6][v] + q[4][v]) + dd[2] * (q[7][v] + q[3][v]) +
             dd[3] * (q[8][v] + q[2][v]) + dd[4] * (q[9][v] + q[1][v]) + dd[5] * (q[10][v] + q[0][v]);
         if constexpr (MIX) stg_nt(uoutp.v[v], pk, u64[v] + (double)K);
@@ -419,10 +444,16 @@ TileRange filter_tiles(const Geom &g, int kc_req) {
   return t;
 }
 
-hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int nt, bool mixed, hipStream_t s) {
+hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int nt, bool mixed, bool wall, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  if (mixed) hipLaunchKernelGGL(k_filter_sf<true>, dim3(nt), dim3(FT), 0, s, tab, nb, nt);
-  else hipLaunchKernelGGL(k_filter_sf<false>, dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  if (wall) {
+    if (mixed) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_filter_sf<false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  } else if (mixed) {
+    hipLaunchKernelGGL((k_filter_sf<true, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  } else {
+    hipLaunchKernelGGL((k_filter_sf<false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  }
   return hipGetLastError();
 }
 

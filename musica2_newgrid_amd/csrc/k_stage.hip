@@ -9,6 +9,10 @@
 //   flux_euler_11pts_c3     src/Compute_flow/smod_flux_euler_11pts_c3.f90:25-38,139-171,763-773
 //   flux_visc_5pts(_c3) divergence part  smod_flux_visc_5pts.f90:119-130, _c3.f90:262-
 //   update_var              src/Time/update.f90:32-44
+// With no-slip walls on j-faces (WALL): no Euler increment on the wall rows
+// (ndx_e = 2, mpi_index_bounds.f90:20-70) and the centred a3/a5/a7/a9
+// closures of rows 2..5 (smod_flux_euler_11pts.f90:267-595 with
+// is_bc_1pt), whose metric is the idy entry of the row.
 //
 // Work decomposition (gfx950): a TX x TY workgroup owns a TX(i) x TY(j)
 // column tile and marches a chunk of k planes.  For each plane the i- and
@@ -35,6 +39,18 @@ template <class R>
 __device__ __forceinline__ R d11c(const R *f, int s) {
   return R(A11_1) * (f[s] - f[-s]) + R(A11_2) * (f[2 * s] - f[-2 * s]) + R(A11_3) * (f[3 * s] - f[-3 * s]) +
          R(A11_4) * (f[4 * s] - f[-4 * s]) + R(A11_5) * (f[5 * s] - f[-5 * s]);
+}
+
+// Coefficients of the centred j-derivative that row j uses next to a wall
+// (a3, a5, a7, a9 by half-width, a11 inside), padded with zeros to 5 terms.
+// The j-flux pencil holds 0 beyond a wall, so the padded terms add exact
+// zeros.
+__device__ __forceinline__ void wall_coeffs(int hw, double c[5]) {
+  c[0] = hw == 5 ? A11_1 : (hw == 4 ? A9_1 : (hw == 3 ? A7_1 : (hw == 2 ? A5_1 : (hw == 1 ? A3_1 : 0.0))));
+  c[1] = hw == 5 ? A11_2 : (hw == 4 ? A9_2 : (hw == 3 ? A7_2 : (hw == 2 ? A5_2 : 0.0)));
+  c[2] = hw == 5 ? A11_3 : (hw == 4 ? A9_3 : (hw == 3 ? A7_3 : 0.0));
+  c[3] = hw == 5 ? A11_4 : (hw == 4 ? A9_4 : 0.0);
+  c[4] = hw == 5 ? A11_5 : 0.0;
 }
 
 // Raw inputs of one flux evaluation: the 5 conserved variables and the 3
@@ -203,8 +219,9 @@ __device__ __forceinline__ void flux_halo(const Prim &q, int dir, const double m
   else flux_r<CURV, MIX>(q, 1, m, rf, F);
 }
 
-template <int TX, int TY, bool CURV, bool VISC, int MODE>
+template <int TX, int TY, bool CURV, bool VISC, int MODE, bool WALL>
 __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__restrict__ tab, int nb, int ntiles) {
+  static_assert(!WALL || (!CURV && MODE == kFP64), "walls: Cartesian FP64 blocks");
   constexpr bool MIX = MODE != kFP64;
   constexpr bool IN32 = MODE == kMixMiddle || MODE == kMixLast;
   constexpr bool OUT32 = MODE == kMixFirst || MODE == kMixMiddle;
@@ -275,6 +292,25 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
   }
   const idx_t own_off = lin(g, i, j, 0);
   const idx_t sz = g.sy;
+  // walls: j-pencil entries beyond a wall hold 0 (their state is never valid),
+  // and the row's closure coefficients
+  bool hs_beyond[NS];
+  bool own_beyond = false;
+  double cw[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  int hw = 5;
+  if constexpr (WALL) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const int gj = j0 - 5 + hs_r[s];
+      hs_beyond[s] = hs_dir[s] == 1 && (((g.jwall & 1) && gj < 1) || ((g.jwall & 2) && gj > g.ny));
+    }
+    own_beyond = (g.jwall & 2) && j > g.ny;
+    hw = jhalf(g, j);
+    wall_coeffs(hw, cw);
+  } else {
+#pragma unroll
+    for (int s = 0; s < NS; s++) hs_beyond[s] = false;
+  }
 
   const RefState &rf = a.ref;
   const double alpha = a.alpha;
@@ -330,7 +366,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
       if (needy) {
         flux_r<CURV, MIX>(pq, 1, rym, rf, F);
 #pragma unroll
-        for (int v = 0; v < 5; v++) sfy[v][ty + 5][tx] = F[v];
+        for (int v = 0; v < 5; v++) sfy[v][ty + 5][tx] = (WALL && own_beyond) ? R(0) : F[v];
       }
     }
     if (hs_on[0]) {
@@ -341,7 +377,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
         for (int v = 0; v < 5; v++) sfx[v][hs_r[0]][hs_c[0]] = F[v];
       } else {
 #pragma unroll
-        for (int v = 0; v < 5; v++) sfy[v][hs_r[0]][hs_c[0]] = F[v];
+        for (int v = 0; v < 5; v++) sfy[v][hs_r[0]][hs_c[0]] = (WALL && hs_beyond[0]) ? R(0) : F[v];
       }
     }
     __builtin_amdgcn_sched_barrier(0);
@@ -361,7 +397,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
             for (int v = 0; v < 5; v++) sfx[v][hs_r[s]][hs_c[s]] = F[v];
           } else {
 #pragma unroll
-            for (int v = 0; v < 5; v++) sfy[v][hs_r[s]][hs_c[s]] = F[v];
+            for (int v = 0; v < 5; v++) sfy[v][hs_r[s]][hs_c[s]] = (WALL && hs_beyond[s]) ? R(0) : F[v];
           }
         }
       }
@@ -373,7 +409,14 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
 #pragma unroll
       for (int v = 0; v < 5; v++) {
         const R dx = d11c(&sfx[v][ty][tx + 5], 1);
-        const R dy = d11c(&sfy[v][ty + 5][tx], TX);
+        R dy;
+        if constexpr (WALL) {
+          const R *f = &sfy[v][ty + 5][tx];
+          dy = cw[0] * (f[TX] - f[-TX]) + cw[1] * (f[2 * TX] - f[-2 * TX]) + cw[2] * (f[3 * TX] - f[-3 * TX]) +
+               cw[3] * (f[4 * TX] - f[-4 * TX]) + cw[4] * (f[5 * TX] - f[-5 * TX]);
+        } else {
+          dy = d11c(&sfy[v][ty + 5][tx], TX);
+        }
         const R dz = R(A11_1) * (q[6][v] - q[4][v]) + R(A11_2) * (q[7][v] - q[3][v]) +
                      R(A11_3) * (q[8][v] - q[2][v]) + R(A11_4) * (q[9][v] - q[1][v]) +
                      R(A11_5) * (q[10][v] - q[0][v]);
@@ -385,6 +428,8 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
           const double ky = (double)dy * ldg(P.idy, (idx_t)(j - 1 + G)) + kx;
           K[v] = (double)dz * ldg(P.idz, (idx_t)(k - 1 + G)) + ky;
         }
+        // wall rows: no Euler increment (bc_wall zeroes it, flux_euler skips the row)
+        if (WALL && hw == 0) K[v] = 0.0;
       }
       if constexpr (VISC) {
 #pragma unroll
@@ -408,39 +453,49 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
 }
 
 template <int TX, int TY, int MODE>
-hipError_t launch_t(const Batch<StageArgs> *tab, int nb, int nt, bool curv, bool visc, hipStream_t s) {
+hipError_t launch_t(const Batch<StageArgs> *tab, int nb, int nt, bool curv, bool visc, bool wall, hipStream_t s) {
   dim3 blk(TX * TY), grd(nt);
   // the viscous increment is added in the last stage only
   constexpr bool VOK = MODE == kFP64 || MODE == kMixLast;
   if (visc && !VOK) return hipErrorInvalidValue;
+  if (wall) {
+    if constexpr (MODE == kFP64) {
+      if (curv) return hipErrorInvalidValue;
+      if (visc) hipLaunchKernelGGL((k_stage<TX, TY, false, true, MODE, true>), grd, blk, 0, s, tab, nb, nt);
+      else hipLaunchKernelGGL((k_stage<TX, TY, false, false, MODE, true>), grd, blk, 0, s, tab, nb, nt);
+      return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+  }
   if (curv) {
     if constexpr (VOK) {
       if (visc) {
-        hipLaunchKernelGGL((k_stage<TX, TY, true, true, MODE>), grd, blk, 0, s, tab, nb, nt);
+        hipLaunchKernelGGL((k_stage<TX, TY, true, true, MODE, false>), grd, blk, 0, s, tab, nb, nt);
         return hipGetLastError();
       }
     }
-    hipLaunchKernelGGL((k_stage<TX, TY, true, false, MODE>), grd, blk, 0, s, tab, nb, nt);
+    hipLaunchKernelGGL((k_stage<TX, TY, true, false, MODE, false>), grd, blk, 0, s, tab, nb, nt);
   } else {
     if constexpr (VOK) {
       if (visc) {
-        hipLaunchKernelGGL((k_stage<TX, TY, false, true, MODE>), grd, blk, 0, s, tab, nb, nt);
+        hipLaunchKernelGGL((k_stage<TX, TY, false, true, MODE, false>), grd, blk, 0, s, tab, nb, nt);
         return hipGetLastError();
       }
     }
-    hipLaunchKernelGGL((k_stage<TX, TY, false, false, MODE>), grd, blk, 0, s, tab, nb, nt);
+    hipLaunchKernelGGL((k_stage<TX, TY, false, false, MODE, false>), grd, blk, 0, s, tab, nb, nt);
   }
   return hipGetLastError();
 }
 
 template <int MODE>
-hipError_t launch_shape(const Batch<StageArgs> *tab, int nb, int nt, bool curv, bool visc, int shape, hipStream_t s) {
+hipError_t launch_shape(const Batch<StageArgs> *tab, int nb, int nt, bool curv, bool visc, int shape, bool wall,
+                        hipStream_t s) {
   switch (shape) {
-    case 1: return launch_t<32, 8, MODE>(tab, nb, nt, curv, visc, s);
-    case 2: return launch_t<64, 8, MODE>(tab, nb, nt, curv, visc, s);
-    case 3: return launch_t<16, 16, MODE>(tab, nb, nt, curv, visc, s);
-    case 4: return launch_t<32, 16, MODE>(tab, nb, nt, curv, visc, s);
-    default: return launch_t<64, 4, MODE>(tab, nb, nt, curv, visc, s);
+    case 1: return launch_t<32, 8, MODE>(tab, nb, nt, curv, visc, wall, s);
+    case 2: return launch_t<64, 8, MODE>(tab, nb, nt, curv, visc, wall, s);
+    case 3: return launch_t<16, 16, MODE>(tab, nb, nt, curv, visc, wall, s);
+    case 4: return launch_t<32, 16, MODE>(tab, nb, nt, curv, visc, wall, s);
+    default: return launch_t<64, 4, MODE>(tab, nb, nt, curv, visc, wall, s);
   }
 }
 
@@ -460,13 +515,13 @@ TileRange stage_tiles(const Geom &g, int shape, int kchunk) {
 }
 
 hipError_t launch_stage(const Batch<StageArgs> *tab, int nb, int nt, bool curv, bool visc, int shape, int mode,
-                        hipStream_t s) {
+                        bool wall, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
   switch (mode) {
-    case kMixFirst: return launch_shape<kMixFirst>(tab, nb, nt, curv, visc, shape, s);
-    case kMixMiddle: return launch_shape<kMixMiddle>(tab, nb, nt, curv, visc, shape, s);
-    case kMixLast: return launch_shape<kMixLast>(tab, nb, nt, curv, visc, shape, s);
-    default: return launch_shape<kFP64>(tab, nb, nt, curv, visc, shape, s);
+    case kMixFirst: return launch_shape<kMixFirst>(tab, nb, nt, curv, visc, shape, wall, s);
+    case kMixMiddle: return launch_shape<kMixMiddle>(tab, nb, nt, curv, visc, shape, wall, s);
+    case kMixLast: return launch_shape<kMixLast>(tab, nb, nt, curv, visc, shape, wall, s);
+    default: return launch_shape<kFP64>(tab, nb, nt, curv, visc, shape, wall, s);
   }
 }
 

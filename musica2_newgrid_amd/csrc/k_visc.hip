@@ -282,13 +282,6 @@ TileRange vtiles(const Geom &g, int kc_req) {
   return t;
 }
 
-// Component c of tau in the flux along direction d of a Cartesian block (row
-// d of the stress tensor, then the energy flux e_d); constant-folds for
-// constant d and c.
-__host__ __device__ constexpr int krow(int d, int c) {
-  return c == 3 ? 6 + d : (d == 0 ? c : (d == 1 ? (c == 0 ? 1 : 2 + c) : (c == 0 ? 2 : (c == 1 ? 4 : 5))));
-}
-
 // Flux along computational direction d at one point: Cartesian, row d of
 // tau (t holds those four values); c3, tau projected on the viscous metric
 // row (mx,my,mz) of d (t holds all nine values).

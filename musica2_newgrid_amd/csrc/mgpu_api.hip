@@ -2,6 +2,7 @@
 // choreography of runge_kutta (src/Compute_flow/runge_kutta.f90:2-152).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -149,9 +150,25 @@ std::string check_topology(const mgpu_case &c) {
     const int na[3] = {A.nx, A.ny, A.nz};
     for (int f = 0; f < 6; f++) {
       const int kind = A.face_kind[f];
-      if (kind != MGPU_FACE_PERIODIC && kind != MGPU_FACE_INTERFACE) {
+      if (kind != MGPU_FACE_PERIODIC && kind != MGPU_FACE_INTERFACE && kind != MGPU_FACE_WALL) {
         snprintf(buf, sizeof buf, "block %d face %d: face kind %d not supported", b, f, kind);
         return buf;
+      }
+      if (kind == MGPU_FACE_WALL) {
+        if (f != 2 && f != 3) {
+          snprintf(buf, sizeof buf, "block %d face %d: walls are supported on the j-faces (2, 3) only", b, f);
+          return buf;
+        }
+        if (c.curvilinear || c.precision != MGPU_PREC_FP64 || c.filter == MGPU_FILTER_SHOCK) {
+          snprintf(buf, sizeof buf,
+                   "block %d face %d: walls need a Cartesian FP64 case with the selective filter or none", b, f);
+          return buf;
+        }
+        if (A.ny < 10) {
+          snprintf(buf, sizeof buf, "block %d: ny = %d, a block with walls needs ny >= 10", b, A.ny);
+          return buf;
+        }
+        continue;
       }
       if (kind == MGPU_FACE_PERIODIC) {
         if (A.face_kind[f ^ 1] != MGPU_FACE_PERIODIC) {
@@ -226,6 +243,7 @@ bool build_tasks(const mgpu_case &c, const std::vector<BlockData> &bds, int ng, 
     const Geom &ga = bds[b].g;
     const int na[3] = {ga.nx, ga.ny, ga.nz};
     for (int f = 0; f < 6; f++) {
+      if (A.face_kind[f] == MGPU_FACE_WALL) continue;
       int nb, nf, o;
       face_nbr(A, b, f, nb, nf, o);
       const Geom &gb = bds[nb].g;
@@ -405,7 +423,8 @@ void free_block(BlockData &bd) {
   for (auto &p : bd.kv) { if (p) (void)hipFree(p); p = nullptr; }
   if (bd.prs) (void)hipFree(bd.prs);
   if (bd.sens) (void)hipFree(bd.sens);
-  bd.prs = bd.sens = nullptr;
+  if (bd.wstash) (void)hipFree(bd.wstash);
+  bd.prs = bd.sens = bd.wstash = nullptr;
 }
 
 int dalloc(mgpu_handle *h, double **p, size_t count) {
@@ -520,6 +539,7 @@ FilterArgs filter_args(const mgpu_handle *h, const BlockData &bd, int in, int ou
     fa.uout.v[v] = bd.st[out][v];
   }
   for (int l = 0; l < 6; l++) fa.dd[l] = -d11[l] * bd.p.dissip_coeff;
+  fa.chi = bd.p.dissip_coeff;
   fa.prs = bd.prs;
   fa.sens = h->c.is_ducros ? bd.sens : nullptr;
   fa.xnu0 = -bd.p.dissip_shock;
@@ -613,8 +633,51 @@ int build_tables(mgpu_handle *h) {
   return 0;
 }
 
+// Wall state of the j-walls of every block with walls: from rows 2,3 of set
+// `src` (-1: from the block's stash) into the wall rows of set `dst` (-1:
+// into the stash).
+int enqueue_walls(mgpu_handle *h, int src, int dst) {
+  for (const auto &bd : h->b) {
+    if (!bd.g.jwall) continue;
+    WallArgs wa{};
+    wa.g = bd.g;
+    wa.gas = gas_of(bd.p);
+    for (int v = 0; v < 5; v++) {
+      wa.src.v[v] = src >= 0 ? bd.st[src][v] : nullptr;
+      wa.dst.v[v] = dst >= 0 ? bd.st[dst][v] : nullptr;
+    }
+    wa.stash = bd.wstash;
+    for (int s = 0; s < 2; s++) {
+      wa.c2[s] = bd.c2[s];
+      wa.c3[s] = bd.c3[s];
+    }
+    wa.cv = 1.0 / (bd.p.gam - 1.0) * bd.p.rg;
+    wa.t_wall = bd.p.t_wall;
+    wa.adiabatic = bd.p.is_adiab;
+    HIPCK(h, launch_wall(wa, h->stream));
+  }
+  return 0;
+}
+
+WallViscArgs wall_visc_args(const BlockData &bd, int s0) {
+  WallViscArgs a{};
+  a.g = bd.g;
+  a.gas = gas_of(bd.p);
+  for (int v = 0; v < 5; v++) a.u0.v[v] = bd.st[s0][v];
+  a.met = metrics_of(bd);
+  for (int q = 0; q < NTAU; q++) a.tau[q] = bd.tau[q];
+  for (int q = 0; q < 4; q++) a.kv[q] = bd.kv[q];
+  return a;
+}
+
 // Enqueues one step with U^n in set s0 (no host synchronisation: the same
 // sequence is captured into a graph).  runge_kutta, runge_kutta.f90:2-152.
+// With walls, bc_wall runs after each stage's exchange, from the stage's
+// input (the primitives the reference computed last), except after the last
+// stage: there the wall state (from U^n after a viscous stage, whose
+// primitives_visc overwrote the primitives, else from the stage's input) is
+// kept aside and imposed after the filter and its exchange
+// (runge_kutta.f90:20-24, 42-45, 130-136).
 int enqueue_step(mgpu_handle *h, int s0) {
   const mgpu_case &c = h->c;
   const bool curv = c.curvilinear != 0, visc = c.iorder_visc != 0;
@@ -627,20 +690,36 @@ int enqueue_step(mgpu_handle *h, int s0) {
     if (last && visc) {
       ProfScope ps(h, MGPU_K_VISC);
       HIPCK(h, launch_visc(h->d_visc + s0 * nb, nb, h->nt_visc, curv, h->stream));
+      if (h->wall)
+        for (const auto &bd : h->b)
+          if (bd.g.jwall) HIPCK(h, launch_visc_wall(wall_visc_args(bd, s0), h->stream));
       HIPCK(h, launch_iface(h->xv, h->nchunks_v, h->stream));
+      if (h->wall)
+        for (const auto &bd : h->b)
+          if (bd.g.jwall) HIPCK(h, launch_tau_wallrow(wall_visc_args(bd, s0), h->stream));
       HIPCK(h, launch_vdiv(h->d_vdiv, nb, h->nt_vdiv, curv, h->stream));
+      if (h->wall)
+        for (const auto &bd : h->b)
+          if (bd.g.jwall) HIPCK(h, launch_vdiv_wall(wall_visc_args(bd, s0), h->stream));
     }
     {
       ProfScope ps(h, MGPU_K_STAGE);
       const int v = (last && visc) ? 1 : 0;
       HIPCK(h, launch_stage(h->d_stage + ((size_t)s0 * nrk + (irk - 1)) * nb, nb, h->nt_stage[v], curv,
-                            last && visc, h->shape[v], stage_mode(h, irk), h->stream));
+                            last && visc, h->shape[v], stage_mode(h, irk), h->wall, h->stream));
     }
     if (h->mixed && !last) {
       ProfScope ps(h, MGPU_K_HALO);
       HIPCK(h, launch_iface(h->xf[fset_of(irk)], h->nchunks_s, h->stream));
     } else if ((rc = exchange_state(h, out))) {
       return rc;
+    }
+    if (h->wall) {
+      ProfScope ps(h, MGPU_K_HALO);
+      if (!last) rc = enqueue_walls(h, in, out);
+      else if (c.filter != MGPU_FILTER_NONE) rc = enqueue_walls(h, visc ? s0 : in, -1);
+      else rc = enqueue_walls(h, visc ? s0 : in, out);
+      if (rc) return rc;
     }
     in = out;
   }
@@ -649,10 +728,14 @@ int enqueue_step(mgpu_handle *h, int s0) {
     {
       ProfScope ps(h, MGPU_K_FILTER);
       if (c.filter == MGPU_FILTER_SF)
-        HIPCK(h, launch_filter_sf(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, h->stream));
+        HIPCK(h, launch_filter_sf(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, h->wall, h->stream));
       else HIPCK(h, launch_filter_shock(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, h->stream));
     }
     if ((rc = exchange_state(h, s0))) return rc;
+    if (h->wall) {
+      ProfScope ps(h, MGPU_K_HALO);
+      if ((rc = enqueue_walls(h, -1, s0))) return rc;
+    }
   }
   return 0;
 }
@@ -739,7 +822,7 @@ size_t block_bytes(const mgpu_case &c, int nx, int ny, int nz) {
   size_t bytes = arrays * n * sizeof(double);
   if (c.precision == MGPU_PREC_MIXED) bytes += 10 * n * sizeof(float);
   if (!c.curvilinear) bytes += 6 * (size_t)(nx + ny + nz + 6 * G) * sizeof(double);
-  return bytes;
+  return bytes;  // (+ 4*nx*nz doubles of wall state for a block with walls)
 }
 
 int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
@@ -760,6 +843,7 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
   bd.g.sx = bl.nx + 2 * G;
   bd.g.sy = (idx_t)bd.g.sx * (idx_t)(bl.ny + 2 * G);
   bd.g.n = (long)bd.g.sy * (bl.nz + 2 * G);
+  bd.g.jwall = (bl.face_kind[2] == MGPU_FACE_WALL ? 1 : 0) | (bl.face_kind[3] == MGPU_FACE_WALL ? 2 : 0);
   bd.bytes = block_bytes(c, bl.nx, bl.ny, bl.nz);
   const size_t n = (size_t)bd.g.n;
   int rc = 0;
@@ -776,6 +860,7 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
     const int len[3] = {bl.nx + 2 * G, bl.ny + 2 * G, bl.nz + 2 * G};
     for (int q = 0; q < 6 && !rc; q++) rc = dalloc(h, &bd.idx[q], (size_t)len[q % 3]);
   }
+  if (bd.g.jwall && !rc) rc = dalloc(h, &bd.wstash, (size_t)4 * bl.nx * bl.nz);
   if (c.iorder_visc) {
     for (int q = 0; q < NTAU && !rc; q++) rc = dalloc(h, &bd.tau[q], n);
     for (int q = 0; q < 4 && !rc; q++) rc = dalloc(h, &bd.kv[q], n);
@@ -790,6 +875,7 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
     h->blocks.pop_back();
     return rc;
   }
+  if (bd.g.jwall) h->wall = true;
   h->exchange_ready = false;
   h->tables_ready = false;
   return 0;
@@ -907,6 +993,8 @@ int step(mgpu_handle *h, int nsteps) {
   for (size_t bI = 0; bI < h->b.size(); bI++) {
     if (!h->b[bI].have_metrics) return fail(h, MGPU_ESTATE, "block %zu: metrics not set", bI);
     if (!h->b[bI].have_state) return fail(h, MGPU_ESTATE, "block %zu: state not set", bI);
+    if (h->b[bI].g.jwall && !h->b[bI].have_wall)
+      return fail(h, MGPU_ESTATE, "block %zu: wall coefficients not set (mgpu_set_wall)", bI);
   }
   if (h->b.empty()) return 0;
   int rc = refresh_ghosts(h);
@@ -942,6 +1030,8 @@ int diagnostics(mgpu_handle *h, mgpu_diag *out) {
   for (size_t bI = 0; bI < h->b.size(); bI++) {
     if (!h->b[bI].have_metrics) return fail(h, MGPU_ESTATE, "block %zu: metrics not set", bI);
     if (!h->b[bI].have_state) return fail(h, MGPU_ESTATE, "block %zu: state not set", bI);
+    if (h->b[bI].g.jwall)
+      return fail(h, MGPU_EUNSUP, "block %zu: diagnostics need valid ghosts on every face (no walls)", bI);
   }
   int rc = refresh_ghosts(h);
   if (rc) return rc;
@@ -1029,7 +1119,8 @@ int tune_aux(mgpu_handle *h, bool force, hipEvent_t e0, hipEvent_t e1) {
       auto launch = [&]() -> hipError_t {
         if (kind == 0) return launch_visc(h->d_visc + s0 * nb, nb, h->nt_visc, curv, h->stream);
         if (kind == 1) return launch_vdiv(h->d_vdiv, nb, h->nt_vdiv, curv, h->stream);
-        if (h->c.filter == MGPU_FILTER_SF) return launch_filter_sf(h->d_tune_f, nb, h->nt_filt, h->mixed, h->stream);
+        if (h->c.filter == MGPU_FILTER_SF)
+          return launch_filter_sf(h->d_tune_f, nb, h->nt_filt, h->mixed, h->wall, h->stream);
         return launch_filter_shock(h->d_tune_f, nb, h->nt_filt, h->mixed, h->stream);
       };
       hipError_t e = launch();
@@ -1111,6 +1202,26 @@ int mgpu_set_metrics_c3(mgpu_handle *h, int block, const double *const *m, const
   return put_metrics_c3(h, block, m, mv);
 }
 
+int mgpu_set_wall(mgpu_handle *h, int block, const double *cextrp2, const double *cextrp3) {
+  if (!h) return MGPU_EINVAL;
+  if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
+  if (!cextrp2 || !cextrp3) return fail(h, MGPU_EINVAL, "null argument");
+  BlockData &bd = h->b[block];
+  if (!bd.g.jwall) return fail(h, MGPU_ESTATE, "block %d has no wall face", block);
+  for (int s = 0; s < 2; s++) {
+    if (!((bd.g.jwall >> s) & 1)) continue;
+    const double a = cextrp2[2 + s], b = cextrp3[2 + s];
+    if (!std::isfinite(a) || !std::isfinite(b))
+      return fail(h, MGPU_EINVAL, "block %d face %d: extrapolation coefficients not finite", block, 2 + s);
+    bd.c2[s] = a;
+    bd.c3[s] = b;
+  }
+  HIPCK(h, hipStreamSynchronize(h->stream));
+  drop_graphs(h);  // the captured steps hold the coefficients
+  bd.have_wall = true;
+  return 0;
+}
+
 int mgpu_put_state(mgpu_handle *h, int block, const double *rho, const double *rhou, const double *rhov,
                    const double *rhow, const double *rhoe) {
   if (!h) return MGPU_EINVAL;
@@ -1182,10 +1293,10 @@ int mgpu_autotune(mgpu_handle *h, int flags) {
         const int mode = stage_mode(h, irk);
         hipError_t e = hipMemcpy(h->d_tune, tab.data(), nb * sizeof(tab[0]), hipMemcpyHostToDevice);
         // one warm-up launch, then three timed ones
-        if (e == hipSuccess) e = launch_stage(h->d_tune, nb, nt, curv, v == 1, sh, mode, h->stream);
+        if (e == hipSuccess) e = launch_stage(h->d_tune, nb, nt, curv, v == 1, sh, mode, h->wall, h->stream);
         if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
         for (int r = 0; r < 3 && e == hipSuccess; r++)
-          e = launch_stage(h->d_tune, nb, nt, curv, v == 1, sh, mode, h->stream);
+          e = launch_stage(h->d_tune, nb, nt, curv, v == 1, sh, mode, h->wall, h->stream);
         if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.f;

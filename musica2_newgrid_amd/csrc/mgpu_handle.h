@@ -25,9 +25,12 @@ struct BlockData {
   double *kv[4] = {};     // viscous increment of rhou..rhoe (last stage)
   double *prs = nullptr, *sens = nullptr;
   RefState ref{};         // mixed precision: interior mean of the state at put_state
+  double *wstash = nullptr;  // walls: rho, rhoE of the end-of-step wall state (2 per wall point)
+  double c2[2] = {0, 0}, c3[2] = {0, 0};  // walls: cextrp2, cextrp3 of the jmin, jmax faces
   size_t bytes = 0;       // device memory held by the block
   bool have_metrics = false;
   bool have_state = false;
+  bool have_wall = false;  // wall coefficients set (blocks with walls)
 };
 
 struct ProfRec {
@@ -80,6 +83,7 @@ struct mgpu_handle {
   bool use_graph = true;
   hipGraphExec_t gexec[3] = {};
   bool mixed = false;  // FP32 flux/derivative/filter arithmetic
+  bool wall = false;   // some block has no-slip walls (wall variants of the stage and filter kernels)
   bool shared_ref = false;  // one mixed-precision reference state for all blocks (set from block 0)
   double *diag_buf = nullptr;  // per-workgroup partial sums of mgpu_diagnostics
   long diag_cap = 0;

@@ -21,6 +21,21 @@ constexpr double D11_0 = 252.0 / 1024.0, D11_1 = -210.0 / 1024.0, D11_2 = 120.0 
                  D11_3 = -45.0 / 1024.0, D11_4 = 10.0 / 1024.0, D11_5 = -1.0 / 1024.0;
 // 9th-order conservative dissipation (mod_filtering_shock.f90:95-100).
 constexpr double D10_1 = -126.0, D10_2 = 84.0, D10_3 = -36.0, D10_4 = 9.0, D10_5 = -1.0;
+// Centred first derivatives of half-width 1, 3, 4 used on rows 2, 4, 5 from a
+// wall (a3, a7, a9; row 3 uses a5): mod_coeff_deriv.f90:102-104, 125-131, 232-241.
+constexpr double A3_1 = 0.5;
+constexpr double A7_1 = 3.0 / 4.0, A7_2 = -3.0 / 20.0, A7_3 = 1.0 / 60.0;
+constexpr double A9_1 = 4.0 / 5.0, A9_2 = -1.0 / 5.0, A9_3 = 4.0 / 105.0, A9_4 = -1.0 / 280.0;
+// 5-pt one-sided first derivatives on the wall row (a04) and the next (a13);
+// a40 = -a04 and a31 = -a13 mirror them at a max face (mod_coeff_deriv.f90:114-124).
+constexpr double A04_1 = -25.0 / 12.0, A04_2 = 4.0, A04_3 = -3.0, A04_4 = 4.0 / 3.0, A04_5 = -1.0 / 4.0;
+constexpr double A13_1 = -1.0 / 4.0, A13_2 = -5.0 / 6.0, A13_3 = 3.0 / 2.0, A13_4 = -1.0 / 2.0, A13_5 = 1.0 / 12.0;
+// Standard 5-, 7- and 9-pt filters on rows 3, 4, 5 from a wall
+// (mod_filtering.f90:24-38), scaled by -chi at run time.
+constexpr double D5_0 = 6.0 / 16.0, D5_1 = -4.0 / 16.0, D5_2 = 1.0 / 16.0;
+constexpr double D7_0 = 20.0 / 64.0, D7_1 = -15.0 / 64.0, D7_2 = 6.0 / 64.0, D7_3 = -1.0 / 64.0;
+constexpr double D9_0 = 70.0 / 256.0, D9_1 = -56.0 / 256.0, D9_2 = 28.0 / 256.0, D9_3 = -8.0 / 256.0,
+                 D9_4 = 1.0 / 256.0;
 
 // Element offset inside one padded array.  32-bit: a block's padded array
 // holds at most 2^28 points (checked at mgpu_create), so byte offsets fit in
@@ -31,10 +46,20 @@ constexpr size_t kMaxPadded = size_t(1) << 28;
 // Geometry of one block's padded arrays (i fastest).
 struct Geom {
   int nx, ny, nz;
-  int sx;    // nx + 2G
-  idx_t sy;  // sx*(ny+2G)
-  long n;    // total padded points
+  int sx;     // nx + 2G
+  idx_t sy;   // sx*(ny+2G)
+  int jwall;  // no-slip walls: bit 0 on the jmin face, bit 1 on the jmax face
+  long n;     // total padded points
 };
+
+// Half-width of the centred stencil that row j may use along j: 5 inside,
+// j-1 (ny-j) near a jmin (jmax) wall, 0 on a wall row.
+__host__ __device__ inline int jhalf(const Geom &g, int j) {
+  int q = 5;
+  if ((g.jwall & 1) && j - 1 < q) q = j - 1;
+  if ((g.jwall & 2) && g.ny - j < q) q = g.ny - j;
+  return q;
+}
 
 __host__ __device__ inline idx_t lin(const Geom &g, int i, int j, int k) {
   return (idx_t)(i + G - 1) + (idx_t)g.sx * (idx_t)(j + G - 1) + g.sy * (idx_t)(k + G - 1);
@@ -207,9 +232,45 @@ struct FilterArgs {
   const double *prs;   // shock filter: p(U^n)
   const double *sens;  // shock filter: Ducros sensor
   double dd[6];        // -chi*d11 (selective filter)
+  double chi;          // filter amplitude (the 5/7/9-pt filters next to walls)
   double xnu0, xnu10, csens;
   RefState ref;        // mixed precision only
 };
+
+// No-slip wall state (bc_wall with dp/dn = 0) of the j-faces of one block:
+// rho and rhoE from p and T extrapolated from rows 2,3 of `src` (or read
+// from `stash`), zero momentum, written to the wall rows of `dst` (or to
+// `stash`: two values per wall point, jmin face first).
+struct WallArgs {
+  Geom g;
+  Gas gas;
+  CState src;
+  State dst;
+  double *stash;
+  double c2[2], c3[2];  // extrapolation coefficients cextrp2, cextrp3 (jmin, jmax)
+  double cv;            // cv = rg/(gam-1)
+  double t_wall;        // isothermal wall temperature
+  int adiabatic;
+};
+
+// The viscous rows next to the j-walls of one block: tau on rows 1,2
+// (ny-1,ny) with one-sided j-gradients, the wall-row rescaling after the
+// exchange, and the viscous divergence on those rows.
+struct WallViscArgs {
+  Geom g;
+  Gas gas;
+  CState u0;  // U^n
+  Metrics met;
+  double *tau[NTAU];
+  double *kv[4];
+};
+
+// Component c of tau in the flux along direction d of a Cartesian block (row
+// d of the stress tensor, then the energy flux e_d); constant-folds for
+// constant d and c.
+__host__ __device__ constexpr int krow(int d, int c) {
+  return c == 3 ? 6 + d : (d == 0 ? c : (d == 1 ? (c == 0 ? 1 : 2 + c) : (c == 0 ? 2 : (c == 1 ? 4 : 5))));
+}
 
 // One face of one destination block in the interface exchange: the ghost
 // layers of block `dst` on that face take the interior layers of block
@@ -255,16 +316,21 @@ extern const int kStageShapes[][2];
 extern const int kNumStageShapes;
 TileRange stage_tiles(const Geom &g, int shape, int kchunk);
 hipError_t launch_stage(const Batch<StageArgs> *tab, int nb, int ntiles, bool curv, bool visc, int shape,
-                        int mode, hipStream_t s);
+                        int mode, bool wall, hipStream_t s);
 TileRange visc_tiles(const Geom &g, int kchunk);  // kchunk 0: default
 hipError_t launch_visc(const Batch<ViscArgs> *tab, int nb, int ntiles, bool curv, hipStream_t s);
 TileRange vdiv_tiles(const Geom &g, int kchunk);
 hipError_t launch_vdiv(const Batch<VdivArgs> *tab, int nb, int ntiles, bool curv, hipStream_t s);
 TileRange filter_tiles(const Geom &g, int kchunk);
-hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, hipStream_t s);
+hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, bool wall,
+                            hipStream_t s);
 hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, hipStream_t s);
 hipError_t launch_iface(const IfaceArgs &a, int nchunks, hipStream_t s);
 hipError_t launch_iface(const IfaceArgs32 &a, int nchunks, hipStream_t s);
 long diag_groups(const Geom &g);
+hipError_t launch_wall(const WallArgs &a, hipStream_t s);
+hipError_t launch_visc_wall(const WallViscArgs &a, hipStream_t s);
+hipError_t launch_tau_wallrow(const WallViscArgs &a, hipStream_t s);
+hipError_t launch_vdiv_wall(const WallViscArgs &a, hipStream_t s);
 hipError_t launch_diag(const DiagArgs &a, bool curv, hipStream_t s);
 }  // namespace mgpu

@@ -13,7 +13,7 @@ module mod_musica_gpu
   use iso_c_binding
   implicit none
 
-  integer(c_int32_t), parameter :: MGPU_FACE_PERIODIC = 0, MGPU_FACE_INTERFACE = 1
+  integer(c_int32_t), parameter :: MGPU_FACE_PERIODIC = 0, MGPU_FACE_INTERFACE = 1, MGPU_FACE_WALL = 2
   integer(c_int32_t), parameter :: MGPU_FILTER_NONE = 0, MGPU_FILTER_SF = 1, MGPU_FILTER_SHOCK = 2
   integer(c_int32_t), parameter :: MGPU_PREC_FP64 = 0, MGPU_PREC_MIXED = 1
   integer(c_int32_t), parameter :: MGPU_LAUNCH_EAGER = 1
@@ -29,6 +29,8 @@ module mod_musica_gpu
      type(c_ptr) :: blocks
      real(c_double) :: crk(8), deltat, gam, rg, c0, S0, diffscale, cok_ratio
      real(c_double) :: dissip_coeff, dissip_shock, csens
+     integer(c_int32_t) :: is_adiab = 1, reserved = 0
+     real(c_double) :: t_wall = 0.0_c_double
   end type mgpu_case_t
 
   ! Hot-path fields of param.ini (read_param, src/File_IO/read_param.f90:88-205)
@@ -59,6 +61,13 @@ module mod_musica_gpu
        integer(c_int), value :: block
        type(c_ptr), intent(in) :: m(10), mv(10)
      end function mgpu_set_metrics_c3
+
+     integer(c_int) function mgpu_set_wall(h, block, cextrp2, cextrp3) bind(C, name='mgpu_set_wall')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int), value :: block
+       real(c_double), intent(in) :: cextrp2(6), cextrp3(6)
+     end function mgpu_set_wall
 
      integer(c_int) function mgpu_put_state(h, block, rho, rhou, rhov, rhow, rhoe) bind(C, name='mgpu_put_state')
        import :: c_int, c_ptr, c_double

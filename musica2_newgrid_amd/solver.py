@@ -20,7 +20,9 @@ def make_case_struct(case, precision: int = _lib.PREC_FP64, topology=None, launc
     """C-ABI case description.  `topology` is a block table (see
     multiblock.py); None means one block with periodic faces."""
     if topology is None:
-        topology = [{"nx": case.nx, "ny": case.ny, "nz": case.nz, "kind": [_lib.FACE_PERIODIC] * 6,
+        faces = tuple(getattr(case, "faces", (0,) * 6))
+        kinds = [_lib.FACE_WALL if k == 2 else _lib.FACE_PERIODIC for k in faces]
+        topology = [{"nx": case.nx, "ny": case.ny, "nz": case.nz, "kind": kinds,
                      "nbr_block": [0] * 6, "nbr_face": [1, 0, 3, 2, 5, 4], "orient": [0] * 6}]
     blocks = (MgpuBlock * len(topology))()
     for b, d in enumerate(topology):
@@ -52,6 +54,8 @@ def make_case_struct(case, precision: int = _lib.PREC_FP64, topology=None, launc
     cs.dissip_coeff = case.chi
     cs.dissip_shock = case.dissip_shock
     cs.csens = case.csens
+    cs.is_adiab = int(getattr(case, "adiabatic", True))
+    cs.t_wall = float(getattr(case, "t_wall", 0.0))
     return cs, blocks
 
 
@@ -75,9 +79,17 @@ class GpuSolver:
         rc = L.mgpu_create(C.byref(self._cs), C.byref(h))
         check(rc, None, "mgpu_create")
         self.h = h
+        self._topology = topology
         for b, bc in enumerate(self.block_cases):
             self._set_metrics(b, bc)
+            if self._has_wall(b):
+                c2 = np.ascontiguousarray(getattr(bc, "cextrp2", (0.0,) * 6), dtype=np.float64)
+                c3 = np.ascontiguousarray(getattr(bc, "cextrp3", (0.0,) * 6), dtype=np.float64)
+                check(L.mgpu_set_wall(self.h, b, dptr(c2), dptr(c3)), self.h, "mgpu_set_wall")
             self.put_state(bc.U, b)
+
+    def _has_wall(self, b: int) -> bool:
+        return any(self._blocks[b].face_kind[f] == _lib.FACE_WALL for f in range(6))
 
     @property
     def nblocks(self) -> int:

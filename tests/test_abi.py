@@ -27,5 +27,5 @@ def test_library_exports_all_symbols():
 def test_struct_layout_matches_header():
     from musica2_newgrid_amd._lib import MgpuBlock, MgpuCase
     assert ctypes.sizeof(MgpuBlock) == 4 * (3 + 24)
-    # 8 int32, one pointer, 8 crk, deltat and 9 gas/filter doubles
-    assert ctypes.sizeof(MgpuCase) == 32 + 8 + 8 * (8 + 1 + 9)
+    # 8 int32, one pointer, 8 crk, deltat and 9 gas/filter doubles, 2 int32 and t_wall
+    assert ctypes.sizeof(MgpuCase) == 32 + 8 + 8 * (8 + 1 + 9) + 8 + 8

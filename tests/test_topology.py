@@ -78,3 +78,42 @@ def test_flag_checks():
     c.nrk = 9
     rc, msg = _create(c, blocks)
     assert rc == _lib.EINVAL and "nrk" in msg
+
+
+def _channel_table(**kw):
+    c = cases.channel(16, 20, 6, **kw)
+    blk = {"nx": c.nx, "ny": c.ny, "nz": c.nz, "kind": [_lib.FACE_PERIODIC, _lib.FACE_PERIODIC, _lib.FACE_WALL,
+                                                        _lib.FACE_WALL, _lib.FACE_PERIODIC, _lib.FACE_PERIODIC],
+           "nbr_block": [0] * 6, "nbr_face": [1, 0, 3, 2, 5, 4], "orient": [0] * 6}
+    return c, [blk]
+
+
+def _create_case(c, blocks, precision=_lib.PREC_FP64):
+    cs, keep = make_case_struct(c, precision=precision, topology=blocks)
+    h = C.c_void_p()
+    rc = _lib.lib().mgpu_create(C.byref(cs), C.byref(h))
+    msg = _lib.lib().mgpu_last_error(None)
+    return rc, (msg.decode() if isinstance(msg, bytes) else msg)
+
+
+def test_wall_limits_are_checked():
+    """Walls: j-faces only, Cartesian FP64 with the selective filter or none,
+    ny >= 10; refused before any device work."""
+    c, blocks = _channel_table()
+    blocks[0]["kind"][0] = blocks[0]["kind"][1] = _lib.FACE_WALL
+    rc, msg = _create_case(c, blocks)
+    assert rc == _lib.EINVAL and "j-faces" in msg
+    c, blocks = _channel_table()
+    rc, msg = _create_case(c, blocks, precision=_lib.PREC_MIXED)
+    assert rc == _lib.EINVAL and "FP64" in msg
+    c, blocks = _channel_table(filter=2)
+    rc, msg = _create_case(c, blocks)
+    assert rc == _lib.EINVAL and "selective filter" in msg
+    c, blocks = _channel_table()
+    c.curvilinear = True
+    rc, msg = _create_case(c, blocks)
+    assert rc == _lib.EINVAL and "Cartesian" in msg
+    c, blocks = _channel_table()
+    blocks[0]["ny"] = 9
+    rc, msg = _create_case(c, blocks)
+    assert rc == _lib.EINVAL and "ny >= 10" in msg

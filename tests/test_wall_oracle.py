@@ -240,20 +240,30 @@ def test_ghosts_beyond_walls_are_never_read():
         assert np.array_equal(gr.interior(a), gr.interior(b))
 
 
-def _split_x(c, nsteps):
-    """c advanced as two blocks cut at i = nx/2 (aligned interfaces)."""
+def split_x_blocks(c):
+    """Block table and block cases of c cut at i = nx/2 into two blocks joined
+    by aligned interfaces (the cut and the periodic wrap), walls on the j-faces."""
     h = c.nx // 2
-    blocks, bcs, Us = [], [], []
+    blocks, bcs = [], []
     for b in range(2):
         sub = cases.Case(f"half{b}", h, c.ny, c.nz, curvilinear=False, iorder_visc=c.iorder_visc,
                          filter=c.filter, ducros=False, diffscale=c.diffscale, chi=c.chi, dt=c.dt,
                          cart=[c.cart[0][:h + 2 * G].copy(), c.cart[1], c.cart[2],
                                c.cart[3][:h + 2 * G].copy(), c.cart[4], c.cart[5]],
-                         faces=(1, 1, 2, 2, 0, 0), cextrp2=c.cextrp2, cextrp3=c.cextrp3)
+                         faces=(1, 1, 2, 2, 1, 1), adiabatic=c.adiabatic, t_wall=c.t_wall,
+                         cextrp2=c.cextrp2, cextrp3=c.cextrp3)
+        sub.U = [np.ascontiguousarray(a[:, :, b * h:b * h + h + 2 * G]) for a in c.U]
         bcs.append(sub)
-        Us.append([np.ascontiguousarray(a[:, :, b * h:b * h + h + 2 * G]) for a in c.U])
         blocks.append(dict(nx=h, ny=c.ny, nz=c.nz, nbr_block=[1 - b, 1 - b, 0, 0, b, b],
-                           nbr_face=[1, 0, 0, 0, 5, 4], orient=[0] * 6, wall=[0, 0, 1, 1, 0, 0]))
+                           nbr_face=[1, 0, 0, 0, 5, 4], orient=[0] * 6, wall=[0, 0, 1, 1, 0, 0],
+                           kind=[1, 1, 2, 2, 1, 1]))
+    return blocks, bcs
+
+
+def _split_x(c, nsteps):
+    """c advanced as two blocks cut at i = nx/2 (aligned interfaces)."""
+    blocks, bcs = split_x_blocks(c)
+    Us = [bc.copy_state() for bc in bcs]
     orc.step_multi(c, blocks, bcs, Us, nsteps)
     return [np.concatenate([gr.interior(Us[b][v]) for b in range(2)], axis=2) for v in range(5)]
 
