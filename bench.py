@@ -156,6 +156,33 @@ def tgv_leg(steps: int = 100):
                         "RK4, CFL 0.8, FP64, graph replay"}
 
 
+def channel_leg(steps: int, warmup: int):
+    """A 512x256x128 Cartesian channel with no-slip walls on both j-faces
+    (stretched y-grid), periodic x and z: the wall variants of the stage and
+    filter kernels plus the wall-row kernels, autotuned, graph replay."""
+    from musica2_newgrid_amd import _lib, cases
+    from musica2_newgrid_amd.solver import GpuSolver
+    case = cases.channel(512, 256, 128, lengths=(8.0, 2.0, 2.0))
+    with GpuSolver(case) as s:
+        s.autotune()
+        s.step(warmup)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        t = time.perf_counter() - t0
+        s.profile(True)
+        s.step(2)
+        times = {_lib.K_NAMES[k]: s.kernel_time(k) for k in _lib.K_NAMES}
+        s.profile(False)
+    return {"value": round(case.ncells * case.nrk * steps / t / 1e6, 2), "unit": "Mcell-RK-stages/s",
+            "ms_per_step": round(t / steps * 1e3, 4), "steps": steps,
+            "workload": "Cartesian channel 512x256x128, no-slip adiabatic walls on both j-faces (tanh-stretched "
+                        "y), periodic x and z, M=0.3 parabolic profile + perturbations + pressure pulse, "
+                        "viscous, selective filter, RK4, FP64, graph replay",
+            "kernel_ms_per_step_eager": {k: round(ms / 2, 4) for k, (n, ms) in times.items()}}
+
+
 def serve_leg(nreq: int, nsteps: int, seed: int = 5):
     """Serving loop: a Poisson stream of TGV / isentropic-vortex requests on
     cubes of edge 64..256, Re log-uniform in [400, 4000], FP64 or mixed,
@@ -193,6 +220,7 @@ def main() -> int:
     ap.add_argument("--no-multiblock", action="store_true", help="skip the uneven 7-block leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--no-tgv", action="store_true", help="skip the TGV 64^3 leg")
+    ap.add_argument("--no-channel", action="store_true", help="skip the walled-channel leg")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-steps", type=int, default=100)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -263,6 +291,9 @@ def main() -> int:
     if not args.no_tgv:
         print("bench: TGV 64^3 leg", file=sys.stderr, flush=True)
         out["tgv64"] = tgv_leg()
+    if not args.no_channel:
+        print("bench: walled-channel leg", file=sys.stderr, flush=True)
+        out["channel"] = channel_leg(args.steps, args.warmup)
     if not args.no_multiblock:
         print("bench: uneven 7-block leg", file=sys.stderr, flush=True)
         out["multiblock"] = multiblock_leg(args.steps, args.warmup)

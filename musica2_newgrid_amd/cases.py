@@ -285,7 +285,8 @@ def channel(nx: int = 64, ny: int = 48, nz: int = 16, lengths=(4.0, 2.0, 1.0), b
     y = ye[G:G + ny]
     x = np.arange(nx) * dx
     z = np.arange(nz) * dz
-    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    # broadcast (nz,1,1), (1,ny,1), (1,1,nx): the fields come out (nz,ny,nx)
+    Z, Y, X = z.reshape(-1, 1, 1), y.reshape(1, -1, 1), x.reshape(1, 1, -1)
     c_ref = math.sqrt(GAM * RG * T_ref)
     u_ref = mach * c_ref
     p_ref = rho_ref * RG * T_ref
@@ -302,8 +303,9 @@ def channel(nx: int = 64, ny: int = 48, nz: int = 16, lengths=(4.0, 2.0, 1.0), b
     yc = 0.5 * Ly if symmetric else 0.4 * Ly
     r2 = ((X - 0.5 * Lx) ** 2 + (Y - yc) ** 2 + (Z - 0.5 * Lz) ** 2) / (0.1 * Ly) ** 2
     p = p_ref * (1.0 + pulse * np.exp(-math.log(2.0) * r2))
-    T = np.full_like(p, T_ref)
+    T = T_ref
     rho = p / (RG * T)
+    u, v, w = (np.broadcast_to(q, p.shape) for q in (u, v, w))
     U = []
     for f in conserved(rho, u, v, w, p):
         a = gr.padded(nx, ny, nz)
