@@ -19,6 +19,8 @@
  *   smod_flux_visc_5pts*.f90     viscous flux divergence       visc_*
  *   mod_filtering.f90:18-205,894-1492  selective filter        filter_sf
  *   mod_filtering_shock.f90:22-1512    shock filter            filter_shock
+ *   mod_artvisc.f90:16-79, 82-874      artificial viscosity    artvisc
+ *   update.f90:568-608            update_varn_artvisc
  *   mod_comm.f90:249-290         halo exchange (periodic self) orc_fill_periodic
  *   mpi_index_bounds.f90         loop bounds for periodic faces and walls
  *   smod_bc_wall.f90:11-98       no-slip wall, dp/dn = 0         bc_wall
@@ -599,6 +601,64 @@ static void filter_shock(const orc_cfg *c, const dims_t *d, double *const Un[5],
 }
 
 /* ------------------------------------------------------------------------ */
+/* Artificial viscosity (artvisc_o9, mod_artvisc.f90:82-874; coefficients   */
+/* init_artvisc :16-79), Cartesian, periodic faces: per direction the face  */
+/* dissipation D_{i+1/2} = sum_l d10(l) (u(i+l) - u(i+1-l)) on 0..n with     */
+/* d10 = (126,-84,36,-9,1) chi/1260, weighted by the spectral radius        */
+/* |u_d| + c of U^n averaged to the faces, K = (sr_{i+1/2} D_{i+1/2} -      */
+/* sr_{i-1/2} D_{i-1/2}) * idx_d(i), x setting K and y, z adding to it.     */
+/* w holds primitives_visc of U^n (velocity and sound speed).               */
+/* ------------------------------------------------------------------------ */
+static void artvisc(const orc_cfg *c, const orc_grid *g, const dims_t *d, double *const Un[5], const work_t *w,
+                    double *K[5]) {
+  const double x10 = c->chi / 1260.0;
+  const double d10[6] = {0.0, 126.0 * x10, -84.0 * x10, 36.0 * x10, -9.0 * x10, 1.0 * x10};
+  const long st[3] = {1, d->sx, d->sx * d->sy};
+  const int nn[3] = {d->nx, d->ny, d->nz};
+  for (int dir = 0; dir < 3; dir++) {
+    const long s = st[dir];
+    const int n = nn[dir];
+    const int o1 = dir == 0 ? 1 : 0, o2 = dir == 2 ? 1 : 2;
+    const double *vel = dir == 0 ? w->uu : (dir == 1 ? w->vv : w->ww);
+    const double *met = dir == 0 ? g->idx : (dir == 1 ? g->idy : g->idz);
+#pragma omp parallel
+    {
+      double *D = (double *)malloc(sizeof(double) * 5 * (size_t)(n + 1));
+#pragma omp for collapse(2)
+      for (int b = 1; b <= nn[o2]; b++)
+        for (int a = 1; a <= nn[o1]; a++) {
+          int idx3[3];
+          idx3[o1] = a; idx3[o2] = b; idx3[dir] = 0;
+          const long p0 = (long)ID(*d, idx3[0], idx3[1], idx3[2]);
+          for (int i = 0; i <= n; i++) {
+            const long p = p0 + i * s;
+            for (int v = 0; v < 5; v++) {
+              const double *u = Un[v];
+              D[v * (n + 1) + i] = d10[1] * (u[p + s] - u[p]) + d10[2] * (u[p + 2 * s] - u[p - s]) +
+                                   d10[3] * (u[p + 3 * s] - u[p - 2 * s]) + d10[4] * (u[p + 4 * s] - u[p - 3 * s]) +
+                                   d10[5] * (u[p + 5 * s] - u[p - 4 * s]);
+            }
+          }
+          for (int i = 1; i <= n; i++) {
+            const long p = p0 + i * s;
+            const double sr = fabs(vel[p]) + w->cs[p];
+            const double srp = 0.5 * (fabs(vel[p + s]) + w->cs[p + s] + sr);
+            const double srm = 0.5 * (fabs(vel[p - s]) + w->cs[p - s] + sr);
+            const double m = met[i - 1 + NG];
+            for (int v = 0; v < 5; v++) {
+              const double *Dv = D + v * (n + 1);
+              const double kd = (srp * Dv[i] - srm * Dv[i - 1]) * m;
+              if (dir == 0) K[v][p] = kd;
+              else K[v][p] = K[v][p] + kd;
+            }
+          }
+        }
+      free(D);
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 /* No-slip wall with dp/dn = 0 (bc_wall_*_adiabatic_dpdn / _isotherm_dpdn,   */
 /* smod_bc_wall.f90:11-98 and its imax..kmax twins), faces in the order of   */
 /* bc_wall (mod_bc_apply.f90:237-261), so a later face wins on an edge.  T   */
@@ -643,7 +703,8 @@ static int check_cfg(const orc_cfg *c) {
   for (int dir = 0; dir < 3; dir++)
     for (int s = 0; s < 2; s++)
       if (c->face[dir][s] != ORC_FACE_PERIODIC) return -2; /* only periodic faces so far */
-  if (c->filter == 2 && !c->visc) return -3;              /* sensor needs the viscous gradients */
+  if (c->filter >= 2 && !c->visc) return -3;              /* sensor / sound speed of primitives_visc */
+  if (c->filter == 3 && c->curv) return -8;               /* artificial viscosity: Cartesian */
   if (c->nrk < 1 || c->nrk > 8) return -4;
   return 0;
 }
@@ -704,6 +765,12 @@ void orc_filter_rhs(const orc_cfg *c, const orc_grid *g, double *const Un[5], do
     for (int q = 0; q < 12; q++) orc_fill_periodic(c, w.grad[q]);
     filter_shock(c, &d, Un, &w, K);
     work_free(&w);
+  } else if (c->filter == 3) {
+    work_t w;
+    if (work_alloc(&d, &w, 1)) { work_free(&w); return; }
+    prim_visc(c, &d, Uref, &w);
+    artvisc(c, g, &d, Un, &w, K);
+    work_free(&w);
   }
 }
 
@@ -743,13 +810,16 @@ int orc_step(const orc_cfg *c, const orc_grid *g, double *U[5], int nsteps) {
     if (c->filter) {
       /* num_dissip then update_varn (runge_kutta.f90:113-126; update.f90:538-565) */
       if (c->filter == 1) filter_sf(c, &d, Un, K);
-      else filter_shock(c, &d, Un, &w, K);           /* sensor on p, grads of U^n */
+      else if (c->filter == 2) filter_shock(c, &d, Un, &w, K);  /* sensor on p, grads of U^n */
+      else artvisc(c, g, &d, Un, &w, K);               /* |u|+c of U^n */
+      /* update_varn_filter adds K, update_varn_artvisc deltat*K (update.f90:538-608) */
+      const double fk = c->filter == 3 ? c->dt : 1.0;
 #pragma omp parallel for collapse(2)
       for (int k = 1; k <= d.nz; k++)
         for (int j = 1; j <= d.ny; j++)
           for (int i = 1; i <= d.nx; i++) {
             long p = (long)ID(d, i, j, k);
-            for (int v = 0; v < 5; v++) Un[v][p] = Un[v][p] + K[v][p];
+            for (int v = 0; v < 5; v++) Un[v][p] = c->filter == 3 ? Un[v][p] + fk * K[v][p] : Un[v][p] + K[v][p];
           }
       for (int v = 0; v < 5; v++) orc_fill_periodic(c, Un[v]);
     }
@@ -807,13 +877,14 @@ void orc_fill_interfaces(int nblocks, const orc_block *bl, double *const *arrs) 
 
 int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_grid *g, double *const *U, int nsteps) {
   if (nb < 1 || nb > 64) return -1;
-  if (c0->filter == 2 && !c0->visc) return -3;
+  if (c0->filter >= 2 && !c0->visc) return -3;
+  if (c0->filter == 3 && c0->curv) return -8;
   for (int b = 0; b < nb; b++)
     for (int f = 0; f < 6; f++)
       if (bl[b].wall[f]) {
         const int ext = f < 2 ? bl[b].nx : (f < 4 ? bl[b].ny : bl[b].nz);
         if (c0->curv) return -5;        /* walls: Cartesian blocks only */
-        if (c0->filter == 2) return -6; /* walls: selective filter or none */
+        if (c0->filter >= 2) return -6; /* walls: selective filter or none */
         if (ext < 10) return -7;        /* closures of the two sides must not overlap */
       }
   orc_cfg cb[64];
@@ -891,14 +962,18 @@ int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_gri
     if (c0->filter) {
       for (int b = 0; b < nb; b++) {
         if (c0->filter == 1) filter_sf(&cb[b], &d[b], Un[b], K[b]);
-        else filter_shock(&cb[b], &d[b], Un[b], &w[b], K[b]);
+        else if (c0->filter == 2) filter_shock(&cb[b], &d[b], Un[b], &w[b], K[b]);
+        else artvisc(&cb[b], &g[b], &d[b], Un[b], &w[b], K[b]);
         const dims_t dd = d[b];
+        const int av = c0->filter == 3;
+        const double fk = c0->dt;
 #pragma omp parallel for collapse(2)
         for (int k = 1; k <= dd.nz; k++)
           for (int j = 1; j <= dd.ny; j++)
             for (int i = 1; i <= dd.nx; i++) {
               long p = (long)ID(dd, i, j, k);
-              for (int v = 0; v < 5; v++) Un[b][v][p] = Un[b][v][p] + K[b][v][p];
+              for (int v = 0; v < 5; v++)
+                Un[b][v][p] = av ? Un[b][v][p] + fk * K[b][v][p] : Un[b][v][p] + K[b][v][p];
             }
       }
       for (int v = 0; v < 5; v++) {

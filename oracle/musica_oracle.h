@@ -37,7 +37,7 @@ typedef struct orc_cfg {
   int nx, ny, nz;          /* interior cells                                */
   int curv;                /* 0 Cartesian (idx 1-D metrics), 1 3-D curvilinear (c3) */
   int visc;                /* 1: 5-pt viscous terms in the last RK stage   */
-  int filter;              /* 0 none, 1 selective filter, 2 shock filter   */
+  int filter;              /* 0 none, 1 selective filter, 2 shock filter, 3 artificial viscosity (Cartesian) */
   int ducros;              /* Ducros sensor in the shock filter            */
   int nrk;                 /* number of RK stages                          */
   int face[3][2];          /* enum orc_face_kind per (direction, side)      */
