@@ -61,6 +61,8 @@ extern "C" {
 #define MGPU_FILTER_NONE 0
 #define MGPU_FILTER_SF 1    /* filtering_11pts, mod_filtering.f90:894      */
 #define MGPU_FILTER_SHOCK 2 /* filter_o10_shock, mod_filtering_shock.f90:204 */
+#define MGPU_FILTER_ARTVISC 3 /* artvisc_o9 (is_SF = F), mod_artvisc.f90:82: Cartesian, FP64,
+                               * iorder_visc = 4 (it reads the sound speed of primitives_visc) */
 
 /* Arithmetic: FP64 everywhere, or FP32 flux/derivative/filter arithmetic with
  * FP64 RK accumulation, metrics and shock sensor. */

@@ -5,6 +5,11 @@
 //   shock filter       filter_o10_shock src/Num_dissipation/mod_filtering_shock.f90:
 //                      pressure sensor :243-260, face dissipation :419-440,
 //                      increment :690-700 (and the j, k repeats)
+//   artificial viscosity artvisc_o9 mod_artvisc.f90:82-874 (coefficients :16-79),
+//                      with update_varn_artvisc (update.f90:568-608): the
+//                      shock filter's face dissipation without the sensor,
+//                      weighted by the spectral radius |u_d| + c of U^n
+//                      averaged to the faces, times the metric, U += deltat*K
 // Interior rows of periodic faces and 1-to-1 interfaces; with no-slip walls
 // on j-faces the selective filter leaves rows 1,2 (ny-1,ny) unfiltered along
 // j and uses the 5/7/9-pt filters on rows 3,4,5 (mod_filtering.f90:900-988,
@@ -128,7 +133,6 @@ __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__
   // the block's array pointers, read once from the table into registers
   const CState uinp = a.uin;
   const State uoutp = a.uout;
-  const double *const prsp = a.prs, *const sensp = a.sens;
   const Geom &g = a.g;
   const int tx = threadIdx.x % FX, ty = threadIdx.x / FX;
   const int i = i0 + tx, j = j0 + ty;
@@ -221,9 +225,19 @@ __device__ __forceinline__ R dface(const R *x, int s, R xk0, R xk10) {
              xk10;
 }
 
-template <bool MIX>
+// 10th difference of the face dissipation (without its amplitude)
+template <class R>
+__device__ __forceinline__ R d10face(const R *x, int s) {
+  return R(D10_1) * (x[s] - x[0]) + R(D10_2) * (x[2 * s] - x[-s]) + R(D10_3) * (x[3 * s] - x[-2 * s]) +
+         R(D10_4) * (x[4 * s] - x[-3 * s]) + R(D10_5) * (x[5 * s] - x[-4 * s]);
+}
+
+// ARTV: the artificial viscosity.  The LDS "pressure" pencils then hold the
+// spectral radii |u|+c (i-pencil) and |v|+c (j-pencil), the k queue |w|+c.
+template <bool MIX, bool ARTV>
 __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> *__restrict__ tab, int nb,
                                                         int ntiles) {
+  static_assert(!(MIX && ARTV), "artificial viscosity: FP64 only");
   using R = typename std::conditional<MIX, float, double>::type;
   __shared__ R sux[5][FY][FX + 10];
   __shared__ R suy[5][FY + 10][FX];
@@ -239,13 +253,14 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
   // the block's array pointers, read once from the table into registers
   const CState uinp = a.uin;
   const State uoutp = a.uout;
-  const double *const prsp = a.prs, *const sensp = a.sens;
+  const double *const prsp = ARTV ? a.sr[2] : a.prs, *const sensp = a.sens;
+  const double *const px = ARTV ? a.sr[0] : a.prs, *const py = ARTV ? a.sr[1] : a.prs;
   const Geom &g = a.g;
   const int tid = threadIdx.x, tx = tid % FX, ty = tid / FX;
   const int i = i0 + tx, j = j0 + ty;
   const bool own = i <= g.nx && j <= g.ny;
   const bool needx = i <= g.nx + 5 && j <= g.ny, needy = i <= g.nx && j <= g.ny + 5;
-  const bool duc = a.sens != nullptr;
+  const bool duc = !ARTV && a.sens != nullptr;
   const double cs = a.csens;
   const idx_t sz = g.sy, own_off = lin(g, i, j, 0);
   Halo hl;
@@ -285,13 +300,18 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
 
   const RefState &rf = a.ref;
   R q[11][5];        // U at k-5..k+5 (mixed: minus the reference state)
-  double qp[5];      // p at k-2..k+2
+  double qp[5];      // p at k-2..k+2 (ARTV: |w|+c at k-1..k+1 in qp[0..2])
   double qs[3];      // sensor at k-1..k+1
   if (own) {
 #pragma unroll
     for (int l = 0; l < 10; l++) ldr<MIX>(uinp, own_off + (idx_t)(kbeg - 5 + l) * sz, rf, q[l]);
+    if constexpr (ARTV) {
 #pragma unroll
-    for (int l = 0; l < 4; l++) qp[l] = ldg(prsp, own_off + (idx_t)(kbeg - 2 + l) * sz);
+      for (int l = 0; l < 2; l++) qp[l] = ldg(prsp, own_off + (idx_t)(kbeg - 1 + l) * sz);
+    } else {
+#pragma unroll
+      for (int l = 0; l < 4; l++) qp[l] = ldg(prsp, own_off + (idx_t)(kbeg - 2 + l) * sz);
+    }
 #pragma unroll
     for (int l = 0; l < 2; l++) qs[l] = duc ? ldg(sensp, own_off + (idx_t)(kbeg - 1 + l) * sz) : 1.0;
   }
@@ -305,7 +325,8 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
     if (own) {
       ldr<MIX>(uinp, pk + 5 * sz, rf, q[10]);
       if constexpr (MIX) ld5(uinp, pk, u64);
-      qp[4] = ldg(prsp, pk + 2 * sz);
+      if constexpr (ARTV) qp[2] = ldg(prsp, pk + sz);
+      else qp[4] = ldg(prsp, pk + 2 * sz);
       qs[2] = duc ? ldg(sensp, pk + sz) : 1.0;
     }
     if (!own && (needx || needy)) ldr<MIX>(uinp, pk, rf, uc);
@@ -315,7 +336,7 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
 #pragma unroll
     for (int s = 0; s < NSP; s++) {
       if (pp_on[s]) {
-        pv[s] = ldg(prsp, pp_off[s] + (idx_t)k * sz);
+        pv[s] = ldg(pp_x[s] ? px : py, pp_off[s] + (idx_t)k * sz);
         sv[s] = (duc && pp_s[s]) ? ldg(sensp, pp_off[s] + (idx_t)k * sz) : 1.0;
       }
     }
@@ -352,13 +373,13 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
     }
     __syncthreads();
     // pressure sensors of the plane
-    for (int t = tid; t < FY * (FX + 2); t += FT) {
+    for (int t = tid; t < (ARTV ? 0 : FY * (FX + 2)); t += FT) {
       const int r = t / (FX + 2), c = t % (FX + 2);
       const int gi = i0 - 1 + c, gj = j0 + r;
       if (gj <= g.ny && gi <= g.nx + 1)
         psx[r][c] = psens3(spx[r][c], spx[r][c + 1], spx[r][c + 2], ssx[r][c + 1], cs);
     }
-    for (int t = tid; t < (FY + 2) * FX; t += FT) {
+    for (int t = tid; t < (ARTV ? 0 : (FY + 2) * FX); t += FT) {
       const int r = t / FX, c = t % FX;
       const int gi = i0 + c, gj = j0 - 1 + r;
       if (gi <= g.nx && gj <= g.ny + 1)
@@ -366,7 +387,46 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
     }
     __syncthreads();
 
-    if (own) {
+    if (ARTV && own) {
+      // K_d = (sr_{+1/2} D_{+1/2} - sr_{-1/2} D_{-1/2}) * metric with D = -x10 * d10face
+      const double x10 = a.xnu10;
+      double K[5];
+      {
+        const double s0 = spx[ty][tx + 2], srp = 0.5 * (spx[ty][tx + 3] + s0), srm = 0.5 * (spx[ty][tx + 1] + s0);
+        const double m = ldg(a.idx, (idx_t)(i - 1 + G));
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+          const R *x = &sux[v][ty][tx + 5];
+          K[v] = (srm * (x10 * (double)d10face<R>(x - 1, 1)) - srp * (x10 * (double)d10face<R>(x, 1))) * m;
+        }
+      }
+      {
+        const double s0 = spy[ty + 2][tx], srp = 0.5 * (spy[ty + 3][tx] + s0), srm = 0.5 * (spy[ty + 1][tx] + s0);
+        const double m = ldg(a.idy, (idx_t)(j - 1 + G));
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+          const R *y = &suy[v][ty + 5][tx];
+          K[v] = K[v] + (srm * (x10 * (double)d10face<R>(y - FX, FX)) - srp * (x10 * (double)d10face<R>(y, FX))) * m;
+        }
+      }
+      {
+        const double s0 = qp[1], srp = 0.5 * (qp[2] + s0), srm = 0.5 * (qp[0] + s0);
+        const double m = ldg(a.idz, (idx_t)(k - 1 + G));
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+          const double dlo = R(D10_1) * (q[5][v] - q[4][v]) + R(D10_2) * (q[6][v] - q[3][v]) +
+                             R(D10_3) * (q[7][v] - q[2][v]) + R(D10_4) * (q[8][v] - q[1][v]) +
+                             R(D10_5) * (q[9][v] - q[0][v]);
+          const double dhi = R(D10_1) * (q[6][v] - q[5][v]) + R(D10_2) * (q[7][v] - q[4][v]) +
+                             R(D10_3) * (q[8][v] - q[3][v]) + R(D10_4) * (q[9][v] - q[2][v]) +
+                             R(D10_5) * (q[10][v] - q[1][v]);
+          K[v] = K[v] + (srm * (x10 * dlo) - srp * (x10 * dhi)) * m;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 5; v++) stg_nt(uoutp.v[v], pk, (double)q[5][v] + a.dt * K[v]);
+    }
+    if (!ARTV && own) {
       R K[5];
       // i faces i-1/2 and i+1/2
       {
@@ -421,8 +481,13 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
     for (int l = 0; l < 10; l++)
 #pragma unroll
       for (int v = 0; v < 5; v++) q[l][v] = q[l + 1][v];
+    if constexpr (ARTV) {
+      qp[0] = qp[1];
+      qp[1] = qp[2];
+    } else {
 #pragma unroll
-    for (int l = 0; l < 4; l++) qp[l] = qp[l + 1];
+      for (int l = 0; l < 4; l++) qp[l] = qp[l + 1];
+    }
     qs[0] = qs[1];
     qs[1] = qs[2];
   }
@@ -457,10 +522,17 @@ hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int nt, bool m
   return hipGetLastError();
 }
 
-hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int nt, bool mixed, hipStream_t s) {
+hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int nt, bool mixed, bool artvisc,
+                               hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  if (mixed) hipLaunchKernelGGL(k_filter_shock<true>, dim3(nt), dim3(FT), 0, s, tab, nb, nt);
-  else hipLaunchKernelGGL(k_filter_shock<false>, dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  if (artvisc) {
+    if (mixed) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_filter_shock<false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  } else if (mixed) {
+    hipLaunchKernelGGL((k_filter_shock<true, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  } else {
+    hipLaunchKernelGGL((k_filter_shock<false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  }
   return hipGetLastError();
 }
 

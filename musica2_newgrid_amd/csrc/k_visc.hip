@@ -8,6 +8,8 @@
 //                                        smod_grad_T_c3.f90:200-213
 //   flux_visc_5pts(_c3) stress and heat flux  smod_flux_visc_5pts.f90:27-67;
 //                                             smod_flux_visc_5pts_c3.f90:27-75
+// and, for the artificial viscosity, the spectral radii |u_d| + c of U^n
+// (c_ of primitives_visc, smod_primitive_pfg.f90:121; mod_artvisc.f90:278-283).
 // It stores the tensor in the physical frame (tau, see mgpu_internal.h), so
 // that the 2-layer interface exchange of tau that follows stands for the
 // reference's exchange of the gradients (communication_3dv,
@@ -71,6 +73,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(const Batch<ViscArgs> *__res
 #pragma unroll
   for (int c = 0; c < NTAU; c++) taup[c] = a.tau[c];
   double *const prsp = a.prs, *const sensp = a.sens;
+  double *const sr0 = a.sr[0], *const sr1 = a.sr[1], *const sr2 = a.sr[2];
   const double *const idxv = a.met.idx_v, *const idyv = a.met.idy_v, *const idzv = a.met.idz_v;
   const Geom &g = a.g;
   int tile_x, tile_y, chunk;
@@ -258,6 +261,13 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(const Batch<ViscArgs> *__res
           s = div2 / (div2 + o1 * o1 + o2 * o2 + o3 * o3 + 1.0e-16);
         }
         stg(sensp, p, s);
+      }
+      if (a.want_sr) {
+        // spectral radii of the artificial viscosity, c = sqrt(gam R T) (primitives_visc)
+        const double c = sqrt(a.gas.gam * rg * T);
+        stg(sr0, p, fabs(u) + c);
+        stg(sr1, p, fabs(v) + c);
+        stg(sr2, p, fabs(w) + c);
       }
     }
     __syncthreads();

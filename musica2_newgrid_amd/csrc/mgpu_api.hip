@@ -159,7 +159,7 @@ std::string check_topology(const mgpu_case &c) {
           snprintf(buf, sizeof buf, "block %d face %d: walls are supported on the j-faces (2, 3) only", b, f);
           return buf;
         }
-        if (c.curvilinear || c.precision != MGPU_PREC_FP64 || c.filter == MGPU_FILTER_SHOCK) {
+        if (c.curvilinear || c.precision != MGPU_PREC_FP64 || c.filter >= MGPU_FILTER_SHOCK) {
           snprintf(buf, sizeof buf,
                    "block %d face %d: walls need a Cartesian FP64 case with the selective filter or none", b, f);
           return buf;
@@ -368,8 +368,8 @@ int setup_exchange(mgpu_handle *h) {
       return fail(h, MGPU_EINVAL, "viscous interface exchange would reach outside a block's arrays");
     if ((rc = upload(h, tasks, &dt)) || (rc = upload(h, chunks, &dc))) return rc;
     h->nchunks_v = (int)chunks.size();
-    const bool shock = h->c.filter == MGPU_FILTER_SHOCK;
-    const int nv = NTAU + (shock ? 2 : 0);
+    const bool shock = h->c.filter == MGPU_FILTER_SHOCK, artv = h->c.filter == MGPU_FILTER_ARTVISC;
+    const int nv = NTAU + (shock ? 2 : 0) + (artv ? 3 : 0);
     std::vector<double *> ptrs;
     for (int b = 0; b < nb; b++) {
       for (int q = 0; q < NTAU; q++) ptrs.push_back(h->b[b].tau[q]);
@@ -377,6 +377,8 @@ int setup_exchange(mgpu_handle *h) {
         ptrs.push_back(h->b[b].prs);
         ptrs.push_back(h->b[b].sens);
       }
+      if (artv)
+        for (int q = 0; q < 3; q++) ptrs.push_back(h->b[b].sr[q]);
     }
     double **dp = nullptr;
     if ((rc = upload(h, ptrs, &dp))) return rc;
@@ -425,6 +427,7 @@ void free_block(BlockData &bd) {
   if (bd.sens) (void)hipFree(bd.sens);
   if (bd.wstash) (void)hipFree(bd.wstash);
   bd.prs = bd.sens = bd.wstash = nullptr;
+  for (auto &p : bd.sr) { if (p) (void)hipFree(p); p = nullptr; }
 }
 
 int dalloc(mgpu_handle *h, double **p, size_t count) {
@@ -546,6 +549,11 @@ FilterArgs filter_args(const mgpu_handle *h, const BlockData &bd, int in, int ou
   fa.xnu10 = bd.p.dissip_coeff / 1260.0;
   fa.csens = bd.p.csens;
   fa.ref = bd.ref;
+  for (int q = 0; q < 3; q++) fa.sr[q] = bd.sr[q];
+  fa.idx = bd.idx[0];
+  fa.idy = bd.idx[1];
+  fa.idz = bd.idx[2];
+  fa.dt = bd.p.deltat;
   return fa;
 }
 
@@ -609,7 +617,9 @@ int build_tables(mgpu_handle *h) {
       for (int q = 0; q < NTAU; q++) va.tau[q] = bd.tau[q];
       va.prs = bd.prs;
       va.sens = bd.sens;
+      for (int q = 0; q < 3; q++) va.sr[q] = bd.sr[q];
       va.want_sensor = c.filter == MGPU_FILTER_SHOCK;
+      va.want_sr = c.filter == MGPU_FILTER_ARTVISC;
       va.ducros = c.is_ducros;
       vi[s0 * nb + b].t = tv;
       fi[s0 * nb + b].a = filter_args(h, bd, in, s0);  // reads the last stage's output
@@ -729,7 +739,9 @@ int enqueue_step(mgpu_handle *h, int s0) {
       ProfScope ps(h, MGPU_K_FILTER);
       if (c.filter == MGPU_FILTER_SF)
         HIPCK(h, launch_filter_sf(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, h->wall, h->stream));
-      else HIPCK(h, launch_filter_shock(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, h->stream));
+      else
+        HIPCK(h, launch_filter_shock(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed,
+                                     c.filter == MGPU_FILTER_ARTVISC, h->stream));
     }
     if ((rc = exchange_state(h, s0))) return rc;
     if (h->wall) {
@@ -784,9 +796,13 @@ std::string check_flags(const mgpu_case &c) {
   if (c.nrk < 1 || c.nrk > 8) snprintf(buf, sizeof buf, "nrk=%d out of range 1..8", c.nrk);
   else if (c.iorder_visc != 0 && c.iorder_visc != 4)
     snprintf(buf, sizeof buf, "iorder_visc=%d: only 0 and 4 are supported", c.iorder_visc);
-  else if (c.filter < 0 || c.filter > 2) snprintf(buf, sizeof buf, "filter=%d", c.filter);
+  else if (c.filter < 0 || c.filter > 3) snprintf(buf, sizeof buf, "filter=%d", c.filter);
   else if (c.filter == MGPU_FILTER_SHOCK && c.iorder_visc == 0)
     snprintf(buf, sizeof buf, "shock filter needs iorder_visc=4 (its sensor reads the viscous gradients of U^n)");
+  else if (c.filter == MGPU_FILTER_ARTVISC && c.iorder_visc == 0)
+    snprintf(buf, sizeof buf, "artificial viscosity needs iorder_visc=4 (it reads the sound speed of U^n)");
+  else if (c.filter == MGPU_FILTER_ARTVISC && (c.curvilinear || c.precision != MGPU_PREC_FP64))
+    snprintf(buf, sizeof buf, "artificial viscosity: Cartesian FP64 cases only");
   else if (c.precision != MGPU_PREC_FP64 && c.precision != MGPU_PREC_MIXED)
     snprintf(buf, sizeof buf, "precision=%d", c.precision);
   else if (c.precision == MGPU_PREC_MIXED && c.nrk < 2)
@@ -818,7 +834,8 @@ size_t block_bytes(const mgpu_case &c, int nx, int ny, int nz) {
   const size_t n = (size_t)(nx + 2 * G) * (ny + 2 * G) * (nz + 2 * G);
   size_t arrays = 15;  // three state sets
   if (c.curvilinear) arrays += c.iorder_visc ? 20 : 10;
-  if (c.iorder_visc) arrays += NTAU + 4 + (c.filter == MGPU_FILTER_SHOCK ? 2 : 0);
+  if (c.iorder_visc)
+    arrays += NTAU + 4 + (c.filter == MGPU_FILTER_SHOCK ? 2 : 0) + (c.filter == MGPU_FILTER_ARTVISC ? 3 : 0);
   size_t bytes = arrays * n * sizeof(double);
   if (c.precision == MGPU_PREC_MIXED) bytes += 10 * n * sizeof(float);
   if (!c.curvilinear) bytes += 6 * (size_t)(nx + ny + nz + 6 * G) * sizeof(double);
@@ -868,6 +885,8 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
       if (!rc) rc = dalloc(h, &bd.prs, n);
       if (!rc) rc = dalloc(h, &bd.sens, n);
     }
+    if (c.filter == MGPU_FILTER_ARTVISC)
+      for (int q = 0; q < 3 && !rc; q++) rc = dalloc(h, &bd.sr[q], n);
   }
   if (rc) {
     free_block(bd);
@@ -1121,7 +1140,8 @@ int tune_aux(mgpu_handle *h, bool force, hipEvent_t e0, hipEvent_t e1) {
         if (kind == 1) return launch_vdiv(h->d_vdiv, nb, h->nt_vdiv, curv, h->stream);
         if (h->c.filter == MGPU_FILTER_SF)
           return launch_filter_sf(h->d_tune_f, nb, h->nt_filt, h->mixed, h->wall, h->stream);
-        return launch_filter_shock(h->d_tune_f, nb, h->nt_filt, h->mixed, h->stream);
+        return launch_filter_shock(h->d_tune_f, nb, h->nt_filt, h->mixed, h->c.filter == MGPU_FILTER_ARTVISC,
+                                   h->stream);
       };
       hipError_t e = launch();
       if (e == hipSuccess) e = hipEventRecord(e0, h->stream);

@@ -24,6 +24,7 @@ struct BlockData {
   double *tau[NTAU] = {}; // physical viscous flux tensor of U^n (last stage)
   double *kv[4] = {};     // viscous increment of rhou..rhoe (last stage)
   double *prs = nullptr, *sens = nullptr;
+  double *sr[3] = {};     // artificial viscosity: |u|+c, |v|+c, |w|+c of U^n (last stage)
   RefState ref{};         // mixed precision: interior mean of the state at put_state
   double *wstash = nullptr;  // walls: rho, rhoE of the end-of-step wall state (2 per wall point)
   double c2[2] = {0, 0}, c3[2] = {0, 0};  // walls: cextrp2, cextrp3 of the jmin, jmax faces

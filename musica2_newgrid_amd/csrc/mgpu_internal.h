@@ -211,7 +211,9 @@ struct ViscArgs {
   double *tau[NTAU];  // out: interior points
   double *prs;        // out: p(U^n), interior points (shock filter)
   double *sens;       // out: Ducros sensor, interior points (shock filter)
+  double *sr[3];      // out: |u|+c, |v|+c, |w|+c of U^n, interior points (artificial viscosity)
   int want_sensor;    // 1: write prs and sens
+  int want_sr;        // 1: write sr
   int ducros;
 };
 
@@ -234,6 +236,9 @@ struct FilterArgs {
   double dd[6];        // -chi*d11 (selective filter)
   double chi;          // filter amplitude (the 5/7/9-pt filters next to walls)
   double xnu0, xnu10, csens;
+  const double *sr[3];  // artificial viscosity: |u|+c, |v|+c, |w|+c of U^n
+  const double *idx, *idy, *idz;  // artificial viscosity: Cartesian metrics
+  double dt;            // artificial viscosity: U += deltat*K
   RefState ref;        // mixed precision only
 };
 
@@ -324,7 +329,10 @@ hipError_t launch_vdiv(const Batch<VdivArgs> *tab, int nb, int ntiles, bool curv
 TileRange filter_tiles(const Geom &g, int kchunk);
 hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, bool wall,
                             hipStream_t s);
-hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, hipStream_t s);
+// The shock filter, or (artvisc) the artificial viscosity, which shares its
+// face-dissipation structure.
+hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, bool artvisc,
+                               hipStream_t s);
 hipError_t launch_iface(const IfaceArgs &a, int nchunks, hipStream_t s);
 hipError_t launch_iface(const IfaceArgs32 &a, int nchunks, hipStream_t s);
 long diag_groups(const Geom &g);

@@ -84,9 +84,18 @@ program musica_gpu_tgv
   cs%nblocks = 1
   cs%curvilinear = 0
   cs%iorder_visc = prm%iorder_visc
-  cs%filter = MGPU_FILTER_NONE
-  if (prm%is_SF) cs%filter = MGPU_FILTER_SF
-  if (prm%is_SF .and. prm%is_shock) cs%filter = MGPU_FILTER_SHOCK
+  ! num_dissip as assign_procedures binds it (is_dissip_in_increments = F):
+  ! filtering_11pts, filter_o10_shock, or artvisc_o9 when is_SF = F
+  if (prm%is_SF) then
+     cs%filter = MGPU_FILTER_SF
+     if (prm%is_shock) cs%filter = MGPU_FILTER_SHOCK
+  else
+     if (prm%is_shock) then
+        write(*,*) 'artificial viscosity with shock capturing (artvisc_o9_shock) is not supported'
+        stop 1
+     endif
+     cs%filter = MGPU_FILTER_ARTVISC
+  endif
   cs%is_ducros = merge(1, 0, prm%is_ducros)
   cs%nrk = prm%nrk
   cs%precision = MGPU_PREC_FP64

@@ -117,3 +117,17 @@ def test_wall_limits_are_checked():
     blocks[0]["ny"] = 9
     rc, msg = _create_case(c, blocks)
     assert rc == _lib.EINVAL and "ny >= 10" in msg
+
+
+def test_artvisc_flags_are_checked():
+    """Artificial viscosity: Cartesian FP64 with the viscous terms."""
+    c = cases.tgv(16, filter=3)
+    rc, msg = _create_case(c, None, precision=_lib.PREC_MIXED)
+    assert rc == _lib.EINVAL and "artificial viscosity" in msg
+    c = cases.tgv(16, filter=3, iorder_visc=0)
+    rc, msg = _create_case(c, None)
+    assert rc == _lib.EINVAL and "sound speed" in msg
+    c = cases.tgv(16, filter=3)
+    c.curvilinear = True
+    rc, msg = _create_case(c, None)
+    assert rc == _lib.EINVAL and "Cartesian" in msg
