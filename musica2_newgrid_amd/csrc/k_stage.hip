@@ -315,6 +315,10 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
   const RefState &rf = a.ref;
   const double alpha = a.alpha;
   R q[11][5];
+  // raw inputs of the queue point k+5, loaded one plane ahead (after the
+  // previous plane's barrier) so that the load is in flight during that
+  // plane's derivatives
+  Raw rq;
   if (own) {
 #pragma unroll
     for (int l = 0; l < 10; l++) {
@@ -322,6 +326,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
       load_raw<CURV, IN32>(P, own_off + (idx_t)(kbeg - 5 + l) * sz, 2, r);
       flux_r<CURV, MIX>(prim_raw(r.u, gam1), 2, r.m, rf, q[l]);
     }
+    load_raw<CURV, IN32>(P, own_off + (idx_t)(kbeg + 5) * sz, 2, rq);
   }
 
   for (int k = kbeg; k <= kend; k++) {
@@ -329,10 +334,9 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
     R (*sfx)[TY][TX + 10] = sfx2[(k - kbeg) & 1];
     R (*sfy)[TY + 10][TX] = sfy2[(k - kbeg) & 1];
     // ---- batch 1: queue point k+5, own point k, U^n and 1/J at k, halo cell 0 ----
-    Raw rq, rh0;
+    Raw rh0;
     double rxm[3] = {0, 0, 0}, rym[3] = {0, 0, 0}, ru[5] = {0, 0, 0, 0, 0}, u0[5] = {0, 0, 0, 0, 0};
     double ijac = 0.0;
-    if (own) load_raw<CURV, IN32>(P, pk + 5 * sz, 2, rq);
     if (needx || needy) {
       load_u<IN32>(P, pk, ru);
       if constexpr (CURV) {
@@ -347,11 +351,11 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
     double kvis[4] = {0, 0, 0, 0};
     if (own) {
 #pragma unroll
-      for (int v = 0; v < 5; v++) u0[v] = ldg(P.u0[v], pk);
-      if constexpr (CURV) ijac = ldg(P.m[9], pk);
+      for (int v = 0; v < 5; v++) u0[v] = ldg_nt(P.u0[v], pk);
+      if constexpr (CURV) ijac = ldg_nt(P.m[9], pk);
       if constexpr (VISC) {
 #pragma unroll
-        for (int c = 0; c < 4; c++) kvis[c] = ldg(P.kv[c], pk);
+        for (int c = 0; c < 4; c++) kvis[c] = ldg_nt(P.kv[c], pk);
       }
     }
     if (own) flux_r<CURV, MIX>(prim_raw(rq.u, gam1), 2, rq.m, rf, q[10]);
@@ -403,6 +407,8 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
       }
     }
     __syncthreads();
+    // queue point of the next plane
+    if (own && k < kend) load_raw<CURV, IN32>(P, pk + 6 * sz, 2, rq);
 
     if (own) {
       double K[5];

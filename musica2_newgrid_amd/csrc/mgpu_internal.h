@@ -85,6 +85,9 @@ __device__ __forceinline__ MGPU_GLOBAL T *gaddr(T *b, idx_t i) {
 }
 __device__ __forceinline__ double ldg(const double *b, idx_t i) { return *gaddr(b, i); }
 __device__ __forceinline__ void stg(double *b, idx_t i, double v) { *gaddr(b, i) = v; }
+// Streaming load for data only the loading point reads (no neighbour re-reads
+// it from L2), so that it does not evict halo data other workgroups re-read.
+__device__ __forceinline__ double ldg_nt(const double *b, idx_t i) { return __builtin_nontemporal_load(gaddr(b, i)); }
 // Streaming store for results no kernel of the same launch reads back.
 __device__ __forceinline__ void stg_nt(double *b, idx_t i, double v) { __builtin_nontemporal_store(v, gaddr(b, i)); }
 // The same for FP32 arrays.
