@@ -328,25 +328,28 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
     }
     load_raw<CURV, IN32>(P, own_off + (idx_t)(kbeg + 5) * sz, 2, rq);
   }
+  // the own point's state and i/j metric terms, likewise one plane ahead
+  double rxm[3] = {0, 0, 0}, rym[3] = {0, 0, 0}, ru[5] = {0, 0, 0, 0, 0};
+  auto load_own = [&](idx_t p) {
+    load_u<IN32>(P, p, ru);
+    if constexpr (CURV) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        rxm[c] = ldg(P.m[c], p);
+        rym[c] = ldg(P.m[3 + c], p);
+      }
+    }
+  };
+  if (needx || needy) load_own(own_off + (idx_t)kbeg * sz);
 
   for (int k = kbeg; k <= kend; k++) {
     const idx_t pk = own_off + (idx_t)k * sz;
     R (*sfx)[TY][TX + 10] = sfx2[(k - kbeg) & 1];
     R (*sfy)[TY + 10][TX] = sfy2[(k - kbeg) & 1];
-    // ---- batch 1: queue point k+5, own point k, U^n and 1/J at k, halo cell 0 ----
+    // ---- batch 1: U^n and 1/J at k, halo cell 0 (queue point and own point: loaded ahead) ----
     Raw rh0;
-    double rxm[3] = {0, 0, 0}, rym[3] = {0, 0, 0}, ru[5] = {0, 0, 0, 0, 0}, u0[5] = {0, 0, 0, 0, 0};
+    double u0[5] = {0, 0, 0, 0, 0};
     double ijac = 0.0;
-    if (needx || needy) {
-      load_u<IN32>(P, pk, ru);
-      if constexpr (CURV) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-          rxm[c] = ldg(P.m[c], pk);
-          rym[c] = ldg(P.m[3 + c], pk);
-        }
-      }
-    }
     if (hs_on[0]) load_raw_halo<CURV, IN32>(P, hs_off[0] + (idx_t)k * sz, hs_dir[0], rh0);
     double kvis[4] = {0, 0, 0, 0};
     if (own) {
@@ -407,8 +410,9 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
       }
     }
     __syncthreads();
-    // queue point of the next plane
+    // queue point and own point of the next plane
     if (own && k < kend) load_raw<CURV, IN32>(P, pk + 6 * sz, 2, rq);
+    if ((needx || needy) && k < kend) load_own(pk + sz);
 
     if (own) {
       double K[5];
