@@ -151,16 +151,25 @@ __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__
 #pragma unroll
     for (int l = 0; l < 10; l++) ldr<MIX>(uinp, own_off + (idx_t)(kbeg - 5 + l) * sz, rf, q[l]);
   }
-  for (int k = kbeg; k <= kend; k++) {
-    const idx_t pk = own_off + (idx_t)k * sz;
-    R uc[5], uh[NS][5];
-    double u64[5];  // mixed: the FP64 state of the own point, to which K is added
-    if (own) ldr<MIX>(uinp, pk + 5 * sz, rf, q[10]);
+  // the state loads of a plane (queue point k+5, pencil cell, halo cells)
+  // are issued one plane ahead, during the previous plane's stencils
+  R qn[5], uc[5], uh[NS][5];
+  {
+    const idx_t pk = own_off + (idx_t)kbeg * sz;
+    if (own) ldr<MIX>(uinp, pk + 5 * sz, rf, qn);
     if (!own && (needx || needy)) ldr<MIX>(uinp, pk, rf, uc);
-    if (MIX && own) ld5(uinp, pk, u64);
 #pragma unroll
     for (int s = 0; s < NS; s++)
-      if (hl.on[s]) ldr<MIX>(uinp, hl.off[s] + (idx_t)k * sz, rf, uh[s]);
+      if (hl.on[s]) ldr<MIX>(uinp, hl.off[s] + (idx_t)kbeg * sz, rf, uh[s]);
+  }
+  for (int k = kbeg; k <= kend; k++) {
+    const idx_t pk = own_off + (idx_t)k * sz;
+    double u64[5];  // mixed: the FP64 state of the own point, to which K is added
+    if (own) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) q[10][v] = qn[v];
+    }
+    if (MIX && own) ld5(uinp, pk, u64);
     if (needx || needy) {
 #pragma unroll
       for (int v = 0; v < 5; v++) {
@@ -180,6 +189,14 @@ __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__
       }
     }
     __syncthreads();
+    if (k < kend) {
+      const idx_t pn = pk + sz;
+      if (own) ldr<MIX>(uinp, pn + 5 * sz, rf, qn);
+      if (!own && (needx || needy)) ldr<MIX>(uinp, pn, rf, uc);
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+        if (hl.on[s]) ldr<MIX>(uinp, hl.off[s] + (idx_t)(k + 1) * sz, rf, uh[s]);
+    }
     if (own) {
       const int hw = WALL ? jhalf(g, j) : 5;
 #pragma unroll
@@ -316,29 +333,41 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
     for (int l = 0; l < 2; l++) qs[l] = duc ? ldg(sensp, own_off + (idx_t)(kbeg - 1 + l) * sz) : 1.0;
   }
 
-  for (int k = kbeg; k <= kend; k++) {
-    const idx_t pk = own_off + (idx_t)k * sz;
-    // ---- every global load of the plane ----
-    R uc[5], uh[NS][5];
-    double pv[NSP], sv[NSP];
-    double u64[5];  // mixed: the FP64 state of the own point, to which K is added
+  // every global load of a plane is issued one plane ahead, during the
+  // previous plane's stencils: queue points, pencil cell, halo cells, p/sensor
+  R qn[5], uc[5], uh[NS][5];
+  double qpn = 0.0, qsn = 1.0, pv[NSP], sv[NSP];
+  auto load_plane = [&](int kk) {
+    const idx_t pkk = own_off + (idx_t)kk * sz;
     if (own) {
-      ldr<MIX>(uinp, pk + 5 * sz, rf, q[10]);
-      if constexpr (MIX) ld5(uinp, pk, u64);
-      if constexpr (ARTV) qp[2] = ldg(prsp, pk + sz);
-      else qp[4] = ldg(prsp, pk + 2 * sz);
-      qs[2] = duc ? ldg(sensp, pk + sz) : 1.0;
+      ldr<MIX>(uinp, pkk + 5 * sz, rf, qn);
+      qpn = ARTV ? ldg(prsp, pkk + sz) : ldg(prsp, pkk + 2 * sz);
+      qsn = duc ? ldg(sensp, pkk + sz) : 1.0;
     }
-    if (!own && (needx || needy)) ldr<MIX>(uinp, pk, rf, uc);
+    if (!own && (needx || needy)) ldr<MIX>(uinp, pkk, rf, uc);
 #pragma unroll
     for (int s = 0; s < NS; s++)
-      if (hl.on[s]) ldr<MIX>(uinp, hl.off[s] + (idx_t)k * sz, rf, uh[s]);
+      if (hl.on[s]) ldr<MIX>(uinp, hl.off[s] + (idx_t)kk * sz, rf, uh[s]);
 #pragma unroll
     for (int s = 0; s < NSP; s++) {
       if (pp_on[s]) {
-        pv[s] = ldg(pp_x[s] ? px : py, pp_off[s] + (idx_t)k * sz);
-        sv[s] = (duc && pp_s[s]) ? ldg(sensp, pp_off[s] + (idx_t)k * sz) : 1.0;
+        pv[s] = ldg(pp_x[s] ? px : py, pp_off[s] + (idx_t)kk * sz);
+        sv[s] = (duc && pp_s[s]) ? ldg(sensp, pp_off[s] + (idx_t)kk * sz) : 1.0;
       }
+    }
+  };
+  load_plane(kbeg);
+
+  for (int k = kbeg; k <= kend; k++) {
+    const idx_t pk = own_off + (idx_t)k * sz;
+    double u64[5];  // mixed: the FP64 state of the own point, to which K is added
+    if (own) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) q[10][v] = qn[v];
+      if constexpr (MIX) ld5(uinp, pk, u64);
+      if constexpr (ARTV) qp[2] = qpn;
+      else qp[4] = qpn;
+      qs[2] = qsn;
     }
     // ---- stage the plane in LDS ----
     if (needx || needy) {
@@ -372,6 +401,7 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
       }
     }
     __syncthreads();
+    if (k < kend) load_plane(k + 1);
     // pressure sensors of the plane
     for (int t = tid; t < (ARTV ? 0 : FY * (FX + 2)); t += FT) {
       const int r = t / (FX + 2), c = t % (FX + 2);
