@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 # viscous last stage): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch,
 # FETCH_SIZE scaled by 2.000 and WRITE_SIZE by 1.000 (their readings for a
 # 1 GiB device copy).  Summary: profiles/step_kernels_c3_512x256x128_fp64.md.
-STAGE_TRAFFIC_BYTES = (3 * 4.475e9 + 5.192e9) / 4
+STAGE_TRAFFIC_BYTES = (3 * 4.589e9 + 5.338e9) / 4
 STAGE_TRAFFIC_WORKLOAD = (512, 256, 128)
 
 
