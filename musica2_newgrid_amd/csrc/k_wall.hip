@@ -11,8 +11,10 @@
 // k_visc_wall   tau on rows 1,2 (ny-1,ny) with the 5-pt one-sided j-gradients
 //               a04/a13 (a31/a40) of grad_vel_5pts / grad_T_5pts
 //               (src/Compute_flow/smod_grad_vel.f90:12-100 and its j part,
-//               smod_grad_T.f90).  On the wall row itself it stores tau/mu and
-//               -(cp/Pr) grad T: see k_tau_wallrow.
+//               smod_grad_T.f90).  An adiabatic wall has no wall-normal
+//               temperature gradient on its row (smod_grad_T.f90:102-109).
+//               On the wall row itself it stores tau/mu and -(cp/Pr) grad T:
+//               see k_tau_wallrow.
 //
 // k_tau_wallrow after the 2-layer exchange of tau: on the wall row, the
 //               reference evaluates the viscous flux at every point, face
@@ -160,6 +162,7 @@ __global__ __launch_bounds__(kWallThreads) void k_visc_wall(WallViscArgs a) {
     }
 #pragma unroll
     for (int n = 0; n < 4; n++) gr[n][1] = gy[n] * iy;
+    if (a.adiabatic && (q == 0 || q == 3)) gr[3][1] = 0.0;
   }
   const idx_t p = lin(g, i, j, k);
   const Pw o = prim_at(a.u0, p, gam1, rg);

@@ -677,6 +677,7 @@ WallViscArgs wall_visc_args(const BlockData &bd, int s0) {
   a.met = metrics_of(bd);
   for (int q = 0; q < NTAU; q++) a.tau[q] = bd.tau[q];
   for (int q = 0; q < 4; q++) a.kv[q] = bd.kv[q];
+  a.adiabatic = bd.p.is_adiab;
   return a;
 }
 

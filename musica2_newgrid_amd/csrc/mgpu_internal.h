@@ -271,6 +271,7 @@ struct WallViscArgs {
   Metrics met;
   double *tau[NTAU];
   double *kv[4];
+  int adiabatic;  // no wall-normal temperature gradient on the wall row
 };
 
 // Component c of tau in the flux along direction d of a Cartesian block (row

@@ -385,6 +385,13 @@ static void gradients(const orc_cfg *c, const orc_grid *g, const dims_t *d, work
           w->grad[3 * q + 1][p] = gy;
           w->grad[3 * q + 2][p] = gz;
         }
+        /* adiabatic walls: no wall-normal temperature gradient on the wall row
+         * (smod_grad_T.f90:102-109 and its twins) */
+        if (c->adiabatic && W.any) {
+          const int rr[3] = {i, j, k}, nn[3] = {d->nx, d->ny, d->nz};
+          for (int a = 0; a < 3; a++)
+            if ((W.lo[a] && rr[a] == 1) || (W.hi[a] && rr[a] == nn[a])) w->grad[DTX + a][p] = 0.0;
+        }
       }
 }
 

@@ -298,3 +298,31 @@ def test_channel_cut_differs_only_next_to_the_cut():
         assert np.array_equal(B[v][:, :, far], ref[:, :, far])
         differs |= not np.array_equal(B[v], ref)
     assert differs
+
+
+def test_adiabatic_wall_has_no_normal_heat_flux_on_its_row():
+    """Adiabatic walls zero the wall-normal temperature gradient on the wall
+    row (smod_grad_T.f90:102-109): on a gas at rest with T linear in y, the
+    adiabatic minus the isothermal viscous increment of rho*E is the removed
+    wall heat flux kappa*a carried by the divergence stencils that touch the
+    wall row: a04(1) on row 1, a13(1) on row 2, -a5(2) on row 3, nothing
+    further in."""
+    ye = (np.arange(NY) - 0.5 * (NY - 1)) * H
+    slope = 40.0
+    T = 300.0 + slope * ye
+    p0 = 1.0e5
+    rho = p0 / (cases.RG * T)
+    out = {}
+    for adiab in (True, False):
+        c, _ = _box(NX, NY, NZ, H, _fields_y(NX, NY, NZ, rho, 0.0, 0.0, 0.0, p0))
+        c.adiabatic = adiab
+        out[adiab] = _rows(orc.Oracle(c).visc_rhs(c.U)[4])
+    T1 = T[0]
+    kappa1 = cases.C0 * math.sqrt(T1 ** 3) / (T1 + cases.S0) * c.diffscale * c.cok_ratio
+    idv = c.cart[4][G:G + NY]
+    coef = np.zeros(NY)
+    coef[0], coef[1], coef[2] = gr.A04[0], gr.A13[0], -gr.A5[1]
+    expect = coef * kappa1 * slope * idv
+    d = out[True] - out[False]
+    assert np.max(np.abs(d[:NY // 2] - expect[:NY // 2])) < 1e-9 * np.max(np.abs(expect))
+    assert abs(d[0]) > 0.0
