@@ -53,8 +53,11 @@ extern "C" {
  * src/File_IO/read_param.f90:75): bc_wall_*_adiabatic_dpdn or
  * _isotherm_dpdn (src/Boundary/smod_bc_wall.f90) with the boundary-row
  * closures of the Euler, viscous and filter operators.  On the j-faces (2, 3)
- * of Cartesian FP64 cases with the selective filter or none, ny >= 10; the
- * block's extrapolation coefficients come from mgpu_set_wall. */
+ * of Cartesian FP64 cases with the selective or shock filter or none,
+ * ny >= 10; the block's extrapolation coefficients come from mgpu_set_wall.
+ * With the shock filter the ghost row beyond a wall must hold a valid state:
+ * the wall row's pressure sensor reads p there (mod_filtering_shock.f90:
+ * 243-260 with ndy_d1 = 1), and nothing on the path writes it. */
 #define MGPU_FACE_WALL 2
 
 /* Filter selection: is_SF / is_shock (src/Main/assign_procedures.f90:161-219) */

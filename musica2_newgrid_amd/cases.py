@@ -310,7 +310,8 @@ def channel(nx: int = 64, ny: int = 48, nz: int = 16, lengths=(4.0, 2.0, 1.0), b
     for f in conserved(rho, u, v, w, p):
         a = gr.padded(nx, ny, nz)
         a[G:-G, G:-G, G:-G] = f
-        a[G:-G, :G, G:-G] = f[:, :1, :]                # ghosts beyond the walls: never read
+        a[G:-G, :G, G:-G] = f[:, :1, :]                # ghosts beyond the walls: only the shock
+        # filter reads them (p one row beyond each wall, in the wall row's pressure sensor)
         a[G:-G, G + ny:, G:-G] = f[:, -1:, :]
         gr.fill_periodic(a, axes=(0, 2))
         U.append(a)

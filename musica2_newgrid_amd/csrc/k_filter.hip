@@ -249,12 +249,27 @@ __device__ __forceinline__ R d10face(const R *x, int s) {
          R(D10_4) * (x[4 * s] - x[-3 * s]) + R(D10_5) * (x[5 * s] - x[-4 * s]);
 }
 
+// Half-width of the shock filter's j-face m+1/2 (between rows m and m+1) of a
+// block with walls: the d2/d4/d6/d8 faces are the first four from a wall, the
+// wall face with a tenth of the sensor term (filter_o10_shock,
+// mod_filtering_shock.f90:724-870 jmin, :915-1057 jmax), d10 inside; <= 0: no
+// face (the wall rows get no j-increment, :1128-1140 runs over ndy_d..nfy_d).
+__device__ __forceinline__ int face_half(const Geom &g, int m) {
+  int h = 5;
+  if ((g.jwall & 1) && m < h) h = m;
+  if ((g.jwall & 2) && g.ny - m < h) h = g.ny - m;
+  return h;
+}
+
 // ARTV: the artificial viscosity.  The LDS "pressure" pencils then hold the
 // spectral radii |u|+c (i-pencil) and |v|+c (j-pencil), the k queue |w|+c.
-template <bool MIX, bool ARTV>
+// WALL: blocks with j-walls; rows 1..5 and ny-4..ny get no j-increment here
+// (k_shock_wall adds it with the reduced faces of the wall closure).
+template <bool MIX, bool ARTV, bool WALL>
 __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> *__restrict__ tab, int nb,
                                                         int ntiles) {
   static_assert(!(MIX && ARTV), "artificial viscosity: FP64 only");
+  static_assert(!(WALL && (MIX || ARTV)), "walls: FP64 shock filter only");
   using R = typename std::conditional<MIX, float, double>::type;
   __shared__ R sux[5][FY][FX + 10];
   __shared__ R suy[5][FY + 10][FX];
@@ -469,8 +484,8 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
           K[v] = dface<R>(x - 1, 1, (R)xk0m, (R)xk10m) - dface<R>(x, 1, (R)xk0p, (R)xk10p);
         }
       }
-      // j faces
-      {
+      // j faces (walls: rows 1..5 and ny-4..ny take theirs from k_shock_wall)
+      if (!WALL || (face_half(g, j - 1) >= 5 && face_half(g, j) >= 5)) {
         const double pm = psy[ty][tx], p0 = psy[ty + 1][tx], pp = psy[ty + 2][tx];
         const double xk0m = a.xnu0 * fmax(p0, pm), xk10m = fmax(0.0, a.xnu10 + xk0m / 105.0);
         const double xk0p = a.xnu0 * fmax(pp, p0), xk10p = fmax(0.0, a.xnu10 + xk0p / 105.0);
@@ -523,7 +538,80 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
   }
 }
 
+// Differences d2, d4, d6, d8, d10 of the face dissipation by half-width
+// (init_filter_shock with iSF = 0, mod_filtering_shock.f90:75-107).
+__constant__ double kShockD[6][5] = {{0.0, 0.0, 0.0, 0.0, 0.0},     {-1.0, 0.0, 0.0, 0.0, 0.0},
+                                     {-3.0, 1.0, 0.0, 0.0, 0.0},    {-10.0, 5.0, -1.0, 0.0, 0.0},
+                                     {-35.0, 21.0, -7.0, 1.0, 0.0}, {D10_1, D10_2, D10_3, D10_4, D10_5}};
+
+constexpr int kShockWallThreads = 256;
+
+// Shock filter j-increment of the rows next to no-slip j-walls, added to the
+// output of k_filter_shock<.., WALL> (which leaves rows 1..5 and ny-4..ny
+// without one): faces 1..4 from a wall use d2/d4/d6/d8 with amplitude
+// max(0, xnu + xk0), the wall face a tenth of the sensor term
+// (filter_o10_shock, mod_filtering_shock.f90:724-870 jmin, :915-1057 jmax);
+// the wall rows get none (:1128-1140 over ndy_d..nfy_d).  The pressure sensor
+// of row 1 (ny) reads p one row beyond the wall.  One thread per point of
+// rows 2..5 and ny-4..ny-1; nothing beyond a wall is read but p.
+__global__ __launch_bounds__(kShockWallThreads) void k_shock_wall(FilterArgs a) {
+  const Geom &g = a.g;
+  const long nface = (long)g.nx * g.nz;
+  const long t = (long)blockIdx.x * kShockWallThreads + threadIdx.x;
+  if (t >= 8 * nface) return;
+  const int q = (int)(t / nface);
+  const int side = q >> 2;
+  if (!((g.jwall >> side) & 1)) return;
+  const long r = t - q * nface;
+  const int i = 1 + (int)(r % g.nx), k = 1 + (int)(r / g.nx);
+  const int j = side ? g.ny - 4 + (q & 3) : 2 + (q & 3);
+  const idx_t sj = (idx_t)g.sx;
+  const idx_t pj = lin(g, i, j, k);
+  // pressure sensors of rows j-1, j, j+1 (faces j-1/2, j+1/2)
+  double ps[3];
+#pragma unroll
+  for (int l = 0; l < 3; l++) {
+    const idx_t pm = pj + (l - 1) * sj;
+    const double sn = a.sens ? ldg(a.sens, pm) : 1.0;
+    ps[l] = psens3(ldg(a.prs, pm - sj), ldg(a.prs, pm), ldg(a.prs, pm + sj), sn, a.csens);
+  }
+  double D[2][5];
+#pragma unroll
+  for (int f = 0; f < 2; f++) {
+    const int m = j - 1 + f;
+    const int hf = face_half(g, m);
+    double xk0 = a.xnu0 * fmax(ps[f + 1], ps[f]);
+    if (hf == 1) xk0 = xk0 / 10.0;
+    double xkh;
+    switch (hf) {
+      case 1: xkh = fmax(0.0, 0.0 + xk0); break;
+      case 2: xkh = fmax(0.0, a.chi / 12.0 + xk0); break;
+      case 3: xkh = fmax(0.0, a.chi / 60.0 + xk0); break;
+      case 4: xkh = fmax(0.0, a.chi / 280.0 + xk0); break;
+      default: xkh = fmax(0.0, a.xnu10 + xk0 / 105.0);
+    }
+    const idx_t pf = pj + (f - 1) * sj;  // row m
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      const double *u = a.uin.v[v];
+      const double d1 = ldg(u, pf + sj) - ldg(u, pf);
+      double dd = 0.0;
+      for (int l = 1; l <= hf; l++) dd = dd + kShockD[hf][l - 1] * (ldg(u, pf + l * sj) - ldg(u, pf + (1 - l) * sj));
+      D[f][v] = d1 * xk0 + dd * xkh;
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 5; v++) stg(a.uout.v[v], pj, ldg(a.uout.v[v], pj) + (D[0][v] - D[1][v]));
+}
+
 }  // namespace
+
+hipError_t launch_shock_wall(const FilterArgs &a, hipStream_t s) {
+  const long n = 8L * a.g.nx * a.g.nz;
+  hipLaunchKernelGGL(k_shock_wall, dim3((unsigned)((n + kShockWallThreads - 1) / kShockWallThreads)),
+                     dim3(kShockWallThreads), 0, s, a);
+  return hipGetLastError();
+}
 
 // Tiles of the filters: chunks of >= 16 planes, about 1024 workgroups per block.
 TileRange filter_tiles(const Geom &g, int kc_req) {
@@ -552,16 +640,19 @@ hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int nt, bool m
   return hipGetLastError();
 }
 
-hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int nt, bool mixed, bool artvisc,
+hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int nt, bool mixed, bool artvisc, bool wall,
                                hipStream_t s) {
   if (nt <= 0) return hipSuccess;
   if (artvisc) {
+    if (mixed || wall) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_filter_shock<false, true, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  } else if (wall) {
     if (mixed) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_filter_shock<false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    hipLaunchKernelGGL((k_filter_shock<false, false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   } else if (mixed) {
-    hipLaunchKernelGGL((k_filter_shock<true, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    hipLaunchKernelGGL((k_filter_shock<true, false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   } else {
-    hipLaunchKernelGGL((k_filter_shock<false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    hipLaunchKernelGGL((k_filter_shock<false, false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   }
   return hipGetLastError();
 }

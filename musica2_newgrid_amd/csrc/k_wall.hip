@@ -166,8 +166,27 @@ __global__ __launch_bounds__(kWallThreads) void k_visc_wall(WallViscArgs a) {
   }
   const idx_t p = lin(g, i, j, k);
   const Pw o = prim_at(a.u0, p, gam1, rg);
-  // the wall row stores tau/mu and -(cp/Pr) grad T (k_tau_wallrow completes it)
   const bool wrow = q == 0 || q == 3;
+  if (a.want_sensor) {
+    // the shock filter's Ducros sensor from these gradients; its pressure
+    // sensor on the wall row reads p one row beyond the wall, where the state
+    // is the ghost layer that nothing on the path writes
+    double s = 1.0;
+    if (a.ducros) {
+      const double dv = gr[0][0] + gr[1][1] + gr[2][2];
+      const double div2 = dv * dv;
+      const double o1 = gr[2][1] - gr[1][2];
+      const double o2 = gr[0][2] - gr[2][0];
+      const double o3 = gr[1][0] - gr[0][1];
+      s = div2 / (div2 + o1 * o1 + o2 * o2 + o3 * o3 + 1.0e-16);
+    }
+    stg(a.sens, p, s);
+    if (wrow) {
+      const idx_t pg = lin(g, i, q == 0 ? 0 : g.ny + 1, k);
+      stg(a.prs, pg, prim_at(a.u0, pg, gam1, rg).p);
+    }
+  }
+  // the wall row stores tau/mu and -(cp/Pr) grad T (k_tau_wallrow completes it)
   const double mu_l = wrow ? 1.0 : sutherland(a.gas, o.T);
   const double ck = a.gas.cok_ratio * mu_l;
   double t11 = gr[0][0], t22 = gr[1][1], t33 = gr[2][2];

@@ -159,9 +159,10 @@ std::string check_topology(const mgpu_case &c) {
           snprintf(buf, sizeof buf, "block %d face %d: walls are supported on the j-faces (2, 3) only", b, f);
           return buf;
         }
-        if (c.curvilinear || c.precision != MGPU_PREC_FP64 || c.filter >= MGPU_FILTER_SHOCK) {
+        if (c.curvilinear || c.precision != MGPU_PREC_FP64 || c.filter == MGPU_FILTER_ARTVISC) {
           snprintf(buf, sizeof buf,
-                   "block %d face %d: walls need a Cartesian FP64 case with the selective filter or none", b, f);
+                   "block %d face %d: walls need a Cartesian FP64 case with the selective or shock filter or none",
+                   b, f);
           return buf;
         }
         if (A.ny < 10) {
@@ -669,7 +670,7 @@ int enqueue_walls(mgpu_handle *h, int src, int dst) {
   return 0;
 }
 
-WallViscArgs wall_visc_args(const BlockData &bd, int s0) {
+WallViscArgs wall_visc_args(const mgpu_handle *h, const BlockData &bd, int s0) {
   WallViscArgs a{};
   a.g = bd.g;
   a.gas = gas_of(bd.p);
@@ -678,6 +679,10 @@ WallViscArgs wall_visc_args(const BlockData &bd, int s0) {
   for (int q = 0; q < NTAU; q++) a.tau[q] = bd.tau[q];
   for (int q = 0; q < 4; q++) a.kv[q] = bd.kv[q];
   a.adiabatic = bd.p.is_adiab;
+  a.prs = bd.prs;
+  a.sens = bd.sens;
+  a.want_sensor = bd.prs != nullptr;
+  a.ducros = h->c.is_ducros;
   return a;
 }
 
@@ -703,15 +708,15 @@ int enqueue_step(mgpu_handle *h, int s0) {
       HIPCK(h, launch_visc(h->d_visc + s0 * nb, nb, h->nt_visc, curv, h->stream));
       if (h->wall)
         for (const auto &bd : h->b)
-          if (bd.g.jwall) HIPCK(h, launch_visc_wall(wall_visc_args(bd, s0), h->stream));
+          if (bd.g.jwall) HIPCK(h, launch_visc_wall(wall_visc_args(h, bd, s0), h->stream));
       HIPCK(h, launch_iface(h->xv, h->nchunks_v, h->stream));
       if (h->wall)
         for (const auto &bd : h->b)
-          if (bd.g.jwall) HIPCK(h, launch_tau_wallrow(wall_visc_args(bd, s0), h->stream));
+          if (bd.g.jwall) HIPCK(h, launch_tau_wallrow(wall_visc_args(h, bd, s0), h->stream));
       HIPCK(h, launch_vdiv(h->d_vdiv, nb, h->nt_vdiv, curv, h->stream));
       if (h->wall)
         for (const auto &bd : h->b)
-          if (bd.g.jwall) HIPCK(h, launch_vdiv_wall(wall_visc_args(bd, s0), h->stream));
+          if (bd.g.jwall) HIPCK(h, launch_vdiv_wall(wall_visc_args(h, bd, s0), h->stream));
     }
     {
       ProfScope ps(h, MGPU_K_STAGE);
@@ -734,7 +739,6 @@ int enqueue_step(mgpu_handle *h, int s0) {
     }
     in = out;
   }
-  (void)in;
   if (c.filter != MGPU_FILTER_NONE) {
     {
       ProfScope ps(h, MGPU_K_FILTER);
@@ -742,7 +746,10 @@ int enqueue_step(mgpu_handle *h, int s0) {
         HIPCK(h, launch_filter_sf(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, h->wall, h->stream));
       else
         HIPCK(h, launch_filter_shock(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed,
-                                     c.filter == MGPU_FILTER_ARTVISC, h->stream));
+                                     c.filter == MGPU_FILTER_ARTVISC, h->wall, h->stream));
+      if (h->wall && c.filter == MGPU_FILTER_SHOCK)
+        for (const auto &bd : h->b)
+          if (bd.g.jwall) HIPCK(h, launch_shock_wall(filter_args(h, bd, in, s0), h->stream));
     }
     if ((rc = exchange_state(h, s0))) return rc;
     if (h->wall) {
@@ -1142,7 +1149,7 @@ int tune_aux(mgpu_handle *h, bool force, hipEvent_t e0, hipEvent_t e1) {
         if (h->c.filter == MGPU_FILTER_SF)
           return launch_filter_sf(h->d_tune_f, nb, h->nt_filt, h->mixed, h->wall, h->stream);
         return launch_filter_shock(h->d_tune_f, nb, h->nt_filt, h->mixed, h->c.filter == MGPU_FILTER_ARTVISC,
-                                   h->stream);
+                                   h->wall, h->stream);
       };
       hipError_t e = launch();
       if (e == hipSuccess) e = hipEventRecord(e0, h->stream);

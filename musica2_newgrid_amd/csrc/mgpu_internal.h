@@ -271,7 +271,10 @@ struct WallViscArgs {
   Metrics met;
   double *tau[NTAU];
   double *kv[4];
-  int adiabatic;  // no wall-normal temperature gradient on the wall row
+  double *prs, *sens;  // shock filter: p and Ducros sensor of U^n
+  int adiabatic;       // no wall-normal temperature gradient on the wall row
+  int want_sensor;     // write the sensor on rows 1,2 (ny-1,ny) and p beyond the walls
+  int ducros;          // Ducros sensor (else 1)
 };
 
 // Component c of tau in the flux along direction d of a Cartesian block (row
@@ -334,9 +337,13 @@ TileRange filter_tiles(const Geom &g, int kchunk);
 hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, bool wall,
                             hipStream_t s);
 // The shock filter, or (artvisc) the artificial viscosity, which shares its
-// face-dissipation structure.
+// face-dissipation structure; `wall`: the shock filter with the closures of
+// blocks with j-walls (FP64).
 hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, bool artvisc,
-                               hipStream_t s);
+                               bool wall, hipStream_t s);
+// The shock filter's j-increment of the rows next to the j-walls of one
+// block, after launch_filter_shock(.., wall) (FP64).
+hipError_t launch_shock_wall(const FilterArgs &a, hipStream_t s);
 hipError_t launch_iface(const IfaceArgs &a, int nchunks, hipStream_t s);
 hipError_t launch_iface(const IfaceArgs32 &a, int nchunks, hipStream_t s);
 long diag_groups(const Geom &g);

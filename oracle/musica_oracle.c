@@ -63,7 +63,6 @@ static const double A5[3] = {0.0, 2.0 / 3.0, -1.0 / 12.0};
 static const double D11[6] = {252.0 / 1024.0, -210.0 / 1024.0, 120.0 / 1024.0,
                               -45.0 / 1024.0, 10.0 / 1024.0, -1.0 / 1024.0};
 /* 9th-order conservative dissipation, mod_filtering_shock.f90:95-100. */
-static const double D10[6] = {0.0, -126.0, 84.0, -36.0, 9.0, -1.0};
 /* Centred closures next to a wall by half-width q = 1..4: a3, a5, a7, a9
  * (mod_coeff_deriv.f90:102-104, 109-113, 125-131, 232-241; stencil 11 keeps
  * the standard a7/a9). */
@@ -527,16 +526,30 @@ static void filter_sf(const orc_cfg *c, const dims_t *d, double *const Un[5], do
 }
 
 /* ------------------------------------------------------------------------ */
-/* Shock filter (filter_o10_shock, mod_filtering_shock.f90:204-1512), for    */
-/* periodic faces: Ducros sensor on 0..n+1 (:223-239), pressure sensor per   */
-/* direction (:243-260), face dissipation D_{i+1/2} on 0..n (:419-440) and   */
-/* K_i = D_{i-1/2} - D_{i+1/2} accumulated over directions (:690-700).       */
+/* Shock filter (filter_o10_shock, mod_filtering_shock.f90:204-1512):       */
+/* Ducros sensor (:223-239), pressure sensor per direction (:243-260), face  */
+/* dissipation D_{i+1/2} on 0..n (:419-440) and K_i = D_{i-1/2} - D_{i+1/2}  */
+/* accumulated over directions (:690-700).  Along an axis with no-slip walls */
+/* the sensor runs over 1..n (row 1 reads p one row beyond the wall, i.e.    */
+/* the pressure of the ghost state, which nothing on the path changes), the  */
+/* faces over 1..n-1 and K over 2..n-1.                                      */
 /* ------------------------------------------------------------------------ */
 static void filter_shock(const orc_cfg *c, const dims_t *d, double *const Un[5], const work_t *w,
                          double *K[5]) {
-  const double xnu10 = c->chi / 1260.0, xnu0 = -c->dissip_shock, cs = c->csens;
+  const double xnu0 = -c->dissip_shock, cs = c->csens;
+  /* face amplitudes and differences by half-width hf = 1..5: d2..d10 with
+   * xnu2 = 0, xnu4 = chi/12, xnu6 = chi/60, xnu8 = chi/280, xnu10 = chi/1260
+   * (init_filter_shock with iSF = 0, mod_filtering_shock.f90:75-107); next to
+   * a wall the faces 1..4 use d2..d8, face 1 a tenth of the sensor term and
+   * xk = max(0, xnu + xk0) without the 1/105 of d10 (filter_o10_shock
+   * :724-1057 j, :266-466 i), and the wall rows get no increment along that
+   * axis (:1128-1140 over ndy_d..nfy_d) */
+  const double xnu[6] = {0.0, 0.0, c->chi / 12.0, c->chi / 60.0, c->chi / 280.0, c->chi / 1260.0};
+  static const double DF[6][6] = {{0}, {0, -1.0}, {0, -3.0, 1.0}, {0, -10.0, 5.0, -1.0},
+                                  {0, -35.0, 21.0, -7.0, 1.0}, {0, -126.0, 84.0, -36.0, 9.0, -1.0}};
+  const walls_t W = walls_of(c);
   double *sens = alloc_arr(d);
-  /* Ducros sensor on 0..n+1 in all directions */
+  /* Ducros sensor on 0..n+1 (1..n along a wall axis: ndx_d1..nfx_d1) */
 #pragma omp parallel for collapse(2)
   for (int k = 0; k <= d->nz + 1; k++)
     for (int j = 0; j <= d->ny + 1; j++)
@@ -560,6 +573,7 @@ static void filter_shock(const orc_cfg *c, const dims_t *d, double *const Un[5],
   for (int dir = 0; dir < 3; dir++) {
     const long s = st[dir];
     const int n = nn[dir];
+    const int lo = W.lo[dir], hi = W.hi[dir];
     int o1 = dir == 0 ? 1 : 0, o2 = dir == 2 ? 1 : 2;
     int n1 = nn[o1], n2 = nn[o2];
 #pragma omp parallel
@@ -572,31 +586,39 @@ static void filter_shock(const orc_cfg *c, const dims_t *d, double *const Un[5],
           int idx3[3];
           idx3[o1] = a; idx3[o2] = b; idx3[dir] = 0;
           long p0 = (long)ID(*d, idx3[0], idx3[1], idx3[2]); /* offset of index 0 */
-          for (int i = 0; i <= n + 1; i++) {
+          /* pressure sensor on 0..n+1 (1..n next to walls; row 1 reads p one row beyond
+           * the wall, the pressure of the ghost state, which nothing changes) */
+          for (int i = lo ? 1 : 0; i <= (hi ? n : n + 1); i++) {
             long p = p0 + i * s;
             double ptrad = prs[p - s] + 2.0 * prs[p] + prs[p + s];
             double ptvd = fabs(prs[p + s] - prs[p]) + fabs(prs[p] - prs[p - s]);
             ps[i] = sens[p] * fabs(prs[p - s] - 2.0 * prs[p] + prs[p + s]) / (cs * ptrad + (1.0 - cs) * ptvd);
           }
-          for (int i = 0; i <= n; i++) {
-            long p = p0 + i * s;
-            double xk0 = xnu0 * fmax(ps[i + 1], ps[i]);
-            double xk10 = fmax(0.0, xnu10 + xk0 / 105.0);
+          /* face dissipation D_{m+1/2} on 0..n (1..n-1 next to walls) */
+          for (int m = lo ? 1 : 0; m <= (hi ? n - 1 : n); m++) {
+            int hf = 5;
+            if (lo && m < hf) hf = m;
+            if (hi && n - m < hf) hf = n - m;
+            long p = p0 + m * s;
+            double xk0 = xnu0 * fmax(ps[m + 1], ps[m]);
+            if (hf == 1) xk0 = xk0 / 10.0;
+            const double xkh = fmax(0.0, xnu[hf] + (hf == 5 ? xk0 / 105.0 : xk0));
             for (int v = 0; v < 5; v++) {
               const double *u = Un[v];
-              D[v * (n + 1) + i] = (u[p + s] - u[p]) * xk0 +
-                                   (D10[1] * (u[p + s] - u[p]) + D10[2] * (u[p + 2 * s] - u[p - s]) +
-                                    D10[3] * (u[p + 3 * s] - u[p - 2 * s]) + D10[4] * (u[p + 4 * s] - u[p - 3 * s]) +
-                                    D10[5] * (u[p + 5 * s] - u[p - 4 * s])) *
-                                       xk10;
+              double dd = 0.0;
+              for (int l = 1; l <= hf; l++) dd = dd + DF[hf][l] * (u[p + l * s] - u[p + (1 - l) * s]);
+              D[v * (n + 1) + m] = (u[p + s] - u[p]) * xk0 + dd * xkh;
             }
           }
+          /* K_i = D_{i-1/2} - D_{i+1/2} on 1..n (2..n-1 next to walls) */
           for (int i = 1; i <= n; i++) {
             long p = p0 + i * s;
+            const int edge = (lo && i == 1) || (hi && i == n);
             for (int v = 0; v < 5; v++) {
               const double *Dv = D + v * (n + 1);
-              if (dir == 0) K[v][p] = Dv[i - 1] - Dv[i];
-              else K[v][p] = K[v][p] + Dv[i - 1] - Dv[i];
+              const double kd = edge ? 0.0 : Dv[i - 1] - Dv[i];
+              if (dir == 0) K[v][p] = kd;
+              else K[v][p] = K[v][p] + kd;
             }
           }
         }
@@ -891,7 +913,7 @@ int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_gri
       if (bl[b].wall[f]) {
         const int ext = f < 2 ? bl[b].nx : (f < 4 ? bl[b].ny : bl[b].nz);
         if (c0->curv) return -5;        /* walls: Cartesian blocks only */
-        if (c0->filter >= 2) return -6; /* walls: selective filter or none */
+        if (c0->filter == 3) return -6; /* walls: selective or shock filter, or none */
         if (ext < 10) return -7;        /* closures of the two sides must not overlap */
       }
   orc_cfg cb[64];

@@ -2,7 +2,9 @@
 with walls on both j-faces (stretched y-grid, periodic x and z), FP64 within
 1e-12 relative.  The cases cover the viscous last stage with the selective
 filter, Euler only without filter (the end-of-step wall state then comes from
-the last stage's input), isothermal walls, ghost layers beyond the walls
+the last stage's input), the shock filter with and without the Ducros sensor
+(reduced faces next to the walls, the wall row's pressure sensor reading p one
+row beyond the wall), isothermal walls, ghost layers beyond the walls
 filled with NaN (never read), the channel cut into two blocks (the wall-row
 face ghosts keep the neighbour's value from before the wall state, as in the
 reference), graph replays against eager launches and the autotuner's tiles."""
@@ -39,12 +41,14 @@ def _interior_relerr(A, B):
     return relerr(B, A)
 
 
-@pytest.mark.parametrize("visc,filt,nrk", [(4, 1, 4), (0, 0, 4), (4, 0, 4), (0, 1, 6)])
-def test_channel_matches_oracle(visc, filt, nrk):
+@pytest.mark.parametrize("visc,filt,nrk,ducros", [(4, 1, 4, False), (0, 0, 4, False), (4, 0, 4, False),
+                                                   (0, 1, 6, False), (4, 2, 4, False), (4, 2, 4, True)])
+def test_channel_matches_oracle(visc, filt, nrk, ducros):
     c = cases.channel(48, 40, 16, filter=filt, iorder_visc=visc)
+    c.ducros = ducros
     if nrk == 6:
         c.nrk, c.crk = 6, cases.CRK6_DRP
-    print(f"channel 48x40x16 visc={visc} filter={filt} nrk={nrk}, 20 steps", flush=True)
+    print(f"channel 48x40x16 visc={visc} filter={filt} nrk={nrk} ducros={ducros}, 20 steps", flush=True)
     Ug = _gpu(c, 20)
     Uc = _oracle(c, 20)
     e = _interior_relerr(Uc, Ug)
@@ -75,12 +79,14 @@ def test_nan_beyond_walls_is_never_read():
         assert np.array_equal(gr.interior(a), gr.interior(b))
 
 
-def test_channel_cut_in_two_blocks_matches_oracle():
+@pytest.mark.parametrize("filt", [1, 2])
+def test_channel_cut_in_two_blocks_matches_oracle(filt):
     from musica2_newgrid_amd.solver import GpuSolver
     from oracle import oracle as orc
-    c = cases.channel(64, 20, 8)
+    c = cases.channel(64, 20, 8, filter=filt)
+    c.ducros = filt == 2
     blocks, bcs = split_x_blocks(c)
-    print("channel cut into 2 blocks, 64x20x8, 10 steps", flush=True)
+    print(f"channel cut into 2 blocks, 64x20x8, filter={filt}, 10 steps", flush=True)
     with GpuSolver(c, topology=blocks, block_cases=bcs) as s:
         s.step(10)
         Ug = s.get_states()
@@ -91,9 +97,10 @@ def test_channel_cut_in_two_blocks_matches_oracle():
     assert e < 1e-12
 
 
-def test_walls_graph_matches_eager_and_tuned():
-    c = cases.channel(48, 40, 16)
-    print("channel graph vs eager vs tuned, 3 x 3 steps", flush=True)
+@pytest.mark.parametrize("filt", [1, 2])
+def test_walls_graph_matches_eager_and_tuned(filt):
+    c = cases.channel(48, 40, 16, filter=filt)
+    print(f"channel filter={filt}: graph vs eager vs tuned, 3 x 3 steps", flush=True)
     from musica2_newgrid_amd.solver import GpuSolver
     with GpuSolver(c, eager=True) as se, GpuSolver(c) as sg:
         for rep in range(3):

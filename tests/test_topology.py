@@ -97,8 +97,9 @@ def _create_case(c, blocks, precision=_lib.PREC_FP64):
 
 
 def test_wall_limits_are_checked():
-    """Walls: j-faces only, Cartesian FP64 with the selective filter or none,
-    ny >= 10; refused before any device work."""
+    """Walls: j-faces only, Cartesian FP64 with the selective or shock filter
+    or none (not the artificial viscosity), ny >= 10; refused before any
+    device work."""
     c, blocks = _channel_table()
     blocks[0]["kind"][0] = blocks[0]["kind"][1] = _lib.FACE_WALL
     rc, msg = _create_case(c, blocks)
@@ -106,9 +107,9 @@ def test_wall_limits_are_checked():
     c, blocks = _channel_table()
     rc, msg = _create_case(c, blocks, precision=_lib.PREC_MIXED)
     assert rc == _lib.EINVAL and "FP64" in msg
-    c, blocks = _channel_table(filter=2)
+    c, blocks = _channel_table(filter=3)
     rc, msg = _create_case(c, blocks)
-    assert rc == _lib.EINVAL and "selective filter" in msg
+    assert rc == _lib.EINVAL and "selective or shock filter" in msg
     c, blocks = _channel_table()
     c.curvilinear = True
     rc, msg = _create_case(c, blocks)

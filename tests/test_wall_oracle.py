@@ -247,7 +247,8 @@ def split_x_blocks(c):
     blocks, bcs = [], []
     for b in range(2):
         sub = cases.Case(f"half{b}", h, c.ny, c.nz, curvilinear=False, iorder_visc=c.iorder_visc,
-                         filter=c.filter, ducros=False, diffscale=c.diffscale, chi=c.chi, dt=c.dt,
+                         filter=c.filter, ducros=c.ducros, diffscale=c.diffscale, chi=c.chi, dt=c.dt,
+                         dissip_shock=c.dissip_shock, csens=c.csens,
                          cart=[c.cart[0][:h + 2 * G].copy(), c.cart[1], c.cart[2],
                                c.cart[3][:h + 2 * G].copy(), c.cart[4], c.cart[5]],
                          faces=(1, 1, 2, 2, 1, 1), adiabatic=c.adiabatic, t_wall=c.t_wall,
