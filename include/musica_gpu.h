@@ -114,7 +114,10 @@ typedef struct mgpu_case {
   double dissip_shock;    /* shock-capturing amplitude                         */
   double csens;           /* pressure-sensor blend                             */
   int32_t is_adiab;       /* walls: 1 adiabatic (dT/dn = 0), 0 isothermal (is_adiab) */
-  int32_t reserved;       /* 0                                                 */
+  int32_t is_sw_edoh;     /* shock filter: Edoh et al. switch, each direction's  */
+                          /* increment times min(1, CFL_d/chi) with CFL_d the   */
+                          /* local (|u_d|+c) dt / dx_d of U^n (contravariant on */
+                          /* c3 blocks) (mod_filtering_shock.f90:620-700)       */
   double t_wall;          /* walls: temperature of an isothermal wall (T_wall) */
 } mgpu_case;
 

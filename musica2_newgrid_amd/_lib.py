@@ -82,7 +82,7 @@ class MgpuCase(C.Structure):
         ("dissip_shock", C.c_double),
         ("csens", C.c_double),
         ("is_adiab", C.c_int32),
-        ("reserved", C.c_int32),
+        ("is_sw_edoh", C.c_int32),
         ("t_wall", C.c_double),
     ]
 

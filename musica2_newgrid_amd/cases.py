@@ -67,6 +67,7 @@ class Case:
     iorder_visc: int = 4
     filter: int = 1            # 0 none, 1 selective filter, 2 shock filter
     ducros: bool = True
+    sw_edoh: bool = False      # Edoh switch of the shock filter (is_sw_edoh)
     nrk: int = 4
     crk: tuple = CRK4
     dt: float = 0.0

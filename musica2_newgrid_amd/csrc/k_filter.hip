@@ -261,15 +261,29 @@ __device__ __forceinline__ int face_half(const Geom &g, int m) {
   return h;
 }
 
+// Edoh switch factor of direction d at element p (position r along d):
+// min(1, CFL/chi) with CFL = sr_d deltat, times idx_d(r) on Cartesian blocks
+// (mod_filtering_shock.f90:668-676 and the j, k repeats; c3 :627-632).
+__device__ __forceinline__ double edoh_mul(const FilterArgs &a, int d, idx_t p, int r) {
+  const double *met = d == 0 ? a.idx : (d == 1 ? a.idy : a.idz);
+  double cfl = ldg(a.sr[d], p) * a.dt;
+  if (met) cfl = cfl * ldg(met, (idx_t)(r - 1 + G));
+  return fmin(1.0, cfl / a.chi);
+}
+
 // ARTV: the artificial viscosity.  The LDS "pressure" pencils then hold the
 // spectral radii |u|+c (i-pencil) and |v|+c (j-pencil), the k queue |w|+c.
 // WALL: blocks with j-walls; rows 1..5 and ny-4..ny get no j-increment here
 // (k_shock_wall adds it with the reduced faces of the wall closure).
-template <bool MIX, bool ARTV, bool WALL>
+// EDOH: the shock filter's Edoh switch, each direction's increment times
+// min(1, CFL_d / chi) with the local CFL_d = sr_d deltat (idx_d on Cartesian
+// blocks) (mod_filtering_shock.f90:620-700, :1059-1140, :1450-1496).
+template <bool MIX, bool ARTV, bool WALL, bool EDOH>
 __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> *__restrict__ tab, int nb,
                                                         int ntiles) {
   static_assert(!(MIX && ARTV), "artificial viscosity: FP64 only");
   static_assert(!(WALL && (MIX || ARTV)), "walls: FP64 shock filter only");
+  static_assert(!(EDOH && ARTV), "the Edoh switch belongs to the shock filter");
   using R = typename std::conditional<MIX, float, double>::type;
   __shared__ R sux[5][FY][FX + 10];
   __shared__ R suy[5][FY + 10][FX];
@@ -279,6 +293,7 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
   __shared__ double ssy[FY + 4][FX];    // sensor at j0-2 .. j0+FY+1 (ends unused)
   __shared__ double psx[FY][FX + 2];    // psens_x at i0-1 .. i0+FX
   __shared__ double psy[FY + 2][FX];    // psens_y at j0-1 .. j0+FY
+  __shared__ double smul[EDOH ? 3 : 1][FY][FX];  // Edoh factors of the plane's own points
 
   int i0, j0, kbeg, kend;
   const FilterArgs &a = tile_of(tab, nb, ntiles, i0, j0, kbeg, kend);
@@ -415,6 +430,15 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
         }
       }
     }
+    if constexpr (EDOH) {
+      // computed here and kept in LDS, so that nothing of them stays in
+      // registers through the stencils
+      if (own) {
+        smul[0][ty][tx] = edoh_mul(a, 0, pk, i);
+        smul[1][ty][tx] = edoh_mul(a, 1, pk, j);
+        smul[2][ty][tx] = edoh_mul(a, 2, pk, k);
+      }
+    }
     __syncthreads();
     if (k < kend) load_plane(k + 1);
     // pressure sensors of the plane
@@ -478,10 +502,12 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
         const double pm = psx[ty][tx], p0 = psx[ty][tx + 1], pp = psx[ty][tx + 2];
         const double xk0m = a.xnu0 * fmax(p0, pm), xk10m = fmax(0.0, a.xnu10 + xk0m / 105.0);
         const double xk0p = a.xnu0 * fmax(pp, p0), xk10p = fmax(0.0, a.xnu10 + xk0p / 105.0);
+        const R mul = EDOH ? (R)smul[0][ty][tx] : R(1);
 #pragma unroll
         for (int v = 0; v < 5; v++) {
           const R *x = &sux[v][ty][tx + 5];
-          K[v] = dface<R>(x - 1, 1, (R)xk0m, (R)xk10m) - dface<R>(x, 1, (R)xk0p, (R)xk10p);
+          if constexpr (EDOH) K[v] = mul * (dface<R>(x - 1, 1, (R)xk0m, (R)xk10m) - dface<R>(x, 1, (R)xk0p, (R)xk10p));
+          else K[v] = dface<R>(x - 1, 1, (R)xk0m, (R)xk10m) - dface<R>(x, 1, (R)xk0p, (R)xk10p);
         }
       }
       // j faces (walls: rows 1..5 and ny-4..ny take theirs from k_shock_wall)
@@ -489,10 +515,13 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
         const double pm = psy[ty][tx], p0 = psy[ty + 1][tx], pp = psy[ty + 2][tx];
         const double xk0m = a.xnu0 * fmax(p0, pm), xk10m = fmax(0.0, a.xnu10 + xk0m / 105.0);
         const double xk0p = a.xnu0 * fmax(pp, p0), xk10p = fmax(0.0, a.xnu10 + xk0p / 105.0);
+        const R mul = EDOH ? (R)smul[1][ty][tx] : R(1);
 #pragma unroll
         for (int v = 0; v < 5; v++) {
           const R *y = &suy[v][ty + 5][tx];
-          K[v] = K[v] + dface<R>(y - FX, FX, (R)xk0m, (R)xk10m) - dface<R>(y, FX, (R)xk0p, (R)xk10p);
+          if constexpr (EDOH)
+            K[v] = K[v] + mul * (dface<R>(y - FX, FX, (R)xk0m, (R)xk10m) - dface<R>(y, FX, (R)xk0p, (R)xk10p));
+          else K[v] = K[v] + dface<R>(y - FX, FX, (R)xk0m, (R)xk10m) - dface<R>(y, FX, (R)xk0p, (R)xk10p);
         }
       }
       // k faces from the register queues
@@ -502,6 +531,7 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
         const double pp = psens3(qp[2], qp[3], qp[4], qs[2], cs);
         const double xk0m = a.xnu0 * fmax(p0, pm), xk10m = fmax(0.0, a.xnu10 + xk0m / 105.0);
         const double xk0p = a.xnu0 * fmax(pp, p0), xk10p = fmax(0.0, a.xnu10 + xk0p / 105.0);
+        const R mul = EDOH ? (R)smul[2][ty][tx] : R(1);
 #pragma unroll
         for (int v = 0; v < 5; v++) {
           const R dlo = (q[5][v] - q[4][v]) * (R)xk0m +
@@ -512,7 +542,8 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
                         (R(D10_1) * (q[6][v] - q[5][v]) + R(D10_2) * (q[7][v] - q[4][v]) +
                          R(D10_3) * (q[8][v] - q[3][v]) + R(D10_4) * (q[9][v] - q[2][v]) +
                          R(D10_5) * (q[10][v] - q[1][v])) * (R)xk10p;
-          K[v] = K[v] + dlo - dhi;
+          if constexpr (EDOH) K[v] = K[v] + mul * (dlo - dhi);
+          else K[v] = K[v] + dlo - dhi;
         }
       }
 #pragma unroll
@@ -600,8 +631,12 @@ __global__ __launch_bounds__(kShockWallThreads) void k_shock_wall(FilterArgs a) 
       D[f][v] = d1 * xk0 + dd * xkh;
     }
   }
+  const double mul = a.edoh ? edoh_mul(a, 1, pj, j) : 1.0;
 #pragma unroll
-  for (int v = 0; v < 5; v++) stg(a.uout.v[v], pj, ldg(a.uout.v[v], pj) + (D[0][v] - D[1][v]));
+  for (int v = 0; v < 5; v++) {
+    const double ky = a.edoh ? mul * (D[0][v] - D[1][v]) : D[0][v] - D[1][v];
+    stg(a.uout.v[v], pj, ldg(a.uout.v[v], pj) + ky);
+  }
 }
 
 }  // namespace
@@ -640,19 +675,26 @@ hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int nt, bool m
   return hipGetLastError();
 }
 
+template <bool EDOH>
+static void launch_shock_variant(const Batch<FilterArgs> *tab, int nb, int nt, bool mixed, bool wall, hipStream_t s) {
+  if (wall)
+    hipLaunchKernelGGL((k_filter_shock<false, false, true, EDOH>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  else if (mixed)
+    hipLaunchKernelGGL((k_filter_shock<true, false, false, EDOH>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  else
+    hipLaunchKernelGGL((k_filter_shock<false, false, false, EDOH>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+}
+
 hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int nt, bool mixed, bool artvisc, bool wall,
-                               hipStream_t s) {
+                               bool edoh, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
   if (artvisc) {
-    if (mixed || wall) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_filter_shock<false, true, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
-  } else if (wall) {
-    if (mixed) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_filter_shock<false, false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
-  } else if (mixed) {
-    hipLaunchKernelGGL((k_filter_shock<true, false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    if (mixed || wall || edoh) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_filter_shock<false, true, false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   } else {
-    hipLaunchKernelGGL((k_filter_shock<false, false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    if (mixed && wall) return hipErrorInvalidValue;
+    if (edoh) launch_shock_variant<true>(tab, nb, nt, mixed, wall, s);
+    else launch_shock_variant<false>(tab, nb, nt, mixed, wall, s);
   }
   return hipGetLastError();
 }

@@ -8,8 +8,10 @@
 //                                        smod_grad_T_c3.f90:200-213
 //   flux_visc_5pts(_c3) stress and heat flux  smod_flux_visc_5pts.f90:27-67;
 //                                             smod_flux_visc_5pts_c3.f90:27-75
-// and, for the artificial viscosity, the spectral radii |u_d| + c of U^n
-// (c_ of primitives_visc, smod_primitive_pfg.f90:121; mod_artvisc.f90:278-283).
+// and, for the artificial viscosity and the shock filter's Edoh switch, the
+// spectral radii |u_d| + c of U^n (c_ of primitives_visc,
+// smod_primitive_pfg.f90:121; mod_artvisc.f90:278-283), contravariant on c3
+// blocks (mod_filtering_shock.f90:627-630).
 // It stores the tensor in the physical frame (tau, see mgpu_internal.h), so
 // that the 2-layer interface exchange of tau that follows stands for the
 // reference's exchange of the gradients (communication_3dv,
@@ -263,11 +265,27 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(const Batch<ViscArgs> *__res
         stg(sensp, p, s);
       }
       if (a.want_sr) {
-        // spectral radii of the artificial viscosity, c = sqrt(gam R T) (primitives_visc)
+        // spectral radii along the three directions, c = sqrt(gam R T)
+        // (primitives_visc); on c3 blocks with the contravariant velocity
+        // (u xi_x + v xi_y + w xi_z) ijacob3 and c |grad xi| |ijacob3| of the
+        // Euler metrics (the Edoh switch, mod_filtering_shock.f90:627-630;
+        // g3_ksi, smod_grid_metrics3_c3.f90:161)
         const double c = sqrt(a.gas.gam * rg * T);
-        stg(sr0, p, fabs(u) + c);
-        stg(sr1, p, fabs(v) + c);
-        stg(sr2, p, fabs(w) + c);
+        if constexpr (CURV) {
+          const double ij = ldg(a.met.m[9], p);
+          double *const srp[3] = {sr0, sr1, sr2};
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            const double mx = ldg(a.met.m[3 * d], p), my = ldg(a.met.m[3 * d + 1], p), mz = ldg(a.met.m[3 * d + 2], p);
+            const double vc = (u * mx + v * my + w * mz) * ij;
+            const double g3 = sqrt(mx * mx + my * my + mz * mz) * fabs(ij);
+            stg(srp[d], p, fabs(vc) + c * g3);
+          }
+        } else {
+          stg(sr0, p, fabs(u) + c);
+          stg(sr1, p, fabs(v) + c);
+          stg(sr2, p, fabs(w) + c);
+        }
       }
     }
     __syncthreads();

@@ -98,6 +98,9 @@ Gas gas_of(const mgpu_case &c) {
   return gs;
 }
 
+// The shock filter's Edoh switch is on (is_sw_edoh acts in filter_o10_shock only).
+bool edoh_on(const mgpu_case &c) { return c.filter == MGPU_FILTER_SHOCK && c.is_sw_edoh != 0; }
+
 Metrics metrics_of(const BlockData &bd) {
   Metrics m{};
   m.idx = bd.idx[0];
@@ -555,6 +558,7 @@ FilterArgs filter_args(const mgpu_handle *h, const BlockData &bd, int in, int ou
   fa.idy = bd.idx[1];
   fa.idz = bd.idx[2];
   fa.dt = bd.p.deltat;
+  fa.edoh = edoh_on(h->c);
   return fa;
 }
 
@@ -620,7 +624,7 @@ int build_tables(mgpu_handle *h) {
       va.sens = bd.sens;
       for (int q = 0; q < 3; q++) va.sr[q] = bd.sr[q];
       va.want_sensor = c.filter == MGPU_FILTER_SHOCK;
-      va.want_sr = c.filter == MGPU_FILTER_ARTVISC;
+      va.want_sr = c.filter == MGPU_FILTER_ARTVISC || edoh_on(c);
       va.ducros = c.is_ducros;
       vi[s0 * nb + b].t = tv;
       fi[s0 * nb + b].a = filter_args(h, bd, in, s0);  // reads the last stage's output
@@ -746,7 +750,7 @@ int enqueue_step(mgpu_handle *h, int s0) {
         HIPCK(h, launch_filter_sf(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, h->wall, h->stream));
       else
         HIPCK(h, launch_filter_shock(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed,
-                                     c.filter == MGPU_FILTER_ARTVISC, h->wall, h->stream));
+                                     c.filter == MGPU_FILTER_ARTVISC, h->wall, edoh_on(c), h->stream));
       if (h->wall && c.filter == MGPU_FILTER_SHOCK)
         for (const auto &bd : h->b)
           if (bd.g.jwall) HIPCK(h, launch_shock_wall(filter_args(h, bd, in, s0), h->stream));
@@ -843,7 +847,8 @@ size_t block_bytes(const mgpu_case &c, int nx, int ny, int nz) {
   size_t arrays = 15;  // three state sets
   if (c.curvilinear) arrays += c.iorder_visc ? 20 : 10;
   if (c.iorder_visc)
-    arrays += NTAU + 4 + (c.filter == MGPU_FILTER_SHOCK ? 2 : 0) + (c.filter == MGPU_FILTER_ARTVISC ? 3 : 0);
+    arrays += NTAU + 4 + (c.filter == MGPU_FILTER_SHOCK ? 2 : 0) +
+              (c.filter == MGPU_FILTER_ARTVISC || edoh_on(c) ? 3 : 0);
   size_t bytes = arrays * n * sizeof(double);
   if (c.precision == MGPU_PREC_MIXED) bytes += 10 * n * sizeof(float);
   if (!c.curvilinear) bytes += 6 * (size_t)(nx + ny + nz + 6 * G) * sizeof(double);
@@ -893,7 +898,7 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
       if (!rc) rc = dalloc(h, &bd.prs, n);
       if (!rc) rc = dalloc(h, &bd.sens, n);
     }
-    if (c.filter == MGPU_FILTER_ARTVISC)
+    if (c.filter == MGPU_FILTER_ARTVISC || edoh_on(c))
       for (int q = 0; q < 3 && !rc; q++) rc = dalloc(h, &bd.sr[q], n);
   }
   if (rc) {
@@ -1149,7 +1154,7 @@ int tune_aux(mgpu_handle *h, bool force, hipEvent_t e0, hipEvent_t e1) {
         if (h->c.filter == MGPU_FILTER_SF)
           return launch_filter_sf(h->d_tune_f, nb, h->nt_filt, h->mixed, h->wall, h->stream);
         return launch_filter_shock(h->d_tune_f, nb, h->nt_filt, h->mixed, h->c.filter == MGPU_FILTER_ARTVISC,
-                                   h->wall, h->stream);
+                                   h->wall, edoh_on(h->c), h->stream);
       };
       hipError_t e = launch();
       if (e == hipSuccess) e = hipEventRecord(e0, h->stream);

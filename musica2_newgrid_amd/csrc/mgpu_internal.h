@@ -214,7 +214,8 @@ struct ViscArgs {
   double *tau[NTAU];  // out: interior points
   double *prs;        // out: p(U^n), interior points (shock filter)
   double *sens;       // out: Ducros sensor, interior points (shock filter)
-  double *sr[3];      // out: |u|+c, |v|+c, |w|+c of U^n, interior points (artificial viscosity)
+  double *sr[3];      // out: |u|+c, |v|+c, |w|+c of U^n, interior points (artificial viscosity,
+                      // Edoh switch); c3: |contravariant velocity| + c |grad xi_d| per direction
   int want_sensor;    // 1: write prs and sens
   int want_sr;        // 1: write sr
   int ducros;
@@ -239,9 +240,10 @@ struct FilterArgs {
   double dd[6];        // -chi*d11 (selective filter)
   double chi;          // filter amplitude (the 5/7/9-pt filters next to walls)
   double xnu0, xnu10, csens;
-  const double *sr[3];  // artificial viscosity: |u|+c, |v|+c, |w|+c of U^n
-  const double *idx, *idy, *idz;  // artificial viscosity: Cartesian metrics
-  double dt;            // artificial viscosity: U += deltat*K
+  const double *sr[3];  // artificial viscosity, Edoh switch: spectral radii of U^n (see ViscArgs)
+  const double *idx, *idy, *idz;  // artificial viscosity, Edoh switch: Cartesian metrics (null on c3)
+  double dt;            // artificial viscosity (U += deltat*K), Edoh switch (local CFL)
+  int edoh;             // shock filter: Edoh switch
   RefState ref;        // mixed precision only
 };
 
@@ -338,9 +340,9 @@ hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int ntiles, bo
                             hipStream_t s);
 // The shock filter, or (artvisc) the artificial viscosity, which shares its
 // face-dissipation structure; `wall`: the shock filter with the closures of
-// blocks with j-walls (FP64).
+// blocks with j-walls (FP64); `edoh`: the shock filter's Edoh switch.
 hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, bool artvisc,
-                               bool wall, hipStream_t s);
+                               bool wall, bool edoh, hipStream_t s);
 // The shock filter's j-increment of the rows next to the j-walls of one
 // block, after launch_filter_shock(.., wall) (FP64).
 hipError_t launch_shock_wall(const FilterArgs &a, hipStream_t s);

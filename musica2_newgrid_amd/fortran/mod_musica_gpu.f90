@@ -30,7 +30,7 @@ module mod_musica_gpu
      type(c_ptr) :: blocks
      real(c_double) :: crk(8), deltat, gam, rg, c0, S0, diffscale, cok_ratio
      real(c_double) :: dissip_coeff, dissip_shock, csens
-     integer(c_int32_t) :: is_adiab = 1, reserved = 0
+     integer(c_int32_t) :: is_adiab = 1, is_sw_edoh = 0
      real(c_double) :: t_wall = 0.0_c_double
   end type mgpu_case_t
 

@@ -97,6 +97,7 @@ program musica_gpu_tgv
      cs%filter = MGPU_FILTER_ARTVISC
   endif
   cs%is_ducros = merge(1, 0, prm%is_ducros)
+  cs%is_sw_edoh = merge(1, 0, prm%is_sw_edoh)
   cs%nrk = prm%nrk
   cs%precision = MGPU_PREC_FP64
   cs%launch_flags = 0

@@ -55,6 +55,7 @@ def make_case_struct(case, precision: int = _lib.PREC_FP64, topology=None, launc
     cs.dissip_shock = case.dissip_shock
     cs.csens = case.csens
     cs.is_adiab = int(getattr(case, "adiabatic", True))
+    cs.is_sw_edoh = int(getattr(case, "sw_edoh", False))
     cs.t_wall = float(getattr(case, "t_wall", 0.0))
     return cs, blocks
 

@@ -534,8 +534,8 @@ static void filter_sf(const orc_cfg *c, const dims_t *d, double *const Un[5], do
 /* the pressure of the ghost state, which nothing on the path changes), the  */
 /* faces over 1..n-1 and K over 2..n-1.                                      */
 /* ------------------------------------------------------------------------ */
-static void filter_shock(const orc_cfg *c, const dims_t *d, double *const Un[5], const work_t *w,
-                         double *K[5]) {
+static void filter_shock(const orc_cfg *c, const orc_grid *g, const dims_t *d, double *const Un[5],
+                         const work_t *w, double *K[5]) {
   const double xnu0 = -c->dissip_shock, cs = c->csens;
   /* face amplitudes and differences by half-width hf = 1..5: d2..d10 with
    * xnu2 = 0, xnu4 = chi/12, xnu6 = chi/60, xnu8 = chi/280, xnu10 = chi/1260
@@ -610,13 +610,34 @@ static void filter_shock(const orc_cfg *c, const dims_t *d, double *const Un[5],
               D[v * (n + 1) + m] = (u[p + s] - u[p]) * xk0 + dd * xkh;
             }
           }
-          /* K_i = D_{i-1/2} - D_{i+1/2} on 1..n (2..n-1 next to walls) */
+          /* K_i = D_{i-1/2} - D_{i+1/2} on 1..n (2..n-1 next to walls); with the
+           * Edoh switch times min(1, cfl/chi), cfl the local CFL number of U^n
+           * along dir: (|u_dir| + c) dt idx_dir(i) (Cartesian), (|vc| + c g3) dt
+           * with the contravariant velocity vc = (u xi_x + v xi_y + w xi_z) ijacob3
+           * and g3 = |grad xi| |ijacob3| (c3) (mod_filtering_shock.f90:620-700,
+           * :1059-1140, :1450-1496; g3_ksi smod_grid_metrics3_c3.f90:161) */
           for (int i = 1; i <= n; i++) {
             long p = p0 + i * s;
             const int edge = (lo && i == 1) || (hi && i == n);
+            double mul = 1.0;
+            if (c->sw_edoh) {
+              double cfl;
+              if (c->curv) {
+                const double mx = g->m[3 * dir][p], my = g->m[3 * dir + 1][p], mz = g->m[3 * dir + 2][p];
+                const double ij = g->m[9][p];
+                const double vc = (w->uu[p] * mx + w->vv[p] * my + w->ww[p] * mz) * ij;
+                const double g3 = sqrt(mx * mx + my * my + mz * mz) * fabs(ij);
+                cfl = (fabs(vc) + w->cs[p] * g3) * c->dt;
+              } else {
+                const double *vel = dir == 0 ? w->uu : (dir == 1 ? w->vv : w->ww);
+                const double *met = dir == 0 ? g->idx : (dir == 1 ? g->idy : g->idz);
+                cfl = (fabs(vel[p]) + w->cs[p]) * c->dt * met[i - 1 + NG];
+              }
+              mul = fmin(1.0, cfl / c->chi);
+            }
             for (int v = 0; v < 5; v++) {
               const double *Dv = D + v * (n + 1);
-              const double kd = edge ? 0.0 : Dv[i - 1] - Dv[i];
+              const double kd = edge ? 0.0 : (c->sw_edoh ? mul * (Dv[i - 1] - Dv[i]) : Dv[i - 1] - Dv[i]);
               if (dir == 0) K[v][p] = kd;
               else K[v][p] = K[v][p] + kd;
             }
@@ -792,7 +813,7 @@ void orc_filter_rhs(const orc_cfg *c, const orc_grid *g, double *const Un[5], do
     prim_visc(c, &d, Uref, &w);
     gradients(c, g, &d, &w);
     for (int q = 0; q < 12; q++) orc_fill_periodic(c, w.grad[q]);
-    filter_shock(c, &d, Un, &w, K);
+    filter_shock(c, g, &d, Un, &w, K);
     work_free(&w);
   } else if (c->filter == 3) {
     work_t w;
@@ -839,7 +860,7 @@ int orc_step(const orc_cfg *c, const orc_grid *g, double *U[5], int nsteps) {
     if (c->filter) {
       /* num_dissip then update_varn (runge_kutta.f90:113-126; update.f90:538-565) */
       if (c->filter == 1) filter_sf(c, &d, Un, K);
-      else if (c->filter == 2) filter_shock(c, &d, Un, &w, K);  /* sensor on p, grads of U^n */
+      else if (c->filter == 2) filter_shock(c, g, &d, Un, &w, K);  /* sensor on p, grads of U^n */
       else artvisc(c, g, &d, Un, &w, K);               /* |u|+c of U^n */
       /* update_varn_filter adds K, update_varn_artvisc deltat*K (update.f90:538-608) */
       const double fk = c->filter == 3 ? c->dt : 1.0;
@@ -991,7 +1012,7 @@ int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_gri
     if (c0->filter) {
       for (int b = 0; b < nb; b++) {
         if (c0->filter == 1) filter_sf(&cb[b], &d[b], Un[b], K[b]);
-        else if (c0->filter == 2) filter_shock(&cb[b], &d[b], Un[b], &w[b], K[b]);
+        else if (c0->filter == 2) filter_shock(&cb[b], &g[b], &d[b], Un[b], &w[b], K[b]);
         else artvisc(&cb[b], &g[b], &d[b], Un[b], &w[b], K[b]);
         const dims_t dd = d[b];
         const int av = c0->filter == 3;

@@ -51,7 +51,7 @@ typedef struct orc_cfg {
   double dissip_shock;     /* shock-capturing amplitude                     */
   double csens;            /* pressure sensor blend                         */
   int adiabatic;           /* walls: 1 adiabatic (dT/dn = 0), 0 isothermal  */
-  int pad2_;
+  int sw_edoh;              /* shock filter: Edoh switch (is_sw_edoh)         */
   double t_wall;           /* wall temperature of isothermal walls          */
 } orc_cfg;
 

@@ -31,7 +31,7 @@ class OrcCfg(C.Structure):
         ("c0", C.c_double), ("S0", C.c_double), ("diffscale", C.c_double),
         ("cok_ratio", C.c_double),
         ("chi", C.c_double), ("dissip_shock", C.c_double), ("csens", C.c_double),
-        ("adiabatic", C.c_int), ("pad2_", C.c_int), ("t_wall", C.c_double),
+        ("adiabatic", C.c_int), ("sw_edoh", C.c_int), ("t_wall", C.c_double),
     ]
 
 
@@ -104,6 +104,7 @@ class Oracle:
         c.visc = int(case.iorder_visc != 0)
         c.filter = int(case.filter)
         c.ducros = int(case.ducros)
+        c.sw_edoh = int(getattr(case, "sw_edoh", False))
         c.nrk = int(case.nrk)
         faces = tuple(getattr(case, "faces", (0,) * 6))
         for d in range(3):
