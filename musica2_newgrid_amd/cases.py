@@ -85,6 +85,10 @@ class Case:
     # curvilinear metrics: 10 padded arrays each (ksi_x..phi_z, ijacob3)
     m: Optional[List[np.ndarray]] = None
     mv: Optional[List[np.ndarray]] = None
+    # 2.5-D curvilinear (curvilinear == 2): padded 2-D x_ksi, x_eta, y_ksi, y_eta,
+    # ijacob (Euler and viscous sets); cart[2], cart[5] give idz, idz_v
+    m2: Optional[List[np.ndarray]] = None
+    m2v: Optional[List[np.ndarray]] = None
     # initial conserved state: rho, rhou, rhov, rhow, rhoe (padded)
     U: Optional[List[np.ndarray]] = None
     # face kinds imin..kmax (0 periodic, 2 no-slip wall) and the wall condition
@@ -210,6 +214,65 @@ def vortex_wavy(nx: int = 512, ny: int = 256, nz: int = 128, mach: float = 0.5, 
                                     np.diff(zc, axis=a) ** 2))) for a in (0, 1, 2))
     cs = Case("vortex_wavy", nx, ny, nz, curvilinear=True, iorder_visc=iorder_visc, filter=filter,
               ducros=True, diffscale=diffscale, m=m, mv=mv, U=U)
+    cs.dt = timestep(U, dmin, cfl)
+    cs.info = dict(mach=mach, beta=beta, radius=radius, re=re, cfl=cfl, lengths=lengths, amp=amp,
+                   dmin=dmin)
+    return cs
+
+
+def wavy_mapping_2d(lengths, amp):
+    """x = xi + ax sin(2pi eta/Ly), y = eta + ay sin(2pi xi/Lx): a smooth,
+    periodic distortion in (x, y), extruded along z."""
+    Lx, Ly = lengths[0], lengths[1]
+    ax, ay = amp
+    tp = 2.0 * math.pi
+
+    def f(XI, ET):
+        return XI + ax * np.sin(tp * ET / Ly), ET + ay * np.sin(tp * XI / Lx)
+
+    return f
+
+
+def vortex_wavy_c2(nx: int = 512, ny: int = 256, nz: int = 128, mach: float = 0.5, beta: float = 5.0,
+                   radius: float = 0.5, re: float = 1.0e4, cfl: float = 0.8, filter: int = 2,
+                   lengths=None, amp=None, T_inf: float = 300.0, rho_inf: float = 1.177,
+                   iorder_visc: int = 4) -> Case:
+    """Isentropic vortex over a z-extruded wavy block run as 2.5-D curvilinear
+    (is_curv: 2-D metrics in (i, j), Cartesian z), periodic faces."""
+    if lengths is None:
+        h = 10.0 / nx
+        lengths = (nx * h, ny * h, nz * h)
+    Lx, Ly, Lz = lengths
+    if amp is None:
+        amp = (0.02 * Lx / 2, 0.05 * Ly / 2)
+    m2, m2v, (x2, y2) = gr.c2_metrics_periodic(wavy_mapping_2d(lengths, amp), nx, ny, lengths,
+                                               viscous=iorder_visc != 0)
+    if m2v is None:
+        m2v = [a.copy() for a in m2]
+    dz = Lz / nz
+    xc = np.broadcast_to(x2, (nz, ny, nx))
+    yc = np.broadcast_to(y2, (nz, ny, nx))
+    c_inf = math.sqrt(GAM * RG * T_inf)
+    u_inf = mach * c_inf
+    xv, yv = Lx / 2.0, Ly / 2.0
+    r2 = ((xc - xv) ** 2 + (yc - yv) ** 2) / radius ** 2
+    e = np.exp(0.5 * (1.0 - r2))
+    u = u_inf * (1.0 - beta / (2.0 * math.pi) * (yc - yv) / radius * e)
+    v = u_inf * (beta / (2.0 * math.pi) * (xc - xv) / radius * e)
+    w = np.zeros_like(u)
+    T = T_inf * (1.0 - (GAM - 1.0) * beta ** 2 * mach ** 2 / (8.0 * math.pi ** 2) * e * e)
+    rho = rho_inf * (T / T_inf) ** (1.0 / (GAM - 1.0))
+    p = rho * RG * T
+    U = _to_padded(conserved(rho, u, v, w, p), nx, ny, nz)
+    mu_ref = rho_inf * u_inf * radius / re
+    diffscale = mu_ref / float(sutherland(np.float64(T_inf)))
+    dmin = min(float(np.min(np.sqrt(np.diff(x2, axis=a) ** 2 + np.diff(y2, axis=a) ** 2))) for a in (0, 1))
+    dmin = min(dmin, dz)
+    idz = np.full(nz + 2 * G, 1.0 / dz)
+    ones_x, ones_y = np.ones(nx + 2 * G), np.ones(ny + 2 * G)
+    cs = Case("vortex_wavy_c2", nx, ny, nz, curvilinear=2, iorder_visc=iorder_visc, filter=filter,
+              ducros=True, diffscale=diffscale, m2=m2, m2v=m2v, U=U,
+              cart=[ones_x, ones_y, idz, ones_x.copy(), ones_y.copy(), idz.copy()])
     cs.dt = timestep(U, dmin, cfl)
     cs.info = dict(mach=mach, beta=beta, radius=radius, re=re, cfl=cfl, lengths=lengths, amp=amp,
                    dmin=dmin)

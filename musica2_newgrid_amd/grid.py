@@ -158,6 +158,53 @@ def curvilinear_metrics_periodic(mapping, nx, ny, nz, lengths, viscous=True):
     return out[0], out[1], coords
 
 
+def c2_metrics_periodic(mapping, nx, ny, lengths, viscous=True):
+    """2-D metrics of a 2.5-D curvilinear block (is_curv), periodic in i and j.
+
+    mapping(xi, eta) -> (x, y) with x - xi, y - eta periodic; z is Cartesian.
+    x_ksi, x_eta, y_ksi, y_eta are the centred derivatives of the node
+    coordinates (a11 for the Euler set, a5 for the viscous set) and ijacob =
+    1/(x_ksi y_eta - x_eta y_ksi) (grid_metrics_ksi/_eta/_ijacob,
+    src/Grid/grid_metrics_curv.f90:2-296, 595-630, and the _v twins), on
+    the interior, ghosts from the periodic image.
+
+    Returns (m2, m2v, (xc, yc)): two lists of 5 padded 2-D arrays of shape
+    (ny+2G, nx+2G), i fastest: x_ksi, x_eta, y_ksi, y_eta, ijacob."""
+    ext = 10
+    Lx, Ly = lengths[0], lengths[1]
+    i = np.arange(1 - ext, nx + ext + 1, dtype=np.float64)
+    j = np.arange(1 - ext, ny + ext + 1, dtype=np.float64)
+    Y2, X2 = np.meshgrid((j - 1) * (Ly / ny), (i - 1) * (Lx / nx), indexing="ij")
+    xe, ye = mapping(X2, Y2)
+    out = []
+    for coeffs in ((A11, A5) if viscous else (A11,)):
+        R = len(coeffs)
+
+        def d(f, axis):
+            g = _deriv(f, axis, coeffs)
+            other = 1 - axis
+            s = [slice(None), slice(None)]
+            s[other] = slice(ext, f.shape[other] - ext)
+            s[axis] = slice(ext - R, g.shape[axis] - (ext - R))
+            return g[tuple(s)]
+
+        xk, xh, yk, yh = d(xe, 1), d(xe, 0), d(ye, 1), d(ye, 0)
+        ij = 1.0 / (xk * yh - xh * yk)
+        arrs = []
+        for f in (xk, xh, yk, yh, ij):
+            a = np.zeros((ny + 2 * G, nx + 2 * G))
+            a[G:-G, G:-G] = f
+            a[G:-G, :G] = a[G:-G, nx:nx + G]
+            a[G:-G, nx + G:] = a[G:-G, G:2 * G]
+            a[:G, :] = a[ny:ny + G, :]
+            a[ny + G:, :] = a[G:2 * G, :]
+            arrs.append(a)
+        out.append(arrs)
+    if not viscous:
+        out.append(None)
+    return out[0], out[1], (xe[ext:-ext, ext:-ext], ye[ext:-ext, ext:-ext])
+
+
 # Wall closures of the Cartesian metrics (src/Grid/grid_metrics_cart.f90:2-186,
 # 189-436; src/Derivatives/mod_coeff_deriv.f90:100-160, 232-241, 1099-1160).
 # a3, a5, a7, a9: centred schemes of half-width 1..4 used on rows 2..5 from a wall.

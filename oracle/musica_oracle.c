@@ -241,6 +241,27 @@ static inline double d11(const double *f, long p, long s) {
          A11[5] * (f[p + 5 * s] - f[p - 5 * s]);
 }
 
+/* Metric term q of the c3 set at element p (ksi_x,ksi_y,ksi_z,eta_x,..,
+ * phi_z, ijacob3 = 0..9), or on a 2.5-D block (is_curv) its counterpart from
+ * the 2-D arrays m2 = x_ksi, x_eta, y_ksi, y_eta, ijacob: the ksi-flux
+ * projects on (y_eta, -x_eta, 0) and the eta-flux on (-y_ksi, x_ksi, 0)
+ * (flux_euler_11pts_c, smod_flux_euler_11pts_c.f90:25-37, :265-277; the
+ * gradients smod_grad_vel_c.f90:186-217), 1/J = ijacob.  The phi rows are
+ * not used there: z is Cartesian. */
+static inline double mtr(const orc_cfg *c, const double *const m3[10], const double *const m2[5], const dims_t *d,
+                         int q, long p) {
+  if (c->curv != 2) return m3[q][p];
+  const long p2 = p % ((long)d->sx * d->sy);
+  switch (q) {
+    case 0: return m2[3][p2];
+    case 1: return -m2[1][p2];
+    case 3: return -m2[2][p2];
+    case 4: return m2[0][p2];
+    case 9: return m2[4][p2];
+    default: return 0.0;
+  }
+}
+
 /* Physical (Cartesian) flux along direction dir at linear offset p. */
 static inline void flux_cart(int dir, double *const U[5], const work_t *w, long p, double F[5]) {
   double pr = w->prs[p];
@@ -253,10 +274,12 @@ static inline void flux_cart(int dir, double *const U[5], const work_t *w, long 
   F[4] = (U[4][p] + pr) * vel;
 }
 
-/* Contravariant flux along direction dir (smod_flux_euler_11pts_c3.f90:25-38). */
-static inline void flux_c3(int dir, double *const U[5], const work_t *w, const orc_grid *g,
-                           long p, double F[5]) {
-  const double mx = g->m[3 * dir][p], my = g->m[3 * dir + 1][p], mz = g->m[3 * dir + 2][p];
+/* Contravariant flux along direction dir (smod_flux_euler_11pts_c3.f90:25-38;
+ * 2.5-D: smod_flux_euler_11pts_c.f90:25-37, :265-277). */
+static inline void flux_c3(const orc_cfg *c, const dims_t *d, int dir, double *const U[5], const work_t *w,
+                           const orc_grid *g, long p, double F[5]) {
+  const double mx = mtr(c, g->m, g->m2, d, 3 * dir, p), my = mtr(c, g->m, g->m2, d, 3 * dir + 1, p),
+               mz = mtr(c, g->m, g->m2, d, 3 * dir + 2, p);
   double u = w->uu[p], v = w->vv[p], ww = w->ww[p], pr = w->prs[p];
   double vc = u * mx + v * my + ww * mz;
   double rovc = U[0][p] * vc;
@@ -301,7 +324,7 @@ static void euler_rhs_w(const orc_cfg *c, const orc_grid *g, const dims_t *d, do
           for (int l = 0; l < n + 2 * NG; l++) {
             double F[5];
             long p = p0 + l * s;
-            if (c->curv) flux_c3(dir, U, w, g, p, F);
+            if (c->curv == 1 || (c->curv == 2 && dir < 2)) flux_c3(c, d, dir, U, w, g, p, F);
             else flux_cart(dir, U, w, p, F);
             for (int v = 0; v < 5; v++) fl[v * (n + 2 * NG) + l] = F[v];
           }
@@ -310,7 +333,8 @@ static void euler_rhs_w(const orc_cfg *c, const orc_grid *g, const dims_t *d, do
             long q = i - 1 + NG; /* position in line buffer */
             const int hw = halfw(i, n, W.lo[dir], W.hi[dir]);
             double metric = 1.0;
-            if (!c->curv) metric = dir == 0 ? g->idx[i - 1 + NG] : (dir == 1 ? g->idy[i - 1 + NG] : g->idz[i - 1 + NG]);
+            const int cart_dir = !c->curv || (c->curv == 2 && dir == 2);
+            if (cart_dir) metric = dir == 0 ? g->idx[i - 1 + NG] : (dir == 1 ? g->idy[i - 1 + NG] : g->idz[i - 1 + NG]);
             for (int v = 0; v < 5; v++) {
               const double *f = fl + v * (n + 2 * NG);
               double dv;
@@ -321,7 +345,7 @@ static void euler_rhs_w(const orc_cfg *c, const orc_grid *g, const dims_t *d, do
                 dv = 0.0;
                 for (int l = 1; l <= hw; l++) dv = dv + ACL[hw][l] * (f[q + l] - f[q - l]);
               }
-              if (!c->curv) dv = dv * metric;
+              if (cart_dir) dv = dv * metric;
               if (dir == 0) K[v][p] = dv;
               else K[v][p] = dv + K[v][p];
             }
@@ -329,8 +353,20 @@ static void euler_rhs_w(const orc_cfg *c, const orc_grid *g, const dims_t *d, do
         }
       free(fl);
     }
+    if (c->curv == 2 && dir == 1) {
+      /* 2.5-D: the ksi and eta parts times ijacob, then the Cartesian z part
+       * (smod_flux_euler_11pts_c.f90:527-540) */
+#pragma omp parallel for collapse(2)
+      for (int k = 1; k <= d->nz; k++)
+        for (int j = 1; j <= d->ny; j++)
+          for (int i = 1; i <= d->nx; i++) {
+            long p = (long)ID(*d, i, j, k);
+            double ij = mtr(c, g->m, g->m2, d, 9, p);
+            for (int v = 0; v < 5; v++) K[v][p] = K[v][p] * ij;
+          }
+    }
   }
-  if (c->curv) {
+  if (c->curv == 1) {
     /* K *= ijacob3 (smod_flux_euler_11pts_c3.f90:763-773) */
 #pragma omp parallel for collapse(2)
     for (int k = 1; k <= d->nz; k++)
@@ -370,6 +406,15 @@ static void gradients(const orc_cfg *c, const orc_grid *g, const dims_t *d, work
           if (!c->curv) {
             gx = gx * g->idx_v[i - 1 + NG];
             gy = gy * g->idy_v[j - 1 + NG];
+            gz = gz * g->idz_v[k - 1 + NG];
+          } else if (c->curv == 2) {
+            /* (f_ksi y_eta_v - f_eta y_ksi_v) ijacob_v, (f_eta x_ksi_v - f_ksi x_eta_v)
+             * ijacob_v, f_z idz_v (smod_grad_vel_c.f90:186-217, :230-300) */
+            const double ijv = mtr(c, g->mv, g->m2v, d, 9, p);
+            double dvx = gx * mtr(c, g->mv, g->m2v, d, 0, p) + gy * mtr(c, g->mv, g->m2v, d, 3, p);
+            double dvy = gx * mtr(c, g->mv, g->m2v, d, 1, p) + gy * mtr(c, g->mv, g->m2v, d, 4, p);
+            gx = dvx * ijv;
+            gy = dvy * ijv;
             gz = gz * g->idz_v[k - 1 + NG];
           } else {
             const double *const *mv = g->mv;
@@ -428,7 +473,14 @@ static void visc_rhs_w(const orc_cfg *c, const orc_grid *g, const dims_t *d, wor
           F[11][p] = (u * F[8][p] + v * F[9][p] + ww * F[10][p]) - ck * gr[DTZ][p];
         } else {
           for (int dir = 0; dir < 3; dir++) {
-            double mx = g->mv[3 * dir][p], my = g->mv[3 * dir + 1][p], mz = g->mv[3 * dir + 2][p];
+            if (c->curv == 2 && dir == 2) {
+              /* 2.5-D: Cartesian z flux (smod_flux_visc_5pts_c.f90:67-72) */
+              F[8][p] = t13; F[9][p] = t23; F[10][p] = t33;
+              F[11][p] = (u * F[8][p] + v * F[9][p] + ww * F[10][p]) - ck * gr[DTZ][p];
+              continue;
+            }
+            double mx = mtr(c, g->mv, g->m2v, d, 3 * dir, p), my = mtr(c, g->mv, g->m2v, d, 3 * dir + 1, p),
+                   mz = mtr(c, g->mv, g->m2v, d, 3 * dir + 2, p);
             double fu = t11 * mx + t12 * my + t13 * mz;
             double fv = t12 * mx + t22 * my + t23 * mz;
             double fw = t13 * mx + t23 * my + t33 * mz;
@@ -453,7 +505,7 @@ static void visc_rhs_w(const orc_cfg *c, const orc_grid *g, const dims_t *d, wor
         const int rr[3] = {i, j, k};
         for (int dir = 0; dir < 3; dir++) {
           double metric;
-          if (c->curv) metric = g->mv[9][p];
+          if (c->curv == 1 || (c->curv == 2 && dir < 2)) metric = mtr(c, g->mv, g->m2v, d, 9, p);
           else metric = dir == 0 ? g->idx_v[i - 1 + NG] : (dir == 1 ? g->idy_v[j - 1 + NG] : g->idz_v[k - 1 + NG]);
           for (int v = 0; v < 4; v++)
             K[1 + v][p] = g5w(F[4 * dir + v], p, st[dir], rr[dir], nn[dir], W.lo[dir], W.hi[dir]) * metric +
@@ -622,9 +674,12 @@ static void filter_shock(const orc_cfg *c, const orc_grid *g, const dims_t *d, d
             double mul = 1.0;
             if (c->sw_edoh) {
               double cfl;
-              if (c->curv) {
-                const double mx = g->m[3 * dir][p], my = g->m[3 * dir + 1][p], mz = g->m[3 * dir + 2][p];
-                const double ij = g->m[9][p];
+              if (c->curv == 1 || (c->curv == 2 && dir < 2)) {
+                /* 2.5-D: vc = (u y_eta - v x_eta) ijacob, g_ksi = |grad ksi| |ijacob|
+                 * (mod_filtering_shock.f90:645-652; grid_metrics_curv.f90:683-685) */
+                const double mx = mtr(c, g->m, g->m2, d, 3 * dir, p), my = mtr(c, g->m, g->m2, d, 3 * dir + 1, p),
+                             mz = mtr(c, g->m, g->m2, d, 3 * dir + 2, p);
+                const double ij = mtr(c, g->m, g->m2, d, 9, p);
                 const double vc = (w->uu[p] * mx + w->vv[p] * my + w->ww[p] * mz) * ij;
                 const double g3 = sqrt(mx * mx + my * my + mz * mz) * fabs(ij);
                 cfl = (fabs(vc) + w->cs[p] * g3) * c->dt;
@@ -755,6 +810,7 @@ static int check_cfg(const orc_cfg *c) {
       if (c->face[dir][s] != ORC_FACE_PERIODIC) return -2; /* only periodic faces so far */
   if (c->filter >= 2 && !c->visc) return -3;              /* sensor / sound speed of primitives_visc */
   if (c->filter == 3 && c->curv) return -8;               /* artificial viscosity: Cartesian */
+  if (c->curv < 0 || c->curv > 2) return -9;
   if (c->nrk < 1 || c->nrk > 8) return -4;
   return 0;
 }

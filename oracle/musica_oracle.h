@@ -35,7 +35,8 @@ enum orc_face_kind {
 
 typedef struct orc_cfg {
   int nx, ny, nz;          /* interior cells                                */
-  int curv;                /* 0 Cartesian (idx 1-D metrics), 1 3-D curvilinear (c3) */
+  int curv;                /* 0 Cartesian (idx 1-D metrics), 1 3-D curvilinear (c3),
+                            * 2 2.5-D curvilinear (2-D metrics in i,j, Cartesian z) */
   int visc;                /* 1: 5-pt viscous terms in the last RK stage   */
   int filter;              /* 0 none, 1 selective filter, 2 shock filter, 3 artificial viscosity (Cartesian) */
   int ducros;              /* Ducros sensor in the shock filter            */
@@ -66,6 +67,11 @@ typedef struct orc_grid {
    * cextrp2*p(2) - cextrp3*p(3) (init_coeff_wall_cart,
    * src/Derivatives/mod_coeff_deriv.f90:1099-1160) */
   double cextrp2[6], cextrp3[6];
+  /* 2.5-D curvilinear (is_curv): padded 2-D arrays (nx+2ng)*(ny+2ng), i fastest:
+   * x_ksi, x_eta, y_ksi, y_eta, ijacob (Euler set, 11-pt) and the viscous set
+   * (*_v, 5-pt); z uses idz / idz_v above (mod_grid.f90, grid_metrics_curv.f90) */
+  const double *m2[5];
+  const double *m2v[5];
 } orc_grid;
 
 /* One block of a multiblock case.  Face f (imin,imax,jmin,jmax,kmin,kmax =

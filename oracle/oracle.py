@@ -50,6 +50,8 @@ class OrcGrid(C.Structure):
         ("m", _dp * 10),
         ("mv", _dp * 10),
         ("cextrp2", C.c_double * 6), ("cextrp3", C.c_double * 6),
+        ("m2", _dp * 5),
+        ("m2v", _dp * 5),
     ]
 
 
@@ -100,7 +102,7 @@ class Oracle:
         self.case = case
         c = OrcCfg()
         c.nx, c.ny, c.nz = case.nx, case.ny, case.nz
-        c.curv = int(case.curvilinear)
+        c.curv = int(case.curvilinear)  # 0 Cartesian, 1 c3, 2 2.5-D
         c.visc = int(case.iorder_visc != 0)
         c.filter = int(case.filter)
         c.ducros = int(case.ducros)
@@ -122,7 +124,17 @@ class Oracle:
         self.cfg = c
         g = OrcGrid()
         self._keep = []
-        if case.curvilinear:
+        if int(case.curvilinear) == 2:
+            for q in range(5):
+                a = np.ascontiguousarray(case.m2[q], dtype=np.float64)
+                b = np.ascontiguousarray(case.m2v[q], dtype=np.float64)
+                self._keep += [a, b]
+                g.m2[q] = _p(a)
+                g.m2v[q] = _p(b)
+            arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in case.cart]
+            self._keep += arrs
+            g.idx, g.idy, g.idz, g.idx_v, g.idy_v, g.idz_v = [_p(a) for a in arrs]
+        elif case.curvilinear:
             for q in range(10):
                 a = np.ascontiguousarray(case.m[q])
                 self._keep.append(a)
