@@ -47,6 +47,12 @@ extern "C" {
 /* Face kinds: restate BC_face(dir,side)%sort (src/Boundary/mod_bc.f90:7-61)
  * for the kinds this library advances.  A periodic face is the reference's
  * neighbour-is-itself interface (src/Parallel/mod_mpi_part.f90:333-). */
+/* Geometry of a case (mgpu_case.curvilinear) */
+#define MGPU_GEOM_CART 0 /* Cartesian, 1-D metrics idx, idy, idz                     */
+#define MGPU_GEOM_C3 1   /* is_curv3: 3-D metrics at every point (flux_euler_11pts_c3) */
+#define MGPU_GEOM_C2 2   /* is_curv: 2-D metrics in (i, j), Cartesian z (flux_euler_11pts_c, */
+                         /* src/Compute_flow/smod_flux_euler_11pts_c.f90)             */
+
 #define MGPU_FACE_PERIODIC 0
 #define MGPU_FACE_INTERFACE 1
 /* No-slip wall, BC_face%sort = 0 with is_wall2 = F (the reference's default,
@@ -96,7 +102,8 @@ typedef struct mgpu_block {
  * (param_blocks.ini, src/Main/mod_block.f90:109-229). */
 typedef struct mgpu_case {
   int32_t nblocks;
-  int32_t curvilinear;    /* 0: Cartesian 1-D metrics (idx,idy,idz); 1: full 3-D (is_curv3) */
+  int32_t curvilinear;    /* MGPU_GEOM_*: Cartesian 1-D metrics (idx,idy,idz),  */
+                          /* full 3-D (is_curv3) or 2.5-D (is_curv)            */
   int32_t iorder_visc;    /* 0 (Euler) or 4 (5-pt viscous terms)               */
   int32_t filter;         /* MGPU_FILTER_*                                     */
   int32_t is_ducros;      /* Ducros sensor in the shock filter                 */
@@ -141,6 +148,14 @@ int mgpu_set_metrics_cart(mgpu_handle *h, int block, const double *idx, const do
  * (*_v), each a padded 3-D array with ghosts
  * (src/Grid/mod_grid.f90:131-145, smod_grid_metrics1_c3.f90:8-484). */
 int mgpu_set_metrics_c3(mgpu_handle *h, int block, const double *const *m, const double *const *mv);
+
+/* 2.5-D curvilinear metrics of a block (MGPU_GEOM_C2, the reference's
+ * is_curv): m2[0..4] = x_ksi, x_eta, y_ksi, y_eta, ijacob (Euler set, 11-pt)
+ * and m2v[0..4] the viscous set (*_v, 5-pt), each a padded 2-D array of
+ * (nx+2*MGPU_NGH)*(ny+2*MGPU_NGH) doubles, i fastest; idz, idz_v as in
+ * mgpu_set_metrics_cart (src/Grid/mod_grid.f90, grid_metrics_curv.f90). */
+int mgpu_set_metrics_c2(mgpu_handle *h, int block, const double *const *m2, const double *const *m2v,
+                        const double *idz, const double *idz_v);
 
 /* Wall extrapolation coefficients of a block with MGPU_FACE_WALL faces:
  * cextrp2[f], cextrp3[f] for faces f = 0..5 (entries of faces without a

@@ -18,6 +18,7 @@ FACE_INTERFACE = 1
 FACE_WALL = 2
 FILTER_NONE, FILTER_SF, FILTER_SHOCK, FILTER_ARTVISC = 0, 1, 2, 3
 PREC_FP64, PREC_MIXED = 0, 1
+GEOM_CART, GEOM_C3, GEOM_C2 = 0, 1, 2
 LAUNCH_EAGER = 1
 OK, EINVAL, EDEVICE, ENOMEM, EUNSUP, ESTATE = 0, -1, -2, -3, -4, -5
 K_STAGE, K_VISC, K_FILTER, K_HALO = 0, 1, 2, 3
@@ -28,6 +29,7 @@ EXPORTS = (
     "mgpu_create",
     "mgpu_set_metrics_cart",
     "mgpu_set_metrics_c3",
+    "mgpu_set_metrics_c2",
     "mgpu_set_wall",
     "mgpu_put_state",
     "mgpu_get_state",
@@ -113,6 +115,7 @@ def lib() -> C.CDLL:
     L.mgpu_create.argtypes = [C.POINTER(MgpuCase), C.POINTER(C.c_void_p)]
     L.mgpu_set_metrics_cart.argtypes = [C.c_void_p, C.c_int] + [_dp] * 6
     L.mgpu_set_metrics_c3.argtypes = [C.c_void_p, C.c_int, C.POINTER(_dp), C.POINTER(_dp)]
+    L.mgpu_set_metrics_c2.argtypes = [C.c_void_p, C.c_int, C.POINTER(_dp), C.POINTER(_dp), _dp, _dp]
     L.mgpu_set_wall.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     L.mgpu_put_state.argtypes = [C.c_void_p, C.c_int] + [_dp] * 5
     L.mgpu_get_state.argtypes = [C.c_void_p, C.c_int] + [_dp] * 5
@@ -143,7 +146,8 @@ def lib() -> C.CDLL:
     for name in ("mgpu_server_create", "mgpu_server_submit", "mgpu_server_tick", "mgpu_server_status",
                  "mgpu_server_take"):
         getattr(L, name).restype = C.c_int
-    for name in ("mgpu_create", "mgpu_set_metrics_cart", "mgpu_set_metrics_c3", "mgpu_set_wall", "mgpu_put_state",
+    for name in ("mgpu_create", "mgpu_set_metrics_cart", "mgpu_set_metrics_c3", "mgpu_set_metrics_c2", "mgpu_set_wall",
+                 "mgpu_put_state",
                  "mgpu_get_state", "mgpu_step", "mgpu_autotune", "mgpu_stage_config", "mgpu_sync", "mgpu_profile", "mgpu_kernel_time"):
         getattr(L, name).restype = C.c_int
     _lib = L

@@ -2,7 +2,7 @@
 // 1/2 rho |curl u|^2, the quantities of stats_tgv
 // (src/File_IO/stats.f90:37-112), with the velocity gradients by the 5-pt
 // scheme of grad_vel (smod_grad_vel.f90, c3 mapping smod_grad_vel_c3.f90:
-// 260-300); and the maximum of |u| + c, the velocity scale of timestep_new
+// 260-300, 2.5-D mapping smod_grad_vel_c.f90:186-217); and the maximum of |u| + c, the velocity scale of timestep_new
 // (src/Time/timestep_new.f90:270-288).  Not on the time-step path: one thread per point, neighbours
 // read straight from global memory; each workgroup writes its three partial
 // sums, which the host adds in a fixed order (deterministic results).
@@ -20,7 +20,7 @@ __device__ __forceinline__ void vel(const CState &U, idx_t p, double o[3]) {
   o[2] = ldg(U.v[3], p) * vi;
 }
 
-template <bool CURV>
+template <int GEO>
 __global__ __launch_bounds__(kDiagThreads) void k_diag(DiagArgs a) {
   __shared__ double red[4][kDiagThreads];
   const Geom &g = a.g;
@@ -46,7 +46,20 @@ __global__ __launch_bounds__(kDiagThreads) void k_diag(DiagArgs a) {
       for (int c = 0; c < 3; c++) gd[d][c] = A5_1 * (a1[c] - b1[c]) + A5_2 * (a2[c] - b2[c]);
     }
     double gr[3][3];  // gr[u,v,w][x,y,z]
-    if constexpr (CURV) {
+    if constexpr (GEO == kGeoC2) {
+      // (f_ksi y_eta - f_eta y_ksi) ijacob, (f_eta x_ksi - f_ksi x_eta) ijacob, f_z idz
+      const idx_t p2 = lin2(g, i, j);
+      double m[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) m[q] = ldg(a.met.m2v[q], p2);
+      const double iz = ldg(a.met.idz_v, (idx_t)(k - 1 + G));
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        gr[c][0] = (gd[0][c] * m[0] + gd[1][c] * m[2]) * m[4];
+        gr[c][1] = (gd[0][c] * m[1] + gd[1][c] * m[3]) * m[4];
+        gr[c][2] = gd[2][c] * iz;
+      }
+    } else if constexpr (GEO == kGeoC3) {
       double m[10];
 #pragma unroll
       for (int q = 0; q < 10; q++) m[q] = ldg(a.met.mv[q], p);
@@ -98,10 +111,11 @@ __global__ __launch_bounds__(kDiagThreads) void k_diag(DiagArgs a) {
 
 long diag_groups(const Geom &g) { return ((long)g.nx * g.ny * g.nz + kDiagThreads - 1) / kDiagThreads; }
 
-hipError_t launch_diag(const DiagArgs &a, bool curv, hipStream_t s) {
+hipError_t launch_diag(const DiagArgs &a, int geo, hipStream_t s) {
   const long nb = diag_groups(a.g);
-  if (curv) hipLaunchKernelGGL(k_diag<true>, dim3((unsigned)nb), dim3(kDiagThreads), 0, s, a);
-  else hipLaunchKernelGGL(k_diag<false>, dim3((unsigned)nb), dim3(kDiagThreads), 0, s, a);
+  if (geo == kGeoC2) hipLaunchKernelGGL(k_diag<kGeoC2>, dim3((unsigned)nb), dim3(kDiagThreads), 0, s, a);
+  else if (geo == kGeoC3) hipLaunchKernelGGL(k_diag<kGeoC3>, dim3((unsigned)nb), dim3(kDiagThreads), 0, s, a);
+  else hipLaunchKernelGGL(k_diag<kGeoCart>, dim3((unsigned)nb), dim3(kDiagThreads), 0, s, a);
   return hipGetLastError();
 }
 

@@ -7,6 +7,9 @@
 //   primitives_pfg          src/Fluid/smod_primitive_pfg.f90:12-59
 //   flux_euler_11pts        src/Compute_flow/smod_flux_euler_11pts.f90:25-172,267-,601-
 //   flux_euler_11pts_c3     src/Compute_flow/smod_flux_euler_11pts_c3.f90:25-38,139-171,763-773
+//   flux_euler_11pts_c      src/Compute_flow/smod_flux_euler_11pts_c.f90:25-37,265-277,527-560
+//                           (2.5-D: the 2-D metrics of a column are loaded once
+//                           for its whole k-march; z is Cartesian)
 //   flux_visc_5pts(_c3) divergence part  smod_flux_visc_5pts.f90:119-130, _c3.f90:262-
 //   update_var              src/Time/update.f90:32-44
 // With no-slip walls on j-faces (WALL): no Euler increment on the wall rows
@@ -68,12 +71,13 @@ struct Ptrs {
   double *uout[5];
   float *uout32[5];
   const double *m[10];
+  const double *m2[5];
   const double *kv[4];
   const double *idx, *idy, *idz;
   double ref[5];
 };
 
-template <bool CURV, bool VISC, bool IN32, bool OUT32>
+template <int GEO, bool VISC, bool IN32, bool OUT32>
 __device__ __forceinline__ void load_ptrs(const StageArgs &a, Ptrs &P) {
 #pragma unroll
   for (int v = 0; v < 5; v++) {
@@ -83,9 +87,13 @@ __device__ __forceinline__ void load_ptrs(const StageArgs &a, Ptrs &P) {
     else P.uout[v] = a.uout.v[v];
     P.u0[v] = a.u0.v[v];
   }
-  if constexpr (CURV) {
+  if constexpr (GEO == kGeoC3) {
 #pragma unroll
     for (int q = 0; q < 10; q++) P.m[q] = a.met.m[q];
+  } else if constexpr (GEO == kGeoC2) {
+#pragma unroll
+    for (int q = 0; q < 5; q++) P.m2[q] = a.met.m2[q];
+    P.idz = a.met.idz;
   } else {
     P.idx = a.met.idx;
     P.idy = a.met.idy;
@@ -219,8 +227,13 @@ __device__ __forceinline__ void flux_halo(const Prim &q, int dir, const double m
   else flux_r<CURV, MIX>(q, 1, m, rf, F);
 }
 
-template <int TX, int TY, bool CURV, bool VISC, int MODE, bool WALL>
+template <int TX, int TY, int GEO, bool VISC, int MODE, bool WALL>
 __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__restrict__ tab, int nb, int ntiles) {
+  // CURV: contravariant i/j fluxes (c3 or 2.5-D); C3: metrics at every point
+  // (and a contravariant k flux); C2: 2-D metrics per column, Cartesian k
+  constexpr bool CURV = GEO != kGeoCart;
+  constexpr bool C3 = GEO == kGeoC3;
+  constexpr bool C2 = GEO == kGeoC2;
   static_assert(!WALL || (!CURV && MODE == kFP64), "walls: Cartesian FP64 blocks");
   constexpr bool MIX = MODE != kFP64;
   constexpr bool IN32 = MODE == kMixMiddle || MODE == kMixLast;
@@ -242,7 +255,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
   // registers for the whole march instead of being re-read from the table
   // in every plane
   Ptrs P;
-  load_ptrs<CURV, VISC, IN32, OUT32>(a, P);
+  load_ptrs<GEO, VISC, IN32, OUT32>(a, P);
   const Geom &g = a.g;
   int tile_x, tile_y, chunk;
   tile_coords(e.t, w, tile_x, tile_y, chunk);
@@ -292,6 +305,32 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
   }
   const idx_t own_off = lin(g, i, j, 0);
   const idx_t sz = g.sy;
+  // 2.5-D: the metric rows of the halo cells and of the own column, and
+  // 1/J, constant along k: loaded once
+  double hm[NS][2];
+  double rxm[3] = {0, 0, 0}, rym[3] = {0, 0, 0};
+  double ijac = 0.0;
+  if constexpr (C2) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      hm[s][0] = hm[s][1] = 0.0;
+      if (hs_on[s]) {
+        const idx_t p2 = hs_off[s] - (idx_t)(G - 1) * sz;  // the cell's (i, j) column
+        // (constant indices only: a run-time index would move the pointer table to scratch)
+        const bool xd = hs_dir[s] == 0;
+        hm[s][0] = ldg(xd ? P.m2[0] : P.m2[2], p2);
+        hm[s][1] = ldg(xd ? P.m2[1] : P.m2[3], p2);
+      }
+    }
+    if (needx || needy) {
+      const idx_t p2 = lin2(g, i, j);
+      rxm[0] = ldg(P.m2[0], p2);
+      rxm[1] = ldg(P.m2[1], p2);
+      rym[0] = ldg(P.m2[2], p2);
+      rym[1] = ldg(P.m2[3], p2);
+      if (own) ijac = ldg(P.m2[4], p2);
+    }
+  }
   // walls: j-pencil entries beyond a wall hold 0 (their state is never valid),
   // and the row's closure coefficients
   bool hs_beyond[NS];
@@ -323,16 +362,16 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
 #pragma unroll
     for (int l = 0; l < 10; l++) {
       Raw r;
-      load_raw<CURV, IN32>(P, own_off + (idx_t)(kbeg - 5 + l) * sz, 2, r);
-      flux_r<CURV, MIX>(prim_raw(r.u, gam1), 2, r.m, rf, q[l]);
+      load_raw<C3, IN32>(P, own_off + (idx_t)(kbeg - 5 + l) * sz, 2, r);
+      flux_r<C3, MIX>(prim_raw(r.u, gam1), 2, r.m, rf, q[l]);
     }
-    load_raw<CURV, IN32>(P, own_off + (idx_t)(kbeg + 5) * sz, 2, rq);
+    load_raw<C3, IN32>(P, own_off + (idx_t)(kbeg + 5) * sz, 2, rq);
   }
-  // the own point's state and i/j metric terms, likewise one plane ahead
-  double rxm[3] = {0, 0, 0}, rym[3] = {0, 0, 0}, ru[5] = {0, 0, 0, 0, 0};
+  // the own point's state and (c3) i/j metric terms, likewise one plane ahead
+  double ru[5] = {0, 0, 0, 0, 0};
   auto load_own = [&](idx_t p) {
     load_u<IN32>(P, p, ru);
-    if constexpr (CURV) {
+    if constexpr (C3) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         rxm[c] = ldg(P.m[c], p);
@@ -349,19 +388,25 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
     // ---- batch 1: U^n and 1/J at k, halo cell 0 (queue point and own point: loaded ahead) ----
     Raw rh0;
     double u0[5] = {0, 0, 0, 0, 0};
-    double ijac = 0.0;
-    if (hs_on[0]) load_raw_halo<CURV, IN32>(P, hs_off[0] + (idx_t)k * sz, hs_dir[0], rh0);
+    if (hs_on[0]) {
+      load_raw_halo<C3, IN32>(P, hs_off[0] + (idx_t)k * sz, hs_dir[0], rh0);
+      if constexpr (C2) {
+        rh0.m[0] = hm[0][0];
+        rh0.m[1] = hm[0][1];
+        rh0.m[2] = 0.0;
+      }
+    }
     double kvis[4] = {0, 0, 0, 0};
     if (own) {
 #pragma unroll
       for (int v = 0; v < 5; v++) u0[v] = ldg_nt(P.u0[v], pk);
-      if constexpr (CURV) ijac = ldg_nt(P.m[9], pk);
+      if constexpr (C3) ijac = ldg_nt(P.m[9], pk);
       if constexpr (VISC) {
 #pragma unroll
         for (int c = 0; c < 4; c++) kvis[c] = ldg_nt(P.kv[c], pk);
       }
     }
-    if (own) flux_r<CURV, MIX>(prim_raw(rq.u, gam1), 2, rq.m, rf, q[10]);
+    if (own) flux_r<C3, MIX>(prim_raw(rq.u, gam1), 2, rq.m, rf, q[10]);
     if (needx || needy) {
       const Prim pq = prim_raw(ru, gam1);
       R F[5];
@@ -393,7 +438,14 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
       Raw rh[NS];
 #pragma unroll
       for (int s = 1; s < NS; s++)
-        if (hs_on[s]) load_raw_halo<CURV, IN32>(P, hs_off[s] + (idx_t)k * sz, hs_dir[s], rh[s]);
+        if (hs_on[s]) {
+          load_raw_halo<C3, IN32>(P, hs_off[s] + (idx_t)k * sz, hs_dir[s], rh[s]);
+          if constexpr (C2) {
+            rh[s].m[0] = hm[s][0];
+            rh[s].m[1] = hm[s][1];
+            rh[s].m[2] = 0.0;
+          }
+        }
 #pragma unroll
       for (int s = 1; s < NS; s++) {
         if (hs_on[s]) {
@@ -411,7 +463,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
     }
     __syncthreads();
     // queue point and own point of the next plane
-    if (own && k < kend) load_raw<CURV, IN32>(P, pk + 6 * sz, 2, rq);
+    if (own && k < kend) load_raw<C3, IN32>(P, pk + 6 * sz, 2, rq);
     if ((needx || needy) && k < kend) load_own(pk + sz);
 
     if (own) {
@@ -431,8 +483,11 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
                      R(A11_3) * (q[8][v] - q[2][v]) + R(A11_4) * (q[9][v] - q[1][v]) +
                      R(A11_5) * (q[10][v] - q[0][v]);
         // metrics and Jacobian multiply in FP64
-        if constexpr (CURV) {
+        if constexpr (C3) {
           K[v] = (double)(dz + (dy + dx)) * ijac;
+        } else if constexpr (C2) {
+          // (ksi + eta parts) * ijacob, then the Cartesian z part
+          K[v] = (double)dz * ldg(P.idz, (idx_t)(k - 1 + G)) + (double)(dy + dx) * ijac;
         } else {
           const double kx = (double)dx * ldg(P.idx, (idx_t)(i - 1 + G));
           const double ky = (double)dy * ldg(P.idy, (idx_t)(j - 1 + G)) + kx;
@@ -462,50 +517,48 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
   }
 }
 
+template <int TX, int TY, int GEO, int MODE>
+hipError_t launch_g(const Batch<StageArgs> *tab, int nb, int nt, bool visc, hipStream_t s) {
+  dim3 blk(TX * TY), grd(nt);
+  if constexpr (MODE == kFP64 || MODE == kMixLast) {
+    if (visc) {
+      hipLaunchKernelGGL((k_stage<TX, TY, GEO, true, MODE, false>), grd, blk, 0, s, tab, nb, nt);
+      return hipGetLastError();
+    }
+  }
+  hipLaunchKernelGGL((k_stage<TX, TY, GEO, false, MODE, false>), grd, blk, 0, s, tab, nb, nt);
+  return hipGetLastError();
+}
+
 template <int TX, int TY, int MODE>
-hipError_t launch_t(const Batch<StageArgs> *tab, int nb, int nt, bool curv, bool visc, bool wall, hipStream_t s) {
+hipError_t launch_t(const Batch<StageArgs> *tab, int nb, int nt, int geo, bool visc, bool wall, hipStream_t s) {
   dim3 blk(TX * TY), grd(nt);
   // the viscous increment is added in the last stage only
   constexpr bool VOK = MODE == kFP64 || MODE == kMixLast;
   if (visc && !VOK) return hipErrorInvalidValue;
   if (wall) {
     if constexpr (MODE == kFP64) {
-      if (curv) return hipErrorInvalidValue;
-      if (visc) hipLaunchKernelGGL((k_stage<TX, TY, false, true, MODE, true>), grd, blk, 0, s, tab, nb, nt);
-      else hipLaunchKernelGGL((k_stage<TX, TY, false, false, MODE, true>), grd, blk, 0, s, tab, nb, nt);
+      if (geo != kGeoCart) return hipErrorInvalidValue;
+      if (visc) hipLaunchKernelGGL((k_stage<TX, TY, kGeoCart, true, MODE, true>), grd, blk, 0, s, tab, nb, nt);
+      else hipLaunchKernelGGL((k_stage<TX, TY, kGeoCart, false, MODE, true>), grd, blk, 0, s, tab, nb, nt);
       return hipGetLastError();
     }
     return hipErrorInvalidValue;
   }
-  if (curv) {
-    if constexpr (VOK) {
-      if (visc) {
-        hipLaunchKernelGGL((k_stage<TX, TY, true, true, MODE, false>), grd, blk, 0, s, tab, nb, nt);
-        return hipGetLastError();
-      }
-    }
-    hipLaunchKernelGGL((k_stage<TX, TY, true, false, MODE, false>), grd, blk, 0, s, tab, nb, nt);
-  } else {
-    if constexpr (VOK) {
-      if (visc) {
-        hipLaunchKernelGGL((k_stage<TX, TY, false, true, MODE, false>), grd, blk, 0, s, tab, nb, nt);
-        return hipGetLastError();
-      }
-    }
-    hipLaunchKernelGGL((k_stage<TX, TY, false, false, MODE, false>), grd, blk, 0, s, tab, nb, nt);
-  }
-  return hipGetLastError();
+  if (geo == kGeoC3) return launch_g<TX, TY, kGeoC3, MODE>(tab, nb, nt, visc, s);
+  if (geo == kGeoC2) return launch_g<TX, TY, kGeoC2, MODE>(tab, nb, nt, visc, s);
+  return launch_g<TX, TY, kGeoCart, MODE>(tab, nb, nt, visc, s);
 }
 
 template <int MODE>
-hipError_t launch_shape(const Batch<StageArgs> *tab, int nb, int nt, bool curv, bool visc, int shape, bool wall,
+hipError_t launch_shape(const Batch<StageArgs> *tab, int nb, int nt, int geo, bool visc, int shape, bool wall,
                         hipStream_t s) {
   switch (shape) {
-    case 1: return launch_t<32, 8, MODE>(tab, nb, nt, curv, visc, wall, s);
-    case 2: return launch_t<64, 8, MODE>(tab, nb, nt, curv, visc, wall, s);
-    case 3: return launch_t<16, 16, MODE>(tab, nb, nt, curv, visc, wall, s);
-    case 4: return launch_t<32, 16, MODE>(tab, nb, nt, curv, visc, wall, s);
-    default: return launch_t<64, 4, MODE>(tab, nb, nt, curv, visc, wall, s);
+    case 1: return launch_t<32, 8, MODE>(tab, nb, nt, geo, visc, wall, s);
+    case 2: return launch_t<64, 8, MODE>(tab, nb, nt, geo, visc, wall, s);
+    case 3: return launch_t<16, 16, MODE>(tab, nb, nt, geo, visc, wall, s);
+    case 4: return launch_t<32, 16, MODE>(tab, nb, nt, geo, visc, wall, s);
+    default: return launch_t<64, 4, MODE>(tab, nb, nt, geo, visc, wall, s);
   }
 }
 
@@ -524,14 +577,14 @@ TileRange stage_tiles(const Geom &g, int shape, int kchunk) {
   return t;
 }
 
-hipError_t launch_stage(const Batch<StageArgs> *tab, int nb, int nt, bool curv, bool visc, int shape, int mode,
+hipError_t launch_stage(const Batch<StageArgs> *tab, int nb, int nt, int geo, bool visc, int shape, int mode,
                         bool wall, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
   switch (mode) {
-    case kMixFirst: return launch_shape<kMixFirst>(tab, nb, nt, curv, visc, shape, wall, s);
-    case kMixMiddle: return launch_shape<kMixMiddle>(tab, nb, nt, curv, visc, shape, wall, s);
-    case kMixLast: return launch_shape<kMixLast>(tab, nb, nt, curv, visc, shape, wall, s);
-    default: return launch_shape<kFP64>(tab, nb, nt, curv, visc, shape, wall, s);
+    case kMixFirst: return launch_shape<kMixFirst>(tab, nb, nt, geo, visc, shape, wall, s);
+    case kMixMiddle: return launch_shape<kMixMiddle>(tab, nb, nt, geo, visc, shape, wall, s);
+    case kMixLast: return launch_shape<kMixLast>(tab, nb, nt, geo, visc, shape, wall, s);
+    default: return launch_shape<kFP64>(tab, nb, nt, geo, visc, shape, wall, s);
   }
 }
 

@@ -54,8 +54,10 @@ __device__ __forceinline__ void ld5(const CState &U, idx_t p, double o[5]) {
   for (int v = 0; v < 5; v++) o[v] = ldg(U.v[v], p);
 }
 
-template <int TX, int TY, bool CURV>
+template <int TX, int TY, int GEO>
 __global__ __launch_bounds__(TX *TY, 2) void k_visc(const Batch<ViscArgs> *__restrict__ tab, int nb, int ntiles) {
+  constexpr bool CURV = GEO == kGeoC3;  // 3-D metrics at every point
+  constexpr bool C2 = GEO == kGeoC2;    // 2-D metrics of the column
   constexpr int NT = TX * TY;
   constexpr int NHX = 4 * TY;             // i-halo cells per plane
   constexpr int NH = NHX + 4 * TX;        // all halo cells per plane
@@ -123,6 +125,15 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(const Batch<ViscArgs> *__res
     }
   }
   const idx_t own_off = lin(g, i, j, 0);
+  // 2.5-D: the column's viscous metrics (y_eta, -x_eta, -y_ksi, x_ksi, ijacob),
+  // the same in every plane
+  double mc2[5] = {0, 0, 0, 0, 0};
+  if constexpr (C2) {
+    if (own) {
+#pragma unroll
+      for (int c = 0; c < 5; c++) mc2[c] = ldg(a.met.m2v[c], lin2(g, i, j));
+    }
+  }
 
   P5 q[5];  // own column, planes k-2..k+2
   if (own) {
@@ -203,7 +214,17 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(const Batch<ViscArgs> *__res
 
       // physical gradients gr[u,v,w,T][x,y,z]
       double gr[4][3];
-      if constexpr (CURV) {
+      if constexpr (C2) {
+        // (f_ksi y_eta - f_eta y_ksi) ijacob, (f_eta x_ksi - f_ksi x_eta) ijacob,
+        // f_z idz (smod_grad_vel_c.f90:186-217, :230-300)
+        const double iz = ldg(idzv, (idx_t)(k - 1 + G));
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          gr[c][0] = (gd[0][c] * mc2[0] + gd[1][c] * mc2[2]) * mc2[4];
+          gr[c][1] = (gd[0][c] * mc2[1] + gd[1][c] * mc2[3]) * mc2[4];
+          gr[c][2] = gd[2][c] * iz;
+        }
+      } else if constexpr (CURV) {
         const double ij = mvv[9];
 #pragma unroll
         for (int c = 0; c < 4; c++) {
@@ -271,7 +292,21 @@ __global__ __launch_bounds__(TX *TY, 2) void k_visc(const Batch<ViscArgs> *__res
         // Euler metrics (the Edoh switch, mod_filtering_shock.f90:627-630;
         // g3_ksi, smod_grid_metrics3_c3.f90:161)
         const double c = sqrt(a.gas.gam * rg * T);
-        if constexpr (CURV) {
+        if constexpr (C2) {
+          // ksi and eta from the 2-D Euler metrics (mod_filtering_shock.f90:650-652, :1089-1091;
+          // g_ksi, g_eta of grid_metrics_curv.f90:709-724), z Cartesian
+          const idx_t p2 = lin2(g, i, j);
+          const double ij = ldg(a.met.m2[4], p2);
+          double *const srp[2] = {sr0, sr1};
+#pragma unroll
+          for (int d = 0; d < 2; d++) {
+            const double mx = ldg(a.met.m2[2 * d], p2), my = ldg(a.met.m2[2 * d + 1], p2);
+            const double vc = (u * mx + v * my + w * 0.0) * ij;
+            const double g3 = sqrt(mx * mx + my * my + 0.0 * 0.0) * fabs(ij);
+            stg(srp[d], p, fabs(vc) + c * g3);
+          }
+          stg(sr2, p, fabs(w) + c);
+        } else if constexpr (CURV) {
           const double ij = ldg(a.met.m[9], p);
           double *const srp[3] = {sr0, sr1, sr2};
 #pragma unroll
@@ -330,14 +365,17 @@ __device__ __forceinline__ void vflux(const double *t, double mx, double my, dou
 
 // 5-pt differences of F^v along i and G^v along j from LDS pencils (tile +
 // 2-cell halos), of H^v along k from a 5-deep register queue; times
-// idx_v,idy_v,idz_v (Cartesian) or ijacob3_v (c3).  Interior points only.
-template <int TX, int TY, bool CURV>
+// idx_v,idy_v,idz_v (Cartesian), ijacob3_v (c3), or ijacob_v along i, j and
+// idz_v along k (2.5-D, smod_flux_visc_5pts_c.f90:52-72, :150-).  Interior
+// points only.
+template <int TX, int TY, int GEO>
 __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__restrict__ tab, int nb, int ntiles) {
+  constexpr bool CURV = GEO == kGeoC3;
+  constexpr bool C2 = GEO == kGeoC2;
   constexpr int NT = TX * TY;
   constexpr int NHX = 4 * TY;
   constexpr int NH = NHX + 4 * TX;
   constexpr int NS = (NH + NT - 1) / NT;
-  constexpr int NL = CURV ? NTAU : 4;  // tau values loaded per point and direction
   __shared__ double sfx[4][TY][TX + 4];
   __shared__ double sfy[4][TY + 4][TX];
 
@@ -370,7 +408,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
   const double *const *mv = mvp;
 
   int hs_r[NS], hs_c[NS];
-  idx_t hs_off[NS];
+  idx_t hs_off[NS], hs_off2[NS];
   bool hs_on[NS], hs_x[NS];
 #pragma unroll
   for (int s = 0; s < NS; s++) {
@@ -378,7 +416,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
     hs_on[s] = false;
     hs_x[s] = true;
     hs_r[s] = hs_c[s] = 0;
-    hs_off[s] = 0;
+    hs_off[s] = hs_off2[s] = 0;
     if (h < NHX) {
       const int r = h / 4, c = h % 4;
       const int li = c < 2 ? c : TX + c;
@@ -387,6 +425,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
       hs_r[s] = r;
       hs_c[s] = li;
       hs_off[s] = lin(g, gi, gj, 0);
+      hs_off2[s] = lin2(g, gi, gj);
     } else if (h < NH) {
       const int hh = h - NHX;
       const int r = hh / TX, c = hh % TX;
@@ -397,35 +436,48 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
       hs_r[s] = lj;
       hs_c[s] = c;
       hs_off[s] = lin(g, gi, gj, 0);
+      hs_off2[s] = lin2(g, gi, gj);
     }
   }
+  const idx_t own_off2 = lin2(g, i, j);
 
-  // tau values of direction d at element p, projected into f[4]
-  auto flux_at = [&](int d, idx_t p, double f[4]) {
-    double t[NL];
+  // flux of direction d at element p (column p2 of the 2-D arrays) from its
+  // nine tau values t9: the projection on the metric row of d (c3; 2.5-D
+  // along i, j) or row d of tau (Cartesian; 2.5-D along k)
+  auto proj = [&](int d, const double *t9, idx_t p, idx_t p2, double f[4]) {
+    if (CURV || (C2 && d < 2)) {
+      double mx, my, mz;
+      if constexpr (C2) {
+        mx = ldg(a.met.m2v[2 * d], p2);
+        my = ldg(a.met.m2v[2 * d + 1], p2);
+        mz = 0.0;
+      } else {
+        mx = ldg(mv[3 * d], p);
+        my = ldg(mv[3 * d + 1], p);
+        mz = ldg(mv[3 * d + 2], p);
+      }
+      vflux<true>(t9, mx, my, mz, f);
+    } else {
+      double t[4];
 #pragma unroll
-    for (int c = 0; c < NL; c++) t[c] = ldg(taup[CURV ? c : krow(d, c)], p);
-    double mx = 0.0, my = 0.0, mz = 0.0;
-    if constexpr (CURV) {
-      mx = ldg(mv[3 * d], p);
-      my = ldg(mv[3 * d + 1], p);
-      mz = ldg(mv[3 * d + 2], p);
+      for (int c = 0; c < 4; c++) t[c] = t9[krow(d, c)];
+      vflux<false>(t, 0.0, 0.0, 0.0, f);
     }
-    vflux<CURV>(t, mx, my, mz, f);
+  };
+  // tau values of direction d at element p, projected into f[4]
+  auto flux_at = [&](int d, idx_t p, idx_t p2, double f[4]) {
+    double t9[NTAU];
+    if (CURV || (C2 && d < 2)) {
+#pragma unroll
+      for (int c = 0; c < NTAU; c++) t9[c] = ldg(taup[c], p);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; c++) t9[krow(d, c)] = ldg(taup[krow(d, c)], p);
+    }
+    proj(d, t9, p, p2, f);
   };
   // the same from tau values already in registers (all nine components)
-  auto flux_of = [&](int d, const double *t9, idx_t p, double f[4]) {
-    double t[NL];
-#pragma unroll
-    for (int c = 0; c < NL; c++) t[c] = t9[CURV ? c : krow(d, c)];
-    double mx = 0.0, my = 0.0, mz = 0.0;
-    if constexpr (CURV) {
-      mx = ldg(mv[3 * d], p);
-      my = ldg(mv[3 * d + 1], p);
-      mz = ldg(mv[3 * d + 2], p);
-    }
-    vflux<CURV>(t, mx, my, mz, f);
-  };
+  auto flux_of = [&](int d, const double *t9, idx_t p, idx_t p2, double f[4]) { proj(d, t9, p, p2, f); };
 
   // own column: H^v at k-2..k+2, and tau of planes k, k+1, k+2 (each plane's
   // tau is loaded once, at k+2, and reused for the i/j pencils two planes on)
@@ -433,13 +485,13 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
   double tw[3][NTAU];
   if (own) {
 #pragma unroll
-    for (int l = 0; l < 2; l++) flux_at(2, own_off + (idx_t)(kbeg - 2 + l) * sz, q[l]);
+    for (int l = 0; l < 2; l++) flux_at(2, own_off + (idx_t)(kbeg - 2 + l) * sz, own_off2, q[l]);
 #pragma unroll
     for (int l = 0; l < 2; l++) {
       const idx_t p = own_off + (idx_t)(kbeg + l) * sz;
 #pragma unroll
       for (int c = 0; c < NTAU; c++) tw[l][c] = ldg(taup[c], p);
-      flux_of(2, tw[l], p, q[2 + l]);
+      flux_of(2, tw[l], p, own_off2, q[2 + l]);
     }
   }
   for (int k = kbeg; k <= kend; k++) {
@@ -450,19 +502,20 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
       const idx_t p2 = pk + 2 * sz;
 #pragma unroll
       for (int c = 0; c < NTAU; c++) tw[2][c] = ldg(taup[c], p2);
-      flux_of(2, tw[2], p2, q[4]);
+      flux_of(2, tw[2], p2, own_off2, q[4]);
       if constexpr (CURV) ijv = ldg(mv[9], pk);
-      flux_of(0, tw[0], pk, fx);
-      flux_of(1, tw[0], pk, fy);
+      if constexpr (C2) ijv = ldg(a.met.m2v[4], own_off2);
+      flux_of(0, tw[0], pk, own_off2, fx);
+      flux_of(1, tw[0], pk, own_off2, fy);
     } else {
-      if (needx) flux_at(0, pk, fx);
-      if (needy) flux_at(1, pk, fy);
+      if (needx) flux_at(0, pk, own_off2, fx);
+      if (needy) flux_at(1, pk, own_off2, fy);
     }
 #pragma unroll
     for (int s = 0; s < NS; s++)
       if (hs_on[s]) {
-        if (hs_x[s]) flux_at(0, hs_off[s] + (idx_t)k * sz, fh[s]);
-        else flux_at(1, hs_off[s] + (idx_t)k * sz, fh[s]);
+        if (hs_x[s]) flux_at(0, hs_off[s] + (idx_t)k * sz, hs_off2[s], fh[s]);
+        else flux_at(1, hs_off[s] + (idx_t)k * sz, hs_off2[s], fh[s]);
       }
     if (needx) {
 #pragma unroll
@@ -487,6 +540,9 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
       double mx, my, mz;
       if constexpr (CURV) {
         mx = my = mz = ijv;
+      } else if constexpr (C2) {
+        mx = my = ijv;
+        mz = ldg(idzv, (idx_t)(k - 1 + G));
       } else {
         mx = ldg(idxv, (idx_t)(i - 1 + G));
         my = ldg(idyv, (idx_t)(j - 1 + G));
@@ -519,17 +575,19 @@ __global__ __launch_bounds__(TX *TY, 2) void k_vdiv(const Batch<VdivArgs> *__res
 TileRange visc_tiles(const Geom &g, int kchunk) { return vtiles(g, kchunk); }
 TileRange vdiv_tiles(const Geom &g, int kchunk) { return vtiles(g, kchunk); }
 
-hipError_t launch_visc(const Batch<ViscArgs> *tab, int nb, int nt, bool curv, hipStream_t s) {
+hipError_t launch_visc(const Batch<ViscArgs> *tab, int nb, int nt, int geo, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  if (curv) hipLaunchKernelGGL((k_visc<VX, VY, true>), dim3(nt), dim3(VX * VY), 0, s, tab, nb, nt);
-  else hipLaunchKernelGGL((k_visc<VX, VY, false>), dim3(nt), dim3(VX * VY), 0, s, tab, nb, nt);
+  if (geo == kGeoC2) hipLaunchKernelGGL((k_visc<VX, VY, kGeoC2>), dim3(nt), dim3(VX * VY), 0, s, tab, nb, nt);
+  else if (geo == kGeoC3) hipLaunchKernelGGL((k_visc<VX, VY, kGeoC3>), dim3(nt), dim3(VX * VY), 0, s, tab, nb, nt);
+  else hipLaunchKernelGGL((k_visc<VX, VY, kGeoCart>), dim3(nt), dim3(VX * VY), 0, s, tab, nb, nt);
   return hipGetLastError();
 }
 
-hipError_t launch_vdiv(const Batch<VdivArgs> *tab, int nb, int nt, bool curv, hipStream_t s) {
+hipError_t launch_vdiv(const Batch<VdivArgs> *tab, int nb, int nt, int geo, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  if (curv) hipLaunchKernelGGL((k_vdiv<VX, VY, true>), dim3(nt), dim3(VX * VY), 0, s, tab, nb, nt);
-  else hipLaunchKernelGGL((k_vdiv<VX, VY, false>), dim3(nt), dim3(VX * VY), 0, s, tab, nb, nt);
+  if (geo == kGeoC2) hipLaunchKernelGGL((k_vdiv<VX, VY, kGeoC2>), dim3(nt), dim3(VX * VY), 0, s, tab, nb, nt);
+  else if (geo == kGeoC3) hipLaunchKernelGGL((k_vdiv<VX, VY, kGeoC3>), dim3(nt), dim3(VX * VY), 0, s, tab, nb, nt);
+  else hipLaunchKernelGGL((k_vdiv<VX, VY, kGeoCart>), dim3(nt), dim3(VX * VY), 0, s, tab, nb, nt);
   return hipGetLastError();
 }
 

@@ -113,6 +113,10 @@ Metrics metrics_of(const BlockData &bd) {
     m.m[q] = bd.m[q];
     m.mv[q] = bd.mv[q];
   }
+  for (int q = 0; q < 5; q++) {
+    m.m2[q] = bd.m2[q];
+    m.m2v[q] = bd.m2v[q];
+  }
   return m;
 }
 
@@ -425,6 +429,8 @@ void free_block(BlockData &bd) {
   for (auto &p : bd.idx) { if (p) (void)hipFree(p); p = nullptr; }
   for (auto &p : bd.m) { if (p) (void)hipFree(p); p = nullptr; }
   for (auto &p : bd.mv) { if (p) (void)hipFree(p); p = nullptr; }
+  for (auto &p : bd.m2) { if (p) (void)hipFree(p); p = nullptr; }
+  for (auto &p : bd.m2v) { if (p) (void)hipFree(p); p = nullptr; }
   for (auto &p : bd.tau) { if (p) (void)hipFree(p); p = nullptr; }
   for (auto &p : bd.kv) { if (p) (void)hipFree(p); p = nullptr; }
   if (bd.prs) (void)hipFree(bd.prs);
@@ -554,8 +560,11 @@ FilterArgs filter_args(const mgpu_handle *h, const BlockData &bd, int in, int ou
   fa.csens = bd.p.csens;
   fa.ref = bd.ref;
   for (int q = 0; q < 3; q++) fa.sr[q] = bd.sr[q];
-  fa.idx = bd.idx[0];
-  fa.idy = bd.idx[1];
+  // Cartesian metrics of the Edoh factor and the artificial viscosity (on a
+  // 2.5-D block the i, j spectral radii carry their metric already)
+  const bool c2 = h->c.curvilinear == MGPU_GEOM_C2;
+  fa.idx = c2 ? nullptr : bd.idx[0];
+  fa.idy = c2 ? nullptr : bd.idx[1];
   fa.idz = bd.idx[2];
   fa.dt = bd.p.deltat;
   fa.edoh = edoh_on(h->c);
@@ -700,7 +709,8 @@ WallViscArgs wall_visc_args(const mgpu_handle *h, const BlockData &bd, int s0) {
 // (runge_kutta.f90:20-24, 42-45, 130-136).
 int enqueue_step(mgpu_handle *h, int s0) {
   const mgpu_case &c = h->c;
-  const bool curv = c.curvilinear != 0, visc = c.iorder_visc != 0;
+  const int geo = c.curvilinear;
+  const bool visc = c.iorder_visc != 0;
   const int nb = (int)h->b.size(), nrk = c.nrk;
   const int sa = (s0 + 1) % 3, sb = (s0 + 2) % 3;
   int rc, in = s0;
@@ -709,7 +719,7 @@ int enqueue_step(mgpu_handle *h, int s0) {
     const int out = (irk % 2 == 1) ? sa : sb;
     if (last && visc) {
       ProfScope ps(h, MGPU_K_VISC);
-      HIPCK(h, launch_visc(h->d_visc + s0 * nb, nb, h->nt_visc, curv, h->stream));
+      HIPCK(h, launch_visc(h->d_visc + s0 * nb, nb, h->nt_visc, geo, h->stream));
       if (h->wall)
         for (const auto &bd : h->b)
           if (bd.g.jwall) HIPCK(h, launch_visc_wall(wall_visc_args(h, bd, s0), h->stream));
@@ -717,7 +727,7 @@ int enqueue_step(mgpu_handle *h, int s0) {
       if (h->wall)
         for (const auto &bd : h->b)
           if (bd.g.jwall) HIPCK(h, launch_tau_wallrow(wall_visc_args(h, bd, s0), h->stream));
-      HIPCK(h, launch_vdiv(h->d_vdiv, nb, h->nt_vdiv, curv, h->stream));
+      HIPCK(h, launch_vdiv(h->d_vdiv, nb, h->nt_vdiv, geo, h->stream));
       if (h->wall)
         for (const auto &bd : h->b)
           if (bd.g.jwall) HIPCK(h, launch_vdiv_wall(wall_visc_args(h, bd, s0), h->stream));
@@ -725,7 +735,7 @@ int enqueue_step(mgpu_handle *h, int s0) {
     {
       ProfScope ps(h, MGPU_K_STAGE);
       const int v = (last && visc) ? 1 : 0;
-      HIPCK(h, launch_stage(h->d_stage + ((size_t)s0 * nrk + (irk - 1)) * nb, nb, h->nt_stage[v], curv,
+      HIPCK(h, launch_stage(h->d_stage + ((size_t)s0 * nrk + (irk - 1)) * nb, nb, h->nt_stage[v], geo,
                             last && visc, h->shape[v], stage_mode(h, irk), h->wall, h->stream));
     }
     if (h->mixed && !last) {
@@ -809,6 +819,8 @@ std::string check_flags(const mgpu_case &c) {
   else if (c.iorder_visc != 0 && c.iorder_visc != 4)
     snprintf(buf, sizeof buf, "iorder_visc=%d: only 0 and 4 are supported", c.iorder_visc);
   else if (c.filter < 0 || c.filter > 3) snprintf(buf, sizeof buf, "filter=%d", c.filter);
+  else if (c.curvilinear < MGPU_GEOM_CART || c.curvilinear > MGPU_GEOM_C2)
+    snprintf(buf, sizeof buf, "curvilinear=%d: 0 Cartesian, 1 3-D, 2 2.5-D", c.curvilinear);
   else if (c.filter == MGPU_FILTER_SHOCK && c.iorder_visc == 0)
     snprintf(buf, sizeof buf, "shock filter needs iorder_visc=4 (its sensor reads the viscous gradients of U^n)");
   else if (c.filter == MGPU_FILTER_ARTVISC && c.iorder_visc == 0)
@@ -845,13 +857,14 @@ int handle_new(const mgpu_case &c, mgpu_handle **out) {
 size_t block_bytes(const mgpu_case &c, int nx, int ny, int nz) {
   const size_t n = (size_t)(nx + 2 * G) * (ny + 2 * G) * (nz + 2 * G);
   size_t arrays = 15;  // three state sets
-  if (c.curvilinear) arrays += c.iorder_visc ? 20 : 10;
+  if (c.curvilinear == MGPU_GEOM_C3) arrays += c.iorder_visc ? 20 : 10;
   if (c.iorder_visc)
     arrays += NTAU + 4 + (c.filter == MGPU_FILTER_SHOCK ? 2 : 0) +
               (c.filter == MGPU_FILTER_ARTVISC || edoh_on(c) ? 3 : 0);
   size_t bytes = arrays * n * sizeof(double);
   if (c.precision == MGPU_PREC_MIXED) bytes += 10 * n * sizeof(float);
-  if (!c.curvilinear) bytes += 6 * (size_t)(nx + ny + nz + 6 * G) * sizeof(double);
+  if (c.curvilinear != MGPU_GEOM_C3) bytes += 6 * (size_t)(nx + ny + nz + 6 * G) * sizeof(double);
+  if (c.curvilinear == MGPU_GEOM_C2) bytes += 10 * (size_t)(nx + 2 * G) * (ny + 2 * G) * sizeof(double);
   return bytes;  // (+ 4*nx*nz doubles of wall state for a block with walls)
 }
 
@@ -882,13 +895,19 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
   if (h->mixed)
     for (int s = 0; s < 2 && !rc; s++)
       for (int v = 0; v < 5 && !rc; v++) rc = dalloc32(h, &bd.sf[s][v], n);
-  if (c.curvilinear) {
+  if (c.curvilinear == MGPU_GEOM_C3) {
     for (int q = 0; q < 10 && !rc; q++) rc = dalloc(h, &bd.m[q], n);
     if (c.iorder_visc)
       for (int q = 0; q < 10 && !rc; q++) rc = dalloc(h, &bd.mv[q], n);
   } else {
+    // Cartesian metrics; a 2.5-D block uses idz, idz_v of them and 2-D arrays
     const int len[3] = {bl.nx + 2 * G, bl.ny + 2 * G, bl.nz + 2 * G};
     for (int q = 0; q < 6 && !rc; q++) rc = dalloc(h, &bd.idx[q], (size_t)len[q % 3]);
+    if (c.curvilinear == MGPU_GEOM_C2) {
+      const size_t n2 = (size_t)bd.g.sx * (bl.ny + 2 * G);
+      for (int q = 0; q < 5 && !rc; q++) rc = dalloc(h, &bd.m2[q], n2);
+      for (int q = 0; q < 5 && !rc; q++) rc = dalloc(h, &bd.m2v[q], n2);
+    }
   }
   if (bd.g.jwall && !rc) rc = dalloc(h, &bd.wstash, (size_t)4 * bl.nx * bl.nz);
   if (c.iorder_visc) {
@@ -927,7 +946,7 @@ int handle_remove_block(mgpu_handle *h, int i) {
 
 int put_metrics_cart(mgpu_handle *h, int block, const double *const *src) {
   if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
-  if (h->c.curvilinear) return fail(h, MGPU_ESTATE, "case is curvilinear: use mgpu_set_metrics_c3");
+  if (h->c.curvilinear) return fail(h, MGPU_ESTATE, "case is curvilinear: use mgpu_set_metrics_c3 or _c2");
   BlockData &bd = h->b[block];
   const int len[3] = {bd.g.nx + 2 * G, bd.g.ny + 2 * G, bd.g.nz + 2 * G};
   for (int q = 0; q < 6; q++) {
@@ -943,7 +962,8 @@ int put_metrics_cart(mgpu_handle *h, int block, const double *const *src) {
 
 int put_metrics_c3(mgpu_handle *h, int block, const double *const *m, const double *const *mv) {
   if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
-  if (!h->c.curvilinear) return fail(h, MGPU_ESTATE, "case is Cartesian: use mgpu_set_metrics_cart");
+  if (h->c.curvilinear != MGPU_GEOM_C3)
+    return fail(h, MGPU_ESTATE, "case is not 3-D curvilinear: use mgpu_set_metrics_cart or _c2");
   if (!m) return fail(h, MGPU_EINVAL, "null metric table");
   BlockData &bd = h->b[block];
   const size_t bytes = sizeof(double) * (size_t)bd.g.n;
@@ -957,6 +977,40 @@ int put_metrics_c3(mgpu_handle *h, int block, const double *const *m, const doub
       if (!mv[q]) return fail(h, MGPU_EINVAL, "viscous metric %d is null", q);
       HIPCK(h, hipMemcpy(bd.mv[q], mv[q], bytes, hipMemcpyHostToDevice));
     }
+  }
+  bd.have_metrics = true;
+  return 0;
+}
+
+int put_metrics_c2(mgpu_handle *h, int block, const double *const *m2, const double *const *m2v, const double *idz,
+                   const double *idz_v) {
+  if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
+  if (h->c.curvilinear != MGPU_GEOM_C2)
+    return fail(h, MGPU_ESTATE, "case is not 2.5-D curvilinear: use mgpu_set_metrics_cart or _c3");
+  if (!m2 || !idz) return fail(h, MGPU_EINVAL, "null metric table");
+  BlockData &bd = h->b[block];
+  const size_t n2 = (size_t)bd.g.sx * (bd.g.ny + 2 * G);
+  // stored as the flux metric rows: ksi (y_eta, -x_eta), eta (-y_ksi, x_ksi), ijacob
+  auto up = [&](const double *const *src, double *const *dst) -> int {
+    for (int q = 0; q < 5; q++)
+      if (!src[q]) return fail(h, MGPU_EINVAL, "2-D metric %d is null", q);
+    std::vector<double> row(n2);
+    const int from[5] = {3, 1, 2, 0, 4};
+    const double sign[5] = {1.0, -1.0, -1.0, 1.0, 1.0};
+    for (int q = 0; q < 5; q++) {
+      for (size_t e = 0; e < n2; e++) row[e] = sign[q] * src[from[q]][e];
+      HIPCK(h, hipMemcpy(dst[q], row.data(), n2 * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return 0;
+  };
+  int rc = up(m2, bd.m2);
+  if (rc) return rc;
+  const size_t nz = (size_t)bd.g.nz + 2 * G;
+  HIPCK(h, hipMemcpy(bd.idx[2], idz, nz * sizeof(double), hipMemcpyHostToDevice));
+  if (h->c.iorder_visc) {
+    if (!m2v || !idz_v) return fail(h, MGPU_EINVAL, "viscous metric table is null");
+    if ((rc = up(m2v, bd.m2v))) return rc;
+    HIPCK(h, hipMemcpy(bd.idx[5], idz_v, nz * sizeof(double), hipMemcpyHostToDevice));
   }
   bd.have_metrics = true;
   return 0;
@@ -1087,7 +1141,7 @@ int diagnostics(mgpu_handle *h, mgpu_diag *out) {
     da.gam = bd.p.gam;
     da.gam1 = bd.p.gam - 1.0;
     da.partial = h->diag_buf;
-    HIPCK(h, launch_diag(da, h->c.curvilinear != 0, h->stream));
+    HIPCK(h, launch_diag(da, h->c.curvilinear, h->stream));
     HIPCK(h, hipStreamSynchronize(h->stream));
     part.resize(4 * ng);
     HIPCK(h, hipMemcpy(part.data(), h->diag_buf, sizeof(double) * 4 * ng, hipMemcpyDeviceToHost));
@@ -1110,7 +1164,7 @@ int diagnostics(mgpu_handle *h, mgpu_diag *out) {
 // stage buffer, never into U^n).
 int tune_aux(mgpu_handle *h, bool force, hipEvent_t e0, hipEvent_t e1) {
   const int nb = (int)h->b.size();
-  const bool curv = h->c.curvilinear != 0;
+  const int geo = h->c.curvilinear;
   const int s0 = h->cur, sa = (h->cur + 1) % 3;
   int maxnz = 0;
   for (const auto &bd : h->b) maxnz = bd.g.nz > maxnz ? bd.g.nz : maxnz;
@@ -1149,8 +1203,8 @@ int tune_aux(mgpu_handle *h, bool force, hipEvent_t e0, hipEvent_t e1) {
         HIPCK(h, hipMemcpy(h->d_tune_f, tab.data(), nb * sizeof(tab[0]), hipMemcpyHostToDevice));
       }
       auto launch = [&]() -> hipError_t {
-        if (kind == 0) return launch_visc(h->d_visc + s0 * nb, nb, h->nt_visc, curv, h->stream);
-        if (kind == 1) return launch_vdiv(h->d_vdiv, nb, h->nt_vdiv, curv, h->stream);
+        if (kind == 0) return launch_visc(h->d_visc + s0 * nb, nb, h->nt_visc, geo, h->stream);
+        if (kind == 1) return launch_vdiv(h->d_vdiv, nb, h->nt_vdiv, geo, h->stream);
         if (h->c.filter == MGPU_FILTER_SF)
           return launch_filter_sf(h->d_tune_f, nb, h->nt_filt, h->mixed, h->wall, h->stream);
         return launch_filter_shock(h->d_tune_f, nb, h->nt_filt, h->mixed, h->c.filter == MGPU_FILTER_ARTVISC,
@@ -1235,6 +1289,12 @@ int mgpu_set_metrics_c3(mgpu_handle *h, int block, const double *const *m, const
   return put_metrics_c3(h, block, m, mv);
 }
 
+int mgpu_set_metrics_c2(mgpu_handle *h, int block, const double *const *m2, const double *const *m2v,
+                        const double *idz, const double *idz_v) {
+  if (!h) return MGPU_EINVAL;
+  return put_metrics_c2(h, block, m2, m2v, idz, idz_v);
+}
+
 int mgpu_set_wall(mgpu_handle *h, int block, const double *cextrp2, const double *cextrp3) {
   if (!h) return MGPU_EINVAL;
   if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
@@ -1286,7 +1346,7 @@ int mgpu_autotune(mgpu_handle *h, int flags) {
   if (rc) return rc;
   const int nb = (int)h->b.size();
   const int s0 = h->cur, sa = (h->cur + 1) % 3;
-  const bool curv = h->c.curvilinear != 0;
+  const int geo = h->c.curvilinear;
   hipEvent_t e0 = get_event(h), e1 = get_event(h);
   if (!e0 || !e1) return fail(h, MGPU_EDEVICE, "hipEventCreate failed");
   for (int v = 0; v < (h->c.iorder_visc ? 2 : 1) && !rc; v++) {
@@ -1326,10 +1386,10 @@ int mgpu_autotune(mgpu_handle *h, int flags) {
         const int mode = stage_mode(h, irk);
         hipError_t e = hipMemcpy(h->d_tune, tab.data(), nb * sizeof(tab[0]), hipMemcpyHostToDevice);
         // one warm-up launch, then three timed ones
-        if (e == hipSuccess) e = launch_stage(h->d_tune, nb, nt, curv, v == 1, sh, mode, h->wall, h->stream);
+        if (e == hipSuccess) e = launch_stage(h->d_tune, nb, nt, geo, v == 1, sh, mode, h->wall, h->stream);
         if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
         for (int r = 0; r < 3 && e == hipSuccess; r++)
-          e = launch_stage(h->d_tune, nb, nt, curv, v == 1, sh, mode, h->wall, h->stream);
+          e = launch_stage(h->d_tune, nb, nt, geo, v == 1, sh, mode, h->wall, h->stream);
         if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.f;

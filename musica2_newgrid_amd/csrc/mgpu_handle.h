@@ -21,6 +21,8 @@ struct BlockData {
   double *idx[6] = {};    // Cartesian metrics idx,idy,idz,idx_v,idy_v,idz_v
   double *m[10] = {};
   double *mv[10] = {};
+  double *m2[5] = {};     // 2.5-D: 2-D metric rows (y_eta, -x_eta, -y_ksi, x_ksi, ijacob), Euler set
+  double *m2v[5] = {};    // 2.5-D: the viscous set
   double *tau[NTAU] = {}; // physical viscous flux tensor of U^n (last stage)
   double *kv[4] = {};     // viscous increment of rhou..rhoe (last stage)
   double *prs = nullptr, *sens = nullptr;
@@ -111,6 +113,8 @@ size_t block_bytes(const mgpu_case &c, int nx, int ny, int nz);
 // Uploads metrics / state of block i (host arrays in the padded layout).
 int put_metrics_cart(mgpu_handle *h, int i, const double *const *a6);
 int put_metrics_c3(mgpu_handle *h, int i, const double *const *m, const double *const *mv);
+int put_metrics_c2(mgpu_handle *h, int i, const double *const *m2, const double *const *m2v, const double *idz,
+                   const double *idz_v);
 int put_state(mgpu_handle *h, int i, const double *const *u5);
 int get_state(mgpu_handle *h, int i, double *const *u5);
 // Advances every block of the handle by nsteps (asynchronous on h->stream).

@@ -165,7 +165,17 @@ struct Metrics {
   // c3 padded 3-D arrays
   const double *m[10];
   const double *mv[10];
+  // 2.5-D padded 2-D arrays (index lin2): the ksi-flux metric row (y_eta,
+  // -x_eta), the eta-flux row (-y_ksi, x_ksi), ijacob; Euler and viscous
+  // sets.  z uses idz / idz_v.
+  const double *m2[5];
+  const double *m2v[5];
 };
+
+// Offset of column (i, j) in a padded 2-D (i, j) array.
+__host__ __device__ inline idx_t lin2(const Geom &g, int i, int j) {
+  return (idx_t)(i + G - 1) + (idx_t)g.sx * (idx_t)(j + G - 1);
+}
 
 // Reference state of the mixed-precision path: the FP32 arithmetic works on
 // fluxes and states minus their values at this constant state.  The 11-pt
@@ -198,6 +208,11 @@ struct StageArgs {
 // and written as FP64 (U^n, the last stage's output) or as FP32 differences
 // with the reference state (the intermediate stages).
 enum StageMode { kFP64 = 0, kMixFirst = 1, kMixMiddle = 2, kMixLast = 3 };
+
+// Geometry of a case (mgpu_case.curvilinear): Cartesian 1-D metrics, full
+// 3-D curvilinear metrics (is_curv3), or 2.5-D curvilinear (is_curv: 2-D
+// metrics in i, j and a Cartesian z).
+enum Geometry { kGeoCart = 0, kGeoC3 = 1, kGeoC2 = 2 };
 
 // Viscous flux tensor of U^n in the physical frame, per interior point:
 // tau[0..5] = -tau_xx,-tau_xy,-tau_xz,-tau_yy,-tau_yz,-tau_zz (the
@@ -329,12 +344,12 @@ namespace mgpu {
 extern const int kStageShapes[][2];
 extern const int kNumStageShapes;
 TileRange stage_tiles(const Geom &g, int shape, int kchunk);
-hipError_t launch_stage(const Batch<StageArgs> *tab, int nb, int ntiles, bool curv, bool visc, int shape,
+hipError_t launch_stage(const Batch<StageArgs> *tab, int nb, int ntiles, int geo, bool visc, int shape,
                         int mode, bool wall, hipStream_t s);
 TileRange visc_tiles(const Geom &g, int kchunk);  // kchunk 0: default
-hipError_t launch_visc(const Batch<ViscArgs> *tab, int nb, int ntiles, bool curv, hipStream_t s);
+hipError_t launch_visc(const Batch<ViscArgs> *tab, int nb, int ntiles, int geo, hipStream_t s);
 TileRange vdiv_tiles(const Geom &g, int kchunk);
-hipError_t launch_vdiv(const Batch<VdivArgs> *tab, int nb, int ntiles, bool curv, hipStream_t s);
+hipError_t launch_vdiv(const Batch<VdivArgs> *tab, int nb, int ntiles, int geo, hipStream_t s);
 TileRange filter_tiles(const Geom &g, int kchunk);
 hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, bool wall,
                             hipStream_t s);
@@ -353,5 +368,5 @@ hipError_t launch_wall(const WallArgs &a, hipStream_t s);
 hipError_t launch_visc_wall(const WallViscArgs &a, hipStream_t s);
 hipError_t launch_tau_wallrow(const WallViscArgs &a, hipStream_t s);
 hipError_t launch_vdiv_wall(const WallViscArgs &a, hipStream_t s);
-hipError_t launch_diag(const DiagArgs &a, bool curv, hipStream_t s);
+hipError_t launch_diag(const DiagArgs &a, int geo, hipStream_t s);
 }  // namespace mgpu

@@ -146,6 +146,8 @@ int mgpu_server_submit(mgpu_server *s, const mgpu_case *c, const double *const *
   if (c->nblocks != 1 || !c->blocks) return sfail(s, MGPU_EINVAL, "a request is one block (nblocks = 1)");
   const std::string msg = check_flags(*c);
   if (!msg.empty()) return sfail(s, MGPU_EINVAL, msg);
+  if (c->curvilinear == MGPU_GEOM_C2)
+    return sfail(s, MGPU_EUNSUP, "requests are Cartesian or 3-D curvilinear (mgpu_set_metrics_c3 layout)");
   const mgpu_block &bl = c->blocks[0];
   for (int f = 0; f < 6; f++)
     if (bl.face_kind[f] != MGPU_FACE_PERIODIC)

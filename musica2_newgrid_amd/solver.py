@@ -98,7 +98,15 @@ class GpuSolver:
 
     def _set_metrics(self, b, c):
         L = lib()
-        if c.curvilinear:
+        if int(c.curvilinear) == _lib.GEOM_C2:
+            m2 = [np.ascontiguousarray(a, dtype=np.float64) for a in c.m2]
+            m2v = [np.ascontiguousarray(a, dtype=np.float64) for a in (c.m2v if c.m2v is not None else c.m2)]
+            tab = (_lib._dp * 5)(*[dptr(a) for a in m2])
+            tabv = (_lib._dp * 5)(*[dptr(a) for a in m2v])
+            idz = np.ascontiguousarray(c.cart[2], dtype=np.float64)
+            idzv = np.ascontiguousarray(c.cart[5], dtype=np.float64)
+            check(L.mgpu_set_metrics_c2(self.h, b, tab, tabv, dptr(idz), dptr(idzv)), self.h, "mgpu_set_metrics_c2")
+        elif c.curvilinear:
             m = [np.ascontiguousarray(a) for a in c.m]
             tab = (_lib._dp * 10)(*[dptr(a) for a in m])
             tabv = None

@@ -676,7 +676,7 @@ static void filter_shock(const orc_cfg *c, const orc_grid *g, const dims_t *d, d
               double cfl;
               if (c->curv == 1 || (c->curv == 2 && dir < 2)) {
                 /* 2.5-D: vc = (u y_eta - v x_eta) ijacob, g_ksi = |grad ksi| |ijacob|
-                 * (mod_filtering_shock.f90:645-652; grid_metrics_curv.f90:683-685) */
+                 * (mod_filtering_shock.f90:650-652, :1089-1091; grid_metrics_curv.f90:709-724) */
                 const double mx = mtr(c, g->m, g->m2, d, 3 * dir, p), my = mtr(c, g->m, g->m2, d, 3 * dir + 1, p),
                              mz = mtr(c, g->m, g->m2, d, 3 * dir + 2, p);
                 const double ij = mtr(c, g->m, g->m2, d, 9, p);
