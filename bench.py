@@ -37,10 +37,11 @@ def stage_words_per_cell(case) -> float:
     """Algorithmic FP64 words per interior cell of one stage-kernel launch,
     averaged over the nrk stages: stage 1 reads U^n and writes U1 (10 W),
     later stages also read U^n (15 W); curvilinear metrics add 10 W per stage
-    (9 metric terms and 1/J); the viscous last stage reads the 4 viscous
-    increments (4 W)."""
+    (9 metric terms and 1/J) on a 3-D curvilinear block, nothing per cell on a
+    2.5-D one (its 2-D metrics are read once per column); the viscous last
+    stage reads the 4 viscous increments (4 W)."""
     w = [10.0] + [15.0] * (case.nrk - 1)
-    if case.curvilinear:
+    if int(case.curvilinear) == 1:
         w = [x + 10.0 for x in w]
     if case.iorder_visc:
         w[-1] += 4.0
@@ -183,6 +184,42 @@ def channel_leg(steps: int, warmup: int):
             "kernel_ms_per_step_eager": {k: round(ms / 2, 4) for k, (n, ms) in times.items()}}
 
 
+def c2_leg(nx: int, ny: int, nz: int, steps: int, warmup: int, tune: bool, c3_value: float, mixed: bool):
+    """The same block and flow run as 2.5-D curvilinear (the reference's
+    is_curv: 2-D metrics in (i, j), Cartesian z), which a z-extruded wavy grid
+    allows: FP64 and mixed, each with the stage kernel's roofline."""
+    import numpy as np
+
+    from musica2_newgrid_amd import _lib, cases
+    case = cases.vortex_wavy_c2(nx, ny, nz, filter=2)
+    res = {"workload": f"2.5-D curvilinear block {nx}x{ny}x{nz} (x = xi + ax sin(2pi eta/Ly), "
+                       "y = eta + ay sin(2pi xi/Lx), extruded in z), isentropic vortex M=0.5, viscous, "
+                       "shock filter + Ducros, RK4"}
+    legs = [("fp64", _lib.PREC_FP64)] + ([("mixed", _lib.PREC_MIXED)] if mixed else [])
+    rho = {}
+    for name, prec in legs:
+        r = run_leg(case, prec, steps, warmup, tune)
+        nst, ms_stage = r["times"]["stage"]
+        avg = ms_stage / nst / 1e3
+        byts = stage_words_per_cell(case) * 8.0 * case.ncells
+        val = case.ncells * case.nrk * steps / r["t"] / 1e6
+        res[name] = {"value": round(val, 2), "unit": "Mcell-RK-stages/s",
+                     "ms_per_step": round(r["t"] / steps * 1e3, 4),
+                     "stage_tiles": {k: {"tile": f"{v[0]}x{v[1]}", "kchunk": v[2]} for k, v in r["tuned"].items()},
+                     "stage_roofline": {"achieved_GBs": round(byts / avg / 1e9, 1),
+                                        "frac": round(byts / avg / 1e9 / HBM_PEAK_GBS, 4),
+                                        "avg_launch_ms": round(avg * 1e3, 4),
+                                        "algorithmic_bytes_per_launch": byts},
+                     "kernel_ms": {k: {"launches": n, "ms": round(ms, 3)} for k, (n, ms) in r["times"].items()}}
+        rho[name] = r["rho"][5:-5, 5:-5, 5:-5]
+    res["fp64"]["speedup_vs_c3_fp64"] = round(res["fp64"]["value"] / c3_value, 3)
+    if mixed:
+        a, b = rho["mixed"], rho["fp64"]
+        res["mixed"]["speedup_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
+        res["mixed"]["rho_linf_vs_fp64"] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    return res
+
+
 def serve_leg(nreq: int, nsteps: int, seed: int = 5):
     """Serving loop: a Poisson stream of TGV / isentropic-vortex requests on
     cubes of edge 64..256, Re log-uniform in [400, 4000], FP64 or mixed,
@@ -221,6 +258,7 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--no-tgv", action="store_true", help="skip the TGV 64^3 leg")
     ap.add_argument("--no-channel", action="store_true", help="skip the walled-channel leg")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 2.5-D curvilinear leg")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-steps", type=int, default=100)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -288,6 +326,10 @@ def main() -> int:
             "after_steps": args.warmup + args.steps,
             "kernel_ms": {k: {"launches": n, "ms": round(ms, 3)} for k, (n, ms) in mx["times"].items()},
         }
+    if not args.no_c2:
+        print("bench: 2.5-D curvilinear leg", file=sys.stderr, flush=True)
+        out["c2"] = c2_leg(args.nx, args.ny, args.nz, args.steps, args.warmup, not args.no_tune, value,
+                           not args.no_mixed)
     if not args.no_tgv:
         print("bench: TGV 64^3 leg", file=sys.stderr, flush=True)
         out["tgv64"] = tgv_leg()

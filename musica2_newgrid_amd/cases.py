@@ -250,8 +250,8 @@ def vortex_wavy_c2(nx: int = 512, ny: int = 256, nz: int = 128, mach: float = 0.
     if m2v is None:
         m2v = [a.copy() for a in m2]
     dz = Lz / nz
-    xc = np.broadcast_to(x2, (nz, ny, nx))
-    yc = np.broadcast_to(y2, (nz, ny, nx))
+    # the flow does not depend on z: fields are built on (j, i) and extruded
+    xc, yc = x2[None], y2[None]
     c_inf = math.sqrt(GAM * RG * T_inf)
     u_inf = mach * c_inf
     xv, yv = Lx / 2.0, Ly / 2.0
@@ -263,7 +263,7 @@ def vortex_wavy_c2(nx: int = 512, ny: int = 256, nz: int = 128, mach: float = 0.
     T = T_inf * (1.0 - (GAM - 1.0) * beta ** 2 * mach ** 2 / (8.0 * math.pi ** 2) * e * e)
     rho = rho_inf * (T / T_inf) ** (1.0 / (GAM - 1.0))
     p = rho * RG * T
-    U = _to_padded(conserved(rho, u, v, w, p), nx, ny, nz)
+    U = _to_padded([np.broadcast_to(f, (nz, ny, nx)) for f in conserved(rho, u, v, w, p)], nx, ny, nz)
     mu_ref = rho_inf * u_inf * radius / re
     diffscale = mu_ref / float(sutherland(np.float64(T_inf)))
     dmin = min(float(np.min(np.sqrt(np.diff(x2, axis=a) ** 2 + np.diff(y2, axis=a) ** 2))) for a in (0, 1))
@@ -273,7 +273,7 @@ def vortex_wavy_c2(nx: int = 512, ny: int = 256, nz: int = 128, mach: float = 0.
     cs = Case("vortex_wavy_c2", nx, ny, nz, curvilinear=2, iorder_visc=iorder_visc, filter=filter,
               ducros=True, diffscale=diffscale, m2=m2, m2v=m2v, U=U,
               cart=[ones_x, ones_y, idz, ones_x.copy(), ones_y.copy(), idz.copy()])
-    cs.dt = timestep(U, dmin, cfl)
+    cs.dt = cfl * dmin / float(np.max(np.sqrt(u * u + v * v + w * w) + np.sqrt(GAM * p / rho)))
     cs.info = dict(mach=mach, beta=beta, radius=radius, re=re, cfl=cfl, lengths=lengths, amp=amp,
                    dmin=dmin)
     return cs
