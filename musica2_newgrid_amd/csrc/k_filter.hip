@@ -109,22 +109,24 @@ __device__ __forceinline__ void ldr(const CState &U, idx_t p, const RefState &rf
 // Filter increment along j of rows next to a wall: nothing on the first two,
 // the 5/7/9-pt filters (times -chi) on the next three; y points at the row
 // with stride s.  Reads nothing beyond the wall.
-__device__ __forceinline__ double fwall(const double *y, int s, int hw, double chi) {
+template <class R>
+__device__ __forceinline__ R fwall(const R *y, int s, int hw, double chi) {
+  const R c = R(-chi);
   switch (hw) {
-    case 2: return -chi * (D5_0 * y[0] + D5_1 * (y[s] + y[-s]) + D5_2 * (y[2 * s] + y[-2 * s]));
+    case 2: return c * (R(D5_0) * y[0] + R(D5_1) * (y[s] + y[-s]) + R(D5_2) * (y[2 * s] + y[-2 * s]));
     case 3:
-      return -chi * (D7_0 * y[0] + D7_1 * (y[s] + y[-s]) + D7_2 * (y[2 * s] + y[-2 * s]) +
-                     D7_3 * (y[3 * s] + y[-3 * s]));
+      return c * (R(D7_0) * y[0] + R(D7_1) * (y[s] + y[-s]) + R(D7_2) * (y[2 * s] + y[-2 * s]) +
+                  R(D7_3) * (y[3 * s] + y[-3 * s]));
     case 4:
-      return -chi * (D9_0 * y[0] + D9_1 * (y[s] + y[-s]) + D9_2 * (y[2 * s] + y[-2 * s]) +
-                     D9_3 * (y[3 * s] + y[-3 * s]) + D9_4 * (y[4 * s] + y[-4 * s]));
-    default: return 0.0;
+      return c * (R(D9_0) * y[0] + R(D9_1) * (y[s] + y[-s]) + R(D9_2) * (y[2 * s] + y[-2 * s]) +
+                  R(D9_3) * (y[3 * s] + y[-3 * s]) + R(D9_4) * (y[4 * s] + y[-4 * s]));
+    default: return R(0);
   }
 }
 
 template <bool MIX, bool WALL>
 __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__restrict__ tab, int nb, int ntiles) {
-  static_assert(!(MIX && WALL), "walls: FP64 only");
+
   using R = typename std::conditional<MIX, float, double>::type;
   __shared__ R sux[5][FY][FX + 10];
   __shared__ R suy[5][FY + 10][FX];
@@ -210,7 +212,7 @@ __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__
               dd[3] * (y[3 * FX] + y[-3 * FX]) + dd[4] * (y[4 * FX] + y[-4 * FX]) +
               dd[5] * (y[5 * FX] + y[-5 * FX]);
         } else {
-          if constexpr (WALL) K = K + fwall(y, FX, hw, a.chi);
+          if constexpr (WALL) K = K + fwall<R>(y, FX, hw, a.chi);
         }
         K = K + dd[0] * q[5][v] + dd[1] * (q[6][v] + q[4][v]) + dd[2] * (q[7][v] + q[3][v]) +
             dd[3] * (q[8][v] + q[2][v]) + dd[4] * (q[9][v] + q[1][v]) + dd[5] * (q[10][v] + q[0][v]);
@@ -282,7 +284,7 @@ template <bool MIX, bool ARTV, bool WALL, bool EDOH>
 __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> *__restrict__ tab, int nb,
                                                         int ntiles) {
   static_assert(!(MIX && ARTV), "artificial viscosity: FP64 only");
-  static_assert(!(WALL && (MIX || ARTV)), "walls: FP64 shock filter only");
+  static_assert(!(WALL && ARTV), "walls: not with the artificial viscosity");
   static_assert(!(EDOH && ARTV), "the Edoh switch belongs to the shock filter");
   using R = typename std::conditional<MIX, float, double>::type;
   __shared__ R sux[5][FY][FX + 10];
@@ -665,8 +667,8 @@ TileRange filter_tiles(const Geom &g, int kc_req) {
 hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int nt, bool mixed, bool wall, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
   if (wall) {
-    if (mixed) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_filter_sf<false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    if (mixed) hipLaunchKernelGGL((k_filter_sf<true, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    else hipLaunchKernelGGL((k_filter_sf<false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   } else if (mixed) {
     hipLaunchKernelGGL((k_filter_sf<true, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   } else {
@@ -677,7 +679,9 @@ hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int nt, bool m
 
 template <bool EDOH>
 static void launch_shock_variant(const Batch<FilterArgs> *tab, int nb, int nt, bool mixed, bool wall, hipStream_t s) {
-  if (wall)
+  if (wall && mixed)
+    hipLaunchKernelGGL((k_filter_shock<true, false, true, EDOH>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  else if (wall)
     hipLaunchKernelGGL((k_filter_shock<false, false, true, EDOH>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   else if (mixed)
     hipLaunchKernelGGL((k_filter_shock<true, false, false, EDOH>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
@@ -692,7 +696,6 @@ hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int nt, boo
     if (mixed || wall || edoh) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_filter_shock<false, true, false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   } else {
-    if (mixed && wall) return hipErrorInvalidValue;
     if (edoh) launch_shock_variant<true>(tab, nb, nt, mixed, wall, s);
     else launch_shock_variant<false>(tab, nb, nt, mixed, wall, s);
   }

@@ -234,7 +234,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
   constexpr bool CURV = GEO != kGeoCart;
   constexpr bool C3 = GEO == kGeoC3;
   constexpr bool C2 = GEO == kGeoC2;
-  static_assert(!WALL || (!CURV && MODE == kFP64), "walls: Cartesian FP64 blocks");
+  static_assert(!WALL || !CURV, "walls: Cartesian blocks");
   constexpr bool MIX = MODE != kFP64;
   constexpr bool IN32 = MODE == kMixMiddle || MODE == kMixLast;
   constexpr bool OUT32 = MODE == kMixFirst || MODE == kMixMiddle;
@@ -474,8 +474,8 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
         R dy;
         if constexpr (WALL) {
           const R *f = &sfy[v][ty + 5][tx];
-          dy = cw[0] * (f[TX] - f[-TX]) + cw[1] * (f[2 * TX] - f[-2 * TX]) + cw[2] * (f[3 * TX] - f[-3 * TX]) +
-               cw[3] * (f[4 * TX] - f[-4 * TX]) + cw[4] * (f[5 * TX] - f[-5 * TX]);
+          dy = R(cw[0]) * (f[TX] - f[-TX]) + R(cw[1]) * (f[2 * TX] - f[-2 * TX]) + R(cw[2]) * (f[3 * TX] - f[-3 * TX]) +
+               R(cw[3]) * (f[4 * TX] - f[-4 * TX]) + R(cw[4]) * (f[5 * TX] - f[-5 * TX]);
         } else {
           dy = d11c(&sfy[v][ty + 5][tx], TX);
         }
@@ -537,13 +537,15 @@ hipError_t launch_t(const Batch<StageArgs> *tab, int nb, int nt, int geo, bool v
   constexpr bool VOK = MODE == kFP64 || MODE == kMixLast;
   if (visc && !VOK) return hipErrorInvalidValue;
   if (wall) {
-    if constexpr (MODE == kFP64) {
-      if (geo != kGeoCart) return hipErrorInvalidValue;
-      if (visc) hipLaunchKernelGGL((k_stage<TX, TY, kGeoCart, true, MODE, true>), grd, blk, 0, s, tab, nb, nt);
-      else hipLaunchKernelGGL((k_stage<TX, TY, kGeoCart, false, MODE, true>), grd, blk, 0, s, tab, nb, nt);
-      return hipGetLastError();
+    if (geo != kGeoCart) return hipErrorInvalidValue;
+    if constexpr (VOK) {
+      if (visc) {
+        hipLaunchKernelGGL((k_stage<TX, TY, kGeoCart, true, MODE, true>), grd, blk, 0, s, tab, nb, nt);
+        return hipGetLastError();
+      }
     }
-    return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_stage<TX, TY, kGeoCart, false, MODE, true>), grd, blk, 0, s, tab, nb, nt);
+    return hipGetLastError();
   }
   if (geo == kGeoC3) return launch_g<TX, TY, kGeoC3, MODE>(tab, nb, nt, visc, s);
   if (geo == kGeoC2) return launch_g<TX, TY, kGeoC2, MODE>(tab, nb, nt, visc, s);

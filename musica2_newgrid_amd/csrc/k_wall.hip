@@ -44,16 +44,31 @@ struct Pw {
   double u, v, w, T, p;
 };
 
-__device__ __forceinline__ Pw prim_at(const CState &U, idx_t p, double gam1, double rg) {
+__device__ __forceinline__ Pw prim_of(const double U[5], double gam1, double rg) {
   Pw q;
-  const double r = ldg(U.v[0], p);
+  const double r = U[0];
   const double vi = 1.0 / r;
-  q.u = ldg(U.v[1], p) * vi;
-  q.v = ldg(U.v[2], p) * vi;
-  q.w = ldg(U.v[3], p) * vi;
-  q.p = gam1 * (ldg(U.v[4], p) - 0.5 * r * (q.u * q.u + q.v * q.v + q.w * q.w));
+  q.u = U[1] * vi;
+  q.v = U[2] * vi;
+  q.w = U[3] * vi;
+  q.p = gam1 * (U[4] - 0.5 * r * (q.u * q.u + q.v * q.v + q.w * q.w));
   q.T = q.p * vi / rg;
   return q;
+}
+
+__device__ __forceinline__ Pw prim_at(const CState &U, idx_t p, double gam1, double rg) {
+  double u[5];
+#pragma unroll
+  for (int v = 0; v < 5; v++) u[v] = ldg(U.v[v], p);
+  return prim_of(u, gam1, rg);
+}
+
+// Primitives of a stage set that is stored as FP32 differences with ref.
+__device__ __forceinline__ Pw prim_at32(const float *const *U, const double *ref, idx_t p, double gam1, double rg) {
+  double u[5];
+#pragma unroll
+  for (int v = 0; v < 5; v++) u[v] = ref[v] + (double)ldg(U[v], p);
+  return prim_of(u, gam1, rg);
 }
 
 __device__ __forceinline__ double sutherland(const Gas &gs, double T) {
@@ -95,9 +110,12 @@ __global__ __launch_bounds__(kWallThreads) void k_wall(WallArgs a) {
   const int i = 1 + (int)(r % g.nx), k = 1 + (int)(r / g.nx);
   const int j1 = side ? g.ny : 1, dj = side ? -1 : 1;
   double rho, rhoe;
-  if (a.src.v[0]) {
-    const Pw q2 = prim_at(a.src, lin(g, i, j1 + dj, k), a.gas.gam1, a.gas.rg);
-    const Pw q3 = prim_at(a.src, lin(g, i, j1 + 2 * dj, k), a.gas.gam1, a.gas.rg);
+  if (a.src.v[0] || a.src32[0]) {
+    const idx_t p2 = lin(g, i, j1 + dj, k), p3 = lin(g, i, j1 + 2 * dj, k);
+    const Pw q2 = a.src.v[0] ? prim_at(a.src, p2, a.gas.gam1, a.gas.rg)
+                             : prim_at32(a.src32, a.ref, p2, a.gas.gam1, a.gas.rg);
+    const Pw q3 = a.src.v[0] ? prim_at(a.src, p3, a.gas.gam1, a.gas.rg)
+                             : prim_at32(a.src32, a.ref, p3, a.gas.gam1, a.gas.rg);
     const double c2 = a.c2[side], c3 = a.c3[side];
     const double tw = a.adiabatic ? c2 * q2.T - c3 * q3.T : a.t_wall;
     const double pw = c2 * q2.p - c3 * q3.p;
@@ -114,6 +132,11 @@ __global__ __launch_bounds__(kWallThreads) void k_wall(WallArgs a) {
     stg(a.dst.v[2], p1, 0.0);
     stg(a.dst.v[3], p1, 0.0);
     stg(a.dst.v[4], p1, rhoe);
+  } else if (a.dst32[0]) {
+    const idx_t p1 = lin(g, i, j1, k);
+    const double w[5] = {rho, 0.0, 0.0, 0.0, rhoe};
+#pragma unroll
+    for (int v = 0; v < 5; v++) stg(a.dst32[v], p1, (float)(w[v] - a.ref[v]));
   } else {
     stg(a.stash, (idx_t)(2 * t), rho);
     stg(a.stash, (idx_t)(2 * t + 1), rhoe);

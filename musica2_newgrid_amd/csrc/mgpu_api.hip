@@ -166,9 +166,9 @@ std::string check_topology(const mgpu_case &c) {
           snprintf(buf, sizeof buf, "block %d face %d: walls are supported on the j-faces (2, 3) only", b, f);
           return buf;
         }
-        if (c.curvilinear || c.precision != MGPU_PREC_FP64 || c.filter == MGPU_FILTER_ARTVISC) {
+        if (c.curvilinear || c.filter == MGPU_FILTER_ARTVISC) {
           snprintf(buf, sizeof buf,
-                   "block %d face %d: walls need a Cartesian FP64 case with the selective or shock filter or none",
+                   "block %d face %d: walls need a Cartesian case with the selective or shock filter or none",
                    b, f);
           return buf;
         }
@@ -659,7 +659,8 @@ int build_tables(mgpu_handle *h) {
 
 // Wall state of the j-walls of every block with walls: from rows 2,3 of set
 // `src` (-1: from the block's stash) into the wall rows of set `dst` (-1:
-// into the stash).
+// into the stash).  A set >= 0 is an FP64 state set, a set <= -2 the
+// mixed-precision FP32 stage set -2 - set.
 int enqueue_walls(mgpu_handle *h, int src, int dst) {
   for (const auto &bd : h->b) {
     if (!bd.g.jwall) continue;
@@ -669,7 +670,14 @@ int enqueue_walls(mgpu_handle *h, int src, int dst) {
     for (int v = 0; v < 5; v++) {
       wa.src.v[v] = src >= 0 ? bd.st[src][v] : nullptr;
       wa.dst.v[v] = dst >= 0 ? bd.st[dst][v] : nullptr;
+      wa.src32[v] = src <= -2 ? bd.sf[-2 - src][v] : nullptr;
+      wa.dst32[v] = dst <= -2 ? bd.sf[-2 - dst][v] : nullptr;
     }
+    wa.ref[0] = bd.ref.r;
+    wa.ref[1] = bd.ref.ru;
+    wa.ref[2] = bd.ref.rv;
+    wa.ref[3] = bd.ref.rw;
+    wa.ref[4] = bd.ref.re;
     wa.stash = bd.wstash;
     for (int s = 0; s < 2; s++) {
       wa.c2[s] = bd.c2[s];
@@ -746,9 +754,13 @@ int enqueue_step(mgpu_handle *h, int s0) {
     }
     if (h->wall) {
       ProfScope ps(h, MGPU_K_HALO);
-      if (!last) rc = enqueue_walls(h, in, out);
-      else if (c.filter != MGPU_FILTER_NONE) rc = enqueue_walls(h, visc ? s0 : in, -1);
-      else rc = enqueue_walls(h, visc ? s0 : in, out);
+      // the stage's input and output sets (mixed precision: the intermediate
+      // stage states are FP32 sets)
+      const int in_d = (h->mixed && irk > 1) ? -2 - fset_of(irk - 1) : in;
+      const int out_d = (h->mixed && !last) ? -2 - fset_of(irk) : out;
+      if (!last) rc = enqueue_walls(h, in_d, out_d);
+      else if (c.filter != MGPU_FILTER_NONE) rc = enqueue_walls(h, visc ? s0 : in_d, -1);
+      else rc = enqueue_walls(h, visc ? s0 : in_d, out);
       if (rc) return rc;
     }
     in = out;

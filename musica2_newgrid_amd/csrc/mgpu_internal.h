@@ -271,6 +271,11 @@ struct WallArgs {
   Gas gas;
   CState src;
   State dst;
+  // mixed precision: an intermediate stage set stored as FP32 differences
+  // with the reference state ref (used when the FP64 pointer is null)
+  const float *src32[5];
+  float *dst32[5];
+  double ref[5];
   double *stash;
   double c2[2], c3[2];  // extrapolation coefficients cextrp2, cextrp3 (jmin, jmax)
   double cv;            // cv = rg/(gam-1)

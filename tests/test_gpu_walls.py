@@ -114,6 +114,28 @@ def test_walls_graph_matches_eager_and_tuned(filt):
     assert e < 1e-12
 
 
+@pytest.mark.parametrize("filt", [1, 2])
+def test_mixed_precision_channel(filt):
+    """Mixed precision next to walls (the wall state of the FP32 intermediate
+    stage sets, the wall variants of the FP32 stage and filter kernels):
+    density error against the FP64 oracle after 300 steps."""
+    from musica2_newgrid_amd import _lib
+    c = cases.channel(48, 40, 16, filter=filt)
+    c.ducros = filt == 2
+    nsteps = 300
+    print(f"mixed-precision channel 48x40x16 filter={filt}, {nsteps} steps", flush=True)
+    from musica2_newgrid_amd.solver import GpuSolver
+    with GpuSolver(c, precision=_lib.PREC_MIXED) as s:
+        s.step(nsteps)
+        Ug = s.get_state()
+    Uc = _oracle(c, nsteps)
+    a, b = gr.interior(Ug[0]), gr.interior(Uc[0])
+    linf = float(np.max(np.abs(a - b))) / float(np.max(np.abs(b)))
+    e_all = relerr(Ug, Uc)
+    print(f"rho Linf {linf:.3e}, all variables {e_all:.3e} vs the FP64 oracle", flush=True)
+    assert 0.0 < linf <= 1e-6
+
+
 def test_step_needs_the_wall_coefficients():
     """A block with walls refuses to step before mgpu_set_wall."""
     import ctypes as C

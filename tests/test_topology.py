@@ -97,16 +97,13 @@ def _create_case(c, blocks, precision=_lib.PREC_FP64):
 
 
 def test_wall_limits_are_checked():
-    """Walls: j-faces only, Cartesian FP64 with the selective or shock filter
-    or none (not the artificial viscosity), ny >= 10; refused before any
-    device work."""
+    """Walls: j-faces only, Cartesian with the selective or shock filter or
+    none (not the artificial viscosity), ny >= 10; refused before any device
+    work."""
     c, blocks = _channel_table()
     blocks[0]["kind"][0] = blocks[0]["kind"][1] = _lib.FACE_WALL
     rc, msg = _create_case(c, blocks)
     assert rc == _lib.EINVAL and "j-faces" in msg
-    c, blocks = _channel_table()
-    rc, msg = _create_case(c, blocks, precision=_lib.PREC_MIXED)
-    assert rc == _lib.EINVAL and "FP64" in msg
     c, blocks = _channel_table(filter=3)
     rc, msg = _create_case(c, blocks)
     assert rc == _lib.EINVAL and "selective or shock filter" in msg
