@@ -59,7 +59,7 @@ extern "C" {
  * src/File_IO/read_param.f90:75): bc_wall_*_adiabatic_dpdn or
  * _isotherm_dpdn (src/Boundary/smod_bc_wall.f90) with the boundary-row
  * closures of the Euler, viscous and filter operators.  On the j-faces (2, 3)
- * of Cartesian FP64 cases with the selective or shock filter or none,
+ * of Cartesian cases (FP64 or mixed) with the selective or shock filter or none,
  * ny >= 10; the block's extrapolation coefficients come from mgpu_set_wall.
  * With the shock filter the ghost row beyond a wall must hold a valid state:
  * the wall row's pressure sensor reads p there (mod_filtering_shock.f90:
