@@ -17,6 +17,7 @@ module mod_musica_gpu
   integer(c_int32_t), parameter :: MGPU_FILTER_NONE = 0, MGPU_FILTER_SF = 1, MGPU_FILTER_SHOCK = 2
   integer(c_int32_t), parameter :: MGPU_FILTER_ARTVISC = 3
   integer(c_int32_t), parameter :: MGPU_PREC_FP64 = 0, MGPU_PREC_MIXED = 1
+  integer(c_int32_t), parameter :: MGPU_GEOM_CART = 0, MGPU_GEOM_C3 = 1, MGPU_GEOM_C2 = 2
   integer(c_int32_t), parameter :: MGPU_LAUNCH_EAGER = 1
   integer, parameter :: MGPU_NGH = 5
 
@@ -62,6 +63,15 @@ module mod_musica_gpu
        integer(c_int), value :: block
        type(c_ptr), intent(in) :: m(10), mv(10)
      end function mgpu_set_metrics_c3
+
+     ! 2.5-D (is_curv): m2 = c_loc of x_ksi, x_eta, y_ksi, y_eta, ijacob (padded 2-D), m2v the *_v set
+     integer(c_int) function mgpu_set_metrics_c2(h, block, m2, m2v, idz, idz_v) bind(C, name='mgpu_set_metrics_c2')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int), value :: block
+       type(c_ptr), intent(in) :: m2(5), m2v(5)
+       real(c_double), intent(in) :: idz(*), idz_v(*)
+     end function mgpu_set_metrics_c2
 
      integer(c_int) function mgpu_set_wall(h, block, cextrp2, cextrp3) bind(C, name='mgpu_set_wall')
        import :: c_int, c_ptr, c_double
