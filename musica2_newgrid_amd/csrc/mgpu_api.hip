@@ -711,10 +711,10 @@ WallViscArgs wall_visc_args(const mgpu_handle *h, const BlockData &bd, int s0) {
 // sequence is captured into a graph).  runge_kutta, runge_kutta.f90:2-152.
 // With walls, bc_wall runs after each stage's exchange, from the stage's
 // input (the primitives the reference computed last), except after the last
-// stage: there the wall state (from U^n after a viscous stage, whose
-// primitives_visc overwrote the primitives, else from the stage's input) is
-// kept aside and imposed after the filter and its exchange
-// (runge_kutta.f90:20-24, 42-45, 130-136).
+// stage: there the wall state (from U^n, whose primitives the last stage's
+// primitives_visc wrote, viscous or not, runge_kutta.f90:71) is kept aside
+// and imposed after the filter and its exchange (runge_kutta.f90:20-24,
+// 42-45, 130-136).
 int enqueue_step(mgpu_handle *h, int s0) {
   const mgpu_case &c = h->c;
   const int geo = c.curvilinear;
@@ -758,9 +758,10 @@ int enqueue_step(mgpu_handle *h, int s0) {
       // stage states are FP32 sets)
       const int in_d = (h->mixed && irk > 1) ? -2 - fset_of(irk - 1) : in;
       const int out_d = (h->mixed && !last) ? -2 - fset_of(irk) : out;
+      // after the last stage from U^n: its primitives_visc runs viscous or not
       if (!last) rc = enqueue_walls(h, in_d, out_d);
-      else if (c.filter != MGPU_FILTER_NONE) rc = enqueue_walls(h, visc ? s0 : in_d, -1);
-      else rc = enqueue_walls(h, visc ? s0 : in_d, out);
+      else if (c.filter != MGPU_FILTER_NONE) rc = enqueue_walls(h, s0, -1);
+      else rc = enqueue_walls(h, s0, out);
       if (rc) return rc;
     }
     in = out;

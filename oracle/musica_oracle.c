@@ -1049,6 +1049,11 @@ int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_gri
         } else {
           prim_euler(&cb[b], &d[b], Un[b], &w[b]);
           euler_rhs_w(&cb[b], &g[b], &d[b], Un[b], &w[b], K[b]);
+          /* the last stage calls primitives_visc(U^n) whether or not the case is
+           * viscous (runge_kutta.f90:71; only grad_vel and flux_visc are void
+           * without viscosity): the primitives bc_wall finds at the end of the
+           * step are those of U^n */
+          if (last) prim_euler(&cb[b], &d[b], &U[5 * b], &w[b]);
         }
         const double alpha = c0->dt * c0->crk[irk - 1];
         const dims_t dd = d[b];
@@ -1088,8 +1093,7 @@ int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_gri
       }
     }
     /* bc_wall at the end of the step (runge_kutta.f90:130-136): from the primitives
-     * of U^n after a viscous last stage (primitives_visc wrote them), else from
-     * those of the last stage's input */
+     * of U^n, which the last stage's primitives_visc wrote */
     for (int b = 0; b < nb; b++) bc_wall(&cb[b], &g[b], &d[b], Un[b], &w[b]);
     for (int b = 0; b < nb; b++)
       for (int v = 0; v < 5; v++) memcpy(U[5 * b + v], Un[b][v], d[b].n * sizeof(double));

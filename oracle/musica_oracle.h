@@ -113,7 +113,8 @@ void orc_fill_interfaces(int nblocks, const orc_block *blocks, double *const *ar
  *   - bc_wall (smod_bc_wall.f90:11-98) at the start of stages 2..nrk and at the
  *     end of the step: zero momentum, p and (adiabatic) T extrapolated from rows
  *     2,3 of the primitives last computed (those of the previous stage's input;
- *     at the end of a viscous step those of U^n, which primitives_visc wrote),
+ *     at the end of the step those of U^n, which the last stage's primitives_visc
+ *     wrote, viscous or not),
  *     rho = p/(R T), rhoE = rho cv T (mod_eos_pfg.f90:170-184, 303-317);
  *   - the exchange runs before bc_wall, so face ghosts on a wall row keep the
  *     neighbour's value from before the wall state was imposed. */

@@ -5,9 +5,9 @@ mathematics fixes.  The reference ships no tests for its wall closures, so:
   next degree (a swapped row or a wrong coefficient fails);
 * the ghost layers beyond a wall hold NaN: a closure that reads them fails;
 * the wall state after a step is the dp/dn = 0 (and dT/dn = 0) extrapolation of
-  the primitives the reference's choreography says (U^n after a viscous last
-  stage, the last stage's input otherwise), exact for quadratics with zero
-  slope at the wall;
+  the primitives the reference's choreography says (those of U^n, which the
+  last stage's primitives_visc writes, viscous or not), exact for quadratics
+  with zero slope at the wall;
 * a channel symmetric about its mid-plane stays symmetric (the max-side
   closures mirror the min-side ones), walls on x-faces give the transposed
   result of walls on y-faces, and a channel cut into two blocks advances like
@@ -156,31 +156,30 @@ def test_wall_state_extrapolates_the_right_primitives(visc, nrk, adiabatic):
     assert np.max(np.abs(row[4] / (rho_w * cv * Tw) - 1.0)) < 1e-13
 
 
-def test_wall_state_lags_one_stage_without_viscosity():
-    """nrk = 2, Euler only, no filter: the end-of-step wall state is the
-    extrapolation of the second stage's input U1 = U^n - dt crk1 K(U^n)."""
+def test_wall_state_after_an_inviscid_step_comes_from_u_n():
+    """nrk = 2, Euler only, no filter: the last stage calls primitives_visc
+    (runge_kutta.f90:71) whether or not the case is viscous, so the end-of-step
+    wall state is the extrapolation of U^n's primitives, not of the second
+    stage's input U1 = U^n - dt crk1 K(U^n)."""
     c, _, _ = _channel_quadratic(0, 2, True, filt=0)
     o = orc.Oracle(c)
-    K = o.euler_rhs(c.U)
-    U1 = [a - c.dt * c.crk[0] * k for a, k in zip(c.U, K)]
-    r1 = [gr.interior(a) for a in U1]
-    rho = r1[0]
-    q2 = (r1[1] ** 2 + r1[2] ** 2 + r1[3] ** 2) / rho
-    p = (cases.GAM - 1.0) * (r1[4] - 0.5 * q2)
-    T = p / (rho * cases.RG)
-    c2, c3 = c.cextrp2[2], c.cextrp3[2]
-    pw = c2 * p[:, 1, :] - c3 * p[:, 2, :]
-    Tw = c2 * T[:, 1, :] - c3 * T[:, 2, :]
-    U = c.copy_state()
-    o.step(U, 1)
-    assert np.max(np.abs(gr.interior(U[0])[:, 0, :] / (pw / (cases.RG * Tw)) - 1.0)) < 1e-13
-    # the extrapolation of U^n itself differs: the lag is visible
     rho0 = gr.interior(c.U[0])
     p0 = (cases.GAM - 1.0) * (gr.interior(c.U[4]) - 0.5 * (gr.interior(c.U[1]) ** 2 + gr.interior(c.U[2]) ** 2
                                                            + gr.interior(c.U[3]) ** 2) / rho0)
     T0 = p0 / (rho0 * cases.RG)
+    c2, c3 = c.cextrp2[2], c.cextrp3[2]
     rw0 = (c2 * p0[:, 1, :] - c3 * p0[:, 2, :]) / (cases.RG * (c2 * T0[:, 1, :] - c3 * T0[:, 2, :]))
-    assert np.max(np.abs(gr.interior(U[0])[:, 0, :] / rw0 - 1.0)) > 1e-9
+    U = c.copy_state()
+    o.step(U, 1)
+    assert np.max(np.abs(gr.interior(U[0])[:, 0, :] / rw0 - 1.0)) < 1e-13
+    # the extrapolation of the second stage's input differs: the test sees the choice
+    K = o.euler_rhs(c.U)
+    r1 = [gr.interior(a - c.dt * c.crk[0] * k) for a, k in zip(c.U, K)]
+    q2 = (r1[1] ** 2 + r1[2] ** 2 + r1[3] ** 2) / r1[0]
+    p1 = (cases.GAM - 1.0) * (r1[4] - 0.5 * q2)
+    T1 = p1 / (r1[0] * cases.RG)
+    rw1 = (c2 * p1[:, 1, :] - c3 * p1[:, 2, :]) / (cases.RG * (c2 * T1[:, 1, :] - c3 * T1[:, 2, :]))
+    assert np.max(np.abs(gr.interior(U[0])[:, 0, :] / rw1 - 1.0)) > 1e-9
 
 
 def _mirror_case():
