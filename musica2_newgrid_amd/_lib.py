@@ -16,7 +16,7 @@ MGPU_NGH = 5
 FACE_PERIODIC = 0
 FACE_INTERFACE = 1
 FACE_WALL = 2
-FILTER_NONE, FILTER_SF, FILTER_SHOCK, FILTER_ARTVISC = 0, 1, 2, 3
+FILTER_NONE, FILTER_SF, FILTER_SHOCK, FILTER_ARTVISC, FILTER_ARTVISC_SHOCK = 0, 1, 2, 3, 4
 PREC_FP64, PREC_MIXED = 0, 1
 GEOM_CART, GEOM_C3, GEOM_C2 = 0, 1, 2
 LAUNCH_EAGER = 1

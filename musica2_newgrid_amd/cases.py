@@ -65,7 +65,8 @@ class Case:
     nz: int
     curvilinear: bool
     iorder_visc: int = 4
-    filter: int = 1            # 0 none, 1 selective filter, 2 shock filter
+    filter: int = 1            # 0 none, 1 selective filter, 2 shock filter, 3 artificial viscosity,
+    #                            4 artificial viscosity with shock capturing
     ducros: bool = True
     sw_edoh: bool = False      # Edoh switch of the shock filter (is_sw_edoh)
     nrk: int = 4

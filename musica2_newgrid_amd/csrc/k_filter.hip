@@ -266,6 +266,12 @@ __device__ __forceinline__ int face_half(const Geom &g, int m) {
 // Edoh switch factor of direction d at element p (position r along d):
 // min(1, CFL/chi) with CFL = sr_d deltat, times idx_d(r) on Cartesian blocks
 // (mod_filtering_shock.f90:668-676 and the j, k repeats; c3 :627-632).
+// Metric factor of direction d at position r: idx_d(r) on Cartesian
+// directions, 1 where the spectral radius carries the metric (null array).
+__device__ __forceinline__ double met_at(const double *met, int r) {
+  return met ? ldg(met, (idx_t)(r - 1 + G)) : 1.0;
+}
+
 __device__ __forceinline__ double edoh_mul(const FilterArgs &a, int d, idx_t p, int r) {
   const double *met = d == 0 ? a.idx : (d == 1 ? a.idy : a.idz);
   double cfl = ldg(a.sr[d], p) * a.dt;
@@ -274,18 +280,24 @@ __device__ __forceinline__ double edoh_mul(const FilterArgs &a, int d, idx_t p, 
 }
 
 // ARTV: the artificial viscosity.  The LDS "pressure" pencils then hold the
-// spectral radii |u|+c (i-pencil) and |v|+c (j-pencil), the k queue |w|+c.
+// spectral radii sr_x (i-pencil) and sr_y (j-pencil), the k queue sr_z.
+// AVS: the artificial viscosity with shock capturing (artvisc_o9_shock,
+// mod_artvisc_shock.f90:113-1287): the shock filter's faces D, each
+// direction's K = (sr_{-1/2} D_{-1/2} - sr_{+1/2} D_{+1/2}) * metric with the
+// spectral radius of U^n averaged to the faces (:480-550 i, :866-935 j,
+// :1243-1283 k), U_n += deltat*K (update_varn_artvisc).
 // WALL: blocks with j-walls; rows 1..5 and ny-4..ny get no j-increment here
 // (k_shock_wall adds it with the reduced faces of the wall closure).
 // EDOH: the shock filter's Edoh switch, each direction's increment times
 // min(1, CFL_d / chi) with the local CFL_d = sr_d deltat (idx_d on Cartesian
 // blocks) (mod_filtering_shock.f90:620-700, :1059-1140, :1450-1496).
-template <bool MIX, bool ARTV, bool WALL, bool EDOH>
+template <bool MIX, bool ARTV, bool WALL, bool EDOH, bool AVS>
 __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> *__restrict__ tab, int nb,
                                                         int ntiles) {
   static_assert(!(MIX && ARTV), "artificial viscosity: FP64 only");
   static_assert(!(WALL && ARTV), "walls: not with the artificial viscosity");
   static_assert(!(EDOH && ARTV), "the Edoh switch belongs to the shock filter");
+  static_assert(!(AVS && (MIX || ARTV || EDOH)), "artificial viscosity with shock capturing: FP64, no Edoh switch");
   using R = typename std::conditional<MIX, float, double>::type;
   __shared__ R sux[5][FY][FX + 10];
   __shared__ R suy[5][FY + 10][FX];
@@ -295,7 +307,9 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
   __shared__ double ssy[FY + 4][FX];    // sensor at j0-2 .. j0+FY+1 (ends unused)
   __shared__ double psx[FY][FX + 2];    // psens_x at i0-1 .. i0+FX
   __shared__ double psy[FY + 2][FX];    // psens_y at j0-1 .. j0+FY
-  __shared__ double smul[EDOH ? 3 : 1][FY][FX];  // Edoh factors of the plane's own points
+  // Edoh factors of the plane's own points (AVS: the face spectral radii
+  // sr_{-1/2}, sr_{+1/2} of the three directions, then their metrics)
+  __shared__ double smul[EDOH ? 3 : (AVS ? 9 : 1)][FY][FX];
 
   int i0, j0, kbeg, kend;
   const FilterArgs &a = tile_of(tab, nb, ntiles, i0, j0, kbeg, kend);
@@ -441,6 +455,22 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
         smul[2][ty][tx] = edoh_mul(a, 2, pk, k);
       }
     }
+    if constexpr (AVS) {
+      if (own) {
+        double s0 = ldg(a.sr[0], pk);
+        smul[0][ty][tx] = 0.5 * (ldg(a.sr[0], pk - 1) + s0);
+        smul[1][ty][tx] = 0.5 * (ldg(a.sr[0], pk + 1) + s0);
+        s0 = ldg(a.sr[1], pk);
+        smul[2][ty][tx] = 0.5 * (ldg(a.sr[1], pk - g.sx) + s0);
+        smul[3][ty][tx] = 0.5 * (ldg(a.sr[1], pk + g.sx) + s0);
+        s0 = ldg(a.sr[2], pk);
+        smul[4][ty][tx] = 0.5 * (ldg(a.sr[2], pk - sz) + s0);
+        smul[5][ty][tx] = 0.5 * (ldg(a.sr[2], pk + sz) + s0);
+        smul[6][ty][tx] = met_at(a.idx, i);
+        smul[7][ty][tx] = met_at(a.idy, j);
+        smul[8][ty][tx] = met_at(a.idz, k);
+      }
+    }
     __syncthreads();
     if (k < kend) load_plane(k + 1);
     // pressure sensors of the plane
@@ -464,7 +494,7 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
       double K[5];
       {
         const double s0 = spx[ty][tx + 2], srp = 0.5 * (spx[ty][tx + 3] + s0), srm = 0.5 * (spx[ty][tx + 1] + s0);
-        const double m = ldg(a.idx, (idx_t)(i - 1 + G));
+        const double m = met_at(a.idx, i);
 #pragma unroll
         for (int v = 0; v < 5; v++) {
           const R *x = &sux[v][ty][tx + 5];
@@ -473,7 +503,7 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
       }
       {
         const double s0 = spy[ty + 2][tx], srp = 0.5 * (spy[ty + 3][tx] + s0), srm = 0.5 * (spy[ty + 1][tx] + s0);
-        const double m = ldg(a.idy, (idx_t)(j - 1 + G));
+        const double m = met_at(a.idy, j);
 #pragma unroll
         for (int v = 0; v < 5; v++) {
           const R *y = &suy[v][ty + 5][tx];
@@ -482,7 +512,7 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
       }
       {
         const double s0 = qp[1], srp = 0.5 * (qp[2] + s0), srm = 0.5 * (qp[0] + s0);
-        const double m = ldg(a.idz, (idx_t)(k - 1 + G));
+        const double m = met_at(a.idz, k);
 #pragma unroll
         for (int v = 0; v < 5; v++) {
           const double dlo = R(D10_1) * (q[5][v] - q[4][v]) + R(D10_2) * (q[6][v] - q[3][v]) +
@@ -505,10 +535,19 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
         const double xk0m = a.xnu0 * fmax(p0, pm), xk10m = fmax(0.0, a.xnu10 + xk0m / 105.0);
         const double xk0p = a.xnu0 * fmax(pp, p0), xk10p = fmax(0.0, a.xnu10 + xk0p / 105.0);
         const R mul = EDOH ? (R)smul[0][ty][tx] : R(1);
+        double srm = 0.0, srp = 0.0, m = 1.0;
+        if constexpr (AVS) {
+          srm = smul[0][ty][tx];
+          srp = smul[1][ty][tx];
+          m = smul[6][ty][tx];
+        }
 #pragma unroll
         for (int v = 0; v < 5; v++) {
           const R *x = &sux[v][ty][tx + 5];
-          if constexpr (EDOH) K[v] = mul * (dface<R>(x - 1, 1, (R)xk0m, (R)xk10m) - dface<R>(x, 1, (R)xk0p, (R)xk10p));
+          if constexpr (AVS)
+            K[v] = (srm * dface<R>(x - 1, 1, (R)xk0m, (R)xk10m) - srp * dface<R>(x, 1, (R)xk0p, (R)xk10p)) * m;
+          else if constexpr (EDOH)
+            K[v] = mul * (dface<R>(x - 1, 1, (R)xk0m, (R)xk10m) - dface<R>(x, 1, (R)xk0p, (R)xk10p));
           else K[v] = dface<R>(x - 1, 1, (R)xk0m, (R)xk10m) - dface<R>(x, 1, (R)xk0p, (R)xk10p);
         }
       }
@@ -518,10 +557,19 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
         const double xk0m = a.xnu0 * fmax(p0, pm), xk10m = fmax(0.0, a.xnu10 + xk0m / 105.0);
         const double xk0p = a.xnu0 * fmax(pp, p0), xk10p = fmax(0.0, a.xnu10 + xk0p / 105.0);
         const R mul = EDOH ? (R)smul[1][ty][tx] : R(1);
+        double srm = 0.0, srp = 0.0, m = 1.0;
+        if constexpr (AVS) {
+          srm = smul[2][ty][tx];
+          srp = smul[3][ty][tx];
+          m = smul[7][ty][tx];
+        }
 #pragma unroll
         for (int v = 0; v < 5; v++) {
           const R *y = &suy[v][ty + 5][tx];
-          if constexpr (EDOH)
+          if constexpr (AVS)
+            K[v] = K[v] +
+                   (srm * dface<R>(y - FX, FX, (R)xk0m, (R)xk10m) - srp * dface<R>(y, FX, (R)xk0p, (R)xk10p)) * m;
+          else if constexpr (EDOH)
             K[v] = K[v] + mul * (dface<R>(y - FX, FX, (R)xk0m, (R)xk10m) - dface<R>(y, FX, (R)xk0p, (R)xk10p));
           else K[v] = K[v] + dface<R>(y - FX, FX, (R)xk0m, (R)xk10m) - dface<R>(y, FX, (R)xk0p, (R)xk10p);
         }
@@ -534,6 +582,12 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
         const double xk0m = a.xnu0 * fmax(p0, pm), xk10m = fmax(0.0, a.xnu10 + xk0m / 105.0);
         const double xk0p = a.xnu0 * fmax(pp, p0), xk10p = fmax(0.0, a.xnu10 + xk0p / 105.0);
         const R mul = EDOH ? (R)smul[2][ty][tx] : R(1);
+        double srm = 0.0, srp = 0.0, m = 1.0;
+        if constexpr (AVS) {
+          srm = smul[4][ty][tx];
+          srp = smul[5][ty][tx];
+          m = smul[8][ty][tx];
+        }
 #pragma unroll
         for (int v = 0; v < 5; v++) {
           const R dlo = (q[5][v] - q[4][v]) * (R)xk0m +
@@ -544,13 +598,15 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
                         (R(D10_1) * (q[6][v] - q[5][v]) + R(D10_2) * (q[7][v] - q[4][v]) +
                          R(D10_3) * (q[8][v] - q[3][v]) + R(D10_4) * (q[9][v] - q[2][v]) +
                          R(D10_5) * (q[10][v] - q[1][v])) * (R)xk10p;
-          if constexpr (EDOH) K[v] = K[v] + mul * (dlo - dhi);
+          if constexpr (AVS) K[v] = K[v] + (srm * dlo - srp * dhi) * m;
+          else if constexpr (EDOH) K[v] = K[v] + mul * (dlo - dhi);
           else K[v] = K[v] + dlo - dhi;
         }
       }
 #pragma unroll
       for (int v = 0; v < 5; v++) {
-        if constexpr (MIX) stg_nt(uoutp.v[v], pk, u64[v] + (double)K[v]);
+        if constexpr (AVS) stg_nt(uoutp.v[v], pk, q[5][v] + a.dt * K[v]);
+        else if constexpr (MIX) stg_nt(uoutp.v[v], pk, u64[v] + (double)K[v]);
         else stg_nt(uoutp.v[v], pk, q[5][v] + K[v]);
       }
     }
@@ -587,6 +643,9 @@ constexpr int kShockWallThreads = 256;
 // the wall rows get none (:1128-1140 over ndy_d..nfy_d).  The pressure sensor
 // of row 1 (ny) reads p one row beyond the wall.  One thread per point of
 // rows 2..5 and ny-4..ny-1; nothing beyond a wall is read but p.
+// a.avs: artvisc_o9_shock, whose faces 3 and 4 from a wall take xk0/5 and
+// 3/70 xk0 (mod_artvisc_shock.f90:577-695 jmin, :744-862 jmax) and whose
+// increment is deltat (sr_{-1/2} D_{-1/2} - sr_{+1/2} D_{+1/2}) idy(j).
 __global__ __launch_bounds__(kShockWallThreads) void k_shock_wall(FilterArgs a) {
   const Geom &g = a.g;
   const long nface = (long)g.nx * g.nz;
@@ -619,8 +678,8 @@ __global__ __launch_bounds__(kShockWallThreads) void k_shock_wall(FilterArgs a) 
     switch (hf) {
       case 1: xkh = fmax(0.0, 0.0 + xk0); break;
       case 2: xkh = fmax(0.0, a.chi / 12.0 + xk0); break;
-      case 3: xkh = fmax(0.0, a.chi / 60.0 + xk0); break;
-      case 4: xkh = fmax(0.0, a.chi / 280.0 + xk0); break;
+      case 3: xkh = fmax(0.0, a.chi / 60.0 + (a.avs ? xk0 / 5.0 : xk0)); break;
+      case 4: xkh = fmax(0.0, a.chi / 280.0 + (a.avs ? 3.0 / 70.0 * xk0 : xk0)); break;
       default: xkh = fmax(0.0, a.xnu10 + xk0 / 105.0);
     }
     const idx_t pf = pj + (f - 1) * sj;  // row m
@@ -632,6 +691,15 @@ __global__ __launch_bounds__(kShockWallThreads) void k_shock_wall(FilterArgs a) 
       for (int l = 1; l <= hf; l++) dd = dd + kShockD[hf][l - 1] * (ldg(u, pf + l * sj) - ldg(u, pf + (1 - l) * sj));
       D[f][v] = d1 * xk0 + dd * xkh;
     }
+  }
+  if (a.avs) {
+    const double s0 = ldg(a.sr[1], pj);
+    const double srp = 0.5 * (ldg(a.sr[1], pj + sj) + s0), srm = 0.5 * (ldg(a.sr[1], pj - sj) + s0);
+    const double m = met_at(a.idy, j);
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+      stg(a.uout.v[v], pj, ldg(a.uout.v[v], pj) + a.dt * ((srm * D[0][v] - srp * D[1][v]) * m));
+    return;
   }
   const double mul = a.edoh ? edoh_mul(a, 1, pj, j) : 1.0;
 #pragma unroll
@@ -680,21 +748,27 @@ hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int nt, bool m
 template <bool EDOH>
 static void launch_shock_variant(const Batch<FilterArgs> *tab, int nb, int nt, bool mixed, bool wall, hipStream_t s) {
   if (wall && mixed)
-    hipLaunchKernelGGL((k_filter_shock<true, false, true, EDOH>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    hipLaunchKernelGGL((k_filter_shock<true, false, true, EDOH, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   else if (wall)
-    hipLaunchKernelGGL((k_filter_shock<false, false, true, EDOH>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    hipLaunchKernelGGL((k_filter_shock<false, false, true, EDOH, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   else if (mixed)
-    hipLaunchKernelGGL((k_filter_shock<true, false, false, EDOH>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    hipLaunchKernelGGL((k_filter_shock<true, false, false, EDOH, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   else
-    hipLaunchKernelGGL((k_filter_shock<false, false, false, EDOH>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    hipLaunchKernelGGL((k_filter_shock<false, false, false, EDOH, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
 }
 
-hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int nt, bool mixed, bool artvisc, bool wall,
+hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int nt, bool mixed, DissipKind kind, bool wall,
                                bool edoh, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  if (artvisc) {
+  if (kind == kDissipArtvisc) {
     if (mixed || wall || edoh) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_filter_shock<false, true, false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    hipLaunchKernelGGL((k_filter_shock<false, true, false, false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  } else if (kind == kDissipArtviscShock) {
+    if (mixed || edoh) return hipErrorInvalidValue;
+    if (wall)
+      hipLaunchKernelGGL((k_filter_shock<false, false, true, false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    else
+      hipLaunchKernelGGL((k_filter_shock<false, false, false, false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   } else {
     if (edoh) launch_shock_variant<true>(tab, nb, nt, mixed, wall, s);
     else launch_shock_variant<false>(tab, nb, nt, mixed, wall, s);

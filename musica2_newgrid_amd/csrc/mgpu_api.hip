@@ -100,6 +100,21 @@ Gas gas_of(const mgpu_case &c) {
 
 // The shock filter's Edoh switch is on (is_sw_edoh acts in filter_o10_shock only).
 bool edoh_on(const mgpu_case &c) { return c.filter == MGPU_FILTER_SHOCK && c.is_sw_edoh != 0; }
+// The dissipation reads the pressure sensor (p and the Ducros sensor of U^n).
+bool has_sensor(const mgpu_case &c) {
+  return c.filter == MGPU_FILTER_SHOCK || c.filter == MGPU_FILTER_ARTVISC_SHOCK;
+}
+// The artificial viscosities: spectral radii of U^n at the faces' two sides.
+bool is_artvisc(const mgpu_case &c) {
+  return c.filter == MGPU_FILTER_ARTVISC || c.filter == MGPU_FILTER_ARTVISC_SHOCK;
+}
+// The spectral radii of U^n are stored (artificial viscosities, Edoh switch).
+bool has_sr(const mgpu_case &c) { return is_artvisc(c) || edoh_on(c); }
+DissipKind dissip_kind(const mgpu_case &c) {
+  if (c.filter == MGPU_FILTER_ARTVISC) return kDissipArtvisc;
+  if (c.filter == MGPU_FILTER_ARTVISC_SHOCK) return kDissipArtviscShock;
+  return kDissipShock;
+}
 
 Metrics metrics_of(const BlockData &bd) {
   Metrics m{};
@@ -168,7 +183,8 @@ std::string check_topology(const mgpu_case &c) {
         }
         if (c.curvilinear || c.filter == MGPU_FILTER_ARTVISC) {
           snprintf(buf, sizeof buf,
-                   "block %d face %d: walls need a Cartesian case with the selective or shock filter or none",
+                   "block %d face %d: walls need a Cartesian case with the selective or shock filter, the "
+                   "artificial viscosity with shock capturing or none",
                    b, f);
           return buf;
         }
@@ -376,7 +392,7 @@ int setup_exchange(mgpu_handle *h) {
       return fail(h, MGPU_EINVAL, "viscous interface exchange would reach outside a block's arrays");
     if ((rc = upload(h, tasks, &dt)) || (rc = upload(h, chunks, &dc))) return rc;
     h->nchunks_v = (int)chunks.size();
-    const bool shock = h->c.filter == MGPU_FILTER_SHOCK, artv = h->c.filter == MGPU_FILTER_ARTVISC;
+    const bool shock = has_sensor(h->c), artv = is_artvisc(h->c);
     const int nv = NTAU + (shock ? 2 : 0) + (artv ? 3 : 0);
     std::vector<double *> ptrs;
     for (int b = 0; b < nb; b++) {
@@ -568,6 +584,7 @@ FilterArgs filter_args(const mgpu_handle *h, const BlockData &bd, int in, int ou
   fa.idz = bd.idx[2];
   fa.dt = bd.p.deltat;
   fa.edoh = edoh_on(h->c);
+  fa.avs = h->c.filter == MGPU_FILTER_ARTVISC_SHOCK;
   return fa;
 }
 
@@ -632,8 +649,8 @@ int build_tables(mgpu_handle *h) {
       va.prs = bd.prs;
       va.sens = bd.sens;
       for (int q = 0; q < 3; q++) va.sr[q] = bd.sr[q];
-      va.want_sensor = c.filter == MGPU_FILTER_SHOCK;
-      va.want_sr = c.filter == MGPU_FILTER_ARTVISC || edoh_on(c);
+      va.want_sensor = has_sensor(c);
+      va.want_sr = has_sr(c);
       va.ducros = c.is_ducros;
       vi[s0 * nb + b].t = tv;
       fi[s0 * nb + b].a = filter_args(h, bd, in, s0);  // reads the last stage's output
@@ -772,9 +789,9 @@ int enqueue_step(mgpu_handle *h, int s0) {
       if (c.filter == MGPU_FILTER_SF)
         HIPCK(h, launch_filter_sf(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, h->wall, h->stream));
       else
-        HIPCK(h, launch_filter_shock(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed,
-                                     c.filter == MGPU_FILTER_ARTVISC, h->wall, edoh_on(c), h->stream));
-      if (h->wall && c.filter == MGPU_FILTER_SHOCK)
+        HIPCK(h, launch_filter_shock(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, dissip_kind(c), h->wall,
+                                     edoh_on(c), h->stream));
+      if (h->wall && has_sensor(c))
         for (const auto &bd : h->b)
           if (bd.g.jwall) HIPCK(h, launch_shock_wall(filter_args(h, bd, in, s0), h->stream));
     }
@@ -831,15 +848,17 @@ std::string check_flags(const mgpu_case &c) {
   if (c.nrk < 1 || c.nrk > 8) snprintf(buf, sizeof buf, "nrk=%d out of range 1..8", c.nrk);
   else if (c.iorder_visc != 0 && c.iorder_visc != 4)
     snprintf(buf, sizeof buf, "iorder_visc=%d: only 0 and 4 are supported", c.iorder_visc);
-  else if (c.filter < 0 || c.filter > 3) snprintf(buf, sizeof buf, "filter=%d", c.filter);
+  else if (c.filter < 0 || c.filter > 4) snprintf(buf, sizeof buf, "filter=%d", c.filter);
   else if (c.curvilinear < MGPU_GEOM_CART || c.curvilinear > MGPU_GEOM_C2)
     snprintf(buf, sizeof buf, "curvilinear=%d: 0 Cartesian, 1 3-D, 2 2.5-D", c.curvilinear);
   else if (c.filter == MGPU_FILTER_SHOCK && c.iorder_visc == 0)
     snprintf(buf, sizeof buf, "shock filter needs iorder_visc=4 (its sensor reads the viscous gradients of U^n)");
-  else if (c.filter == MGPU_FILTER_ARTVISC && c.iorder_visc == 0)
+  else if (is_artvisc(c) && c.iorder_visc == 0)
     snprintf(buf, sizeof buf, "artificial viscosity needs iorder_visc=4 (it reads the sound speed of U^n)");
-  else if (c.filter == MGPU_FILTER_ARTVISC && (c.curvilinear || c.precision != MGPU_PREC_FP64))
-    snprintf(buf, sizeof buf, "artificial viscosity: Cartesian FP64 cases only");
+  else if (c.filter == MGPU_FILTER_ARTVISC && c.curvilinear == MGPU_GEOM_C3)
+    snprintf(buf, sizeof buf, "artificial viscosity: Cartesian or 2.5-D cases (artvisc_o9 has no c3 form)");
+  else if (is_artvisc(c) && c.precision != MGPU_PREC_FP64)
+    snprintf(buf, sizeof buf, "artificial viscosity: FP64 only");
   else if (c.precision != MGPU_PREC_FP64 && c.precision != MGPU_PREC_MIXED)
     snprintf(buf, sizeof buf, "precision=%d", c.precision);
   else if (c.precision == MGPU_PREC_MIXED && c.nrk < 2)
@@ -872,8 +891,7 @@ size_t block_bytes(const mgpu_case &c, int nx, int ny, int nz) {
   size_t arrays = 15;  // three state sets
   if (c.curvilinear == MGPU_GEOM_C3) arrays += c.iorder_visc ? 20 : 10;
   if (c.iorder_visc)
-    arrays += NTAU + 4 + (c.filter == MGPU_FILTER_SHOCK ? 2 : 0) +
-              (c.filter == MGPU_FILTER_ARTVISC || edoh_on(c) ? 3 : 0);
+    arrays += NTAU + 4 + (has_sensor(c) ? 2 : 0) + (has_sr(c) ? 3 : 0);
   size_t bytes = arrays * n * sizeof(double);
   if (c.precision == MGPU_PREC_MIXED) bytes += 10 * n * sizeof(float);
   if (c.curvilinear != MGPU_GEOM_C3) bytes += 6 * (size_t)(nx + ny + nz + 6 * G) * sizeof(double);
@@ -926,11 +944,11 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
   if (c.iorder_visc) {
     for (int q = 0; q < NTAU && !rc; q++) rc = dalloc(h, &bd.tau[q], n);
     for (int q = 0; q < 4 && !rc; q++) rc = dalloc(h, &bd.kv[q], n);
-    if (c.filter == MGPU_FILTER_SHOCK) {
+    if (has_sensor(c)) {
       if (!rc) rc = dalloc(h, &bd.prs, n);
       if (!rc) rc = dalloc(h, &bd.sens, n);
     }
-    if (c.filter == MGPU_FILTER_ARTVISC || edoh_on(c))
+    if (has_sr(c))
       for (int q = 0; q < 3 && !rc; q++) rc = dalloc(h, &bd.sr[q], n);
   }
   if (rc) {
@@ -1220,8 +1238,8 @@ int tune_aux(mgpu_handle *h, bool force, hipEvent_t e0, hipEvent_t e1) {
         if (kind == 1) return launch_vdiv(h->d_vdiv, nb, h->nt_vdiv, geo, h->stream);
         if (h->c.filter == MGPU_FILTER_SF)
           return launch_filter_sf(h->d_tune_f, nb, h->nt_filt, h->mixed, h->wall, h->stream);
-        return launch_filter_shock(h->d_tune_f, nb, h->nt_filt, h->mixed, h->c.filter == MGPU_FILTER_ARTVISC,
-                                   h->wall, edoh_on(h->c), h->stream);
+        return launch_filter_shock(h->d_tune_f, nb, h->nt_filt, h->mixed, dissip_kind(h->c), h->wall,
+                                   edoh_on(h->c), h->stream);
       };
       hipError_t e = launch();
       if (e == hipSuccess) e = hipEventRecord(e0, h->stream);

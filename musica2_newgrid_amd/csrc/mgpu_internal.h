@@ -256,9 +256,13 @@ struct FilterArgs {
   double chi;          // filter amplitude (the 5/7/9-pt filters next to walls)
   double xnu0, xnu10, csens;
   const double *sr[3];  // artificial viscosity, Edoh switch: spectral radii of U^n (see ViscArgs)
-  const double *idx, *idy, *idz;  // artificial viscosity, Edoh switch: Cartesian metrics (null on c3)
+  // artificial viscosity, Edoh switch: Cartesian metrics (null where the
+  // spectral radius carries its metric: all directions of c3 blocks, i and j
+  // of 2.5-D blocks)
+  const double *idx, *idy, *idz;
   double dt;            // artificial viscosity (U += deltat*K), Edoh switch (local CFL)
   int edoh;             // shock filter: Edoh switch
+  int avs;              // artificial viscosity with shock capturing (k_shock_wall)
   RefState ref;        // mixed precision only
 };
 
@@ -358,13 +362,19 @@ hipError_t launch_vdiv(const Batch<VdivArgs> *tab, int nb, int ntiles, int geo, 
 TileRange filter_tiles(const Geom &g, int kchunk);
 hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, bool wall,
                             hipStream_t s);
-// The shock filter, or (artvisc) the artificial viscosity, which shares its
-// face-dissipation structure; `wall`: the shock filter with the closures of
-// blocks with j-walls (FP64); `edoh`: the shock filter's Edoh switch.
-hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, bool artvisc,
+// The operators sharing the shock filter's face-dissipation structure.
+enum DissipKind {
+  kDissipShock = 0,         // filter_o10_shock: U_n += K
+  kDissipArtvisc = 1,       // artvisc_o9: U_n += deltat*K, spectral-radius weighted d10 faces
+  kDissipArtviscShock = 2,  // artvisc_o9_shock: U_n += deltat*K, spectral-radius weighted shock faces
+};
+// `wall`: the closures of blocks with j-walls (shock filter FP64 or mixed,
+// artvisc_o9_shock FP64); `edoh`: the shock filter's Edoh switch.
+hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, DissipKind kind,
                                bool wall, bool edoh, hipStream_t s);
-// The shock filter's j-increment of the rows next to the j-walls of one
-// block, after launch_filter_shock(.., wall) (FP64).
+// The j-increment of the rows next to the j-walls of one block, after
+// launch_filter_shock(.., wall): the shock filter (FP64 or mixed, a.avs = 0)
+// or artvisc_o9_shock (a.avs = 1).
 hipError_t launch_shock_wall(const FilterArgs &a, hipStream_t s);
 hipError_t launch_iface(const IfaceArgs &a, int nchunks, hipStream_t s);
 hipError_t launch_iface(const IfaceArgs32 &a, int nchunks, hipStream_t s);

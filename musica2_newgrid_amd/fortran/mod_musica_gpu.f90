@@ -15,7 +15,7 @@ module mod_musica_gpu
 
   integer(c_int32_t), parameter :: MGPU_FACE_PERIODIC = 0, MGPU_FACE_INTERFACE = 1, MGPU_FACE_WALL = 2
   integer(c_int32_t), parameter :: MGPU_FILTER_NONE = 0, MGPU_FILTER_SF = 1, MGPU_FILTER_SHOCK = 2
-  integer(c_int32_t), parameter :: MGPU_FILTER_ARTVISC = 3
+  integer(c_int32_t), parameter :: MGPU_FILTER_ARTVISC = 3, MGPU_FILTER_ARTVISC_SHOCK = 4
   integer(c_int32_t), parameter :: MGPU_PREC_FP64 = 0, MGPU_PREC_MIXED = 1
   integer(c_int32_t), parameter :: MGPU_GEOM_CART = 0, MGPU_GEOM_C3 = 1, MGPU_GEOM_C2 = 2
   integer(c_int32_t), parameter :: MGPU_LAUNCH_EAGER = 1

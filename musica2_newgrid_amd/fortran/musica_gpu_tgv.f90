@@ -85,16 +85,13 @@ program musica_gpu_tgv
   cs%curvilinear = 0
   cs%iorder_visc = prm%iorder_visc
   ! num_dissip as assign_procedures binds it (is_dissip_in_increments = F):
-  ! filtering_11pts, filter_o10_shock, or artvisc_o9 when is_SF = F
+  ! filtering_11pts, filter_o10_shock, or artvisc_o9 / artvisc_o9_shock when is_SF = F
   if (prm%is_SF) then
      cs%filter = MGPU_FILTER_SF
      if (prm%is_shock) cs%filter = MGPU_FILTER_SHOCK
   else
-     if (prm%is_shock) then
-        write(*,*) 'artificial viscosity with shock capturing (artvisc_o9_shock) is not supported'
-        stop 1
-     endif
      cs%filter = MGPU_FILTER_ARTVISC
+     if (prm%is_shock) cs%filter = MGPU_FILTER_ARTVISC_SHOCK
   endif
   cs%is_ducros = merge(1, 0, prm%is_ducros)
   cs%is_sw_edoh = merge(1, 0, prm%is_sw_edoh)

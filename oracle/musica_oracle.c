@@ -578,7 +578,9 @@ static void filter_sf(const orc_cfg *c, const dims_t *d, double *const Un[5], do
 }
 
 /* ------------------------------------------------------------------------ */
-/* Shock filter (filter_o10_shock, mod_filtering_shock.f90:204-1512):       */
+/* Shock filter (filter_o10_shock, mod_filtering_shock.f90:204-1512), and   */
+/* with filter 4 the artificial viscosity with shock capturing that shares  */
+/* its sensor and faces (artvisc_o9_shock, mod_artvisc_shock.f90:113-1287): */
 /* Ducros sensor (:223-239), pressure sensor per direction (:243-260), face  */
 /* dissipation D_{i+1/2} on 0..n (:419-440) and K_i = D_{i-1/2} - D_{i+1/2}  */
 /* accumulated over directions (:690-700).  Along an axis with no-slip walls */
@@ -586,6 +588,27 @@ static void filter_sf(const orc_cfg *c, const dims_t *d, double *const Un[5], do
 /* the pressure of the ghost state, which nothing on the path changes), the  */
 /* faces over 1..n-1 and K over 2..n-1.                                      */
 /* ------------------------------------------------------------------------ */
+/* Spectral radius of U^n along dir at p and the metric its face differences
+ * are multiplied by: Cartesian |u_dir| + c and idx_dir(r); curvilinear (c3,
+ * and the i, j directions of a 2.5-D block) |contravariant velocity| +
+ * c |grad xi_dir| |1/J| and 1 (artvisc_o9 mod_artvisc.f90:324-372;
+ * artvisc_o9_shock mod_artvisc_shock.f90:431-470; g_ksi grid_metrics_curv.f90:709-724,
+ * g3_ksi smod_grid_metrics3_c3.f90:161). */
+static inline double spec_radius(const orc_cfg *c, const orc_grid *g, const dims_t *d, const work_t *w, int dir,
+                                 long p, int r, double *met) {
+  if (c->curv == 1 || (c->curv == 2 && dir < 2)) {
+    const double mx = mtr(c, g->m, g->m2, d, 3 * dir, p), my = mtr(c, g->m, g->m2, d, 3 * dir + 1, p),
+                 mz = mtr(c, g->m, g->m2, d, 3 * dir + 2, p), ij = mtr(c, g->m, g->m2, d, 9, p);
+    const double uc = (w->uu[p] * mx + w->vv[p] * my + w->ww[p] * mz) * ij;
+    *met = 1.0;
+    return fabs(uc) + w->cs[p] * (sqrt(mx * mx + my * my + mz * mz) * fabs(ij));
+  }
+  const double *vel = dir == 0 ? w->uu : (dir == 1 ? w->vv : w->ww);
+  const double *m = dir == 0 ? g->idx : (dir == 1 ? g->idy : g->idz);
+  *met = m[r - 1 + NG];
+  return fabs(vel[p]) + w->cs[p];
+}
+
 static void filter_shock(const orc_cfg *c, const orc_grid *g, const dims_t *d, double *const Un[5],
                          const work_t *w, double *K[5]) {
   const double xnu0 = -c->dissip_shock, cs = c->csens;
@@ -654,7 +677,12 @@ static void filter_shock(const orc_cfg *c, const orc_grid *g, const dims_t *d, d
             long p = p0 + m * s;
             double xk0 = xnu0 * fmax(ps[m + 1], ps[m]);
             if (hf == 1) xk0 = xk0 / 10.0;
-            const double xkh = fmax(0.0, xnu[hf] + (hf == 5 ? xk0 / 105.0 : xk0));
+            /* artvisc_o9_shock scales the sensor term of faces 3 and 4 from a wall
+             * by 1/5 and 3/70 (mod_artvisc_shock.f90:577-695, :744-862) */
+            double xks = hf == 5 ? xk0 / 105.0 : xk0;
+            if (c->filter == 4 && hf == 3) xks = xk0 / 5.0;
+            if (c->filter == 4 && hf == 4) xks = 3.0 / 70.0 * xk0;
+            const double xkh = fmax(0.0, xnu[hf] + xks);
             for (int v = 0; v < 5; v++) {
               const double *u = Un[v];
               double dd = 0.0;
@@ -690,9 +718,21 @@ static void filter_shock(const orc_cfg *c, const orc_grid *g, const dims_t *d, d
               }
               mul = fmin(1.0, cfl / c->chi);
             }
+            /* artvisc_o9_shock: the faces weighted by the spectral radius of U^n
+             * averaged to them, times the metric (mod_artvisc_shock.f90:480-550 i,
+             * :866-935 j, :1243-1283 k); the wall rows get no increment along
+             * the wall axis (K over ndy_d..nfy_d) */
+            double srm = 0.0, srp = 0.0, met = 1.0;
+            if (c->filter == 4) {
+              double m1, m2;
+              const double sr = spec_radius(c, g, d, w, dir, p, i, &met);
+              srp = 0.5 * (spec_radius(c, g, d, w, dir, p + s, i + 1, &m1) + sr);
+              srm = 0.5 * (spec_radius(c, g, d, w, dir, p - s, i - 1, &m2) + sr);
+            }
             for (int v = 0; v < 5; v++) {
               const double *Dv = D + v * (n + 1);
-              const double kd = edge ? 0.0 : (c->sw_edoh ? mul * (Dv[i - 1] - Dv[i]) : Dv[i - 1] - Dv[i]);
+              double kd = edge ? 0.0 : (c->sw_edoh ? mul * (Dv[i - 1] - Dv[i]) : Dv[i - 1] - Dv[i]);
+              if (c->filter == 4) kd = edge ? 0.0 : (srm * Dv[i - 1] - srp * Dv[i]) * met;
               if (dir == 0) K[v][p] = kd;
               else K[v][p] = K[v][p] + kd;
             }
@@ -707,11 +747,13 @@ static void filter_shock(const orc_cfg *c, const orc_grid *g, const dims_t *d, d
 
 /* ------------------------------------------------------------------------ */
 /* Artificial viscosity (artvisc_o9, mod_artvisc.f90:82-874; coefficients   */
-/* init_artvisc :16-79), Cartesian, periodic faces: per direction the face  */
-/* dissipation D_{i+1/2} = sum_l d10(l) (u(i+l) - u(i+1-l)) on 0..n with     */
-/* d10 = (126,-84,36,-9,1) chi/1260, weighted by the spectral radius        */
-/* |u_d| + c of U^n averaged to the faces, K = (sr_{i+1/2} D_{i+1/2} -      */
-/* sr_{i-1/2} D_{i-1/2}) * idx_d(i), x setting K and y, z adding to it.     */
+/* init_artvisc :16-79), periodic faces: per direction the face dissipation */
+/* D_{i+1/2} = sum_l d10(l) (u(i+l) - u(i+1-l)) on 0..n with                */
+/* d10 = (126,-84,36,-9,1) chi/1260, weighted by the spectral radius of U^n */
+/* averaged to the faces, K = (sr_{i+1/2} D_{i+1/2} - sr_{i-1/2} D_{i-1/2}) */
+/* * metric, x setting K and y, z adding to it: Cartesian |u_d| + c and     */
+/* idx_d(i); 2.5-D (is_curv, :324-372) i, j with the contravariant velocity */
+/* and g_ksi / g_eta and metric 1, z Cartesian (spec_radius).               */
 /* w holds primitives_visc of U^n (velocity and sound speed).               */
 /* ------------------------------------------------------------------------ */
 static void artvisc(const orc_cfg *c, const orc_grid *g, const dims_t *d, double *const Un[5], const work_t *w,
@@ -724,8 +766,6 @@ static void artvisc(const orc_cfg *c, const orc_grid *g, const dims_t *d, double
     const long s = st[dir];
     const int n = nn[dir];
     const int o1 = dir == 0 ? 1 : 0, o2 = dir == 2 ? 1 : 2;
-    const double *vel = dir == 0 ? w->uu : (dir == 1 ? w->vv : w->ww);
-    const double *met = dir == 0 ? g->idx : (dir == 1 ? g->idy : g->idz);
 #pragma omp parallel
     {
       double *D = (double *)malloc(sizeof(double) * 5 * (size_t)(n + 1));
@@ -746,10 +786,10 @@ static void artvisc(const orc_cfg *c, const orc_grid *g, const dims_t *d, double
           }
           for (int i = 1; i <= n; i++) {
             const long p = p0 + i * s;
-            const double sr = fabs(vel[p]) + w->cs[p];
-            const double srp = 0.5 * (fabs(vel[p + s]) + w->cs[p + s] + sr);
-            const double srm = 0.5 * (fabs(vel[p - s]) + w->cs[p - s] + sr);
-            const double m = met[i - 1 + NG];
+            double m, m1, m2;
+            const double sr = spec_radius(c, g, d, w, dir, p, i, &m);
+            const double srp = 0.5 * (spec_radius(c, g, d, w, dir, p + s, i + 1, &m1) + sr);
+            const double srm = 0.5 * (spec_radius(c, g, d, w, dir, p - s, i - 1, &m2) + sr);
             for (int v = 0; v < 5; v++) {
               const double *Dv = D + v * (n + 1);
               const double kd = (srp * Dv[i] - srm * Dv[i - 1]) * m;
@@ -809,7 +849,8 @@ static int check_cfg(const orc_cfg *c) {
     for (int s = 0; s < 2; s++)
       if (c->face[dir][s] != ORC_FACE_PERIODIC) return -2; /* only periodic faces so far */
   if (c->filter >= 2 && !c->visc) return -3;              /* sensor / sound speed of primitives_visc */
-  if (c->filter == 3 && c->curv) return -8;               /* artificial viscosity: Cartesian */
+  if (c->filter == 3 && c->curv == 1) return -8;          /* artvisc_o9: Cartesian or 2.5-D */
+  if (c->filter < 0 || c->filter > 4) return -11;
   if (c->curv < 0 || c->curv > 2) return -9;
   if (c->nrk < 1 || c->nrk > 8) return -4;
   return 0;
@@ -863,7 +904,7 @@ void orc_filter_rhs(const orc_cfg *c, const orc_grid *g, double *const Un[5], do
   dims_t d = mkdims(c);
   if (c->filter == 1) {
     filter_sf(c, &d, Un, K);
-  } else if (c->filter == 2) {
+  } else if (c->filter == 2 || c->filter == 4) {
     work_t w;
     if (work_alloc(&d, &w, 1)) { work_free(&w); return; }
     prim_visc(c, &d, Uref, &w);
@@ -916,16 +957,17 @@ int orc_step(const orc_cfg *c, const orc_grid *g, double *U[5], int nsteps) {
     if (c->filter) {
       /* num_dissip then update_varn (runge_kutta.f90:113-126; update.f90:538-565) */
       if (c->filter == 1) filter_sf(c, &d, Un, K);
-      else if (c->filter == 2) filter_shock(c, g, &d, Un, &w, K);  /* sensor on p, grads of U^n */
+      else if (c->filter == 2 || c->filter == 4) filter_shock(c, g, &d, Un, &w, K); /* sensor on p, grads of U^n */
       else artvisc(c, g, &d, Un, &w, K);               /* |u|+c of U^n */
       /* update_varn_filter adds K, update_varn_artvisc deltat*K (update.f90:538-608) */
-      const double fk = c->filter == 3 ? c->dt : 1.0;
+      const int av = c->filter >= 3;
+      const double fk = c->dt;
 #pragma omp parallel for collapse(2)
       for (int k = 1; k <= d.nz; k++)
         for (int j = 1; j <= d.ny; j++)
           for (int i = 1; i <= d.nx; i++) {
             long p = (long)ID(d, i, j, k);
-            for (int v = 0; v < 5; v++) Un[v][p] = c->filter == 3 ? Un[v][p] + fk * K[v][p] : Un[v][p] + K[v][p];
+            for (int v = 0; v < 5; v++) Un[v][p] = av ? Un[v][p] + fk * K[v][p] : Un[v][p] + K[v][p];
           }
       for (int v = 0; v < 5; v++) orc_fill_periodic(c, Un[v]);
     }
@@ -984,13 +1026,14 @@ void orc_fill_interfaces(int nblocks, const orc_block *bl, double *const *arrs) 
 int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_grid *g, double *const *U, int nsteps) {
   if (nb < 1 || nb > 64) return -1;
   if (c0->filter >= 2 && !c0->visc) return -3;
-  if (c0->filter == 3 && c0->curv) return -8;
+  if (c0->filter == 3 && c0->curv == 1) return -8;
+  if (c0->filter < 0 || c0->filter > 4) return -11;
   for (int b = 0; b < nb; b++)
     for (int f = 0; f < 6; f++)
       if (bl[b].wall[f]) {
         const int ext = f < 2 ? bl[b].nx : (f < 4 ? bl[b].ny : bl[b].nz);
         if (c0->curv) return -5;        /* walls: Cartesian blocks only */
-        if (c0->filter == 3) return -6; /* walls: selective or shock filter, or none */
+        if (c0->filter == 3) return -6; /* walls: not with artvisc_o9 */
         if (ext < 10) return -7;        /* closures of the two sides must not overlap */
       }
   orc_cfg cb[64];
@@ -1073,10 +1116,10 @@ int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_gri
     if (c0->filter) {
       for (int b = 0; b < nb; b++) {
         if (c0->filter == 1) filter_sf(&cb[b], &d[b], Un[b], K[b]);
-        else if (c0->filter == 2) filter_shock(&cb[b], &g[b], &d[b], Un[b], &w[b], K[b]);
+        else if (c0->filter == 2 || c0->filter == 4) filter_shock(&cb[b], &g[b], &d[b], Un[b], &w[b], K[b]);
         else artvisc(&cb[b], &g[b], &d[b], Un[b], &w[b], K[b]);
         const dims_t dd = d[b];
-        const int av = c0->filter == 3;
+        const int av = c0->filter >= 3;
         const double fk = c0->dt;
 #pragma omp parallel for collapse(2)
         for (int k = 1; k <= dd.nz; k++)

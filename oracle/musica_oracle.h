@@ -38,7 +38,8 @@ typedef struct orc_cfg {
   int curv;                /* 0 Cartesian (idx 1-D metrics), 1 3-D curvilinear (c3),
                             * 2 2.5-D curvilinear (2-D metrics in i,j, Cartesian z) */
   int visc;                /* 1: 5-pt viscous terms in the last RK stage   */
-  int filter;              /* 0 none, 1 selective filter, 2 shock filter, 3 artificial viscosity (Cartesian) */
+  int filter;              /* 0 none, 1 selective filter, 2 shock filter, 3 artificial viscosity
+                            * (Cartesian or 2.5-D), 4 artificial viscosity with shock capturing */
   int ducros;              /* Ducros sensor in the shock filter            */
   int nrk;                 /* number of RK stages                          */
   int face[3][2];          /* enum orc_face_kind per (direction, side)      */
@@ -101,8 +102,9 @@ void orc_fill_interfaces(int nblocks, const orc_block *blocks, double *const *ar
  * flags; its nx,ny,nz and face[][] are ignored.  U[5*b+v] is variable v of
  * block b, grids[b] its metrics.
  *
- * Wall faces (Cartesian blocks, selective or no filter; at least 10 cells
- * along a wall axis) follow the reference with is_wall2 = F:
+ * Wall faces (Cartesian blocks, no filter, the selective or shock filter or
+ * the artificial viscosity with shock capturing; at least 10 cells along a
+ * wall axis) follow the reference with is_wall2 = F:
  *   - no Euler increment on the wall rows (ndx_e = 2, mpi_index_bounds.f90:20-70);
  *     rows 2..5 use the centred a3/a5/a7/a9 closures with the metric row's value
  *     (smod_flux_euler_11pts.f90:36-133, the idx2..idx8 of grid_metrics_wall_cart);
@@ -110,6 +112,8 @@ void orc_fill_interfaces(int nblocks, const orc_block *blocks, double *const *ar
  *     1,2 (n-1,n) (smod_grad_vel.f90:12-100, smod_flux_visc_5pts.f90:80-160);
  *   - selective filter: nothing from the wall axis on rows 1,2, d5/d7/d9 on
  *     rows 3,4,5 (mod_filtering.f90:900-988, 1116-1190);
+ *   - shock filter and artvisc_o9_shock: faces 1..4 from a wall with d2..d8,
+ *     nothing from the wall axis on the wall rows (filter_shock);
  *   - bc_wall (smod_bc_wall.f90:11-98) at the start of stages 2..nrk and at the
  *     end of the step: zero momentum, p and (adiabatic) T extrapolated from rows
  *     2,3 of the primitives last computed (those of the previous stage's input;
@@ -134,8 +138,9 @@ void orc_euler_rhs(const orc_cfg *c, const orc_grid *g, double *const U[5], doub
 /* K += viscous flux divergence of Uref (grad_vel/grad_T + flux_visc). */
 void orc_visc_rhs(const orc_cfg *c, const orc_grid *g, double *const Uref[5], double *K[5]);
 
-/* K = numerical dissipation increment of Un (selective filter or shock
- * filter; the shock filter takes its sensor from Uref). */
+/* K = numerical dissipation increment of Un (selective filter, shock
+ * filter, artificial viscosities; the sensor, velocity and sound speed come
+ * from Uref).  The artificial viscosities' update is Un + deltat*K. */
 void orc_filter_rhs(const orc_cfg *c, const orc_grid *g, double *const Un[5],
                     double *const Uref[5], double *K[5]);
 

@@ -97,9 +97,9 @@ def _create_case(c, blocks, precision=_lib.PREC_FP64):
 
 
 def test_wall_limits_are_checked():
-    """Walls: j-faces only, Cartesian with the selective or shock filter or
-    none (not the artificial viscosity), ny >= 10; refused before any device
-    work."""
+    """Walls: j-faces only, Cartesian with the selective or shock filter, the
+    artificial viscosity with shock capturing or none (not artvisc_o9),
+    ny >= 10; refused before any device work."""
     c, blocks = _channel_table()
     blocks[0]["kind"][0] = blocks[0]["kind"][1] = _lib.FACE_WALL
     rc, msg = _create_case(c, blocks)
@@ -129,3 +129,17 @@ def test_artvisc_flags_are_checked():
     c.curvilinear = True
     rc, msg = _create_case(c, None)
     assert rc == _lib.EINVAL and "Cartesian" in msg
+
+
+def test_artvisc_shock_flags_are_checked():
+    """Artificial viscosity with shock capturing: FP64 with the viscous terms
+    (any geometry, walls allowed); filter codes above 4 are refused."""
+    c = cases.tgv(16, filter=4)
+    rc, msg = _create_case(c, None, precision=_lib.PREC_MIXED)
+    assert rc == _lib.EINVAL and "FP64 only" in msg
+    c = cases.tgv(16, filter=4, iorder_visc=0)
+    rc, msg = _create_case(c, None)
+    assert rc == _lib.EINVAL and "sound speed" in msg
+    c = cases.tgv(16, filter=5)
+    rc, msg = _create_case(c, None)
+    assert rc == _lib.EINVAL and "filter=5" in msg
