@@ -59,8 +59,8 @@ extern "C" {
  * src/File_IO/read_param.f90:75): bc_wall_*_adiabatic_dpdn or
  * _isotherm_dpdn (src/Boundary/smod_bc_wall.f90) with the boundary-row
  * closures of the Euler, viscous and filter operators.  On the j-faces (2, 3)
- * of Cartesian cases (FP64 or mixed) with the selective or shock filter or none
- * (or FP64 with the artificial viscosity with shock capturing),
+ * of Cartesian cases (FP64 or mixed) with the selective or shock filter, the
+ * artificial viscosity with shock capturing or none,
  * ny >= 10; the block's extrapolation coefficients come from mgpu_set_wall.
  * With a shock sensor the ghost row beyond a wall must hold a valid state:
  * the wall row's pressure sensor reads p there (mod_filtering_shock.f90:
@@ -72,11 +72,11 @@ extern "C" {
 #define MGPU_FILTER_SF 1    /* filtering_11pts, mod_filtering.f90:894      */
 #define MGPU_FILTER_SHOCK 2 /* filter_o10_shock, mod_filtering_shock.f90:204 */
 #define MGPU_FILTER_ARTVISC 3 /* artvisc_o9 (is_SF = F), mod_artvisc.f90:82: Cartesian or 2.5-D,
-                               * FP64, iorder_visc = 4 (it reads the sound speed of primitives_visc) */
+                               * iorder_visc = 4 (it reads the sound speed of primitives_visc) */
 #define MGPU_FILTER_ARTVISC_SHOCK 4 /* artvisc_o9_shock (is_SF = F, is_shock = T),
                                      * mod_artvisc_shock.f90:113: the shock filter's sensor-weighted
                                      * faces times the spectral radius, U += deltat*K; every geometry,
-                                     * FP64, iorder_visc = 4 */
+                                     * iorder_visc = 4 */
 
 /* Arithmetic: FP64 everywhere, or FP32 flux/derivative/filter arithmetic with
  * FP64 RK accumulation, metrics and shock sensor. */

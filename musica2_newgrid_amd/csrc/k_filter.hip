@@ -294,10 +294,9 @@ __device__ __forceinline__ double edoh_mul(const FilterArgs &a, int d, idx_t p, 
 template <bool MIX, bool ARTV, bool WALL, bool EDOH, bool AVS>
 __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> *__restrict__ tab, int nb,
                                                         int ntiles) {
-  static_assert(!(MIX && ARTV), "artificial viscosity: FP64 only");
-  static_assert(!(WALL && ARTV), "walls: not with the artificial viscosity");
-  static_assert(!(EDOH && ARTV), "the Edoh switch belongs to the shock filter");
-  static_assert(!(AVS && (MIX || ARTV || EDOH)), "artificial viscosity with shock capturing: FP64, no Edoh switch");
+  static_assert(!(WALL && ARTV), "walls: not with artvisc_o9");
+  static_assert(!(EDOH && (ARTV || AVS)), "the Edoh switch belongs to the shock filter");
+  static_assert(!(AVS && ARTV), "one artificial viscosity at a time");
   using R = typename std::conditional<MIX, float, double>::type;
   __shared__ R sux[5][FY][FX + 10];
   __shared__ R suy[5][FY + 10][FX];
@@ -308,8 +307,9 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
   __shared__ double psx[FY][FX + 2];    // psens_x at i0-1 .. i0+FX
   __shared__ double psy[FY + 2][FX];    // psens_y at j0-1 .. j0+FY
   // Edoh factors of the plane's own points (AVS: the face spectral radii
-  // sr_{-1/2}, sr_{+1/2} of the three directions, then their metrics)
-  __shared__ double smul[EDOH ? 3 : (AVS ? 9 : 1)][FY][FX];
+  // sr_{-1/2}, sr_{+1/2} of the three directions, then their metrics; ARTV:
+  // the metrics)
+  __shared__ double smul[EDOH || ARTV ? 3 : (AVS ? 9 : 1)][FY][FX];
 
   int i0, j0, kbeg, kend;
   const FilterArgs &a = tile_of(tab, nb, ntiles, i0, j0, kbeg, kend);
@@ -471,6 +471,13 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
         smul[8][ty][tx] = met_at(a.idz, k);
       }
     }
+    if constexpr (ARTV) {
+      if (own) {
+        smul[0][ty][tx] = met_at(a.idx, i);
+        smul[1][ty][tx] = met_at(a.idy, j);
+        smul[2][ty][tx] = met_at(a.idz, k);
+      }
+    }
     __syncthreads();
     if (k < kend) load_plane(k + 1);
     // pressure sensors of the plane
@@ -494,7 +501,7 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
       double K[5];
       {
         const double s0 = spx[ty][tx + 2], srp = 0.5 * (spx[ty][tx + 3] + s0), srm = 0.5 * (spx[ty][tx + 1] + s0);
-        const double m = met_at(a.idx, i);
+        const double m = smul[0][ty][tx];
 #pragma unroll
         for (int v = 0; v < 5; v++) {
           const R *x = &sux[v][ty][tx + 5];
@@ -503,7 +510,7 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
       }
       {
         const double s0 = spy[ty + 2][tx], srp = 0.5 * (spy[ty + 3][tx] + s0), srm = 0.5 * (spy[ty + 1][tx] + s0);
-        const double m = met_at(a.idy, j);
+        const double m = smul[1][ty][tx];
 #pragma unroll
         for (int v = 0; v < 5; v++) {
           const R *y = &suy[v][ty + 5][tx];
@@ -512,7 +519,7 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
       }
       {
         const double s0 = qp[1], srp = 0.5 * (qp[2] + s0), srm = 0.5 * (qp[0] + s0);
-        const double m = met_at(a.idz, k);
+        const double m = smul[2][ty][tx];
 #pragma unroll
         for (int v = 0; v < 5; v++) {
           const double dlo = R(D10_1) * (q[5][v] - q[4][v]) + R(D10_2) * (q[6][v] - q[3][v]) +
@@ -525,7 +532,10 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
         }
       }
 #pragma unroll
-      for (int v = 0; v < 5; v++) stg_nt(uoutp.v[v], pk, (double)q[5][v] + a.dt * K[v]);
+      for (int v = 0; v < 5; v++) {
+        if constexpr (MIX) stg_nt(uoutp.v[v], pk, u64[v] + a.dt * K[v]);
+        else stg_nt(uoutp.v[v], pk, q[5][v] + a.dt * K[v]);
+      }
     }
     if (!ARTV && own) {
       R K[5];
@@ -605,7 +615,8 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
       }
 #pragma unroll
       for (int v = 0; v < 5; v++) {
-        if constexpr (AVS) stg_nt(uoutp.v[v], pk, q[5][v] + a.dt * K[v]);
+        if constexpr (AVS && MIX) stg_nt(uoutp.v[v], pk, u64[v] + a.dt * (double)K[v]);
+        else if constexpr (AVS) stg_nt(uoutp.v[v], pk, q[5][v] + a.dt * K[v]);
         else if constexpr (MIX) stg_nt(uoutp.v[v], pk, u64[v] + (double)K[v]);
         else stg_nt(uoutp.v[v], pk, q[5][v] + K[v]);
       }
@@ -761,12 +772,19 @@ hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int nt, boo
                                bool edoh, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
   if (kind == kDissipArtvisc) {
-    if (mixed || wall || edoh) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_filter_shock<false, true, false, false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    if (wall || edoh) return hipErrorInvalidValue;
+    if (mixed)
+      hipLaunchKernelGGL((k_filter_shock<true, true, false, false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    else
+      hipLaunchKernelGGL((k_filter_shock<false, true, false, false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   } else if (kind == kDissipArtviscShock) {
-    if (mixed || edoh) return hipErrorInvalidValue;
-    if (wall)
+    if (edoh) return hipErrorInvalidValue;
+    if (wall && mixed)
+      hipLaunchKernelGGL((k_filter_shock<true, false, true, false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    else if (wall)
       hipLaunchKernelGGL((k_filter_shock<false, false, true, false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    else if (mixed)
+      hipLaunchKernelGGL((k_filter_shock<true, false, false, false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
     else
       hipLaunchKernelGGL((k_filter_shock<false, false, false, false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   } else {

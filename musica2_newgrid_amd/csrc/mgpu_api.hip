@@ -857,8 +857,6 @@ std::string check_flags(const mgpu_case &c) {
     snprintf(buf, sizeof buf, "artificial viscosity needs iorder_visc=4 (it reads the sound speed of U^n)");
   else if (c.filter == MGPU_FILTER_ARTVISC && c.curvilinear == MGPU_GEOM_C3)
     snprintf(buf, sizeof buf, "artificial viscosity: Cartesian or 2.5-D cases (artvisc_o9 has no c3 form)");
-  else if (is_artvisc(c) && c.precision != MGPU_PREC_FP64)
-    snprintf(buf, sizeof buf, "artificial viscosity: FP64 only");
   else if (c.precision != MGPU_PREC_FP64 && c.precision != MGPU_PREC_MIXED)
     snprintf(buf, sizeof buf, "precision=%d", c.precision);
   else if (c.precision == MGPU_PREC_MIXED && c.nrk < 2)

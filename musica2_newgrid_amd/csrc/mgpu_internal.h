@@ -368,13 +368,14 @@ enum DissipKind {
   kDissipArtvisc = 1,       // artvisc_o9: U_n += deltat*K, spectral-radius weighted d10 faces
   kDissipArtviscShock = 2,  // artvisc_o9_shock: U_n += deltat*K, spectral-radius weighted shock faces
 };
-// `wall`: the closures of blocks with j-walls (shock filter FP64 or mixed,
-// artvisc_o9_shock FP64); `edoh`: the shock filter's Edoh switch.
+// `wall`: the closures of blocks with j-walls (not artvisc_o9); `edoh`: the
+// shock filter's Edoh switch; `mixed`: FP32 faces of U - U_ref, FP64 update.
 hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, DissipKind kind,
                                bool wall, bool edoh, hipStream_t s);
 // The j-increment of the rows next to the j-walls of one block, after
-// launch_filter_shock(.., wall): the shock filter (FP64 or mixed, a.avs = 0)
-// or artvisc_o9_shock (a.avs = 1).
+// launch_filter_shock(.., wall), in FP64 (the filter's input is the last
+// stage's FP64 output): the shock filter (a.avs = 0) or artvisc_o9_shock
+// (a.avs = 1).
 hipError_t launch_shock_wall(const FilterArgs &a, hipStream_t s);
 hipError_t launch_iface(const IfaceArgs &a, int nchunks, hipStream_t s);
 hipError_t launch_iface(const IfaceArgs32 &a, int nchunks, hipStream_t s);

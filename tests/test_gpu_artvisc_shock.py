@@ -3,8 +3,9 @@ and the 2.5-D artvisc_o9 on the GPU against the oracle, through the C-ABI,
 FP64 within 1e-12 relative: the TGV (Ducros sensor off and on), the c3 and
 2.5-D wavy-grid vortices, the walled channel (reduced faces next to the
 walls in k_shock_wall), the TGV cut into three blocks with reversed axes
-(spectral radii, p and the sensor cross the interfaces), and graph replays
-against eager launches and the autotuner's tiles."""
+(spectral radii, p and the sensor cross the interfaces), graph replays
+against eager launches and the autotuner's tiles; the mixed path's density
+error against the FP64 oracle."""
 import numpy as np
 import pytest
 
@@ -97,3 +98,29 @@ def test_graph_matches_eager_and_tuned(which):
     e = relerr(_gpu(c, 4, tune=True), _oracle(c, 4))
     print("tuned relerr", e, flush=True)
     assert e < 1e-12
+
+
+@pytest.mark.parametrize("which", ["tgv_artvisc", "tgv_ducros", "vortex_c3", "vortex_c2", "channel"])
+def test_mixed_precision_error(which):
+    """Mixed path (FP32 faces of U - U_ref, FP64 update): density error
+    against the FP64 oracle after 200 steps (100 for the channel)."""
+    from musica2_newgrid_amd import _lib
+    from musica2_newgrid_amd.solver import GpuSolver
+    if which == "tgv_artvisc":
+        c, nsteps = cases.tgv(24, filter=3), 200
+    elif which in ("vortex_c3", "vortex_c2"):
+        c = (cases.vortex_wavy(32, 24, 16, filter=4) if which == "vortex_c3"
+             else cases.vortex_wavy_c2(32, 24, 16, filter=4))
+        nsteps = 200
+    else:
+        c, _ = _case(which)
+        nsteps = 100 if which == "channel" else 200
+    print(f"mixed precision, filter {c.filter}, {which} {c.nx}x{c.ny}x{c.nz}, {nsteps} steps", flush=True)
+    with GpuSolver(c, precision=_lib.PREC_MIXED) as s:
+        s.step(nsteps)
+        Ug = s.get_state()
+    Uc = _oracle(c, nsteps)
+    a, b = gr.interior(Ug[0]), gr.interior(Uc[0])
+    linf = float(np.max(np.abs(a - b))) / float(np.max(np.abs(b)))
+    print(f"rho Linf {linf:.3e} vs the FP64 oracle", flush=True)
+    assert 0.0 < linf <= 1e-5

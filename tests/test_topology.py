@@ -118,10 +118,7 @@ def test_wall_limits_are_checked():
 
 
 def test_artvisc_flags_are_checked():
-    """Artificial viscosity: Cartesian FP64 with the viscous terms."""
-    c = cases.tgv(16, filter=3)
-    rc, msg = _create_case(c, None, precision=_lib.PREC_MIXED)
-    assert rc == _lib.EINVAL and "artificial viscosity" in msg
+    """Artificial viscosity: Cartesian or 2.5-D with the viscous terms."""
     c = cases.tgv(16, filter=3, iorder_visc=0)
     rc, msg = _create_case(c, None)
     assert rc == _lib.EINVAL and "sound speed" in msg
@@ -132,11 +129,8 @@ def test_artvisc_flags_are_checked():
 
 
 def test_artvisc_shock_flags_are_checked():
-    """Artificial viscosity with shock capturing: FP64 with the viscous terms
-    (any geometry, walls allowed); filter codes above 4 are refused."""
-    c = cases.tgv(16, filter=4)
-    rc, msg = _create_case(c, None, precision=_lib.PREC_MIXED)
-    assert rc == _lib.EINVAL and "FP64 only" in msg
+    """Artificial viscosity with shock capturing: the viscous terms (any
+    geometry, walls allowed); filter codes above 4 are refused."""
     c = cases.tgv(16, filter=4, iorder_visc=0)
     rc, msg = _create_case(c, None)
     assert rc == _lib.EINVAL and "sound speed" in msg
