@@ -252,9 +252,11 @@ typedef struct mgpu_server mgpu_server;
 int mgpu_server_create(int64_t hbm_budget_bytes, mgpu_server **out);
 
 /* Queue a request.  c->nblocks must be 1 with periodic faces; metrics: the 6
- * Cartesian arrays idx,idy,idz,idx_v,idy_v,idz_v, or (curvilinear) the 10
- * c3 arrays followed by the 10 viscous ones; state: rho..rhoe padded.  The
- * inputs are copied.  *id receives the request id. */
+ * Cartesian arrays idx,idy,idz,idx_v,idy_v,idz_v, or (MGPU_GEOM_C3) the 10
+ * c3 arrays followed by the 10 viscous ones, or (MGPU_GEOM_C2) the 5 2-D
+ * arrays of mgpu_set_metrics_c2, its 5 viscous ones, idz and idz_v (the
+ * viscous entries may be null for inviscid cases); state: rho..rhoe padded.
+ * The inputs are copied.  *id receives the request id. */
 int mgpu_server_submit(mgpu_server *s, const mgpu_case *c, const double *const *metrics,
                        const double *const *state, int nsteps, int64_t *id);
 

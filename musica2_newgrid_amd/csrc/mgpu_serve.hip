@@ -34,7 +34,7 @@ struct Request {
   int nsteps = 0, done_steps = 0;
   int state = 0;  // 0 queued, 1 resident, 2 done
   size_t dev_bytes = 0;
-  std::vector<double> metrics;  // host copies (6 1-D arrays, or 20 padded arrays)
+  std::vector<double> metrics;  // host copies (6 1-D arrays, 20 padded arrays, or 10 2-D + 2 1-D)
   std::vector<double> u;        // 5 padded arrays: initial state, then the result
   size_t n = 0;                 // padded points
 };
@@ -95,7 +95,18 @@ int admit(mgpu_server *s, Request *r) {
   rc = handle_add_block(g.h, r->blk, r->c);
   if (rc) return sfail(s, rc, g.h->err);
   const int b = (int)g.h->b.size() - 1;
-  if (r->c.curvilinear) {
+  if (r->c.curvilinear == MGPU_GEOM_C2) {
+    const size_t n2 = (size_t)(r->blk.nx + 2 * G) * (r->blk.ny + 2 * G);
+    const double *m2[5], *m2v[5];
+    for (int q = 0; q < 5; q++) {
+      m2[q] = r->metrics.data() + q * n2;
+      m2v[q] = r->metrics.data() + (5 + q) * n2;
+    }
+    const double *idz = r->metrics.data() + 10 * n2;
+    const double *idz_v = idz + (r->blk.nz + 2 * G);
+    const bool visc = r->c.iorder_visc != 0;
+    rc = put_metrics_c2(g.h, b, m2, visc ? m2v : nullptr, idz, visc ? idz_v : nullptr);
+  } else if (r->c.curvilinear) {
     const size_t n = r->n;
     const double *m[10], *mv[10];
     for (int q = 0; q < 10; q++) {
@@ -146,8 +157,6 @@ int mgpu_server_submit(mgpu_server *s, const mgpu_case *c, const double *const *
   if (c->nblocks != 1 || !c->blocks) return sfail(s, MGPU_EINVAL, "a request is one block (nblocks = 1)");
   const std::string msg = check_flags(*c);
   if (!msg.empty()) return sfail(s, MGPU_EINVAL, msg);
-  if (c->curvilinear == MGPU_GEOM_C2)
-    return sfail(s, MGPU_EUNSUP, "requests are Cartesian or 3-D curvilinear (mgpu_set_metrics_c3 layout)");
   const mgpu_block &bl = c->blocks[0];
   for (int f = 0; f < 6; f++)
     if (bl.face_kind[f] != MGPU_FACE_PERIODIC)
@@ -165,7 +174,19 @@ int mgpu_server_submit(mgpu_server *s, const mgpu_case *c, const double *const *
   r->n = n;
   r->dev_bytes = block_bytes(*c, bl.nx, bl.ny, bl.nz);
   if ((int64_t)r->dev_bytes > s->budget) return sfail(s, MGPU_EINVAL, "request larger than the whole budget");
-  if (c->curvilinear) {
+  if (c->curvilinear == MGPU_GEOM_C2) {
+    const size_t n2 = (size_t)(bl.nx + 2 * G) * (bl.ny + 2 * G), nz = (size_t)bl.nz + 2 * G;
+    const bool visc = c->iorder_visc != 0;
+    r->metrics.resize(10 * n2 + 2 * nz, 0.0);
+    for (int q = 0; q < (visc ? 10 : 5); q++) {
+      if (!metrics[q]) return sfail(s, MGPU_EINVAL, "metric array is null");
+      memcpy(r->metrics.data() + q * n2, metrics[q], n2 * sizeof(double));
+    }
+    for (int q = 0; q < (visc ? 2 : 1); q++) {
+      if (!metrics[10 + q]) return sfail(s, MGPU_EINVAL, "metric array is null");
+      memcpy(r->metrics.data() + 10 * n2 + q * nz, metrics[10 + q], nz * sizeof(double));
+    }
+  } else if (c->curvilinear) {
     const int na = c->iorder_visc ? 20 : 10;
     r->metrics.resize(20 * n, 0.0);
     for (int q = 0; q < na; q++) {

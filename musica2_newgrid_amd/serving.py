@@ -39,7 +39,11 @@ class Server:
 
     def submit(self, case, nsteps: int, precision: int = _lib.PREC_FP64) -> int:
         cs, blocks = make_case_struct(case, precision)
-        if case.curvilinear:
+        if case.curvilinear == 2:
+            arrs = ([np.ascontiguousarray(a) for a in case.m2] + [np.ascontiguousarray(a) for a in case.m2v] +
+                    [np.ascontiguousarray(case.cart[2], dtype=np.float64),
+                     np.ascontiguousarray(case.cart[5], dtype=np.float64)])
+        elif case.curvilinear:
             arrs = [np.ascontiguousarray(a) for a in case.m] + [np.ascontiguousarray(a) for a in case.mv]
         else:
             arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in case.cart]

@@ -13,7 +13,9 @@ pytestmark = pytest.mark.gpu
 def _requests():
     return [(cases.tgv(24), 7, _lib.PREC_FP64), (cases.vortex_periodic(32, re=900.0), 5, _lib.PREC_MIXED),
             (cases.tgv(32, re=3000.0), 9, _lib.PREC_MIXED), (cases.vortex_periodic(24), 6, _lib.PREC_FP64),
-            (cases.vortex_wavy(24, 16, 12, filter=2), 4, _lib.PREC_FP64)]
+            (cases.vortex_wavy(24, 16, 12, filter=2), 4, _lib.PREC_FP64),
+            (cases.vortex_wavy_c2(24, 16, 12, filter=2), 5, _lib.PREC_FP64),
+            (cases.vortex_wavy_c2(28, 20, 12, filter=4), 4, _lib.PREC_MIXED)]
 
 
 def test_each_result_is_the_same_alone_and_in_the_mix():
