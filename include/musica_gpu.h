@@ -58,10 +58,12 @@ extern "C" {
 /* No-slip wall, BC_face%sort = 0 with is_wall2 = F (the reference's default,
  * src/File_IO/read_param.f90:75): bc_wall_*_adiabatic_dpdn or
  * _isotherm_dpdn (src/Boundary/smod_bc_wall.f90) with the boundary-row
- * closures of the Euler, viscous and filter operators.  On the j-faces (2, 3)
- * of Cartesian cases (FP64 or mixed) with the selective or shock filter, the
- * artificial viscosity with shock capturing or none,
- * ny >= 10; the block's extrapolation coefficients come from mgpu_set_wall.
+ * closures of the Euler, viscous and filter operators.  On the faces of one
+ * axis per block (i, j or k; a block with walls on its i- or k-faces is stored
+ * with that axis exchanged with j, invisibly to the caller) of Cartesian cases
+ * (FP64 or mixed) with the selective or shock filter, the artificial
+ * viscosity with shock capturing or none, at least 10 cells between the
+ * walls; the block's extrapolation coefficients come from mgpu_set_wall.
  * With a shock sensor the ghost row beyond a wall must hold a valid state:
  * the wall row's pressure sensor reads p there (mod_filtering_shock.f90:
  * 243-260 with ndy_d1 = 1), and nothing on the path writes it. */

@@ -165,6 +165,118 @@ void face_axes(int f, int &a, int &t1, int &t2) {
 // Checks the block table: neighbours exist, faces match in extent under the
 // orientation, and every interface is declared from both sides with the
 // inverse orientation.  Returns an error message or an empty string.
+// ---- axis relabeling of blocks with walls on the i- or k-faces ----
+// face f of the caller's block is face kFaceMap[perm][f] of the stored block
+// (and back: the map is its own inverse); the caller's axis a is the stored
+// axis kAxisMap[perm][a]; momentum v of the caller is stored as kVarMap[perm][v].
+constexpr int kFaceMap[3][6] = {{0, 1, 2, 3, 4, 5}, {2, 3, 0, 1, 4, 5}, {0, 1, 4, 5, 2, 3}};
+constexpr int kAxisMap[3][3] = {{0, 1, 2}, {1, 0, 2}, {0, 2, 1}};
+constexpr int kVarMap[3][5] = {{0, 1, 2, 3, 4}, {0, 2, 1, 3, 4}, {0, 1, 3, 2, 4}};
+
+// 0: walls (if any) on the j-faces; 1: on the i-faces; 2: on the k-faces;
+// -1: walls on more than one axis.
+int wall_perm(const mgpu_block &b) {
+  int axes = 0, perm = 0;
+  for (int a = 0; a < 3; a++)
+    if (b.face_kind[2 * a] == MGPU_FACE_WALL || b.face_kind[2 * a + 1] == MGPU_FACE_WALL) {
+      axes++;
+      perm = a == 0 ? 1 : (a == 2 ? 2 : 0);
+    }
+  return axes > 1 ? -1 : perm;
+}
+
+// The stored block's tangential axes of a face (the two other axes in
+// increasing order) come out in the caller's reverse order for the caller's
+// k-faces under perm 1 and i-faces under perm 2.
+bool tangential_flipped(int perm, int f) { return (perm == 1 && f / 2 == 2) || (perm == 2 && f / 2 == 0); }
+
+// Orientation code of an interface after relabeling: the caller's code maps
+// this face's tangential coordinates (p, q) to the neighbour's (s1, s2) by a
+// signed permutation; the stored faces see (p, q) and (s1, s2) in swapped
+// order where flipped.  The composed signed permutation is re-encoded.
+int relabel_orient(int code, bool flip_self, bool flip_nbr) {
+  // M maps (p, q) to (s1, s2): rows of a signed permutation matrix
+  int M[2][2] = {{0, 0}, {0, 0}};
+  const bool sw = code & 1;
+  M[0][sw ? 1 : 0] = (code & 2) ? -1 : 1;
+  M[1][sw ? 0 : 1] = (code & 4) ? -1 : 1;
+  int C[2][2];
+  for (int r = 0; r < 2; r++)
+    for (int col = 0; col < 2; col++) {
+      const int rr = flip_nbr ? 1 - r : r, cc = flip_self ? 1 - col : col;
+      C[r][col] = M[rr][cc];
+    }
+  const bool sw2 = C[0][0] == 0;
+  const int s1 = sw2 ? C[0][1] : C[0][0], s2 = sw2 ? C[1][0] : C[1][1];
+  return (sw2 ? 1 : 0) | (s1 < 0 ? 2 : 0) | (s2 < 0 ? 4 : 0);
+}
+
+// The stored block table of a case (see BlockData::perm); "" or an error.
+std::string relabel_blocks(const mgpu_case &c, std::vector<mgpu_block> &out, std::vector<int> &perm) {
+  char buf[256];
+  const int nb = c.nblocks;
+  out.assign(c.blocks, c.blocks + nb);
+  perm.assign(nb, 0);
+  for (int b = 0; b < nb; b++) {
+    perm[b] = wall_perm(c.blocks[b]);
+    if (perm[b] < 0) {
+      snprintf(buf, sizeof buf, "block %d: walls on more than one axis are not supported", b);
+      return buf;
+    }
+  }
+  for (int b = 0; b < nb; b++) {
+    const mgpu_block &U = c.blocks[b];
+    mgpu_block &S = out[b];
+    const int ext[3] = {U.nx, U.ny, U.nz};
+    int se[3];
+    for (int a = 0; a < 3; a++) se[kAxisMap[perm[b]][a]] = ext[a];
+    S.nx = se[0];
+    S.ny = se[1];
+    S.nz = se[2];
+    for (int fu = 0; fu < 6; fu++) {
+      const int fs = kFaceMap[perm[b]][fu];
+      S.face_kind[fs] = U.face_kind[fu];
+      S.nbr_block[fs] = U.nbr_block[fu];
+      S.nbr_face[fs] = U.nbr_face[fu];
+      S.orient[fs] = U.orient[fu];
+      if (U.face_kind[fu] != MGPU_FACE_INTERFACE) continue;
+      const int n = U.nbr_block[fu], g = U.nbr_face[fu];
+      if (n < 0 || n >= nb || g < 0 || g > 5) continue;  // reported by check_topology
+      S.nbr_face[fs] = kFaceMap[perm[n]][g];
+      S.orient[fs] = relabel_orient(U.orient[fu], tangential_flipped(perm[b], fu), tangential_flipped(perm[n], g));
+    }
+  }
+  return "";
+}
+
+// A caller's padded array of block extents (as the caller sees them) to the
+// stored layout (to_stored) or back.
+void relabel_array(const Geom &g, int perm, const double *src, double *dst, bool to_stored) {
+  const long Q0 = g.nx + 2 * G, Q1 = g.ny + 2 * G, Q2 = g.nz + 2 * G;
+  long P0 = Q0, P1 = Q1;  // the caller's padded extents along its i and j
+  if (perm == 1) {
+    P0 = Q1;
+    P1 = Q0;
+  } else if (perm == 2) {
+    P1 = Q2;
+  }
+  for (long c = 0; c < Q2; c++)
+    for (long b = 0; b < Q1; b++)
+      for (long a = 0; a < Q0; a++) {
+        long ui = a, uj = b, uk = c;
+        if (perm == 1) {
+          ui = b;
+          uj = a;
+        } else if (perm == 2) {
+          uj = c;
+          uk = b;
+        }
+        const long s = a + Q0 * (b + Q1 * c), u = ui + P0 * (uj + P1 * uk);
+        if (to_stored) dst[s] = src[u];
+        else dst[u] = src[s];
+      }
+}
+
 std::string check_topology(const mgpu_case &c) {
   char buf[256];
   for (int b = 0; b < c.nblocks; b++) {
@@ -189,7 +301,8 @@ std::string check_topology(const mgpu_case &c) {
           return buf;
         }
         if (A.ny < 10) {
-          snprintf(buf, sizeof buf, "block %d: ny = %d, a block with walls needs ny >= 10", b, A.ny);
+          snprintf(buf, sizeof buf, "block %d: %d cells between its walls, a block with walls needs >= 10", b,
+                   A.ny);
           return buf;
         }
         continue;
@@ -973,11 +1086,14 @@ int handle_remove_block(mgpu_handle *h, int i) {
   return 0;
 }
 
-int put_metrics_cart(mgpu_handle *h, int block, const double *const *src) {
+int put_metrics_cart(mgpu_handle *h, int block, const double *const *src_in) {
   if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
   if (h->c.curvilinear) return fail(h, MGPU_ESTATE, "case is curvilinear: use mgpu_set_metrics_c3 or _c2");
   BlockData &bd = h->b[block];
   const int len[3] = {bd.g.nx + 2 * G, bd.g.ny + 2 * G, bd.g.nz + 2 * G};
+  const double *sp[6];  // in the stored axis order
+  for (int q = 0; q < 6; q++) sp[3 * (q / 3) + kAxisMap[bd.perm][q % 3]] = src_in[q];
+  const double *const *src = sp;
   for (int q = 0; q < 6; q++) {
     if (!src[q]) {
       if (q >= 3 && h->c.iorder_visc == 0) continue;
@@ -1045,12 +1161,25 @@ int put_metrics_c2(mgpu_handle *h, int block, const double *const *m2, const dou
   return 0;
 }
 
-int put_state(mgpu_handle *h, int block, const double *const *src) {
+int put_state(mgpu_handle *h, int block, const double *const *src_in) {
   if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
   BlockData &bd = h->b[block];
   const size_t bytes = sizeof(double) * (size_t)bd.g.n;
   for (int v = 0; v < 5; v++)
-    if (!src[v]) return fail(h, MGPU_EINVAL, "state array %d is null", v);
+    if (!src_in[v]) return fail(h, MGPU_EINVAL, "state array %d is null", v);
+  // a relabeled block: the caller's arrays in the stored axis order
+  std::vector<double> tmp[5];
+  const double *sp[5];
+  for (int v = 0; v < 5; v++) {
+    const double *u = src_in[kVarMap[bd.perm][v]];
+    if (bd.perm) {
+      tmp[v].resize((size_t)bd.g.n);
+      relabel_array(bd.g, bd.perm, u, tmp[v].data(), true);
+      u = tmp[v].data();
+    }
+    sp[v] = u;
+  }
+  const double *const *src = sp;
   HIPCK(h, hipStreamSynchronize(h->stream));
   for (int v = 0; v < 5; v++) HIPCK(h, hipMemcpy(bd.st[h->cur][v], src[v], bytes, hipMemcpyHostToDevice));
   bd.have_state = true;
@@ -1099,7 +1228,15 @@ int get_state(mgpu_handle *h, int block, double *const *dst) {
   int rc = refresh_ghosts(h);
   if (rc) return rc;
   HIPCK(h, hipStreamSynchronize(h->stream));
-  for (int v = 0; v < 5; v++) HIPCK(h, hipMemcpy(dst[v], bd.st[h->cur][v], bytes, hipMemcpyDeviceToHost));
+  if (!bd.perm) {
+    for (int v = 0; v < 5; v++) HIPCK(h, hipMemcpy(dst[v], bd.st[h->cur][v], bytes, hipMemcpyDeviceToHost));
+    return 0;
+  }
+  std::vector<double> tmp((size_t)bd.g.n);
+  for (int v = 0; v < 5; v++) {
+    HIPCK(h, hipMemcpy(tmp.data(), bd.st[h->cur][v], bytes, hipMemcpyDeviceToHost));
+    relabel_array(bd.g, bd.perm, tmp.data(), dst[kVarMap[bd.perm][v]], false);
+  }
   return 0;
 }
 
@@ -1286,14 +1423,22 @@ int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
       return fail(nullptr, MGPU_EINVAL, "block %d: %zu padded points exceed the 2^28 limit of one block", bI,
                   padded);
   }
+  std::vector<mgpu_block> stored;
+  std::vector<int> perm;
   {
-    const std::string msg = check_topology(*c);
+    std::string msg = relabel_blocks(*c, stored, perm);
+    mgpu_case cs = *c;
+    cs.blocks = stored.data();
+    if (msg.empty()) msg = check_topology(cs);
     if (!msg.empty()) return fail(nullptr, MGPU_EINVAL, "%s", msg.c_str());
   }
   mgpu_handle *h = nullptr;
   int rc = handle_new(*c, &h);
   if (rc) return rc;
-  for (int bI = 0; bI < c->nblocks && !rc; bI++) rc = handle_add_block(h, c->blocks[bI], *c);
+  for (int bI = 0; bI < c->nblocks && !rc; bI++) {
+    rc = handle_add_block(h, stored[bI], *c);
+    if (!rc) h->b[bI].perm = perm[bI];
+  }
   if (!rc) rc = setup_exchange(h);
   if (rc) {
     g_create_error = h->err;
@@ -1332,9 +1477,10 @@ int mgpu_set_wall(mgpu_handle *h, int block, const double *cextrp2, const double
   if (!bd.g.jwall) return fail(h, MGPU_ESTATE, "block %d has no wall face", block);
   for (int s = 0; s < 2; s++) {
     if (!((bd.g.jwall >> s) & 1)) continue;
-    const double a = cextrp2[2 + s], b = cextrp3[2 + s];
+    const int fu = kFaceMap[bd.perm][2 + s];  // the caller's face stored as j-face 2 + s
+    const double a = cextrp2[fu], b = cextrp3[fu];
     if (!std::isfinite(a) || !std::isfinite(b))
-      return fail(h, MGPU_EINVAL, "block %d face %d: extrapolation coefficients not finite", block, 2 + s);
+      return fail(h, MGPU_EINVAL, "block %d face %d: extrapolation coefficients not finite", block, fu);
     bd.c2[s] = a;
     bd.c3[s] = b;
   }

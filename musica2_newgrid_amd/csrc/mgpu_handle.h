@@ -34,6 +34,11 @@ struct BlockData {
   bool have_metrics = false;
   bool have_state = false;
   bool have_wall = false;  // wall coefficients set (blocks with walls)
+  // Axis relabeling: the kernels close walls on the j-faces only, so a block
+  // with walls on its i-faces (perm 1) is stored with i and j exchanged, one
+  // with walls on its k-faces (perm 2) with j and k exchanged; the state,
+  // Cartesian metrics and wall coefficients are exchanged with it at the API.
+  int perm = 0;
 };
 
 struct ProfRec {

@@ -97,13 +97,13 @@ def _create_case(c, blocks, precision=_lib.PREC_FP64):
 
 
 def test_wall_limits_are_checked():
-    """Walls: j-faces only, Cartesian with the selective or shock filter, the
-    artificial viscosity with shock capturing or none (not artvisc_o9),
-    ny >= 10; refused before any device work."""
+    """Walls: the faces of one axis per block, Cartesian with the selective or
+    shock filter, the artificial viscosity with shock capturing or none (not
+    artvisc_o9), >= 10 cells between them; refused before any device work."""
     c, blocks = _channel_table()
     blocks[0]["kind"][0] = blocks[0]["kind"][1] = _lib.FACE_WALL
     rc, msg = _create_case(c, blocks)
-    assert rc == _lib.EINVAL and "j-faces" in msg
+    assert rc == _lib.EINVAL and "more than one axis" in msg
     c, blocks = _channel_table(filter=3)
     rc, msg = _create_case(c, blocks)
     assert rc == _lib.EINVAL and "selective or shock filter" in msg
@@ -114,7 +114,7 @@ def test_wall_limits_are_checked():
     c, blocks = _channel_table()
     blocks[0]["ny"] = 9
     rc, msg = _create_case(c, blocks)
-    assert rc == _lib.EINVAL and "ny >= 10" in msg
+    assert rc == _lib.EINVAL and ">= 10" in msg
 
 
 def test_artvisc_flags_are_checked():
