@@ -90,3 +90,49 @@ def test_fortran_host_sets_up_the_case_without_crashing(tmp_path):
     assert r.returncode >= 0, r.stderr  # no signal (abort, segfault)
     if r.returncode != 0:
         assert "mgpu_create" in r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+def test_fortran_host_artvisc_shock_matches_oracle(tmp_path):
+    """is_SF = F with is_shock = T in param.ini: the host selects
+    artvisc_o9_shock (assign_procedures.f90:184), checked against the
+    oracle."""
+    lines = open(FIXTURE).read().splitlines()
+    q = lines.index("! selective filter")
+    assert lines[q + 1] == "T"
+    lines[q + 1] = "F"
+    s = lines.index("! shock capturing, low-order coefficient")
+    lines[s + 1] = "T 0.1"
+    param = tmp_path / "param_avs.ini"
+    param.write_text("\n".join(lines) + "\n")
+    n, nsteps = 24, 5
+    prefix = str(tmp_path / "avs")
+    print(f"fortran host: TGV {n}^3 with artvisc_o9_shock, {nsteps} steps", flush=True)
+    r = subprocess.run([os.path.join(BIN, "musica_gpu_tgv"), str(param), str(n), str(nsteps), prefix],
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    shape = (n + 10, n + 10, n + 10)
+    sz = int(np.prod(shape))
+    U0 = np.fromfile(prefix + ".init.bin", dtype=np.float64)
+    U1 = np.fromfile(prefix + ".final.bin", dtype=np.float64)
+    U0 = [U0[k * sz:(k + 1) * sz].reshape(shape).copy() for k in range(5)]
+    U1 = [U1[k * sz:(k + 1) * sz].reshape(shape).copy() for k in range(5)]
+    info = dict(line.split() for line in open(prefix + ".case.txt"))
+    assert int(info["filter"]) == 4
+    c = cases.tgv(n)
+    c.U = U0
+    c.dt = float(info["dt"])
+    c.diffscale = float(info["diffscale"])
+    c.filter = 4
+    c.iorder_visc = int(info["iorder_visc"])
+    c.chi = float(info["chi"])
+    c.dissip_shock = float(info["dissip_shock"])
+    c.csens = float(info["csens"])
+    c.ducros = bool(int(info["ducros"]))
+    from oracle.oracle import Oracle
+    Uc = c.copy_state()
+    Oracle(c).step(Uc, nsteps)
+    e = relerr(U1, Uc)
+    print("fortran-host GPU (artvisc_o9_shock) vs oracle relerr", e)
+    assert e < 1e-12
