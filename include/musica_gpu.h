@@ -61,10 +61,10 @@ extern "C" {
  * closures of the Euler, viscous and filter operators.  On the faces of one
  * axis per block (i, j or k; a block with walls on its i- or k-faces is stored
  * with that axis exchanged with j, invisibly to the caller) of Cartesian cases
- * (FP64 or mixed) with the selective or shock filter, the artificial
- * viscosity with shock capturing or none, at least 10 cells between the
- * walls; the block's extrapolation coefficients come from mgpu_set_wall.
- * With a shock sensor the ghost row beyond a wall must hold a valid state:
+ * (FP64 or mixed, any filter), at least 10 cells between the walls; the
+ * block's extrapolation coefficients come from mgpu_set_wall.
+ * With a shock sensor (filters 2, 4) the ghost row beyond a wall must hold a
+ * valid state:
  * the wall row's pressure sensor reads p there (mod_filtering_shock.f90:
  * 243-260 with ndy_d1 = 1), and nothing on the path writes it. */
 #define MGPU_FACE_WALL 2

@@ -683,7 +683,8 @@ __global__ __launch_bounds__(kShockWallThreads) void k_shock_wall(FilterArgs a) 
   for (int f = 0; f < 2; f++) {
     const int m = j - 1 + f;
     const int hf = face_half(g, m);
-    double xk0 = a.xnu0 * fmax(ps[f + 1], ps[f]);
+    // no sensor term (artvisc_o9): p beyond the wall is not read into it
+    double xk0 = a.xnu0 == 0.0 ? 0.0 : a.xnu0 * fmax(ps[f + 1], ps[f]);
     if (hf == 1) xk0 = xk0 / 10.0;
     double xkh;
     switch (hf) {

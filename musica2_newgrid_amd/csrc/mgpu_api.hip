@@ -100,9 +100,13 @@ Gas gas_of(const mgpu_case &c) {
 
 // The shock filter's Edoh switch is on (is_sw_edoh acts in filter_o10_shock only).
 bool edoh_on(const mgpu_case &c) { return c.filter == MGPU_FILTER_SHOCK && c.is_sw_edoh != 0; }
-// The dissipation reads the pressure sensor (p and the Ducros sensor of U^n).
+// The dissipation kernels read p and the Ducros sensor of U^n (artvisc_o9:
+// only through the kernels of artvisc_o9_shock it shares on blocks with
+// walls, with no sensor term; kept for every artvisc_o9 case, so that every
+// block of a case holds the same arrays).
 bool has_sensor(const mgpu_case &c) {
-  return c.filter == MGPU_FILTER_SHOCK || c.filter == MGPU_FILTER_ARTVISC_SHOCK;
+  return c.filter == MGPU_FILTER_SHOCK || c.filter == MGPU_FILTER_ARTVISC_SHOCK ||
+         c.filter == MGPU_FILTER_ARTVISC;
 }
 // The artificial viscosities: spectral radii of U^n at the faces' two sides.
 bool is_artvisc(const mgpu_case &c) {
@@ -110,8 +114,11 @@ bool is_artvisc(const mgpu_case &c) {
 }
 // The spectral radii of U^n are stored (artificial viscosities, Edoh switch).
 bool has_sr(const mgpu_case &c) { return is_artvisc(c) || edoh_on(c); }
-DissipKind dissip_kind(const mgpu_case &c) {
-  if (c.filter == MGPU_FILTER_ARTVISC) return kDissipArtvisc;
+// artvisc_o9 next to walls is artvisc_o9_shock without its sensor term: the
+// faces 1..4 from a wall with d2..d8 times xnu2..xnu8 (mod_artvisc.f90:98-586)
+// are its reduced faces with xk0 = 0.
+DissipKind dissip_kind(const mgpu_case &c, bool wall) {
+  if (c.filter == MGPU_FILTER_ARTVISC) return wall ? kDissipArtviscShock : kDissipArtvisc;
   if (c.filter == MGPU_FILTER_ARTVISC_SHOCK) return kDissipArtviscShock;
   return kDissipShock;
 }
@@ -293,11 +300,8 @@ std::string check_topology(const mgpu_case &c) {
           snprintf(buf, sizeof buf, "block %d face %d: walls are supported on the j-faces (2, 3) only", b, f);
           return buf;
         }
-        if (c.curvilinear || c.filter == MGPU_FILTER_ARTVISC) {
-          snprintf(buf, sizeof buf,
-                   "block %d face %d: walls need a Cartesian case with the selective or shock filter, the "
-                   "artificial viscosity with shock capturing or none",
-                   b, f);
+        if (c.curvilinear) {
+          snprintf(buf, sizeof buf, "block %d face %d: walls need a Cartesian case", b, f);
           return buf;
         }
         if (A.ny < 10) {
@@ -684,7 +688,7 @@ FilterArgs filter_args(const mgpu_handle *h, const BlockData &bd, int in, int ou
   fa.chi = bd.p.dissip_coeff;
   fa.prs = bd.prs;
   fa.sens = h->c.is_ducros ? bd.sens : nullptr;
-  fa.xnu0 = -bd.p.dissip_shock;
+  fa.xnu0 = h->c.filter == MGPU_FILTER_ARTVISC ? 0.0 : -bd.p.dissip_shock;
   fa.xnu10 = bd.p.dissip_coeff / 1260.0;
   fa.csens = bd.p.csens;
   fa.ref = bd.ref;
@@ -697,7 +701,7 @@ FilterArgs filter_args(const mgpu_handle *h, const BlockData &bd, int in, int ou
   fa.idz = bd.idx[2];
   fa.dt = bd.p.deltat;
   fa.edoh = edoh_on(h->c);
-  fa.avs = h->c.filter == MGPU_FILTER_ARTVISC_SHOCK;
+  fa.avs = h->c.filter == MGPU_FILTER_ARTVISC_SHOCK || (h->c.filter == MGPU_FILTER_ARTVISC && h->wall);
   return fa;
 }
 
@@ -902,7 +906,7 @@ int enqueue_step(mgpu_handle *h, int s0) {
       if (c.filter == MGPU_FILTER_SF)
         HIPCK(h, launch_filter_sf(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, h->wall, h->stream));
       else
-        HIPCK(h, launch_filter_shock(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, dissip_kind(c), h->wall,
+        HIPCK(h, launch_filter_shock(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, dissip_kind(c, h->wall), h->wall,
                                      edoh_on(c), h->stream));
       if (h->wall && has_sensor(c))
         for (const auto &bd : h->b)
@@ -1373,7 +1377,7 @@ int tune_aux(mgpu_handle *h, bool force, hipEvent_t e0, hipEvent_t e1) {
         if (kind == 1) return launch_vdiv(h->d_vdiv, nb, h->nt_vdiv, geo, h->stream);
         if (h->c.filter == MGPU_FILTER_SF)
           return launch_filter_sf(h->d_tune_f, nb, h->nt_filt, h->mixed, h->wall, h->stream);
-        return launch_filter_shock(h->d_tune_f, nb, h->nt_filt, h->mixed, dissip_kind(h->c), h->wall,
+        return launch_filter_shock(h->d_tune_f, nb, h->nt_filt, h->mixed, dissip_kind(h->c, h->wall), h->wall,
                                    edoh_on(h->c), h->stream);
       };
       hipError_t e = launch();

@@ -758,13 +758,22 @@ static void filter_shock(const orc_cfg *c, const orc_grid *g, const dims_t *d, d
 /* ------------------------------------------------------------------------ */
 static void artvisc(const orc_cfg *c, const orc_grid *g, const dims_t *d, double *const Un[5], const work_t *w,
                     double *K[5]) {
-  const double x10 = c->chi / 1260.0;
-  const double d10[6] = {0.0, 126.0 * x10, -84.0 * x10, 36.0 * x10, -9.0 * x10, 1.0 * x10};
+  /* coefficients times their amplitude (init_artvisc :30-77): d2, d4, d6, d8
+   * for the faces 1..4 from a wall (xnu2 = 0), d10 inside */
+  const double xnu[6] = {0.0, 0.0, c->chi / 12.0, c->chi / 60.0, c->chi / 280.0, c->chi / 1260.0};
+  static const double DP[6][6] = {{0}, {0, 1.0}, {0, 3.0, -1.0}, {0, 10.0, -5.0, 1.0},
+                                  {0, 35.0, -21.0, 7.0, -1.0}, {0, 126.0, -84.0, 36.0, -9.0, 1.0}};
+  double dk[6][6];
+  for (int h = 0; h < 6; h++)
+    for (int l = 0; l < 6; l++) dk[h][l] = DP[h][l] * xnu[h];
+  const double *d10 = dk[5];
+  const walls_t W = walls_of(c);
   const long st[3] = {1, d->sx, d->sx * d->sy};
   const int nn[3] = {d->nx, d->ny, d->nz};
   for (int dir = 0; dir < 3; dir++) {
     const long s = st[dir];
     const int n = nn[dir];
+    const int lo = W.lo[dir], hi = W.hi[dir];
     const int o1 = dir == 0 ? 1 : 0, o2 = dir == 2 ? 1 : 2;
 #pragma omp parallel
     {
@@ -775,17 +784,33 @@ static void artvisc(const orc_cfg *c, const orc_grid *g, const dims_t *d, double
           int idx3[3];
           idx3[o1] = a; idx3[o2] = b; idx3[dir] = 0;
           const long p0 = (long)ID(*d, idx3[0], idx3[1], idx3[2]);
-          for (int i = 0; i <= n; i++) {
+          /* faces 0..n (1..n-1 along a wall axis: mod_artvisc.f90:98-376 i, :380-586 j) */
+          for (int i = lo ? 1 : 0; i <= (hi ? n - 1 : n); i++) {
             const long p = p0 + i * s;
+            int hf = 5;
+            if (lo && i < hf) hf = i;
+            if (hi && n - i < hf) hf = n - i;
             for (int v = 0; v < 5; v++) {
               const double *u = Un[v];
-              D[v * (n + 1) + i] = d10[1] * (u[p + s] - u[p]) + d10[2] * (u[p + 2 * s] - u[p - s]) +
-                                   d10[3] * (u[p + 3 * s] - u[p - 2 * s]) + d10[4] * (u[p + 4 * s] - u[p - 3 * s]) +
-                                   d10[5] * (u[p + 5 * s] - u[p - 4 * s]);
+              if (hf == 5) {
+                D[v * (n + 1) + i] = d10[1] * (u[p + s] - u[p]) + d10[2] * (u[p + 2 * s] - u[p - s]) +
+                                     d10[3] * (u[p + 3 * s] - u[p - 2 * s]) + d10[4] * (u[p + 4 * s] - u[p - 3 * s]) +
+                                     d10[5] * (u[p + 5 * s] - u[p - 4 * s]);
+              } else {
+                double dd = 0.0;
+                for (int l = 1; l <= hf; l++) dd = dd + dk[hf][l] * (u[p + l * s] - u[p + (1 - l) * s]);
+                D[v * (n + 1) + i] = dd;
+              }
             }
           }
+          /* K on 1..n; the wall rows get nothing along the wall axis (ndy_d..nfy_d) */
           for (int i = 1; i <= n; i++) {
             const long p = p0 + i * s;
+            if ((lo && i == 1) || (hi && i == n)) {
+              if (dir == 0)
+                for (int v = 0; v < 5; v++) K[v][p] = 0.0;
+              continue;
+            }
             double m, m1, m2;
             const double sr = spec_radius(c, g, d, w, dir, p, i, &m);
             const double srp = 0.5 * (spec_radius(c, g, d, w, dir, p + s, i + 1, &m1) + sr);
@@ -1033,7 +1058,6 @@ int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_gri
       if (bl[b].wall[f]) {
         const int ext = f < 2 ? bl[b].nx : (f < 4 ? bl[b].ny : bl[b].nz);
         if (c0->curv) return -5;        /* walls: Cartesian blocks only */
-        if (c0->filter == 3) return -6; /* walls: not with artvisc_o9 */
         if (ext < 10) return -7;        /* closures of the two sides must not overlap */
       }
   orc_cfg cb[64];

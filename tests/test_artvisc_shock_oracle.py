@@ -157,3 +157,33 @@ def test_filter_codes_are_checked():
     c = cases.vortex_wavy(16, 16, 16, filter=3)
     with pytest.raises(RuntimeError):
         Oracle(c).step(c.copy_state(), 1)
+
+
+def test_artvisc_at_walls_scales_the_shock_filter_without_sensor():
+    """artvisc_o9 next to no-slip walls (faces 1..4 from a wall with d2..d8
+    times xnu2..xnu8, nothing from the wall axis on the wall rows,
+    mod_artvisc.f90:98-586): with one spectral radius c on a uniform grid its
+    step increment is deltat c/h times that of the shock filter without its
+    sensor term, whose wall faces are pinned on their own; and it equals
+    artvisc_o9_shock without the sensor term."""
+    c, cs, h = _sawtooth_wall_box()
+    c.dissip_shock = 0.0
+    out = {}
+    for f in (0, 2, 3, 4):
+        c.filter = f
+        U = c.copy_state()
+        Oracle(c).step(U, 1)
+        out[f] = U
+    rows = slice(1, c.ny - 1)
+    for v in (0, 1, 2, 4):
+        base = gr.interior(out[0][v])[:, rows, :]
+        d2 = gr.interior(out[2][v])[:, rows, :] - base
+        d3 = gr.interior(out[3][v])[:, rows, :] - base
+        d4 = gr.interior(out[4][v])[:, rows, :] - base
+        ref = c.dt * cs / h * d2
+        scale = max(float(np.max(np.abs(ref))), 1e-300)
+        if v != 2:
+            assert float(np.max(np.abs(ref))) > 0.0
+        tol = 1e-9 * scale + 1e-15 * float(np.max(np.abs(base)))
+        assert np.max(np.abs(d3 - ref)) <= tol
+        assert np.max(np.abs(d3 - d4)) <= tol

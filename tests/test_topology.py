@@ -97,16 +97,12 @@ def _create_case(c, blocks, precision=_lib.PREC_FP64):
 
 
 def test_wall_limits_are_checked():
-    """Walls: the faces of one axis per block, Cartesian with the selective or
-    shock filter, the artificial viscosity with shock capturing or none (not
-    artvisc_o9), >= 10 cells between them; refused before any device work."""
+    """Walls: the faces of one axis per block, Cartesian, >= 10 cells between
+    them; refused before any device work."""
     c, blocks = _channel_table()
     blocks[0]["kind"][0] = blocks[0]["kind"][1] = _lib.FACE_WALL
     rc, msg = _create_case(c, blocks)
     assert rc == _lib.EINVAL and "more than one axis" in msg
-    c, blocks = _channel_table(filter=3)
-    rc, msg = _create_case(c, blocks)
-    assert rc == _lib.EINVAL and "selective or shock filter" in msg
     c, blocks = _channel_table()
     c.curvilinear = True
     rc, msg = _create_case(c, blocks)
