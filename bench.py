@@ -259,6 +259,9 @@ def main() -> int:
     ap.add_argument("--no-tgv", action="store_true", help="skip the TGV 64^3 leg")
     ap.add_argument("--no-channel", action="store_true", help="skip the walled-channel leg")
     ap.add_argument("--no-c2", action="store_true", help="skip the 2.5-D curvilinear leg")
+    ap.add_argument("--dissipation", choices=["shock", "artvisc_shock"], default="shock",
+                    help="dissipation of the main leg: the shock filter (filter_o10_shock) or the artificial "
+                         "viscosity with shock capturing (artvisc_o9_shock)")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-steps", type=int, default=100)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -274,7 +277,9 @@ def main() -> int:
 
     from musica2_newgrid_amd import _lib, cases
 
-    case = cases.vortex_wavy(args.nx, args.ny, args.nz, filter=2)
+    diss = {"shock": (2, "shock filter + Ducros"),
+            "artvisc_shock": (4, "artificial viscosity with shock capturing + Ducros")}[args.dissipation]
+    case = cases.vortex_wavy(args.nx, args.ny, args.nz, filter=diss[0])
     print(f"bench: {args.nx}x{args.ny}x{args.nz} FP64 leg, {args.steps} steps", file=sys.stderr, flush=True)
     fp = run_leg(case, _lib.PREC_FP64, args.steps, args.warmup, not args.no_tune)
     t = fp["t"]
@@ -298,7 +303,7 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"curvilinear block {args.nx}x{args.ny}x{args.nz}, isentropic vortex "
-                               "M=0.5, viscous, shock filter + Ducros, RK4, FP64",
+                               f"M=0.5, viscous, {diss[1]}, RK4, FP64",
                    "cells": case.ncells, "nrk": case.nrk, "precision": "fp64",
                    "stage_tiles": {k: {"tile": f"{v[0]}x{v[1]}", "kchunk": v[2]} for k, v in fp["tuned"].items()}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
