@@ -250,6 +250,16 @@ std::string relabel_blocks(const mgpu_case &c, std::vector<mgpu_block> &out, std
       const int n = U.nbr_block[fu], g = U.nbr_face[fu];
       if (n < 0 || n >= nb || g < 0 || g > 5) continue;  // reported by check_topology
       S.nbr_face[fs] = kFaceMap[perm[n]][g];
+      // an interface whose two sides' stored tangential axes do not come out
+      // reversed alike needs a re-encoded (swapping) orientation code, which
+      // has given results different from the oracle's: refused
+      if (tangential_flipped(perm[b], fu) != tangential_flipped(perm[n], g)) {
+        snprintf(buf, sizeof buf,
+                 "block %d face %d: an interface between a block walled on its %s-faces and this neighbour "
+                 "face is not supported",
+                 perm[b] ? b : n, perm[b] ? fu : g, (perm[b] ? perm[b] : perm[n]) == 1 ? "i" : "k");
+        return buf;
+      }
       S.orient[fs] = relabel_orient(U.orient[fu], tangential_flipped(perm[b], fu), tangential_flipped(perm[n], g));
     }
   }

@@ -123,3 +123,4 @@ def test_z_walled_channel_in_two_blocks():
     e = max(relerr(a, b) for a, b in zip(Ug, Uc))
     print(f"relerr {e:.3e}", flush=True)
     assert e < 1e-12
+
