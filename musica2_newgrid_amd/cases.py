@@ -12,6 +12,8 @@ step and the initial conserved state.  Builders:
 * :func:`entropy_wave`   -- small-amplitude entropy/shear waves for closed-form tests
 * :func:`channel`        -- Cartesian channel: periodic x and z, no-slip walls at
                             y = 0 and y = Ly on a stretched grid
+* :func:`shock_tube`     -- periodic shock tube with a wavy front (the shock
+                            sensor across a real discontinuity)
 
 The time step follows timestep_new (src/Time/timestep_new.f90:270-288):
 deltat = CFL * delta_min / max(|u| + c).
@@ -320,6 +322,38 @@ def cartesian_box(nx, ny, nz, lengths, fields, iorder_visc=0, filter=0, diffscal
     U = _to_padded(conserved(*fields), nx, ny, nz)
     return Case("box", nx, ny, nz, curvilinear=False, iorder_visc=iorder_visc, filter=filter,
                 ducros=False, diffscale=diffscale, cart=cart, U=U, dt=dt, chi=chi)
+
+
+def shock_tube(nx: int = 96, ny: int = 32, nz: int = 8, ducros: bool = True, dissip_shock: float = 0.3,
+               csens: float = 0.3, cfl: float = 0.5, filter: int = 2, ratio: float = 2.5,
+               noise: float = 5.0, seed: int = 20261203) -> Case:
+    """Periodic shock tube with a wavy front: a slab of gas at (2 rho0, ratio p0)
+    between fronts x = Lx/2 + 0.1 Lx sin(2 pi y/Ly) and that plus Lx/2, gas at
+    (rho0, p0) elsewhere, seeded velocity noise of amplitude `noise` m/s.  The
+    jumps cross the i and j directions, so the pressure sensor of both is O(1)
+    on the first steps (0.42 at csens = 0.3 for ratio 2.5), and the noise gives
+    the Ducros sensor a non-trivial divergence/vorticity blend.  Viscous (the
+    sensor reads the gradients of U^n), shock filter."""
+    Lx, Ly, Lz = 2.0, 2.0 * ny / nx, 2.0 * nz / nx
+    x = (np.arange(nx) + 0.5) * Lx / nx
+    y = (np.arange(ny) + 0.5) * Ly / ny
+    z = (np.arange(nz) + 0.5) * Lz / nz
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    front = 0.5 * Lx + 0.1 * Lx * np.sin(2.0 * math.pi * Y / Ly)
+    inside = np.mod(X - front, Lx) > 0.5 * Lx  # x in (front - Lx/2, front), periodically
+    rho0, p0 = 1.177, 1.0e5
+    rho = np.where(inside, 2.0 * rho0, rho0)
+    p = np.where(inside, ratio * p0, p0)
+    rng = np.random.default_rng(seed)
+    u, v, w = (noise * rng.uniform(-1.0, 1.0, X.shape) for _ in range(3))
+    c = cartesian_box(nx, ny, nz, (Lx, Ly, Lz), (rho, u, v, w, p), iorder_visc=4, filter=filter, chi=0.2)
+    c.name = "shock_tube"
+    c.ducros = ducros
+    c.dissip_shock = dissip_shock
+    c.csens = csens
+    c.dt = timestep(c.U, Lx / nx, cfl)
+    c.info = dict(ratio=ratio, noise=noise, seed=seed, cfl=cfl, lengths=(Lx, Ly, Lz))
+    return c
 
 
 def stretched_wall_coords(n: int, L: float, beta: float) -> np.ndarray:

@@ -37,6 +37,16 @@
 
 #define NG ORC_NG
 
+/* Planted mistakes, for the test that shows the pins fail on them
+ * (tests/test_planted_mistakes.py builds copies with -DORC_PLANT=n into
+ * _build/plant_n.so).  0 -- the default -- in every other build:
+ *   1  pressure sensor: 4 p_i instead of 2 p_i in the cs-weighted denominator
+ *   2  interior face amplitude: xk0/10.5 instead of xk0/105
+ *   3  Ducros sensor: omega_z missing from the denominator */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+
 typedef struct {
   int nx, ny, nz;
   long sx, sy, sz;
@@ -636,7 +646,7 @@ static void filter_shock(const orc_cfg *c, const orc_grid *g, const dims_t *d, d
           double div2 = dv * dv;
           double o1 = gr[DWY][p] - gr[DVZ][p];
           double o2 = gr[DUZ][p] - gr[DWX][p];
-          double o3 = gr[DVX][p] - gr[DUY][p];
+          double o3 = ORC_PLANT == 3 ? 0.0 : gr[DVX][p] - gr[DUY][p];
           sens[p] = div2 / (div2 + o1 * o1 + o2 * o2 + o3 * o3 + 1.0e-16);
         } else {
           sens[p] = 1.0;
@@ -665,7 +675,7 @@ static void filter_shock(const orc_cfg *c, const orc_grid *g, const dims_t *d, d
            * the wall, the pressure of the ghost state, which nothing changes) */
           for (int i = lo ? 1 : 0; i <= (hi ? n : n + 1); i++) {
             long p = p0 + i * s;
-            double ptrad = prs[p - s] + 2.0 * prs[p] + prs[p + s];
+            double ptrad = prs[p - s] + (ORC_PLANT == 1 ? 4.0 : 2.0) * prs[p] + prs[p + s];
             double ptvd = fabs(prs[p + s] - prs[p]) + fabs(prs[p] - prs[p - s]);
             ps[i] = sens[p] * fabs(prs[p - s] - 2.0 * prs[p] + prs[p + s]) / (cs * ptrad + (1.0 - cs) * ptvd);
           }
@@ -679,7 +689,7 @@ static void filter_shock(const orc_cfg *c, const orc_grid *g, const dims_t *d, d
             if (hf == 1) xk0 = xk0 / 10.0;
             /* artvisc_o9_shock scales the sensor term of faces 3 and 4 from a wall
              * by 1/5 and 3/70 (mod_artvisc_shock.f90:577-695, :744-862) */
-            double xks = hf == 5 ? xk0 / 105.0 : xk0;
+            double xks = hf == 5 ? xk0 / (ORC_PLANT == 2 ? 10.5 : 105.0) : xk0;
             if (c->filter == 4 && hf == 3) xks = xk0 / 5.0;
             if (c->filter == 4 && hf == 4) xks = 3.0 / 70.0 * xk0;
             const double xkh = fmax(0.0, xnu[hf] + xks);

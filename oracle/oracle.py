@@ -13,7 +13,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+# MUSICA_ORACLE_LIB: another build of the same sources (the planted-mistake test only)
+LIB_PATH = os.environ.get("MUSICA_ORACLE_LIB") or os.path.join(_HERE, "_build", "liboracle.so")
 
 _dp = C.POINTER(C.c_double)
 
