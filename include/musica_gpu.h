@@ -239,6 +239,22 @@ int mgpu_diagnostics(mgpu_handle *h, mgpu_diag *out);
 /* Release every device resource of the handle. */
 void mgpu_destroy(mgpu_handle *h);
 
+/* Ghost-fill plan of the interface exchange of a block table, computed on the
+ * host (no device, no allocation on it): what mgpu_step's halo copies will
+ * write, in the caller's frame.  Entry e of map (6 int64 each, at most cap
+ * entries written, *count = the total) reads: destination block, element of
+ * the destination's padded array (i fastest), destination array, source
+ * block, source element, source array.  set 0: the 5 conserved variables
+ * rho..rhoe over 5 ghost layers (communication3d, src/Parallel/
+ * mod_comm.f90:249-290); set 1: the viscous set over 2 layers
+ * (communication_3dv :376-402): 0..5 the stress components xx,xy,xz,yy,yz,zz,
+ * 6..8 the energy fluxes along x,y,z, 9 p, 10 the Ducros sensor, 11..13 the
+ * spectral radii along i,j,k.  Blocks with walls on their i- or k-faces are
+ * stored with that axis exchanged with j; the plan undoes that, so it can be
+ * checked against the reference's face types (src/Parallel/
+ * mod_mpi_types_two_sided.f90:47-197) on the caller's arrays. */
+int mgpu_exchange_plan(const mgpu_case *c, int set, int64_t *map, int64_t cap, int64_t *count);
+
 /* ------------------------------------------------------------------------
  * Serving loop.  No counterpart in the reference, which runs one case per
  * MPI job (src/Main/musica_main.f90): this is for callers that advance many

@@ -42,6 +42,7 @@ EXPORTS = (
     "mgpu_last_error",
     "mgpu_diagnostics",
     "mgpu_destroy",
+    "mgpu_exchange_plan",
     "mgpu_server_create",
     "mgpu_server_submit",
     "mgpu_server_tick",
@@ -131,6 +132,9 @@ def lib() -> C.CDLL:
     L.mgpu_destroy.restype = None
     L.mgpu_diagnostics.argtypes = [C.c_void_p, C.POINTER(MgpuDiag)]
     L.mgpu_diagnostics.restype = C.c_int
+    L.mgpu_exchange_plan.argtypes = [C.POINTER(MgpuCase), C.c_int, C.POINTER(C.c_int64), C.c_int64,
+                                     C.POINTER(C.c_int64)]
+    L.mgpu_exchange_plan.restype = C.c_int
     L.mgpu_server_create.argtypes = [C.c_int64, C.POINTER(C.c_void_p)]
     L.mgpu_server_submit.argtypes = [C.c_void_p, C.POINTER(MgpuCase), C.POINTER(_dp), C.POINTER(_dp), C.c_int,
                                      C.POINTER(C.c_int64)]

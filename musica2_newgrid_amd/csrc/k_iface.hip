@@ -12,7 +12,9 @@
 // element stride per box axis on the destination (ghost layers) and on the
 // source (the neighbour's interior layers), box axis 0 being the
 // destination's contiguous one.  Workgroups map to (task, first point)
-// through a chunk table.  Only face ghosts are written (tangential extent of
+// through a chunk table; array v of the destination takes array vmap[v] of
+// the source (the same quantity in the source block's stored axis order).
+// Only face ghosts are written (tangential extent of
 // the face), as the reference's face types do; edges and corners are never
 // read by the kernels.  Sources are interior points only, so a launch never
 // reads what it writes.
@@ -37,7 +39,7 @@ __global__ __launch_bounds__(kIfaceThreads) void k_iface(IfaceArgsT<T> a) {
   const idx_t ps = (idx_t)(t.src_base + a0 * t.src_s[0] + a1 * t.src_s[1] + a2 * t.src_s[2]);
   T *const *dst = a.ptrs + (size_t)t.dst * a.nvar;
   T *const *src = a.ptrs + (size_t)t.src * a.nvar;
-  for (int v = 0; v < a.nvar; v++) stg(dst[v], pd, ldg(src[v], ps));
+  for (int v = 0; v < a.nvar; v++) stg(dst[v], pd, ldg(src[t.vmap[v]], ps));
 }
 
 }  // namespace

@@ -250,17 +250,33 @@ std::string relabel_blocks(const mgpu_case &c, std::vector<mgpu_block> &out, std
       const int n = U.nbr_block[fu], g = U.nbr_face[fu];
       if (n < 0 || n >= nb || g < 0 || g > 5) continue;  // reported by check_topology
       S.nbr_face[fs] = kFaceMap[perm[n]][g];
-      // an interface whose two sides' stored tangential axes do not come out
-      // reversed alike needs a re-encoded (swapping) orientation code, which
-      // has given results different from the oracle's: refused
-      if (tangential_flipped(perm[b], fu) != tangential_flipped(perm[n], g)) {
+      S.orient[fs] = relabel_orient(U.orient[fu], tangential_flipped(perm[b], fu), tangential_flipped(perm[n], g));
+    }
+  }
+  return "";
+}
+
+// On a Cartesian case each block's index axes are the physical axes (the
+// fluxes tie rho u to i through idx, etc.), so an interface can only join
+// faces of the same normal axis without exchanging the tangential axes;
+// reversed axes are fine (the metrics of a reversed block are negative).
+// Checked on the caller's block table.  "" or an error.
+std::string check_cartesian_frames(const mgpu_case &c) {
+  char buf[256];
+  if (c.curvilinear) return "";
+  for (int b = 0; b < c.nblocks; b++) {
+    const mgpu_block &A = c.blocks[b];
+    for (int f = 0; f < 6; f++) {
+      if (A.face_kind[f] != MGPU_FACE_INTERFACE) continue;
+      const int nf = A.nbr_face[f], o = A.orient[f];
+      if (nf < 0 || nf > 5) continue;  // reported by check_topology
+      if (nf / 2 != f / 2 || (o & 1)) {
         snprintf(buf, sizeof buf,
-                 "block %d face %d: an interface between a block walled on its %s-faces and this neighbour "
-                 "face is not supported",
-                 perm[b] ? b : n, perm[b] ? fu : g, (perm[b] ? perm[b] : perm[n]) == 1 ? "i" : "k");
+                 "block %d face %d: on a Cartesian case an interface must join faces of the same axis without "
+                 "exchanging the tangential axes (neighbour face %d, orient %d)",
+                 b, f, nf, o);
         return buf;
       }
-      S.orient[fs] = relabel_orient(U.orient[fu], tangential_flipped(perm[b], fu), tangential_flipped(perm[n], g));
     }
   }
   return "";
@@ -382,11 +398,54 @@ bool task_in_bounds(const FaceTask &t, long ndst, long nsrc) {
   return true;
 }
 
+// Arrays an exchange moves: the 5 conserved variables, or the viscous set
+// (the NTAU components of tau, then p and the Ducros sensor when `sensor`,
+// then the 3 spectral radii when `radii`).
+struct XSet {
+  bool visc = false, sensor = false, radii = false;
+  int nvar() const { return visc ? NTAU + (sensor ? 2 : 0) + (radii ? 3 : 0) : 5; }
+};
+
+// Stored axis of block n holding the caller's axis that block b stores as s.
+int axis_between(int pb, int pn, int s) { return kAxisMap[pn][kAxisMap[pb][s]]; }
+
+// Index of the symmetric tensor component (s, t) in tau[0..5] (xx, xy, xz, yy, yz, zz).
+int tau_index(int s, int t) {
+  static const int I[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+  return I[s][t];
+}
+const int kTauPair[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+
+// Source array of each destination array of a face from block b (perm pb,
+// face axes a,t1,t2) to block n (perm pn; the axes bb, m1, m2 of its face
+// that meet a, t1, t2): the conserved variables and tau follow the caller's
+// frame through both blocks' axis relabelings; the spectral radius along an
+// index axis is that of the neighbour's index axis it meets (on c3 and 2.5-D
+// blocks a swapping interface exchanges them; the radius has no sign).
+void face_vmap(const XSet &xs, int pb, int pn, const int dst_ax[3], const int src_ax[3], int8_t vmap[kIfaceMaxVar]) {
+  for (int v = 0; v < kIfaceMaxVar; v++) vmap[v] = (int8_t)v;
+  if (!xs.visc) {
+    for (int v = 0; v < 5; v++) vmap[v] = (int8_t)kVarMap[pn][kVarMap[pb][v]];
+    return;
+  }
+  for (int q = 0; q < 6; q++)
+    vmap[q] = (int8_t)tau_index(axis_between(pb, pn, kTauPair[q][0]), axis_between(pb, pn, kTauPair[q][1]));
+  for (int d = 0; d < 3; d++) vmap[6 + d] = (int8_t)(6 + axis_between(pb, pn, d));
+  if (xs.radii) {
+    const int off = NTAU + (xs.sensor ? 2 : 0);
+    int meet[3];
+    for (int x = 0; x < 3; x++) meet[dst_ax[x]] = src_ax[x];
+    for (int d = 0; d < 3; d++) vmap[off + d] = (int8_t)(off + meet[d]);
+  }
+}
+
 // Face tasks of every face of every block for `ng` ghost layers, and the
 // chunk table that maps workgroups onto them.  False if a task would reach
-// outside its arrays (a block table that passed check_topology never does).
-bool build_tasks(const mgpu_case &c, const std::vector<BlockData> &bds, int ng, std::vector<FaceTask> &tasks,
-                 std::vector<IfaceChunk> &chunks) {
+// outside its arrays (a block table that passed check_topology never does)
+// or if, on a Cartesian case, a face's orientation does not meet the
+// neighbour's axes as the two blocks' axis relabelings say it must.
+bool build_tasks(const mgpu_case &c, const std::vector<BlockData> &bds, int ng, const XSet &xs,
+                 std::vector<FaceTask> &tasks, std::vector<IfaceChunk> &chunks) {
   tasks.clear();
   chunks.clear();
   for (int b = 0; b < c.nblocks; b++) {
@@ -435,6 +494,12 @@ bool build_tasks(const mgpu_case &c, const std::vector<BlockData> &bds, int ng, 
       t.src = nb;
       t.dst_base = (int)db;
       t.src_base = (int)sb;
+      const int dst_ax[3] = {a, t1, t2};
+      const int src_ax[3] = {bb, (o & 1) ? u2 : u1, (o & 1) ? u1 : u2};
+      if (!c.curvilinear)
+        for (int x = 0; x < 3; x++)
+          if (src_ax[x] != axis_between(bds[b].perm, bds[nb].perm, dst_ax[x])) return false;
+      face_vmap(xs, bds[b].perm, bds[nb].perm, dst_ax, src_ax, t.vmap);
       for (int x = 0; x < 3; x++) {
         t.n[x] = ext[ord[x]];
         t.dst_s[x] = (int)dv[ord[x]];
@@ -489,8 +554,10 @@ int setup_exchange(mgpu_handle *h) {
   const int nb = (int)h->b.size();
   int rc;
   // state: 5 layers, 5 variables, one pointer table per state set
-  if (!build_tasks(h->c, h->b, G, tasks, chunks))
-    return fail(h, MGPU_EINVAL, "interface exchange would reach outside a block's arrays");
+  if (!build_tasks(h->c, h->b, G, XSet{}, tasks, chunks))
+    return fail(h, MGPU_EINVAL,
+                "interface exchange would reach outside a block's arrays, or a face meets its neighbour's axes "
+                "otherwise than the blocks' axis relabelings require");
   FaceTask *dt = nullptr;
   IfaceChunk *dc = nullptr;
   if ((rc = upload(h, tasks, &dt)) || (rc = upload(h, chunks, &dc))) return rc;
@@ -515,12 +582,16 @@ int setup_exchange(mgpu_handle *h) {
   }
   // viscous: 2 layers of tau (+ p and the sensor for the shock filter)
   if (h->c.iorder_visc) {
-    if (!build_tasks(h->c, h->b, 2, tasks, chunks))
+    const bool shock = has_sensor(h->c), artv = is_artvisc(h->c);
+    XSet xv;
+    xv.visc = true;
+    xv.sensor = shock;
+    xv.radii = artv;
+    const int nv = xv.nvar();
+    if (!build_tasks(h->c, h->b, 2, xv, tasks, chunks))
       return fail(h, MGPU_EINVAL, "viscous interface exchange would reach outside a block's arrays");
     if ((rc = upload(h, tasks, &dt)) || (rc = upload(h, chunks, &dc))) return rc;
     h->nchunks_v = (int)chunks.size();
-    const bool shock = has_sensor(h->c), artv = is_artvisc(h->c);
-    const int nv = NTAU + (shock ? 2 : 0) + (artv ? 3 : 0);
     std::vector<double *> ptrs;
     for (int b = 0; b < nb; b++) {
       for (int q = 0; q < NTAU; q++) ptrs.push_back(h->b[b].tau[q]);
@@ -1213,21 +1284,27 @@ int put_state(mgpu_handle *h, int block, const double *const *src_in) {
         }
       mean[v] = acc / ((double)g.nx * g.ny * g.nz);
     }
-    RefState r{};
-    r.r = mean[0];
-    r.ru = mean[1];
-    r.rv = mean[2];
-    r.rw = mean[3];
-    r.re = mean[4];
-    r.u = r.ru / r.r;
-    r.v = r.rv / r.r;
-    r.w = r.rw / r.r;
-    r.p = (bd.p.gam - 1.0) * (r.re - 0.5 * r.r * (r.u * r.u + r.v * r.v + r.w * r.w));
-    r.H = r.re + r.p;
+    // the mean in the caller's frame, then in each block's stored axis order
+    double mc[5];
+    for (int v = 0; v < 5; v++) mc[kVarMap[bd.perm][v]] = mean[v];
+    auto ref_of = [&](int perm) {
+      RefState r{};
+      r.r = mc[kVarMap[perm][0]];
+      r.ru = mc[kVarMap[perm][1]];
+      r.rv = mc[kVarMap[perm][2]];
+      r.rw = mc[kVarMap[perm][3]];
+      r.re = mc[kVarMap[perm][4]];
+      r.u = r.ru / r.r;
+      r.v = r.rv / r.r;
+      r.w = r.rw / r.r;
+      r.p = (bd.p.gam - 1.0) * (r.re - 0.5 * r.r * (r.u * r.u + r.v * r.v + r.w * r.w));
+      r.H = r.re + r.p;
+      return r;
+    };
     if (h->shared_ref)
-      for (auto &b : h->b) b.ref = r;
+      for (auto &b : h->b) b.ref = ref_of(b.perm);
     else
-      bd.ref = r;
+      bd.ref = ref_of(bd.perm);
     h->tables_ready = false;
   }
   return 0;
@@ -1440,7 +1517,8 @@ int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
   std::vector<mgpu_block> stored;
   std::vector<int> perm;
   {
-    std::string msg = relabel_blocks(*c, stored, perm);
+    std::string msg = check_cartesian_frames(*c);
+    if (msg.empty()) msg = relabel_blocks(*c, stored, perm);
     mgpu_case cs = *c;
     cs.blocks = stored.data();
     if (msg.empty()) msg = check_topology(cs);
@@ -1660,6 +1738,88 @@ int mgpu_kernel_time(mgpu_handle *h, int kind, int64_t *launches, double *ms) {
 const char *mgpu_last_error(const mgpu_handle *h) {
   if (!h) return g_create_error.c_str();
   return h->err.c_str();
+}
+
+int mgpu_exchange_plan(const mgpu_case *c, int set, int64_t *map, int64_t cap, int64_t *count) {
+  if (!c || !count || (cap > 0 && !map)) return fail(nullptr, MGPU_EINVAL, "null argument");
+  if (set != 0 && set != 1) return fail(nullptr, MGPU_EINVAL, "set=%d: 0 state, 1 viscous", set);
+  if (c->nblocks < 1 || !c->blocks) return fail(nullptr, MGPU_EINVAL, "nblocks must be >= 1");
+  std::vector<mgpu_block> stored;
+  std::vector<int> perm;
+  std::string msg = check_cartesian_frames(*c);
+  if (msg.empty()) msg = relabel_blocks(*c, stored, perm);
+  mgpu_case cs = *c;
+  cs.blocks = stored.data();
+  if (msg.empty()) msg = check_topology(cs);
+  if (!msg.empty()) return fail(nullptr, MGPU_EINVAL, "%s", msg.c_str());
+  std::vector<BlockData> bds(c->nblocks);
+  for (int b = 0; b < c->nblocks; b++) {
+    const mgpu_block &bl = stored[b];
+    Geom &g = bds[b].g;
+    g.nx = bl.nx;
+    g.ny = bl.ny;
+    g.nz = bl.nz;
+    g.sx = bl.nx + 2 * G;
+    g.sy = (idx_t)g.sx * (idx_t)(bl.ny + 2 * G);
+    g.n = (long)g.sy * (bl.nz + 2 * G);
+    bds[b].perm = perm[b];
+  }
+  XSet xs;
+  xs.visc = set == 1;
+  xs.sensor = xs.radii = set == 1;
+  std::vector<FaceTask> tasks;
+  std::vector<IfaceChunk> chunks;
+  if (!build_tasks(cs, bds, set == 1 ? 2 : G, xs, tasks, chunks))
+    return fail(nullptr, MGPU_EINVAL,
+                "interface exchange would reach outside a block's arrays, or a face meets its neighbour's axes "
+                "otherwise than the blocks' axis relabelings require");
+  // stored element and array of block b -> the caller's element and array
+  auto caller_elem = [&](int b, long off) -> long {
+    const Geom &g = bds[b].g;
+    const long Q0 = g.sx, Q1 = g.ny + 2 * G, Q2 = g.nz + 2 * G;
+    const long a0 = off % Q0, a1 = (off / Q0) % Q1, a2 = off / (Q0 * Q1);
+    long ui = a0, uj = a1, uk = a2, P0 = Q0, P1 = Q1;
+    if (perm[b] == 1) {
+      ui = a1;
+      uj = a0;
+      P0 = Q1;
+      P1 = Q0;
+    } else if (perm[b] == 2) {
+      uj = a2;
+      uk = a1;
+      P1 = Q2;
+    }
+    return ui + P0 * (uj + P1 * uk);
+  };
+  auto caller_var = [&](int b, int v) -> int {
+    const int p = perm[b];
+    if (set == 0) return kVarMap[p][v];
+    if (v < 6) return tau_index(kAxisMap[p][kTauPair[v][0]], kAxisMap[p][kTauPair[v][1]]);
+    if (v < NTAU) return 6 + kAxisMap[p][v - 6];
+    if (v < NTAU + 2) return v;
+    return NTAU + 2 + kAxisMap[p][v - NTAU - 2];
+  };
+  const int nv = xs.nvar();
+  int64_t n = 0;
+  for (const FaceTask &t : tasks)
+    for (int a2 = 0; a2 < t.n[2]; a2++)
+      for (int a1 = 0; a1 < t.n[1]; a1++)
+        for (int a0 = 0; a0 < t.n[0]; a0++) {
+          const long pd = t.dst_base + (long)a0 * t.dst_s[0] + (long)a1 * t.dst_s[1] + (long)a2 * t.dst_s[2];
+          const long ps = t.src_base + (long)a0 * t.src_s[0] + (long)a1 * t.src_s[1] + (long)a2 * t.src_s[2];
+          for (int v = 0; v < nv; v++, n++) {
+            if (n >= cap) continue;
+            int64_t *e = map + 6 * n;
+            e[0] = t.dst;
+            e[1] = caller_elem(t.dst, pd);
+            e[2] = caller_var(t.dst, v);
+            e[3] = t.src;
+            e[4] = caller_elem(t.src, ps);
+            e[5] = caller_var(t.src, t.vmap[v]);
+          }
+        }
+  *count = n;
+  return 0;
 }
 
 int mgpu_diagnostics(mgpu_handle *h, mgpu_diag *out) {

@@ -313,13 +313,17 @@ __host__ __device__ constexpr int krow(int d, int c) {
 // One face of one destination block in the interface exchange: the ghost
 // layers of block `dst` on that face take the interior layers of block
 // `src` on the neighbour face.  Point (a0,a1,a2) of the box n[0]xn[1]xn[2]
-// writes element dst_base + sum a_d*dst_s[d] of each of the dst block's
-// arrays and reads src_base + sum a_d*src_s[d] of the src block's.
+// writes element dst_base + sum a_d*dst_s[d] of array v of the dst block and
+// reads src_base + sum a_d*src_s[d] of array vmap[v] of the src block: the
+// two blocks may store their axes in different orders (blocks with walls on
+// their i- or k-faces), and vector and tensor components follow the axes.
+constexpr int kIfaceMaxVar = 16;
 struct FaceTask {
   int dst, src;
   int n[3];
   int dst_base, src_base;
   int dst_s[3], src_s[3];
+  int8_t vmap[kIfaceMaxVar];
 };
 struct IfaceChunk {
   int task, start;

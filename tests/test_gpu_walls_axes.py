@@ -124,3 +124,67 @@ def test_z_walled_channel_in_two_blocks():
     print(f"relerr {e:.3e}", flush=True)
     assert e < 1e-12
 
+
+
+def _cut_mixed(t, axis, wall_axis):
+    """Two blocks cut from case t along `axis` (0 i, 1 j): block 0 keeps the
+    walls of t on `wall_axis`, block 1 has periodic faces there, so the two
+    blocks are stored with different axis orders and every interface joins
+    a relabeled block to one stored as given."""
+    n = (t.nx, t.ny, t.nz)
+    h = n[axis] // 2
+    blocks, bcs = [], []
+    for b in range(2):
+        ext = list(n)
+        ext[axis] = h
+        sl = [slice(None)] * 3
+        sl[2 - axis] = slice(b * h, b * h + h + 2 * G)
+        cart = list(t.cart)
+        for q in (axis, axis + 3):
+            cart[q] = t.cart[q][b * h:b * h + h + 2 * G].copy()
+        faces = [0] * 6
+        if b == 0:
+            faces[2 * wall_axis] = faces[2 * wall_axis + 1] = 2
+        sub = cases.Case(f"part{b}", *ext, curvilinear=False, iorder_visc=t.iorder_visc, filter=t.filter,
+                         ducros=t.ducros, diffscale=t.diffscale, chi=t.chi, dt=t.dt, cart=cart, faces=tuple(faces),
+                         adiabatic=t.adiabatic, t_wall=t.t_wall, cextrp2=t.cextrp2, cextrp3=t.cextrp3)
+        sub.U = [np.ascontiguousarray(a[tuple(sl)]) for a in t.U]
+        bcs.append(sub)
+        kind = [_lib.FACE_PERIODIC] * 6
+        nbr_block = [b] * 6
+        for s in range(2):
+            kind[2 * axis + s] = _lib.FACE_INTERFACE
+            nbr_block[2 * axis + s] = 1 - b
+        if b == 0:
+            kind[2 * wall_axis] = kind[2 * wall_axis + 1] = _lib.FACE_WALL
+        blocks.append(dict(nx=ext[0], ny=ext[1], nz=ext[2], kind=kind, nbr_block=nbr_block,
+                           nbr_face=[1, 0, 3, 2, 5, 4], orient=[0] * 6,
+                           wall=[int(k == _lib.FACE_WALL) for k in kind]))
+    return blocks, bcs
+
+
+@pytest.mark.parametrize("which", ["z_cut_x", "x_cut_y"])
+def test_relabeled_block_joined_to_a_block_stored_as_given(which):
+    """Interfaces between a block stored with exchanged axes (walls on its k-
+    or i-faces) and one stored as given: the momentum, stress and sensor
+    components cross in the neighbour's axis order.  FP64 against the oracle
+    within 1e-12 after 5 steps; mixed precision (one reference state for both
+    blocks, each in its own axis order) within 1e-6 in density after 20."""
+    from oracle import oracle as orc
+    t = _turned("z" if which == "z_cut_x" else "x")
+    axis, wall_axis = (0, 2) if which == "z_cut_x" else (1, 0)
+    blocks, bcs = _cut_mixed(t, axis, wall_axis)
+    print(f"{which}: {t.nx}x{t.ny}x{t.nz} in 2 blocks, walls on axis {wall_axis} of block 0 only", flush=True)
+    Ug = _gpu(t, 5, topology=blocks, block_cases=bcs)
+    Uc = [bc.copy_state() for bc in bcs]
+    orc.step_multi(t, blocks, bcs, Uc, 5)
+    e = max(relerr(a, b) for a, b in zip(Ug, Uc))
+    print(f"FP64 relerr {e:.3e}", flush=True)
+    assert e < 1e-12
+    Um = _gpu(t, 20, precision=_lib.PREC_MIXED, topology=blocks, block_cases=bcs)
+    Uc = [bc.copy_state() for bc in bcs]
+    orc.step_multi(t, blocks, bcs, Uc, 20)
+    linf = max(float(np.max(np.abs(gr.interior(a[0]) - gr.interior(b[0])))) / float(np.max(np.abs(gr.interior(b[0]))))
+               for a, b in zip(Um, Uc))
+    print(f"mixed rho Linf {linf:.3e} after 20 steps", flush=True)
+    assert linf <= 1e-6

@@ -133,19 +133,3 @@ def test_artvisc_shock_flags_are_checked():
     c = cases.tgv(16, filter=5)
     rc, msg = _create_case(c, None)
     assert rc == _lib.EINVAL and "filter=5" in msg
-
-
-def test_interface_with_one_relabeled_side_is_refused():
-    """A block walled on its k-faces is stored with j and k exchanged, which
-    reverses its i-faces' tangential axes; an interface from such a face to a
-    face of a block stored as given is refused at create."""
-    c = cases.channel(16, 20, 12)
-    blocks = []
-    for b in range(2):
-        kind = [_lib.FACE_INTERFACE] * 6
-        if b == 0:
-            kind[4] = kind[5] = _lib.FACE_WALL
-        blocks.append({"nx": 8, "ny": 20, "nz": 12, "kind": kind, "nbr_block": [1 - b, 1 - b, b, b, b, b],
-                       "nbr_face": [1, 0, 3, 2, 5, 4], "orient": [0] * 6})
-    rc, msg = _create_case(c, blocks)
-    assert rc == _lib.EINVAL and "not supported" in msg
