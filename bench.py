@@ -48,21 +48,46 @@ def stage_words_per_cell(case) -> float:
     return sum(w) / len(w)
 
 
-def cpu_baseline(case, nsteps: int):
-    """Time the CPU oracle on the host cores for a bounded sample."""
-    from oracle.oracle import Oracle, num_threads
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(case, nsteps: int, single_steps: int = 1):
+    """Time the CPU oracle on the host cores for a bounded sample: every
+    thread OpenMP gives it (all host cores), then one thread."""
+    from oracle.oracle import Oracle, num_threads, set_num_threads
     o = Oracle(case)
     U = case.copy_state()
     o.step(U, 1)  # first-touch of its work arrays
-    U = case.copy_state()
-    t0 = time.perf_counter()
-    o.step(U, nsteps)
-    t = time.perf_counter() - t0
+    threads = num_threads()
+
+    def timed(n):
+        U = case.copy_state()
+        t0 = time.perf_counter()
+        o.step(U, n)
+        return time.perf_counter() - t0
+
+    t = timed(nsteps)
     val = case.ncells * case.nrk * nsteps / t / 1e6
-    return {"value": round(val, 3), "unit": "Mcell-RK-stages/s", "cores": num_threads(),
+    set_num_threads(1)
+    t1 = timed(single_steps)
+    set_num_threads(threads)
+    val1 = case.ncells * case.nrk * single_steps / t1 / 1e6
+    return {"value": round(val, 3), "unit": "Mcell-RK-stages/s", "cores": threads,
             "kind": "port",
-            "sample": f"{nsteps} full RK steps of the same 512x256x128 block, oracle/musica_oracle.c, "
-                      f"OpenMP over k, {t:.2f} s"}
+            "sample": f"{nsteps} full RK steps of the same {case.nx}x{case.ny}x{case.nz} block, "
+                      f"oracle/musica_oracle.c, OpenMP over k on {threads} threads, {t:.2f} s; "
+                      f"{single_steps} step on 1 thread, {t1:.2f} s",
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(),
+            "single_thread": {"value": round(val1, 3), "unit": "Mcell-RK-stages/s", "cores": 1,
+                              "steps": single_steps, "seconds": round(t1, 2)}}
 
 
 def copy_calibration(nbytes: int = 1 << 30, reps: int = 5) -> int:
@@ -79,9 +104,13 @@ def copy_calibration(nbytes: int = 1 << 30, reps: int = 5) -> int:
     return 0
 
 
-def run_leg(case, precision: int, steps: int, warmup: int, tune: bool):
+def run_leg(case, precision: int, steps: int, warmup: int, tune: bool, total_steps: int = 0):
     """Advance `case` on the GPU: autotune, warm up, then time `steps` steps
-    with per-kernel HIP events on the library's stream."""
+    replayed from the captured graph (the production path); then the same
+    number of steps launched one by one with per-kernel HIP events on the
+    library's stream (the per-kernel split and the stage kernel's launch
+    time); then, untimed, up to `total_steps` steps in all (the state the
+    precision comparison reads)."""
     from musica2_newgrid_amd import _lib
     from musica2_newgrid_amd.solver import GpuSolver
 
@@ -92,19 +121,26 @@ def run_leg(case, precision: int, steps: int, warmup: int, tune: bool):
              "visc": solver.stage_config(2), "vdiv": solver.stage_config(3), "filter": solver.stage_config(4)}
     solver.step(warmup)
     solver.sync()
+    t0 = time.perf_counter()
+    solver.step(steps)
+    solver.sync()
+    t = time.perf_counter() - t0
 
     solver.profile(True)
     solver.sync()
     t0 = time.perf_counter()
     solver.step(steps)
     solver.sync()
-    t = time.perf_counter() - t0
-
+    t_eager = time.perf_counter() - t0
     times = {_lib.K_NAMES[k]: solver.kernel_time(k) for k in _lib.K_NAMES}
     solver.profile(False)
+    done = warmup + 2 * steps
+    if total_steps > done:
+        solver.step(total_steps - done)
+        done = total_steps
     rho = solver.get_state()[0]
     solver.close()
-    return {"t": t, "times": times, "tuned": tuned, "rho": rho}
+    return {"t": t, "t_eager": t_eager, "times": times, "tuned": tuned, "rho": rho, "steps_done": done}
 
 
 def multiblock_leg(steps: int, warmup: int):
@@ -229,17 +265,20 @@ def serve_leg(nreq: int, nsteps: int, seed: int = 5):
     from musica2_newgrid_amd import serving
     edges = (64, 96, 128, 192, 256)
     mean_cells = sum(e ** 3 for e in edges) / len(edges)
-    # arrival rate for about 80 % of the device: service time of the mean
-    # request at ~9e9 cell-stages/s
-    rate = 0.8 / (mean_cells * 4 * nsteps / 9.0e9)
-    reqs = serving.make_requests(nreq, seed=seed, edges=edges, nsteps=nsteps, rate=rate)
     budget = 64 << 30
-    lat, wall, work = serving.run_stream(reqs, budget_bytes=budget, quantum=10)
+    # arrival rate for about 80 % of the device: the service rate measured on
+    # a saturated mix of the same request kinds
+    svc = serving.service_rate(edges, budget_bytes=budget)
+    rate = 0.8 / (mean_cells * 4 * nsteps / svc)
+    reqs = serving.make_requests(nreq, seed=seed, edges=edges, nsteps=nsteps, rate=rate)
+    stats = {}
+    lat, wall, work = serving.run_stream(reqs, budget_bytes=budget, quantum=10, stats=stats)
     p50, p99 = np.percentile(lat, [50, 99])
     return {"requests": nreq, "steps_per_request": nsteps, "arrival_rate_per_s": round(rate, 3),
-            "hbm_budget_GiB": budget >> 30, "wall_s": round(wall, 3),
+            "measured_service_rate": round(svc / 1e6, 2), "hbm_budget_GiB": budget >> 30, "wall_s": round(wall, 3),
             "throughput": round(work / wall / 1e6, 2), "unit": "Mcell-RK-stages/s",
-            "p50_ms": round(p50 * 1e3, 2), "p99_ms": round(p99 * 1e3, 2), "seed": seed}
+            "p50_ms": round(p50 * 1e3, 2), "p99_ms": round(p99 * 1e3, 2), "seed": seed,
+            "co_batching": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in stats.items()}}
 
 
 def main() -> int:
@@ -251,6 +290,8 @@ def main() -> int:
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--error-steps", type=int, default=1000,
+                    help="steps after which the mixed leg's density error against the FP64 leg is measured")
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--ny", type=int, default=256)
     ap.add_argument("--nz", type=int, default=128)
@@ -280,8 +321,9 @@ def main() -> int:
     diss = {"shock": (2, "shock filter + Ducros"),
             "artvisc_shock": (4, "artificial viscosity with shock capturing + Ducros")}[args.dissipation]
     case = cases.vortex_wavy(args.nx, args.ny, args.nz, filter=diss[0])
+    total = 0 if args.no_mixed else args.error_steps
     print(f"bench: {args.nx}x{args.ny}x{args.nz} FP64 leg, {args.steps} steps", file=sys.stderr, flush=True)
-    fp = run_leg(case, _lib.PREC_FP64, args.steps, args.warmup, not args.no_tune)
+    fp = run_leg(case, _lib.PREC_FP64, args.steps, args.warmup, not args.no_tune, total)
     t = fp["t"]
     nst, ms_stage = fp["times"]["stage"]
 
@@ -302,10 +344,15 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"curvilinear block {args.nx}x{args.ny}x{args.nz}, isentropic vortex "
-                               f"M=0.5, viscous, {diss[1]}, RK4, FP64",
+        "config": {"workload": f"curvilinear block {args.nx}x{args.ny}x{args.nz} with 3-D GCL metrics of a fully "
+                               f"periodic wavy map (x, y and z each displaced by a product of sines; periodic "
+                               f"faces, no walls, no tanh stretching), isentropic vortex M=0.5, viscous, "
+                               f"{diss[1]}, RK4, FP64, steps replayed from the captured HIP graph",
                    "cells": case.ncells, "nrk": case.nrk, "precision": "fp64",
                    "stage_tiles": {k: {"tile": f"{v[0]}x{v[1]}", "kchunk": v[2]} for k, v in fp["tuned"].items()}},
+        "timing": {"value": "graph replay of the timed steps",
+                   "roofline": f"per-launch HIP events on the library's stream over {args.steps} more steps "
+                               f"launched one by one ({fp['t_eager'] / args.steps * 1e3:.4f} ms per step)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": (STAGE_TRAFFIC_BYTES if (args.nx, args.ny, args.nz) == STAGE_TRAFFIC_WORKLOAD
@@ -318,7 +365,8 @@ def main() -> int:
     }
     if not args.no_mixed:
         print("bench: mixed-precision leg", file=sys.stderr, flush=True)
-        mx = run_leg(case, _lib.PREC_MIXED, args.steps, args.warmup, not args.no_tune)
+        mx = run_leg(case, _lib.PREC_MIXED, args.steps, args.warmup, not args.no_tune, total)
+        assert mx["steps_done"] == fp["steps_done"]
         a = mx["rho"][5:-5, 5:-5, 5:-5]
         b = fp["rho"][5:-5, 5:-5, 5:-5]
         mval = case.ncells * case.nrk * args.steps / mx["t"] / 1e6
@@ -328,7 +376,7 @@ def main() -> int:
             "speedup_vs_fp64": round(mval / value, 3),
             "rho_linf_vs_fp64": float(np.max(np.abs(a - b)) / np.max(np.abs(b))),
             "rho_l2_vs_fp64": float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2))),
-            "after_steps": args.warmup + args.steps,
+            "after_steps": mx["steps_done"],
             "kernel_ms": {k: {"launches": n, "ms": round(ms, 3)} for k, (n, ms) in mx["times"].items()},
         }
     if not args.no_c2:

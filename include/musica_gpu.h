@@ -295,6 +295,13 @@ int mgpu_server_take(mgpu_server *s, int64_t id, double *rho, double *rhou, doub
 /* Device bytes held by resident requests. */
 int64_t mgpu_server_memory(const mgpu_server *s);
 
+/* Co-batching record since mgpu_server_create: *group_steps = time steps
+ * launched for groups (one batched launch per kernel and stage each),
+ * *request_steps = request time steps they advanced (their ratio is the mean
+ * number of requests per launch), *max_members = the most requests one group
+ * step advanced. */
+int mgpu_server_batching(const mgpu_server *s, int64_t *group_steps, int64_t *request_steps, int32_t *max_members);
+
 const char *mgpu_server_last_error(const mgpu_server *s);
 void mgpu_server_destroy(mgpu_server *s);
 

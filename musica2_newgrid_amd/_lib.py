@@ -49,6 +49,7 @@ EXPORTS = (
     "mgpu_server_status",
     "mgpu_server_take",
     "mgpu_server_memory",
+    "mgpu_server_batching",
     "mgpu_server_last_error",
     "mgpu_server_destroy",
 )
@@ -143,6 +144,8 @@ def lib() -> C.CDLL:
     L.mgpu_server_take.argtypes = [C.c_void_p, C.c_int64] + [_dp] * 5
     L.mgpu_server_memory.argtypes = [C.c_void_p]
     L.mgpu_server_memory.restype = C.c_int64
+    L.mgpu_server_batching.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+    L.mgpu_server_batching.restype = C.c_int
     L.mgpu_server_last_error.argtypes = [C.c_void_p]
     L.mgpu_server_last_error.restype = C.c_char_p
     L.mgpu_server_destroy.argtypes = [C.c_void_p]

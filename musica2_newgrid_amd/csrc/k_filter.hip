@@ -738,7 +738,7 @@ TileRange filter_tiles(const Geom &g, int kc_req) {
   const long nxy = (long)t.tiles_x * t.tiles_y;
   const long want = (1024 + nxy - 1) / nxy;
   int kc = kc_req > 0 ? kc_req : (int)((g.nz + want - 1) / want);
-  if (kc_req <= 0 && kc < 16) kc = 16;
+  if (kc_req <= 0 && kc < 16) kc = nxy * ((g.nz + 15) / 16) >= 512 ? 16 : (kc < 4 ? 4 : kc);
   if (kc > g.nz) kc = g.nz;
   t.kchunk = kc;
   return t;

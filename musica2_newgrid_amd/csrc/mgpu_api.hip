@@ -611,14 +611,16 @@ int setup_exchange(mgpu_handle *h) {
 }
 
 // Default k-chunk: aim at >= 2048 workgroups, at least 32 planes per chunk
-// (the 10-plane queue warm-up is paid once per chunk).
+// (the 10-plane queue warm-up is paid once per chunk), but at least 8 planes
+// only on blocks too small to give every CU a workgroup otherwise.
 int kchunk_for(const Geom &g, int shape) {
   const int TX = kStageShapes[shape][0], TY = kStageShapes[shape][1];
   const long nxy = (long)((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY);
   long want = (2048 + nxy - 1) / nxy;
   if (want < 1) want = 1;
   long kc = (g.nz + want - 1) / want;
-  if (kc < 32) kc = 32;
+  const long kmin = nxy * ((g.nz + 31) / 32) >= 512 ? 32 : 8;
+  if (kc < kmin) kc = kmin;
   if (kc > g.nz) kc = g.nz;
   return (int)kc;
 }
@@ -717,6 +719,7 @@ int stage_ranges(const mgpu_handle *h, int v, std::vector<TileRange> &tr) {
   for (const auto &bd : h->b) {
     const int kc = h->kchunk[v] > 0 ? h->kchunk[v] : kchunk_for(bd.g, h->shape[v]);
     TileRange t = stage_tiles(bd.g, h->shape[v], kc);
+    t.jfirst = h->jfirst[v];
     t.first = first;
     first += tile_count(t, bd.g.nz);
     tr.push_back(t);
@@ -821,6 +824,9 @@ int build_tables(mgpu_handle *h) {
     const BlockData &bd = h->b[b];
     TileRange tv = visc_tiles(bd.g, h->kc_aux[0]), td = vdiv_tiles(bd.g, h->kc_aux[1]),
               tf = filter_tiles(bd.g, h->kc_aux[2]);
+    tv.jfirst = h->jf_aux[0];
+    td.jfirst = h->jf_aux[1];
+    tf.jfirst = h->jf_aux[2];
     tv.first = fv;
     td.first = fd;
     tf.first = ff;
@@ -1425,7 +1431,7 @@ int tune_aux(mgpu_handle *h, bool force, hipEvent_t e0, hipEvent_t e1) {
   const int s0 = h->cur, sa = (h->cur + 1) % 3;
   int maxnz = 0;
   for (const auto &bd : h->b) maxnz = bd.g.nz > maxnz ? bd.g.nz : maxnz;
-  const int kcs[] = {0, 16, 32, 64, 128, 256};
+  const int kcs[] = {0, 4, 8, 16, 32, 64, 128, 256};
   int rc = 0;
   for (int kind = 0; kind < 3 && !rc; kind++) {
     if (kind < 2 && !h->c.iorder_visc) continue;
@@ -1436,16 +1442,19 @@ int tune_aux(mgpu_handle *h, bool force, hipEvent_t e0, hipEvent_t e1) {
       std::lock_guard<std::mutex> lk(g_tune_mu);
       auto it = g_tune_cache.find(key);
       if (it != g_tune_cache.end()) {
+        h->jf_aux[kind] = it->second.first;
         h->kc_aux[kind] = it->second.second;
         h->tables_ready = false;
         continue;
       }
     }
     float best = 1e30f;
-    int best_kc = 0;
-    for (int kc : kcs) {
+    int best_kc = 0, best_jf = 0;
+    for (int cand = 0; cand < 2 * (int)(sizeof kcs / sizeof kcs[0]) && !rc; cand++) {
+      const int jf = cand & 1, kc = kcs[cand >> 1];
       if (kc > 16 && kc / 2 >= maxnz) break;
       h->kc_aux[kind] = kc;
+      h->jf_aux[kind] = jf;
       h->tables_ready = false;
       if ((rc = build_tables(h))) break;
       if (kind == 2) {
@@ -1454,6 +1463,7 @@ int tune_aux(mgpu_handle *h, bool force, hipEvent_t e0, hipEvent_t e1) {
         for (int b = 0; b < nb; b++) {
           tab[b].a = filter_args(h, h->b[b], s0, sa);
           tab[b].t = filter_tiles(h->b[b].g, kc);
+          tab[b].t.jfirst = jf;
           tab[b].t.first = first;
           first += tile_count(tab[b].t, h->b[b].g.nz);
         }
@@ -1481,12 +1491,14 @@ int tune_aux(mgpu_handle *h, bool force, hipEvent_t e0, hipEvent_t e1) {
       if (ms < best) {
         best = ms;
         best_kc = kc;
+        best_jf = jf;
       }
     }
     h->kc_aux[kind] = best_kc;
+    h->jf_aux[kind] = best_jf;
     h->tables_ready = false;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune_cache[key] = {0, best_kc};
+    g_tune_cache[key] = {best_jf, best_kc};
   }
   return rc;
 }
@@ -1623,7 +1635,8 @@ int mgpu_autotune(mgpu_handle *h, int flags) {
       std::lock_guard<std::mutex> lk(g_tune_mu);
       auto it = g_tune_cache.find(key);
       if (it != g_tune_cache.end()) {
-        h->shape[v] = it->second.first;
+        h->shape[v] = it->second.first & 0xff;
+        h->jfirst[v] = it->second.first >> 8;
         h->kchunk[v] = it->second.second;
         continue;
       }
@@ -1632,16 +1645,18 @@ int mgpu_autotune(mgpu_handle *h, int flags) {
     // block (0 = the per-block default), timed on one batched launch of
     // the stage (U^n in, a stage buffer out)
     float best = 1e30f;
-    int best_shape = h->shape[v], best_kc = h->kchunk[v];
-    const int kcs[] = {0, 16, 32, 64, 128, 256};
+    int best_shape = h->shape[v], best_kc = h->kchunk[v], best_jf = h->jfirst[v];
+    const int kcs[] = {0, 4, 8, 16, 32, 64, 128, 256};
     int maxnz = 0;
     for (const auto &bd : h->b) maxnz = bd.g.nz > maxnz ? bd.g.nz : maxnz;
     std::vector<Batch<StageArgs>> tab(nb);
-    for (int sh = 0; sh < kNumStageShapes && !rc; sh++) {
+    for (int sj = 0; sj < 2 * kNumStageShapes && !rc; sj++) {
+      const int sh = sj >> 1, jf = sj & 1;
       for (int kc : kcs) {
         if (kc > 16 && kc / 2 >= maxnz) break;  // same as the previous candidate for every block
         h->shape[v] = sh;
         h->kchunk[v] = kc;
+        h->jfirst[v] = jf;
         std::vector<TileRange> tr;
         const int nt = stage_ranges(h, v, tr);
         // Euler variant timed as an intermediate stage, viscous variant as the last one
@@ -1669,13 +1684,15 @@ int mgpu_autotune(mgpu_handle *h, int flags) {
           best = ms;
           best_shape = sh;
           best_kc = kc;
+          best_jf = jf;
         }
       }
     }
     h->shape[v] = best_shape;
     h->kchunk[v] = best_kc;
+    h->jfirst[v] = best_jf;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune_cache[key] = {best_shape, best_kc};
+    g_tune_cache[key] = {best_shape | (best_jf << 8), best_kc};
   }
   if (!rc) rc = tune_aux(h, force, e0, e1);
   h->ev_pool.push_back(e0);

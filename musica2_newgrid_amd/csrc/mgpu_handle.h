@@ -77,6 +77,10 @@ struct mgpu_handle {
   int kchunk[2] = {0, 0};
   // planes per workgroup of k_visc, k_vdiv and the filter (0: per-block default)
   int kc_aux[3] = {0, 0, 0};
+  // tile numbering (TileRange::jfirst) of the stage variants and of k_visc,
+  // k_vdiv and the filter
+  int jfirst[2] = {0, 0};
+  int jf_aux[3] = {0, 0, 0};
   // batched launch tables in device memory, one entry per block
   mgpu::Batch<mgpu::StageArgs> *d_stage = nullptr;  // [cur][irk-1][block]
   mgpu::Batch<mgpu::ViscArgs> *d_visc = nullptr;    // [cur][block]

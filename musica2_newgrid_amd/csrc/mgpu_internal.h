@@ -112,6 +112,11 @@ __device__ __forceinline__ int xcd_remap(int w, int n) {
 // `first` and reads that block's arguments from a device table.
 struct TileRange {
   int first, tiles_x, tiles_y, kchunk;
+  // tile numbering inside the block: 0 i fastest, 1 j fastest.  With j
+  // fastest the consecutive logical tiles that xcd_remap gives one XCD form
+  // columns of tiles along j, whose 5-row j-halos are their neighbours' own
+  // rows: those re-reads can then hit the XCD's L2.
+  int jfirst;
 };
 template <class A>
 struct Batch {
@@ -138,10 +143,17 @@ __device__ __forceinline__ const E &batch_find(const E *__restrict__ tab, int nb
 // Tile coordinates of logical tile w inside its block.
 __device__ __forceinline__ void tile_coords(const TileRange &t, int w, int &tile_x, int &tile_y, int &chunk) {
   const int l = w - t.first;
-  tile_x = l % t.tiles_x;
-  const int r = l / t.tiles_x;
-  tile_y = r % t.tiles_y;
-  chunk = r / t.tiles_y;
+  if (t.jfirst) {
+    tile_y = l % t.tiles_y;
+    const int r = l / t.tiles_y;
+    tile_x = r % t.tiles_x;
+    chunk = r / t.tiles_x;
+  } else {
+    tile_x = l % t.tiles_x;
+    const int r = l / t.tiles_x;
+    tile_y = r % t.tiles_y;
+    chunk = r / t.tiles_y;
+  }
 }
 
 // Thermodynamics / transport constants of a case.

@@ -55,6 +55,10 @@ struct Group {
 struct mgpu_server {
   int64_t budget = 0, used = 0;
   int64_t next_id = 1;
+  // co-batching record: group steps launched, request steps they advanced,
+  // and the most requests one group step has advanced
+  int64_t group_steps = 0, request_steps = 0;
+  int max_members = 0;
   std::map<int64_t, std::unique_ptr<Request>> reqs;
   std::vector<Request *> queue;  // submission order
   std::map<Key, Group> groups;
@@ -245,6 +249,9 @@ int mgpu_server_tick(mgpu_server *s, int max_steps, int *n_resident, int *n_queu
       rc = step(g.h, k);
       if (rc) return sfail(s, rc, g.h->err);
       for (Request *r : g.members) r->done_steps += k;
+      s->group_steps += k;
+      s->request_steps += (int64_t)k * (int64_t)g.members.size();
+      if ((int)g.members.size() > s->max_members) s->max_members = (int)g.members.size();
     }
     resident += (int)g.members.size();
   }
@@ -279,6 +286,14 @@ int mgpu_server_take(mgpu_server *s, int64_t id, double *rho, double *rhou, doub
 }
 
 int64_t mgpu_server_memory(const mgpu_server *s) { return s ? s->used : 0; }
+
+int mgpu_server_batching(const mgpu_server *s, int64_t *group_steps, int64_t *request_steps, int32_t *max_members) {
+  if (!s) return MGPU_EINVAL;
+  if (group_steps) *group_steps = s->group_steps;
+  if (request_steps) *request_steps = s->request_steps;
+  if (max_members) *max_members = s->max_members;
+  return 0;
+}
 
 const char *mgpu_server_last_error(const mgpu_server *s) {
   if (!s) return g_create_error.c_str();

@@ -77,6 +77,15 @@ class Server:
     def memory(self) -> int:
         return int(lib().mgpu_server_memory(self.h))
 
+    def batching(self) -> dict:
+        """Group steps launched, request steps advanced, their ratio (requests
+        per batched launch) and the largest group."""
+        g, r, m = C.c_int64(), C.c_int64(), C.c_int32()
+        _check(lib().mgpu_server_batching(self.h, C.byref(g), C.byref(r), C.byref(m)), self.h,
+               "mgpu_server_batching")
+        return {"group_steps": g.value, "request_steps": r.value, "max_members": m.value,
+                "requests_per_launch": (r.value / g.value) if g.value else 0.0}
+
     def close(self):
         if getattr(self, "h", None):
             lib().mgpu_server_destroy(self.h)
@@ -126,10 +135,11 @@ def make_requests(n: int, seed: int, edges=(64, 96, 128, 192, 256), re_range=(40
 
 
 def run_stream(reqs: Sequence[Request], budget_bytes: int, quantum: int = 10,
-               results: Optional[dict] = None):
+               results: Optional[dict] = None, stats: Optional[dict] = None):
     """Replay the arrival sequence in wall-clock time against a server.
     Returns (latencies in s, wall time in s, cell-stage count); results, if
-    given, receives the final state of each request keyed by its index."""
+    given, receives the final state of each request keyed by its index, and
+    stats the server's co-batching record."""
     order = sorted(range(len(reqs)), key=lambda q: reqs[q].arrival)
     lat = [0.0] * len(reqs)
     ids = {}
@@ -154,5 +164,22 @@ def run_stream(reqs: Sequence[Request], budget_bytes: int, quantum: int = 10,
                     if results is not None:
                         results[q] = U
         wall = time.perf_counter() - t0
+        if stats is not None:
+            stats.update(srv.batching())
     work = sum(r.case.ncells * r.case.nrk * r.nsteps for r in reqs)
     return lat, wall, work
+
+
+def service_rate(edges=(64, 96, 128, 192, 256), nsteps: int = 20, budget_bytes: int = 64 << 30,
+                 seed: int = 11) -> float:
+    """Measured throughput of the server when saturated (cell-stages/s): one
+    TGV and one vortex request of every edge, FP64 and mixed, all submitted at
+    once and run to completion."""
+    reqs = []
+    for q, e in enumerate(edges):
+        reqs.append(Request(cases.tgv(e), nsteps, _lib.PREC_FP64, 0.0, f"tgv{e}"))
+        reqs.append(Request(cases.vortex_periodic(e), nsteps, _lib.PREC_MIXED, 0.0, f"vortex{e}"))
+    # warm-up: the first graphs and tables of each variant
+    run_stream(reqs[:2], budget_bytes)
+    lat, wall, work = run_stream(reqs, budget_bytes)
+    return work / wall

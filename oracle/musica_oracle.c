@@ -145,6 +145,14 @@ int orc_num_threads(void) {
 #endif
 }
 
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 static double *alloc_arr(const dims_t *d) {
   double *p = (double *)calloc(d->n, sizeof(double));
   return p;

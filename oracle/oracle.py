@@ -79,6 +79,8 @@ def lib():
         L.orc_filter_rhs.restype = None
         L.orc_fill_periodic.argtypes = [C.POINTER(OrcCfg), _dp]
         L.orc_num_threads.restype = C.c_int
+        L.orc_set_num_threads.argtypes = [C.c_int]
+        L.orc_set_num_threads.restype = None
         L.orc_step_multi.argtypes = [C.POINTER(OrcCfg), C.c_int, C.POINTER(OrcBlock), C.POINTER(OrcGrid),
                                      C.POINTER(_dp), C.c_int]
         L.orc_step_multi.restype = C.c_int
@@ -240,3 +242,7 @@ def fill_interfaces(blocks, arrs):
 
 def num_threads() -> int:
     return int(lib().orc_num_threads())
+
+
+def set_num_threads(n: int) -> None:
+    lib().orc_set_num_threads(int(n))

@@ -40,6 +40,10 @@ def test_each_result_is_the_same_alone_and_in_the_mix():
                 break
         assert all(srv.status(r)[0] == DONE for r in ids)
         assert seen_queued
+        bat = srv.batching()
+        print(f"co-batching: {bat}", flush=True)
+        # some group step advanced two or more resident requests in one batched launch
+        assert bat["max_members"] >= 2 and bat["request_steps"] > bat["group_steps"]
         mixed = [srv.take(r) for r in ids]
     for q, (a, b) in enumerate(zip(alone, mixed)):
         for x, y in zip(a, b):
