@@ -228,13 +228,22 @@ const char *mgpu_last_error(const mgpu_handle *h);
  * MPI_REDUCE (stats_tgv takes the gradients its last stage computed; these
  * are the current state's) -- and umax = max(|u| + c), the velocity scale
  * of timestep_new (src/Time/timestep_new.f90:270-288: deltat =
- * CFL*delta_min/umax).  Deterministic: fixed summation order.  Blocks with
- * walls: MGPU_EUNSUP (the gradients read ghosts on every face). */
+ * CFL*delta_min/umax).  Deterministic: fixed summation order.  Next to a
+ * no-slip wall the velocity gradients take the one-sided 5-pt stencils of
+ * grad_vel (rows 1, 2 and n-1, n). */
 typedef struct mgpu_diag {
   double mass, kinetic, enstrophy, umax;
   int64_t cells;
 } mgpu_diag;
 int mgpu_diagnostics(mgpu_handle *h, mgpu_diag *out);
+
+/* Maximum CFL number along i, j and k over the interior cells of all blocks
+ * of the current state U^n, as check_max_cfl prints it
+ * (src/Time/timestep_new.f90:660-842): Cartesian (|u_d| + c) deltat idx_d;
+ * 3-D curvilinear (|(u xi_x + v xi_y + w xi_z) ijacob3| + c |grad xi|
+ * |ijacob3|) deltat; 2.5-D the same with the 2-D metrics in i and j,
+ * Cartesian in k; c = sqrt(gam rg T).  Synchronous. */
+int mgpu_max_cfl(mgpu_handle *h, double cfl[3]);
 
 /* Release every device resource of the handle. */
 void mgpu_destroy(mgpu_handle *h);

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmusica_gpu.so")
 
 SOURCES = ["mgpu_api.hip", "k_stage.hip", "k_visc.hip", "k_filter.hip", "k_iface.hip", "mgpu_serve.hip", "k_diag.hip",
-           "k_wall.hip"]
+           "k_wall.hip", "k_cfl.hip"]
 
 # -Xclang -target-feature -Xclang -packed-fp32-ops: no packed FP32 VALU ops
 HIPCC_FLAGS = [

@@ -1371,6 +1371,7 @@ void destroy(mgpu_handle *h) {
   free_exchange(h);
   free_tables(h);
   if (h->diag_buf) (void)hipFree(h->diag_buf);
+  if (h->cfl_buf) (void)hipFree(h->cfl_buf);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -1379,8 +1380,6 @@ int diagnostics(mgpu_handle *h, mgpu_diag *out) {
   for (size_t bI = 0; bI < h->b.size(); bI++) {
     if (!h->b[bI].have_metrics) return fail(h, MGPU_ESTATE, "block %zu: metrics not set", bI);
     if (!h->b[bI].have_state) return fail(h, MGPU_ESTATE, "block %zu: state not set", bI);
-    if (h->b[bI].g.jwall)
-      return fail(h, MGPU_EUNSUP, "block %zu: diagnostics need valid ghosts on every face (no walls)", bI);
   }
   int rc = refresh_ghosts(h);
   if (rc) return rc;
@@ -1419,6 +1418,48 @@ int diagnostics(mgpu_handle *h, mgpu_diag *out) {
   out->enstrophy = sum[2];
   out->umax = umax;
   out->cells = cells;
+  return 0;
+}
+
+// Maximum CFL number along the caller's i, j, k over every block
+// (check_max_cfl): per-workgroup maxima of each block, maximum on the host;
+// a relabeled block's stored axes are mapped back to the caller's.
+int max_cfl(mgpu_handle *h, double out[3]) {
+  for (size_t bI = 0; bI < h->b.size(); bI++) {
+    if (!h->b[bI].have_metrics) return fail(h, MGPU_ESTATE, "block %zu: metrics not set", bI);
+    if (!h->b[bI].have_state) return fail(h, MGPU_ESTATE, "block %zu: state not set", bI);
+  }
+  double cm[3] = {0.0, 0.0, 0.0};
+  std::vector<double> part;
+  for (auto &bd : h->b) {
+    const long ng = diag_groups(bd.g);
+    if (3 * ng > h->cfl_cap) {
+      HIPCK(h, hipStreamSynchronize(h->stream));
+      if (h->cfl_buf) (void)hipFree(h->cfl_buf);
+      h->cfl_buf = nullptr;
+      h->cfl_cap = 0;
+      HIPCK(h, hipMalloc((void **)&h->cfl_buf, sizeof(double) * 3 * ng));
+      h->cfl_cap = 3 * ng;
+    }
+    CflArgs ca{};
+    ca.g = bd.g;
+    for (int v = 0; v < 5; v++) ca.u.v[v] = bd.st[h->cur][v];
+    ca.met = metrics_of(bd);
+    ca.gam = bd.p.gam;
+    ca.gam1 = bd.p.gam - 1.0;
+    ca.rg = bd.p.rg;
+    ca.dt = bd.p.deltat;
+    ca.partial = h->cfl_buf;
+    HIPCK(h, launch_cfl(ca, h->c.curvilinear, h->stream));
+    part.resize(3 * ng);
+    HIPCK(h, hipMemcpyAsync(part.data(), h->cfl_buf, sizeof(double) * 3 * ng, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    for (int a = 0; a < 3; a++) {
+      const int s = kAxisMap[bd.perm][a];  // the caller's axis a is the block's stored axis s
+      for (long w = 0; w < ng; w++) cm[a] = part[3 * w + s] > cm[a] ? part[3 * w + s] : cm[a];
+    }
+  }
+  for (int a = 0; a < 3; a++) out[a] = cm[a];
   return 0;
 }
 
@@ -1755,6 +1796,12 @@ int mgpu_kernel_time(mgpu_handle *h, int kind, int64_t *launches, double *ms) {
 const char *mgpu_last_error(const mgpu_handle *h) {
   if (!h) return g_create_error.c_str();
   return h->err.c_str();
+}
+
+int mgpu_max_cfl(mgpu_handle *h, double cfl[3]) {
+  if (!h) return MGPU_EINVAL;
+  if (!cfl) return fail(h, MGPU_EINVAL, "null argument");
+  return max_cfl(h, cfl);
 }
 
 int mgpu_exchange_plan(const mgpu_case *c, int set, int64_t *map, int64_t cap, int64_t *count) {

@@ -99,6 +99,8 @@ struct mgpu_handle {
   bool shared_ref = false;  // one mixed-precision reference state for all blocks (set from block 0)
   double *diag_buf = nullptr;  // per-workgroup partial sums of mgpu_diagnostics
   long diag_cap = 0;
+  double *cfl_buf = nullptr;   // per-workgroup maxima of mgpu_max_cfl
+  long cfl_cap = 0;
 };
 
 namespace mgpu_host {

@@ -361,6 +361,16 @@ struct DiagArgs {
   double *partial;
 };
 
+// Maximum CFL number per direction of one block: per-workgroup maxima for i,
+// j, k (3 per workgroup, diag_groups workgroups).
+struct CflArgs {
+  Geom g;
+  CState u;
+  Metrics met;
+  double gam, gam1, rg, dt;
+  double *partial;
+};
+
 }  // namespace mgpu
 
 // Kernel launchers (defined in the .hip files).  Each takes a device table
@@ -396,6 +406,7 @@ hipError_t launch_shock_wall(const FilterArgs &a, hipStream_t s);
 hipError_t launch_iface(const IfaceArgs &a, int nchunks, hipStream_t s);
 hipError_t launch_iface(const IfaceArgs32 &a, int nchunks, hipStream_t s);
 long diag_groups(const Geom &g);
+hipError_t launch_cfl(const CflArgs &a, int geo, hipStream_t s);
 hipError_t launch_wall(const WallArgs &a, hipStream_t s);
 hipError_t launch_visc_wall(const WallViscArgs &a, hipStream_t s);
 hipError_t launch_tau_wallrow(const WallViscArgs &a, hipStream_t s);

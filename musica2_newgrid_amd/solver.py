@@ -152,6 +152,12 @@ class GpuSolver:
         return {"mass": d.mass, "kinetic": d.kinetic, "enstrophy": d.enstrophy, "umax": d.umax,
                 "cells": d.cells}
 
+    def max_cfl(self):
+        """Maximum CFL number along i, j, k of the current state (check_max_cfl)."""
+        out = np.zeros(3)
+        check(lib().mgpu_max_cfl(self.h, dptr(out)), self.h, "mgpu_max_cfl")
+        return out
+
     def sync(self) -> None:
         check(lib().mgpu_sync(self.h), self.h, "mgpu_sync")
 

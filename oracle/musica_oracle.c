@@ -1231,3 +1231,48 @@ int orc_diag(const orc_cfg *c, const orc_grid *g, double *const U[5], double out
   work_free(&w);
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Maximum CFL number per direction (check_max_cfl,                          */
+/* src/Time/timestep_new.f90:660-842): over the interior, the local           */
+/* characteristic speed times deltat -- Cartesian (|u_d| + c) deltat idx_d,   */
+/* c3 (|u.grad xi_d| |1/J| ... as |(u xi_x + v xi_y + w xi_z) ijacob3| +      */
+/* c g3_xi with g3_xi = |grad xi| |ijacob3|) times deltat, and on a 2.5-D     */
+/* block the same with the 2-D metrics in i and j and Cartesian k.  c is the  */
+/* sound speed of primitives_visc, sqrt(gam rg T).                             */
+/* ------------------------------------------------------------------------ */
+int orc_max_cfl(const orc_cfg *c, const orc_grid *g, double *const U[5], double out[3]) {
+  int rc = check_cfg(c);
+  if (rc) return rc;
+  dims_t d = mkdims(c);
+  work_t w;
+  memset(&w, 0, sizeof w);
+  if (work_alloc(&d, &w, 1)) return -10;
+  prim_visc(c, &d, U, &w);
+  for (int dir = 0; dir < 3; dir++) {
+    const double *vel = dir == 0 ? w.uu : (dir == 1 ? w.vv : w.ww);
+    const double *met = dir == 0 ? g->idx : (dir == 1 ? g->idy : g->idz);
+    const int curv = c->curv == 1 || (c->curv == 2 && dir < 2);
+    double cm = 0.0;
+    for (int k = 1; k <= d.nz; k++)
+      for (int j = 1; j <= d.ny; j++)
+        for (int i = 1; i <= d.nx; i++) {
+          const long p = (long)ID(d, i, j, k);
+          double cfl;
+          if (curv) {
+            const double mx = mtr(c, g->m, g->m2, &d, 3 * dir, p), my = mtr(c, g->m, g->m2, &d, 3 * dir + 1, p),
+                         mz = mtr(c, g->m, g->m2, &d, 3 * dir + 2, p), ij = mtr(c, g->m, g->m2, &d, 9, p);
+            const double val = fabs((w.uu[p] * mx + w.vv[p] * my + w.ww[p] * mz) * ij) +
+                               w.cs[p] * (sqrt(mx * mx + my * my + mz * mz) * fabs(ij));
+            cfl = val * c->dt;
+          } else {
+            const int r = dir == 0 ? i : (dir == 1 ? j : k);
+            cfl = (fabs(vel[p]) + w.cs[p]) * c->dt * met[r - 1 + NG];
+          }
+          cm = fmax(cm, cfl);
+        }
+    out[dir] = cm;
+  }
+  work_free(&w);
+  return 0;
+}

@@ -154,6 +154,10 @@ int orc_step(const orc_cfg *c, const orc_grid *g, double *U[5], int nsteps);
  * interior (timestep_new.f90:270-288).  Returns 0 on success. */
 int orc_diag(const orc_cfg *c, const orc_grid *g, double *const U[5], double out[4]);
 
+/* Maximum CFL number along i, j, k of U over the interior (check_max_cfl,
+ * src/Time/timestep_new.f90:660-842).  Returns 0 on success. */
+int orc_max_cfl(const orc_cfg *c, const orc_grid *g, double *const U[5], double out[3]);
+
 /* Number of OpenMP threads the oracle uses, and setting it (n > 0). */
 int orc_num_threads(void);
 void orc_set_num_threads(int n);

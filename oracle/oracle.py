@@ -88,6 +88,8 @@ def lib():
         L.orc_fill_interfaces.restype = None
         L.orc_diag.argtypes = [C.POINTER(OrcCfg), C.POINTER(OrcGrid), C.POINTER(_dp), _dp]
         L.orc_diag.restype = C.c_int
+        L.orc_max_cfl.argtypes = [C.POINTER(OrcCfg), C.POINTER(OrcGrid), C.POINTER(_dp), _dp]
+        L.orc_max_cfl.restype = C.c_int
         _lib = L
     return _lib
 
@@ -190,6 +192,14 @@ class Oracle:
         rc = lib().orc_diag(C.byref(self.cfg), C.byref(self.grid), self._tab(U), _p(out))
         if rc != 0:
             raise RuntimeError(f"orc_diag failed rc={rc}")
+        return out
+
+    def max_cfl(self, U):
+        """Maximum CFL number along i, j, k over the interior (check_max_cfl)."""
+        out = np.zeros(3)
+        rc = lib().orc_max_cfl(C.byref(self.cfg), C.byref(self.grid), self._tab(U), _p(out))
+        if rc != 0:
+            raise RuntimeError(f"orc_max_cfl failed rc={rc}")
         return out
 
     def filter_rhs(self, Un, Uref=None):

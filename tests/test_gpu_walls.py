@@ -157,7 +157,32 @@ def test_step_needs_the_wall_coefficients():
         c2 = np.ascontiguousarray(c.cextrp2, dtype=np.float64)
         c3 = np.ascontiguousarray(c.cextrp3, dtype=np.float64)
         _lib.check(L.mgpu_set_wall(h, 0, _lib.dptr(c2), _lib.dptr(c3)), h, "set_wall")
-        assert L.mgpu_diagnostics(h, C.byref(d)) == _lib.EUNSUP
+        _lib.check(L.mgpu_diagnostics(h, C.byref(d)), h, "diagnostics")
         _lib.check(L.mgpu_step(h, 1), h, "step")
     finally:
         L.mgpu_destroy(h)
+
+
+@pytest.mark.parametrize("which", ["y", "x", "z"])
+def test_diagnostics_next_to_walls_match_oracle(which):
+    """mgpu_diagnostics on walled channels (one-sided velocity gradients on
+    the two rows next to each wall, grad_vel with is_boundary): the stats_tgv
+    sums and max(|u| + c) against the oracle, before and after steps; walls
+    on the x- or z-faces are relabeled blocks."""
+    from musica2_newgrid_amd.solver import GpuSolver
+    from oracle.oracle import Oracle
+    from tests.test_gpu_walls_axes import _turned
+    c = cases.channel(32, 32, 12) if which == "y" else _turned(which)
+    print(f"diagnostics, walls on the {which}-faces", flush=True)
+    with GpuSolver(c) as s:
+        d0 = s.diagnostics()
+        s.step(4)
+        d1 = s.diagnostics()
+        U1 = s.get_state()
+    o = Oracle(c)
+    for d, U in ((d0, c.U), (d1, U1)):
+        ref = o.diag(U)
+        got = np.array([d["mass"], d["kinetic"], d["enstrophy"], d["umax"]])
+        e = float(np.max(np.abs(got - ref) / np.abs(ref)))
+        print("diag relerr", e, got)
+        assert e < 1e-12
