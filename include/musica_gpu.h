@@ -89,6 +89,14 @@ extern "C" {
  * on first use (one per U^n buffer), with one batched launch per kernel for
  * all blocks; EAGER launches the same kernels one by one instead. */
 #define MGPU_LAUNCH_EAGER 1
+/* Stage-kernel family.  The stage kernel exists in two forms with the same
+ * arithmetic: the classic one (each halo cell loaded into registers in the
+ * plane that uses it) and the pipelined one (the halo of plane k+1 copied
+ * straight into LDS while plane k is computed).  By default the autotuner
+ * picks among both; these flags restrict it, and the untuned default, to
+ * one family. */
+#define MGPU_LAUNCH_STAGE_CLASSIC 2
+#define MGPU_LAUNCH_STAGE_PIPELINED 4
 
 /* Faces are numbered imin,imax,jmin,jmax,kmin,kmax = 0..5.  An interface
  * face f of block b takes its ghosts from face nbr_face[f] of block
@@ -201,6 +209,10 @@ int mgpu_autotune(mgpu_handle *h, int flags);
  * stage with viscous divergence, 2 = viscous fluxes, 3 = viscous divergence,
  * 4 = filter (tile extents and planes per workgroup). */
 int mgpu_stage_config(mgpu_handle *h, int block, int variant, int *shape_tx, int *shape_ty, int *kchunk);
+
+/* 1 if stage variant 0 (Euler stages) or 1 (the last stage) runs the
+ * pipelined stage kernel, 0 if the classic one, < 0 on error. */
+int mgpu_stage_pipelined(mgpu_handle *h, int variant);
 
 /* Wait for all work queued on the handle. */
 int mgpu_sync(mgpu_handle *h);

@@ -20,6 +20,7 @@ FILTER_NONE, FILTER_SF, FILTER_SHOCK, FILTER_ARTVISC, FILTER_ARTVISC_SHOCK = 0, 
 PREC_FP64, PREC_MIXED = 0, 1
 GEOM_CART, GEOM_C3, GEOM_C2 = 0, 1, 2
 LAUNCH_EAGER = 1
+LAUNCH_STAGE_CLASSIC, LAUNCH_STAGE_PIPELINED = 2, 4
 OK, EINVAL, EDEVICE, ENOMEM, EUNSUP, ESTATE = 0, -1, -2, -3, -4, -5
 K_STAGE, K_VISC, K_FILTER, K_HALO = 0, 1, 2, 3
 K_NAMES = {K_STAGE: "stage", K_VISC: "visc", K_FILTER: "filter", K_HALO: "halo"}
@@ -36,6 +37,7 @@ EXPORTS = (
     "mgpu_step",
     "mgpu_autotune",
     "mgpu_stage_config",
+    "mgpu_stage_pipelined",
     "mgpu_sync",
     "mgpu_profile",
     "mgpu_kernel_time",
@@ -125,6 +127,8 @@ def lib() -> C.CDLL:
     L.mgpu_step.argtypes = [C.c_void_p, C.c_int]
     L.mgpu_autotune.argtypes = [C.c_void_p, C.c_int]
     L.mgpu_stage_config.argtypes = [C.c_void_p, C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 3
+    L.mgpu_stage_pipelined.argtypes = [C.c_void_p, C.c_int]
+    L.mgpu_stage_pipelined.restype = C.c_int
     L.mgpu_sync.argtypes = [C.c_void_p]
     L.mgpu_profile.argtypes = [C.c_void_p, C.c_int]
     L.mgpu_kernel_time.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]

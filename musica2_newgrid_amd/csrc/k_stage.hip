@@ -517,6 +517,549 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
   }
 }
 
+// ---------------------------------------------------------------------------
+// Pipelined variant.  The same arithmetic as k_stage, but the raw inputs of
+// the halo cells of plane k+1 (the state and, c3, the flux-direction metrics)
+// are copied from HBM straight into LDS by global_load_lds while plane k is
+// computed, so they hold no registers and their latency is covered by a whole
+// plane of work; the own point and the queue point of plane k+1 are loaded
+// into registers after plane k's flux barrier, and U^n, 1/J and K_v of plane
+// k at the start of plane k.  Each plane's results are stored at the start of
+// the next plane, after its copies are issued, so that the wait at the top of
+// a plane does not wait for the stores.  Barriers are raw s_barrier with an
+// LDS-only wait: a __syncthreads() fence would also wait for the copies in
+// flight.
+
+// Barrier after every wave's LDS accesses (and, with VM, every memory
+// access including the LDS copies) of the workgroup have completed.
+__device__ __forceinline__ void barrier_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+__device__ __forceinline__ void barrier_all() {
+  asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// An empty use of registers, so that the compiler places its wait for their
+// loads here.
+__device__ __forceinline__ void touch(double x) { asm volatile("" ::"v"(x)); }
+template <bool C3>
+__device__ __forceinline__ void touch_ready(const double ru[5], const double rxm[3], const double rym[3],
+                                            const Raw &rq) {
+#pragma unroll
+  for (int v = 0; v < 5; v++) {
+    touch(ru[v]);
+    touch(rq.u[v]);
+  }
+  if constexpr (C3) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      touch(rxm[c]);
+      touch(rym[c]);
+      touch(rq.m[c]);
+    }
+  }
+}
+
+// Halo cells of a TX x TY tile, numbered j-halo first (10 rows of TX cells:
+// 5 below the tile, 5 above), then the i-halo (TY rows of 10 cells: 5 left,
+// 5 right).  Cell h: its LDS pencil position, its (i, j), and whether it
+// lies inside the arrays' valid region (the points some own point reads).
+template <int TX, int TY>
+struct HaloMap {
+  static constexpr int NHY = 10 * TX, NHX = 10 * TY, NH = NHY + NHX;
+  __device__ static __forceinline__ void cell(int h, int i0, int j0, const Geom &g, int &gi, int &gj, bool &ok,
+                                              bool &jdir, int &pr, int &pc) {
+    if (h < NHY) {
+      const int r = h / TX, c = h % TX;
+      const int lj = r < 5 ? r : TY + r;
+      gi = i0 + c;
+      gj = j0 - 5 + lj;
+      ok = gi <= g.nx && gj <= g.ny + 5;
+      jdir = true;
+      pr = lj;
+      pc = c;
+    } else {
+      const int hh = h - NHY;
+      const int r = hh / 10, c = hh % 10;
+      const int li = c < 5 ? c : TX + c;
+      gi = i0 - 5 + li;
+      gj = j0 + r;
+      ok = gj <= g.ny && gi <= g.nx + 5;
+      jdir = false;
+      pr = r;
+      pc = li;
+    }
+  }
+};
+
+// Halo copies of one array of EW-dword elements into LDS, one dword per lane
+// and instruction.  The LDS image is the j-halo (10 rows of TX cells, row by
+// row) followed by the i-halo (TY rows of 10 cells).  Each j-halo row takes
+// R = ceil(TX EW / 64) instructions whose global source is the row's start
+// (wave-uniform) plus the lane's dword: one lane offset per part p < R,
+// whatever the row.  The i-halo takes NIX instructions with a per-lane offset
+// each.  Offsets are computed once per kernel; a cell outside the valid
+// region reads the tile's first own point instead (always inside the
+// arrays): its LDS slot is never read.
+template <int TX, int TY, int EW>
+struct DmaPlan {
+  using HM = HaloMap<TX, TY>;
+  static constexpr int RW = TX * EW;                  // dwords per j-halo row
+  static constexpr int R = (RW + 63) / 64;            // instructions per j-halo row
+  static constexpr int NDX = HM::NHX * EW;            // i-halo dwords
+  static constexpr int NIX = (NDX + 63) / 64;         // i-halo instructions
+};
+
+template <int TX, int TY, int EW>
+struct DmaLane {
+  uint32_t jl[DmaPlan<TX, TY, EW>::R];      // j-halo: byte offset from the row's start
+  uint32_t ix[DmaPlan<TX, TY, EW>::NIX];    // i-halo: byte offset in plane k = 0
+};
+
+template <int TX, int TY, int EW>
+__device__ __forceinline__ void dma_lane(const Geom &g, int i0, int j0, int lane, DmaLane<TX, TY, EW> &L) {
+  using D = DmaPlan<TX, TY, EW>;
+  using HM = HaloMap<TX, TY>;
+  const idx_t org = lin(g, i0, j0, 0);
+#pragma unroll
+  for (int p = 0; p < D::R; p++) {
+    const int dw = 64 * p + lane;  // dword in the row
+    const int c = dw / EW;         // cell in the row
+    L.jl[p] = (dw < D::RW && i0 + c <= g.nx) ? (uint32_t)(dw * 4) : 0u;
+  }
+#pragma unroll
+  for (int t = 0; t < D::NIX; t++) {
+    const int d = 64 * t + lane;
+    const int h = HM::NHY + d / EW, part = d % EW;
+    idx_t off = org;
+    uint32_t extra = 0;
+    if (d < D::NDX) {
+      int gi, gj, pr, pc;
+      bool ok, jd;
+      HM::cell(h, i0, j0, g, gi, gj, ok, jd, pr, pc);
+      if (ok) {
+        off = lin(g, gi, gj, 0);
+        extra = (uint32_t)(part * 4);
+      }
+    }
+    L.ix[t] = (uint32_t)(off * (idx_t)(4 * EW)) + extra;
+  }
+}
+
+// Issue the copies of plane k: bj / bi are the byte bases of plane k of the
+// arrays the j-halo / i-halo read (wave-uniform), dst the LDS image.
+template <int TX, int TY, int EW>
+__device__ __forceinline__ void dma_issue(const MGPU_GLOBAL char *bj, const MGPU_GLOBAL char *bi, void *dst,
+                                          const DmaLane<TX, TY, EW> &L, const Geom &g, int i0, int j0, int lane) {
+  using D = DmaPlan<TX, TY, EW>;
+  uint32_t *ldst = reinterpret_cast<uint32_t *>(dst);
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    const int lj = r < 5 ? r : TY + r;
+    const int gj = j0 - 5 + lj;
+    if (gj > g.ny + 5) continue;  // beyond the valid region: nothing reads this row
+    const MGPU_GLOBAL char *row = bj + (uint32_t)(lin(g, i0, gj, 0) * (idx_t)(4 * EW));
+#pragma unroll
+    for (int p = 0; p < D::R; p++) {
+      lds_void_t *l = (lds_void_t *)(ldst + r * D::RW + 64 * p);
+      if (64 * p + 64 <= D::RW || 64 * p + lane < D::RW)
+        __builtin_amdgcn_global_load_lds((const MGPU_GLOBAL void *)(row + L.jl[p]), l, 4, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < D::NIX; t++) {
+    lds_void_t *l = (lds_void_t *)(ldst + 10 * D::RW + 64 * t);
+    if (64 * t + 64 <= D::NDX || 64 * t + lane < D::NDX)
+      __builtin_amdgcn_global_load_lds((const MGPU_GLOBAL void *)(bi + L.ix[t]), l, 4, 0, 0);
+  }
+}
+
+template <int TX, int TY, int GEO, bool VISC, int MODE, bool WALL>
+__global__ __launch_bounds__(TX *TY, 1) void k_stage_p(const Batch<StageArgs> *__restrict__ tab, int nb,
+                                                       int ntiles) {
+  constexpr bool CURV = GEO != kGeoCart;
+  constexpr bool C3 = GEO == kGeoC3;
+  constexpr bool C2 = GEO == kGeoC2;
+  static_assert(!WALL || !CURV, "walls: Cartesian blocks");
+  constexpr bool MIX = MODE != kFP64;
+  constexpr bool IN32 = MODE == kMixMiddle || MODE == kMixLast;
+  constexpr bool OUT32 = MODE == kMixFirst || MODE == kMixMiddle;
+  using R = typename std::conditional<MIX, float, double>::type;
+  using TU = typename std::conditional<IN32, float, double>::type;  // stored stage-input element
+  using HM = HaloMap<TX, TY>;
+  constexpr int NT = TX * TY;
+  constexpr int NW = NT / 64;
+  constexpr int NH = HM::NH;
+  constexpr int NS = (NH + NT - 1) / NT;  // halo cells per thread
+  constexpr int NM = C3 ? 3 : 0;          // metric arrays per halo cell
+  static_assert(NT % 64 == 0, "whole waves");
+  __shared__ R sfx[5][TY][TX + 10];
+  __shared__ R sfy[5][TY + 10][TX];
+  __shared__ TU hu[2][5][NH];                          // halo state, two planes
+  __shared__ double hmt[2][NM > 0 ? NM : 1][NM > 0 ? NH : 1];  // halo metrics (c3), two planes
+
+  const int w = xcd_remap(blockIdx.x, ntiles);
+  const Batch<StageArgs> &e = batch_find(tab, nb, w);
+  const StageArgs &a = e.a;
+  Ptrs P;
+  load_ptrs<GEO, VISC, IN32, OUT32>(a, P);
+  const Geom &g = a.g;
+  int tile_x, tile_y, chunk;
+  tile_coords(e.t, w, tile_x, tile_y, chunk);
+  const int kchunk = e.t.kchunk;
+
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX, tid = threadIdx.x;
+  const int wv = tid / 64, lane = tid % 64;
+  const int i0 = tile_x * TX + 1, j0 = tile_y * TY + 1;
+  const int i = i0 + tx, j = j0 + ty;
+  const int kbeg = chunk * kchunk + 1;
+  const int kend = min(kbeg + kchunk - 1, g.nz);
+  const bool own = (i <= g.nx) && (j <= g.ny);
+  const bool needx = (i <= g.nx + 5) && (j <= g.ny);
+  const bool needy = (i <= g.nx) && (j <= g.ny + 5);
+  const double gam1 = a.gas.gam1;
+  const idx_t sz = g.sy;
+
+  // the thread's halo cells: index, pencil position, direction, validity
+  int hs_h[NS], hs_r[NS], hs_c[NS];
+  bool hs_on[NS], hs_jd[NS], hs_beyond[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int h = tid + s * NT;
+    hs_h[s] = h;
+    hs_on[s] = false;
+    hs_jd[s] = false;
+    hs_r[s] = hs_c[s] = 0;
+    hs_beyond[s] = false;
+    if (h < NH) {
+      int gi, gj;
+      HM::cell(h, i0, j0, g, gi, gj, hs_on[s], hs_jd[s], hs_r[s], hs_c[s]);
+      if constexpr (WALL)
+        hs_beyond[s] = hs_jd[s] && (((g.jwall & 1) && gj < 1) || ((g.jwall & 2) && gj > g.ny));
+    }
+  }
+  const idx_t own_off = lin(g, i, j, 0);
+  // 2.5-D: metric rows of the halo cells and the own column, 1/J (constant in k)
+  double hm[NS][2];
+  double rxm[3] = {0, 0, 0}, rym[3] = {0, 0, 0};
+  double ijac = 0.0;
+  if constexpr (C2) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      hm[s][0] = hm[s][1] = 0.0;
+      if (hs_on[s]) {
+        int gi, gj, pr, pc;
+        bool ok, jd;
+        HM::cell(hs_h[s], i0, j0, g, gi, gj, ok, jd, pr, pc);
+        const idx_t p2 = lin2(g, gi, gj);
+        hm[s][0] = ldg(jd ? P.m2[2] : P.m2[0], p2);
+        hm[s][1] = ldg(jd ? P.m2[3] : P.m2[1], p2);
+      }
+    }
+    if (needx || needy) {
+      const idx_t p2 = lin2(g, i, j);
+      rxm[0] = ldg(P.m2[0], p2);
+      rxm[1] = ldg(P.m2[1], p2);
+      rym[0] = ldg(P.m2[2], p2);
+      rym[1] = ldg(P.m2[3], p2);
+      if (own) ijac = ldg(P.m2[4], p2);
+    }
+  }
+  bool own_beyond = false;
+  double cw[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  int hw = 5;
+  if constexpr (WALL) {
+    own_beyond = (g.jwall & 2) && j > g.ny;
+    hw = jhalf(g, j);
+    wall_coeffs(hw, cw);
+  }
+
+  // this wave's share of the halo copies of plane k into LDS buffer b:
+  // wave q copies array q (the 5 states, then the NM metric components; c3:
+  // the i-halo reads the xi metrics, the j-halo the eta metrics)
+  static_assert(NW >= 5 + NM, "one array per wave");
+  constexpr int EWU = IN32 ? 1 : 2;
+  DmaLane<TX, TY, EWU> dlu;
+  DmaLane<TX, TY, 2> dlm;
+  const MGPU_GLOBAL char *dbj = nullptr, *dbi = nullptr;  // plane-0 bases of this wave's arrays
+  const bool dstate = wv < 5, dmet = C3 && wv >= 5 && wv < 5 + NM;
+  if (dstate) {
+    const TU *src = nullptr;
+#pragma unroll
+    for (int q = 0; q < 5; q++)
+      if (q == wv) {
+        if constexpr (IN32) src = P.uin32[q];
+        else src = P.uin[q];
+      }
+    dbj = dbi = reinterpret_cast<const MGPU_GLOBAL char *>((const MGPU_GLOBAL TU *)src);
+    dma_lane<TX, TY, EWU>(g, i0, j0, lane, dlu);
+  } else if (dmet) {
+    if constexpr (C3) {
+#pragma unroll
+      for (int c = 0; c < NM; c++)
+        if (c == wv - 5) {
+          dbi = reinterpret_cast<const MGPU_GLOBAL char *>((const MGPU_GLOBAL double *)P.m[c]);
+          dbj = reinterpret_cast<const MGPU_GLOBAL char *>((const MGPU_GLOBAL double *)P.m[3 + c]);
+        }
+      dma_lane<TX, TY, 2>(g, i0, j0, lane, dlm);
+    }
+  }
+  auto dma = [&](int k, int b) {
+    if (dstate) {
+      const uint32_t kb = (uint32_t)((idx_t)k * sz * (idx_t)sizeof(TU));
+#pragma unroll
+      for (int q = 0; q < 5; q++)
+        if (q == wv) dma_issue<TX, TY, EWU>(dbj + kb, dbi + kb, &hu[b][q][0], dlu, g, i0, j0, lane);
+    } else if (dmet) {
+      if constexpr (C3) {
+        const uint32_t kb = (uint32_t)((idx_t)k * sz * (idx_t)sizeof(double));
+#pragma unroll
+        for (int c = 0; c < NM; c++)
+          if (c == wv - 5) dma_issue<TX, TY, 2>(dbj + kb, dbi + kb, &hmt[b][c][0], dlm, g, i0, j0, lane);
+      }
+    }
+  };
+
+  // the mixed path's reference state, read once (not in every plane)
+  const RefState rf = a.ref;
+  const double alpha = a.alpha;
+  R q[11][5];
+  Raw rq;
+  if (own) {
+#pragma unroll
+    for (int l = 0; l < 10; l++) {
+      Raw r;
+      load_raw<C3, IN32>(P, own_off + (idx_t)(kbeg - 5 + l) * sz, 2, r);
+      flux_r<C3, MIX>(prim_raw(r.u, gam1), 2, r.m, rf, q[l]);
+    }
+    load_raw<C3, IN32>(P, own_off + (idx_t)(kbeg + 5) * sz, 2, rq);
+  }
+  double ru[5] = {0, 0, 0, 0, 0};
+  auto load_own = [&](idx_t p) {
+    load_u<IN32>(P, p, ru);
+    if constexpr (C3) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        rxm[c] = ldg(P.m[c], p);
+        rym[c] = ldg(P.m[3 + c], p);
+      }
+    }
+  };
+  if (needx || needy) load_own(own_off + (idx_t)kbeg * sz);
+  // U^n, 1/J (c3) and K_v of the own point, one plane ahead like the rest
+  // (Cartesian and 2.5-D: idz of the plane; the own column's idx and idy
+  // are constant along the march)
+  double u0n[5] = {0, 0, 0, 0, 0}, kvisn[4] = {0, 0, 0, 0}, ijacn = 0.0, idzn = 0.0;
+  double idxi = 0.0, idyj = 0.0;
+  if constexpr (GEO == kGeoCart) {
+    if (own) {
+      idxi = ldg(P.idx, (idx_t)(i - 1 + G));
+      idyj = ldg(P.idy, (idx_t)(j - 1 + G));
+    }
+  }
+  auto load_u0 = [&](idx_t p, int kk) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) u0n[v] = ldg_nt(P.u0[v], p);
+    if constexpr (C3) ijacn = ldg_nt(P.m[9], p);
+    else idzn = ldg(P.idz, (idx_t)(kk - 1 + G));
+    if constexpr (VISC) {
+#pragma unroll
+      for (int c = 0; c < 4; c++) kvisn[c] = ldg_nt(P.kv[c], p);
+    }
+  };
+  if (own) load_u0(own_off + (idx_t)kbeg * sz, kbeg);
+  dma(kbeg, 0);
+
+  // the previous plane's result, stored at the start of the next plane
+  double outv[5] = {0, 0, 0, 0, 0};
+  idx_t out_p = 0;
+  bool out_pending = false;
+  auto store_out = [&]() {
+    if constexpr (OUT32) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) stg_nt(P.uout32[v], out_p, (float)outv[v]);
+    } else {
+#pragma unroll
+      for (int v = 0; v < 5; v++) stg_nt(P.uout[v], out_p, outv[v]);
+    }
+  };
+
+  for (int k = kbeg; k <= kend; k++) {
+    const int b = (k - kbeg) & 1;
+    const idx_t pk = own_off + (idx_t)k * sz;
+    // plane k's halo copies and own-point loads have landed in every wave;
+    // the pencils are free (every wave is past plane k-1's derivatives)
+    barrier_all();
+    // the own and queue point's registers loaded during plane k-1 are ready
+    // (waited for above); say so to the compiler here, before the copies are
+    // issued -- otherwise its wait before their first use also waits for the
+    // copies just issued
+    touch_ready<C3>(ru, rxm, rym, rq);
+    double u0[5], kvis[4];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      touch(u0n[v]);
+      u0[v] = u0n[v];
+    }
+    double idzk = 0.0;
+    if constexpr (C3) {
+      touch(ijacn);
+      ijac = ijacn;
+    } else {
+      touch(idzn);
+      idzk = idzn;
+    }
+    if constexpr (GEO == kGeoCart) {
+      touch(idxi);
+      touch(idyj);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      if constexpr (VISC) touch(kvisn[c]);
+      kvis[c] = kvisn[c];
+    }
+    if (k < kend) dma(k + 1, b ^ 1);
+    if (out_pending) store_out();
+    out_pending = false;
+    if (own) flux_r<C3, MIX>(prim_raw(rq.u, gam1), 2, rq.m, rf, q[10]);
+    if (needx || needy) {
+      const Prim pq = prim_raw(ru, gam1);
+      R F[5];
+      if (needx) {
+        flux_r<CURV, MIX>(pq, 0, rxm, rf, F);
+#pragma unroll
+        for (int v = 0; v < 5; v++) sfx[v][ty][tx + 5] = F[v];
+      }
+      if (needy) {
+        flux_r<CURV, MIX>(pq, 1, rym, rf, F);
+#pragma unroll
+        for (int v = 0; v < 5; v++) sfy[v][ty + 5][tx] = (WALL && own_beyond) ? R(0) : F[v];
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      if (hs_on[s]) {
+        const int h = hs_h[s];
+        Raw r;
+        if constexpr (IN32) {
+#pragma unroll
+          for (int v = 0; v < 5; v++) r.u[v] = P.ref[v] + (double)hu[b][v][h];
+        } else {
+#pragma unroll
+          for (int v = 0; v < 5; v++) r.u[v] = hu[b][v][h];
+        }
+        if constexpr (C3) {
+#pragma unroll
+          for (int c = 0; c < 3; c++) r.m[c] = hmt[b][c < NM ? c : 0][h];
+        } else if constexpr (C2) {
+          r.m[0] = hm[s][0];
+          r.m[1] = hm[s][1];
+          r.m[2] = 0.0;
+        }
+        R F[5];
+        flux_halo<CURV, MIX>(prim_raw(r.u, gam1), hs_jd[s] ? 1 : 0, r.m, rf, F);
+        if (!hs_jd[s]) {
+#pragma unroll
+          for (int v = 0; v < 5; v++) sfx[v][hs_r[s]][hs_c[s]] = F[v];
+        } else {
+#pragma unroll
+          for (int v = 0; v < 5; v++) sfy[v][hs_r[s]][hs_c[s]] = (WALL && hs_beyond[s]) ? R(0) : F[v];
+        }
+      }
+    }
+    barrier_lds();
+    // own point and queue point of plane k+1, in flight during the derivatives
+    if (own && k < kend) {
+      load_raw<C3, IN32>(P, pk + 6 * sz, 2, rq);
+      load_u0(pk + sz, k + 1);
+    }
+    if ((needx || needy) && k < kend) load_own(pk + sz);
+
+    if (own) {
+      double K[5];
+#pragma unroll
+      for (int v = 0; v < 5; v++) {
+        const R dx = d11c(&sfx[v][ty][tx + 5], 1);
+        R dy;
+        if constexpr (WALL) {
+          const R *f = &sfy[v][ty + 5][tx];
+          dy = R(cw[0]) * (f[TX] - f[-TX]) + R(cw[1]) * (f[2 * TX] - f[-2 * TX]) + R(cw[2]) * (f[3 * TX] - f[-3 * TX]) +
+               R(cw[3]) * (f[4 * TX] - f[-4 * TX]) + R(cw[4]) * (f[5 * TX] - f[-5 * TX]);
+        } else {
+          dy = d11c(&sfy[v][ty + 5][tx], TX);
+        }
+        const R dz = R(A11_1) * (q[6][v] - q[4][v]) + R(A11_2) * (q[7][v] - q[3][v]) +
+                     R(A11_3) * (q[8][v] - q[2][v]) + R(A11_4) * (q[9][v] - q[1][v]) +
+                     R(A11_5) * (q[10][v] - q[0][v]);
+        if constexpr (C3) {
+          K[v] = (double)(dz + (dy + dx)) * ijac;
+        } else if constexpr (C2) {
+          K[v] = (double)dz * idzk + (double)(dy + dx) * ijac;
+        } else {
+          const double kx = (double)dx * idxi;
+          const double ky = (double)dy * idyj + kx;
+          K[v] = (double)dz * idzk + ky;
+        }
+        if (WALL && hw == 0) K[v] = 0.0;
+      }
+      if constexpr (VISC) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) K[1 + c] = K[1 + c] + kvis[c];
+      }
+      if constexpr (OUT32) {
+        const double ref[5] = {rf.r, rf.ru, rf.rv, rf.rw, rf.re};
+#pragma unroll
+        for (int v = 0; v < 5; v++) outv[v] = (u0[v] - alpha * K[v]) - ref[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < 5; v++) outv[v] = u0[v] - alpha * K[v];
+      }
+      out_p = pk;
+      out_pending = true;
+    }
+
+#pragma unroll
+    for (int l = 0; l < 10; l++)
+#pragma unroll
+      for (int v = 0; v < 5; v++) q[l][v] = q[l + 1][v];
+  }
+  if (out_pending) store_out();
+}
+
+template <int TX, int TY, int GEO, int MODE>
+hipError_t launch_gp(const Batch<StageArgs> *tab, int nb, int nt, bool visc, hipStream_t s) {
+  dim3 blk(TX * TY), grd(nt);
+  if constexpr (MODE == kFP64 || MODE == kMixLast) {
+    if (visc) {
+      hipLaunchKernelGGL((k_stage_p<TX, TY, GEO, true, MODE, false>), grd, blk, 0, s, tab, nb, nt);
+      return hipGetLastError();
+    }
+  }
+  hipLaunchKernelGGL((k_stage_p<TX, TY, GEO, false, MODE, false>), grd, blk, 0, s, tab, nb, nt);
+  return hipGetLastError();
+}
+
+template <int TX, int TY, int MODE>
+hipError_t launch_tp(const Batch<StageArgs> *tab, int nb, int nt, int geo, bool visc, bool wall, hipStream_t s) {
+  dim3 blk(TX * TY), grd(nt);
+  constexpr bool VOK = MODE == kFP64 || MODE == kMixLast;
+  if (visc && !VOK) return hipErrorInvalidValue;
+  if (wall) {
+    if (geo != kGeoCart) return hipErrorInvalidValue;
+    if constexpr (VOK) {
+      if (visc) {
+        hipLaunchKernelGGL((k_stage_p<TX, TY, kGeoCart, true, MODE, true>), grd, blk, 0, s, tab, nb, nt);
+        return hipGetLastError();
+      }
+    }
+    hipLaunchKernelGGL((k_stage_p<TX, TY, kGeoCart, false, MODE, true>), grd, blk, 0, s, tab, nb, nt);
+    return hipGetLastError();
+  }
+  if (geo == kGeoC3) return launch_gp<TX, TY, kGeoC3, MODE>(tab, nb, nt, visc, s);
+  if (geo == kGeoC2) return launch_gp<TX, TY, kGeoC2, MODE>(tab, nb, nt, visc, s);
+  return launch_gp<TX, TY, kGeoCart, MODE>(tab, nb, nt, visc, s);
+}
+
 template <int TX, int TY, int GEO, int MODE>
 hipError_t launch_g(const Batch<StageArgs> *tab, int nb, int nt, bool visc, hipStream_t s) {
   dim3 blk(TX * TY), grd(nt);
@@ -560,6 +1103,8 @@ hipError_t launch_shape(const Batch<StageArgs> *tab, int nb, int nt, int geo, bo
     case 2: return launch_t<64, 8, MODE>(tab, nb, nt, geo, visc, wall, s);
     case 3: return launch_t<16, 16, MODE>(tab, nb, nt, geo, visc, wall, s);
     case 4: return launch_t<32, 16, MODE>(tab, nb, nt, geo, visc, wall, s);
+    case 5: return launch_tp<64, 8, MODE>(tab, nb, nt, geo, visc, wall, s);
+    case 6: return launch_tp<32, 16, MODE>(tab, nb, nt, geo, visc, wall, s);
     default: return launch_t<64, 4, MODE>(tab, nb, nt, geo, visc, wall, s);
   }
 }
@@ -567,8 +1112,10 @@ hipError_t launch_shape(const Batch<StageArgs> *tab, int nb, int nt, int geo, bo
 }  // namespace
 
 // Tile shapes the autotuner may choose from.
-const int kStageShapes[][2] = {{64, 4}, {32, 8}, {64, 8}, {16, 16}, {32, 16}};
-const int kNumStageShapes = 5;
+// Shapes 5 and 6 are the pipelined kernel (halo copies straight into LDS).
+const int kStageShapes[][2] = {{64, 4}, {32, 8}, {64, 8}, {16, 16}, {32, 16}, {64, 8}, {32, 16}};
+const int kNumStageShapes = 7;
+const int kFirstPipelinedShape = 5;
 
 TileRange stage_tiles(const Geom &g, int shape, int kchunk) {
   const int TX = kStageShapes[shape][0], TY = kStageShapes[shape][1];

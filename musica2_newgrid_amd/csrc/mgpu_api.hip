@@ -383,6 +383,72 @@ struct Aff {
   long c0, cl, cp, cq;
 };
 
+// Ghost point (l, p, q) of face f (layer l from the face, p and q along the
+// face's first and second tangential axes, all from 0) as block coordinates
+// d[axis] of the destination block, and the neighbour point it copies as
+// coordinates sr[axis] of the neighbour (face nf, orientation o), each an
+// affine function c0 + cl l + cp p + cq q.  na / nbx: the two blocks' extents.
+void face_affine(int f, int nf, int o, const int na[3], const int nbx[3], Aff d[3], Aff sr[3]) {
+  int a, t1, t2, bb, u1, u2;
+  face_axes(f, a, t1, t2);
+  face_axes(nf, bb, u1, u2);
+  d[a] = (f & 1) ? Aff{na[a] + 1, 1, 0, 0} : Aff{0, -1, 0, 0};
+  d[t1] = Aff{1, 0, 1, 0};
+  d[t2] = Aff{1, 0, 0, 1};
+  sr[bb] = (nf & 1) ? Aff{nbx[bb], -1, 0, 0} : Aff{1, 1, 0, 0};
+  // neighbour tangential coordinates: s1 = swap ? q : p, s2 = swap ? p : q
+  Aff s1 = (o & 1) ? Aff{0, 0, 0, 1} : Aff{0, 0, 1, 0};
+  Aff s2 = (o & 1) ? Aff{0, 0, 1, 0} : Aff{0, 0, 0, 1};
+  sr[u1] = (o & 2) ? Aff{nbx[u1] - s1.c0, -s1.cl, -s1.cp, -s1.cq} : Aff{1 + s1.c0, s1.cl, s1.cp, s1.cq};
+  sr[u2] = (o & 4) ? Aff{nbx[u2] - s2.c0, -s2.cl, -s2.cp, -s2.cq} : Aff{1 + s2.c0, s2.cl, s2.cp, s2.cq};
+}
+
+// Interfaces must join wall rows to wall rows.  On the row of a no-slip wall
+// the viscous kernels exchange tau/mu and the heat-flux gradient, and every
+// receiving point of that row rebuilds its flux from its own primitives
+// (the reference evaluates the flux at a face ghost from the ghost's own
+// primitives, which hold the neighbour's state from before the wall state was
+// imposed).  A wall row facing a row that is not a wall row on the other
+// side (a wall that ends at the interface, the neighbour being periodic
+// there) would mix the two forms, so such a table is refused.  c: the blocks
+// as stored (walls on j-faces only).
+std::string check_wall_rows(const mgpu_case &c) {
+  char buf[320];
+  for (int b = 0; b < c.nblocks; b++) {
+    const mgpu_block &A = c.blocks[b];
+    const int na[3] = {A.nx, A.ny, A.nz};
+    const int jwa = (A.face_kind[2] == MGPU_FACE_WALL ? 1 : 0) | (A.face_kind[3] == MGPU_FACE_WALL ? 2 : 0);
+    for (int f = 0; f < 6; f++) {
+      if (A.face_kind[f] == MGPU_FACE_WALL) continue;
+      int nb, nf, o;
+      face_nbr(A, b, f, nb, nf, o);
+      const mgpu_block &B = c.blocks[nb];
+      const int nbx[3] = {B.nx, B.ny, B.nz};
+      const int jwb = (B.face_kind[2] == MGPU_FACE_WALL ? 1 : 0) | (B.face_kind[3] == MGPU_FACE_WALL ? 2 : 0);
+      if (!jwa && !jwb) continue;
+      int a, t1, t2;
+      face_axes(f, a, t1, t2);
+      Aff d[3] = {}, sr[3] = {};
+      face_affine(f, nf, o, na, nbx, d, sr);
+      for (int p = 0; p < na[t1]; p++)
+        for (int q = 0; q < na[t2]; q++) {
+          const long dj = d[1].c0 + d[1].cp * p + d[1].cq * q;  // layer 0
+          const long sj = sr[1].c0 + sr[1].cp * p + sr[1].cq * q;
+          const bool dw = ((jwa & 1) && dj == 1) || ((jwa & 2) && dj == na[1]);
+          const bool sw = ((jwb & 1) && sj == 1) || ((jwb & 2) && sj == nbx[1]);
+          if (dw != sw) {
+            snprintf(buf, sizeof buf,
+                     "block %d face %d: a no-slip wall row of block %d meets a row of block %d that is not a wall "
+                     "row (walls must continue across an interface; not supported)",
+                     b, f, dw ? b : nb, dw ? nb : b);
+            return buf;
+          }
+        }
+    }
+  }
+  return std::string();
+}
+
 // Every element a task touches lies inside the padded arrays: the offsets
 // are affine in the box indices, so their extremes sit at the box corners.
 bool task_in_bounds(const FaceTask &t, long ndst, long nsrc) {
@@ -462,15 +528,7 @@ bool build_tasks(const mgpu_case &c, const std::vector<BlockData> &bds, int ng, 
       face_axes(f, a, t1, t2);
       face_axes(nf, bb, u1, u2);
       Aff d[3] = {}, sr[3] = {};
-      d[a] = (f & 1) ? Aff{na[a] + 1, 1, 0, 0} : Aff{0, -1, 0, 0};
-      d[t1] = Aff{1, 0, 1, 0};
-      d[t2] = Aff{1, 0, 0, 1};
-      sr[bb] = (nf & 1) ? Aff{nbx[bb], -1, 0, 0} : Aff{1, 1, 0, 0};
-      // neighbour tangential coordinates: s1 = swap ? q : p, s2 = swap ? p : q
-      Aff s1 = (o & 1) ? Aff{0, 0, 0, 1} : Aff{0, 0, 1, 0};
-      Aff s2 = (o & 1) ? Aff{0, 0, 1, 0} : Aff{0, 0, 0, 1};
-      sr[u1] = (o & 2) ? Aff{nbx[u1] - s1.c0, -s1.cl, -s1.cp, -s1.cq} : Aff{1 + s1.c0, s1.cl, s1.cp, s1.cq};
-      sr[u2] = (o & 4) ? Aff{nbx[u2] - s2.c0, -s2.cl, -s2.cp, -s2.cq} : Aff{1 + s2.c0, s2.cl, s2.cp, s2.cq};
+      face_affine(f, nf, o, na, nbx, d, sr);
       const long sta[3] = {1, ga.sx, (long)ga.sy}, stb[3] = {1, gb.sx, (long)gb.sy};
       long db = 0, sb = 0, dv[3] = {0, 0, 0}, sv[3] = {0, 0, 0};
       for (int x = 0; x < 3; x++) {
@@ -628,10 +686,10 @@ int kchunk_for(const Geom &g, int shape) {
 // Process-wide cache of tuned stage parameters, keyed by block geometry and
 // kernel variant, so that equal blocks (and later handles) are tuned once.
 struct TuneKey {
-  int dev, curv, visc;
+  int dev, curv, visc, family;
   std::vector<int> dims;  // nx,ny,nz of every block of the batch
   bool operator<(const TuneKey &o) const {
-    return std::tie(dev, curv, visc, dims) < std::tie(o.dev, o.curv, o.visc, o.dims);
+    return std::tie(dev, curv, visc, family, dims) < std::tie(o.dev, o.curv, o.visc, o.family, o.dims);
   }
 };
 std::mutex g_tune_mu;
@@ -1065,6 +1123,8 @@ std::string check_flags(const mgpu_case &c) {
     snprintf(buf, sizeof buf, "precision=%d", c.precision);
   else if (c.precision == MGPU_PREC_MIXED && c.nrk < 2)
     snprintf(buf, sizeof buf, "mixed precision needs nrk >= 2");
+  else if ((c.launch_flags & MGPU_LAUNCH_STAGE_CLASSIC) && (c.launch_flags & MGPU_LAUNCH_STAGE_PIPELINED))
+    snprintf(buf, sizeof buf, "launch_flags: MGPU_LAUNCH_STAGE_CLASSIC and MGPU_LAUNCH_STAGE_PIPELINED together");
   return std::string(buf);
 }
 
@@ -1075,6 +1135,9 @@ int handle_new(const mgpu_case &c, mgpu_handle **out) {
   h->c.nblocks = 0;
   h->c.blocks = nullptr;
   h->use_graph = (c.launch_flags & MGPU_LAUNCH_EAGER) == 0;
+  h->stage_family = (c.launch_flags & MGPU_LAUNCH_STAGE_PIPELINED) ? 2
+                    : ((c.launch_flags & MGPU_LAUNCH_STAGE_CLASSIC) ? 1 : 0);
+  if (h->stage_family == 2) h->shape[0] = h->shape[1] = kFirstPipelinedShape;
   h->mixed = c.precision == MGPU_PREC_MIXED;
   if (hipGetDevice(&h->device) != hipSuccess) {
     delete h;
@@ -1575,6 +1638,7 @@ int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
     mgpu_case cs = *c;
     cs.blocks = stored.data();
     if (msg.empty()) msg = check_topology(cs);
+    if (msg.empty()) msg = check_wall_rows(cs);
     if (!msg.empty()) return fail(nullptr, MGPU_EINVAL, "%s", msg.c_str());
   }
   mgpu_handle *h = nullptr;
@@ -1670,7 +1734,7 @@ int mgpu_autotune(mgpu_handle *h, int flags) {
   hipEvent_t e0 = get_event(h), e1 = get_event(h);
   if (!e0 || !e1) return fail(h, MGPU_EDEVICE, "hipEventCreate failed");
   for (int v = 0; v < (h->c.iorder_visc ? 2 : 1) && !rc; v++) {
-    TuneKey key{h->device, h->c.curvilinear, v + (h->mixed ? 2 : 0), {}};
+    TuneKey key{h->device, h->c.curvilinear, v + (h->mixed ? 2 : 0), h->stage_family, {}};
     for (const auto &bd : h->b) key.dims.insert(key.dims.end(), {bd.g.nx, bd.g.ny, bd.g.nz});
     if (!force) {
       std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1693,6 +1757,8 @@ int mgpu_autotune(mgpu_handle *h, int flags) {
     std::vector<Batch<StageArgs>> tab(nb);
     for (int sj = 0; sj < 2 * kNumStageShapes && !rc; sj++) {
       const int sh = sj >> 1, jf = sj & 1;
+      if ((h->stage_family == 1 && sh >= kFirstPipelinedShape) || (h->stage_family == 2 && sh < kFirstPipelinedShape))
+        continue;
       for (int kc : kcs) {
         if (kc > 16 && kc / 2 >= maxnz) break;  // same as the previous candidate for every block
         h->shape[v] = sh;
@@ -1763,6 +1829,12 @@ int mgpu_stage_config(mgpu_handle *h, int block, int variant, int *shape_tx, int
     *kchunk = stage_tiles(bd.g, h->shape[variant], kc).kchunk;
   }
   return 0;
+}
+
+int mgpu_stage_pipelined(mgpu_handle *h, int variant) {
+  if (!h) return MGPU_EINVAL;
+  if (variant < 0 || variant > 1) return fail(h, MGPU_EINVAL, "variant %d: 0 Euler stages, 1 last stage", variant);
+  return h->shape[variant] >= kFirstPipelinedShape ? 1 : 0;
 }
 
 int mgpu_sync(mgpu_handle *h) {

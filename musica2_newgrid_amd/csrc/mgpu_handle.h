@@ -74,6 +74,7 @@ struct mgpu_handle {
   // viscous divergence (chosen by mgpu_autotune, else a default); kchunk 0
   // means the per-block default
   int shape[2] = {2, 2};
+  int stage_family = 0;  // 0: the tuner may pick either stage kernel, 1: classic only, 2: pipelined only
   int kchunk[2] = {0, 0};
   // planes per workgroup of k_visc, k_vdiv and the filter (0: per-block default)
   int kc_aux[3] = {0, 0, 0};

@@ -378,6 +378,7 @@ struct CflArgs {
 namespace mgpu {
 extern const int kStageShapes[][2];
 extern const int kNumStageShapes;
+extern const int kFirstPipelinedShape;  // shapes from here on: the pipelined stage kernel
 TileRange stage_tiles(const Geom &g, int shape, int kchunk);
 hipError_t launch_stage(const Batch<StageArgs> *tab, int nb, int ntiles, int geo, bool visc, int shape,
                         int mode, bool wall, hipStream_t s);

@@ -68,14 +68,17 @@ class GpuSolver:
     block_cases[b] the extents, metrics and initial state of block b."""
 
     def __init__(self, case, precision: int = _lib.PREC_FP64, topology=None, block_cases=None,
-                 eager: bool = False):
+                 eager: bool = False, stage_kernel: str = None):
+        """stage_kernel: None (the autotuner picks), "classic" or "pipelined"
+        (the stage kernel family, MGPU_LAUNCH_STAGE_*)."""
         L = lib()
         self.case = case
         self.block_cases = [case] if block_cases is None else list(block_cases)
         if topology is not None and len(topology) != len(self.block_cases):
             raise ValueError("topology and block_cases differ in length")
-        self._cs, self._blocks = make_case_struct(case, precision, topology,
-                                                  _lib.LAUNCH_EAGER if eager else 0)
+        flags = _lib.LAUNCH_EAGER if eager else 0
+        flags |= {None: 0, "classic": _lib.LAUNCH_STAGE_CLASSIC, "pipelined": _lib.LAUNCH_STAGE_PIPELINED}[stage_kernel]
+        self._cs, self._blocks = make_case_struct(case, precision, topology, flags)
         h = C.c_void_p()
         rc = L.mgpu_create(C.byref(self._cs), C.byref(h))
         check(rc, None, "mgpu_create")
@@ -143,6 +146,12 @@ class GpuSolver:
         check(lib().mgpu_stage_config(self.h, block, variant, C.byref(tx), C.byref(ty), C.byref(kc)), self.h,
               "mgpu_stage_config")
         return tx.value, ty.value, kc.value
+
+    def stage_pipelined(self, variant: int = 0) -> bool:
+        """Whether stage variant 0 (Euler stages) / 1 (last stage) runs the pipelined kernel."""
+        r = lib().mgpu_stage_pipelined(self.h, variant)
+        check(min(r, 0), self.h, "mgpu_stage_pipelined")
+        return bool(r)
 
     def diagnostics(self) -> dict:
         """Sums of rho, 1/2 rho|u|^2, 1/2 rho|curl u|^2 over all interior cells

@@ -886,11 +886,14 @@ static void bc_wall(const orc_cfg *c, const orc_grid *g, const dims_t *d, double
 /* ------------------------------------------------------------------------ */
 /* Public entry points                                                       */
 /* ------------------------------------------------------------------------ */
-static int check_cfg(const orc_cfg *c) {
+/* walls_ok: the routine reads the state one block at a time with its ghosts
+ * as given (diagnostics, CFL): wall faces are allowed; the time step itself
+ * runs walled blocks through orc_step_multi. */
+static int check_cfg_faces(const orc_cfg *c, int walls_ok) {
   if (c->nx < NG || c->ny < NG || c->nz < NG) return -1;
   for (int dir = 0; dir < 3; dir++)
     for (int s = 0; s < 2; s++)
-      if (c->face[dir][s] != ORC_FACE_PERIODIC) return -2; /* only periodic faces so far */
+      if (c->face[dir][s] != ORC_FACE_PERIODIC && !(walls_ok && c->face[dir][s] == ORC_FACE_WALL)) return -2;
   if (c->filter >= 2 && !c->visc) return -3;              /* sensor / sound speed of primitives_visc */
   if (c->filter == 3 && c->curv == 1) return -8;          /* artvisc_o9: Cartesian or 2.5-D */
   if (c->filter < 0 || c->filter > 4) return -11;
@@ -898,6 +901,8 @@ static int check_cfg(const orc_cfg *c) {
   if (c->nrk < 1 || c->nrk > 8) return -4;
   return 0;
 }
+
+static int check_cfg(const orc_cfg *c) { return check_cfg_faces(c, 0); }
 
 static int work_alloc(const dims_t *d, work_t *w, int visc) {
   memset(w, 0, sizeof(*w));
@@ -1199,7 +1204,7 @@ out:
 /* they are those of U itself, whose ghosts must be valid.                  */
 /* ------------------------------------------------------------------------ */
 int orc_diag(const orc_cfg *c, const orc_grid *g, double *const U[5], double out[4]) {
-  int rc = check_cfg(c);
+  int rc = check_cfg_faces(c, 1);
   if (rc) return rc;
   dims_t d = mkdims(c);
   work_t w;
@@ -1242,7 +1247,7 @@ int orc_diag(const orc_cfg *c, const orc_grid *g, double *const U[5], double out
 /* sound speed of primitives_visc, sqrt(gam rg T).                             */
 /* ------------------------------------------------------------------------ */
 int orc_max_cfl(const orc_cfg *c, const orc_grid *g, double *const U[5], double out[3]) {
-  int rc = check_cfg(c);
+  int rc = check_cfg_faces(c, 1);
   if (rc) return rc;
   dims_t d = mkdims(c);
   work_t w;

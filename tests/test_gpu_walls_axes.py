@@ -5,7 +5,8 @@ z-faces, with the shock filter and Ducros sensor (FP64 within 1e-12, mixed
 density error), the GPU result of the turned channel equal to the turned
 GPU result of the original (round-off), and a z-walled channel cut in two
 blocks along x (interfaces on faces whose tangential axes the relabeling
-reverses)."""
+reverses).  A block walled on axes its neighbour is not walled on is refused
+at create (tests/test_topology.py)."""
 import numpy as np
 import pytest
 
@@ -161,30 +162,3 @@ def _cut_mixed(t, axis, wall_axis):
                            nbr_face=[1, 0, 3, 2, 5, 4], orient=[0] * 6,
                            wall=[int(k == _lib.FACE_WALL) for k in kind]))
     return blocks, bcs
-
-
-@pytest.mark.parametrize("which", ["z_cut_x", "x_cut_y"])
-def test_relabeled_block_joined_to_a_block_stored_as_given(which):
-    """Interfaces between a block stored with exchanged axes (walls on its k-
-    or i-faces) and one stored as given: the momentum, stress and sensor
-    components cross in the neighbour's axis order.  FP64 against the oracle
-    within 1e-12 after 5 steps; mixed precision (one reference state for both
-    blocks, each in its own axis order) within 1e-6 in density after 20."""
-    from oracle import oracle as orc
-    t = _turned("z" if which == "z_cut_x" else "x")
-    axis, wall_axis = (0, 2) if which == "z_cut_x" else (1, 0)
-    blocks, bcs = _cut_mixed(t, axis, wall_axis)
-    print(f"{which}: {t.nx}x{t.ny}x{t.nz} in 2 blocks, walls on axis {wall_axis} of block 0 only", flush=True)
-    Ug = _gpu(t, 5, topology=blocks, block_cases=bcs)
-    Uc = [bc.copy_state() for bc in bcs]
-    orc.step_multi(t, blocks, bcs, Uc, 5)
-    e = max(relerr(a, b) for a, b in zip(Ug, Uc))
-    print(f"FP64 relerr {e:.3e}", flush=True)
-    assert e < 1e-12
-    Um = _gpu(t, 20, precision=_lib.PREC_MIXED, topology=blocks, block_cases=bcs)
-    Uc = [bc.copy_state() for bc in bcs]
-    orc.step_multi(t, blocks, bcs, Uc, 20)
-    linf = max(float(np.max(np.abs(gr.interior(a[0]) - gr.interior(b[0])))) / float(np.max(np.abs(gr.interior(b[0]))))
-               for a, b in zip(Um, Uc))
-    print(f"mixed rho Linf {linf:.3e} after 20 steps", flush=True)
-    assert linf <= 1e-6

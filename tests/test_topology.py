@@ -133,3 +133,42 @@ def test_artvisc_shock_flags_are_checked():
     c = cases.tgv(16, filter=5)
     rc, msg = _create_case(c, None)
     assert rc == _lib.EINVAL and "filter=5" in msg
+
+
+@pytest.mark.parametrize("which", ["z_cut_x", "x_cut_y", "y_cut_x"])
+def test_wall_row_meeting_a_row_that_is_no_wall_is_refused(which):
+    """A channel cut in two blocks where only block 0 keeps the walls (block
+    1 is periodic on that axis): the wall rows of block 0 face ordinary rows
+    of block 1 across the interface.  The viscous exchange carries tau/mu on
+    wall rows, so mgpu_create refuses the table with a message (on the walls
+    of the k-, i- and j-faces: relabeled or not)."""
+    from tests.test_gpu_walls_axes import _channel, _cut_mixed, _turned
+    if which == "y_cut_x":
+        t, axis, wall_axis = _channel(), 0, 1
+    else:
+        t = _turned("z" if which == "z_cut_x" else "x")
+        axis, wall_axis = (0, 2) if which == "z_cut_x" else (1, 0)
+    blocks, bcs = _cut_mixed(t, axis, wall_axis)
+    cs, keep = make_case_struct(t, topology=blocks)
+    h = C.c_void_p()
+    rc = _lib.lib().mgpu_create(C.byref(cs), C.byref(h))
+    msg = _lib.lib().mgpu_last_error(None).decode()
+    assert rc == _lib.EINVAL, rc
+    assert "wall row" in msg, msg
+
+
+def test_walls_continuing_across_an_interface_are_accepted_by_the_check():
+    """The same cut with the walls kept on both blocks passes the wall-row
+    check (the create may still need a device after it)."""
+    from tests.test_gpu_walls_axes import _channel, _cut_mixed
+    t = _channel()
+    blocks, bcs = _cut_mixed(t, 0, 1)
+    blocks[1]["kind"][2] = blocks[1]["kind"][3] = _lib.FACE_WALL
+    blocks[1]["wall"] = [int(k == _lib.FACE_WALL) for k in blocks[1]["kind"]]
+    cs, keep = make_case_struct(t, topology=blocks)
+    h = C.c_void_p()
+    rc = _lib.lib().mgpu_create(C.byref(cs), C.byref(h))
+    msg = _lib.lib().mgpu_last_error(None).decode()
+    if rc == 0:
+        _lib.lib().mgpu_destroy(h)
+    assert "wall row" not in msg
