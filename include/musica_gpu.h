@@ -249,6 +249,20 @@ typedef struct mgpu_diag {
 } mgpu_diag;
 int mgpu_diagnostics(mgpu_handle *h, mgpu_diag *out);
 
+/* mgpu_diagnostics without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The per-workgroup partial results and their totals go to a
+ * static device array of the library (4 MiB per GPU, held until the process
+ * ends), so a block may hold at most 2^17 * 256 = 33 554 432 interior cells
+ * (MGPU_EINVAL beyond).  It needs the handle's exchange tables, which
+ * mgpu_create builds (MGPU_ESTATE if they are missing).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (other handles on other threads or streams) race and must be serialised
+ * by the caller.  Synchronous; the same results as mgpu_diagnostics up to
+ * the order of the floating-point additions. */
+int mgpu_diagnostics_noalloc(mgpu_handle *h, mgpu_diag *out);
+
 /* Maximum CFL number along i, j and k over the interior cells of all blocks
  * of the current state U^n, as check_max_cfl prints it
  * (src/Time/timestep_new.f90:660-842): Cartesian (|u_d| + c) deltat idx_d;

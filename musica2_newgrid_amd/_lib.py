@@ -43,6 +43,7 @@ EXPORTS = (
     "mgpu_kernel_time",
     "mgpu_last_error",
     "mgpu_diagnostics",
+    "mgpu_diagnostics_noalloc",
     "mgpu_max_cfl",
     "mgpu_destroy",
     "mgpu_exchange_plan",
@@ -138,6 +139,8 @@ def lib() -> C.CDLL:
     L.mgpu_destroy.restype = None
     L.mgpu_diagnostics.argtypes = [C.c_void_p, C.POINTER(MgpuDiag)]
     L.mgpu_diagnostics.restype = C.c_int
+    L.mgpu_diagnostics_noalloc.argtypes = [C.c_void_p, C.POINTER(MgpuDiag)]
+    L.mgpu_diagnostics_noalloc.restype = C.c_int
     L.mgpu_max_cfl.argtypes = [C.c_void_p, _dp]
     L.mgpu_max_cfl.restype = C.c_int
     L.mgpu_exchange_plan.argtypes = [C.POINTER(MgpuCase), C.c_int, C.POINTER(C.c_int64), C.c_int64,

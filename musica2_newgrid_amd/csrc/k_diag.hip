@@ -6,6 +6,8 @@
 // (src/Time/timestep_new.f90:270-288).  Not on the time-step path: one thread per point, neighbours
 // read straight from global memory; each workgroup writes its three partial
 // sums, which the host adds in a fixed order (deterministic results).
+#include <mutex>
+
 #include "mgpu_internal.h"
 
 namespace mgpu {
@@ -130,9 +132,79 @@ __global__ __launch_bounds__(kDiagThreads) void k_diag(DiagArgs a) {
   if (threadIdx.x < 4) a.partial[4 * (long)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// Totals of the per-workgroup results: sums of the three sums, maximum of the
+// maxima.  One workgroup; each thread adds a fixed strided subset, then a
+// fixed tree (deterministic).
+__global__ __launch_bounds__(kDiagThreads) void k_diag_total(const double *__restrict__ partial, long ng,
+                                                             double *__restrict__ total) {
+  __shared__ double red[4][kDiagThreads];
+  double s[3] = {0.0, 0.0, 0.0}, m = 0.0;
+  for (long w = threadIdx.x; w < ng; w += kDiagThreads) {
+#pragma unroll
+    for (int q = 0; q < 3; q++) s[q] += partial[4 * w + q];
+    m = fmax(m, partial[4 * w + 3]);
+  }
+#pragma unroll
+  for (int q = 0; q < 3; q++) red[q][threadIdx.x] = s[q];
+  red[3][threadIdx.x] = m;
+  __syncthreads();
+  for (int h = kDiagThreads / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) red[q][threadIdx.x] += red[q][threadIdx.x + h];
+      red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) total[threadIdx.x] = red[threadIdx.x][0];
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};  // device address of `scratch`, per device
+
 }  // namespace
 
+// Device memory of the allocation-free diagnostics: the per-workgroup
+// results of one block (at most kDiagScratchGroups workgroups) followed by
+// its 4 totals (4 MiB).  Each GPU holds its own copy for the life of the
+// process.  Host code never passes this name to a kernel or a copy: it uses
+// the address diag_scratch_init looked up for the device.
+static __device__ __align__(256) double scratch[4 * kDiagScratchGroups + 4];
+
 long diag_groups(const Geom &g) { return ((long)g.nx * g.ny * g.nz + kDiagThreads - 1) / kDiagThreads; }
+
+hipError_t diag_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return hipSuccess;
+  double *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch[device] = p;
+  return hipSuccess;
+}
+
+hipError_t diag_block_noalloc(DiagArgs a, int geo, int device, hipStream_t s, double tot[4]) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  double *p;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    p = g_scratch[device];
+  }
+  if (!p) return hipErrorNotInitialized;
+  const long ng = diag_groups(a.g);
+  if (ng > kDiagScratchGroups) return hipErrorInvalidValue;
+  a.partial = p;  // every workgroup writes its 4 results: nothing to clear
+  hipError_t e = launch_diag(a, geo, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_diag_total, dim3(1), dim3(kDiagThreads), 0, s, p, ng, p + 4 * kDiagScratchGroups);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(tot, p + 4 * kDiagScratchGroups, 4 * sizeof(double), hipMemcpyDeviceToHost, s)) !=
+      hipSuccess)
+    return e;
+  return hipStreamSynchronize(s);
+}
 
 hipError_t launch_diag(const DiagArgs &a, int geo, hipStream_t s) {
   const long nb = diag_groups(a.g);

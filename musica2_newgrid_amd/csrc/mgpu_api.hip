@@ -1147,6 +1147,13 @@ int handle_new(const mgpu_case &c, mgpu_handle **out) {
     delete h;
     return fail(nullptr, MGPU_EDEVICE, "hipStreamCreate failed");
   }
+  // device address of the allocation-free diagnostics' static array, looked
+  // up once per device (this device is current)
+  if (diag_scratch_init(h->device) != hipSuccess) {
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return fail(nullptr, MGPU_EDEVICE, "device address of the diagnostics scratch array not found");
+  }
   *out = h;
   return 0;
 }
@@ -1474,6 +1481,51 @@ int diagnostics(mgpu_handle *h, mgpu_diag *out) {
       for (int q = 0; q < 3; q++) sum[q] += part[4 * w + q];
       umax = part[4 * w + 3] > umax ? part[4 * w + 3] : umax;
     }
+    cells += (int64_t)bd.g.nx * bd.g.ny * bd.g.nz;
+  }
+  out->mass = sum[0];
+  out->kinetic = sum[1];
+  out->enstrophy = sum[2];
+  out->umax = umax;
+  out->cells = cells;
+  return 0;
+}
+
+// mgpu_diagnostics without any device allocation: the per-workgroup results
+// go to the static array of k_diag.hip, block by block; the block totals are
+// added on the host in block order.  The exchange tables must exist already
+// (mgpu_create builds them), since building them allocates; stale ghosts are
+// refreshed with the exchange kernel alone.
+int diagnostics_noalloc(mgpu_handle *h, mgpu_diag *out) {
+  for (size_t bI = 0; bI < h->b.size(); bI++) {
+    if (!h->b[bI].have_metrics) return fail(h, MGPU_ESTATE, "block %zu: metrics not set", bI);
+    if (!h->b[bI].have_state) return fail(h, MGPU_ESTATE, "block %zu: state not set", bI);
+    const long ng = diag_groups(h->b[bI].g);
+    if (ng > kDiagScratchGroups)
+      return fail(h, MGPU_EINVAL,
+                  "block %zu: %ld interior cells; the allocation-free diagnostics take at most %ld per block",
+                  bI, (long)h->b[bI].g.nx * h->b[bI].g.ny * h->b[bI].g.nz, kDiagScratchGroups * kDiagThreads);
+  }
+  if (!h->exchange_ready)
+    return fail(h, MGPU_ESTATE,
+                "exchange tables not built (building them allocates); mgpu_create builds them");
+  if (h->ghosts_dirty) {
+    HIPCK(h, launch_iface(h->xs[h->cur], h->nchunks_s, h->stream));
+    h->ghosts_dirty = false;
+  }
+  double sum[3] = {0.0, 0.0, 0.0}, umax = 0.0;
+  int64_t cells = 0;
+  for (auto &bd : h->b) {
+    DiagArgs da{};
+    da.g = bd.g;
+    for (int v = 0; v < 5; v++) da.u.v[v] = bd.st[h->cur][v];
+    da.met = metrics_of(bd);
+    da.gam = bd.p.gam;
+    da.gam1 = bd.p.gam - 1.0;
+    double tot[4];
+    HIPCK(h, diag_block_noalloc(da, h->c.curvilinear, h->device, h->stream, tot));
+    for (int q = 0; q < 3; q++) sum[q] += tot[q];
+    umax = tot[3] > umax ? tot[3] : umax;
     cells += (int64_t)bd.g.nx * bd.g.ny * bd.g.nz;
   }
   out->mass = sum[0];
@@ -1962,6 +2014,12 @@ int mgpu_diagnostics(mgpu_handle *h, mgpu_diag *out) {
   if (!h) return MGPU_EINVAL;
   if (!out) return fail(h, MGPU_EINVAL, "null argument");
   return diagnostics(h, out);
+}
+
+int mgpu_diagnostics_noalloc(mgpu_handle *h, mgpu_diag *out) {
+  if (!h) return MGPU_EINVAL;
+  if (!out) return fail(h, MGPU_EINVAL, "null argument");
+  return diagnostics_noalloc(h, out);
 }
 
 void mgpu_destroy(mgpu_handle *h) { destroy(h); }

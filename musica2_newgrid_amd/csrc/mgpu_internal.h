@@ -413,4 +413,13 @@ hipError_t launch_visc_wall(const WallViscArgs &a, hipStream_t s);
 hipError_t launch_tau_wallrow(const WallViscArgs &a, hipStream_t s);
 hipError_t launch_vdiv_wall(const WallViscArgs &a, hipStream_t s);
 hipError_t launch_diag(const DiagArgs &a, int geo, hipStream_t s);
+// Allocation-free diagnostics of one block (mgpu_diagnostics_noalloc): the
+// per-workgroup results and their totals go to a static device array of
+// 4 * kDiagScratchGroups + 4 doubles; diag_scratch_init looks its address up
+// once per device (that device current).  diag_block_noalloc returns the
+// block's 3 sums and maximum in tot, synchronously; hipErrorInvalidValue if
+// the block needs more than kDiagScratchGroups workgroups.
+constexpr long kDiagScratchGroups = 1L << 17;
+hipError_t diag_scratch_init(int device);
+hipError_t diag_block_noalloc(DiagArgs a, int geo, int device, hipStream_t s, double tot[4]);
 }  // namespace mgpu
