@@ -272,7 +272,7 @@ def serve_leg(nreq: int, nsteps: int, seed: int = 5):
     rate = 0.8 / (mean_cells * 4 * nsteps / svc)
     reqs = serving.make_requests(nreq, seed=seed, edges=edges, nsteps=nsteps, rate=rate)
     stats = {}
-    lat, wall, work = serving.run_stream(reqs, budget_bytes=budget, quantum=10, stats=stats)
+    lat, wall, work = serving.run_stream(reqs, budget_bytes=budget, quantum=10, stats=stats, progress=True)
     p50, p99 = np.percentile(lat, [50, 99])
     return {"requests": nreq, "steps_per_request": nsteps, "arrival_rate_per_s": round(rate, 3),
             "measured_service_rate": round(svc / 1e6, 2), "hbm_budget_GiB": budget >> 30, "wall_s": round(wall, 3),

@@ -13,6 +13,7 @@ from __future__ import annotations
 import ctypes as C
 import dataclasses
 import math
+import sys
 import time
 from typing import List, Optional, Sequence
 
@@ -135,11 +136,12 @@ def make_requests(n: int, seed: int, edges=(64, 96, 128, 192, 256), re_range=(40
 
 
 def run_stream(reqs: Sequence[Request], budget_bytes: int, quantum: int = 10,
-               results: Optional[dict] = None, stats: Optional[dict] = None):
+               results: Optional[dict] = None, stats: Optional[dict] = None, progress: bool = False):
     """Replay the arrival sequence in wall-clock time against a server.
     Returns (latencies in s, wall time in s, cell-stage count); results, if
     given, receives the final state of each request keyed by its index, and
-    stats the server's co-batching record."""
+    stats the server's co-batching record.  progress: one line on stderr per
+    completed request."""
     order = sorted(range(len(reqs)), key=lambda q: reqs[q].arrival)
     lat = [0.0] * len(reqs)
     ids = {}
@@ -163,6 +165,10 @@ def run_stream(reqs: Sequence[Request], budget_bytes: int, quantum: int = 10,
                     lat[q] = time.perf_counter() - t0 - reqs[q].arrival
                     if results is not None:
                         results[q] = U
+                    if progress:
+                        done = len(reqs) - len(pending) - len(ids)
+                        print(f"serving: {done}/{len(reqs)} done at {time.perf_counter() - t0:.1f} s "
+                              f"({reqs[q].kind}, latency {lat[q]:.2f} s)", file=sys.stderr, flush=True)
         wall = time.perf_counter() - t0
         if stats is not None:
             stats.update(srv.batching())
@@ -170,11 +176,12 @@ def run_stream(reqs: Sequence[Request], budget_bytes: int, quantum: int = 10,
     return lat, wall, work
 
 
-def service_rate(edges=(64, 96, 128, 192, 256), nsteps: int = 20, budget_bytes: int = 64 << 30,
+def service_rate(edges=(64, 96, 128, 192, 256), nsteps: int = 100, budget_bytes: int = 64 << 30,
                  seed: int = 11) -> float:
     """Measured throughput of the server when saturated (cell-stages/s): one
     TGV and one vortex request of every edge, FP64 and mixed, all submitted at
-    once and run to completion."""
+    once and run to completion (enough steps that admission, upload and
+    retrieval do not dominate)."""
     reqs = []
     for q, e in enumerate(edges):
         reqs.append(Request(cases.tgv(e), nsteps, _lib.PREC_FP64, 0.0, f"tgv{e}"))
