@@ -438,3 +438,121 @@ def affine_c3_metrics(nx, ny, nz, lengths, viscous=True):
     m = [gr.padded(nx, ny, nz, float(v)) for v in vals]
     mv = [gr.padded(nx, ny, nz, float(v)) for v in vals] if viscous else None
     return m, mv
+
+
+def oblique_shock(mach: float, theta_deg: float, gam: float = GAM):
+    """Weak oblique-shock solution of the theta-beta-Mach relation: shock angle
+    beta (radians), and the ratios p2/p1, rho2/rho1 and M2 behind it."""
+    th = math.radians(theta_deg)
+
+    def f(b):
+        return math.tan(th) - 2.0 / math.tan(b) * (mach ** 2 * math.sin(b) ** 2 - 1.0) / (
+            mach ** 2 * (gam + math.cos(2.0 * b)) + 2.0)
+
+    lo, hi = math.asin(1.0 / mach) + 1e-9, math.radians(64.0)  # the weak branch lies below ~64 deg at M = 2
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if f(lo) * f(mid) <= 0.0:
+            hi = mid
+        else:
+            lo = mid
+    b = 0.5 * (lo + hi)
+    mn1 = mach * math.sin(b)
+    p21 = 1.0 + 2.0 * gam / (gam + 1.0) * (mn1 ** 2 - 1.0)
+    r21 = (gam + 1.0) * mn1 ** 2 / ((gam - 1.0) * mn1 ** 2 + 2.0)
+    mn2 = math.sqrt((1.0 + 0.5 * (gam - 1.0) * mn1 ** 2) / (gam * mn1 ** 2 - 0.5 * (gam - 1.0)))
+    return b, p21, r21, mn2 / math.sin(b - th)
+
+
+def one_sided_coords(n: int, L: float, beta: float) -> np.ndarray:
+    """Nodes of rows 1..n on [0, L] clustered at 0 (tanh), padded with G ghost
+    coordinates of the end spacings."""
+    s = np.arange(n) / (n - 1)
+    y = L * (1.0 - np.tanh(beta * (1.0 - s)) / math.tanh(beta)) if beta > 0 else L * s
+    ye = np.empty(n + 2 * G)
+    ye[G:G + n] = y
+    ye[:G] = y[0] - (y[1] - y[0]) * np.arange(G, 0, -1)
+    ye[G + n:] = y[-1] + (y[-1] - y[-2]) * np.arange(1, G + 1)
+    return ye
+
+
+def shock_bl(nx: int = 1024, ny: int = 512, nz: int = 8, mach: float = 2.0, wedge_deg: float = 10.0,
+             re_delta: float = 1.0e4, lengths=(2.0, 1.0, 0.05), delta: float = 0.05, beta_y: float = 1.2,
+             x_shock: float = 0.35, noise: float = 1.0e-4, cfl: float = 0.8, filter: int = 2,
+             dissip_shock: float = 0.3, chi: float = 0.2, csens: float = 0.5, adiabatic: bool = True,
+             seed: int = 4, T_inf: float = 300.0, rho_inf: float = 1.177, iorder_visc: int = 4) -> Case:
+    """Mach 2 oblique shock (wedge angle 10 deg) meeting a laminar boundary layer
+    (configuration 4): a 2-D-extruded Cartesian block, supersonic inlet at
+    imin (the initial first row), characteristic outflow at imax and jmax
+    (bc_charac_imax / bc_charac_jmax), a no-slip wall at jmin, z periodic.
+    Initial field: a Pohlhausen (Blasius-like) profile u/U = 2 eta - 2 eta^3 +
+    eta^4 of thickness delta with a Crocco-Busemann temperature (recovery factor
+    Pr^1/2), uniform pressure; above the shock line, which enters through the
+    top at x_shock at the angle beta of the theta-beta-M relation, the
+    post-shock state (deflected by the wedge angle), blended over two cells;
+    seeded uniform velocity noise of amplitude noise*U (zero on the wall row).
+    Re_delta = rho U delta / mu sets the Sutherland scale; shock filter with
+    Ducros sensor; y-nodes clustered at the wall."""
+    Lx, Ly, Lz = lengths
+    dx, dz = Lx / nx, Lz / nz
+    xe = np.arange(-G, nx + G) * dx
+    ye = one_sided_coords(ny, Ly, beta_y)
+    x, y = xe[G:G + nx], ye[G:G + ny]
+    z = np.arange(nz) * dz
+    Z, Y, X = z.reshape(-1, 1, 1), y.reshape(1, -1, 1), x.reshape(1, 1, -1)
+    c_inf = math.sqrt(GAM * RG * T_inf)
+    U_inf = mach * c_inf
+    p_inf = rho_inf * RG * T_inf
+    eta = np.minimum(Y / delta, 1.0)
+    fu = 2.0 * eta - 2.0 * eta ** 3 + eta ** 4
+    rec = math.sqrt(PR)
+    T = T_inf * (1.0 + rec * 0.5 * (GAM - 1.0) * mach ** 2 * (1.0 - fu * fu))
+    u = U_inf * fu
+    v = np.zeros_like(u)
+    p = np.full_like(u, p_inf)
+    # the oblique shock: post-shock state above the line y = Ly - (x - x_shock) tan(beta)
+    b, p21, r21, m2 = oblique_shock(mach, wedge_deg)
+    th = math.radians(wedge_deg)
+    T2 = T_inf * p21 / r21
+    U2 = m2 * math.sqrt(GAM * RG * T2)
+    dist = (Y - (Ly - (X - x_shock) * math.tan(b))) * math.cos(b)  # > 0 above the shock
+    hs = 0.5 * (1.0 + np.tanh(dist / (2.0 * max(dx, float(np.min(np.diff(y)))))))
+    u = u * (1.0 - hs) + U2 * math.cos(th) * hs
+    v = v * (1.0 - hs) - U2 * math.sin(th) * hs
+    p = p * (1.0 - hs) + p_inf * p21 * hs
+    T = T * (1.0 - hs) + T2 * hs
+    shape = (nz, ny, nx)
+    u, v, p, T = (np.broadcast_to(q, shape).copy() for q in (u, v, p, T))
+    rng = np.random.default_rng(seed)
+    w = np.zeros(shape)
+    for q in (u, v, w):
+        q += noise * U_inf * rng.uniform(-1.0, 1.0, shape)
+    u[:, 0, :] = v[:, 0, :] = w[:, 0, :] = 0.0  # no-slip wall row
+    rho = p / (RG * T)
+    U = []
+    for f in conserved(rho, u, v, w, p):
+        a = gr.padded(nx, ny, nz)
+        a[G:-G, G:-G, G:-G] = f
+        # ghosts beyond the boundary faces repeat the boundary row (only the
+        # shock filter's pressure sensor of a boundary row reads them)
+        a[G:-G, G:-G, :G] = f[:, :, :1]
+        a[G:-G, G:-G, G + nx:] = f[:, :, -1:]
+        a[G:-G, :G, :] = a[G:-G, G:G + 1, :]
+        a[G:-G, G + ny:, :] = a[G:-G, G + ny - 1:G + ny, :]
+        gr.fill_periodic(a, axes=(0,))
+        U.append(a)
+    idxe, idxv, _, _ = gr.wall_metrics(xe, True, True)
+    idye, idyv, c2, c3 = gr.wall_metrics(ye, True, True)
+    idz = np.full(nz + 2 * G, 1.0 / dz)
+    mu_inf = rho_inf * U_inf * delta / re_delta
+    ext2, ext3 = [0.0] * 6, [0.0] * 6
+    ext2[2], ext3[2] = c2[0], c3[0]
+    c = Case("shock_bl", nx, ny, nz, curvilinear=False, iorder_visc=iorder_visc, filter=filter, ducros=True,
+             diffscale=mu_inf / float(sutherland(np.float64(T_inf))), chi=chi, dissip_shock=dissip_shock,
+             csens=csens, cart=[idxe, idye, idz, idxv, idyv, idz.copy()], U=U, faces=(4, 3, 2, 3, 0, 0),
+             adiabatic=adiabatic, t_wall=float(T[0, 0, 0]), cextrp2=tuple(ext2), cextrp3=tuple(ext3))
+    c.p_ref = p_inf
+    c.dt = timestep(U, min(dx, dz, float(np.min(np.diff(y)))), cfl)
+    c.info = dict(mach=mach, wedge_deg=wedge_deg, beta_deg=math.degrees(b), re_delta=re_delta, delta=delta,
+                  lengths=lengths, x_shock=x_shock, seed=seed, p_inf=p_inf, U_inf=U_inf)
+    return c

@@ -110,6 +110,35 @@ static walls_t walls_of(const orc_cfg *c) {
   return W;
 }
 
+/* Boundary faces of any kind (no-slip wall, characteristic outflow,
+ * supersonic inlet): the operators close them alike (is_boundary / is_bc_1pt
+ * of mpi_index_bounds.f90:20-70: ndx_e = 2 for sort 0 and sort <= -3; the
+ * one-sided gradients; the filters' closures, mod_filtering.f90:900-988). */
+static walls_t bounds_of(const orc_cfg *c) {
+  walls_t W;
+  W.any = 0;
+  for (int a = 0; a < 3; a++) {
+    W.lo[a] = c->face[a][0] >= ORC_FACE_WALL;
+    W.hi[a] = c->face[a][1] >= ORC_FACE_WALL;
+    W.any |= W.lo[a] | W.hi[a];
+  }
+  return W;
+}
+
+/* Characteristic faces (sort -3: outflow or supersonic inlet): their row
+ * gets no Euler and no viscous increment at all (ndx_e = ndx_v = 2); the
+ * boundary routines set it. */
+static walls_t chars_of(const orc_cfg *c) {
+  walls_t W;
+  W.any = 0;
+  for (int a = 0; a < 3; a++) {
+    W.lo[a] = c->face[a][0] >= ORC_FACE_CHARAC;
+    W.hi[a] = c->face[a][1] >= ORC_FACE_CHARAC;
+    W.any |= W.lo[a] | W.hi[a];
+  }
+  return W;
+}
+
 /* Half-width of the centred stencil at row r (1..n) of an axis: 5 inside,
  * r-1 (n-r) near a min (max) wall, 0 on the wall row itself. */
 static inline int halfw(int r, int n, int lo, int hi) {
@@ -313,16 +342,15 @@ static void euler_rhs_w(const orc_cfg *c, const orc_grid *g, const dims_t *d, do
   const long sx = 1, sy = d->sx, sz = d->sx * d->sy;
   const long strides[3] = {sx, sy, sz};
   const int nn[3] = {d->nx, d->ny, d->nz};
-  const walls_t W = walls_of(c);
-  /* rows with an Euler increment along each axis: 2..n-1 next to walls (ndx_e, nfx_e) */
+  const walls_t W = bounds_of(c);
+  /* rows with an Euler increment along each axis: 2..n-1 next to boundaries
+   * (ndx_e, nfx_e); K on the boundary rows is left as it is (bc_wall zeroes it
+   * on wall rows, the characteristic routines set it on theirs) */
   int lo_e[3], hi_e[3];
   for (int a = 0; a < 3; a++) {
     lo_e[a] = W.lo[a] ? 2 : 1;
     hi_e[a] = nn[a] - (W.hi[a] ? 1 : 0);
   }
-  /* the wall rows keep K = 0 (bc_wall zeroes them, flux_euler skips them) */
-  if (W.any)
-    for (int v = 0; v < 5; v++) memset(K[v], 0, d->n * sizeof(double));
   /* line buffers: fluxes along a line with ghosts */
   for (int dir = 0; dir < 3; dir++) {
     const long s = strides[dir];
@@ -411,7 +439,7 @@ static inline double d5(const double *f, long p, long s) {
 static void gradients(const orc_cfg *c, const orc_grid *g, const dims_t *d, work_t *w) {
   const long sy = d->sx, sz = d->sx * d->sy;
   double *src[4] = {w->uu, w->vv, w->ww, w->tmp};
-  const walls_t W = walls_of(c);
+  const walls_t W = bounds_of(c), Wall = walls_of(c);
 #pragma omp parallel for collapse(2)
   for (int k = 1; k <= d->nz; k++)
     for (int j = 1; j <= d->ny; j++)
@@ -449,10 +477,10 @@ static void gradients(const orc_cfg *c, const orc_grid *g, const dims_t *d, work
         }
         /* adiabatic walls: no wall-normal temperature gradient on the wall row
          * (smod_grad_T.f90:102-109 and its twins) */
-        if (c->adiabatic && W.any) {
+        if (c->adiabatic && Wall.any) {
           const int rr[3] = {i, j, k}, nn[3] = {d->nx, d->ny, d->nz};
           for (int a = 0; a < 3; a++)
-            if ((W.lo[a] && rr[a] == 1) || (W.hi[a] && rr[a] == nn[a])) w->grad[DTX + a][p] = 0.0;
+            if ((Wall.lo[a] && rr[a] == 1) || (Wall.hi[a] && rr[a] == nn[a])) w->grad[DTX + a][p] = 0.0;
         }
       }
 }
@@ -512,7 +540,7 @@ static void visc_rhs_w(const orc_cfg *c, const orc_grid *g, const dims_t *d, wor
       }
   /* divergence, x then y then z, each ( ... )*metric + K; one-sided rows at walls
    * (smod_flux_visc_5pts.f90:80-160) */
-  const walls_t W = walls_of(c);
+  const walls_t W = bounds_of(c), Wc = chars_of(c);
   const int nn[3] = {d->nx, d->ny, d->nz};
 #pragma omp parallel for collapse(2)
   for (int k = 1; k <= d->nz; k++)
@@ -521,6 +549,11 @@ static void visc_rhs_w(const orc_cfg *c, const orc_grid *g, const dims_t *d, wor
         long p = (long)ID(*d, i, j, k);
         const long st[3] = {1, sy, sz};
         const int rr[3] = {i, j, k};
+        /* the row of a characteristic face gets no viscous increment along any
+         * axis (ndx_v = 2 there: smod_flux_visc_5pts.f90:80-175 loops) */
+        int onrow = 0;
+        for (int a = 0; a < 3; a++) onrow |= (Wc.lo[a] && rr[a] == 1) || (Wc.hi[a] && rr[a] == nn[a]);
+        if (onrow) continue;
         for (int dir = 0; dir < 3; dir++) {
           double metric;
           if (c->curv == 1 || (c->curv == 2 && dir < 2)) metric = mtr(c, g->mv, g->m2v, d, 9, p);
@@ -541,9 +574,9 @@ static void filter_sf(const orc_cfg *c, const dims_t *d, double *const Un[5], do
   double dd[6];
   for (int l = 0; l < 6; l++) dd[l] = -D11[l] * c->chi;
   const long st[3] = {1, d->sx, d->sx * d->sy};
-  const walls_t W = walls_of(c);
+  const walls_t W = bounds_of(c);
   if (W.any) {
-    /* rows 1,2 from a wall get nothing along its axis, rows 3,4,5 the 5/7/9-pt
+    /* rows 1,2 from a boundary get nothing along its axis, rows 3,4,5 the 5/7/9-pt
      * filters (mod_filtering.f90:900-988, 1116-1190) */
     const int nn[3] = {d->nx, d->ny, d->nz};
     double dc[6][6];
@@ -640,7 +673,7 @@ static void filter_shock(const orc_cfg *c, const orc_grid *g, const dims_t *d, d
   const double xnu[6] = {0.0, 0.0, c->chi / 12.0, c->chi / 60.0, c->chi / 280.0, c->chi / 1260.0};
   static const double DF[6][6] = {{0}, {0, -1.0}, {0, -3.0, 1.0}, {0, -10.0, 5.0, -1.0},
                                   {0, -35.0, 21.0, -7.0, 1.0}, {0, -126.0, 84.0, -36.0, 9.0, -1.0}};
-  const walls_t W = walls_of(c);
+  const walls_t W = bounds_of(c);
   double *sens = alloc_arr(d);
   /* Ducros sensor on 0..n+1 (1..n along a wall axis: ndx_d1..nfx_d1) */
 #pragma omp parallel for collapse(2)
@@ -785,7 +818,7 @@ static void artvisc(const orc_cfg *c, const orc_grid *g, const dims_t *d, double
   for (int h = 0; h < 6; h++)
     for (int l = 0; l < 6; l++) dk[h][l] = DP[h][l] * xnu[h];
   const double *d10 = dk[5];
-  const walls_t W = walls_of(c);
+  const walls_t W = bounds_of(c);
   const long st[3] = {1, d->sx, d->sx * d->sy};
   const int nn[3] = {d->nx, d->ny, d->nz};
   for (int dir = 0; dir < 3; dir++) {
@@ -852,10 +885,28 @@ static void artvisc(const orc_cfg *c, const orc_grid *g, const dims_t *d, double
 /* bc_wall (mod_bc_apply.f90:237-261), so a later face wins on an edge.  T   */
 /* and p are the primitives last computed (w), not those of Un.              */
 /* ------------------------------------------------------------------------ */
-static void bc_wall(const orc_cfg *c, const orc_grid *g, const dims_t *d, double *const Un[5], const work_t *w) {
+static void bc_wall(const orc_cfg *c, const orc_grid *g, const dims_t *d, double *const Un[5], const work_t *w,
+                    double *K[5]) {
   const walls_t W = walls_of(c);
   if (!W.any) return;
+  const walls_t B = bounds_of(c);
   const int nn[3] = {d->nx, d->ny, d->nz};
+  /* K = 0 on the wall rows over the tangential rows with an Euler increment
+   * (ndx_e..nfx_e, ndz_e..nfz_e: smod_bc_wall.f90:226-232); the rows of a
+   * characteristic face keep the increment their routine gives them */
+  if (K)
+    for (int f = 0; f < 6; f++) {
+      const int a = f / 2, side = f % 2;
+      if (!(side ? W.hi[a] : W.lo[a])) continue;
+      const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+      for (int q2 = B.lo[t2] ? 2 : 1; q2 <= nn[t2] - (B.hi[t2] ? 1 : 0); q2++)
+        for (int q1 = B.lo[t1] ? 2 : 1; q1 <= nn[t1] - (B.hi[t1] ? 1 : 0); q1++) {
+          int idx3[3];
+          idx3[t1] = q1; idx3[t2] = q2; idx3[a] = side ? nn[a] : 1;
+          const long p1 = (long)ID(*d, idx3[0], idx3[1], idx3[2]);
+          for (int v = 0; v < 5; v++) K[v][p1] = 0.0;
+        }
+    }
   const long st[3] = {1, d->sx, d->sx * d->sy};
   const double cv = 1.0 / (c->gam - 1.0) * c->rg; /* cvfg, mod_eos_pfg.f90:33 */
   for (int f = 0; f < 6; f++) {
@@ -881,6 +932,121 @@ static void bc_wall(const orc_cfg *c, const orc_grid *g, const dims_t *d, double
         Un[4][p1] = ro * (cv * t_wall); /* ro*ecalc_tro_pfg */
       }
   }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Characteristic (LODI) outflow of a Cartesian block: the increment K of the */
+/* boundary row from one-sided 5-pt derivatives (a40, times idx_v / idy_v)   */
+/* of the primitives of the stage state Un, as bc_charac_imax and            */
+/* bc_charac_jmax write it (src/Boundary/mod_charac.f90:350-469, 816-969),   */
+/* for a perfect gas (cp/av = cp T, mod_eos_pfg.f90:38-84):                  */
+/*   imax: L1 = (u-c)(px - rho c ux), or 0.1 (p - 0.528 p_ref) if u-c <= 0;  */
+/*         L5 = (u+c)(px + rho c ux), 0 if u+c <= 0; L2 = u (px - c^2 rhox); */
+/*         L3 = u vx; L4 = u wx; pt = (L5+L1)/2; rt = -(pt+L2)/c^2;          */
+/*         ut = (L5-L1)/(2 rho c); vt = L3; wt = L4, with c the sound speed  */
+/*         that primitives_visc last computed (cstale);                      */
+/*   jmax: L1 = 0, L5 = (v+c)(py + rho c vy); v <= 0: L2 = L3 = L4 = 0 (and  */
+/*         L5 = 0 if v <= -c); else L2 = v uy, L3 = v (py - c^2 rhoy),       */
+/*         L4 = v wy (L1 = (v-c)(py - rho c vy) if v >= c); pt = (L5+L1)/2;  */
+/*         rt = (pt+L3)/c^2; ut = L2; vt = (L5-L1)/(2 rho c); wt = L4, with  */
+/*         c = sqrt(gam rg T) of the current primitives, on i = ndx_c..nfx_c;*/
+/*   K = (rt, rho ut + u rt, rho vt + v rt, rho wt + w rt,                   */
+/*        cp T (pt/c^2 - rt) + (rhoE + p)/rho rt + rho (u ut + v vt + w wt)).*/
+/* The density equations are restated as written (they are not the textbook */
+/* LODI ones); see the notes where the tests pin the other four.             */
+/* ------------------------------------------------------------------------ */
+static void bc_charac(const orc_cfg *c, const orc_grid *g, const dims_t *d, double *const Un[5],
+                      const work_t *w, const double *cstale, double *K[5]) {
+  const double cp = c->gam * c->rg / (c->gam - 1.0); /* cpfg */
+  const long sy = d->sx;
+  const walls_t B = bounds_of(c);
+  if (c->face[0][1] == ORC_FACE_CHARAC) {
+    const int i = d->nx;
+    const double ix = g->idx_v[i - 1 + NG];
+#pragma omp parallel for collapse(2)
+    for (int k = 1; k <= d->nz; k++)
+      for (int j = 1; j <= d->ny; j++) {
+        const long p = (long)ID(*d, i, j, k);
+        double dd[5];
+        const double *f[5] = {w->uu, w->vv, w->ww, w->prs, Un[0]};
+        for (int q = 0; q < 5; q++)
+          dd[q] = (ORC_PLANT == 7 ? -1.0 : 1.0) *
+                  ((-A04[5]) * f[q][p - 4] + (-A04[4]) * f[q][p - 3] + (-A04[3]) * f[q][p - 2] +
+                   (-A04[2]) * f[q][p - 1] + (-A04[1]) * f[q][p]) * ix;
+        const double dux = dd[0], dvx = dd[1], dwx = dd[2], dpx = dd[3], drx = dd[4];
+        const double rho = Un[0][p], u = w->uu[p], v = w->vv[p], ww = w->ww[p], pr = w->prs[p];
+        const double cj = cstale[p], cj2 = cj * cj, upc = u + cj, umc = u - cj;
+        const double L1 = umc <= 0.0 ? 0.1 * (pr - (ORC_PLANT == 5 ? 0.5 : 0.528) * c->p_ref)
+                                     : umc * (dpx - rho * cj * dux);
+        const double L5 = upc <= 0.0 ? 0.0 : upc * (dpx + rho * cj * dux);
+        const double L2 = u * (dpx - cj2 * drx), L3 = u * dvx, L4 = u * dwx;
+        const double pt = 0.5 * (L5 + L1);
+        const double rt = ORC_PLANT == 4 ? (pt - L2) / cj2 : -(pt + L2) / cj2;
+        const double ut = 0.5 * (L5 - L1) / (rho * cj), vt = L3, wt = L4;
+        const double cpav = cp * w->tmp[p];
+        K[0][p] = rt;
+        K[1][p] = rho * ut + u * rt;
+        K[2][p] = rho * vt + v * rt;
+        K[3][p] = rho * wt + ww * rt;
+        K[4][p] = cpav * (pt / (cj * cj) - rt) + (Un[4][p] + pr) / rho * rt + rho * (u * ut + v * vt + ww * wt);
+      }
+  }
+  if (c->face[1][1] == ORC_FACE_CHARAC) {
+    const int j = d->ny;
+    const double iy = g->idy_v[j - 1 + NG];
+    /* i = ndx_c..nfx_c: one row in from a characteristic or inlet i-face
+     * (mpi_index_bounds.f90:298-360) */
+    const walls_t Wc = chars_of(c);
+    const int i0 = Wc.lo[0] ? 2 : 1, i1 = d->nx - (Wc.hi[0] ? 1 : 0);
+    (void)B;
+#pragma omp parallel for collapse(2)
+    for (int k = 1; k <= d->nz; k++)
+      for (int i = i0; i <= i1; i++) {
+        const long p = (long)ID(*d, i, j, k);
+        double dd[5];
+        const double *f[5] = {w->uu, w->vv, w->ww, w->prs, Un[0]};
+        for (int q = 0; q < 5; q++)
+          dd[q] = ((-A04[5]) * f[q][p - 4 * sy] + (-A04[4]) * f[q][p - 3 * sy] + (-A04[3]) * f[q][p - 2 * sy] +
+                   (-A04[2]) * f[q][p - sy] + (-A04[1]) * f[q][p]) * iy;
+        const double duy = dd[0], dvy = dd[1], dwy = dd[2], dpy = dd[3], dry = dd[4];
+        const double rho = Un[0][p], u = w->uu[p], v = w->vv[p], ww = w->ww[p], pr = w->prs[p];
+        const double cc = sqrt(c->gam * c->rg * w->tmp[p]); /* c2calc_tro */
+        double L1 = 0.0, L2, L3, L4;
+        double L5 = (ORC_PLANT == 6 ? v : v + cc) * (dpy + rho * cc * dvy);
+        if (v <= 0.0) {
+          L2 = L3 = L4 = 0.0;
+          if (v <= -cc) L5 = 0.0;
+        } else {
+          L2 = v * duy;
+          L3 = v * (dpy - (cc * cc) * dry);
+          L4 = v * dwy;
+          if (v >= cc) L1 = (v - cc) * (dpy - rho * cc * dvy);
+        }
+        const double pt = 0.5 * (L5 + L1);
+        const double rt = (pt + L3) / (cc * cc);
+        const double ut = L2, vt = (L5 - L1) / (2.0 * rho * cc), wt = L4;
+        const double cpav = cp * w->tmp[p];
+        K[0][p] = rt;
+        K[1][p] = rho * ut + u * rt;
+        K[2][p] = rho * vt + v * rt;
+        K[3][p] = rho * wt + ww * rt;
+        K[4][p] = cpav * (pt / (cc * cc) - rt) + (Un[4][p] + pr) / rho * rt + rho * (u * ut + v * vt + ww * wt);
+      }
+  }
+}
+
+/* Supersonic inlet (bc_supersonic_inlet without eigenmodes or RFM,
+ * mod_charac.f90:1133-1146): the conserved state of row i = 1 is the inflow
+ * state Uin (the row of the initial field, BC_face(1,1)%Uref(:,:,7:10) of
+ * init_BC_ref with is_BCref_init, mod_init_flow.f90:418-440).  The primitives
+ * the caller computed before keep their values. */
+static void bc_inlet(const orc_cfg *c, const dims_t *d, double *const Un[5], double *const Uin[5]) {
+  if (c->face[0][0] != ORC_FACE_INLET) return;
+  for (int k = 1; k <= d->nz; k++)
+    for (int j = 1; j <= d->ny; j++) {
+      const long p = (long)ID(*d, 1, j, k);
+      for (int v = 0; v < 5; v++) Un[v][p] = Uin[v][p];
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1080,31 +1246,66 @@ int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_gri
     for (int f = 0; f < 6; f++)
       if (bl[b].wall[f]) {
         const int ext = f < 2 ? bl[b].nx : (f < 4 ? bl[b].ny : bl[b].nz);
-        if (c0->curv) return -5;        /* walls: Cartesian blocks only */
+        if (c0->curv) return -5;        /* boundaries: Cartesian blocks only */
         if (ext < 10) return -7;        /* closures of the two sides must not overlap */
+        if (bl[b].wall[f] < 1 || bl[b].wall[f] > 3) return -12;
+        /* characteristic outflow at imax or jmax, supersonic inlet at imin
+         * (the faces config. 4 uses); walls on the i-faces together with a
+         * characteristic face are not covered */
+        if (bl[b].wall[f] == 2 && f != 1 && f != 3) return -12;
+        if (bl[b].wall[f] == 3 && f != 0) return -12;
+        if (bl[b].wall[f] >= 2 && (bl[b].wall[0] == 1 || bl[b].wall[1] == 1)) return -12;
       }
   orc_cfg cb[64];
   dims_t d[64];
   work_t w[64];
   double *Un[64][5], *K[64][5];
+  /* characteristic faces: the module arrays the reference keeps between calls
+   * of its boundary routines -- the sound speed primitives_visc last computed
+   * (c_, read by bc_charac_imax), the primitives of row i = 1 from the last
+   * `primitives` before the inlet reset (the flux at i = 1 of stage 1 reads
+   * them), the inflow state -- and K itself, whose rows of characteristic and
+   * inlet faces only the boundary routines and the filter write */
+  double *cst[64], *pst[64][5], *Uin[64][5];
+  int chb[64];
   int rc = 0;
   for (int b = 0; b < nb; b++) {
     cb[b] = *c0;
     cb[b].nx = bl[b].nx;
     cb[b].ny = bl[b].ny;
     cb[b].nz = bl[b].nz;
-    for (int f = 0; f < 6; f++) cb[b].face[f / 2][f % 2] = bl[b].wall[f] ? ORC_FACE_WALL : ORC_FACE_INTERFACE;
+    static const int kinds[4] = {ORC_FACE_INTERFACE, ORC_FACE_WALL, ORC_FACE_CHARAC, ORC_FACE_INLET};
+    for (int f = 0; f < 6; f++) cb[b].face[f / 2][f % 2] = kinds[bl[b].wall[f]];
     if (bl[b].nx < NG || bl[b].ny < NG || bl[b].nz < NG) return -1;
     d[b] = mkdims(&cb[b]);
+    chb[b] = chars_of(&cb[b]).any;
+    cst[b] = NULL;
     for (int v = 0; v < 5; v++) {
       Un[b][v] = alloc_arr(&d[b]);
       K[b][v] = alloc_arr(&d[b]);
-      if (!Un[b][v] || !K[b][v]) rc = -10;
+      pst[b][v] = chb[b] ? alloc_arr(&d[b]) : NULL;
+      Uin[b][v] = chb[b] ? alloc_arr(&d[b]) : NULL;
+      if (!Un[b][v] || !K[b][v] || (chb[b] && (!pst[b][v] || !Uin[b][v]))) rc = -10;
     }
-    if (work_alloc(&d[b], &w[b], c0->visc)) rc = -10;
+    if (chb[b] && !(cst[b] = alloc_arr(&d[b]))) rc = -10;
+    if (work_alloc(&d[b], &w[b], 1)) rc = -10;
   }
   double *tab[64];
   if (rc) goto out;
+  /* start_runge_kutta (runge_kutta.f90:316-342) on blocks with characteristic
+   * faces: the increments of their rows from the primitives of U, the sound
+   * speed of U (the one primitives_visc gave at set-up), the inflow state =
+   * the first row of U */
+  for (int b = 0; b < nb; b++) {
+    if (!chb[b]) continue;
+    prim_visc(&cb[b], &d[b], &U[5 * b], &w[b]);
+    memcpy(cst[b], w[b].cs, d[b].n * sizeof(double));
+    for (int v = 0; v < 5; v++) memcpy(Uin[b][v], U[5 * b + v], d[b].n * sizeof(double));
+    prim_euler(&cb[b], &d[b], &U[5 * b], &w[b]);
+    bc_charac(&cb[b], &g[b], &d[b], &U[5 * b], &w[b], cst[b], K[b]);
+    double *pw[5] = {w[b].uu, w[b].vv, w[b].ww, w[b].prs, w[b].tmp};
+    for (int q = 0; q < 5; q++) memcpy(pst[b][q], pw[q], d[b].n * sizeof(double));
+  }
   for (int step = 0; step < nsteps; step++) {
     for (int b = 0; b < nb; b++)
       for (int v = 0; v < 5; v++) memcpy(Un[b][v], U[5 * b + v], d[b].n * sizeof(double));
@@ -1112,7 +1313,7 @@ int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_gri
       const int last = irk == c0->nrk;
       /* bc_wall at the start of stages 2..nrk, from the previous stage's primitives */
       if (irk > 1)
-        for (int b = 0; b < nb; b++) bc_wall(&cb[b], &g[b], &d[b], Un[b], &w[b]);
+        for (int b = 0; b < nb; b++) bc_wall(&cb[b], &g[b], &d[b], Un[b], &w[b], K[b]);
       if (last && c0->visc) {
         for (int b = 0; b < nb; b++) {
           prim_visc(&cb[b], &d[b], &U[5 * b], &w[b]);
@@ -1124,27 +1325,44 @@ int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_gri
         }
       }
       for (int b = 0; b < nb; b++) {
+        /* the primitives of the stage state (`primitives`); stage 1 keeps the
+         * end-of-step ones, which differ from those of U^n only on an inlet row */
+        work_t we = w[b];
+        double *tmp5[5] = {NULL, NULL, NULL, NULL, NULL};
         if (last && c0->visc) {
           /* primitives of U_n for the Euler part, primitives_visc of U^n kept for the sensor */
-          work_t we = w[b];
-          double *save[5] = {w[b].uu, w[b].vv, w[b].ww, w[b].prs, w[b].tmp};
-          double *tmp5[5];
           for (int q = 0; q < 5; q++) tmp5[q] = alloc_arr(&d[b]);
           we.uu = tmp5[0]; we.vv = tmp5[1]; we.ww = tmp5[2]; we.prs = tmp5[3]; we.tmp = tmp5[4];
-          prim_euler(&cb[b], &d[b], Un[b], &we);
-          euler_rhs_w(&cb[b], &g[b], &d[b], Un[b], &we, K[b]);
-          for (int q = 0; q < 5; q++) free(tmp5[q]);
-          (void)save;
+        }
+        prim_euler(&cb[b], &d[b], Un[b], &we);
+        if (chb[b]) {
+          double *pw[5] = {we.uu, we.vv, we.ww, we.prs, we.tmp};
+          if (irk == 1 && cb[b].face[0][0] == ORC_FACE_INLET)
+            for (int k = 1; k <= d[b].nz; k++)
+              for (int j = 1; j <= d[b].ny; j++) {
+                const long p = (long)ID(d[b], 1, j, k);
+                for (int q = 0; q < 5; q++) pw[q][p] = pst[b][q][p];
+              }
+          /* bc_non_reflecting after `primitives` (runge_kutta.f90:30-37, 64-70) */
+          if (irk > 1) {
+            bc_charac(&cb[b], &g[b], &d[b], Un[b], &we, cst[b], K[b]);
+            bc_inlet(&cb[b], &d[b], Un[b], Uin[b]);
+          }
+        }
+        euler_rhs_w(&cb[b], &g[b], &d[b], Un[b], &we, K[b]);
+        for (int q = 0; q < 5; q++) free(tmp5[q]);
+        if (last && c0->visc) {
           visc_rhs_w(&cb[b], &g[b], &d[b], &w[b], K[b]);
-        } else {
-          prim_euler(&cb[b], &d[b], Un[b], &w[b]);
-          euler_rhs_w(&cb[b], &g[b], &d[b], Un[b], &w[b], K[b]);
+        } else if (last) {
           /* the last stage calls primitives_visc(U^n) whether or not the case is
            * viscous (runge_kutta.f90:71; only grad_vel and flux_visc are void
            * without viscosity): the primitives bc_wall finds at the end of the
            * step are those of U^n */
-          if (last) prim_euler(&cb[b], &d[b], &U[5 * b], &w[b]);
+          prim_visc(&cb[b], &d[b], &U[5 * b], &w[b]);
         }
+        /* primitives_visc's sound speed of U^n: bc_charac_imax reads it until the
+         * next step's last stage */
+        if (last && chb[b]) memcpy(cst[b], w[b].cs, d[b].n * sizeof(double));
         const double alpha = c0->dt * c0->crk[irk - 1];
         const dims_t dd = d[b];
 #pragma omp parallel for collapse(2)
@@ -1184,13 +1402,29 @@ int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_gri
     }
     /* bc_wall at the end of the step (runge_kutta.f90:130-136): from the primitives
      * of U^n, which the last stage's primitives_visc wrote */
-    for (int b = 0; b < nb; b++) bc_wall(&cb[b], &g[b], &d[b], Un[b], &w[b]);
+    for (int b = 0; b < nb; b++) bc_wall(&cb[b], &g[b], &d[b], Un[b], &w[b], K[b]);
+    /* then `primitives` and bc_non_reflecting (:137-139): the increments of the
+     * characteristic rows for the next step's first stage, the inlet reset */
+    for (int b = 0; b < nb; b++) {
+      if (!chb[b]) continue;
+      prim_euler(&cb[b], &d[b], Un[b], &w[b]);
+      bc_charac(&cb[b], &g[b], &d[b], Un[b], &w[b], cst[b], K[b]);
+      double *pw[5] = {w[b].uu, w[b].vv, w[b].ww, w[b].prs, w[b].tmp};
+      for (int q = 0; q < 5; q++) memcpy(pst[b][q], pw[q], d[b].n * sizeof(double));
+      bc_inlet(&cb[b], &d[b], Un[b], Uin[b]);
+    }
     for (int b = 0; b < nb; b++)
       for (int v = 0; v < 5; v++) memcpy(U[5 * b + v], Un[b][v], d[b].n * sizeof(double));
   }
 out:
   for (int b = 0; b < nb; b++) {
-    for (int v = 0; v < 5; v++) { free(Un[b][v]); free(K[b][v]); }
+    for (int v = 0; v < 5; v++) {
+      free(Un[b][v]);
+      free(K[b][v]);
+      free(pst[b][v]);
+      free(Uin[b][v]);
+    }
+    free(cst[b]);
     work_free(&w[b]);
   }
   return rc;

@@ -30,7 +30,12 @@ extern "C" {
 enum orc_face_kind {
   ORC_FACE_PERIODIC = 0,   /* neighbour block is the block itself          */
   ORC_FACE_INTERFACE = 1,  /* 1-to-1 interface with another block          */
-  ORC_FACE_WALL = 2        /* no-slip wall, sort=0 (is_wall2 = F): Cartesian */
+  ORC_FACE_WALL = 2,       /* no-slip wall, sort=0 (is_wall2 = F): Cartesian */
+  ORC_FACE_CHARAC = 3,     /* characteristic (LODI) outflow, sort=-3 at imax or jmax:
+                            * bc_charac_imax / bc_charac_jmax, src/Boundary/mod_charac.f90:350-653,
+                            * 816-969 (Cartesian)                              */
+  ORC_FACE_INLET = 4       /* supersonic inlet, sort=-3 at imin with Mach >= 1:
+                            * bc_supersonic_inlet, mod_charac.f90:997-1149    */
 };
 
 typedef struct orc_cfg {
@@ -55,6 +60,8 @@ typedef struct orc_cfg {
   int adiabatic;           /* walls: 1 adiabatic (dT/dn = 0), 0 isothermal  */
   int sw_edoh;              /* shock filter: Edoh switch (is_sw_edoh)         */
   double t_wall;           /* wall temperature of isothermal walls          */
+  double p_ref;            /* reference pressure: the subsonic-outflow relaxation of
+                            * bc_charac_imax, L1 = 0.1 (p - 0.528 p_ref)      */
 } orc_cfg;
 
 typedef struct orc_grid {
@@ -89,7 +96,8 @@ typedef struct orc_block {
   int nbr_block[6];
   int nbr_face[6];
   int orient[6];
-  int wall[6];             /* 1: the face is a wall (nbr_* ignored)         */
+  int wall[6];             /* the face is a boundary (nbr_* ignored): 1 no-slip wall,
+                            * 2 characteristic outflow (imax, jmax), 3 supersonic inlet (imin) */
 } orc_block;
 
 /* Fill the face ghosts of every block from its neighbours' interior

@@ -33,6 +33,7 @@ class OrcCfg(C.Structure):
         ("cok_ratio", C.c_double),
         ("chi", C.c_double), ("dissip_shock", C.c_double), ("csens", C.c_double),
         ("adiabatic", C.c_int), ("sw_edoh", C.c_int), ("t_wall", C.c_double),
+        ("p_ref", C.c_double),
     ]
 
 
@@ -117,9 +118,11 @@ class Oracle:
         for d in range(3):
             for s in range(2):
                 c.face[d][s] = faces[2 * d + s]
-        self.walls = [int(k == 2) for k in faces]
+        # boundary faces (wall 2, characteristic outflow 3, supersonic inlet 4)
+        self.walls = [int(k >= 2) for k in faces]
         c.adiabatic = int(getattr(case, "adiabatic", True))
         c.t_wall = float(getattr(case, "t_wall", 0.0))
+        c.p_ref = float(getattr(case, "p_ref", 0.0))
         for q, v in enumerate(case.crk):
             c.crk[q] = v
         c.dt = case.dt
@@ -211,11 +214,13 @@ class Oracle:
 
 
 def self_block(case):
-    """One-block table: periodic faces are interfaces with the block itself,
-    faces of kind 2 walls."""
+    """One-block table: periodic faces are interfaces with the block itself;
+    faces of kind 2 (wall), 3 (characteristic outflow), 4 (supersonic inlet)
+    are boundaries of that kind."""
     faces = tuple(getattr(case, "faces", (0,) * 6))
     return dict(nx=case.nx, ny=case.ny, nz=case.nz, nbr_block=[0] * 6,
-                nbr_face=[f ^ 1 for f in range(6)], orient=[0] * 6, wall=[int(k == 2) for k in faces])
+                nbr_face=[f ^ 1 for f in range(6)], orient=[0] * 6,
+                wall=[max(0, int(k) - 1) for k in faces])
 
 
 def make_blocks(blocks):

@@ -15,6 +15,13 @@ PLANTS = {
     1: ("tests/test_shock_sensor_oracle.py", "test_pressure_sensor_on_a_step"),
     2: ("tests/test_shock_sensor_oracle.py", "test_interior_face_amplitude_with_an_active_sensor"),
     3: ("tests/test_shock_sensor_oracle.py", "test_ducros_sensor_with_divergence"),
+    # characteristic boundaries: the textbook sign of the imax density equation,
+    # 0.5 for the 0.528 of the subsonic relaxation, v for v + c in the jmax
+    # outgoing wave, the a04 rows for the a40 ones at imax
+    4: ("tests/test_charac_oracle.py", "test_imax_subsonic_relaxation_at_rest"),
+    5: ("tests/test_charac_oracle.py", "test_imax_subsonic_relaxation_at_rest"),
+    6: ("tests/test_charac_oracle.py", "test_jmax_subsonic_outflow_and_inflow"),
+    7: ("tests/test_charac_oracle.py", "test_imax_supersonic_outflow_and_closures"),
 }
 
 
