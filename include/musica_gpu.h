@@ -68,6 +68,28 @@ extern "C" {
  * the wall row's pressure sensor reads p there (mod_filtering_shock.f90:
  * 243-260 with ndy_d1 = 1), and nothing on the path writes it. */
 #define MGPU_FACE_WALL 2
+/* Characteristic (LODI) outflow, BC_face%sort = -3 at imax or jmax:
+ * bc_charac_imax / bc_charac_jmax (src/Boundary/mod_charac.f90:350-653,
+ * 816-969) for a perfect gas.  The boundary row gets no Euler or viscous
+ * increment; its increment comes from one-sided 5-pt derivatives of the
+ * stage state's primitives, computed after `primitives` in stages 2..nrk and
+ * at the end of the step (runge_kutta.f90:30-37, 64-70, 137-139), and kept
+ * for the next stage.  At imax the sound speed is the one primitives_visc
+ * last computed (U^n of the previous step's last stage), and a subsonic
+ * outflow relaxes p towards 0.528*p_ref (mgpu_case.p_ref). */
+#define MGPU_FACE_CHARAC 3
+/* Supersonic inlet, BC_face%sort = -3 at imin: bc_supersonic_inlet without
+ * eigenmodes (mod_charac.f90:1133-1146).  Row 1 is held at the inflow state,
+ * which is row 1 of the state given by mgpu_put_state (init_BC_ref with
+ * is_BCref_init, mod_init_flow.f90:418-440).
+ * Characteristic and inlet faces: Cartesian cases, blocks whose other
+ * boundary faces are j-walls, filter NONE, SF or SHOCK, the classic stage
+ * kernel, >= 10 cells along a bounded axis.  mgpu_put_state
+ * starts a run as start_runge_kutta does (runge_kutta.f90:316-342): the first
+ * mgpu_step after it derives the characteristic increments, the sound speed
+ * and the inflow state from the uploaded state.  Like walls, the ghost row
+ * beyond such a face must hold a valid state when the shock filter runs. */
+#define MGPU_FACE_INLET 4
 
 /* Filter selection: is_SF / is_shock (src/Main/assign_procedures.f90:161-219) */
 #define MGPU_FILTER_NONE 0
@@ -141,6 +163,8 @@ typedef struct mgpu_case {
                           /* local (|u_d|+c) dt / dx_d of U^n (contravariant on */
                           /* c3 blocks) (mod_filtering_shock.f90:620-700)       */
   double t_wall;          /* walls: temperature of an isothermal wall (T_wall) */
+  double p_ref;           /* characteristic imax outflow: reference pressure of */
+                          /* its subsonic relaxation (p_ref, mod_charac.f90:426) */
 } mgpu_case;
 
 typedef struct mgpu_handle mgpu_handle;

@@ -16,6 +16,8 @@ MGPU_NGH = 5
 FACE_PERIODIC = 0
 FACE_INTERFACE = 1
 FACE_WALL = 2
+FACE_CHARAC = 3
+FACE_INLET = 4
 FILTER_NONE, FILTER_SF, FILTER_SHOCK, FILTER_ARTVISC, FILTER_ARTVISC_SHOCK = 0, 1, 2, 3, 4
 PREC_FP64, PREC_MIXED = 0, 1
 GEOM_CART, GEOM_C3, GEOM_C2 = 0, 1, 2
@@ -92,6 +94,7 @@ class MgpuCase(C.Structure):
         ("is_adiab", C.c_int32),
         ("is_sw_edoh", C.c_int32),
         ("t_wall", C.c_double),
+        ("p_ref", C.c_double),
     ]
 
 

@@ -47,26 +47,30 @@ __global__ __launch_bounds__(kDiagThreads) void k_diag(DiagArgs a) {
 #pragma unroll
       for (int c = 0; c < 3; c++) gd[d][c] = A5_1 * (a1[c] - b1[c]) + A5_2 * (a2[c] - b2[c]);
     }
-    // rows 1, 2 (ny-1, ny) next to a no-slip j-wall: the one-sided a04 / a13
-    // (a31 / a40) stencils of grad_vel (smod_grad_vel.f90, is_boundary), which
-    // read nothing beyond the wall
-    const bool wlo = (g.jwall & 1) && j <= 2, whi = (g.jwall & 2) && j >= g.ny - 1;
-    if (wlo || whi) {
-      double f[5][3];  // rows j0..j0+4 of the stencil
-      const int j0 = wlo ? 1 : g.ny - 4;
+    // rows 1, 2 (n-1, n) next to a bounded i- or j-face (wall, characteristic
+    // outflow, supersonic inlet): the one-sided a04 / a13 (a31 / a40) stencils
+    // of grad_vel (smod_grad_vel.f90, is_boundary), which read nothing beyond
+    // the face
 #pragma unroll
-      for (int l = 0; l < 5; l++) vel(a.u, lin(g, i, j0 + l, k), f[l]);
+    for (int d = 0; d < 2; d++) {
+      const int bnd = d ? g.jbnd : g.ibnd, n = d ? g.ny : g.nx, r = d ? j : i;
+      const bool wlo = (bnd & 1) && r <= 2, whi = (bnd & 2) && r >= n - 1;
+      if (!(wlo || whi)) continue;
+      double f[5][3];  // rows r0..r0+4 of the stencil
+      const int r0 = wlo ? 1 : n - 4;
+#pragma unroll
+      for (int l = 0; l < 5; l++) vel(a.u, d ? lin(g, i, r0 + l, k) : lin(g, r0 + l, j, k), f[l]);
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        if (wlo && j == 1)
-          gd[1][c] = A04_1 * f[0][c] + A04_2 * f[1][c] + A04_3 * f[2][c] + A04_4 * f[3][c] + A04_5 * f[4][c];
+        if (wlo && r == 1)
+          gd[d][c] = A04_1 * f[0][c] + A04_2 * f[1][c] + A04_3 * f[2][c] + A04_4 * f[3][c] + A04_5 * f[4][c];
         else if (wlo)
-          gd[1][c] = A13_1 * f[0][c] + A13_2 * f[1][c] + A13_3 * f[2][c] + A13_4 * f[3][c] + A13_5 * f[4][c];
-        else if (j == g.ny - 1)
-          gd[1][c] = (-A13_5) * f[0][c] + (-A13_4) * f[1][c] + (-A13_3) * f[2][c] + (-A13_2) * f[3][c] +
+          gd[d][c] = A13_1 * f[0][c] + A13_2 * f[1][c] + A13_3 * f[2][c] + A13_4 * f[3][c] + A13_5 * f[4][c];
+        else if (r == n - 1)
+          gd[d][c] = (-A13_5) * f[0][c] + (-A13_4) * f[1][c] + (-A13_3) * f[2][c] + (-A13_2) * f[3][c] +
                      (-A13_1) * f[4][c];
         else
-          gd[1][c] = (-A04_5) * f[0][c] + (-A04_4) * f[1][c] + (-A04_3) * f[2][c] + (-A04_2) * f[3][c] +
+          gd[d][c] = (-A04_5) * f[0][c] + (-A04_4) * f[1][c] + (-A04_3) * f[2][c] + (-A04_2) * f[3][c] +
                      (-A04_1) * f[4][c];
       }
     }

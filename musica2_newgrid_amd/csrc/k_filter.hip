@@ -10,10 +10,11 @@
 //                      shock filter's face dissipation without the sensor,
 //                      weighted by the spectral radius |u_d| + c of U^n
 //                      averaged to the faces, times the metric, U += deltat*K
-// Interior rows of periodic faces and 1-to-1 interfaces; with no-slip walls
-// on j-faces the selective filter leaves rows 1,2 (ny-1,ny) unfiltered along
-// j and uses the 5/7/9-pt filters on rows 3,4,5 (mod_filtering.f90:900-988,
-// 1116-1190); along i and k every row is filtered.
+// Interior rows of periodic faces and 1-to-1 interfaces; next to boundary
+// faces on i or j (walls, characteristic outflow, supersonic inlet) the
+// selective filter leaves rows 1,2 (n-1,n) unfiltered along that axis and
+// uses the 5/7/9-pt filters on rows 3,4,5 (mod_filtering.f90:900-988,
+// 1116-1190); along k every row is filtered.
 //
 // 2.5-D layout: the state of the tile + 5-cell halos of the current k plane
 // sits in LDS for the i/j stencils, the own column keeps an 11-deep register
@@ -201,12 +202,19 @@ __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__
     }
     if (own) {
       const int hw = WALL ? jhalf(g, j) : 5;
+      const int hwi = WALL ? ihalf(g, i) : 5;
 #pragma unroll
       for (int v = 0; v < 5; v++) {
         const R *x = &sux[v][ty][tx + 5];
         const R *y = &suy[v][ty + 5][tx];
-        R K = dd[0] * x[0] + dd[1] * (x[1] + x[-1]) + dd[2] * (x[2] + x[-2]) + dd[3] * (x[3] + x[-3]) +
-                   dd[4] * (x[4] + x[-4]) + dd[5] * (x[5] + x[-5]);
+        R K;
+        if (!WALL || hwi == 5) {
+          K = dd[0] * x[0] + dd[1] * (x[1] + x[-1]) + dd[2] * (x[2] + x[-2]) + dd[3] * (x[3] + x[-3]) +
+              dd[4] * (x[4] + x[-4]) + dd[5] * (x[5] + x[-5]);
+        } else {
+          K = R(0);
+          if constexpr (WALL) K = fwall<R>(x, 1, hwi, a.chi);
+        }
         if (!WALL || hw == 5) {
           K = K + dd[0] * y[0] + dd[1] * (y[FX] + y[-FX]) + dd[2] * (y[2 * FX] + y[-2 * FX]) +
               dd[3] * (y[3 * FX] + y[-3 * FX]) + dd[4] * (y[4 * FX] + y[-4 * FX]) +
@@ -251,17 +259,19 @@ __device__ __forceinline__ R d10face(const R *x, int s) {
          R(D10_4) * (x[4 * s] - x[-3 * s]) + R(D10_5) * (x[5 * s] - x[-4 * s]);
 }
 
-// Half-width of the shock filter's j-face m+1/2 (between rows m and m+1) of a
-// block with walls: the d2/d4/d6/d8 faces are the first four from a wall, the
-// wall face with a tenth of the sensor term (filter_o10_shock,
-// mod_filtering_shock.f90:724-870 jmin, :915-1057 jmax), d10 inside; <= 0: no
-// face (the wall rows get no j-increment, :1128-1140 runs over ndy_d..nfy_d).
-__device__ __forceinline__ int face_half(const Geom &g, int m) {
+// Half-width of the shock filter's face m+1/2 (between rows m and m+1) along
+// an axis of n rows with bounded faces bnd: the d2/d4/d6/d8 faces are the
+// first four from a boundary, the boundary face with a tenth of the sensor
+// term (filter_o10_shock, mod_filtering_shock.f90:266-466 i, :724-1057 j),
+// d10 inside; <= 0: no face (the boundary rows get no increment along that
+// axis, :1128-1140 runs over ndy_d..nfy_d).
+__device__ __forceinline__ int face_half_n(int bnd, int n, int m) {
   int h = 5;
-  if ((g.jwall & 1) && m < h) h = m;
-  if ((g.jwall & 2) && g.ny - m < h) h = g.ny - m;
+  if ((bnd & 1) && m < h) h = m;
+  if ((bnd & 2) && n - m < h) h = n - m;
   return h;
 }
+__device__ __forceinline__ int face_half(const Geom &g, int m) { return face_half_n(g.jbnd, g.ny, m); }
 
 // Edoh switch factor of direction d at element p (position r along d):
 // min(1, CFL/chi) with CFL = sr_d deltat, times idx_d(r) on Cartesian blocks
@@ -286,8 +296,9 @@ __device__ __forceinline__ double edoh_mul(const FilterArgs &a, int d, idx_t p, 
 // direction's K = (sr_{-1/2} D_{-1/2} - sr_{+1/2} D_{+1/2}) * metric with the
 // spectral radius of U^n averaged to the faces (:480-550 i, :866-935 j,
 // :1243-1283 k), U_n += deltat*K (update_varn_artvisc).
-// WALL: blocks with j-walls; rows 1..5 and ny-4..ny get no j-increment here
-// (k_shock_wall adds it with the reduced faces of the wall closure).
+// WALL: blocks with boundary faces on i or j; rows 1..5 and n-4..n of a
+// bounded axis get no increment along it here (k_shock_wall adds it with the
+// reduced faces of the boundary closure).
 // EDOH: the shock filter's Edoh switch, each direction's increment times
 // min(1, CFL_d / chi) with the local CFL_d = sr_d deltat (idx_d on Cartesian
 // blocks) (mod_filtering_shock.f90:620-700, :1059-1140, :1450-1496).
@@ -539,8 +550,11 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
     }
     if (!ARTV && own) {
       R K[5];
-      // i faces i-1/2 and i+1/2
-      {
+      // i faces i-1/2 and i+1/2 (bounded i: rows 1..5 and nx-4..nx take theirs from k_shock_wall)
+      if (WALL && (face_half_n(g.ibnd, g.nx, i - 1) < 5 || face_half_n(g.ibnd, g.nx, i) < 5)) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) K[v] = R(0);
+      } else {
         const double pm = psx[ty][tx], p0 = psx[ty][tx + 1], pp = psx[ty][tx + 2];
         const double xk0m = a.xnu0 * fmax(p0, pm), xk10m = fmax(0.0, a.xnu10 + xk0m / 105.0);
         const double xk0p = a.xnu0 * fmax(pp, p0), xk10p = fmax(0.0, a.xnu10 + xk0p / 105.0);
@@ -646,31 +660,35 @@ __constant__ double kShockD[6][5] = {{0.0, 0.0, 0.0, 0.0, 0.0},     {-1.0, 0.0, 
 
 constexpr int kShockWallThreads = 256;
 
-// Shock filter j-increment of the rows next to no-slip j-walls, added to the
-// output of k_filter_shock<.., WALL> (which leaves rows 1..5 and ny-4..ny
-// without one): faces 1..4 from a wall use d2/d4/d6/d8 with amplitude
-// max(0, xnu + xk0), the wall face a tenth of the sensor term
-// (filter_o10_shock, mod_filtering_shock.f90:724-870 jmin, :915-1057 jmax);
-// the wall rows get none (:1128-1140 over ndy_d..nfy_d).  The pressure sensor
-// of row 1 (ny) reads p one row beyond the wall.  One thread per point of
-// rows 2..5 and ny-4..ny-1; nothing beyond a wall is read but p.
-// a.avs: artvisc_o9_shock, whose faces 3 and 4 from a wall take xk0/5 and
-// 3/70 xk0 (mod_artvisc_shock.f90:577-695 jmin, :744-862 jmax) and whose
-// increment is deltat (sr_{-1/2} D_{-1/2} - sr_{+1/2} D_{+1/2}) idy(j).
-__global__ __launch_bounds__(kShockWallThreads) void k_shock_wall(FilterArgs a) {
+// Shock filter increment along axis ax (0: i, 1: j) of the rows next to the
+// bounded faces of that axis, added to the output of k_filter_shock<.., WALL>
+// (which leaves rows 1..5 and n-4..n of a bounded axis without one): faces
+// 1..4 from a boundary use d2/d4/d6/d8 with amplitude max(0, xnu + xk0), the
+// boundary face a tenth of the sensor term (filter_o10_shock,
+// mod_filtering_shock.f90:266-466 i, :724-870 jmin, :915-1057 jmax); the
+// boundary rows get none (:1128-1140 over ndy_d..nfy_d).  The pressure
+// sensor of row 1 (n) reads p one row beyond the face.  One thread per point
+// of rows 2..5 and n-4..n-1 of each bounded side; nothing beyond a face is
+// read but p.
+// a.avs: artvisc_o9_shock (j only), whose faces 3 and 4 from a wall take
+// xk0/5 and 3/70 xk0 (mod_artvisc_shock.f90:577-695 jmin, :744-862 jmax) and
+// whose increment is deltat (sr_{-1/2} D_{-1/2} - sr_{+1/2} D_{+1/2}) idy(j).
+__global__ __launch_bounds__(kShockWallThreads) void k_shock_wall(FilterArgs a, int ax) {
   const Geom &g = a.g;
-  const long nface = (long)g.nx * g.nz;
+  const int bnd = ax ? g.jbnd : g.ibnd, n = ax ? g.ny : g.nx;
+  const int n1 = ax ? g.nx : g.ny;  // the other in-plane axis
+  const long nface = (long)n1 * g.nz;
   const long t = (long)blockIdx.x * kShockWallThreads + threadIdx.x;
   if (t >= 8 * nface) return;
   const int q = (int)(t / nface);
   const int side = q >> 2;
-  if (!((g.jwall >> side) & 1)) return;
+  if (!((bnd >> side) & 1)) return;
   const long r = t - q * nface;
-  const int i = 1 + (int)(r % g.nx), k = 1 + (int)(r / g.nx);
-  const int j = side ? g.ny - 4 + (q & 3) : 2 + (q & 3);
-  const idx_t sj = (idx_t)g.sx;
-  const idx_t pj = lin(g, i, j, k);
-  // pressure sensors of rows j-1, j, j+1 (faces j-1/2, j+1/2)
+  const int a1 = 1 + (int)(r % n1), k = 1 + (int)(r / n1);
+  const int m0 = side ? n - 4 + (q & 3) : 2 + (q & 3);  // row along ax
+  const idx_t sj = ax ? (idx_t)g.sx : (idx_t)1;
+  const idx_t pj = ax ? lin(g, a1, m0, k) : lin(g, m0, a1, k);
+  // pressure sensors of rows m0-1, m0, m0+1 (faces m0-1/2, m0+1/2)
   double ps[3];
 #pragma unroll
   for (int l = 0; l < 3; l++) {
@@ -681,8 +699,8 @@ __global__ __launch_bounds__(kShockWallThreads) void k_shock_wall(FilterArgs a) 
   double D[2][5];
 #pragma unroll
   for (int f = 0; f < 2; f++) {
-    const int m = j - 1 + f;
-    const int hf = face_half(g, m);
+    const int m = m0 - 1 + f;
+    const int hf = face_half_n(bnd, n, m);
     // no sensor term (artvisc_o9): p beyond the wall is not read into it
     double xk0 = a.xnu0 == 0.0 ? 0.0 : a.xnu0 * fmax(ps[f + 1], ps[f]);
     if (hf == 1) xk0 = xk0 / 10.0;
@@ -707,26 +725,27 @@ __global__ __launch_bounds__(kShockWallThreads) void k_shock_wall(FilterArgs a) 
   if (a.avs) {
     const double s0 = ldg(a.sr[1], pj);
     const double srp = 0.5 * (ldg(a.sr[1], pj + sj) + s0), srm = 0.5 * (ldg(a.sr[1], pj - sj) + s0);
-    const double m = met_at(a.idy, j);
+    const double m = met_at(a.idy, m0);
 #pragma unroll
     for (int v = 0; v < 5; v++)
       stg(a.uout.v[v], pj, ldg(a.uout.v[v], pj) + a.dt * ((srm * D[0][v] - srp * D[1][v]) * m));
     return;
   }
-  const double mul = a.edoh ? edoh_mul(a, 1, pj, j) : 1.0;
+  const double mul = a.edoh ? edoh_mul(a, ax, pj, m0) : 1.0;
 #pragma unroll
   for (int v = 0; v < 5; v++) {
-    const double ky = a.edoh ? mul * (D[0][v] - D[1][v]) : D[0][v] - D[1][v];
-    stg(a.uout.v[v], pj, ldg(a.uout.v[v], pj) + ky);
+    const double kd = a.edoh ? mul * (D[0][v] - D[1][v]) : D[0][v] - D[1][v];
+    stg(a.uout.v[v], pj, ldg(a.uout.v[v], pj) + kd);
   }
 }
 
 }  // namespace
 
-hipError_t launch_shock_wall(const FilterArgs &a, hipStream_t s) {
-  const long n = 8L * a.g.nx * a.g.nz;
+hipError_t launch_shock_wall(const FilterArgs &a, int axis, hipStream_t s) {
+  if (axis == 0 && a.avs) return hipErrorInvalidValue;  // artvisc_o9_shock: j-walls only
+  const long n = 8L * (axis ? a.g.nx : a.g.ny) * a.g.nz;
   hipLaunchKernelGGL(k_shock_wall, dim3((unsigned)((n + kShockWallThreads - 1) / kShockWallThreads)),
-                     dim3(kShockWallThreads), 0, s, a);
+                     dim3(kShockWallThreads), 0, s, a, axis);
   return hipGetLastError();
 }
 

@@ -12,10 +12,18 @@
 //                           for its whole k-march; z is Cartesian)
 //   flux_visc_5pts(_c3) divergence part  smod_flux_visc_5pts.f90:119-130, _c3.f90:262-
 //   update_var              src/Time/update.f90:32-44
-// With no-slip walls on j-faces (WALL): no Euler increment on the wall rows
-// (ndx_e = 2, mpi_index_bounds.f90:20-70) and the centred a3/a5/a7/a9
-// closures of rows 2..5 (smod_flux_euler_11pts.f90:267-595 with
-// is_bc_1pt), whose metric is the idy entry of the row.
+// With boundary faces on i or j (WALL: no-slip walls, characteristic outflow,
+// supersonic inlet): no Euler increment on the boundary rows (ndx_e = 2,
+// mpi_index_bounds.f90:20-70) and the centred a3/a5/a7/a9 closures of rows
+// 2..5 (smod_flux_euler_11pts.f90:36-133 along i, :267-595 along j, with
+// is_bc_1pt), whose metric is the idx / idy entry of the row.  A wall row
+// keeps K = 0 plus the viscous increment (bc_wall, smod_bc_wall.f90:226-232);
+// the row of a characteristic or inlet face takes the increment its boundary
+// routine wrote (bc_charac_imax / _jmax, mod_charac.f90:350-469, 816-969) or,
+// at the inlet, the filter increment of the previous step that K still holds
+// there.  The i-flux of an inlet row mixes the inflow state's conserved
+// variables with the primitives computed before it was imposed
+// (bc_supersonic_inlet after `primitives`, runge_kutta.f90:30-37, 64-70).
 //
 // Work decomposition (gfx950): a TX x TY workgroup owns a TX(i) x TY(j)
 // column tile and marches a chunk of k planes.  For each plane the i- and
@@ -75,6 +83,8 @@ struct Ptrs {
   const double *kv[4];
   const double *idx, *idy, *idz;
   double ref[5];
+  const double *kbc[4];
+  const double *uin_row, *prim_row;
 };
 
 template <int GEO, bool VISC, bool IN32, bool OUT32>
@@ -331,21 +341,31 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
       if (own) ijac = ldg(P.m2[4], p2);
     }
   }
-  // walls: j-pencil entries beyond a wall hold 0 (their state is never valid),
-  // and the row's closure coefficients
+  // bounded faces: pencil entries beyond a boundary face hold 0 (their state
+  // is never valid), and the row's closure coefficients along j (cw) and i (ci)
   bool hs_beyond[NS];
-  bool own_beyond = false;
-  double cw[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  int hw = 5;
+  bool own_beyond = false, own_beyond_x = false;
+  double cw[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ci[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  int hw = 5, hwi = 5;
   if constexpr (WALL) {
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-      const int gj = j0 - 5 + hs_r[s];
-      hs_beyond[s] = hs_dir[s] == 1 && (((g.jwall & 1) && gj < 1) || ((g.jwall & 2) && gj > g.ny));
+      const int gj = j0 - 5 + hs_r[s], gi = i0 - 5 + hs_c[s];
+      hs_beyond[s] = hs_dir[s] == 1 ? (((g.jbnd & 1) && gj < 1) || ((g.jbnd & 2) && gj > g.ny))
+                                    : (((g.ibnd & 1) && gi < 1) || ((g.ibnd & 2) && gi > g.nx));
     }
-    own_beyond = (g.jwall & 2) && j > g.ny;
+    own_beyond = (g.jbnd & 2) && j > g.ny;
+    own_beyond_x = (g.ibnd & 2) && i > g.nx;
     hw = jhalf(g, j);
     wall_coeffs(hw, cw);
+    hwi = ihalf(g, i);
+    wall_coeffs(hwi, ci);
+    P.kbc[0] = a.kbc[0];
+    P.kbc[1] = a.kbc[1];
+    P.kbc[2] = a.kbc[2];
+    P.kbc[3] = a.kbc[3];
+    P.uin_row = a.uin_row;
+    P.prim_row = a.prim_row;
   } else {
 #pragma unroll
     for (int s = 0; s < NS; s++) hs_beyond[s] = false;
@@ -411,9 +431,26 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
       const Prim pq = prim_raw(ru, gam1);
       R F[5];
       if (needx) {
-        flux_r<CURV, MIX>(pq, 0, rxm, rf, F);
+        if (WALL && i == 1 && P.uin_row) {
+          // supersonic inlet: row 1's conserved variables are the inflow
+          // state, its primitives those computed before the inflow state was
+          // imposed (bc_supersonic_inlet runs after `primitives`)
+          const idx_t sl = (idx_t)(j - 1) + (idx_t)g.ny * (idx_t)(k - 1), ss = (idx_t)g.ny * (idx_t)g.nz;
+          double uu[5];
 #pragma unroll
-        for (int v = 0; v < 5; v++) sfx[v][ty][tx + 5] = F[v];
+          for (int v = 0; v < 5; v++) uu[v] = P.prim_row ? ldg(P.prim_row, (idx_t)v * ss + sl) : ru[v];
+          Prim pi = prim_raw(uu, gam1);
+          pi.r = ldg(P.uin_row, sl);
+          pi.ru = ldg(P.uin_row, ss + sl);
+          pi.rv = ldg(P.uin_row, 2 * ss + sl);
+          pi.rw = ldg(P.uin_row, 3 * ss + sl);
+          pi.re = ldg(P.uin_row, 4 * ss + sl);
+          flux_r<CURV, MIX>(pi, 0, rxm, rf, F);
+        } else {
+          flux_r<CURV, MIX>(pq, 0, rxm, rf, F);
+        }
+#pragma unroll
+        for (int v = 0; v < 5; v++) sfx[v][ty][tx + 5] = (WALL && own_beyond_x) ? R(0) : F[v];
       }
       if (needy) {
         flux_r<CURV, MIX>(pq, 1, rym, rf, F);
@@ -426,7 +463,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
       flux_halo<CURV, MIX>(prim_raw(rh0.u, gam1), hs_dir[0], rh0.m, rf, F);
       if (hs_dir[0] == 0) {
 #pragma unroll
-        for (int v = 0; v < 5; v++) sfx[v][hs_r[0]][hs_c[0]] = F[v];
+        for (int v = 0; v < 5; v++) sfx[v][hs_r[0]][hs_c[0]] = (WALL && hs_beyond[0]) ? R(0) : F[v];
       } else {
 #pragma unroll
         for (int v = 0; v < 5; v++) sfy[v][hs_r[0]][hs_c[0]] = (WALL && hs_beyond[0]) ? R(0) : F[v];
@@ -453,7 +490,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
           flux_halo<CURV, MIX>(prim_raw(rh[s].u, gam1), hs_dir[s], rh[s].m, rf, F);
           if (hs_dir[s] == 0) {
 #pragma unroll
-            for (int v = 0; v < 5; v++) sfx[v][hs_r[s]][hs_c[s]] = F[v];
+            for (int v = 0; v < 5; v++) sfx[v][hs_r[s]][hs_c[s]] = (WALL && hs_beyond[s]) ? R(0) : F[v];
           } else {
 #pragma unroll
             for (int v = 0; v < 5; v++) sfy[v][hs_r[s]][hs_c[s]] = (WALL && hs_beyond[s]) ? R(0) : F[v];
@@ -470,7 +507,14 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
       double K[5];
 #pragma unroll
       for (int v = 0; v < 5; v++) {
-        const R dx = d11c(&sfx[v][ty][tx + 5], 1);
+        R dx;
+        if constexpr (WALL) {
+          const R *f = &sfx[v][ty][tx + 5];
+          dx = R(ci[0]) * (f[1] - f[-1]) + R(ci[1]) * (f[2] - f[-2]) + R(ci[2]) * (f[3] - f[-3]) +
+               R(ci[3]) * (f[4] - f[-4]) + R(ci[4]) * (f[5] - f[-5]);
+        } else {
+          dx = d11c(&sfx[v][ty][tx + 5], 1);
+        }
         R dy;
         if constexpr (WALL) {
           const R *f = &sfy[v][ty + 5][tx];
@@ -493,12 +537,34 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
           const double ky = (double)dy * ldg(P.idy, (idx_t)(j - 1 + G)) + kx;
           K[v] = (double)dz * ldg(P.idz, (idx_t)(k - 1 + G)) + ky;
         }
-        // wall rows: no Euler increment (bc_wall zeroes it, flux_euler skips the row)
-        if (WALL && hw == 0) K[v] = 0.0;
+      }
+      // boundary rows: no Euler increment (flux_euler skips them); a wall row
+      // keeps 0 (bc_wall zeroes it) plus the viscous increment, a
+      // characteristic or inlet row takes its face slab, set by the boundary
+      // routines (no viscous increment there either: ndx_v = 2)
+      bool kslab = false;
+      if constexpr (WALL) {
+        if (hwi == 0 || hw == 0) {
+          const bool ia = hwi == 0;
+          const double *kb = ia ? (i == 1 ? P.kbc[0] : P.kbc[1]) : (j == 1 ? P.kbc[2] : P.kbc[3]);
+          if (kb) {
+            const idx_t sl = ia ? (idx_t)(j - 1) + (idx_t)g.ny * (idx_t)(k - 1)
+                                : (idx_t)(i - 1) + (idx_t)g.nx * (idx_t)(k - 1);
+            const idx_t ss = (ia ? (idx_t)g.ny : (idx_t)g.nx) * (idx_t)g.nz;
+#pragma unroll
+            for (int v = 0; v < 5; v++) K[v] = ldg(kb, (idx_t)v * ss + sl);
+            kslab = true;
+          } else {
+#pragma unroll
+            for (int v = 0; v < 5; v++) K[v] = 0.0;
+          }
+        }
       }
       if constexpr (VISC) {
+        if (!kslab) {
 #pragma unroll
-        for (int c = 0; c < 4; c++) K[1 + c] = K[1 + c] + kvis[c];
+          for (int c = 0; c < 4; c++) K[1 + c] = K[1 + c] + kvis[c];
+        }
       }
       if constexpr (OUT32) {
         const double ref[5] = {rf.r, rf.ru, rf.rv, rf.rw, rf.re};
@@ -735,7 +801,7 @@ __global__ __launch_bounds__(TX *TY, 1) void k_stage_p(const Batch<StageArgs> *_
       int gi, gj;
       HM::cell(h, i0, j0, g, gi, gj, hs_on[s], hs_jd[s], hs_r[s], hs_c[s]);
       if constexpr (WALL)
-        hs_beyond[s] = hs_jd[s] && (((g.jwall & 1) && gj < 1) || ((g.jwall & 2) && gj > g.ny));
+        hs_beyond[s] = hs_jd[s] && (((g.jbnd & 1) && gj < 1) || ((g.jbnd & 2) && gj > g.ny));
     }
   }
   const idx_t own_off = lin(g, i, j, 0);
@@ -769,7 +835,7 @@ __global__ __launch_bounds__(TX *TY, 1) void k_stage_p(const Batch<StageArgs> *_
   double cw[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   int hw = 5;
   if constexpr (WALL) {
-    own_beyond = (g.jwall & 2) && j > g.ny;
+    own_beyond = (g.jbnd & 2) && j > g.ny;
     hw = jhalf(g, j);
     wall_coeffs(hw, cw);
   }

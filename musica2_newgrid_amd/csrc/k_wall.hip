@@ -1,5 +1,7 @@
-// No-slip walls on the j-faces of Cartesian blocks, as the reference runs them
-// with is_wall2 = F (the default, src/File_IO/read_param.f90:75):
+// Boundary faces of Cartesian blocks: no-slip walls on the j-faces, as the
+// reference runs them with is_wall2 = F (the default,
+// src/File_IO/read_param.f90:75), characteristic outflow at imax / jmax and
+// the supersonic inlet at imin.
 //
 // k_wall        bc_wall_jmin/jmax_adiabatic_dpdn and _isotherm_dpdn
 //               (src/Boundary/smod_bc_wall.f90:191-368): zero momentum on the
@@ -8,13 +10,15 @@
 //               rho = p/(R T) (rocalc_pt_pfg, mod_eos_pfg.f90:303-317),
 //               rhoE = rho cv T (ecalc_tro_pfg, mod_eos_pfg.f90:170-184).
 //
-// k_visc_wall   tau on rows 1,2 (ny-1,ny) with the 5-pt one-sided j-gradients
-//               a04/a13 (a31/a40) of grad_vel_5pts / grad_T_5pts
+// k_visc_wall   tau on rows 1,2 and n-1,n of every bounded i or j axis, with
+//               the 5-pt one-sided gradients a04/a13 (a31/a40) of
+//               grad_vel_5pts / grad_T_5pts along each bounded axis
 //               (src/Compute_flow/smod_grad_vel.f90:12-100 and its j part,
 //               smod_grad_T.f90).  An adiabatic wall has no wall-normal
 //               temperature gradient on its row (smod_grad_T.f90:102-109).
-//               On the wall row itself it stores tau/mu and -(cp/Pr) grad T:
-//               see k_tau_wallrow.
+//               On a wall row it stores tau/mu and -(cp/Pr) grad T: see
+//               k_tau_wallrow.  With the shock filter, also the Ducros sensor
+//               of those rows and p one row beyond each bounded face.
 //
 // k_tau_wallrow after the 2-layer exchange of tau: on the wall row, the
 //               reference evaluates the viscous flux at every point, face
@@ -26,12 +30,19 @@
 //               and the heat-flux gradient exchanged, this kernel rebuilds the
 //               flux of each wall-row point from its own u, v, w and mu(T).
 //
-// k_vdiv_wall   the viscous divergence on rows 1,2 (ny-1,ny) with one-sided
-//               a04/a13 (a31/a40) along j (smod_flux_visc_5pts.f90:80-160, j part).
+// k_vdiv_wall   the viscous divergence on the same rows with one-sided
+//               a04/a13 (a31/a40) along each bounded axis
+//               (smod_flux_visc_5pts.f90:80-160).
 //
-// The main k_visc / k_vdiv compute those rows with the interior stencil
-// first; these kernels overwrite them.  All of them touch one thin slab per
-// wall face, each thread one point.
+// k_charac      the increments of the characteristic rows, bc_charac_imax and
+//               bc_charac_jmax (src/Boundary/mod_charac.f90:350-469, 816-969;
+//               perfect gas, mod_eos_pfg.f90:38-84), the sound speed the imax
+//               routine reads, and the inflow row of bc_supersonic_inlet
+//               (:1133-1146; init_BC_ref's state, mod_init_flow.f90:418-440).
+//
+// The main k_visc / k_vdiv compute the rim rows with the interior stencil
+// first; these kernels overwrite them.  All of them touch thin slabs along
+// the boundary faces, each thread one point.
 #include "mgpu_internal.h"
 
 namespace mgpu {
@@ -75,8 +86,8 @@ __device__ __forceinline__ double sutherland(const Gas &gs, double T) {
   return gs.c0 * sqrt(T * T * T) / (T + gs.S0) * gs.diffscale;
 }
 
-// Coefficients of the one-sided j-derivative of row-slot q (0: row 1, 1: row 2,
-// 2: row ny-1, 3: row ny) on the five rows r0..r0+4 of its side.
+// Coefficients of the one-sided derivative of row-slot q (0: row 1, 1: row 2,
+// 2: row n-1, 3: row n) on the five rows r0..r0+4 of its side.
 __device__ __forceinline__ void onesided(int q, double c[5]) {
   const double a04[5] = {A04_1, A04_2, A04_3, A04_4, A04_5};
   const double a13[5] = {A13_1, A13_2, A13_3, A13_4, A13_5};
@@ -89,14 +100,83 @@ __device__ __forceinline__ void onesided(int q, double c[5]) {
   }
 }
 
-// Row slot q -> (row j, first row r0 of its one-sided stencil); false if
-// that side has no wall.
-__device__ __forceinline__ bool slot_row(const Geom &g, int q, int &j, int &r0) {
-  const int side = q >> 1;
-  if (!((g.jwall >> side) & 1)) return false;
-  j = q == 0 ? 1 : (q == 1 ? 2 : (q == 2 ? g.ny - 1 : g.ny));
-  r0 = side ? g.ny - 4 : 1;
+// One-sided slot of row r (1..n) along an axis whose bounded faces are bnd:
+// 0, 1 for rows 1, 2 next to a bounded min face, 2, 3 for rows n-1, n next to
+// a bounded max face, -1 for the centred stencil.
+__device__ __forceinline__ int oslot(int bnd, int n, int r) {
+  if ((bnd & 1) && r <= 2) return r - 1;
+  if ((bnd & 2) && r >= n - 1) return r == n ? 3 : 2;
+  return -1;
+}
+
+// Point of the rim of a block handled by thread t: section J covers rows 1,2
+// (ny-1,ny) of bounded j-faces over every i; section I rows 1,2 (nx-1,nx) of
+// bounded i-faces over the j that section J leaves.  False if t maps to no
+// point.
+__device__ __forceinline__ bool rim_point(const Geom &g, long t, int &i, int &j, int &k) {
+  const long nfj = (long)g.nx * g.nz, nfi = (long)g.ny * g.nz;
+  if (t < 4 * nfj) {
+    const int q = (int)(t / nfj);
+    if (!((g.jbnd >> (q >> 1)) & 1)) return false;
+    const long r = t - q * nfj;
+    i = 1 + (int)(r % g.nx);
+    k = 1 + (int)(r / g.nx);
+    j = q == 0 ? 1 : (q == 1 ? 2 : (q == 2 ? g.ny - 1 : g.ny));
+    return true;
+  }
+  t -= 4 * nfj;
+  if (t >= 4 * nfi) return false;
+  const int q = (int)(t / nfi);
+  if (!((g.ibnd >> (q >> 1)) & 1)) return false;
+  const long r = t - q * nfi;
+  j = 1 + (int)(r % g.ny);
+  k = 1 + (int)(r / g.ny);
+  if (oslot(g.jbnd, g.ny, j) >= 0) return false;  // section J has it
+  i = q == 0 ? 1 : (q == 1 ? 2 : (q == 2 ? g.nx - 1 : g.nx));
   return true;
+}
+
+// 5-pt first derivatives of u, v, w, T of U^n at (i, j, k) along index axis
+// ax (stride st, row r of n): centred, or one-sided in slot q.
+__device__ __forceinline__ void grad5(const CState &U, const Geom &g, idx_t p, idx_t st, int q, int r, int n,
+                                      double gam1, double rg, double out[4]) {
+  if (q < 0) {
+    const Pw m2 = prim_at(U, p - 2 * st, gam1, rg), m1 = prim_at(U, p - st, gam1, rg);
+    const Pw p1 = prim_at(U, p + st, gam1, rg), p2 = prim_at(U, p + 2 * st, gam1, rg);
+    out[0] = A5_1 * (p1.u - m1.u) + A5_2 * (p2.u - m2.u);
+    out[1] = A5_1 * (p1.v - m1.v) + A5_2 * (p2.v - m2.v);
+    out[2] = A5_1 * (p1.w - m1.w) + A5_2 * (p2.w - m2.w);
+    out[3] = A5_1 * (p1.T - m1.T) + A5_2 * (p2.T - m2.T);
+    return;
+  }
+  (void)g;
+  double c[5];
+  onesided(q, c);
+  const int r0 = q < 2 ? 1 : n - 4;
+  const idx_t p0 = p - (idx_t)(r - r0) * st;
+  out[0] = out[1] = out[2] = out[3] = 0.0;
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const Pw y = prim_at(U, p0 + (idx_t)m * st, gam1, rg);
+    out[0] = out[0] + c[m] * y.u;
+    out[1] = out[1] + c[m] * y.v;
+    out[2] = out[2] + c[m] * y.w;
+    out[3] = out[3] + c[m] * y.T;
+  }
+}
+
+// 5-pt first derivative of f along the index axis of stride st (row r of n):
+// centred, or one-sided in slot q.
+__device__ __forceinline__ double d5at(const double *f, idx_t p, idx_t st, int q, int r, int n) {
+  if (q < 0) return A5_1 * (ldg(f, p + st) - ldg(f, p - st)) + A5_2 * (ldg(f, p + 2 * st) - ldg(f, p - 2 * st));
+  double c[5];
+  onesided(q, c);
+  const int r0 = q < 2 ? 1 : n - 4;
+  const idx_t p0 = p - (idx_t)(r - r0) * st;
+  double d = 0.0;
+#pragma unroll
+  for (int m = 0; m < 5; m++) d = d + c[m] * ldg(f, p0 + (idx_t)m * st);
+  return d;
 }
 
 __global__ __launch_bounds__(kWallThreads) void k_wall(WallArgs a) {
@@ -145,55 +225,65 @@ __global__ __launch_bounds__(kWallThreads) void k_wall(WallArgs a) {
 
 __global__ __launch_bounds__(kWallThreads) void k_visc_wall(WallViscArgs a) {
   const Geom &g = a.g;
-  const long nface = (long)g.nx * g.nz;
-  const long t = (long)blockIdx.x * kWallThreads + threadIdx.x;
-  if (t >= 4 * nface) return;
-  const int q = (int)(t / nface);
-  int j, r0;
-  if (!slot_row(g, q, j, r0)) return;
-  const long r = t - q * nface;
-  const int i = 1 + (int)(r % g.nx), k = 1 + (int)(r / g.nx);
+  const long nrim = 4L * g.nx * g.nz + 4L * g.ny * g.nz;
+  long t = (long)blockIdx.x * kWallThreads + threadIdx.x;
   const double gam1 = a.gas.gam1, rg = a.gas.rg;
+  if (t >= nrim) {
+    // p of U^n one row beyond each bounded face, which the pressure sensor of
+    // the boundary row reads: the ghost state, which nothing on the path writes
+    if (!a.want_sensor) return;
+    t -= nrim;
+    const long nfi = (long)g.ny * g.nz, nfj = (long)g.nx * g.nz;
+    int i, j, k;
+    if (t < 2 * nfi) {
+      const int side = (int)(t / nfi);
+      if (!((g.ibnd >> side) & 1)) return;
+      const long r = t - side * nfi;
+      j = 1 + (int)(r % g.ny);
+      k = 1 + (int)(r / g.ny);
+      i = side ? g.nx + 1 : 0;
+    } else {
+      t -= 2 * nfi;
+      if (t >= 2 * nfj) return;
+      const int side = (int)(t / nfj);
+      if (!((g.jbnd >> side) & 1)) return;
+      const long r = t - side * nfj;
+      i = 1 + (int)(r % g.nx);
+      k = 1 + (int)(r / g.nx);
+      j = side ? g.ny + 1 : 0;
+    }
+    const idx_t pg = lin(g, i, j, k);
+    stg(a.prs, pg, prim_at(a.u0, pg, gam1, rg).p);
+    return;
+  }
+  int i, j, k;
+  if (!rim_point(g, t, i, j, k)) return;
+  const idx_t p = lin(g, i, j, k);
+  const int qi = oslot(g.ibnd, g.nx, i), qj = oslot(g.jbnd, g.ny, j);
 
   // gradients gr[u,v,w,T][x,y,z]
   double gr[4][3];
   {
-    const Pw xm2 = prim_at(a.u0, lin(g, i - 2, j, k), gam1, rg), xm1 = prim_at(a.u0, lin(g, i - 1, j, k), gam1, rg);
-    const Pw xp1 = prim_at(a.u0, lin(g, i + 1, j, k), gam1, rg), xp2 = prim_at(a.u0, lin(g, i + 2, j, k), gam1, rg);
-    const Pw zm2 = prim_at(a.u0, lin(g, i, j, k - 2), gam1, rg), zm1 = prim_at(a.u0, lin(g, i, j, k - 1), gam1, rg);
-    const Pw zp1 = prim_at(a.u0, lin(g, i, j, k + 1), gam1, rg), zp2 = prim_at(a.u0, lin(g, i, j, k + 2), gam1, rg);
+    double gx[4], gy[4], gz[4];
+    grad5(a.u0, g, p, 1, qi, i, g.nx, gam1, rg, gx);
+    grad5(a.u0, g, p, (idx_t)g.sx, qj, j, g.ny, gam1, rg, gy);
+    grad5(a.u0, g, p, g.sy, -1, k, g.nz, gam1, rg, gz);
     const double ix = ldg(a.met.idx_v, (idx_t)(i - 1 + G)), iy = ldg(a.met.idy_v, (idx_t)(j - 1 + G)),
                  iz = ldg(a.met.idz_v, (idx_t)(k - 1 + G));
-    gr[0][0] = (A5_1 * (xp1.u - xm1.u) + A5_2 * (xp2.u - xm2.u)) * ix;
-    gr[1][0] = (A5_1 * (xp1.v - xm1.v) + A5_2 * (xp2.v - xm2.v)) * ix;
-    gr[2][0] = (A5_1 * (xp1.w - xm1.w) + A5_2 * (xp2.w - xm2.w)) * ix;
-    gr[3][0] = (A5_1 * (xp1.T - xm1.T) + A5_2 * (xp2.T - xm2.T)) * ix;
-    gr[0][2] = (A5_1 * (zp1.u - zm1.u) + A5_2 * (zp2.u - zm2.u)) * iz;
-    gr[1][2] = (A5_1 * (zp1.v - zm1.v) + A5_2 * (zp2.v - zm2.v)) * iz;
-    gr[2][2] = (A5_1 * (zp1.w - zm1.w) + A5_2 * (zp2.w - zm2.w)) * iz;
-    gr[3][2] = (A5_1 * (zp1.T - zm1.T) + A5_2 * (zp2.T - zm2.T)) * iz;
-    double c[5];
-    onesided(q, c);
-    double gy[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int m = 0; m < 5; m++) {
-      const Pw y = prim_at(a.u0, lin(g, i, r0 + m, k), gam1, rg);
-      gy[0] = gy[0] + c[m] * y.u;
-      gy[1] = gy[1] + c[m] * y.v;
-      gy[2] = gy[2] + c[m] * y.w;
-      gy[3] = gy[3] + c[m] * y.T;
+    for (int n = 0; n < 4; n++) {
+      gr[n][0] = gx[n] * ix;
+      gr[n][1] = gy[n] * iy;
+      gr[n][2] = gz[n] * iz;
     }
-#pragma unroll
-    for (int n = 0; n < 4; n++) gr[n][1] = gy[n] * iy;
-    if (a.adiabatic && (q == 0 || q == 3)) gr[3][1] = 0.0;
   }
-  const idx_t p = lin(g, i, j, k);
+  // no-slip wall rows: an adiabatic wall has no wall-normal temperature
+  // gradient there (smod_grad_T.f90:102-109)
+  const bool wrow = ((g.jwall & 1) && j == 1) || ((g.jwall & 2) && j == g.ny);
+  if (a.adiabatic && wrow) gr[3][1] = 0.0;
   const Pw o = prim_at(a.u0, p, gam1, rg);
-  const bool wrow = q == 0 || q == 3;
   if (a.want_sensor) {
-    // the shock filter's Ducros sensor from these gradients; its pressure
-    // sensor on the wall row reads p one row beyond the wall, where the state
-    // is the ghost layer that nothing on the path writes
+    // the shock filter's Ducros sensor from these gradients
     double s = 1.0;
     if (a.ducros) {
       const double dv = gr[0][0] + gr[1][1] + gr[2][2];
@@ -204,10 +294,6 @@ __global__ __launch_bounds__(kWallThreads) void k_visc_wall(WallViscArgs a) {
       s = div2 / (div2 + o1 * o1 + o2 * o2 + o3 * o3 + 1.0e-16);
     }
     stg(a.sens, p, s);
-    if (wrow) {
-      const idx_t pg = lin(g, i, q == 0 ? 0 : g.ny + 1, k);
-      stg(a.prs, pg, prim_at(a.u0, pg, gam1, rg).p);
-    }
   }
   // the wall row stores tau/mu and -(cp/Pr) grad T (k_tau_wallrow completes it)
   const double mu_l = wrow ? 1.0 : sutherland(a.gas, o.T);
@@ -253,6 +339,8 @@ __global__ __launch_bounds__(kWallThreads) void k_tau_wallrow(WallViscArgs a) {
   const int i = -1 + (int)(r % ex), k = -1 + (int)(r / ex);
   const bool in_i = i >= 1 && i <= g.nx, in_k = k >= 1 && k <= g.nz;
   if (!in_i && !in_k) return;  // edge ghosts: never read
+  // beyond a bounded i-face: never read (the i-derivatives are one-sided there)
+  if (((g.ibnd & 1) && i < 1) || ((g.ibnd & 2) && i > g.nx)) return;
   const int j = side ? g.ny : 1;
   const idx_t p = lin(g, i, j, k);
   const Pw o = prim_at(a.u0, p, a.gas.gam1, a.gas.rg);
@@ -270,35 +358,183 @@ __global__ __launch_bounds__(kWallThreads) void k_tau_wallrow(WallViscArgs a) {
 
 __global__ __launch_bounds__(kWallThreads) void k_vdiv_wall(WallViscArgs a) {
   const Geom &g = a.g;
-  const long nface = (long)g.nx * g.nz;
   const long t = (long)blockIdx.x * kWallThreads + threadIdx.x;
-  if (t >= 4 * nface) return;
-  const int q = (int)(t / nface);
-  int j, r0;
-  if (!slot_row(g, q, j, r0)) return;
-  const long r = t - q * nface;
-  const int i = 1 + (int)(r % g.nx), k = 1 + (int)(r / g.nx);
+  int i, j, k;
+  if (!rim_point(g, t, i, j, k)) return;
+  const int qi = oslot(g.ibnd, g.nx, i), qj = oslot(g.jbnd, g.ny, j);
   const double mx = ldg(a.met.idx_v, (idx_t)(i - 1 + G)), my = ldg(a.met.idy_v, (idx_t)(j - 1 + G)),
                mz = ldg(a.met.idz_v, (idx_t)(k - 1 + G));
-  double c[5];
-  onesided(q, c);
   const idx_t p = lin(g, i, j, k);
 #pragma unroll
   for (int n = 0; n < 4; n++) {
     const double *fx = a.tau[krow(0, n)], *fy = a.tau[krow(1, n)], *fz = a.tau[krow(2, n)];
-    double kv = (A5_1 * (ldg(fx, lin(g, i + 1, j, k)) - ldg(fx, lin(g, i - 1, j, k))) +
-                 A5_2 * (ldg(fx, lin(g, i + 2, j, k)) - ldg(fx, lin(g, i - 2, j, k)))) *
-                mx;
-    double dy = 0.0;
-#pragma unroll
-    for (int m = 0; m < 5; m++) dy = dy + c[m] * ldg(fy, lin(g, i, r0 + m, k));
-    kv = dy * my + kv;
-    kv = (A5_1 * (ldg(fz, lin(g, i, j, k + 1)) - ldg(fz, lin(g, i, j, k - 1))) +
-          A5_2 * (ldg(fz, lin(g, i, j, k + 2)) - ldg(fz, lin(g, i, j, k - 2)))) *
-             mz +
-         kv;
+    double kv = d5at(fx, p, 1, qi, i, g.nx) * mx;
+    kv = d5at(fy, p, (idx_t)g.sx, qj, j, g.ny) * my + kv;
+    kv = d5at(fz, p, g.sy, -1, k, g.nz) * mz + kv;
     stg(a.kv[n], p, kv);
   }
+}
+
+// One-sided 5-pt derivative a40 at the last row n of an axis (stride st) of
+// the values f[0..4] at rows n-4..n, times the metric.
+__device__ __forceinline__ double d40(const double f[5]) {
+  return (-A04_5) * f[0] + (-A04_4) * f[1] + (-A04_3) * f[2] + (-A04_2) * f[3] + (-A04_1) * f[4];
+}
+
+// Conserved variables of the state at element p.
+__device__ __forceinline__ void cons_at(const CharArgs &a, idx_t p, double u[5]) {
+  if (a.src.v[0]) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) u[v] = ldg(a.src.v[v], p);
+  } else {
+#pragma unroll
+    for (int v = 0; v < 5; v++) u[v] = a.ref[v] + (double)ldg(a.src32[v], p);
+  }
+}
+
+// The characteristic increment of a row point from the primitives of the
+// rows n-4..n along its normal (LODI, perfect gas: cp/av = cp T).
+// Section imax: (j, k) of the imax row; section jmax: (i, k) of the jmax row.
+__device__ void charac_point(const CharArgs &a, bool imax, int i, int j, int k, double *kout, idx_t sl, idx_t ss) {
+  const Geom &g = a.g;
+  const idx_t st = imax ? (idx_t)1 : (idx_t)g.sx;
+  const idx_t p = lin(g, i, j, k);
+  double fu[5], fv[5], fw[5], fp[5], fr[5], ue[5];
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    double u[5];
+    cons_at(a, p - (idx_t)(4 - m) * st, u);
+    const Pw q = prim_of(u, a.gas.gam1, a.gas.rg);
+    fu[m] = q.u;
+    fv[m] = q.v;
+    fw[m] = q.w;
+    fp[m] = q.p;
+    fr[m] = u[0];
+    if (m == 4) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) ue[v] = u[v];
+    }
+  }
+  const double met = imax ? ldg(a.idx_v, (idx_t)(i - 1 + G)) : ldg(a.idy_v, (idx_t)(j - 1 + G));
+  const double du = d40(fu) * met, dv = d40(fv) * met, dw = d40(fw) * met, dp = d40(fp) * met,
+               dr = d40(fr) * met;
+  const double rho = ue[0], u = fu[4], v = fv[4], w = fw[4], pr = fp[4];
+  const double T = pr * (1.0 / rho) / a.gas.rg;
+  double rt, ut, vt, wt, pt, c2;
+  if (imax) {
+    // bc_charac_imax (mod_charac.f90:350-469): the sound speed that
+    // primitives_visc last computed; subsonic outflow relaxes p towards
+    // 0.528 p_ref
+    const double cj = ldg(a.cst, sl);
+    c2 = cj * cj;
+    const double upc = u + cj, umc = u - cj;
+    const double L1 = umc <= 0.0 ? 0.1 * (pr - 0.528 * a.p_ref) : umc * (dp - rho * cj * du);
+    const double L5 = upc <= 0.0 ? 0.0 : upc * (dp + rho * cj * du);
+    const double L2 = u * (dp - c2 * dr), L3 = u * dv, L4 = u * dw;
+    pt = 0.5 * (L5 + L1);
+    rt = -(pt + L2) / c2;
+    ut = 0.5 * (L5 - L1) / (rho * cj);
+    vt = L3;
+    wt = L4;
+  } else {
+    // bc_charac_jmax (mod_charac.f90:816-969): c of the current primitives
+    const double cc = sqrt(a.gas.gam * a.gas.rg * T);
+    c2 = cc * cc;
+    double L1 = 0.0, L2, L3, L4;
+    double L5 = (v + cc) * (dp + rho * cc * dv);
+    if (v <= 0.0) {
+      L2 = L3 = L4 = 0.0;
+      if (v <= -cc) L5 = 0.0;
+    } else {
+      L2 = v * du;
+      L3 = v * (dp - c2 * dr);
+      L4 = v * dw;
+      if (v >= cc) L1 = (v - cc) * (dp - rho * cc * dv);
+    }
+    pt = 0.5 * (L5 + L1);
+    rt = (pt + L3) / c2;
+    ut = L2;
+    vt = (L5 - L1) / (2.0 * rho * cc);
+    wt = L4;
+  }
+  const double cpav = a.cp * T;
+  stg(kout, sl, rt);
+  stg(kout, ss + sl, rho * ut + u * rt);
+  stg(kout, 2 * ss + sl, rho * vt + v * rt);
+  stg(kout, 3 * ss + sl, rho * wt + w * rt);
+  stg(kout, 4 * ss + sl, cpav * (pt / c2 - rt) + (ue[4] + pr) / rho * rt + rho * (u * ut + v * vt + w * wt));
+}
+
+// Sound speed sqrt(gam rg T) of the FP64 state U at p (primitives_visc).
+__device__ __forceinline__ double sound(const CState &U, idx_t p, const Gas &gs) {
+  const Pw q = prim_at(U, p, gs.gam1, gs.rg);
+  return sqrt(gs.gam * gs.rg * q.T);
+}
+
+__global__ __launch_bounds__(kWallThreads) void k_charac(CharArgs a) {
+  const Geom &g = a.g;
+  const long nfi = (long)g.ny * g.nz, nfj = (long)g.nx * g.nz;
+  long t = (long)blockIdx.x * kWallThreads + threadIdx.x;
+  if (t < nfi) {
+    // the imax row (every j)
+    if (!a.kimax) return;
+    const int j = 1 + (int)(t % g.ny), k = 1 + (int)(t / g.ny);
+    const idx_t sl = (idx_t)t, p = lin(g, g.nx, j, k);
+    if (a.cst_mode == 3) {
+      stg(a.cst, sl, sound(a.cnew, p, a.gas));
+      return;
+    }
+    if (a.cst_mode == 2) {
+      double u[5];
+      cons_at(a, p, u);
+      const Pw q = prim_of(u, a.gas.gam1, a.gas.rg);
+      stg(a.cst, sl, sqrt(a.gas.gam * a.gas.rg * q.T));
+    }
+    charac_point(a, true, g.nx, j, k, a.kimax, sl, (idx_t)nfi);
+    if (a.cst_mode == 1) stg(a.cst, sl, sound(a.cnew, p, a.gas));
+    return;
+  }
+  t -= nfi;
+  if (t < nfj) {
+    // the jmax row, i = i0..i1
+    if (!a.kjmax || a.cst_mode == 3) return;
+    const int i = 1 + (int)(t % g.nx), k = 1 + (int)(t / g.nx);
+    if (i < a.i0 || i > a.i1) return;
+    charac_point(a, false, i, g.ny, k, a.kjmax, (idx_t)t, (idx_t)nfj);
+    return;
+  }
+  t -= nfj;
+  if (t < nfi) {
+    // the inlet row i = 1 (every j)
+    if (!a.uin || !a.inlet_mode) return;
+    const int j = 1 + (int)(t % g.ny), k = 1 + (int)(t / g.ny);
+    const idx_t sl = (idx_t)t, ss = (idx_t)nfi, p = lin(g, 1, j, k);
+    double u[5];
+    cons_at(a, p, u);
+#pragma unroll
+    for (int v = 0; v < 5; v++) stg(a.ppre, (idx_t)v * ss + sl, u[v]);
+    if (a.inlet_mode == 1) {
+      // bc_supersonic_inlet (mod_charac.f90:1133-1146): the inflow state
+#pragma unroll
+      for (int v = 0; v < 5; v++) stg(a.row1.v[v], p, ldg(a.uin, (idx_t)v * ss + sl));
+    } else {
+#pragma unroll
+      for (int v = 0; v < 5; v++) {
+        stg(a.uin, (idx_t)v * ss + sl, u[v]);
+        stg(a.kf, (idx_t)v * ss + sl, 0.0);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kWallThreads) void k_inlet_kf(Geom g, CState after, CState before, double *kf) {
+  const long nfi = (long)g.ny * g.nz;
+  const long t = (long)blockIdx.x * kWallThreads + threadIdx.x;
+  if (t >= nfi) return;
+  const int j = 1 + (int)(t % g.ny), k = 1 + (int)(t / g.ny);
+  const idx_t p = lin(g, 1, j, k);
+#pragma unroll
+  for (int v = 0; v < 5; v++) stg(kf, (idx_t)v * (idx_t)nfi + (idx_t)t, ldg(after.v[v], p) - ldg(before.v[v], p));
 }
 
 inline dim3 grid_for(long n) { return dim3((unsigned)((n + kWallThreads - 1) / kWallThreads)); }
@@ -312,7 +548,7 @@ hipError_t launch_wall(const WallArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_visc_wall(const WallViscArgs &a, hipStream_t s) {
-  const long n = 4L * a.g.nx * a.g.nz;
+  const long n = 6L * a.g.nx * a.g.nz + 6L * a.g.ny * a.g.nz;
   hipLaunchKernelGGL(k_visc_wall, grid_for(n), dim3(kWallThreads), 0, s, a);
   return hipGetLastError();
 }
@@ -324,8 +560,20 @@ hipError_t launch_tau_wallrow(const WallViscArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_vdiv_wall(const WallViscArgs &a, hipStream_t s) {
-  const long n = 4L * a.g.nx * a.g.nz;
+  const long n = 4L * a.g.nx * a.g.nz + 4L * a.g.ny * a.g.nz;
   hipLaunchKernelGGL(k_vdiv_wall, grid_for(n), dim3(kWallThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_charac(const CharArgs &a, hipStream_t s) {
+  const long n = 2L * a.g.ny * a.g.nz + (long)a.g.nx * a.g.nz;
+  hipLaunchKernelGGL(k_charac, grid_for(n), dim3(kWallThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_inlet_kf(const Geom &g, const CState &after, const CState &before, double *kf, hipStream_t s) {
+  const long n = (long)g.ny * g.nz;
+  hipLaunchKernelGGL(k_inlet_kf, grid_for(n), dim3(kWallThreads), 0, s, g, after, before, kf);
   return hipGetLastError();
 }
 

@@ -123,6 +123,66 @@ DissipKind dissip_kind(const mgpu_case &c, bool wall) {
   return kDissipShock;
 }
 
+// Boundary faces: no neighbour; the operators take their closures there.
+bool is_bound(int kind) {
+  return kind == MGPU_FACE_WALL || kind == MGPU_FACE_CHARAC || kind == MGPU_FACE_INLET;
+}
+bool is_charac(int kind) { return kind == MGPU_FACE_CHARAC || kind == MGPU_FACE_INLET; }
+
+// Characteristic outflow and supersonic inlet, checked on the caller's block
+// table: bc_charac_imax / bc_charac_jmax at imax / jmax and the inlet at imin
+// of Cartesian blocks whose other boundary faces are j-walls (so that no
+// axis relabeling moves them), with the flags their kernels cover.  "" or an
+// error.
+std::string check_charac(const mgpu_case &c) {
+  char buf[256];
+  for (int b = 0; b < c.nblocks; b++) {
+    const mgpu_block &A = c.blocks[b];
+    bool any = false;
+    for (int f = 0; f < 6; f++) {
+      const int kind = A.face_kind[f];
+      if (!is_charac(kind)) continue;
+      any = true;
+      if (kind == MGPU_FACE_CHARAC && f != 1 && f != 3) {
+        snprintf(buf, sizeof buf, "block %d face %d: characteristic outflow is supported at imax and jmax (faces 1, 3)", b,
+                 f);
+        return buf;
+      }
+      if (kind == MGPU_FACE_INLET && f != 0) {
+        snprintf(buf, sizeof buf, "block %d face %d: the supersonic inlet is supported at imin (face 0)", b, f);
+        return buf;
+      }
+    }
+    if (!any) continue;
+    for (int f : {0, 1, 4, 5})
+      if (A.face_kind[f] == MGPU_FACE_WALL) {
+        snprintf(buf, sizeof buf, "block %d: walls with characteristic faces are supported on the j-faces only", b);
+        return buf;
+      }
+    if (c.curvilinear) {
+      snprintf(buf, sizeof buf, "block %d: characteristic faces need a Cartesian case", b);
+      return buf;
+    }
+    if (c.filter != MGPU_FILTER_NONE && c.filter != MGPU_FILTER_SF && c.filter != MGPU_FILTER_SHOCK) {
+      snprintf(buf, sizeof buf, "block %d: characteristic faces with filter %d are not supported", b, c.filter);
+      return buf;
+    }
+    if (c.launch_flags & MGPU_LAUNCH_STAGE_PIPELINED) {
+      snprintf(buf, sizeof buf, "block %d: characteristic faces run the classic stage kernel only", b);
+      return buf;
+    }
+    if ((is_bound(A.face_kind[0]) || is_bound(A.face_kind[1])) && A.nx < 10) {
+      snprintf(buf, sizeof buf, "block %d: %d cells between its i boundary faces, >= 10 needed", b, A.nx);
+      return buf;
+    }
+    if ((is_bound(A.face_kind[2]) || is_bound(A.face_kind[3])) && A.ny < 10) {
+      snprintf(buf, sizeof buf, "block %d: %d cells between its j boundary faces, >= 10 needed", b, A.ny);
+      return buf;
+    }
+  }
+  return std::string();
+}
+
 Metrics metrics_of(const BlockData &bd) {
   Metrics m{};
   m.idx = bd.idx[0];
@@ -317,10 +377,11 @@ std::string check_topology(const mgpu_case &c) {
     const int na[3] = {A.nx, A.ny, A.nz};
     for (int f = 0; f < 6; f++) {
       const int kind = A.face_kind[f];
-      if (kind != MGPU_FACE_PERIODIC && kind != MGPU_FACE_INTERFACE && kind != MGPU_FACE_WALL) {
+      if (kind != MGPU_FACE_PERIODIC && kind != MGPU_FACE_INTERFACE && !is_bound(kind)) {
         snprintf(buf, sizeof buf, "block %d face %d: face kind %d not supported", b, f, kind);
         return buf;
       }
+      if (is_charac(kind)) continue;  // check_charac
       if (kind == MGPU_FACE_WALL) {
         if (f != 2 && f != 3) {
           snprintf(buf, sizeof buf, "block %d face %d: walls are supported on the j-faces (2, 3) only", b, f);
@@ -419,7 +480,7 @@ std::string check_wall_rows(const mgpu_case &c) {
     const int na[3] = {A.nx, A.ny, A.nz};
     const int jwa = (A.face_kind[2] == MGPU_FACE_WALL ? 1 : 0) | (A.face_kind[3] == MGPU_FACE_WALL ? 2 : 0);
     for (int f = 0; f < 6; f++) {
-      if (A.face_kind[f] == MGPU_FACE_WALL) continue;
+      if (is_bound(A.face_kind[f])) continue;
       int nb, nf, o;
       face_nbr(A, b, f, nb, nf, o);
       const mgpu_block &B = c.blocks[nb];
@@ -519,7 +580,7 @@ bool build_tasks(const mgpu_case &c, const std::vector<BlockData> &bds, int ng, 
     const Geom &ga = bds[b].g;
     const int na[3] = {ga.nx, ga.ny, ga.nz};
     for (int f = 0; f < 6; f++) {
-      if (A.face_kind[f] == MGPU_FACE_WALL) continue;
+      if (is_bound(A.face_kind[f])) continue;
       int nb, nf, o;
       face_nbr(A, b, f, nb, nf, o);
       const Geom &gb = bds[nb].g;
@@ -711,6 +772,10 @@ void free_block(BlockData &bd) {
   if (bd.sens) (void)hipFree(bd.sens);
   if (bd.wstash) (void)hipFree(bd.wstash);
   bd.prs = bd.sens = bd.wstash = nullptr;
+  for (double **q : {&bd.kimax, &bd.kjmax, &bd.cst, &bd.uin, &bd.ppre, &bd.kf}) {
+    if (*q) (void)hipFree(*q);
+    *q = nullptr;
+  }
   for (auto &p : bd.sr) { if (p) (void)hipFree(p); p = nullptr; }
 }
 
@@ -759,6 +824,14 @@ StageArgs stage_args(const mgpu_handle *h, const BlockData &bd, int in, int s0, 
   for (int q = 0; q < 4; q++) sa.kv[q] = bd.kv[q];
   sa.alpha = bd.p.deltat * bd.p.crk[irk - 1];
   sa.ref = bd.ref;
+  // boundary rows: the inlet row keeps the filter increment, the
+  // characteristic rows their routines' increments; walls K = 0
+  sa.kbc[0] = bd.kf;
+  sa.kbc[1] = bd.kimax;
+  sa.kbc[2] = nullptr;
+  sa.kbc[3] = bd.kjmax;
+  sa.uin_row = bd.uin;
+  sa.prim_row = irk == 1 ? bd.ppre : nullptr;
   if (h->mixed) {
     const int mode = stage_mode(h, irk);
     for (int v = 0; v < 5; v++) {
@@ -986,6 +1059,83 @@ WallViscArgs wall_visc_args(const mgpu_handle *h, const BlockData &bd, int s0) {
   return a;
 }
 
+// Characteristic rows and inlet of every block that has them, from set `src`
+// (>= 0 an FP64 set, <= -2 the mixed-precision FP32 stage set -2 - src).
+// cst_mode / inlet_mode as in CharArgs; cnew: the FP64 set of U^n (cst_mode 1).
+int enqueue_charac(mgpu_handle *h, int src, int cst_mode, int cnew, int inlet_mode) {
+  for (auto &bd : h->b) {
+    if (!bd.kimax && !bd.kjmax && !bd.uin) continue;
+    CharArgs a{};
+    a.g = bd.g;
+    a.gas = gas_of(bd.p);
+    a.cp = bd.p.gam * bd.p.rg / (bd.p.gam - 1.0);
+    a.p_ref = bd.p.p_ref;
+    for (int v = 0; v < 5; v++) {
+      a.src.v[v] = src >= 0 ? bd.st[src][v] : nullptr;
+      a.src32[v] = src <= -2 ? bd.sf[-2 - src][v] : nullptr;
+      a.cnew.v[v] = cnew >= 0 ? bd.st[cnew][v] : nullptr;
+      a.row1.v[v] = src >= 0 ? bd.st[src][v] : nullptr;
+    }
+    a.ref[0] = bd.ref.r;
+    a.ref[1] = bd.ref.ru;
+    a.ref[2] = bd.ref.rv;
+    a.ref[3] = bd.ref.rw;
+    a.ref[4] = bd.ref.re;
+    a.idx_v = bd.idx[3];
+    a.idy_v = bd.idx[4];
+    a.kimax = bd.kimax;
+    a.kjmax = bd.kjmax;
+    a.cst = bd.cst;
+    a.cst_mode = cst_mode;
+    a.uin = bd.uin;
+    a.ppre = bd.ppre;
+    a.kf = bd.kf;
+    a.inlet_mode = inlet_mode;
+    // bc_charac_jmax runs over i = ndx_c..nfx_c: one row in from a
+    // characteristic or inlet i-face (mpi_index_bounds.f90:298-360)
+    a.i0 = bd.uin ? 2 : 1;
+    a.i1 = bd.kimax ? bd.g.nx - 1 : bd.g.nx;
+    if (inlet_mode == 1 && src < 0) return fail(h, MGPU_EINVAL, "inlet reset needs an FP64 set");
+    if (cst_mode == 3 && !bd.kimax) continue;
+    HIPCK(h, launch_charac(a, h->stream));
+  }
+  return 0;
+}
+
+// start_runge_kutta (runge_kutta.f90:316-342) for blocks whose state was just
+// uploaded: the sound speed of U (primitives_visc at set-up), the increments
+// of the characteristic rows from U, and the inflow state = row 1 of U, with
+// no filter increment yet on the inlet row.
+int start_charac(mgpu_handle *h) {
+  bool any = false;
+  for (auto &bd : h->b) any |= bd.bc_start;
+  if (!any) return 0;
+  for (auto &bd : h->b) {
+    if (!bd.bc_start) continue;
+    CharArgs a{};
+    a.g = bd.g;
+    a.gas = gas_of(bd.p);
+    a.cp = bd.p.gam * bd.p.rg / (bd.p.gam - 1.0);
+    a.p_ref = bd.p.p_ref;
+    for (int v = 0; v < 5; v++) a.src.v[v] = bd.st[h->cur][v];
+    a.idx_v = bd.idx[3];
+    a.idy_v = bd.idx[4];
+    a.kimax = bd.kimax;
+    a.kjmax = bd.kjmax;
+    a.cst = bd.cst;
+    a.cst_mode = 2;
+    a.uin = bd.uin;
+    a.ppre = bd.ppre;
+    a.kf = bd.kf;
+    a.inlet_mode = 2;
+    a.i0 = bd.uin ? 2 : 1;
+    a.i1 = bd.kimax ? bd.g.nx - 1 : bd.g.nx;
+    HIPCK(h, launch_charac(a, h->stream));
+    bd.bc_start = false;
+  }
+  return 0;
+}
+
 // Enqueues one step with U^n in set s0 (no host synchronisation: the same
 // sequence is captured into a graph).  runge_kutta, runge_kutta.f90:2-152.
 // With walls, bc_wall runs after each stage's exchange, from the stage's
@@ -994,6 +1144,14 @@ WallViscArgs wall_visc_args(const mgpu_handle *h, const BlockData &bd, int s0) {
 // primitives_visc wrote, viscous or not, runge_kutta.f90:71) is kept aside
 // and imposed after the filter and its exchange (runge_kutta.f90:20-24,
 // 42-45, 130-136).
+// With characteristic faces, bc_non_reflecting runs after `primitives` in
+// stages 2..nrk (runge_kutta.f90:30-37, 64-70: after that stage's wall
+// state, before its fluxes) and at the end of the step (:137-139, after the
+// final wall state), where the inlet row also takes the inflow state; the
+// last stage then records the sound speed of U^n that bc_charac_imax reads
+// until the next step's last stage (primitives_visc, :71).  The filter
+// increment of the inlet row is kept: K there is not written again before
+// the next step's stages use it.
 int enqueue_step(mgpu_handle *h, int s0) {
   const mgpu_case &c = h->c;
   const int geo = c.curvilinear;
@@ -1007,23 +1165,33 @@ int enqueue_step(mgpu_handle *h, int s0) {
     if (last && visc) {
       ProfScope ps(h, MGPU_K_VISC);
       HIPCK(h, launch_visc(h->d_visc + s0 * nb, nb, h->nt_visc, geo, h->stream));
-      if (h->wall)
+      if (h->bnd)
         for (const auto &bd : h->b)
-          if (bd.g.jwall) HIPCK(h, launch_visc_wall(wall_visc_args(h, bd, s0), h->stream));
+          if (bd.g.ibnd || bd.g.jbnd) HIPCK(h, launch_visc_wall(wall_visc_args(h, bd, s0), h->stream));
       HIPCK(h, launch_iface(h->xv, h->nchunks_v, h->stream));
       if (h->wall)
         for (const auto &bd : h->b)
           if (bd.g.jwall) HIPCK(h, launch_tau_wallrow(wall_visc_args(h, bd, s0), h->stream));
       HIPCK(h, launch_vdiv(h->d_vdiv, nb, h->nt_vdiv, geo, h->stream));
-      if (h->wall)
+      if (h->bnd)
         for (const auto &bd : h->b)
-          if (bd.g.jwall) HIPCK(h, launch_vdiv_wall(wall_visc_args(h, bd, s0), h->stream));
+          if (bd.g.ibnd || bd.g.jbnd) HIPCK(h, launch_vdiv_wall(wall_visc_args(h, bd, s0), h->stream));
+    }
+    if (h->charac && irk > 1) {
+      ProfScope ps(h, MGPU_K_HALO);
+      // the stage input (mixed precision: an FP32 stage set)
+      const int in_d = h->mixed ? -2 - fset_of(irk - 1) : in;
+      if ((rc = enqueue_charac(h, in_d, last ? 1 : 0, s0, 0))) return rc;
+    } else if (h->charac && last) {
+      // one stage: no characteristic rows before it, only the sound speed of U^n
+      ProfScope ps(h, MGPU_K_HALO);
+      if ((rc = enqueue_charac(h, -1, 3, s0, 0))) return rc;
     }
     {
       ProfScope ps(h, MGPU_K_STAGE);
       const int v = (last && visc) ? 1 : 0;
       HIPCK(h, launch_stage(h->d_stage + ((size_t)s0 * nrk + (irk - 1)) * nb, nb, h->nt_stage[v], geo,
-                            last && visc, h->shape[v], stage_mode(h, irk), h->wall, h->stream));
+                            last && visc, h->shape[v], stage_mode(h, irk), h->bnd, h->stream));
     }
     if (h->mixed && !last) {
       ProfScope ps(h, MGPU_K_HALO);
@@ -1049,19 +1217,36 @@ int enqueue_step(mgpu_handle *h, int s0) {
     {
       ProfScope ps(h, MGPU_K_FILTER);
       if (c.filter == MGPU_FILTER_SF)
-        HIPCK(h, launch_filter_sf(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, h->wall, h->stream));
+        HIPCK(h, launch_filter_sf(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, h->bnd, h->stream));
       else
-        HIPCK(h, launch_filter_shock(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, dissip_kind(c, h->wall), h->wall,
+        HIPCK(h, launch_filter_shock(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, dissip_kind(c, h->wall), h->bnd,
                                      edoh_on(c), h->stream));
-      if (h->wall && has_sensor(c))
-        for (const auto &bd : h->b)
-          if (bd.g.jwall) HIPCK(h, launch_shock_wall(filter_args(h, bd, in, s0), h->stream));
+      if (h->bnd && has_sensor(c))
+        for (const auto &bd : h->b) {
+          if (bd.g.jbnd) HIPCK(h, launch_shock_wall(filter_args(h, bd, in, s0), 1, h->stream));
+          if (bd.g.ibnd) HIPCK(h, launch_shock_wall(filter_args(h, bd, in, s0), 0, h->stream));
+        }
+      // the filter increment of the inlet row, before the wall state changes
+      // its corner
+      for (const auto &bd : h->b)
+        if (bd.kf) {
+          CState after{}, before{};
+          for (int v = 0; v < 5; v++) {
+            after.v[v] = bd.st[s0][v];
+            before.v[v] = bd.st[in][v];
+          }
+          HIPCK(h, launch_inlet_kf(bd.g, after, before, bd.kf, h->stream));
+        }
     }
     if ((rc = exchange_state(h, s0))) return rc;
     if (h->wall) {
       ProfScope ps(h, MGPU_K_HALO);
       if ((rc = enqueue_walls(h, -1, s0))) return rc;
     }
+  }
+  if (h->charac) {
+    ProfScope ps(h, MGPU_K_HALO);
+    if ((rc = enqueue_charac(h, c.filter != MGPU_FILTER_NONE ? s0 : in, 0, -1, 1))) return rc;
   }
   return 0;
 }
@@ -1190,6 +1375,8 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
   bd.g.sy = (idx_t)bd.g.sx * (idx_t)(bl.ny + 2 * G);
   bd.g.n = (long)bd.g.sy * (bl.nz + 2 * G);
   bd.g.jwall = (bl.face_kind[2] == MGPU_FACE_WALL ? 1 : 0) | (bl.face_kind[3] == MGPU_FACE_WALL ? 2 : 0);
+  bd.g.ibnd = (is_bound(bl.face_kind[0]) ? 1 : 0) | (is_bound(bl.face_kind[1]) ? 2 : 0);
+  bd.g.jbnd = (is_bound(bl.face_kind[2]) ? 1 : 0) | (is_bound(bl.face_kind[3]) ? 2 : 0);
   bd.bytes = block_bytes(c, bl.nx, bl.ny, bl.nz);
   const size_t n = (size_t)bd.g.n;
   int rc = 0;
@@ -1213,6 +1400,19 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
     }
   }
   if (bd.g.jwall && !rc) rc = dalloc(h, &bd.wstash, (size_t)4 * bl.nx * bl.nz);
+  {
+    const size_t fi = (size_t)bl.ny * bl.nz, fj = (size_t)bl.nx * bl.nz;
+    if (bl.face_kind[1] == MGPU_FACE_CHARAC) {
+      if (!rc) rc = dalloc(h, &bd.kimax, 5 * fi);
+      if (!rc) rc = dalloc(h, &bd.cst, fi);
+    }
+    if (bl.face_kind[3] == MGPU_FACE_CHARAC && !rc) rc = dalloc(h, &bd.kjmax, 5 * fj);
+    if (bl.face_kind[0] == MGPU_FACE_INLET) {
+      if (!rc) rc = dalloc(h, &bd.uin, 5 * fi);
+      if (!rc) rc = dalloc(h, &bd.ppre, 5 * fi);
+      if (!rc) rc = dalloc(h, &bd.kf, 5 * fi);
+    }
+  }
   if (c.iorder_visc) {
     for (int q = 0; q < NTAU && !rc; q++) rc = dalloc(h, &bd.tau[q], n);
     for (int q = 0; q < 4 && !rc; q++) rc = dalloc(h, &bd.kv[q], n);
@@ -1230,6 +1430,11 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
     return rc;
   }
   if (bd.g.jwall) h->wall = true;
+  if (bd.g.ibnd || bd.g.jbnd) h->bnd = true;
+  if (bd.kimax || bd.kjmax || bd.uin) {
+    h->charac = true;
+    h->stage_family = 1;  // the classic stage kernel closes i-faces
+  }
   h->exchange_ready = false;
   h->tables_ready = false;
   return 0;
@@ -1257,7 +1462,10 @@ int put_metrics_cart(mgpu_handle *h, int block, const double *const *src_in) {
   const double *const *src = sp;
   for (int q = 0; q < 6; q++) {
     if (!src[q]) {
-      if (q >= 3 && h->c.iorder_visc == 0) continue;
+      // the characteristic rows take their one-sided derivatives with the
+      // viscous metrics idx_v, idy_v (mod_charac.f90:386-404 and the jmax twin)
+      const bool charac = bd.kimax || bd.kjmax;
+      if (q >= 3 && h->c.iorder_visc == 0 && !(charac && q < 5)) continue;
       return fail(h, MGPU_EINVAL, "metric array %d is null", q);
     }
     HIPCK(h, hipMemcpy(bd.idx[q], src[q], sizeof(double) * len[q % 3], hipMemcpyHostToDevice));
@@ -1345,6 +1553,7 @@ int put_state(mgpu_handle *h, int block, const double *const *src_in) {
   for (int v = 0; v < 5; v++) HIPCK(h, hipMemcpy(bd.st[h->cur][v], src[v], bytes, hipMemcpyHostToDevice));
   bd.have_state = true;
   h->ghosts_dirty = true;  // the exchange runs before the next use of the ghosts
+  bd.bc_start = bd.kimax || bd.kjmax || bd.uin;
   if (h->mixed && (block == 0 || !h->shared_ref)) {
     // reference state of the mixed path: interior mean of this block's state;
     // blocks joined by interfaces exchange FP32 differences with it, so they
@@ -1418,6 +1627,7 @@ int step(mgpu_handle *h, int nsteps) {
   if (h->b.empty()) return 0;
   int rc = refresh_ghosts(h);
   if (!rc) rc = build_tables(h);
+  if (!rc) rc = start_charac(h);
   if (rc) return rc;
   const bool graph = h->use_graph && !h->profiling;
   for (int s = 0; s < nsteps; s++) {
@@ -1629,8 +1839,8 @@ int tune_aux(mgpu_handle *h, bool force, hipEvent_t e0, hipEvent_t e1) {
         if (kind == 0) return launch_visc(h->d_visc + s0 * nb, nb, h->nt_visc, geo, h->stream);
         if (kind == 1) return launch_vdiv(h->d_vdiv, nb, h->nt_vdiv, geo, h->stream);
         if (h->c.filter == MGPU_FILTER_SF)
-          return launch_filter_sf(h->d_tune_f, nb, h->nt_filt, h->mixed, h->wall, h->stream);
-        return launch_filter_shock(h->d_tune_f, nb, h->nt_filt, h->mixed, dissip_kind(h->c, h->wall), h->wall,
+          return launch_filter_sf(h->d_tune_f, nb, h->nt_filt, h->mixed, h->bnd, h->stream);
+        return launch_filter_shock(h->d_tune_f, nb, h->nt_filt, h->mixed, dissip_kind(h->c, h->wall), h->bnd,
                                    edoh_on(h->c), h->stream);
       };
       hipError_t e = launch();
@@ -1686,6 +1896,7 @@ int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
   std::vector<int> perm;
   {
     std::string msg = check_cartesian_frames(*c);
+    if (msg.empty()) msg = check_charac(*c);
     if (msg.empty()) msg = relabel_blocks(*c, stored, perm);
     mgpu_case cs = *c;
     cs.blocks = stored.data();
@@ -1827,10 +2038,10 @@ int mgpu_autotune(mgpu_handle *h, int flags) {
         const int mode = stage_mode(h, irk);
         hipError_t e = hipMemcpy(h->d_tune, tab.data(), nb * sizeof(tab[0]), hipMemcpyHostToDevice);
         // one warm-up launch, then three timed ones
-        if (e == hipSuccess) e = launch_stage(h->d_tune, nb, nt, geo, v == 1, sh, mode, h->wall, h->stream);
+        if (e == hipSuccess) e = launch_stage(h->d_tune, nb, nt, geo, v == 1, sh, mode, h->bnd, h->stream);
         if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
         for (int r = 0; r < 3 && e == hipSuccess; r++)
-          e = launch_stage(h->d_tune, nb, nt, geo, v == 1, sh, mode, h->wall, h->stream);
+          e = launch_stage(h->d_tune, nb, nt, geo, v == 1, sh, mode, h->bnd, h->stream);
         if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.f;
@@ -1935,6 +2146,7 @@ int mgpu_exchange_plan(const mgpu_case *c, int set, int64_t *map, int64_t cap, i
   std::vector<mgpu_block> stored;
   std::vector<int> perm;
   std::string msg = check_cartesian_frames(*c);
+  if (msg.empty()) msg = check_charac(*c);
   if (msg.empty()) msg = relabel_blocks(*c, stored, perm);
   mgpu_case cs = *c;
   cs.blocks = stored.data();

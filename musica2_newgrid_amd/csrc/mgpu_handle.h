@@ -30,6 +30,15 @@ struct BlockData {
   RefState ref{};         // mixed precision: interior mean of the state at put_state
   double *wstash = nullptr;  // walls: rho, rhoE of the end-of-step wall state (2 per wall point)
   double c2[2] = {0, 0}, c3[2] = {0, 0};  // walls: cextrp2, cextrp3 of the jmin, jmax faces
+  // characteristic outflow (imax, jmax) and supersonic inlet (imin): face
+  // slabs (see slab_size) of what the reference keeps between the calls of
+  // its boundary routines -- the increments of the characteristic rows, the
+  // sound speed bc_charac_imax reads, and at the inlet the inflow state, the
+  // state whose primitives row 1 takes in the first stage and the filter
+  // increment of row 1 (K there is written by the filter only)
+  double *kimax = nullptr, *kjmax = nullptr, *cst = nullptr;
+  double *uin = nullptr, *ppre = nullptr, *kf = nullptr;
+  bool bc_start = false;   // start_runge_kutta pending (state uploaded)
   size_t bytes = 0;       // device memory held by the block
   bool have_metrics = false;
   bool have_state = false;
@@ -96,7 +105,9 @@ struct mgpu_handle {
   bool use_graph = true;
   hipGraphExec_t gexec[3] = {};
   bool mixed = false;  // FP32 flux/derivative/filter arithmetic
-  bool wall = false;   // some block has no-slip walls (wall variants of the stage and filter kernels)
+  bool wall = false;   // some block has no-slip walls
+  bool bnd = false;    // some block has boundary faces (closure variants of the stage and filter kernels)
+  bool charac = false; // some block has characteristic or inlet faces
   bool shared_ref = false;  // one mixed-precision reference state for all blocks (set from block 0)
   double *diag_buf = nullptr;  // per-workgroup partial sums of mgpu_diagnostics
   long diag_cap = 0;

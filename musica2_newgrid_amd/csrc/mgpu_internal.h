@@ -49,16 +49,29 @@ struct Geom {
   int sx;     // nx + 2G
   idx_t sy;   // sx*(ny+2G)
   int jwall;  // no-slip walls: bit 0 on the jmin face, bit 1 on the jmax face
+  // boundary faces of any kind (no-slip wall, characteristic outflow,
+  // supersonic inlet), at which the operators take their closures: bit 0 on
+  // the min face, bit 1 on the max face of axis i (ibnd) and j (jbnd)
+  int ibnd, jbnd;
   long n;     // total padded points
 };
 
-// Half-width of the centred stencil that row j may use along j: 5 inside,
-// j-1 (ny-j) near a jmin (jmax) wall, 0 on a wall row.
-__host__ __device__ inline int jhalf(const Geom &g, int j) {
+// Half-width of the centred stencil that row r (1..n) of an axis may use: 5
+// inside, r-1 (n-r) near a bounded min (max) face, 0 on the boundary row.
+__host__ __device__ inline int bhalf(int bnd, int n, int r) {
   int q = 5;
-  if ((g.jwall & 1) && j - 1 < q) q = j - 1;
-  if ((g.jwall & 2) && g.ny - j < q) q = g.ny - j;
+  if ((bnd & 1) && r - 1 < q) q = r - 1;
+  if ((bnd & 2) && n - r < q) q = n - r;
   return q;
+}
+__host__ __device__ inline int jhalf(const Geom &g, int j) { return bhalf(g.jbnd, g.ny, j); }
+__host__ __device__ inline int ihalf(const Geom &g, int i) { return bhalf(g.ibnd, g.nx, i); }
+
+// Face slabs of the boundary rows: one value per point of the face's row,
+// the i-faces indexed (j-1) + ny*(k-1), the j-faces (i-1) + nx*(k-1); a slab
+// of 5 variables stores them one after the other.
+__host__ __device__ inline long slab_size(const Geom &g, int axis) {
+  return axis == 0 ? (long)g.ny * g.nz : (long)g.nx * g.nz;
 }
 
 __host__ __device__ inline idx_t lin(const Geom &g, int i, int j, int k) {
@@ -214,6 +227,16 @@ struct StageArgs {
   // mixed precision: intermediate stage states as FP32 differences with ref
   const float *uin32[5];
   float *uout32[5];
+  // boundary rows, which get no Euler increment: their increment K per face
+  // imin, imax, jmin, jmax, as a face slab of 5 variables (the characteristic
+  // increment, or at a supersonic inlet the filter increment of the previous
+  // step); null: K = 0 and the viscous increment (no-slip walls).  A row of
+  // an i-face takes its face's slab also where it meets a j-face.
+  const double *kbc[4];
+  // supersonic inlet at imin: the i-flux of row 1 takes its conserved
+  // variables from uin_row (face slab) and its primitives from prim_row
+  // (face slab; first stage) or from the stage state (later stages)
+  const double *uin_row, *prim_row;
 };
 
 // Stage kernel variants: FP64, or mixed precision with the stage state read
@@ -299,9 +322,10 @@ struct WallArgs {
   int adiabatic;
 };
 
-// The viscous rows next to the j-walls of one block: tau on rows 1,2
-// (ny-1,ny) with one-sided j-gradients, the wall-row rescaling after the
-// exchange, and the viscous divergence on those rows.
+// The viscous rows next to the bounded faces of one block (rows 1,2 and
+// n-1,n of a bounded i or j axis): tau with the one-sided gradients along
+// the bounded axes, the wall-row rescaling after the exchange, and the
+// viscous divergence on those rows.
 struct WallViscArgs {
   Geom g;
   Gas gas;
@@ -311,8 +335,35 @@ struct WallViscArgs {
   double *kv[4];
   double *prs, *sens;  // shock filter: p and Ducros sensor of U^n
   int adiabatic;       // no wall-normal temperature gradient on the wall row
-  int want_sensor;     // write the sensor on rows 1,2 (ny-1,ny) and p beyond the walls
+  int want_sensor;     // write the sensor on those rows and p beyond the bounded faces
   int ducros;          // Ducros sensor (else 1)
+};
+
+// Characteristic (LODI) boundaries and the supersonic inlet of one block:
+// the increments of the characteristic rows from the primitives of a state
+// set, and the bookkeeping of the inflow row.
+struct CharArgs {
+  Geom g;
+  Gas gas;
+  double cp, p_ref;       // cp = gam rg/(gam-1); reference pressure of the imax relaxation
+  CState src;             // the state (FP64), or with src.v[0] null
+  const float *src32[5];  // FP32 differences with ref
+  double ref[5];
+  const double *idx_v, *idy_v;
+  double *kimax, *kjmax;  // out: increment slabs of the imax / jmax row (null: no such face)
+  double *cst;            // imax: sound speed primitives_visc last computed (face slab)
+  // cst_mode 1: after the increments, cst := c of state cnew (U^n, FP64);
+  // 2: before them, cst := c of src; 3: only cst := c of cnew, no increments
+  int cst_mode;
+  CState cnew;
+  // supersonic inlet at imin (null: none): inflow state, state whose
+  // primitives row 1 takes in the first stage, filter increment of row 1
+  double *uin, *ppre, *kf;
+  // inlet_mode 1 (end of a step): ppre := src row 1, then row 1 of row1 (the
+  // src set) := uin; 2 (start of a run): uin := ppre := src row 1, kf := 0
+  int inlet_mode;
+  State row1;
+  int i0, i1;             // jmax row: i = i0..i1
 };
 
 // Component c of tau in the flux along direction d of a Cartesian block (row
@@ -399,11 +450,11 @@ enum DissipKind {
 // shock filter's Edoh switch; `mixed`: FP32 faces of U - U_ref, FP64 update.
 hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, DissipKind kind,
                                bool wall, bool edoh, hipStream_t s);
-// The j-increment of the rows next to the j-walls of one block, after
-// launch_filter_shock(.., wall), in FP64 (the filter's input is the last
-// stage's FP64 output): the shock filter (a.avs = 0) or artvisc_o9_shock
-// (a.avs = 1).
-hipError_t launch_shock_wall(const FilterArgs &a, hipStream_t s);
+// The increment along axis (0: i, 1: j) of the rows next to the bounded
+// faces of that axis of one block, after launch_filter_shock(.., wall), in
+// FP64 (the filter's input is the last stage's FP64 output): the shock
+// filter (a.avs = 0) or artvisc_o9_shock (a.avs = 1, j only).
+hipError_t launch_shock_wall(const FilterArgs &a, int axis, hipStream_t s);
 hipError_t launch_iface(const IfaceArgs &a, int nchunks, hipStream_t s);
 hipError_t launch_iface(const IfaceArgs32 &a, int nchunks, hipStream_t s);
 long diag_groups(const Geom &g);
@@ -412,6 +463,9 @@ hipError_t launch_wall(const WallArgs &a, hipStream_t s);
 hipError_t launch_visc_wall(const WallViscArgs &a, hipStream_t s);
 hipError_t launch_tau_wallrow(const WallViscArgs &a, hipStream_t s);
 hipError_t launch_vdiv_wall(const WallViscArgs &a, hipStream_t s);
+hipError_t launch_charac(const CharArgs &a, hipStream_t s);
+// Filter increment of the inlet row: kf = after - before on row 1 (face slab).
+hipError_t launch_inlet_kf(const Geom &g, const CState &after, const CState &before, double *kf, hipStream_t s);
 hipError_t launch_diag(const DiagArgs &a, int geo, hipStream_t s);
 // Allocation-free diagnostics of one block (mgpu_diagnostics_noalloc): the
 // per-workgroup results and their totals go to a static device array of

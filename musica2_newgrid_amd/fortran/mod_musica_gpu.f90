@@ -13,7 +13,8 @@ module mod_musica_gpu
   use iso_c_binding
   implicit none
 
-  integer(c_int32_t), parameter :: MGPU_FACE_PERIODIC = 0, MGPU_FACE_INTERFACE = 1, MGPU_FACE_WALL = 2
+  integer(c_int32_t), parameter :: MGPU_FACE_PERIODIC = 0, MGPU_FACE_INTERFACE = 1, MGPU_FACE_WALL = 2, &
+                                   MGPU_FACE_CHARAC = 3, MGPU_FACE_INLET = 4
   integer(c_int32_t), parameter :: MGPU_FILTER_NONE = 0, MGPU_FILTER_SF = 1, MGPU_FILTER_SHOCK = 2
   integer(c_int32_t), parameter :: MGPU_FILTER_ARTVISC = 3, MGPU_FILTER_ARTVISC_SHOCK = 4
   integer(c_int32_t), parameter :: MGPU_PREC_FP64 = 0, MGPU_PREC_MIXED = 1
@@ -33,6 +34,7 @@ module mod_musica_gpu
      real(c_double) :: dissip_coeff, dissip_shock, csens
      integer(c_int32_t) :: is_adiab = 1, is_sw_edoh = 0
      real(c_double) :: t_wall = 0.0_c_double
+     real(c_double) :: p_ref = 0.0_c_double
   end type mgpu_case_t
 
   ! Hot-path fields of param.ini (read_param, src/File_IO/read_param.f90:88-205)

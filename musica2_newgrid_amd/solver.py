@@ -21,7 +21,9 @@ def make_case_struct(case, precision: int = _lib.PREC_FP64, topology=None, launc
     multiblock.py); None means one block with periodic faces."""
     if topology is None:
         faces = tuple(getattr(case, "faces", (0,) * 6))
-        kinds = [_lib.FACE_WALL if k == 2 else _lib.FACE_PERIODIC for k in faces]
+        # the case's face codes: 0 periodic, 2 wall, 3 characteristic outflow, 4 supersonic inlet
+        code = {2: _lib.FACE_WALL, 3: _lib.FACE_CHARAC, 4: _lib.FACE_INLET}
+        kinds = [code.get(k, _lib.FACE_PERIODIC) for k in faces]
         topology = [{"nx": case.nx, "ny": case.ny, "nz": case.nz, "kind": kinds,
                      "nbr_block": [0] * 6, "nbr_face": [1, 0, 3, 2, 5, 4], "orient": [0] * 6}]
     blocks = (MgpuBlock * len(topology))()
@@ -57,6 +59,7 @@ def make_case_struct(case, precision: int = _lib.PREC_FP64, topology=None, launc
     cs.is_adiab = int(getattr(case, "adiabatic", True))
     cs.is_sw_edoh = int(getattr(case, "sw_edoh", False))
     cs.t_wall = float(getattr(case, "t_wall", 0.0))
+    cs.p_ref = float(getattr(case, "p_ref", 0.0))
     return cs, blocks
 
 

@@ -24,8 +24,27 @@ def test_library_exports_all_symbols():
     assert not missing, missing
 
 
-def test_struct_layout_matches_header():
+def test_struct_layout_matches_header(tmp_path):
+    """Every field offset of the ctypes mirrors equals the C compiler's for
+    include/musica_gpu.h."""
+    import subprocess
     from musica2_newgrid_amd._lib import MgpuBlock, MgpuCase
     assert ctypes.sizeof(MgpuBlock) == 4 * (3 + 24)
-    # 8 int32, one pointer, 8 crk, deltat and 9 gas/filter doubles, 2 int32 and t_wall
-    assert ctypes.sizeof(MgpuCase) == 32 + 8 + 8 * (8 + 1 + 9) + 8 + 8
+    # 8 int32, one pointer, 8 crk, deltat and 9 gas/filter doubles, 2 int32, t_wall and p_ref
+    assert ctypes.sizeof(MgpuCase) == 32 + 8 + 8 * (8 + 1 + 9) + 8 + 8 + 8
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{ROOT}/include/musica_gpu.h"', "int main(void) {"]
+    for st, cname in ((MgpuCase, "mgpu_case"), (MgpuBlock, "mgpu_block")):
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in st._fields_:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                        check=True).stdout.splitlines())
+    for st, cname in ((MgpuCase, "mgpu_case"), (MgpuBlock, "mgpu_block")):
+        assert int(got[cname]) == ctypes.sizeof(st)
+        for f, _ in st._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(st, f).offset, (cname, f)

@@ -172,3 +172,48 @@ def test_walls_continuing_across_an_interface_are_accepted_by_the_check():
     if rc == 0:
         _lib.lib().mgpu_destroy(h)
     assert "wall row" not in msg
+
+
+def _shock_bl_table(**kw):
+    c = cases.shock_bl(24, 20, 6, **kw)
+    blk = {"nx": c.nx, "ny": c.ny, "nz": c.nz,
+           "kind": [_lib.FACE_INLET, _lib.FACE_CHARAC, _lib.FACE_WALL, _lib.FACE_CHARAC, _lib.FACE_PERIODIC,
+                    _lib.FACE_PERIODIC],
+           "nbr_block": [0] * 6, "nbr_face": [1, 0, 3, 2, 5, 4], "orient": [0] * 6}
+    return c, [blk]
+
+
+def test_characteristic_faces_are_checked():
+    """Characteristic outflow at imax / jmax and the supersonic inlet at imin
+    of Cartesian blocks whose other boundary faces are j-walls, with the
+    selective or shock filter (or none), the classic stage kernel and >= 10
+    cells along a bounded axis; anything else is refused before any device
+    work, with a message."""
+    def refused(c, blocks, text, flags=0):
+        cs, keep = make_case_struct(c, topology=blocks, launch_flags=flags)
+        h = C.c_void_p()
+        rc = _lib.lib().mgpu_create(C.byref(cs), C.byref(h))
+        msg = _lib.lib().mgpu_last_error(None).decode()
+        assert rc == _lib.EINVAL, (rc, msg)
+        assert text in msg, msg
+
+    c, b = _shock_bl_table()
+    b[0]["kind"][0] = _lib.FACE_CHARAC
+    refused(c, b, "imax and jmax")
+    c, b = _shock_bl_table()
+    b[0]["kind"][1] = _lib.FACE_INLET
+    refused(c, b, "imin")
+    c, b = _shock_bl_table()
+    b[0]["kind"][4] = b[0]["kind"][5] = _lib.FACE_WALL
+    refused(c, b, "j-faces only")
+    c, b = _shock_bl_table()
+    c.curvilinear = True
+    refused(c, b, "Cartesian")
+    c, b = _shock_bl_table()
+    c.filter = 4
+    refused(c, b, "filter 4")
+    c, b = _shock_bl_table()
+    refused(c, b, "classic", flags=_lib.LAUNCH_STAGE_PIPELINED)
+    c, b = _shock_bl_table()
+    b[0]["nx"] = 9
+    refused(c, b, ">= 10")
