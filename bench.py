@@ -256,6 +256,55 @@ def c2_leg(nx: int, ny: int, nz: int, steps: int, warmup: int, tune: bool, c3_va
     return res
 
 
+def shock_bl_leg(nx: int, ny: int, nz: int, steps: int, warmup: int, tune: bool, error_steps: int, mixed: bool):
+    """The shock/boundary-layer workload (Cartesian, Mach 2 oblique shock of a
+    10 degree wedge entering at jmax onto a Pohlhausen boundary layer,
+    supersonic inlet at imin, characteristic outflow at imax and jmax, no-slip
+    adiabatic wall at jmin, z periodic, seeded velocity noise): FP64 and mixed
+    throughput (graph replay), the stage kernel's roofline, and the density
+    of the mixed run against the FP64 run after `error_steps` steps."""
+    import numpy as np
+
+    from musica2_newgrid_amd import _lib, cases
+    case = cases.shock_bl(nx, ny, nz)
+    info = case.info
+    res = {"workload": f"Cartesian block {nx}x{ny}x{nz}, Mach {info['mach']} inflow (supersonic inlet at imin), "
+                       f"oblique shock of a {info['wedge_deg']} deg wedge (beta {info['beta_deg']:.2f} deg) entering "
+                       f"at jmax onto a Pohlhausen boundary layer (Re_delta {info['re_delta']:g}), characteristic "
+                       f"outflow at imax and jmax, no-slip adiabatic wall at jmin, z periodic, velocity noise "
+                       f"1e-4 U (seed {info['seed']}), viscous, shock filter + Ducros, RK4, graph replay",
+           "cells": case.ncells}
+    legs = [("fp64", _lib.PREC_FP64)] + ([("mixed", _lib.PREC_MIXED)] if mixed else [])
+    rho = {}
+    for name, prec in legs:
+        print(f"bench: shock/boundary-layer {name}, {steps} timed steps, {error_steps} in all", file=sys.stderr,
+              flush=True)
+        r = run_leg(case, prec, steps, warmup, tune, error_steps)
+        nst, ms_stage = r["times"]["stage"]
+        avg = ms_stage / nst / 1e3
+        byts = stage_words_per_cell(case) * 8.0 * case.ncells
+        val = case.ncells * case.nrk * steps / r["t"] / 1e6
+        res[name] = {"value": round(val, 2), "unit": "Mcell-RK-stages/s",
+                     "ms_per_step": round(r["t"] / steps * 1e3, 4),
+                     "stage_tiles": {k: {"tile": f"{v[0]}x{v[1]}", "kchunk": v[2]} for k, v in r["tuned"].items()},
+                     "stage_roofline": {"achieved_GBs": round(byts / avg / 1e9, 1),
+                                        "frac": round(byts / avg / 1e9 / HBM_PEAK_GBS, 4),
+                                        "avg_launch_ms": round(avg * 1e3, 4),
+                                        "algorithmic_bytes_per_launch": byts},
+                     "kernel_ms": {k: {"launches": n, "ms": round(ms, 3)} for k, (n, ms) in r["times"].items()},
+                     "steps_done": r["steps_done"]}
+        rho[name] = r["rho"][5:-5, 5:-5, 5:-5]
+        if not np.all(np.isfinite(rho[name])):
+            res[name]["error"] = "non-finite density"
+    if mixed:
+        a, b = rho["mixed"], rho["fp64"]
+        res["mixed"]["speedup_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
+        res["mixed"]["rho_linf_vs_fp64"] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        res["mixed"]["rho_l2_vs_fp64"] = float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2)))
+        res["mixed"]["after_steps"] = res["mixed"]["steps_done"]
+    return res
+
+
 def serve_leg(nreq: int, nsteps: int, seed: int = 5):
     """Serving loop: a Poisson stream of TGV / isentropic-vortex requests on
     cubes of edge 64..256, Re log-uniform in [400, 4000], FP64 or mixed,
@@ -300,6 +349,10 @@ def main() -> int:
     ap.add_argument("--no-tgv", action="store_true", help="skip the TGV 64^3 leg")
     ap.add_argument("--no-channel", action="store_true", help="skip the walled-channel leg")
     ap.add_argument("--no-c2", action="store_true", help="skip the 2.5-D curvilinear leg")
+    ap.add_argument("--no-shock-bl", action="store_true", help="skip the shock/boundary-layer leg")
+    ap.add_argument("--shock-bl-size", type=int, nargs=3, default=(1024, 512, 8), metavar=("NX", "NY", "NZ"))
+    ap.add_argument("--shock-bl-error-steps", type=int, default=2000,
+                    help="steps after which the shock/boundary-layer mixed density is compared with FP64")
     ap.add_argument("--dissipation", choices=["shock", "artvisc_shock"], default="shock",
                     help="dissipation of the main leg: the shock filter (filter_o10_shock) or the artificial "
                          "viscosity with shock capturing (artvisc_o9_shock)")
@@ -383,6 +436,10 @@ def main() -> int:
         print("bench: 2.5-D curvilinear leg", file=sys.stderr, flush=True)
         out["c2"] = c2_leg(args.nx, args.ny, args.nz, args.steps, args.warmup, not args.no_tune, value,
                            not args.no_mixed)
+    if not args.no_shock_bl:
+        print("bench: shock/boundary-layer leg", file=sys.stderr, flush=True)
+        out["shock_bl"] = shock_bl_leg(*args.shock_bl_size, args.steps, args.warmup, not args.no_tune,
+                                       args.shock_bl_error_steps, not args.no_mixed)
     if not args.no_tgv:
         print("bench: TGV 64^3 leg", file=sys.stderr, flush=True)
         out["tgv64"] = tgv_leg()
