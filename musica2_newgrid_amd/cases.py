@@ -223,6 +223,95 @@ def vortex_wavy(nx: int = 512, ny: int = 256, nz: int = 128, mach: float = 0.5, 
     return cs
 
 
+def tanh_two_walls(ny: int, Ly: float, ratio: float = 1.05):
+    """Computational coordinate eta(j) of rows j (any real j, smooth beyond
+    the walls) clustered at both walls j = 1 and j = ny by tanh stretching,
+    eta = Ly/2 (1 + tanh(b (2s - 1))/tanh(b)), s = (j-1)/(ny-1), with b
+    chosen so that the largest ratio of neighbouring spacings is `ratio`."""
+    def nodes(b, s):
+        return 0.5 * Ly * (1.0 + np.tanh(b * (2.0 * s - 1.0)) / math.tanh(b))
+
+    s = np.arange(ny) / (ny - 1)
+
+    def max_ratio(b):
+        d = np.diff(nodes(b, s))
+        return float(np.max(np.maximum(d[1:] / d[:-1], d[:-1] / d[1:])))
+
+    lo, hi = 1e-6, 10.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if max_ratio(mid) > ratio:
+            hi = mid
+        else:
+            lo = mid
+    b = lo
+    return (lambda j: nodes(b, (np.asarray(j, dtype=np.float64) - 1.0) / (ny - 1))), b
+
+
+def vortex_channel(nx: int = 512, ny: int = 256, nz: int = 128, mach: float = 0.5, beta: float = 5.0,
+                   radius: float = 0.5, re: float = 1.0e4, cfl: float = 0.8, filter: int = 2,
+                   lengths=None, amp: float = 0.05, ratio: float = 1.05, delta: float = 0.1,
+                   adiabatic: bool = True, T_inf: float = 300.0, rho_inf: float = 1.177,
+                   iorder_visc: int = 4) -> Case:
+    """The curvilinear headline block as stated: x uniform and periodic, y
+    between no-slip walls at j = 1 and j = ny with tanh clustering at both
+    (largest spacing ratio `ratio`), the wavy map y' = y + amp sin(2 pi x/Lx)
+    (1 - y/Ly) (a wavy lower wall, a flat upper one), z uniform and periodic;
+    3-D GCL metrics; an isentropic vortex (strength beta, radius R) advected
+    at Mach `mach` in the middle of the channel.  The free stream meets the
+    walls through a thin laminar layer u = U tanh(d/delta) of the distance d
+    to the nearer wall (no slip on the wall rows)."""
+    if lengths is None:
+        h = 10.0 / nx
+        lengths = (nx * h, ny * h, nz * h)
+    Lx, Ly, Lz = lengths
+    eta_of, b = tanh_two_walls(ny, Ly, ratio)
+    tp = 2.0 * math.pi
+
+    def mapping(XI, ET, ZE):
+        return XI, ET + amp * np.sin(tp * XI / Lx) * (1.0 - ET / Ly), ZE
+
+    m, mv, (xc, yc, zc), wall = gr.curvilinear_metrics_jwalls(mapping, eta_of, nx, ny, nz, lengths,
+                                                              viscous=iorder_visc != 0)
+    c_inf = math.sqrt(GAM * RG * T_inf)
+    u_inf = mach * c_inf
+    xv, yv = Lx / 2.0, Ly / 2.0
+    r2 = ((xc - xv) ** 2 + (yc - yv) ** 2) / radius ** 2
+    e = np.exp(0.5 * (1.0 - r2))
+    eta = eta_of(np.arange(1, ny + 1)).reshape(1, -1, 1)
+    dwall = np.minimum(eta, Ly - eta)
+    layer = np.tanh(dwall / delta)
+    u = u_inf * (layer - beta / (2.0 * math.pi) * (yc - yv) / radius * e)
+    v = u_inf * (beta / (2.0 * math.pi) * (xc - xv) / radius * e)
+    w = np.zeros_like(u)
+    T = T_inf * (1.0 - (GAM - 1.0) * beta ** 2 * mach ** 2 / (8.0 * math.pi ** 2) * e * e)
+    rho = rho_inf * (T / T_inf) ** (1.0 / (GAM - 1.0))
+    p = rho * RG * T
+    u[:, 0, :] = v[:, 0, :] = u[:, -1, :] = v[:, -1, :] = 0.0
+    U = []
+    for f in conserved(rho, u, v, w, p):
+        a = gr.padded(nx, ny, nz)
+        a[G:-G, G:-G, G:-G] = f
+        # ghost rows beyond the walls repeat the wall rows (only the shock
+        # filter's pressure sensor of a wall row reads them)
+        a[G:-G, :G, G:-G] = a[G:-G, G:G + 1, G:-G]
+        a[G:-G, G + ny:, G:-G] = a[G:-G, G + ny - 1:G + ny, G:-G]
+        gr.fill_periodic(a, axes=(0, 2))
+        U.append(a)
+    mu_ref = rho_inf * u_inf * radius / re
+    diffscale = mu_ref / float(sutherland(np.float64(T_inf)))
+    dmin = min(float(np.min(np.sqrt(np.diff(xc, axis=a) ** 2 + np.diff(yc, axis=a) ** 2 +
+                                    np.diff(zc, axis=a) ** 2))) for a in (0, 1, 2))
+    cs = Case("vortex_channel", nx, ny, nz, curvilinear=True, iorder_visc=iorder_visc, filter=filter,
+              ducros=True, diffscale=diffscale, m=m, mv=mv, U=U, faces=(0, 0, 2, 2, 0, 0),
+              adiabatic=adiabatic, t_wall=T_inf)
+    cs.wall_c3 = wall
+    cs.dt = timestep(U, dmin, cfl)
+    cs.info = dict(mach=mach, beta=beta, radius=radius, re=re, cfl=cfl, lengths=lengths, amp=amp,
+                   ratio=ratio, tanh_b=b, delta=delta, dmin=dmin)
+    return cs
+
+
 def wavy_mapping_2d(lengths, amp):
     """x = xi + ax sin(2pi eta/Ly), y = eta + ay sin(2pi xi/Lx): a smooth,
     periodic distortion in (x, y), extruded along z."""

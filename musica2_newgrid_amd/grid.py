@@ -215,6 +215,74 @@ A04 = (-25.0 / 12.0, 4.0, -3.0, 4.0 / 3.0, -1.0 / 4.0)
 A13 = (-1.0 / 4.0, -5.0 / 6.0, 3.0 / 2.0, -1.0 / 2.0, 1.0 / 12.0)
 
 
+def curvilinear_metrics_jwalls(mapping, eta_of, nx, ny, nz, lengths, viscous=True):
+    """Metrics of a block periodic in i and k with no-slip walls on its j-faces.
+
+    mapping(xi, eta, zeta) -> (x, y, z), periodic in xi and zeta (periods Lx,
+    Lz); eta_of(j) gives the computational coordinate of row j (rows 1 and ny
+    are the walls) and must extend smoothly beyond them, so that the
+    coordinates can be evaluated on a box extended by 10 layers and the metrics
+    computed with the centred 11-pt (5-pt, viscous) stencils everywhere, ghosts
+    beyond the walls included (the kernels never read metrics there).  The
+    reference computes its metrics from the grid file alone, with one-sided
+    stencils next to the walls (smod_grid_metrics1_c3.f90); this synthetic
+    set-up uses the analytic map instead.  i- and k-ghosts take the periodic
+    image.
+
+    Returns (m, mv, coords, wall): m, mv the 10 padded Euler / viscous arrays
+    (ksi_x..phi_z, ijacob3); coords the node coordinates on the interior;
+    wall[f] for f = 2 (jmin), 3 (jmax) the 8 per-point coefficients of the
+    wall routines (grid_normals.f90:243-300): -grad(ksi).grad(eta)/|grad(eta)|^2,
+    -grad(phi).grad(eta)/|grad(eta)|^2 (Euler metrics of the wall row) and
+    the unit tangents x_ksi/|x_ksi|, x_phi/|x_phi| (11-pt derivatives of the
+    coordinates), each an array (nz, nx)."""
+    ext = 10
+    Lx, Ly, Lz = lengths
+    e2 = ext + G
+    i = np.arange(1 - e2, nx + e2 + 1, dtype=np.float64)
+    j = np.arange(1 - e2, ny + e2 + 1, dtype=np.float64)
+    k = np.arange(1 - e2, nz + e2 + 1, dtype=np.float64)
+    XI = (i - 1) * (Lx / nx)
+    ET = eta_of(j)
+    ZE = (k - 1) * (Lz / nz)
+    Z3, Y3, X3 = np.meshgrid(ZE, ET, XI, indexing="ij")
+    xe, ye, ze = mapping(X3, Y3, Z3)
+    out = []
+    for coeffs in ((A11, A5) if viscous else (A11,)):
+        # coordinates extended by ext + G layers: the metrics of the padded box
+        m9, ij = gcl_metrics(xe, ye, ze, ext, coeffs)
+        arrs = []
+        for f in m9 + [ij]:
+            a = np.ascontiguousarray(f)
+            fill_periodic(a, axes=(0, 2))
+            arrs.append(a)
+        out.append(arrs)
+    if not viscous:
+        out.append(None)
+    m = out[0]
+    # unit tangents along ksi and phi of the wall rows
+    wall = {}
+    for f, row in ((2, 1), (3, ny)):
+        jr = G + row - 1 + ext  # row index in the extended box
+        line = lambda a: a[:, jr, :]  # noqa: E731
+        xk = [_deriv(line(a), 1, A11) for a in (xe, ye, ze)]  # (nz+2e2, nx+2(e2-5))
+        xp = [_deriv(line(a), 0, A11) for a in (xe, ye, ze)]
+        ck = [c[e2:e2 + nz, e2 - 5:e2 - 5 + nx] for c in xk]
+        cp = [c[e2 - 5:e2 - 5 + nz, e2:e2 + nx] for c in xp]
+        gk = np.sqrt(ck[0] ** 2 + ck[1] ** 2 + ck[2] ** 2)
+        gp = np.sqrt(cp[0] ** 2 + cp[1] ** 2 + cp[2] ** 2)
+        w = lambda q: m[q][G:G + nz, G + row - 1, G:G + nx]  # noqa: E731
+        n2 = w(3) ** 2 + w(4) ** 2 + w(5) ** 2
+        g1 = -(w(0) * w(3) + w(1) * w(4) + w(2) * w(5)) / n2
+        g2 = -(w(6) * w(3) + w(7) * w(4) + w(8) * w(5)) / n2
+        g1[np.abs(g1) < 1e-16] = 0.0
+        g2[np.abs(g2) < 1e-16] = 0.0
+        wall[f] = [np.ascontiguousarray(a) for a in (g1, g2, ck[0] / gk, ck[1] / gk, ck[2] / gk,
+                                                    cp[0] / gp, cp[1] / gp, cp[2] / gp)]
+    coords = tuple(a[e2:e2 + nz, e2:e2 + ny, e2:e2 + nx] for a in (xe, ye, ze))
+    return out[0], out[1], coords, wall
+
+
 def wall_metrics(ye: np.ndarray, lo_wall: bool, hi_wall: bool):
     """Metrics of one axis whose min and/or max face is a wall, from the node
     coordinates ye of rows 1-G..n+G (padded; the wall rows are rows 1 and n,

@@ -475,9 +475,23 @@ static void gradients(const orc_cfg *c, const orc_grid *g, const dims_t *d, work
           w->grad[3 * q + 1][p] = gy;
           w->grad[3 * q + 2][p] = gz;
         }
+        /* adiabatic c3 j-walls: on the wall row the temperature gradient is
+         * the index-space derivatives along ksi and phi times the unit tangents
+         * x_ksi/|x_ksi|, x_phi/|x_phi| (smod_grad_T_c3.f90:80-85, 194-197,
+         * 249-275 as written: the derivatives are not divided by the tangents'
+         * lengths) */
+        if (c->curv == 1 && c->adiabatic && ((Wall.lo[1] && j == 1) || (Wall.hi[1] && j == d->ny))) {
+          const int f = j == 1 ? 2 : 3;
+          const long q = (long)(i - 1) + (long)d->nx * (k - 1);
+          const double dk = g5w(src[3], p, 1, i, d->nx, W.lo[0], W.hi[0]);
+          const double dp = g5w(src[3], p, sz, k, d->nz, W.lo[2], W.hi[2]);
+          for (int a = 0; a < 3; a++)
+            w->grad[DTX + a][p] = ORC_PLANT == 9 ? dk * g->wc3[f][5 + a][q] + dp * g->wc3[f][2 + a][q]
+                                                 : dk * g->wc3[f][2 + a][q] + dp * g->wc3[f][5 + a][q];
+        }
         /* adiabatic walls: no wall-normal temperature gradient on the wall row
          * (smod_grad_T.f90:102-109 and its twins) */
-        if (c->adiabatic && Wall.any) {
+        if (c->curv != 1 && c->adiabatic && Wall.any) {
           const int rr[3] = {i, j, k}, nn[3] = {d->nx, d->ny, d->nz};
           for (int a = 0; a < 3; a++)
             if ((Wall.lo[a] && rr[a] == 1) || (Wall.hi[a] && rr[a] == nn[a])) w->grad[DTX + a][p] = 0.0;
@@ -914,6 +928,38 @@ static void bc_wall(const orc_cfg *c, const orc_grid *g, const dims_t *d, double
     if (!(side ? W.hi[a] : W.lo[a])) continue;
     const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
     const long s = side ? -st[a] : st[a]; /* from the wall into the block */
+    if (c->curv == 1) {
+      /* c3 j-wall (bc_wall_jmin/jmax_adiabatic_dpdn_c3, _isotherm_dpdn_c3,
+       * smod_bc_wall_c3.f90:842-1065, 1068-1255, 1258-1671): p (and T) from
+       * rows 2, 3 along eta plus the tangential terms
+       *   v_wall = (4 v2 - v3 + (v(i+1,1) - v(i-1,1)) gksigeta3 + (v(i,1,k+1) - v(i,1,k-1)) gphigeta3)/3
+       * with the wall row's own neighbours (i and k run over interfaces or
+       * periodic faces here, so no edge rows) */
+      const double *g1 = g->wc3[f][0], *g2 = g->wc3[f][1];
+#pragma omp parallel for
+      for (int k = 1; k <= d->nz; k++)
+        for (int i = 1; i <= d->nx; i++) {
+          const int j1 = side ? d->ny : 1;
+          const long p1 = (long)ID(*d, i, j1, k);
+          const long p2 = p1 + s, p3 = p1 + 2 * s;
+          const long q = (long)(i - 1) + (long)d->nx * (k - 1);
+          const long si = 1, sk = st[2];
+          const double gk = (ORC_PLANT == 8 ? -1.0 : 1.0) * g1[q], gp = g2[q];
+          const double t_wall =
+              c->adiabatic ? (1.0 / 3.0) * (4.0 * w->tmp[p2] - w->tmp[p3] + (w->tmp[p1 + si] - w->tmp[p1 - si]) * gk +
+                                             (w->tmp[p1 + sk] - w->tmp[p1 - sk]) * gp)
+                           : c->t_wall;
+          const double p_wall = (1.0 / 3.0) * (4.0 * w->prs[p2] - w->prs[p3] + (w->prs[p1 + si] - w->prs[p1 - si]) * gk +
+                                                (w->prs[p1 + sk] - w->prs[p1 - sk]) * gp);
+          const double ro = p_wall / (c->rg * t_wall);
+          Un[1][p1] = 0.0;
+          Un[2][p1] = 0.0;
+          Un[3][p1] = 0.0;
+          Un[0][p1] = ro;
+          Un[4][p1] = ro * (cv * t_wall);
+        }
+      continue;
+    }
     const double c2 = g->cextrp2[f], c3 = g->cextrp3[f];
 #pragma omp parallel for
     for (int q2 = 1; q2 <= nn[t2]; q2++)
@@ -1246,7 +1292,12 @@ int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_gri
     for (int f = 0; f < 6; f++)
       if (bl[b].wall[f]) {
         const int ext = f < 2 ? bl[b].nx : (f < 4 ? bl[b].ny : bl[b].nz);
-        if (c0->curv) return -5;        /* boundaries: Cartesian blocks only */
+        /* boundaries: Cartesian blocks; c3 blocks take no-slip walls on their
+         * j-faces, with interfaces or periodic faces on i and k */
+        if (c0->curv == 2) return -5;
+        if (c0->curv == 1 && (bl[b].wall[f] != 1 || f < 2 || f > 3 || bl[b].wall[0] || bl[b].wall[1] ||
+                              bl[b].wall[4] || bl[b].wall[5]))
+          return -5;
         if (ext < 10) return -7;        /* closures of the two sides must not overlap */
         if (bl[b].wall[f] < 1 || bl[b].wall[f] > 3) return -12;
         /* characteristic outflow at imax or jmax, supersonic inlet at imin

@@ -30,7 +30,8 @@ extern "C" {
 enum orc_face_kind {
   ORC_FACE_PERIODIC = 0,   /* neighbour block is the block itself          */
   ORC_FACE_INTERFACE = 1,  /* 1-to-1 interface with another block          */
-  ORC_FACE_WALL = 2,       /* no-slip wall, sort=0 (is_wall2 = F): Cartesian */
+  ORC_FACE_WALL = 2,       /* no-slip wall, sort=0 (is_wall2 = F): Cartesian, or the
+                            * j-faces of c3 blocks                          */
   ORC_FACE_CHARAC = 3,     /* characteristic (LODI) outflow, sort=-3 at imax or jmax:
                             * bc_charac_imax / bc_charac_jmax, src/Boundary/mod_charac.f90:350-653,
                             * 816-969 (Cartesian)                              */
@@ -80,6 +81,13 @@ typedef struct orc_grid {
    * (*_v, 5-pt); z uses idz / idz_v above (mod_grid.f90, grid_metrics_curv.f90) */
   const double *m2[5];
   const double *m2v[5];
+  /* c3 no-slip walls on the j-faces (jmin = 2, jmax = 3; the rest unused):
+   * per wall point, i fastest, index (i-1) + nx*(k-1), the coefficients
+   * grid_normals.f90:243-300 computes from the metrics and the grid:
+   * [0] gksigeta3 = -grad(ksi).grad(eta)/|grad(eta)|^2, [1] gphigeta3 (phi for
+   * ksi), [2..4] the unit tangent x_ksi/|x_ksi| (txn_ksi, tyn_ksi, tzn_ksi),
+   * [5..7] x_phi/|x_phi| (txn_phi, tyn_phi, tzn_phi) */
+  const double *wc3[6][8];
 } orc_grid;
 
 /* One block of a multiblock case.  Face f (imin,imax,jmin,jmax,kmin,kmax =

@@ -54,6 +54,7 @@ class OrcGrid(C.Structure):
         ("cextrp2", C.c_double * 6), ("cextrp3", C.c_double * 6),
         ("m2", _dp * 5),
         ("m2v", _dp * 5),
+        ("wc3", (_dp * 8) * 6),
     ]
 
 
@@ -158,6 +159,12 @@ class Oracle:
         for f in range(6):
             g.cextrp2[f] = float(getattr(case, "cextrp2", (0.0,) * 6)[f])
             g.cextrp3[f] = float(getattr(case, "cextrp3", (0.0,) * 6)[f])
+        # c3 walls: the per-point coefficients of each wall face (8 slabs)
+        for f, slabs in (getattr(case, "wall_c3", None) or {}).items():
+            for q in range(8):
+                a = np.ascontiguousarray(slabs[q], dtype=np.float64)
+                self._keep.append(a)
+                g.wc3[f][q] = _p(a)
         self.grid = g
 
     @staticmethod

@@ -22,6 +22,11 @@ PLANTS = {
     5: ("tests/test_charac_oracle.py", "test_imax_subsonic_relaxation_at_rest"),
     6: ("tests/test_charac_oracle.py", "test_jmax_subsonic_outflow_and_inflow"),
     7: ("tests/test_charac_oracle.py", "test_imax_supersonic_outflow_and_closures"),
+    # c3 walls: the tangential term of the wall extrapolation with the sign a
+    # second-order dp/dn = 0 would give, the two tangents of the adiabatic
+    # gradient exchanged
+    8: ("tests/test_c3_wall_oracle.py", "test_sheared_wall_state_is_the_closed_form"),
+    9: ("tests/test_c3_wall_oracle.py", "test_unit_c3_walls_equal_cartesian_walls"),
 }
 
 
