@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 # FETCH_SIZE scaled by 2.000 and WRITE_SIZE by 1.000 (their readings for a
 # 1 GiB device copy).  Summary: profiles/step_kernels_c3_512x256x128_fp64.md.
 STAGE_TRAFFIC_BYTES = (3 * 4.589e9 + 5.338e9) / 4
-STAGE_TRAFFIC_WORKLOAD = (512, 256, 128)
+STAGE_TRAFFIC_WORKLOAD = ("periodic", 512, 256, 128)
 
 
 def stage_words_per_cell(case) -> float:
@@ -350,6 +350,9 @@ def main() -> int:
     ap.add_argument("--no-channel", action="store_true", help="skip the walled-channel leg")
     ap.add_argument("--no-c2", action="store_true", help="skip the 2.5-D curvilinear leg")
     ap.add_argument("--no-shock-bl", action="store_true", help="skip the shock/boundary-layer leg")
+    ap.add_argument("--headline", choices=["channel", "periodic"], default="periodic",
+                    help="the curvilinear headline block: the walled channel as stated, or the fully periodic "
+                         "wavy block")
     ap.add_argument("--shock-bl-size", type=int, nargs=3, default=(1024, 512, 8), metavar=("NX", "NY", "NZ"))
     ap.add_argument("--shock-bl-error-steps", type=int, default=2000,
                     help="steps after which the shock/boundary-layer mixed density is compared with FP64")
@@ -373,7 +376,21 @@ def main() -> int:
 
     diss = {"shock": (2, "shock filter + Ducros"),
             "artvisc_shock": (4, "artificial viscosity with shock capturing + Ducros")}[args.dissipation]
-    case = cases.vortex_wavy(args.nx, args.ny, args.nz, filter=diss[0])
+    if args.headline == "channel":
+        case = cases.vortex_channel(args.nx, args.ny, args.nz, filter=diss[0])
+        inf = case.info
+        workload = (f"curvilinear channel {args.nx}x{args.ny}x{args.nz}: x uniform and periodic, no-slip adiabatic "
+                    f"walls on both y-faces with tanh clustering (largest neighbouring-spacing ratio "
+                    f"{inf['ratio']}), wavy lower wall y' = y + {inf['amp']} sin(2 pi x/Lx)(1 - y/Ly), z uniform and "
+                    f"periodic, 3-D GCL metrics; isentropic vortex (strength 5, radius 0.5) at Mach 0.5 in a "
+                    f"free stream meeting the walls through a tanh layer of thickness {inf['delta']}; viscous, "
+                    f"{diss[1]} (quiescent), RK4, FP64, steps replayed from the captured HIP graph")
+    else:
+        case = cases.vortex_wavy(args.nx, args.ny, args.nz, filter=diss[0])
+        workload = (f"curvilinear block {args.nx}x{args.ny}x{args.nz} with 3-D GCL metrics of a fully "
+                    f"periodic wavy map (x, y and z each displaced by a product of sines; periodic "
+                    f"faces, no walls, no tanh stretching), isentropic vortex M=0.5, viscous, "
+                    f"{diss[1]}, RK4, FP64, steps replayed from the captured HIP graph")
     total = 0 if args.no_mixed else args.error_steps
     print(f"bench: {args.nx}x{args.ny}x{args.nz} FP64 leg, {args.steps} steps", file=sys.stderr, flush=True)
     fp = run_leg(case, _lib.PREC_FP64, args.steps, args.warmup, not args.no_tune, total)
@@ -397,10 +414,7 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"curvilinear block {args.nx}x{args.ny}x{args.nz} with 3-D GCL metrics of a fully "
-                               f"periodic wavy map (x, y and z each displaced by a product of sines; periodic "
-                               f"faces, no walls, no tanh stretching), isentropic vortex M=0.5, viscous, "
-                               f"{diss[1]}, RK4, FP64, steps replayed from the captured HIP graph",
+        "config": {"workload": workload,
                    "cells": case.ncells, "nrk": case.nrk, "precision": "fp64",
                    "stage_tiles": {k: {"tile": f"{v[0]}x{v[1]}", "kchunk": v[2]} for k, v in fp["tuned"].items()}},
         "timing": {"value": "graph replay of the timed steps",
@@ -408,8 +422,8 @@ def main() -> int:
                                f"launched one by one ({fp['t_eager'] / args.steps * 1e3:.4f} ms per step)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": (STAGE_TRAFFIC_BYTES if (args.nx, args.ny, args.nz) == STAGE_TRAFFIC_WORKLOAD
-                                 else None),
+                     "traffic": (STAGE_TRAFFIC_BYTES if (args.headline, args.nx, args.ny, args.nz) ==
+                                 STAGE_TRAFFIC_WORKLOAD else None),
                      "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, "
                                        "profiles/step_kernels_c3_512x256x128_fp64.md",
                      "kernel": "k_stage", "avg_launch_ms": round(avg_stage_s * 1e3, 4),

@@ -57,12 +57,14 @@ extern "C" {
 #define MGPU_FACE_INTERFACE 1
 /* No-slip wall, BC_face%sort = 0 with is_wall2 = F (the reference's default,
  * src/File_IO/read_param.f90:75): bc_wall_*_adiabatic_dpdn or
- * _isotherm_dpdn (src/Boundary/smod_bc_wall.f90) with the boundary-row
- * closures of the Euler, viscous and filter operators.  On the faces of one
- * axis per block (i, j or k; a block with walls on its i- or k-faces is stored
- * with that axis exchanged with j, invisibly to the caller) of Cartesian cases
- * (FP64 or mixed, any filter), at least 10 cells between the walls; the
- * block's extrapolation coefficients come from mgpu_set_wall.
+ * _isotherm_dpdn (src/Boundary/smod_bc_wall.f90, smod_bc_wall_c3.f90) with
+ * the boundary-row closures of the Euler, viscous and filter operators.  On
+ * the faces of one axis per block (i, j or k; a block with walls on its i- or
+ * k-faces is stored with that axis exchanged with j, invisibly to the caller)
+ * of Cartesian cases, or on the j-faces of 3-D curvilinear cases (FP64 or
+ * mixed, any filter), at least 10 cells between the walls; the block's
+ * extrapolation coefficients come from mgpu_set_wall (Cartesian) or
+ * mgpu_set_wall_c3 (curvilinear).
  * With a shock sensor (filters 2, 4) the ghost row beyond a wall must hold a
  * valid state:
  * the wall row's pressure sensor reads p there (mod_filtering_shock.f90:
@@ -205,6 +207,18 @@ int mgpu_set_metrics_c2(mgpu_handle *h, int block, const double *const *m2, cons
  * (ny-1..ny-4), the idy2/idy4/idy6/idy8 of grid_metrics_wall_cart
  * (src/Grid/grid_metrics_cart.f90:190-436), which the closure rows use. */
 int mgpu_set_wall(mgpu_handle *h, int block, const double *cextrp2, const double *cextrp3);
+
+/* Wall coefficients of a j-face (face 2 or 3) of a 3-D curvilinear block
+ * (MGPU_GEOM_C3, where walls sit on j-faces only), as grid_normals.f90:
+ * 232-300 computes them for bc_wall_jmin/jmax_*_dpdn_c3
+ * (src/Boundary/smod_bc_wall_c3.f90:842-1671) and the adiabatic wall's
+ * tangential temperature gradient (src/Compute_flow/smod_grad_T_c3.f90:
+ * 249-275): coef[0..7] are arrays of nx*nz doubles (i fastest) over the wall
+ * points: gksigeta3 = -grad(ksi).grad(eta)/|grad(eta)|^2, gphigeta3 (phi for
+ * ksi), the unit tangent x_ksi/|x_ksi| (x, y, z: txn_ksi, tyn_ksi, tzn_ksi)
+ * and x_phi/|x_phi| (txn_phi, tyn_phi, tzn_phi).  Required for every wall
+ * face of such a block before its first step. */
+int mgpu_set_wall_c3(mgpu_handle *h, int block, int face, const double *const *coef);
 
 /* Upload the conserved variables rho,rhou,rhov,rhow,rhoe of a block (padded
  * arrays; ghosts are refreshed on the device). */

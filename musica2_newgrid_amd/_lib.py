@@ -34,6 +34,7 @@ EXPORTS = (
     "mgpu_set_metrics_c3",
     "mgpu_set_metrics_c2",
     "mgpu_set_wall",
+    "mgpu_set_wall_c3",
     "mgpu_put_state",
     "mgpu_get_state",
     "mgpu_step",
@@ -126,6 +127,7 @@ def lib() -> C.CDLL:
     L.mgpu_set_metrics_c3.argtypes = [C.c_void_p, C.c_int, C.POINTER(_dp), C.POINTER(_dp)]
     L.mgpu_set_metrics_c2.argtypes = [C.c_void_p, C.c_int, C.POINTER(_dp), C.POINTER(_dp), _dp, _dp]
     L.mgpu_set_wall.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    L.mgpu_set_wall_c3.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(_dp)]
     L.mgpu_put_state.argtypes = [C.c_void_p, C.c_int] + [_dp] * 5
     L.mgpu_get_state.argtypes = [C.c_void_p, C.c_int] + [_dp] * 5
     L.mgpu_step.argtypes = [C.c_void_p, C.c_int]
@@ -167,6 +169,7 @@ def lib() -> C.CDLL:
                  "mgpu_server_take"):
         getattr(L, name).restype = C.c_int
     for name in ("mgpu_create", "mgpu_set_metrics_cart", "mgpu_set_metrics_c3", "mgpu_set_metrics_c2", "mgpu_set_wall",
+                 "mgpu_set_wall_c3",
                  "mgpu_put_state",
                  "mgpu_get_state", "mgpu_step", "mgpu_autotune", "mgpu_stage_config", "mgpu_sync", "mgpu_profile", "mgpu_kernel_time"):
         getattr(L, name).restype = C.c_int

@@ -244,7 +244,10 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
   constexpr bool CURV = GEO != kGeoCart;
   constexpr bool C3 = GEO == kGeoC3;
   constexpr bool C2 = GEO == kGeoC2;
-  static_assert(!WALL || !CURV, "walls: Cartesian blocks");
+  static_assert(!WALL || !C2, "walls: Cartesian or c3 blocks");
+  // bounded i-faces (characteristic outflow, inlet): Cartesian blocks only;
+  // c3 blocks take walls on their j-faces
+  constexpr bool IBND = WALL && !CURV;
   constexpr bool MIX = MODE != kFP64;
   constexpr bool IN32 = MODE == kMixMiddle || MODE == kMixLast;
   constexpr bool OUT32 = MODE == kMixFirst || MODE == kMixMiddle;
@@ -355,11 +358,13 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
                                     : (((g.ibnd & 1) && gi < 1) || ((g.ibnd & 2) && gi > g.nx));
     }
     own_beyond = (g.jbnd & 2) && j > g.ny;
-    own_beyond_x = (g.ibnd & 2) && i > g.nx;
     hw = jhalf(g, j);
     wall_coeffs(hw, cw);
-    hwi = ihalf(g, i);
-    wall_coeffs(hwi, ci);
+    if constexpr (IBND) {
+      own_beyond_x = (g.ibnd & 2) && i > g.nx;
+      hwi = ihalf(g, i);
+      wall_coeffs(hwi, ci);
+    }
     P.kbc[0] = a.kbc[0];
     P.kbc[1] = a.kbc[1];
     P.kbc[2] = a.kbc[2];
@@ -431,7 +436,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
       const Prim pq = prim_raw(ru, gam1);
       R F[5];
       if (needx) {
-        if (WALL && i == 1 && P.uin_row) {
+        if (IBND && i == 1 && P.uin_row) {
           // supersonic inlet: row 1's conserved variables are the inflow
           // state, its primitives those computed before the inflow state was
           // imposed (bc_supersonic_inlet runs after `primitives`)
@@ -450,7 +455,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
           flux_r<CURV, MIX>(pq, 0, rxm, rf, F);
         }
 #pragma unroll
-        for (int v = 0; v < 5; v++) sfx[v][ty][tx + 5] = (WALL && own_beyond_x) ? R(0) : F[v];
+        for (int v = 0; v < 5; v++) sfx[v][ty][tx + 5] = (IBND && own_beyond_x) ? R(0) : F[v];
       }
       if (needy) {
         flux_r<CURV, MIX>(pq, 1, rym, rf, F);
@@ -508,7 +513,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
 #pragma unroll
       for (int v = 0; v < 5; v++) {
         R dx;
-        if constexpr (WALL) {
+        if constexpr (IBND) {
           const R *f = &sfx[v][ty][tx + 5];
           dx = R(ci[0]) * (f[1] - f[-1]) + R(ci[1]) * (f[2] - f[-2]) + R(ci[2]) * (f[3] - f[-3]) +
                R(ci[3]) * (f[4] - f[-4]) + R(ci[4]) * (f[5] - f[-5]);
@@ -1146,7 +1151,17 @@ hipError_t launch_t(const Batch<StageArgs> *tab, int nb, int nt, int geo, bool v
   constexpr bool VOK = MODE == kFP64 || MODE == kMixLast;
   if (visc && !VOK) return hipErrorInvalidValue;
   if (wall) {
-    if (geo != kGeoCart) return hipErrorInvalidValue;
+    if (geo == kGeoC2) return hipErrorInvalidValue;
+    if (geo == kGeoC3) {
+      if constexpr (VOK) {
+        if (visc) {
+          hipLaunchKernelGGL((k_stage<TX, TY, kGeoC3, true, MODE, true>), grd, blk, 0, s, tab, nb, nt);
+          return hipGetLastError();
+        }
+      }
+      hipLaunchKernelGGL((k_stage<TX, TY, kGeoC3, false, MODE, true>), grd, blk, 0, s, tab, nb, nt);
+      return hipGetLastError();
+    }
     if constexpr (VOK) {
       if (visc) {
         hipLaunchKernelGGL((k_stage<TX, TY, kGeoCart, true, MODE, true>), grd, blk, 0, s, tab, nb, nt);

@@ -191,14 +191,26 @@ __global__ __launch_bounds__(kWallThreads) void k_wall(WallArgs a) {
   const int j1 = side ? g.ny : 1, dj = side ? -1 : 1;
   double rho, rhoe;
   if (a.src.v[0] || a.src32[0]) {
+    auto prim = [&](idx_t q) {
+      return a.src.v[0] ? prim_at(a.src, q, a.gas.gam1, a.gas.rg) : prim_at32(a.src32, a.ref, q, a.gas.gam1, a.gas.rg);
+    };
     const idx_t p2 = lin(g, i, j1 + dj, k), p3 = lin(g, i, j1 + 2 * dj, k);
-    const Pw q2 = a.src.v[0] ? prim_at(a.src, p2, a.gas.gam1, a.gas.rg)
-                             : prim_at32(a.src32, a.ref, p2, a.gas.gam1, a.gas.rg);
-    const Pw q3 = a.src.v[0] ? prim_at(a.src, p3, a.gas.gam1, a.gas.rg)
-                             : prim_at32(a.src32, a.ref, p3, a.gas.gam1, a.gas.rg);
-    const double c2 = a.c2[side], c3 = a.c3[side];
-    const double tw = a.adiabatic ? c2 * q2.T - c3 * q3.T : a.t_wall;
-    const double pw = c2 * q2.p - c3 * q3.p;
+    const Pw q2 = prim(p2), q3 = prim(p3);
+    double tw, pw;
+    if (a.wc3[side]) {
+      // c3 wall (bc_wall_jmin/jmax_*_dpdn_c3, smod_bc_wall_c3.f90:842-1671):
+      // rows 2, 3 along eta plus the tangential terms with the wall row's own
+      // neighbours along ksi and phi (gksigeta3, gphigeta3)
+      const idx_t p1 = lin(g, i, j1, k);
+      const Pw xm = prim(p1 - 1), xp = prim(p1 + 1), zm = prim(p1 - g.sy), zp = prim(p1 + g.sy);
+      const double gk = ldg(a.wc3[side], (idx_t)r), gp = ldg(a.wc3[side], (idx_t)(nface + r));
+      tw = a.adiabatic ? (1.0 / 3.0) * (4.0 * q2.T - q3.T + (xp.T - xm.T) * gk + (zp.T - zm.T) * gp) : a.t_wall;
+      pw = (1.0 / 3.0) * (4.0 * q2.p - q3.p + (xp.p - xm.p) * gk + (zp.p - zm.p) * gp);
+    } else {
+      const double c2 = a.c2[side], c3 = a.c3[side];
+      tw = a.adiabatic ? c2 * q2.T - c3 * q3.T : a.t_wall;
+      pw = c2 * q2.p - c3 * q3.p;
+    }
     rho = pw / (a.gas.rg * tw);
     rhoe = rho * (a.cv * tw);
   } else {
@@ -261,6 +273,8 @@ __global__ __launch_bounds__(kWallThreads) void k_visc_wall(WallViscArgs a) {
   const idx_t p = lin(g, i, j, k);
   const int qi = oslot(g.ibnd, g.nx, i), qj = oslot(g.jbnd, g.ny, j);
 
+  // no-slip wall rows
+  const bool wrow = ((g.jwall & 1) && j == 1) || ((g.jwall & 2) && j == g.ny);
   // gradients gr[u,v,w,T][x,y,z]
   double gr[4][3];
   {
@@ -268,19 +282,44 @@ __global__ __launch_bounds__(kWallThreads) void k_visc_wall(WallViscArgs a) {
     grad5(a.u0, g, p, 1, qi, i, g.nx, gam1, rg, gx);
     grad5(a.u0, g, p, (idx_t)g.sx, qj, j, g.ny, gam1, rg, gy);
     grad5(a.u0, g, p, g.sy, -1, k, g.nz, gam1, rg, gz);
-    const double ix = ldg(a.met.idx_v, (idx_t)(i - 1 + G)), iy = ldg(a.met.idy_v, (idx_t)(j - 1 + G)),
-                 iz = ldg(a.met.idz_v, (idx_t)(k - 1 + G));
+    if (a.geo == kGeoC3) {
+      // index-space derivatives mapped with the viscous metrics
+      // (smod_grad_vel_c3.f90:260-300, smod_grad_T_c3.f90:200-213)
+      double m[10];
 #pragma unroll
-    for (int n = 0; n < 4; n++) {
-      gr[n][0] = gx[n] * ix;
-      gr[n][1] = gy[n] * iy;
-      gr[n][2] = gz[n] * iz;
+      for (int q = 0; q < 10; q++) m[q] = ldg(a.met.mv[q], p);
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const double dvx = gx[n] * m[0] + gy[n] * m[3] + gz[n] * m[6];
+        const double dvy = gx[n] * m[1] + gy[n] * m[4] + gz[n] * m[7];
+        const double dvz = gx[n] * m[2] + gy[n] * m[5] + gz[n] * m[8];
+        gr[n][0] = dvx * m[9];
+        gr[n][1] = dvy * m[9];
+        gr[n][2] = dvz * m[9];
+      }
+      if (a.adiabatic && wrow) {
+        // adiabatic c3 wall: the index-space derivatives along ksi and phi
+        // times the unit tangents, as written (smod_grad_T_c3.f90:249-275)
+        const double *wc = a.wc3[j == 1 ? 0 : 1];
+        const idx_t nf = (idx_t)g.nx * (idx_t)g.nz, q = (idx_t)(i - 1) + (idx_t)g.nx * (idx_t)(k - 1);
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+          gr[3][c] = gx[3] * ldg(wc, (idx_t)(2 + c) * nf + q) + gz[3] * ldg(wc, (idx_t)(5 + c) * nf + q);
+      }
+    } else {
+      const double ix = ldg(a.met.idx_v, (idx_t)(i - 1 + G)), iy = ldg(a.met.idy_v, (idx_t)(j - 1 + G)),
+                   iz = ldg(a.met.idz_v, (idx_t)(k - 1 + G));
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        gr[n][0] = gx[n] * ix;
+        gr[n][1] = gy[n] * iy;
+        gr[n][2] = gz[n] * iz;
+      }
+      // an adiabatic wall has no wall-normal temperature gradient on its row
+      // (smod_grad_T.f90:102-109)
+      if (a.adiabatic && wrow) gr[3][1] = 0.0;
     }
   }
-  // no-slip wall rows: an adiabatic wall has no wall-normal temperature
-  // gradient there (smod_grad_T.f90:102-109)
-  const bool wrow = ((g.jwall & 1) && j == 1) || ((g.jwall & 2) && j == g.ny);
-  if (a.adiabatic && wrow) gr[3][1] = 0.0;
   const Pw o = prim_at(a.u0, p, gam1, rg);
   if (a.want_sensor) {
     // the shock filter's Ducros sensor from these gradients
@@ -362,9 +401,54 @@ __global__ __launch_bounds__(kWallThreads) void k_vdiv_wall(WallViscArgs a) {
   int i, j, k;
   if (!rim_point(g, t, i, j, k)) return;
   const int qi = oslot(g.ibnd, g.nx, i), qj = oslot(g.jbnd, g.ny, j);
+  const idx_t p = lin(g, i, j, k);
+  if (a.geo == kGeoC3) {
+    // the fluxes of tau projected on the viscous metric row of each direction
+    // at every stencil point, their 5-pt differences times ijacob3_v
+    // (smod_flux_visc_5pts_c3.f90)
+    const idx_t st[3] = {1, (idx_t)g.sx, g.sy};
+    const int qd[3] = {qi, qj, -1}, rd[3] = {i, j, k}, nd[3] = {g.nx, g.ny, g.nz};
+    double kv[4] = {0.0, 0.0, 0.0, 0.0};
+    const double ij = ldg(a.met.mv[9], p);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      // the five stencil points: rows r0..r0+4 (one-sided) or r-2..r+2 (centred)
+      const int r0 = qd[d] < 0 ? rd[d] - 2 : (qd[d] < 2 ? 1 : nd[d] - 4);
+      double f[5][4];
+#pragma unroll
+      for (int m = 0; m < 5; m++) {
+        const idx_t pm = p + (idx_t)(r0 + m - rd[d]) * st[d];
+        double t[NTAU];
+#pragma unroll
+        for (int c = 0; c < NTAU; c++) t[c] = ldg(a.tau[c], pm);
+        const double mx = ldg(a.met.mv[3 * d], pm), my = ldg(a.met.mv[3 * d + 1], pm),
+                     mz = ldg(a.met.mv[3 * d + 2], pm);
+        f[m][0] = t[0] * mx + t[1] * my + t[2] * mz;
+        f[m][1] = t[1] * mx + t[3] * my + t[4] * mz;
+        f[m][2] = t[2] * mx + t[4] * my + t[5] * mz;
+        f[m][3] = t[6] * mx + t[7] * my + t[8] * mz;
+      }
+      double c5[5];
+      if (qd[d] >= 0) onesided(qd[d], c5);
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        double dv;
+        if (qd[d] < 0) {
+          dv = A5_1 * (f[3][n] - f[1][n]) + A5_2 * (f[4][n] - f[0][n]);
+        } else {
+          dv = 0.0;
+#pragma unroll
+          for (int m = 0; m < 5; m++) dv = dv + c5[m] * f[m][n];
+        }
+        kv[n] = dv * ij + kv[n];
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < 4; n++) stg(a.kv[n], p, kv[n]);
+    return;
+  }
   const double mx = ldg(a.met.idx_v, (idx_t)(i - 1 + G)), my = ldg(a.met.idy_v, (idx_t)(j - 1 + G)),
                mz = ldg(a.met.idz_v, (idx_t)(k - 1 + G));
-  const idx_t p = lin(g, i, j, k);
 #pragma unroll
   for (int n = 0; n < 4; n++) {
     const double *fx = a.tau[krow(0, n)], *fy = a.tau[krow(1, n)], *fz = a.tau[krow(2, n)];

@@ -183,6 +183,21 @@ std::string check_charac(const mgpu_case &c) {
   return std::string();
 }
 
+// Walls of 3-D curvilinear blocks: on their j-faces only (the axis
+// relabeling that turns i- or k-walls into j-walls covers Cartesian metrics
+// alone), checked on the caller's block table.  "" or an error.
+std::string check_c3_walls(const mgpu_case &c) {
+  char buf[256];
+  if (c.curvilinear != MGPU_GEOM_C3) return std::string();
+  for (int b = 0; b < c.nblocks; b++)
+    for (int f : {0, 1, 4, 5})
+      if (c.blocks[b].face_kind[f] == MGPU_FACE_WALL) {
+        snprintf(buf, sizeof buf, "block %d face %d: 3-D curvilinear blocks take walls on their j-faces only", b, f);
+        return buf;
+      }
+  return std::string();
+}
+
 Metrics metrics_of(const BlockData &bd) {
   Metrics m{};
   m.idx = bd.idx[0];
@@ -387,8 +402,8 @@ std::string check_topology(const mgpu_case &c) {
           snprintf(buf, sizeof buf, "block %d face %d: walls are supported on the j-faces (2, 3) only", b, f);
           return buf;
         }
-        if (c.curvilinear) {
-          snprintf(buf, sizeof buf, "block %d face %d: walls need a Cartesian case", b, f);
+        if (c.curvilinear == MGPU_GEOM_C2) {
+          snprintf(buf, sizeof buf, "block %d face %d: walls need a Cartesian or a 3-D curvilinear case", b, f);
           return buf;
         }
         if (A.ny < 10) {
@@ -772,7 +787,7 @@ void free_block(BlockData &bd) {
   if (bd.sens) (void)hipFree(bd.sens);
   if (bd.wstash) (void)hipFree(bd.wstash);
   bd.prs = bd.sens = bd.wstash = nullptr;
-  for (double **q : {&bd.kimax, &bd.kjmax, &bd.cst, &bd.uin, &bd.ppre, &bd.kf}) {
+  for (double **q : {&bd.kimax, &bd.kjmax, &bd.cst, &bd.uin, &bd.ppre, &bd.kf, &bd.wc3[0], &bd.wc3[1]}) {
     if (*q) (void)hipFree(*q);
     *q = nullptr;
   }
@@ -1038,6 +1053,8 @@ int enqueue_walls(mgpu_handle *h, int src, int dst) {
     wa.cv = 1.0 / (bd.p.gam - 1.0) * bd.p.rg;
     wa.t_wall = bd.p.t_wall;
     wa.adiabatic = bd.p.is_adiab;
+    wa.wc3[0] = bd.wc3[0];
+    wa.wc3[1] = bd.wc3[1];
     HIPCK(h, launch_wall(wa, h->stream));
   }
   return 0;
@@ -1056,6 +1073,9 @@ WallViscArgs wall_visc_args(const mgpu_handle *h, const BlockData &bd, int s0) {
   a.sens = bd.sens;
   a.want_sensor = bd.prs != nullptr;
   a.ducros = h->c.is_ducros;
+  a.geo = h->c.curvilinear;
+  a.wc3[0] = bd.wc3[0];
+  a.wc3[1] = bd.wc3[1];
   return a;
 }
 
@@ -1400,6 +1420,9 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
     }
   }
   if (bd.g.jwall && !rc) rc = dalloc(h, &bd.wstash, (size_t)4 * bl.nx * bl.nz);
+  if (c.curvilinear == MGPU_GEOM_C3)
+    for (int s = 0; s < 2 && !rc; s++)
+      if ((bd.g.jwall >> s) & 1) rc = dalloc(h, &bd.wc3[s], (size_t)8 * bl.nx * bl.nz);
   {
     const size_t fi = (size_t)bl.ny * bl.nz, fj = (size_t)bl.nx * bl.nz;
     if (bl.face_kind[1] == MGPU_FACE_CHARAC) {
@@ -1435,6 +1458,7 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
     h->charac = true;
     h->stage_family = 1;  // the classic stage kernel closes i-faces
   }
+  if (bd.wc3[0] || bd.wc3[1]) h->stage_family = 1;  // and c3 walls
   h->exchange_ready = false;
   h->tables_ready = false;
   return 0;
@@ -1897,6 +1921,7 @@ int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
   {
     std::string msg = check_cartesian_frames(*c);
     if (msg.empty()) msg = check_charac(*c);
+    if (msg.empty()) msg = check_c3_walls(*c);
     if (msg.empty()) msg = relabel_blocks(*c, stored, perm);
     mgpu_case cs = *c;
     cs.blocks = stored.data();
@@ -1947,6 +1972,8 @@ int mgpu_set_wall(mgpu_handle *h, int block, const double *cextrp2, const double
   if (!cextrp2 || !cextrp3) return fail(h, MGPU_EINVAL, "null argument");
   BlockData &bd = h->b[block];
   if (!bd.g.jwall) return fail(h, MGPU_ESTATE, "block %d has no wall face", block);
+  if (h->c.curvilinear == MGPU_GEOM_C3)
+    return fail(h, MGPU_ESTATE, "block %d: curvilinear walls take their coefficients from mgpu_set_wall_c3", block);
   for (int s = 0; s < 2; s++) {
     if (!((bd.g.jwall >> s) & 1)) continue;
     const int fu = kFaceMap[bd.perm][2 + s];  // the caller's face stored as j-face 2 + s
@@ -1959,6 +1986,35 @@ int mgpu_set_wall(mgpu_handle *h, int block, const double *cextrp2, const double
   HIPCK(h, hipStreamSynchronize(h->stream));
   drop_graphs(h);  // the captured steps hold the coefficients
   bd.have_wall = true;
+  return 0;
+}
+
+int mgpu_set_wall_c3(mgpu_handle *h, int block, int face, const double *const *coef) {
+  if (!h) return MGPU_EINVAL;
+  if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
+  if (!coef) return fail(h, MGPU_EINVAL, "null argument");
+  BlockData &bd = h->b[block];
+  if (h->c.curvilinear != MGPU_GEOM_C3) return fail(h, MGPU_ESTATE, "case is not 3-D curvilinear: use mgpu_set_wall");
+  if (face != 2 && face != 3) return fail(h, MGPU_EINVAL, "face %d: curvilinear walls are j-faces (2, 3)", face);
+  const int s = face - 2;
+  if (!bd.wc3[s]) return fail(h, MGPU_ESTATE, "block %d face %d is no wall", block, face);
+  const size_t nf = (size_t)bd.g.nx * bd.g.nz;
+  std::vector<double> tmp(8 * nf);
+  for (int q = 0; q < 8; q++) {
+    if (!coef[q]) return fail(h, MGPU_EINVAL, "coefficient array %d is null", q);
+    for (size_t e = 0; e < nf; e++) {
+      if (!std::isfinite(coef[q][e]))
+        return fail(h, MGPU_EINVAL, "block %d face %d: coefficient %d not finite at %zu", block, face, q, e);
+      tmp[q * nf + e] = coef[q][e];
+    }
+  }
+  HIPCK(h, hipStreamSynchronize(h->stream));
+  HIPCK(h, hipMemcpy(bd.wc3[s], tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+  // a block is ready once every wall face has its coefficients
+  bool all = true;
+  for (int q = 0; q < 2; q++) all &= !((bd.g.jwall >> q) & 1) || q == s || bd.wc3_set[q];
+  bd.wc3_set[s] = true;
+  bd.have_wall = all;
   return 0;
 }
 
@@ -2147,6 +2203,7 @@ int mgpu_exchange_plan(const mgpu_case *c, int set, int64_t *map, int64_t cap, i
   std::vector<int> perm;
   std::string msg = check_cartesian_frames(*c);
   if (msg.empty()) msg = check_charac(*c);
+  if (msg.empty()) msg = check_c3_walls(*c);
   if (msg.empty()) msg = relabel_blocks(*c, stored, perm);
   mgpu_case cs = *c;
   cs.blocks = stored.data();

@@ -30,6 +30,9 @@ struct BlockData {
   RefState ref{};         // mixed precision: interior mean of the state at put_state
   double *wstash = nullptr;  // walls: rho, rhoE of the end-of-step wall state (2 per wall point)
   double c2[2] = {0, 0}, c3[2] = {0, 0};  // walls: cextrp2, cextrp3 of the jmin, jmax faces
+  // c3 walls: per j-face, the 8 coefficient slabs of mgpu_set_wall_c3 (8*nx*nz doubles)
+  double *wc3[2] = {nullptr, nullptr};
+  bool wc3_set[2] = {false, false};
   // characteristic outflow (imax, jmax) and supersonic inlet (imin): face
   // slabs (see slab_size) of what the reference keeps between the calls of
   // its boundary routines -- the increments of the characteristic rows, the

@@ -320,6 +320,11 @@ struct WallArgs {
   double cv;            // cv = rg/(gam-1)
   double t_wall;        // isothermal wall temperature
   int adiabatic;
+  // c3 blocks: per side (jmin, jmax) the 8 coefficient slabs of the wall
+  // points (nx*nz each, i fastest): gksigeta3, gphigeta3, then the unit
+  // tangents x_ksi/|x_ksi| and x_phi/|x_phi| (x, y, z components); null on
+  // Cartesian blocks, whose extrapolation uses c2, c3
+  const double *wc3[2];
 };
 
 // The viscous rows next to the bounded faces of one block (rows 1,2 and
@@ -337,6 +342,8 @@ struct WallViscArgs {
   int adiabatic;       // no wall-normal temperature gradient on the wall row
   int want_sensor;     // write the sensor on those rows and p beyond the bounded faces
   int ducros;          // Ducros sensor (else 1)
+  int geo;             // kGeoCart or kGeoC3 (viscous metrics mapping the index-space derivatives)
+  const double *wc3[2];  // c3 walls: the coefficient slabs of WallArgs::wc3 (the unit tangents)
 };
 
 // Characteristic (LODI) boundaries and the supersonic inlet of one block:

@@ -82,6 +82,14 @@ module mod_musica_gpu
        real(c_double), intent(in) :: cextrp2(6), cextrp3(6)
      end function mgpu_set_wall
 
+     ! coef: c_loc of the 8 coefficient arrays (nx*nz each) of a c3 j-wall
+     integer(c_int) function mgpu_set_wall_c3(h, block, face, coef) bind(C, name='mgpu_set_wall_c3')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int), value :: block, face
+       type(c_ptr), intent(in) :: coef(8)
+     end function mgpu_set_wall_c3
+
      integer(c_int) function mgpu_put_state(h, block, rho, rhou, rhov, rhow, rhoe) bind(C, name='mgpu_put_state')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: h

@@ -89,7 +89,13 @@ class GpuSolver:
         self._topology = topology
         for b, bc in enumerate(self.block_cases):
             self._set_metrics(b, bc)
-            if self._has_wall(b):
+            if self._has_wall(b) and int(case.curvilinear) == _lib.GEOM_C3:
+                # curvilinear walls: the per-point coefficients of each wall face
+                for f, slabs in bc.wall_c3.items():
+                    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in slabs]
+                    tab = (_lib._dp * 8)(*[dptr(a) for a in arrs])
+                    check(L.mgpu_set_wall_c3(self.h, b, int(f), tab), self.h, "mgpu_set_wall_c3")
+            elif self._has_wall(b):
                 c2 = np.ascontiguousarray(getattr(bc, "cextrp2", (0.0,) * 6), dtype=np.float64)
                 c3 = np.ascontiguousarray(getattr(bc, "cextrp3", (0.0,) * 6), dtype=np.float64)
                 check(L.mgpu_set_wall(self.h, b, dptr(c2), dptr(c3)), self.h, "mgpu_set_wall")

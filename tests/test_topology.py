@@ -97,14 +97,14 @@ def _create_case(c, blocks, precision=_lib.PREC_FP64):
 
 
 def test_wall_limits_are_checked():
-    """Walls: the faces of one axis per block, Cartesian, >= 10 cells between
-    them; refused before any device work."""
+    """Walls: the faces of one axis per block, Cartesian or 3-D curvilinear
+    (not 2.5-D), >= 10 cells between them; refused before any device work."""
     c, blocks = _channel_table()
     blocks[0]["kind"][0] = blocks[0]["kind"][1] = _lib.FACE_WALL
     rc, msg = _create_case(c, blocks)
     assert rc == _lib.EINVAL and "more than one axis" in msg
     c, blocks = _channel_table()
-    c.curvilinear = True
+    c.curvilinear = 2
     rc, msg = _create_case(c, blocks)
     assert rc == _lib.EINVAL and "Cartesian" in msg
     c, blocks = _channel_table()
@@ -217,3 +217,21 @@ def test_characteristic_faces_are_checked():
     c, b = _shock_bl_table()
     b[0]["nx"] = 9
     refused(c, b, ">= 10")
+
+
+def test_curvilinear_walls_are_checked():
+    """Walls on 3-D curvilinear blocks sit on their j-faces (no axis
+    relabeling of c3 metrics); 2.5-D blocks take no walls; refused before any
+    device work."""
+    c = cases.vortex_channel(24, 20, 8)
+    blk = {"nx": c.nx, "ny": c.ny, "nz": c.nz,
+           "kind": [_lib.FACE_WALL, _lib.FACE_WALL, _lib.FACE_PERIODIC, _lib.FACE_PERIODIC, _lib.FACE_PERIODIC,
+                    _lib.FACE_PERIODIC],
+           "nbr_block": [0] * 6, "nbr_face": [1, 0, 3, 2, 5, 4], "orient": [0] * 6}
+    rc, msg = _create_case(c, [blk])
+    assert rc == _lib.EINVAL and "j-faces only" in msg, msg
+    c.curvilinear = 2
+    blk["kind"] = [_lib.FACE_PERIODIC, _lib.FACE_PERIODIC, _lib.FACE_WALL, _lib.FACE_WALL, _lib.FACE_PERIODIC,
+                   _lib.FACE_PERIODIC]
+    rc, msg = _create_case(c, [blk])
+    assert rc == _lib.EINVAL and "3-D curvilinear" in msg, msg
