@@ -24,13 +24,17 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
-# HBM bytes per k_stage launch on the 512x256x128 curvilinear block (tuned
+# HBM bytes per k_stage launch on the 512x256x128 curvilinear blocks (tuned
 # 64x8 tile, k-chunk 128), averaged over the 4 stages (3 Euler launches + the
 # viscous last stage): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch,
 # FETCH_SIZE scaled by 2.000 and WRITE_SIZE by 1.000 (their readings for a
-# 1 GiB device copy).  Summary: profiles/step_kernels_c3_512x256x128_fp64.md.
-STAGE_TRAFFIC_BYTES = (3 * 4.589e9 + 5.338e9) / 4
-STAGE_TRAFFIC_WORKLOAD = ("periodic", 512, 256, 128)
+# 1 GiB device copy).  Summaries: profiles/step_kernels_c3_channel_512x256x128_fp64.md
+# (the walled channel), profiles/step_kernels_c3_512x256x128_fp64.md (the
+# periodic block).
+STAGE_TRAFFIC = {
+    ("channel", 512, 256, 128): ((3 * 4.597e9 + 5.510e9) / 4, "profiles/step_kernels_c3_channel_512x256x128_fp64.md"),
+    ("periodic", 512, 256, 128): ((3 * 4.589e9 + 5.338e9) / 4, "profiles/step_kernels_c3_512x256x128_fp64.md"),
+}
 
 
 def stage_words_per_cell(case) -> float:
@@ -256,6 +260,25 @@ def c2_leg(nx: int, ny: int, nz: int, steps: int, warmup: int, tune: bool, c3_va
     return res
 
 
+def periodic_leg(nx: int, ny: int, nz: int, steps: int, warmup: int, tune: bool, filt: int):
+    """The same vortex on a fully periodic wavy block (no walls), FP64: the
+    curvilinear workload the headline used before the walled channel ran."""
+    from musica2_newgrid_amd import _lib, cases
+    case = cases.vortex_wavy(nx, ny, nz, filter=filt)
+    r = run_leg(case, _lib.PREC_FP64, steps, warmup, tune)
+    nst, ms_stage = r["times"]["stage"]
+    avg = ms_stage / nst / 1e3
+    byts = stage_words_per_cell(case) * 8.0 * case.ncells
+    traffic = STAGE_TRAFFIC.get(("periodic", nx, ny, nz))
+    return {"value": round(case.ncells * case.nrk * steps / r["t"] / 1e6, 2), "unit": "Mcell-RK-stages/s",
+            "ms_per_step": round(r["t"] / steps * 1e3, 4),
+            "workload": f"curvilinear block {nx}x{ny}x{nz} with 3-D GCL metrics of a fully periodic wavy map, "
+                        f"isentropic vortex M=0.5, viscous, shock filter + Ducros, RK4, FP64, graph replay",
+            "stage_roofline": {"achieved_GBs": round(byts / avg / 1e9, 1),
+                               "frac": round(byts / avg / 1e9 / HBM_PEAK_GBS, 4), "avg_launch_ms": round(avg * 1e3, 4),
+                               "algorithmic_bytes_per_launch": byts, "traffic": traffic[0] if traffic else None}}
+
+
 def shock_bl_leg(nx: int, ny: int, nz: int, steps: int, warmup: int, tune: bool, error_steps: int, mixed: bool):
     """The shock/boundary-layer workload (Cartesian, Mach 2 oblique shock of a
     10 degree wedge entering at jmax onto a Pohlhausen boundary layer,
@@ -350,7 +373,8 @@ def main() -> int:
     ap.add_argument("--no-channel", action="store_true", help="skip the walled-channel leg")
     ap.add_argument("--no-c2", action="store_true", help="skip the 2.5-D curvilinear leg")
     ap.add_argument("--no-shock-bl", action="store_true", help="skip the shock/boundary-layer leg")
-    ap.add_argument("--headline", choices=["channel", "periodic"], default="periodic",
+    ap.add_argument("--no-periodic", action="store_true", help="skip the fully periodic curvilinear leg")
+    ap.add_argument("--headline", choices=["channel", "periodic"], default="channel",
                     help="the curvilinear headline block: the walled channel as stated, or the fully periodic "
                          "wavy block")
     ap.add_argument("--shock-bl-size", type=int, nargs=3, default=(1024, 512, 8), metavar=("NX", "NY", "NZ"))
@@ -401,6 +425,7 @@ def main() -> int:
     avg_stage_s = ms_stage / nst / 1e3
     bytes_launch = stage_words_per_cell(case) * 8.0 * case.ncells
     achieved = bytes_launch / avg_stage_s / 1e9
+    traffic = STAGE_TRAFFIC.get((args.headline, args.nx, args.ny, args.nz))
     out = {
         "metric": "Mcell-RK-stages/s",
         "value": round(value, 2),
@@ -422,10 +447,8 @@ def main() -> int:
                                f"launched one by one ({fp['t_eager'] / args.steps * 1e3:.4f} ms per step)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": (STAGE_TRAFFIC_BYTES if (args.headline, args.nx, args.ny, args.nz) ==
-                                 STAGE_TRAFFIC_WORKLOAD else None),
-                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, "
-                                       "profiles/step_kernels_c3_512x256x128_fp64.md",
+                     "traffic": traffic[0] if traffic else None,
+                     "traffic_source": ("rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, " + traffic[1]) if traffic else None,
                      "kernel": "k_stage", "avg_launch_ms": round(avg_stage_s * 1e3, 4),
                      "algorithmic_bytes_per_launch": bytes_launch},
         "kernel_ms": {k: {"launches": n, "ms": round(ms, 3)} for k, (n, ms) in fp["times"].items()},
@@ -446,6 +469,9 @@ def main() -> int:
             "after_steps": mx["steps_done"],
             "kernel_ms": {k: {"launches": n, "ms": round(ms, 3)} for k, (n, ms) in mx["times"].items()},
         }
+    if args.headline == "channel" and not args.no_periodic:
+        print("bench: periodic curvilinear leg", file=sys.stderr, flush=True)
+        out["periodic_c3"] = periodic_leg(args.nx, args.ny, args.nz, args.steps, args.warmup, not args.no_tune, diss[0])
     if not args.no_c2:
         print("bench: 2.5-D curvilinear leg", file=sys.stderr, flush=True)
         out["c2"] = c2_leg(args.nx, args.ny, args.nz, args.steps, args.warmup, not args.no_tune, value,

@@ -85,6 +85,7 @@ struct Ptrs {
   double ref[5];
   const double *kbc[4];
   const double *uin_row, *prim_row;
+  const float *m32[9];
 };
 
 template <int GEO, bool VISC, bool IN32, bool OUT32>
@@ -100,6 +101,10 @@ __device__ __forceinline__ void load_ptrs(const StageArgs &a, Ptrs &P) {
   if constexpr (GEO == kGeoC3) {
 #pragma unroll
     for (int q = 0; q < 10; q++) P.m[q] = a.met.m[q];
+    if constexpr (IN32 || OUT32) {
+#pragma unroll
+      for (int q = 0; q < 9; q++) P.m32[q] = a.m32[q];
+    }
   } else if constexpr (GEO == kGeoC2) {
 #pragma unroll
     for (int q = 0; q < 5; q++) P.m2[q] = a.met.m2[q];
@@ -135,21 +140,27 @@ __device__ __forceinline__ void load_u(const Ptrs &P, idx_t p, double u[5]) {
   }
 }
 
-template <bool CURV, bool IN32>
+// The flux-direction metric terms are read as FP32 in the mixed path (its
+// flux arithmetic rounds them to FP32 anyway: the same values, half the
+// bytes), as FP64 otherwise.
+template <bool CURV, bool IN32, bool MIX>
 __device__ __forceinline__ void load_raw(const Ptrs &P, idx_t p, int dir, Raw &r) {
   load_u<IN32>(P, p, r.u);
   if constexpr (CURV) {
 #pragma unroll
-    for (int c = 0; c < 3; c++) r.m[c] = ldg(P.m[3 * dir + c], p);
+    for (int c = 0; c < 3; c++) {
+      if constexpr (MIX) r.m[c] = (double)ldg(P.m32[3 * dir + c], p);
+      else r.m[c] = ldg(P.m[3 * dir + c], p);
+    }
   }
 }
 
 // Raw inputs of a halo cell, whose direction is known at run time only:
 // branch to a constant direction so that the pointer table stays in registers.
-template <bool CURV, bool IN32>
+template <bool CURV, bool IN32, bool MIX>
 __device__ __forceinline__ void load_raw_halo(const Ptrs &P, idx_t p, int dir, Raw &r) {
-  if (dir == 0) load_raw<CURV, IN32>(P, p, 0, r);
-  else load_raw<CURV, IN32>(P, p, 1, r);
+  if (dir == 0) load_raw<CURV, IN32, MIX>(P, p, 0, r);
+  else load_raw<CURV, IN32, MIX>(P, p, 1, r);
 }
 
 __device__ __forceinline__ Prim prim_raw(const double u[5], double gam1) {
@@ -387,10 +398,10 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
 #pragma unroll
     for (int l = 0; l < 10; l++) {
       Raw r;
-      load_raw<C3, IN32>(P, own_off + (idx_t)(kbeg - 5 + l) * sz, 2, r);
+      load_raw<C3, IN32, MIX>(P, own_off + (idx_t)(kbeg - 5 + l) * sz, 2, r);
       flux_r<C3, MIX>(prim_raw(r.u, gam1), 2, r.m, rf, q[l]);
     }
-    load_raw<C3, IN32>(P, own_off + (idx_t)(kbeg + 5) * sz, 2, rq);
+    load_raw<C3, IN32, MIX>(P, own_off + (idx_t)(kbeg + 5) * sz, 2, rq);
   }
   // the own point's state and (c3) i/j metric terms, likewise one plane ahead
   double ru[5] = {0, 0, 0, 0, 0};
@@ -399,8 +410,8 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
     if constexpr (C3) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        rxm[c] = ldg(P.m[c], p);
-        rym[c] = ldg(P.m[3 + c], p);
+        rxm[c] = MIX ? (double)ldg(P.m32[c], p) : ldg(P.m[c], p);
+        rym[c] = MIX ? (double)ldg(P.m32[3 + c], p) : ldg(P.m[3 + c], p);
       }
     }
   };
@@ -414,7 +425,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
     Raw rh0;
     double u0[5] = {0, 0, 0, 0, 0};
     if (hs_on[0]) {
-      load_raw_halo<C3, IN32>(P, hs_off[0] + (idx_t)k * sz, hs_dir[0], rh0);
+      load_raw_halo<C3, IN32, MIX>(P, hs_off[0] + (idx_t)k * sz, hs_dir[0], rh0);
       if constexpr (C2) {
         rh0.m[0] = hm[0][0];
         rh0.m[1] = hm[0][1];
@@ -481,7 +492,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
 #pragma unroll
       for (int s = 1; s < NS; s++)
         if (hs_on[s]) {
-          load_raw_halo<C3, IN32>(P, hs_off[s] + (idx_t)k * sz, hs_dir[s], rh[s]);
+          load_raw_halo<C3, IN32, MIX>(P, hs_off[s] + (idx_t)k * sz, hs_dir[s], rh[s]);
           if constexpr (C2) {
             rh[s].m[0] = hm[s][0];
             rh[s].m[1] = hm[s][1];
@@ -505,7 +516,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
     }
     __syncthreads();
     // queue point and own point of the next plane
-    if (own && k < kend) load_raw<C3, IN32>(P, pk + 6 * sz, 2, rq);
+    if (own && k < kend) load_raw<C3, IN32, MIX>(P, pk + 6 * sz, 2, rq);
     if ((needx || needy) && k < kend) load_own(pk + sz);
 
     if (own) {
@@ -900,10 +911,10 @@ __global__ __launch_bounds__(TX *TY, 1) void k_stage_p(const Batch<StageArgs> *_
 #pragma unroll
     for (int l = 0; l < 10; l++) {
       Raw r;
-      load_raw<C3, IN32>(P, own_off + (idx_t)(kbeg - 5 + l) * sz, 2, r);
+      load_raw<C3, IN32, MIX>(P, own_off + (idx_t)(kbeg - 5 + l) * sz, 2, r);
       flux_r<C3, MIX>(prim_raw(r.u, gam1), 2, r.m, rf, q[l]);
     }
-    load_raw<C3, IN32>(P, own_off + (idx_t)(kbeg + 5) * sz, 2, rq);
+    load_raw<C3, IN32, MIX>(P, own_off + (idx_t)(kbeg + 5) * sz, 2, rq);
   }
   double ru[5] = {0, 0, 0, 0, 0};
   auto load_own = [&](idx_t p) {
@@ -911,8 +922,8 @@ __global__ __launch_bounds__(TX *TY, 1) void k_stage_p(const Batch<StageArgs> *_
     if constexpr (C3) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        rxm[c] = ldg(P.m[c], p);
-        rym[c] = ldg(P.m[3 + c], p);
+        rxm[c] = MIX ? (double)ldg(P.m32[c], p) : ldg(P.m[c], p);
+        rym[c] = MIX ? (double)ldg(P.m32[3 + c], p) : ldg(P.m[3 + c], p);
       }
     }
   };
@@ -1041,7 +1052,7 @@ __global__ __launch_bounds__(TX *TY, 1) void k_stage_p(const Batch<StageArgs> *_
     barrier_lds();
     // own point and queue point of plane k+1, in flight during the derivatives
     if (own && k < kend) {
-      load_raw<C3, IN32>(P, pk + 6 * sz, 2, rq);
+      load_raw<C3, IN32, MIX>(P, pk + 6 * sz, 2, rq);
       load_u0(pk + sz, k + 1);
     }
     if ((needx || needy) && k < kend) load_own(pk + sz);

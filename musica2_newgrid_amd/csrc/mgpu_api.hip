@@ -778,6 +778,7 @@ void free_block(BlockData &bd) {
     for (auto &p : s) { if (p) (void)hipFree(p); p = nullptr; }
   for (auto &p : bd.idx) { if (p) (void)hipFree(p); p = nullptr; }
   for (auto &p : bd.m) { if (p) (void)hipFree(p); p = nullptr; }
+  for (auto &p : bd.m32) { if (p) (void)hipFree(p); p = nullptr; }
   for (auto &p : bd.mv) { if (p) (void)hipFree(p); p = nullptr; }
   for (auto &p : bd.m2) { if (p) (void)hipFree(p); p = nullptr; }
   for (auto &p : bd.m2v) { if (p) (void)hipFree(p); p = nullptr; }
@@ -847,6 +848,7 @@ StageArgs stage_args(const mgpu_handle *h, const BlockData &bd, int in, int s0, 
   sa.kbc[3] = bd.kjmax;
   sa.uin_row = bd.uin;
   sa.prim_row = irk == 1 ? bd.ppre : nullptr;
+  for (int q = 0; q < 9; q++) sa.m32[q] = bd.m32[q];
   if (h->mixed) {
     const int mode = stage_mode(h, irk);
     for (int v = 0; v < 5; v++) {
@@ -1371,6 +1373,7 @@ size_t block_bytes(const mgpu_case &c, int nx, int ny, int nz) {
     arrays += NTAU + 4 + (has_sensor(c) ? 2 : 0) + (has_sr(c) ? 3 : 0);
   size_t bytes = arrays * n * sizeof(double);
   if (c.precision == MGPU_PREC_MIXED) bytes += 10 * n * sizeof(float);
+  if (c.precision == MGPU_PREC_MIXED && c.curvilinear == MGPU_GEOM_C3) bytes += 9 * n * sizeof(float);
   if (c.curvilinear != MGPU_GEOM_C3) bytes += 6 * (size_t)(nx + ny + nz + 6 * G) * sizeof(double);
   if (c.curvilinear == MGPU_GEOM_C2) bytes += 10 * (size_t)(nx + 2 * G) * (ny + 2 * G) * sizeof(double);
   return bytes;  // (+ 4*nx*nz doubles of wall state for a block with walls)
@@ -1407,6 +1410,8 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
       for (int v = 0; v < 5 && !rc; v++) rc = dalloc32(h, &bd.sf[s][v], n);
   if (c.curvilinear == MGPU_GEOM_C3) {
     for (int q = 0; q < 10 && !rc; q++) rc = dalloc(h, &bd.m[q], n);
+    if (h->mixed)
+      for (int q = 0; q < 9 && !rc; q++) rc = dalloc32(h, &bd.m32[q], n);
     if (c.iorder_visc)
       for (int q = 0; q < 10 && !rc; q++) rc = dalloc(h, &bd.mv[q], n);
   } else {
@@ -1508,6 +1513,14 @@ int put_metrics_c3(mgpu_handle *h, int block, const double *const *m, const doub
   for (int q = 0; q < 10; q++) {
     if (!m[q]) return fail(h, MGPU_EINVAL, "metric %d is null", q);
     HIPCK(h, hipMemcpy(bd.m[q], m[q], bytes, hipMemcpyHostToDevice));
+  }
+  if (bd.m32[0]) {
+    // the mixed path's FP32 copies of the flux metric terms
+    std::vector<float> f((size_t)bd.g.n);
+    for (int q = 0; q < 9; q++) {
+      for (size_t e = 0; e < f.size(); e++) f[e] = (float)m[q][e];
+      HIPCK(h, hipMemcpy(bd.m32[q], f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
   }
   if (h->c.iorder_visc) {
     if (!mv) return fail(h, MGPU_EINVAL, "viscous metric table is null");

@@ -20,6 +20,7 @@ struct BlockData {
   float *sf[2][5] = {};   // mixed precision: intermediate stage states, FP32 differences with ref
   double *idx[6] = {};    // Cartesian metrics idx,idy,idz,idx_v,idy_v,idz_v
   double *m[10] = {};
+  float *m32[9] = {};     // mixed precision, c3: the flux metric terms of m as FP32
   double *mv[10] = {};
   double *m2[5] = {};     // 2.5-D: 2-D metric rows (y_eta, -x_eta, -y_ksi, x_ksi, ijacob), Euler set
   double *m2v[5] = {};    // 2.5-D: the viscous set

@@ -237,6 +237,9 @@ struct StageArgs {
   // variables from uin_row (face slab) and its primitives from prim_row
   // (face slab; first stage) or from the stage state (later stages)
   const double *uin_row, *prim_row;
+  // mixed precision, 3-D curvilinear: the 9 flux metric terms as FP32
+  // (ksi_x..phi_z rounded; the FP32 flux arithmetic rounds them so anyway)
+  const float *m32[9];
 };
 
 // Stage kernel variants: FP64, or mixed precision with the stage state read
