@@ -520,6 +520,64 @@ def channel(nx: int = 64, ny: int = 48, nz: int = 16, lengths=(4.0, 2.0, 1.0), b
     return c
 
 
+def duct(nx: int = 48, ny: int = 40, nz: int = 16, lengths=(2.0, 1.6, 2.0), beta: float = 1.2,
+         mach: float = 0.3, re: float = 1000.0, cfl: float = 0.5, chi: float = 0.2, filter: int = 1,
+         iorder_visc: int = 4, adiabatic: bool = True, t_wall: float = 300.0, pulse: float = 1e-2,
+         seed: int = 7, T_ref: float = 300.0, rho_ref: float = 1.177) -> Case:
+    """Square-ish duct: no-slip walls on the i-faces (x = 0, Lx) and on the
+    j-faces (y = 0, Ly), both axes clustered at their walls, periodic along z.
+    Flow: a streamwise (z) profile of Mach mach vanishing on all four walls,
+    cross-flow perturbations vanishing there too, and a Gaussian pressure pulse
+    off the centre; temperature T_ref (the isothermal wall temperature t_wall)."""
+    Lx, Ly, Lz = lengths
+    dz = Lz / nz
+    xe = stretched_wall_coords(nx, Lx, beta)
+    ye = stretched_wall_coords(ny, Ly, beta)
+    x, y = xe[G:G + nx], ye[G:G + ny]
+    z = np.arange(nz) * dz
+    Z, Y, X = z.reshape(-1, 1, 1), y.reshape(1, -1, 1), x.reshape(1, 1, -1)
+    c_ref = math.sqrt(GAM * RG * T_ref)
+    w_ref = mach * c_ref
+    p_ref = rho_ref * RG * T_ref
+    ex, ey = 4.0 * (X / Lx) * (1.0 - X / Lx), 4.0 * (Y / Ly) * (1.0 - Y / Ly)
+    env = ex * ey                                      # 0 on the four walls
+    rng = np.random.default_rng(seed)
+    ph = rng.uniform(0.0, 2.0 * math.pi, 6)
+    kz = 2.0 * math.pi / Lz
+    w = w_ref * env * (1.0 + 0.05 * np.sin(kz * Z + ph[0]))
+    u = 0.05 * w_ref * env * np.sin(math.pi * Y / Ly + ph[1]) * np.cos(kz * Z + ph[2])
+    v = 0.05 * w_ref * env * np.sin(math.pi * X / Lx + ph[3]) * np.sin(kz * Z + ph[4])
+    r2 = ((X - 0.4 * Lx) ** 2 + (Y - 0.45 * Ly) ** 2 + (Z - 0.5 * Lz) ** 2) / (0.1 * Ly) ** 2
+    p = p_ref * (1.0 + pulse * np.exp(-math.log(2.0) * r2))
+    rho = p / (RG * T_ref)
+    u, v, w = (np.broadcast_to(q, p.shape) for q in (u, v, w))
+    U = []
+    for f in conserved(rho, u, v, w, p):
+        a = gr.padded(nx, ny, nz)
+        a[G:-G, G:-G, G:-G] = f
+        # ghosts beyond the walls repeat the wall rows (only the shock filter's
+        # pressure sensor of a wall row reads them)
+        a[G:-G, G:-G, :G] = f[:, :, :1]
+        a[G:-G, G:-G, G + nx:] = f[:, :, -1:]
+        a[G:-G, :G, :] = a[G:-G, G:G + 1, :]
+        a[G:-G, G + ny:, :] = a[G:-G, G + ny - 1:G + ny, :]
+        gr.fill_periodic(a, axes=(0,))
+        U.append(a)
+    idxe, idxv, cx2, cx3 = gr.wall_metrics(xe, True, True)
+    idye, idyv, cy2, cy3 = gr.wall_metrics(ye, True, True)
+    idz = np.full(nz + 2 * G, 1.0 / dz)
+    mu_ref = rho_ref * w_ref * (0.5 * Ly) / re
+    ext2 = (cx2[0], cx2[1], cy2[0], cy2[1], 0.0, 0.0)
+    ext3 = (cx3[0], cx3[1], cy3[0], cy3[1], 0.0, 0.0)
+    c = Case("duct", nx, ny, nz, curvilinear=False, iorder_visc=iorder_visc, filter=filter, ducros=False,
+             diffscale=mu_ref / float(sutherland(np.float64(T_ref))), chi=chi,
+             cart=[idxe, idye, idz, idxv, idyv, idz.copy()], U=U, faces=(2, 2, 2, 2, 0, 0),
+             adiabatic=adiabatic, t_wall=t_wall, cextrp2=ext2, cextrp3=ext3)
+    c.dt = timestep(U, min(float(np.min(np.diff(x))), float(np.min(np.diff(y))), dz), cfl)
+    c.info = dict(mach=mach, re=re, cfl=cfl, beta=beta, lengths=lengths, w_ref=w_ref, p_ref=p_ref)
+    return c
+
+
 def affine_c3_metrics(nx, ny, nz, lengths, viscous=True):
     """c3 metrics of an undistorted box: ksi_x = dy*dz etc., 1/J = 1/(dx dy dz)."""
     dx, dy, dz = (L / n for L, n in zip(lengths, (nx, ny, nz)))

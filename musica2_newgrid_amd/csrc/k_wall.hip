@@ -181,57 +181,73 @@ __device__ __forceinline__ double d5at(const double *f, idx_t p, idx_t st, int q
 
 __global__ __launch_bounds__(kWallThreads) void k_wall(WallArgs a) {
   const Geom &g = a.g;
-  const long nface = (long)g.nx * g.nz;
+  const long nface = (long)g.nx * g.nz, nfi = (long)g.ny * g.nz;
   const long t = (long)blockIdx.x * kWallThreads + threadIdx.x;
-  if (t >= 2 * nface) return;
-  const int side = (int)(t / nface);
-  if (!((g.jwall >> side) & 1)) return;
-  const long r = t - side * nface;
-  const int i = 1 + (int)(r % g.nx), k = 1 + (int)(r / g.nx);
-  const int j1 = side ? g.ny : 1, dj = side ? -1 : 1;
+  if (t >= 2 * nface + 2 * nfi) return;
+  // section j (t < 2 nface): the point (i, k) of the jmin / jmax wall row;
+  // section i: the point (j, k) of the imin / imax wall row, except the rows
+  // of j-walls (their j-face state wins on the edge)
+  const bool jsec = t < 2 * nface;
+  const long ti = t - 2 * nface;
+  const int side = jsec ? (int)(t / nface) : (int)(ti / nfi);
+  if (!(((jsec ? g.jwall : g.iwall) >> side) & 1)) return;
+  const long r = jsec ? t - side * nface : ti - side * nfi;
+  int i, j, k;
+  idx_t st;  // from the wall into the block
+  if (jsec) {
+    i = 1 + (int)(r % g.nx);
+    k = 1 + (int)(r / g.nx);
+    j = side ? g.ny : 1;
+    st = (idx_t)g.sx;
+  } else {
+    j = 1 + (int)(r % g.ny);
+    k = 1 + (int)(r / g.ny);
+    if (((g.jwall & 1) && j == 1) || ((g.jwall & 2) && j == g.ny)) return;
+    i = side ? g.nx : 1;
+    st = 1;
+  }
+  const idx_t p1 = lin(g, i, j, k);
+  const idx_t p2 = side ? p1 - st : p1 + st, p3 = side ? p1 - 2 * st : p1 + 2 * st;
+  const idx_t sidx = (idx_t)(2 * t);  // stash slot: the sections follow each other
   double rho, rhoe;
   if (a.src.v[0] || a.src32[0]) {
     auto prim = [&](idx_t q) {
       return a.src.v[0] ? prim_at(a.src, q, a.gas.gam1, a.gas.rg) : prim_at32(a.src32, a.ref, q, a.gas.gam1, a.gas.rg);
     };
-    const idx_t p2 = lin(g, i, j1 + dj, k), p3 = lin(g, i, j1 + 2 * dj, k);
     const Pw q2 = prim(p2), q3 = prim(p3);
     double tw, pw;
-    if (a.wc3[side]) {
+    if (jsec && a.wc3[side]) {
       // c3 wall (bc_wall_jmin/jmax_*_dpdn_c3, smod_bc_wall_c3.f90:842-1671):
       // rows 2, 3 along eta plus the tangential terms with the wall row's own
       // neighbours along ksi and phi (gksigeta3, gphigeta3)
-      const idx_t p1 = lin(g, i, j1, k);
       const Pw xm = prim(p1 - 1), xp = prim(p1 + 1), zm = prim(p1 - g.sy), zp = prim(p1 + g.sy);
       const double gk = ldg(a.wc3[side], (idx_t)r), gp = ldg(a.wc3[side], (idx_t)(nface + r));
       tw = a.adiabatic ? (1.0 / 3.0) * (4.0 * q2.T - q3.T + (xp.T - xm.T) * gk + (zp.T - zm.T) * gp) : a.t_wall;
       pw = (1.0 / 3.0) * (4.0 * q2.p - q3.p + (xp.p - xm.p) * gk + (zp.p - zm.p) * gp);
     } else {
-      const double c2 = a.c2[side], c3 = a.c3[side];
+      const double c2 = jsec ? a.c2[side] : a.ci2[side], c3 = jsec ? a.c3[side] : a.ci3[side];
       tw = a.adiabatic ? c2 * q2.T - c3 * q3.T : a.t_wall;
       pw = c2 * q2.p - c3 * q3.p;
     }
     rho = pw / (a.gas.rg * tw);
     rhoe = rho * (a.cv * tw);
   } else {
-    rho = ldg(a.stash, (idx_t)(2 * t));
-    rhoe = ldg(a.stash, (idx_t)(2 * t + 1));
+    rho = ldg(a.stash, sidx);
+    rhoe = ldg(a.stash, sidx + 1);
   }
   if (a.dst.v[0]) {
-    const idx_t p1 = lin(g, i, j1, k);
     stg(a.dst.v[0], p1, rho);
     stg(a.dst.v[1], p1, 0.0);
     stg(a.dst.v[2], p1, 0.0);
     stg(a.dst.v[3], p1, 0.0);
     stg(a.dst.v[4], p1, rhoe);
   } else if (a.dst32[0]) {
-    const idx_t p1 = lin(g, i, j1, k);
     const double w[5] = {rho, 0.0, 0.0, 0.0, rhoe};
 #pragma unroll
     for (int v = 0; v < 5; v++) stg(a.dst32[v], p1, (float)(w[v] - a.ref[v]));
   } else {
-    stg(a.stash, (idx_t)(2 * t), rho);
-    stg(a.stash, (idx_t)(2 * t + 1), rhoe);
+    stg(a.stash, sidx, rho);
+    stg(a.stash, sidx + 1, rhoe);
   }
 }
 
@@ -273,8 +289,10 @@ __global__ __launch_bounds__(kWallThreads) void k_visc_wall(WallViscArgs a) {
   const idx_t p = lin(g, i, j, k);
   const int qi = oslot(g.ibnd, g.nx, i), qj = oslot(g.jbnd, g.ny, j);
 
-  // no-slip wall rows
-  const bool wrow = ((g.jwall & 1) && j == 1) || ((g.jwall & 2) && j == g.ny);
+  // no-slip wall rows (of a j- or an i-wall)
+  const bool jrow = ((g.jwall & 1) && j == 1) || ((g.jwall & 2) && j == g.ny);
+  const bool irow = ((g.iwall & 1) && i == 1) || ((g.iwall & 2) && i == g.nx);
+  const bool wrow = jrow || irow;
   // gradients gr[u,v,w,T][x,y,z]
   double gr[4][3];
   {
@@ -301,6 +319,7 @@ __global__ __launch_bounds__(kWallThreads) void k_visc_wall(WallViscArgs a) {
         // adiabatic c3 wall: the index-space derivatives along ksi and phi
         // times the unit tangents, as written (smod_grad_T_c3.f90:249-275)
         const double *wc = a.wc3[j == 1 ? 0 : 1];
+        // (c3 blocks have walls on their j-faces only: wrow is a j row)
         const idx_t nf = (idx_t)g.nx * (idx_t)g.nz, q = (idx_t)(i - 1) + (idx_t)g.nx * (idx_t)(k - 1);
 #pragma unroll
         for (int c = 0; c < 3; c++)
@@ -316,8 +335,9 @@ __global__ __launch_bounds__(kWallThreads) void k_visc_wall(WallViscArgs a) {
         gr[n][2] = gz[n] * iz;
       }
       // an adiabatic wall has no wall-normal temperature gradient on its row
-      // (smod_grad_T.f90:102-109)
-      if (a.adiabatic && wrow) gr[3][1] = 0.0;
+      // (smod_grad_T.f90:102-109 and its i twin)
+      if (a.adiabatic && jrow) gr[3][1] = 0.0;
+      if (a.adiabatic && irow) gr[3][0] = 0.0;
     }
   }
   const Pw o = prim_at(a.u0, p, gam1, rg);
@@ -368,19 +388,38 @@ __global__ __launch_bounds__(kWallThreads) void k_visc_wall(WallViscArgs a) {
 
 __global__ __launch_bounds__(kWallThreads) void k_tau_wallrow(WallViscArgs a) {
   const Geom &g = a.g;
-  const int ex = g.nx + 4, ez = g.nz + 4;
-  const long nrow = (long)ex * ez;
+  const int ex = g.nx + 4, ey = g.ny + 4, ez = g.nz + 4;
+  const long nrow = (long)ex * ez, nrowi = (long)ey * ez;
   const long t = (long)blockIdx.x * kWallThreads + threadIdx.x;
-  if (t >= 2 * nrow) return;
-  const int side = (int)(t / nrow);
-  if (!((g.jwall >> side) & 1)) return;
-  const long r = t - side * nrow;
-  const int i = -1 + (int)(r % ex), k = -1 + (int)(r / ex);
-  const bool in_i = i >= 1 && i <= g.nx, in_k = k >= 1 && k <= g.nz;
-  if (!in_i && !in_k) return;  // edge ghosts: never read
-  // beyond a bounded i-face: never read (the i-derivatives are one-sided there)
-  if (((g.ibnd & 1) && i < 1) || ((g.ibnd & 2) && i > g.nx)) return;
-  const int j = side ? g.ny : 1;
+  if (t >= 2 * nrow + 2 * nrowi) return;
+  int i, j, k;
+  if (t < 2 * nrow) {
+    // j-wall rows, over i = -1..nx+2 and k = -1..nz+2
+    const int side = (int)(t / nrow);
+    if (!((g.jwall >> side) & 1)) return;
+    const long r = t - side * nrow;
+    i = -1 + (int)(r % ex);
+    k = -1 + (int)(r / ex);
+    const bool in_i = i >= 1 && i <= g.nx, in_k = k >= 1 && k <= g.nz;
+    if (!in_i && !in_k) return;  // edge ghosts: never read
+    // beyond a bounded i-face: never read (the i-derivatives are one-sided there)
+    if (((g.ibnd & 1) && i < 1) || ((g.ibnd & 2) && i > g.nx)) return;
+    j = side ? g.ny : 1;
+  } else {
+    // i-wall rows, over j = -1..ny+2 and k = -1..nz+2, less the points of
+    // j-wall rows (done above)
+    const long ti = t - 2 * nrow;
+    const int side = (int)(ti / nrowi);
+    if (!((g.iwall >> side) & 1)) return;
+    const long r = ti - side * nrowi;
+    j = -1 + (int)(r % ey);
+    k = -1 + (int)(r / ey);
+    const bool in_j = j >= 1 && j <= g.ny, in_k = k >= 1 && k <= g.nz;
+    if (!in_j && !in_k) return;
+    if (((g.jbnd & 1) && j < 1) || ((g.jbnd & 2) && j > g.ny)) return;
+    if (((g.jwall & 1) && j == 1) || ((g.jwall & 2) && j == g.ny)) return;
+    i = side ? g.nx : 1;
+  }
   const idx_t p = lin(g, i, j, k);
   const Pw o = prim_at(a.u0, p, a.gas.gam1, a.gas.rg);
   const double mu = sutherland(a.gas, o.T);
@@ -626,7 +665,7 @@ inline dim3 grid_for(long n) { return dim3((unsigned)((n + kWallThreads - 1) / k
 }  // namespace
 
 hipError_t launch_wall(const WallArgs &a, hipStream_t s) {
-  const long n = 2L * a.g.nx * a.g.nz;
+  const long n = 2L * a.g.nx * a.g.nz + 2L * a.g.ny * a.g.nz;
   hipLaunchKernelGGL(k_wall, grid_for(n), dim3(kWallThreads), 0, s, a);
   return hipGetLastError();
 }
@@ -638,7 +677,7 @@ hipError_t launch_visc_wall(const WallViscArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_tau_wallrow(const WallViscArgs &a, hipStream_t s) {
-  const long n = 2L * (a.g.nx + 4) * (a.g.nz + 4);
+  const long n = 2L * (a.g.nx + 4) * (a.g.nz + 4) + 2L * (a.g.ny + 4) * (a.g.nz + 4);
   hipLaunchKernelGGL(k_tau_wallrow, grid_for(n), dim3(kWallThreads), 0, s, a);
   return hipGetLastError();
 }

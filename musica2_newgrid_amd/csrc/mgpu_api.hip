@@ -255,16 +255,16 @@ constexpr int kFaceMap[3][6] = {{0, 1, 2, 3, 4, 5}, {2, 3, 0, 1, 4, 5}, {0, 1, 4
 constexpr int kAxisMap[3][3] = {{0, 1, 2}, {1, 0, 2}, {0, 2, 1}};
 constexpr int kVarMap[3][5] = {{0, 1, 2, 3, 4}, {0, 2, 1, 3, 4}, {0, 1, 3, 2, 4}};
 
-// 0: walls (if any) on the j-faces; 1: on the i-faces; 2: on the k-faces;
-// -1: walls on more than one axis.
+// 0: walls (if any) on the j-faces, or on the i- and j-faces; 1: on the
+// i-faces alone; 2: on the k-faces alone, or on the i- and k-faces (stored as
+// i and j); -1: walls on the j- and k-faces, or on all three axes.
 int wall_perm(const mgpu_block &b) {
-  int axes = 0, perm = 0;
-  for (int a = 0; a < 3; a++)
-    if (b.face_kind[2 * a] == MGPU_FACE_WALL || b.face_kind[2 * a + 1] == MGPU_FACE_WALL) {
-      axes++;
-      perm = a == 0 ? 1 : (a == 2 ? 2 : 0);
-    }
-  return axes > 1 ? -1 : perm;
+  bool w[3];
+  for (int a = 0; a < 3; a++) w[a] = b.face_kind[2 * a] == MGPU_FACE_WALL || b.face_kind[2 * a + 1] == MGPU_FACE_WALL;
+  if (w[1] && w[2]) return -1;
+  if (w[0] && !w[1] && !w[2]) return 1;
+  if (w[2]) return 2;
+  return 0;
 }
 
 // The stored block's tangential axes of a face (the two other axes in
@@ -302,7 +302,8 @@ std::string relabel_blocks(const mgpu_case &c, std::vector<mgpu_block> &out, std
   for (int b = 0; b < nb; b++) {
     perm[b] = wall_perm(c.blocks[b]);
     if (perm[b] < 0) {
-      snprintf(buf, sizeof buf, "block %d: walls on more than one axis are not supported", b);
+      snprintf(buf, sizeof buf,
+               "block %d: walls on the j- and k-faces together (or on all three axes) are not supported", b);
       return buf;
     }
   }
@@ -398,8 +399,24 @@ std::string check_topology(const mgpu_case &c) {
       }
       if (is_charac(kind)) continue;  // check_charac
       if (kind == MGPU_FACE_WALL) {
-        if (f != 2 && f != 3) {
-          snprintf(buf, sizeof buf, "block %d face %d: walls are supported on the j-faces (2, 3) only", b, f);
+        // the stored table: walls on the j-faces, or on the i- and j-faces
+        if (f > 3 || (f < 2 && !(A.face_kind[2] == MGPU_FACE_WALL || A.face_kind[3] == MGPU_FACE_WALL))) {
+          snprintf(buf, sizeof buf, "block %d face %d: walls are supported on the j-faces (2, 3), or on the "
+                   "i- and j-faces together", b, f);
+          return buf;
+        }
+        if (f < 2 && c.curvilinear) {
+          snprintf(buf, sizeof buf, "block %d face %d: walls on two axes need a Cartesian case", b, f);
+          return buf;
+        }
+        if (f < 2 && (c.filter == MGPU_FILTER_ARTVISC || c.filter == MGPU_FILTER_ARTVISC_SHOCK)) {
+          snprintf(buf, sizeof buf, "block %d: walls on two axes with the artificial viscosities are not supported",
+                   b);
+          return buf;
+        }
+        if (f < 2 && A.nx < 10) {
+          snprintf(buf, sizeof buf, "block %d: %d cells between its i-walls, a block with walls needs >= 10", b,
+                   A.nx);
           return buf;
         }
         if (c.curvilinear == MGPU_GEOM_C2) {
@@ -487,39 +504,45 @@ void face_affine(int f, int nf, int o, const int na[3], const int nbx[3], Aff d[
 // imposed).  A wall row facing a row that is not a wall row on the other
 // side (a wall that ends at the interface, the neighbour being periodic
 // there) would mix the two forms, so such a table is refused.  c: the blocks
-// as stored (walls on j-faces only).
+// as stored (walls on the j-faces, or on the i- and j-faces).
 std::string check_wall_rows(const mgpu_case &c) {
   char buf[320];
+  // wall bits of a block along axis ax (0: i, 1: j)
+  auto wbits = [](const mgpu_block &X, int ax) {
+    return (X.face_kind[2 * ax] == MGPU_FACE_WALL ? 1 : 0) | (X.face_kind[2 * ax + 1] == MGPU_FACE_WALL ? 2 : 0);
+  };
   for (int b = 0; b < c.nblocks; b++) {
     const mgpu_block &A = c.blocks[b];
     const int na[3] = {A.nx, A.ny, A.nz};
-    const int jwa = (A.face_kind[2] == MGPU_FACE_WALL ? 1 : 0) | (A.face_kind[3] == MGPU_FACE_WALL ? 2 : 0);
     for (int f = 0; f < 6; f++) {
       if (is_bound(A.face_kind[f])) continue;
       int nb, nf, o;
       face_nbr(A, b, f, nb, nf, o);
       const mgpu_block &B = c.blocks[nb];
       const int nbx[3] = {B.nx, B.ny, B.nz};
-      const int jwb = (B.face_kind[2] == MGPU_FACE_WALL ? 1 : 0) | (B.face_kind[3] == MGPU_FACE_WALL ? 2 : 0);
-      if (!jwa && !jwb) continue;
       int a, t1, t2;
       face_axes(f, a, t1, t2);
-      Aff d[3] = {}, sr[3] = {};
-      face_affine(f, nf, o, na, nbx, d, sr);
-      for (int p = 0; p < na[t1]; p++)
-        for (int q = 0; q < na[t2]; q++) {
-          const long dj = d[1].c0 + d[1].cp * p + d[1].cq * q;  // layer 0
-          const long sj = sr[1].c0 + sr[1].cp * p + sr[1].cq * q;
-          const bool dw = ((jwa & 1) && dj == 1) || ((jwa & 2) && dj == na[1]);
-          const bool sw = ((jwb & 1) && sj == 1) || ((jwb & 2) && sj == nbx[1]);
-          if (dw != sw) {
-            snprintf(buf, sizeof buf,
-                     "block %d face %d: a no-slip wall row of block %d meets a row of block %d that is not a wall "
-                     "row (walls must continue across an interface; not supported)",
-                     b, f, dw ? b : nb, dw ? nb : b);
-            return buf;
+      for (int ax = 0; ax < 2; ax++) {
+        if (ax == a) continue;  // the face's own axis: its ghost layers are no rows of it
+        const int wa = wbits(A, ax), wb = wbits(B, ax);
+        if (!wa && !wb) continue;
+        Aff d[3] = {}, sr[3] = {};
+        face_affine(f, nf, o, na, nbx, d, sr);
+        for (int p = 0; p < na[t1]; p++)
+          for (int q = 0; q < na[t2]; q++) {
+            const long dj = d[ax].c0 + d[ax].cp * p + d[ax].cq * q;  // layer 0
+            const long sj = sr[ax].c0 + sr[ax].cp * p + sr[ax].cq * q;
+            const bool dw = ((wa & 1) && dj == 1) || ((wa & 2) && dj == na[ax]);
+            const bool sw = ((wb & 1) && sj == 1) || ((wb & 2) && sj == nbx[ax]);
+            if (dw != sw) {
+              snprintf(buf, sizeof buf,
+                       "block %d face %d: a no-slip wall row of block %d meets a row of block %d that is not a wall "
+                       "row (walls must continue across an interface; not supported)",
+                       b, f, dw ? b : nb, dw ? nb : b);
+              return buf;
+            }
           }
-        }
+      }
     }
   }
   return std::string();
@@ -1032,7 +1055,7 @@ int build_tables(mgpu_handle *h) {
 // mixed-precision FP32 stage set -2 - set.
 int enqueue_walls(mgpu_handle *h, int src, int dst) {
   for (const auto &bd : h->b) {
-    if (!bd.g.jwall) continue;
+    if (!bd.g.jwall && !bd.g.iwall) continue;
     WallArgs wa{};
     wa.g = bd.g;
     wa.gas = gas_of(bd.p);
@@ -1051,6 +1074,8 @@ int enqueue_walls(mgpu_handle *h, int src, int dst) {
     for (int s = 0; s < 2; s++) {
       wa.c2[s] = bd.c2[s];
       wa.c3[s] = bd.c3[s];
+      wa.ci2[s] = bd.ci2[s];
+      wa.ci3[s] = bd.ci3[s];
     }
     wa.cv = 1.0 / (bd.p.gam - 1.0) * bd.p.rg;
     wa.t_wall = bd.p.t_wall;
@@ -1193,7 +1218,7 @@ int enqueue_step(mgpu_handle *h, int s0) {
       HIPCK(h, launch_iface(h->xv, h->nchunks_v, h->stream));
       if (h->wall)
         for (const auto &bd : h->b)
-          if (bd.g.jwall) HIPCK(h, launch_tau_wallrow(wall_visc_args(h, bd, s0), h->stream));
+          if (bd.g.jwall || bd.g.iwall) HIPCK(h, launch_tau_wallrow(wall_visc_args(h, bd, s0), h->stream));
       HIPCK(h, launch_vdiv(h->d_vdiv, nb, h->nt_vdiv, geo, h->stream));
       if (h->bnd)
         for (const auto &bd : h->b)
@@ -1398,6 +1423,7 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
   bd.g.sy = (idx_t)bd.g.sx * (idx_t)(bl.ny + 2 * G);
   bd.g.n = (long)bd.g.sy * (bl.nz + 2 * G);
   bd.g.jwall = (bl.face_kind[2] == MGPU_FACE_WALL ? 1 : 0) | (bl.face_kind[3] == MGPU_FACE_WALL ? 2 : 0);
+  bd.g.iwall = (bl.face_kind[0] == MGPU_FACE_WALL ? 1 : 0) | (bl.face_kind[1] == MGPU_FACE_WALL ? 2 : 0);
   bd.g.ibnd = (is_bound(bl.face_kind[0]) ? 1 : 0) | (is_bound(bl.face_kind[1]) ? 2 : 0);
   bd.g.jbnd = (is_bound(bl.face_kind[2]) ? 1 : 0) | (is_bound(bl.face_kind[3]) ? 2 : 0);
   bd.bytes = block_bytes(c, bl.nx, bl.ny, bl.nz);
@@ -1424,7 +1450,8 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
       for (int q = 0; q < 5 && !rc; q++) rc = dalloc(h, &bd.m2v[q], n2);
     }
   }
-  if (bd.g.jwall && !rc) rc = dalloc(h, &bd.wstash, (size_t)4 * bl.nx * bl.nz);
+  if ((bd.g.jwall || bd.g.iwall) && !rc)
+    rc = dalloc(h, &bd.wstash, (size_t)4 * bl.nx * bl.nz + (size_t)4 * bl.ny * bl.nz);
   if (c.curvilinear == MGPU_GEOM_C3)
     for (int s = 0; s < 2 && !rc; s++)
       if ((bd.g.jwall >> s) & 1) rc = dalloc(h, &bd.wc3[s], (size_t)8 * bl.nx * bl.nz);
@@ -1457,8 +1484,9 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
     h->blocks.pop_back();
     return rc;
   }
-  if (bd.g.jwall) h->wall = true;
+  if (bd.g.jwall || bd.g.iwall) h->wall = true;
   if (bd.g.ibnd || bd.g.jbnd) h->bnd = true;
+  if (bd.g.ibnd) h->stage_family = 1;  // the classic stage kernel closes i-faces
   if (bd.kimax || bd.kjmax || bd.uin) {
     h->charac = true;
     h->stage_family = 1;  // the classic stage kernel closes i-faces
@@ -1658,7 +1686,7 @@ int step(mgpu_handle *h, int nsteps) {
   for (size_t bI = 0; bI < h->b.size(); bI++) {
     if (!h->b[bI].have_metrics) return fail(h, MGPU_ESTATE, "block %zu: metrics not set", bI);
     if (!h->b[bI].have_state) return fail(h, MGPU_ESTATE, "block %zu: state not set", bI);
-    if (h->b[bI].g.jwall && !h->b[bI].have_wall)
+    if ((h->b[bI].g.jwall || h->b[bI].g.iwall) && !h->b[bI].have_wall)
       return fail(h, MGPU_ESTATE, "block %zu: wall coefficients not set (mgpu_set_wall)", bI);
   }
   if (h->b.empty()) return 0;
@@ -1984,7 +2012,7 @@ int mgpu_set_wall(mgpu_handle *h, int block, const double *cextrp2, const double
   if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
   if (!cextrp2 || !cextrp3) return fail(h, MGPU_EINVAL, "null argument");
   BlockData &bd = h->b[block];
-  if (!bd.g.jwall) return fail(h, MGPU_ESTATE, "block %d has no wall face", block);
+  if (!bd.g.jwall && !bd.g.iwall) return fail(h, MGPU_ESTATE, "block %d has no wall face", block);
   if (h->c.curvilinear == MGPU_GEOM_C3)
     return fail(h, MGPU_ESTATE, "block %d: curvilinear walls take their coefficients from mgpu_set_wall_c3", block);
   for (int s = 0; s < 2; s++) {
@@ -1995,6 +2023,15 @@ int mgpu_set_wall(mgpu_handle *h, int block, const double *cextrp2, const double
       return fail(h, MGPU_EINVAL, "block %d face %d: extrapolation coefficients not finite", block, fu);
     bd.c2[s] = a;
     bd.c3[s] = b;
+  }
+  for (int s = 0; s < 2; s++) {
+    if (!((bd.g.iwall >> s) & 1)) continue;
+    const int fu = kFaceMap[bd.perm][s];  // the caller's face stored as i-face s
+    const double a = cextrp2[fu], b = cextrp3[fu];
+    if (!std::isfinite(a) || !std::isfinite(b))
+      return fail(h, MGPU_EINVAL, "block %d face %d: extrapolation coefficients not finite", block, fu);
+    bd.ci2[s] = a;
+    bd.ci3[s] = b;
   }
   HIPCK(h, hipStreamSynchronize(h->stream));
   drop_graphs(h);  // the captured steps hold the coefficients

@@ -31,6 +31,7 @@ struct BlockData {
   RefState ref{};         // mixed precision: interior mean of the state at put_state
   double *wstash = nullptr;  // walls: rho, rhoE of the end-of-step wall state (2 per wall point)
   double c2[2] = {0, 0}, c3[2] = {0, 0};  // walls: cextrp2, cextrp3 of the jmin, jmax faces
+  double ci2[2] = {0, 0}, ci3[2] = {0, 0};  // and of the imin, imax faces (walls on two axes)
   // c3 walls: per j-face, the 8 coefficient slabs of mgpu_set_wall_c3 (8*nx*nz doubles)
   double *wc3[2] = {nullptr, nullptr};
   bool wc3_set[2] = {false, false};

@@ -49,6 +49,7 @@ struct Geom {
   int sx;     // nx + 2G
   idx_t sy;   // sx*(ny+2G)
   int jwall;  // no-slip walls: bit 0 on the jmin face, bit 1 on the jmax face
+  int iwall;  // no-slip walls on the i-faces (a block walled on two axes), bits as jwall
   // boundary faces of any kind (no-slip wall, characteristic outflow,
   // supersonic inlet), at which the operators take their closures: bit 0 on
   // the min face, bit 1 on the max face of axis i (ibnd) and j (jbnd)
@@ -304,10 +305,12 @@ struct FilterArgs {
   RefState ref;        // mixed precision only
 };
 
-// No-slip wall state (bc_wall with dp/dn = 0) of the j-faces of one block:
-// rho and rhoE from p and T extrapolated from rows 2,3 of `src` (or read
-// from `stash`), zero momentum, written to the wall rows of `dst` (or to
-// `stash`: two values per wall point, jmin face first).
+// No-slip wall state (bc_wall with dp/dn = 0) of the j-faces (and i-faces)
+// of one block: rho and rhoE from p and T extrapolated from rows 2,3 of `src`
+// (or read from `stash`), zero momentum, written to the wall rows of `dst` (or
+// to `stash`: two values per wall point, jmin, jmax, then imin, imax).  On an
+// edge the j-face's state wins (bc_wall applies imin..jmax in order,
+// mod_bc_apply.f90:237-261, each face from the same primitives).
 struct WallArgs {
   Geom g;
   Gas gas;
@@ -320,6 +323,7 @@ struct WallArgs {
   double ref[5];
   double *stash;
   double c2[2], c3[2];  // extrapolation coefficients cextrp2, cextrp3 (jmin, jmax)
+  double ci2[2], ci3[2];  // the same of the imin, imax faces
   double cv;            // cv = rg/(gam-1)
   double t_wall;        // isothermal wall temperature
   int adiabatic;
