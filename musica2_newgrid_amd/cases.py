@@ -94,10 +94,13 @@ class Case:
     m2v: Optional[List[np.ndarray]] = None
     # initial conserved state: rho, rhou, rhov, rhow, rhoe (padded)
     U: Optional[List[np.ndarray]] = None
-    # face kinds imin..kmax (0 periodic, 2 no-slip wall) and the wall condition
+    # face kinds imin..kmax (0 periodic, 2 wall, 3 characteristic outflow,
+    # 4 supersonic inlet, 5 wall with the slip flag 's') and the wall condition
     faces: tuple = (0, 0, 0, 0, 0, 0)
     adiabatic: bool = True
     t_wall: float = 0.0
+    # type_wall_bc = 2 (is_wall2): wall rows advanced, slip walls possible
+    wall2: bool = False
     # wall extrapolation coefficients per face (cextrp2, cextrp3)
     cextrp2: tuple = (0.0,) * 6
     cextrp3: tuple = (0.0,) * 6

@@ -231,11 +231,12 @@ def curvilinear_metrics_jwalls(mapping, eta_of, nx, ny, nz, lengths, viscous=Tru
 
     Returns (m, mv, coords, wall): m, mv the 10 padded Euler / viscous arrays
     (ksi_x..phi_z, ijacob3); coords the node coordinates on the interior;
-    wall[f] for f = 2 (jmin), 3 (jmax) the 8 per-point coefficients of the
+    wall[f] for f = 2 (jmin), 3 (jmax) the 14 per-point coefficients of the
     wall routines (grid_normals.f90:243-300): -grad(ksi).grad(eta)/|grad(eta)|^2,
-    -grad(phi).grad(eta)/|grad(eta)|^2 (Euler metrics of the wall row) and
-    the unit tangents x_ksi/|x_ksi|, x_phi/|x_phi| (11-pt derivatives of the
-    coordinates), each an array (nz, nx)."""
+    -grad(phi).grad(eta)/|grad(eta)|^2 (Euler metrics of the wall row), the
+    unit tangents x_ksi/|x_ksi|, x_phi/|x_phi| (11-pt derivatives of the
+    coordinates) and their dual basis (dual_tangents, for slip walls), each an
+    array (nz, nx)."""
     ext = 10
     Lx, Ly, Lz = lengths
     e2 = ext + G
@@ -277,10 +278,24 @@ def curvilinear_metrics_jwalls(mapping, eta_of, nx, ny, nz, lengths, viscous=Tru
         g2 = -(w(6) * w(3) + w(7) * w(4) + w(8) * w(5)) / n2
         g1[np.abs(g1) < 1e-16] = 0.0
         g2[np.abs(g2) < 1e-16] = 0.0
-        wall[f] = [np.ascontiguousarray(a) for a in (g1, g2, ck[0] / gk, ck[1] / gk, ck[2] / gk,
-                                                    cp[0] / gp, cp[1] / gp, cp[2] / gp)]
+        ek = [c / gk for c in ck]
+        ep = [c / gp for c in cp]
+        wall[f] = [np.ascontiguousarray(a) for a in [g1, g2] + ek + ep + dual_tangents(ek, ep)]
     coords = tuple(a[e2:e2 + nz, e2:e2 + ny, e2:e2 + nx] for a in (xe, ye, ze))
     return out[0], out[1], coords, wall
+
+
+def dual_tangents(ek, ep):
+    """The dual basis (t_ksi, t_phi) of the unit wall tangents ek, ep (3 arrays
+    each) in their plane, as grid_normals.f90:279-291 builds it for the slip
+    walls: with c = ek.ep, t_ksi = (ek - c ep)/(1 - c^2) and t_phi = (ep - c
+    ek)/(1 - c^2), so that m.t_ksi and m.t_phi are the components of m's
+    tangential part on ek and ep.  Returns the 6 arrays tx_ksi, ty_ksi, tz_ksi,
+    tx_phi, ty_phi, tz_phi."""
+    c = ek[0] * ep[0] + ek[1] * ep[1] + ek[2] * ep[2]
+    fe = 1.0 / (1.0 - c * c)
+    return ([fe * (ek[q] - c * ep[q]) for q in range(3)] +
+            [fe * (ep[q] - c * ek[q]) for q in range(3)])
 
 
 def wall_metrics(ye: np.ndarray, lo_wall: bool, hi_wall: bool):

@@ -24,6 +24,7 @@
  *   mod_comm.f90:249-290         halo exchange (periodic self) orc_fill_periodic
  *   mpi_index_bounds.f90         loop bounds for periodic faces and walls
  *   smod_bc_wall.f90:11-98       no-slip wall, dp/dn = 0         bc_wall
+ *   smod_bc_wall_2*.f90          advanced walls (is_wall2)       bc_wall2_face
  *   mod_coeff_deriv.f90:100-160, 1099-1160  closure and extrapolation coefficients
  */
 #include "musica_oracle.h"
@@ -42,7 +43,15 @@
  * _build/plant_n.so).  0 -- the default -- in every other build:
  *   1  pressure sensor: 4 p_i instead of 2 p_i in the cs-weighted denominator
  *   2  interior face amplitude: xk0/10.5 instead of xk0/105
- *   3  Ducros sensor: omega_z missing from the denominator */
+ *   3  Ducros sensor: omega_z missing from the denominator
+ *   4  bc_charac_imax: the textbook sign of the density equation
+ *   5  bc_charac_imax: 0.5 for the 0.528 of the subsonic relaxation
+ *   6  bc_charac_jmax: v for v + c in the outgoing wave
+ *   7  bc_charac_imax: the a04 rows for the a40 ones
+ *   8  c3 wall state: the tangential term with the opposite sign
+ *   9  c3 adiabatic wall gradient: the two tangents exchanged
+ *   10 advanced wall row at the max face: a02 for a20 (no sign change)
+ *   11 c3 slip wall: the unit tangents for their dual basis */
 #ifndef ORC_PLANT
 #define ORC_PLANT 0
 #endif
@@ -81,6 +90,9 @@ static const double ACL[5][5] = {{0.0, 0.0, 0.0, 0.0, 0.0},
                                  {0.0, 2.0 / 3.0, -1.0 / 12.0, 0.0, 0.0},
                                  {0.0, 3.0 / 4.0, -3.0 / 20.0, 1.0 / 60.0, 0.0},
                                  {0.0, 4.0 / 5.0, -1.0 / 5.0, 4.0 / 105.0, -1.0 / 280.0}};
+/* 3-pt one-sided first derivative of an advanced wall row (is_wall2),
+ * mod_coeff_deriv.f90:113-117; a20 = -a02 at the max face. */
+static const double A02[3] = {-3.0 / 2.0, 2.0, -1.0 / 2.0};
 /* 5-pt one-sided first derivatives at the first two rows (mod_coeff_deriv.f90:114-124);
  * a31 = -a13, a40 = -a04 at the last two. */
 static const double A04[6] = {0.0, -25.0 / 12.0, 4.0, -3.0, 4.0 / 3.0, -1.0 / 4.0};
@@ -103,8 +115,8 @@ static walls_t walls_of(const orc_cfg *c) {
   walls_t W;
   W.any = 0;
   for (int a = 0; a < 3; a++) {
-    W.lo[a] = c->face[a][0] == ORC_FACE_WALL;
-    W.hi[a] = c->face[a][1] == ORC_FACE_WALL;
+    W.lo[a] = c->face[a][0] == ORC_FACE_WALL || c->face[a][0] == ORC_FACE_SLIP;
+    W.hi[a] = c->face[a][1] == ORC_FACE_WALL || c->face[a][1] == ORC_FACE_SLIP;
     W.any |= W.lo[a] | W.hi[a];
   }
   return W;
@@ -132,11 +144,22 @@ static walls_t chars_of(const orc_cfg *c) {
   walls_t W;
   W.any = 0;
   for (int a = 0; a < 3; a++) {
-    W.lo[a] = c->face[a][0] >= ORC_FACE_CHARAC;
-    W.hi[a] = c->face[a][1] >= ORC_FACE_CHARAC;
+    W.lo[a] = c->face[a][0] == ORC_FACE_CHARAC || c->face[a][0] == ORC_FACE_INLET;
+    W.hi[a] = c->face[a][1] == ORC_FACE_CHARAC || c->face[a][1] == ORC_FACE_INLET;
     W.any |= W.lo[a] | W.hi[a];
   }
   return W;
+}
+
+/* Slip wall state on face f (0..5): the bindings of assign_procedures.f90:
+ * 415-470 -- with is_wall2, every wall of an inviscid case and the walls with
+ * the slip flag of an adiabatic one; isothermal cases bind no slip routine,
+ * and without is_wall2 the flag selects nothing on this path. */
+static int slip_face(const orc_cfg *c, int f) {
+  const int k = c->face[f / 2][f % 2];
+  if (!c->wall2 || (k != ORC_FACE_WALL && k != ORC_FACE_SLIP)) return 0;
+  if (!c->visc) return 1;
+  return c->adiabatic && k == ORC_FACE_SLIP;
 }
 
 /* Half-width of the centred stencil at row r (1..n) of an axis: 5 inside,
@@ -342,14 +365,17 @@ static void euler_rhs_w(const orc_cfg *c, const orc_grid *g, const dims_t *d, do
   const long sx = 1, sy = d->sx, sz = d->sx * d->sy;
   const long strides[3] = {sx, sy, sz};
   const int nn[3] = {d->nx, d->ny, d->nz};
-  const walls_t W = bounds_of(c);
+  const walls_t W = bounds_of(c), Wl = walls_of(c);
   /* rows with an Euler increment along each axis: 2..n-1 next to boundaries
    * (ndx_e, nfx_e); K on the boundary rows is left as it is (bc_wall zeroes it
-   * on wall rows, the characteristic routines set it on theirs) */
-  int lo_e[3], hi_e[3];
+   * on wall rows, the characteristic routines set it on theirs).  With
+   * is_wall2 the wall rows are advanced too (mpi_index_bounds.f90:28-70). */
+  int lo_e[3], hi_e[3], adv_lo[3], adv_hi[3];
   for (int a = 0; a < 3; a++) {
-    lo_e[a] = W.lo[a] ? 2 : 1;
-    hi_e[a] = nn[a] - (W.hi[a] ? 1 : 0);
+    adv_lo[a] = c->wall2 && Wl.lo[a];
+    adv_hi[a] = c->wall2 && Wl.hi[a];
+    lo_e[a] = (W.lo[a] && !adv_lo[a]) ? 2 : 1;
+    hi_e[a] = nn[a] - ((W.hi[a] && !adv_hi[a]) ? 1 : 0);
   }
   /* line buffers: fluxes along a line with ghosts */
   for (int dir = 0; dir < 3; dir++) {
@@ -386,6 +412,11 @@ static void euler_rhs_w(const orc_cfg *c, const orc_grid *g, const dims_t *d, do
               double dv;
               if (hw == 5) {
                 dv = d11(f, q, 1);
+              } else if (hw == 0) {
+                /* advanced wall row: a02 at the min face, a20 at the max face
+                 * (smod_flux_euler_11pts.f90:42-54, 254-265; _c3.f90:291-302) */
+                dv = i == 1 ? A02[0] * f[q] + A02[1] * f[q + 1] + A02[2] * f[q + 2]
+                            : (ORC_PLANT == 10 ? 1.0 : -1.0) * (A02[0] * f[q] + A02[1] * f[q - 1] + A02[2] * f[q - 2]);
               } else {
                 /* rows 2..5 from a wall: a3, a5, a7, a9 (smod_flux_euler_11pts.f90:36-133) */
                 dv = 0.0;
@@ -899,10 +930,75 @@ static void artvisc(const orc_cfg *c, const orc_grid *g, const dims_t *d, double
 /* bc_wall (mod_bc_apply.f90:237-261), so a later face wins on an edge.  T   */
 /* and p are the primitives last computed (w), not those of Un.              */
 /* ------------------------------------------------------------------------ */
+/* Advanced wall (is_wall2) on face f, over every point of its row
+ * (smod_bc_wall_2.f90:11-473, smod_bc_wall_2_c3.f90:363-712):
+ *   slip: the wall-normal momentum component = 0 on Cartesian blocks
+ *         (bc_wall_jmin_slip :165-185); on the j-faces of c3 blocks the
+ *         momentum m becomes (m.t_ksi) e_ksi + (m.t_phi) e_phi with the unit
+ *         tangents e and their dual basis t (bc_wall_jmin_slip_c3 :363-393);
+ *   no slip: zero momentum and rhoE = rho cv T_wall with the advanced rho,
+ *         T_wall = cextrp2 T(2) - cextrp3 T(3) (adiabatic, :188-214), on c3
+ *         blocks (4 T(2) - T(3) + the tangential terms)/3 (_c3.f90:396-536),
+ *         or the isothermal wall's T_wall (:217-241), T the primitives last
+ *         computed. */
+static void bc_wall2_face(const orc_cfg *c, const orc_grid *g, const dims_t *d, double *const Un[5],
+                          const work_t *w, int f) {
+  const int a = f / 2, side = f % 2;
+  const int nn[3] = {d->nx, d->ny, d->nz};
+  const long st[3] = {1, d->sx, d->sx * d->sy};
+  const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+  const long s = side ? -st[a] : st[a]; /* from the wall into the block */
+  const double cv = 1.0 / (c->gam - 1.0) * c->rg;
+  const int slip = slip_face(c, f);
+  const double c2 = g->cextrp2[f], c3 = g->cextrp3[f];
+  const double *const *wc = g->wc3[f];
+#pragma omp parallel for
+  for (int q2 = 1; q2 <= nn[t2]; q2++)
+    for (int q1 = 1; q1 <= nn[t1]; q1++) {
+      int idx3[3];
+      idx3[t1] = q1; idx3[t2] = q2; idx3[a] = side ? nn[a] : 1;
+      const long p1 = (long)ID(*d, idx3[0], idx3[1], idx3[2]);
+      const long q = (long)(q1 - 1) + (long)nn[t1] * (q2 - 1); /* c3 j-face: (i-1) + nx (k-1) */
+      if (slip) {
+        if (c->curv == 1) {
+          const double m[3] = {Un[1][p1], Un[2][p1], Un[3][p1]};
+          const int o = ORC_PLANT == 11 ? 2 : 8; /* planted: the unit tangents for their dual basis */
+          const double vk = m[0] * wc[o][q] + m[1] * wc[o + 1][q] + m[2] * wc[o + 2][q];
+          const double vp = m[0] * wc[o + 3][q] + m[1] * wc[o + 4][q] + m[2] * wc[o + 5][q];
+          for (int x = 0; x < 3; x++) Un[1 + x][p1] = vk * wc[2 + x][q] + vp * wc[5 + x][q];
+        } else {
+          Un[1 + a][p1] = 0.0;
+        }
+        continue;
+      }
+      const long p2 = p1 + s, p3 = p1 + 2 * s;
+      double t_wall = c->t_wall;
+      if (c->adiabatic) {
+        if (c->curv == 1) {
+          const long si = 1, sk = st[2];
+          t_wall = (1.0 / 3.0) * (4.0 * w->tmp[p2] - w->tmp[p3] + (w->tmp[p1 + si] - w->tmp[p1 - si]) * wc[0][q] +
+                                  (w->tmp[p1 + sk] - w->tmp[p1 - sk]) * wc[1][q]);
+        } else {
+          t_wall = c2 * w->tmp[p2] - c3 * w->tmp[p3];
+        }
+      }
+      Un[1][p1] = 0.0;
+      Un[2][p1] = 0.0;
+      Un[3][p1] = 0.0;
+      Un[4][p1] = Un[0][p1] * (cv * t_wall); /* rho_n*ecalc_tro_pfg */
+    }
+}
+
 static void bc_wall(const orc_cfg *c, const orc_grid *g, const dims_t *d, double *const Un[5], const work_t *w,
                     double *K[5]) {
   const walls_t W = walls_of(c);
   if (!W.any) return;
+  if (c->wall2) {
+    /* faces in the order of bc_wall (mod_bc_apply.f90:237-261); K stays */
+    for (int f = 0; f < 6; f++)
+      if (f % 2 ? W.hi[f / 2] : W.lo[f / 2]) bc_wall2_face(c, g, d, Un, w, f);
+    return;
+  }
   const walls_t B = bounds_of(c);
   const int nn[3] = {d->nx, d->ny, d->nz};
   /* K = 0 on the wall rows over the tangential rows with an Euler increment
@@ -1105,7 +1201,9 @@ static int check_cfg_faces(const orc_cfg *c, int walls_ok) {
   if (c->nx < NG || c->ny < NG || c->nz < NG) return -1;
   for (int dir = 0; dir < 3; dir++)
     for (int s = 0; s < 2; s++)
-      if (c->face[dir][s] != ORC_FACE_PERIODIC && !(walls_ok && c->face[dir][s] == ORC_FACE_WALL)) return -2;
+      if (c->face[dir][s] != ORC_FACE_PERIODIC &&
+          !(walls_ok && (c->face[dir][s] == ORC_FACE_WALL || c->face[dir][s] == ORC_FACE_SLIP)))
+        return -2;
   if (c->filter >= 2 && !c->visc) return -3;              /* sensor / sound speed of primitives_visc */
   if (c->filter == 3 && c->curv == 1) return -8;          /* artvisc_o9: Cartesian or 2.5-D */
   if (c->filter < 0 || c->filter > 4) return -11;
@@ -1295,17 +1393,21 @@ int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_gri
         /* boundaries: Cartesian blocks; c3 blocks take no-slip walls on their
          * j-faces, with interfaces or periodic faces on i and k */
         if (c0->curv == 2) return -5;
-        if (c0->curv == 1 && (bl[b].wall[f] != 1 || f < 2 || f > 3 || bl[b].wall[0] || bl[b].wall[1] ||
+        const int wk = bl[b].wall[f];
+        const int is_w = wk == 1 || wk == 4;
+        if (c0->curv == 1 && (!is_w || f < 2 || f > 3 || bl[b].wall[0] || bl[b].wall[1] ||
                               bl[b].wall[4] || bl[b].wall[5]))
           return -5;
         if (ext < 10) return -7;        /* closures of the two sides must not overlap */
-        if (bl[b].wall[f] < 1 || bl[b].wall[f] > 3) return -12;
+        if (wk < 1 || wk > 4) return -12;
         /* characteristic outflow at imax or jmax, supersonic inlet at imin
-         * (the faces config. 4 uses); walls on the i-faces together with a
-         * characteristic face are not covered */
-        if (bl[b].wall[f] == 2 && f != 1 && f != 3) return -12;
-        if (bl[b].wall[f] == 3 && f != 0) return -12;
-        if (bl[b].wall[f] >= 2 && (bl[b].wall[0] == 1 || bl[b].wall[1] == 1)) return -12;
+         * (the faces config. 4 uses); walls on the i-faces, or advanced walls
+         * (is_wall2), together with a characteristic face are not covered */
+        if (wk == 2 && f != 1 && f != 3) return -12;
+        if (wk == 3 && f != 0) return -12;
+        if ((wk == 2 || wk == 3) &&
+            (c0->wall2 || bl[b].wall[0] == 1 || bl[b].wall[1] == 1 || bl[b].wall[0] == 4 || bl[b].wall[1] == 4))
+          return -12;
       }
   orc_cfg cb[64];
   dims_t d[64];
@@ -1325,9 +1427,13 @@ int orc_step_multi(const orc_cfg *c0, int nb, const orc_block *bl, const orc_gri
     cb[b].nx = bl[b].nx;
     cb[b].ny = bl[b].ny;
     cb[b].nz = bl[b].nz;
-    static const int kinds[4] = {ORC_FACE_INTERFACE, ORC_FACE_WALL, ORC_FACE_CHARAC, ORC_FACE_INLET};
+    static const int kinds[5] = {ORC_FACE_INTERFACE, ORC_FACE_WALL, ORC_FACE_CHARAC, ORC_FACE_INLET, ORC_FACE_SLIP};
     for (int f = 0; f < 6; f++) cb[b].face[f / 2][f % 2] = kinds[bl[b].wall[f]];
     if (bl[b].nx < NG || bl[b].ny < NG || bl[b].nz < NG) return -1;
+    /* c3 walls need their coefficient slabs, slip ones also the dual tangents */
+    if (cb[b].curv == 1)
+      for (int f = 2; f < 4; f++)
+        if (bl[b].wall[f] && (!g[b].wc3[f][0] || (slip_face(&cb[b], f) && !g[b].wc3[f][8]))) return -13;
     d[b] = mkdims(&cb[b]);
     chb[b] = chars_of(&cb[b]).any;
     cst[b] = NULL;

@@ -30,13 +30,18 @@ extern "C" {
 enum orc_face_kind {
   ORC_FACE_PERIODIC = 0,   /* neighbour block is the block itself          */
   ORC_FACE_INTERFACE = 1,  /* 1-to-1 interface with another block          */
-  ORC_FACE_WALL = 2,       /* no-slip wall, sort=0 (is_wall2 = F): Cartesian, or the
-                            * j-faces of c3 blocks                          */
+  ORC_FACE_WALL = 2,       /* wall, sort=0: no-slip with dp/dn = 0 (is_wall2 = F) or
+                            * advanced (is_wall2 = T); Cartesian, or the j-faces of
+                            * c3 blocks                                     */
   ORC_FACE_CHARAC = 3,     /* characteristic (LODI) outflow, sort=-3 at imax or jmax:
                             * bc_charac_imax / bc_charac_jmax, src/Boundary/mod_charac.f90:350-653,
                             * 816-969 (Cartesian)                              */
-  ORC_FACE_INLET = 4       /* supersonic inlet, sort=-3 at imin with Mach >= 1:
+  ORC_FACE_INLET = 4,      /* supersonic inlet, sort=-3 at imin with Mach >= 1:
                             * bc_supersonic_inlet, mod_charac.f90:997-1149    */
+  ORC_FACE_SLIP = 5        /* wall with the slip flag 's' (is_slip, mod_bc.f90:127-182):
+                            * a slip wall where assign_procedures.f90:415-470 binds
+                            * one (is_wall2 with an inviscid or adiabatic case), a
+                            * wall of the case's kind otherwise                  */
 };
 
 typedef struct orc_cfg {
@@ -63,6 +68,9 @@ typedef struct orc_cfg {
   double t_wall;           /* wall temperature of isothermal walls          */
   double p_ref;            /* reference pressure: the subsonic-outflow relaxation of
                             * bc_charac_imax, L1 = 0.1 (p - 0.528 p_ref)      */
+  int wall2;               /* is_wall2 (type_wall_bc = 2, read_param.f90:373): wall
+                            * rows advanced, smod_bc_wall_2*.f90 wall states  */
+  int pad2_;
 } orc_cfg;
 
 typedef struct orc_grid {
@@ -86,8 +94,10 @@ typedef struct orc_grid {
    * grid_normals.f90:243-300 computes from the metrics and the grid:
    * [0] gksigeta3 = -grad(ksi).grad(eta)/|grad(eta)|^2, [1] gphigeta3 (phi for
    * ksi), [2..4] the unit tangent x_ksi/|x_ksi| (txn_ksi, tyn_ksi, tzn_ksi),
-   * [5..7] x_phi/|x_phi| (txn_phi, tyn_phi, tzn_phi) */
-  const double *wc3[6][8];
+   * [5..7] x_phi/|x_phi| (txn_phi, tyn_phi, tzn_phi); slip walls also
+   * [8..10] (tx_ksi, ty_ksi, tz_ksi) and [11..13] (tx_phi, ty_phi, tz_phi),
+   * the dual basis of the two unit tangents (grid_normals.f90:279-291) */
+  const double *wc3[6][14];
 } orc_grid;
 
 /* One block of a multiblock case.  Face f (imin,imax,jmin,jmax,kmin,kmax =
@@ -104,8 +114,9 @@ typedef struct orc_block {
   int nbr_block[6];
   int nbr_face[6];
   int orient[6];
-  int wall[6];             /* the face is a boundary (nbr_* ignored): 1 no-slip wall,
-                            * 2 characteristic outflow (imax, jmax), 3 supersonic inlet (imin) */
+  int wall[6];             /* the face is a boundary (nbr_* ignored): 1 wall,
+                            * 2 characteristic outflow (imax, jmax), 3 supersonic inlet (imin),
+                            * 4 wall with the slip flag (ORC_FACE_SLIP) */
 } orc_block;
 
 /* Fill the face ghosts of every block from its neighbours' interior
@@ -137,7 +148,17 @@ void orc_fill_interfaces(int nblocks, const orc_block *blocks, double *const *ar
  *     wrote, viscous or not),
  *     rho = p/(R T), rhoE = rho cv T (mod_eos_pfg.f90:170-184, 303-317);
  *   - the exchange runs before bc_wall, so face ghosts on a wall row keep the
- *     neighbour's value from before the wall state was imposed. */
+ *     neighbour's value from before the wall state was imposed.
+ * With c->wall2 (is_wall2 = T) the wall rows are advanced instead
+ * (ndx_e = 1, mpi_index_bounds.f90:28-70): the interior derivatives along
+ * the wall plus the 3-point one-sided a02 (a20) derivative across it
+ * (smod_flux_euler_11pts.f90:42-54, 286-297; times idy1_jmin, the wall row's
+ * entry of idy, on Cartesian blocks), the viscous increment as above, and
+ * bc_wall imposes the smod_bc_wall_2*.f90 states on the advanced rows: a slip
+ * wall removes the wall-normal momentum (the Cartesian normal component, or
+ * the projection on the wall's tangent plane of a c3 wall), a no-slip wall
+ * zeroes the momentum and sets rhoE = rho cv T_wall with the advanced rho
+ * and T_wall extrapolated (adiabatic) or given (isothermal). */
 int orc_step_multi(const orc_cfg *c, int nblocks, const orc_block *blocks, const orc_grid *grids,
                    double *const *U, int nsteps);
 

@@ -33,7 +33,7 @@ class OrcCfg(C.Structure):
         ("cok_ratio", C.c_double),
         ("chi", C.c_double), ("dissip_shock", C.c_double), ("csens", C.c_double),
         ("adiabatic", C.c_int), ("sw_edoh", C.c_int), ("t_wall", C.c_double),
-        ("p_ref", C.c_double),
+        ("p_ref", C.c_double), ("wall2", C.c_int), ("pad2_", C.c_int),
     ]
 
 
@@ -54,7 +54,7 @@ class OrcGrid(C.Structure):
         ("cextrp2", C.c_double * 6), ("cextrp3", C.c_double * 6),
         ("m2", _dp * 5),
         ("m2v", _dp * 5),
-        ("wc3", (_dp * 8) * 6),
+        ("wc3", (_dp * 14) * 6),
     ]
 
 
@@ -119,8 +119,10 @@ class Oracle:
         for d in range(3):
             for s in range(2):
                 c.face[d][s] = faces[2 * d + s]
-        # boundary faces (wall 2, characteristic outflow 3, supersonic inlet 4)
+        # boundary faces (wall 2, characteristic outflow 3, supersonic inlet 4,
+        # wall with the slip flag 5)
         self.walls = [int(k >= 2) for k in faces]
+        c.wall2 = int(getattr(case, "wall2", False))
         c.adiabatic = int(getattr(case, "adiabatic", True))
         c.t_wall = float(getattr(case, "t_wall", 0.0))
         c.p_ref = float(getattr(case, "p_ref", 0.0))
@@ -159,9 +161,10 @@ class Oracle:
         for f in range(6):
             g.cextrp2[f] = float(getattr(case, "cextrp2", (0.0,) * 6)[f])
             g.cextrp3[f] = float(getattr(case, "cextrp3", (0.0,) * 6)[f])
-        # c3 walls: the per-point coefficients of each wall face (8 slabs)
+        # c3 walls: the per-point coefficients of each wall face (8 slabs, 14
+        # with the dual tangents of a slip wall)
         for f, slabs in (getattr(case, "wall_c3", None) or {}).items():
-            for q in range(8):
+            for q in range(len(slabs)):
                 a = np.ascontiguousarray(slabs[q], dtype=np.float64)
                 self._keep.append(a)
                 g.wc3[f][q] = _p(a)
@@ -222,8 +225,8 @@ class Oracle:
 
 def self_block(case):
     """One-block table: periodic faces are interfaces with the block itself;
-    faces of kind 2 (wall), 3 (characteristic outflow), 4 (supersonic inlet)
-    are boundaries of that kind."""
+    faces of kind 2 (wall), 3 (characteristic outflow), 4 (supersonic inlet),
+    5 (wall with the slip flag) are boundaries of that kind."""
     faces = tuple(getattr(case, "faces", (0,) * 6))
     return dict(nx=case.nx, ny=case.ny, nz=case.nz, nbr_block=[0] * 6,
                 nbr_face=[f ^ 1 for f in range(6)], orient=[0] * 6,

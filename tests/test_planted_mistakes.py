@@ -27,6 +27,10 @@ PLANTS = {
     # gradient exchanged
     8: ("tests/test_c3_wall_oracle.py", "test_sheared_wall_state_is_the_closed_form"),
     9: ("tests/test_c3_wall_oracle.py", "test_unit_c3_walls_equal_cartesian_walls"),
+    # advanced walls (is_wall2): a02 at the max face without the sign of a20,
+    # the unit tangents of a c3 slip wall for their dual basis
+    10: ("tests/test_wall2_oracle.py", "test_advanced_rows_take_the_exact_derivative"),
+    11: ("tests/test_wall2_oracle.py", "test_c3_slip_wall_keeps_exactly_the_tangential_momentum"),
 }
 
 
