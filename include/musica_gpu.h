@@ -55,16 +55,22 @@ extern "C" {
 
 #define MGPU_FACE_PERIODIC 0
 #define MGPU_FACE_INTERFACE 1
-/* No-slip wall, BC_face%sort = 0 with is_wall2 = F (the reference's default,
- * src/File_IO/read_param.f90:75): bc_wall_*_adiabatic_dpdn or
- * _isotherm_dpdn (src/Boundary/smod_bc_wall.f90, smod_bc_wall_c3.f90) with
- * the boundary-row closures of the Euler, viscous and filter operators.  On
- * the faces of one axis per block (i, j or k; a block with walls on its i- or
- * k-faces is stored with that axis exchanged with j, invisibly to the caller)
- * of Cartesian cases, or on the j-faces of 3-D curvilinear cases (FP64 or
- * mixed, any filter), at least 10 cells between the walls; the block's
- * extrapolation coefficients come from mgpu_set_wall (Cartesian) or
- * mgpu_set_wall_c3 (curvilinear).
+/* Wall, BC_face%sort = 0.  With is_wall2 = F (the reference's default,
+ * src/File_IO/read_param.f90:75) a no-slip wall: bc_wall_*_adiabatic_dpdn
+ * or _isotherm_dpdn (src/Boundary/smod_bc_wall.f90, smod_bc_wall_c3.f90)
+ * with the boundary-row closures of the Euler, viscous and filter operators.
+ * With mgpu_case.is_wall2 (type_wall_bc = 2) the wall rows are advanced
+ * (mpi_index_bounds.f90:28-70: the interior derivatives along the wall, the
+ * 3-point one-sided a02 / a20 across it, smod_flux_euler_11pts.f90:42-54,
+ * 286-297) and take the states of smod_bc_wall_2*.f90: no slip (zero
+ * momentum, rhoE = rho cv T_wall with the advanced rho), or slip for every
+ * wall of an inviscid case (see MGPU_FACE_SLIP).  On the faces of one axis
+ * per block (i, j or k; a block with walls on its i- or k-faces is stored
+ * with that axis exchanged with j, invisibly to the caller) or on the i- and
+ * j-faces together of Cartesian cases, or on the j-faces of 3-D curvilinear
+ * cases (FP64 or mixed, any filter), at least 10 cells between the walls;
+ * the block's extrapolation coefficients come from mgpu_set_wall
+ * (Cartesian) or mgpu_set_wall_c3 (curvilinear).
  * With a shock sensor (filters 2, 4) the ghost row beyond a wall must hold a
  * valid state:
  * the wall row's pressure sensor reads p there (mod_filtering_shock.f90:
@@ -92,6 +98,16 @@ extern "C" {
  * and the inflow state from the uploaded state.  Like walls, the ghost row
  * beyond such a face must hold a valid state when the shock filter runs. */
 #define MGPU_FACE_INLET 4
+/* Wall with the slip flag 's' (bl%flag = 's', is_slip, src/Boundary/mod_bc.f90:
+ * 127-182): with is_wall2 and an adiabatic (or inviscid) case a slip wall,
+ * bc_wall_*_slip (smod_bc_wall_2.f90:11-30, 165-185): the advanced wall row
+ * loses its wall-normal momentum component and keeps the rest; on a 3-D
+ * curvilinear block its momentum becomes its projection on the wall's
+ * tangent plane (bc_wall_jmin_slip_c3, smod_bc_wall_2_c3.f90:363-393), which
+ * needs mgpu_set_wall_c3_slip.  Otherwise (is_wall2 = F, or an isothermal
+ * viscous case) exactly an MGPU_FACE_WALL: the binding of
+ * src/Main/assign_procedures.f90:415-530 selects no slip routine there. */
+#define MGPU_FACE_SLIP 5
 
 /* Filter selection: is_SF / is_shock (src/Main/assign_procedures.f90:161-219) */
 #define MGPU_FILTER_NONE 0
@@ -167,6 +183,10 @@ typedef struct mgpu_case {
   double t_wall;          /* walls: temperature of an isothermal wall (T_wall) */
   double p_ref;           /* characteristic imax outflow: reference pressure of */
                           /* its subsonic relaxation (p_ref, mod_charac.f90:426) */
+  int32_t is_wall2;       /* type_wall_bc = 2 (is_wall2, read_param.f90:373):  */
+                          /* walls advanced, smod_bc_wall_2*.f90 wall states;   */
+                          /* not with characteristic or inlet faces            */
+  int32_t reserved0;      /* 0                                                 */
 } mgpu_case;
 
 typedef struct mgpu_handle mgpu_handle;
@@ -205,7 +225,9 @@ int mgpu_set_metrics_c2(mgpu_handle *h, int block, const double *const *m2, cons
  * cextrp3*p(3) along the wall normal.  Required before the first step of
  * such a block.  The Euler metric (idy) of a wall block holds, on rows 2..5
  * (ny-1..ny-4), the idy2/idy4/idy6/idy8 of grid_metrics_wall_cart
- * (src/Grid/grid_metrics_cart.f90:190-436), which the closure rows use. */
+ * (src/Grid/grid_metrics_cart.f90:190-436), which the closure rows use, and
+ * on the wall row idy1_jmin = 1/(a02 y) (idy1_jmax), which an advanced wall
+ * row (is_wall2) uses (:245-256). */
 int mgpu_set_wall(mgpu_handle *h, int block, const double *cextrp2, const double *cextrp3);
 
 /* Wall coefficients of a j-face (face 2 or 3) of a 3-D curvilinear block
@@ -219,6 +241,18 @@ int mgpu_set_wall(mgpu_handle *h, int block, const double *cextrp2, const double
  * and x_phi/|x_phi| (txn_phi, tyn_phi, tzn_phi).  Required for every wall
  * face of such a block before its first step. */
 int mgpu_set_wall_c3(mgpu_handle *h, int block, int face, const double *const *coef);
+
+/* Dual tangents of a slip j-face (face 2 or 3) of a 3-D curvilinear block,
+ * as grid_normals.f90:279-291 computes them for bc_wall_jmin/jmax_slip_c3
+ * (src/Boundary/smod_bc_wall_2_c3.f90:363-393, 539-569): t[0..5] are arrays
+ * of nx*nz doubles (i fastest): tx_ksi, ty_ksi, tz_ksi, tx_phi, ty_phi,
+ * tz_phi, the dual basis of the unit tangents of mgpu_set_wall_c3 in the
+ * tangent plane ((e_ksi - c e_phi)/(1 - c^2) and its twin, c = e_ksi.e_phi),
+ * so that the wall momentum m becomes (m.t_ksi) e_ksi + (m.t_phi) e_phi.
+ * Required, with mgpu_set_wall_c3, for every face of such a block that is a
+ * slip wall (MGPU_FACE_SLIP with is_wall2 and an adiabatic case, or any wall
+ * of an inviscid is_wall2 case) before its first step. */
+int mgpu_set_wall_c3_slip(mgpu_handle *h, int block, int face, const double *const *t);
 
 /* Upload the conserved variables rho,rhou,rhov,rhow,rhoe of a block (padded
  * arrays; ghosts are refreshed on the device). */

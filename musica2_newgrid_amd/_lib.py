@@ -18,6 +18,7 @@ FACE_INTERFACE = 1
 FACE_WALL = 2
 FACE_CHARAC = 3
 FACE_INLET = 4
+FACE_SLIP = 5
 FILTER_NONE, FILTER_SF, FILTER_SHOCK, FILTER_ARTVISC, FILTER_ARTVISC_SHOCK = 0, 1, 2, 3, 4
 PREC_FP64, PREC_MIXED = 0, 1
 GEOM_CART, GEOM_C3, GEOM_C2 = 0, 1, 2
@@ -35,6 +36,7 @@ EXPORTS = (
     "mgpu_set_metrics_c2",
     "mgpu_set_wall",
     "mgpu_set_wall_c3",
+    "mgpu_set_wall_c3_slip",
     "mgpu_put_state",
     "mgpu_get_state",
     "mgpu_step",
@@ -96,6 +98,8 @@ class MgpuCase(C.Structure):
         ("is_sw_edoh", C.c_int32),
         ("t_wall", C.c_double),
         ("p_ref", C.c_double),
+        ("is_wall2", C.c_int32),
+        ("reserved0", C.c_int32),
     ]
 
 
@@ -128,6 +132,7 @@ def lib() -> C.CDLL:
     L.mgpu_set_metrics_c2.argtypes = [C.c_void_p, C.c_int, C.POINTER(_dp), C.POINTER(_dp), _dp, _dp]
     L.mgpu_set_wall.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     L.mgpu_set_wall_c3.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(_dp)]
+    L.mgpu_set_wall_c3_slip.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(_dp)]
     L.mgpu_put_state.argtypes = [C.c_void_p, C.c_int] + [_dp] * 5
     L.mgpu_get_state.argtypes = [C.c_void_p, C.c_int] + [_dp] * 5
     L.mgpu_step.argtypes = [C.c_void_p, C.c_int]
@@ -169,7 +174,7 @@ def lib() -> C.CDLL:
                  "mgpu_server_take"):
         getattr(L, name).restype = C.c_int
     for name in ("mgpu_create", "mgpu_set_metrics_cart", "mgpu_set_metrics_c3", "mgpu_set_metrics_c2", "mgpu_set_wall",
-                 "mgpu_set_wall_c3",
+                 "mgpu_set_wall_c3", "mgpu_set_wall_c3_slip",
                  "mgpu_put_state",
                  "mgpu_get_state", "mgpu_step", "mgpu_autotune", "mgpu_stage_config", "mgpu_sync", "mgpu_profile", "mgpu_kernel_time"):
         getattr(L, name).restype = C.c_int

@@ -255,15 +255,20 @@ def vortex_channel(nx: int = 512, ny: int = 256, nz: int = 128, mach: float = 0.
                    radius: float = 0.5, re: float = 1.0e4, cfl: float = 0.8, filter: int = 2,
                    lengths=None, amp: float = 0.05, ratio: float = 1.05, delta: float = 0.1,
                    adiabatic: bool = True, T_inf: float = 300.0, rho_inf: float = 1.177,
-                   iorder_visc: int = 4) -> Case:
+                   iorder_visc: int = 4, slip: bool = False) -> Case:
     """The curvilinear headline block as stated: x uniform and periodic, y
-    between no-slip walls at j = 1 and j = ny with tanh clustering at both
-    (largest spacing ratio `ratio`), the wavy map y' = y + amp sin(2 pi x/Lx)
-    (1 - y/Ly) (a wavy lower wall, a flat upper one), z uniform and periodic;
-    3-D GCL metrics; an isentropic vortex (strength beta, radius R) advected
-    at Mach `mach` in the middle of the channel.  The free stream meets the
-    walls through a thin laminar layer u = U tanh(d/delta) of the distance d
-    to the nearer wall (no slip on the wall rows)."""
+    between walls at j = 1 and j = ny with tanh clustering at both (largest
+    spacing ratio `ratio`), the wavy map y' = y + amp sin(2 pi x/Lx) (1 -
+    y/Ly) (a wavy lower wall, a flat upper one), z uniform and periodic; 3-D
+    GCL metrics; an isentropic vortex (strength beta, radius R) advected at
+    Mach `mach` in the middle of the channel.
+    slip=False: no-slip walls (is_wall2 = F); the free stream meets them
+    through a thin laminar layer u = U tanh(d/delta) of the distance d to the
+    nearer wall, zero velocity on the wall rows.
+    slip=True: slip walls, type_wall_bc = 2 with the flag 's' on both y-faces
+    (an adiabatic case); the uniform free stream runs along them."""
+    if slip and iorder_visc and not adiabatic:
+        raise ValueError("slip walls need an adiabatic case (an isothermal one binds no slip routine)")
     if lengths is None:
         h = 10.0 / nx
         lengths = (nx * h, ny * h, nz * h)
@@ -283,14 +288,15 @@ def vortex_channel(nx: int = 512, ny: int = 256, nz: int = 128, mach: float = 0.
     e = np.exp(0.5 * (1.0 - r2))
     eta = eta_of(np.arange(1, ny + 1)).reshape(1, -1, 1)
     dwall = np.minimum(eta, Ly - eta)
-    layer = np.tanh(dwall / delta)
+    layer = 1.0 if slip else np.tanh(dwall / delta)
     u = u_inf * (layer - beta / (2.0 * math.pi) * (yc - yv) / radius * e)
     v = u_inf * (beta / (2.0 * math.pi) * (xc - xv) / radius * e)
     w = np.zeros_like(u)
     T = T_inf * (1.0 - (GAM - 1.0) * beta ** 2 * mach ** 2 / (8.0 * math.pi ** 2) * e * e)
     rho = rho_inf * (T / T_inf) ** (1.0 / (GAM - 1.0))
     p = rho * RG * T
-    u[:, 0, :] = v[:, 0, :] = u[:, -1, :] = v[:, -1, :] = 0.0
+    if not slip:
+        u[:, 0, :] = v[:, 0, :] = u[:, -1, :] = v[:, -1, :] = 0.0
     U = []
     for f in conserved(rho, u, v, w, p):
         a = gr.padded(nx, ny, nz)
@@ -306,12 +312,12 @@ def vortex_channel(nx: int = 512, ny: int = 256, nz: int = 128, mach: float = 0.
     dmin = min(float(np.min(np.sqrt(np.diff(xc, axis=a) ** 2 + np.diff(yc, axis=a) ** 2 +
                                     np.diff(zc, axis=a) ** 2))) for a in (0, 1, 2))
     cs = Case("vortex_channel", nx, ny, nz, curvilinear=True, iorder_visc=iorder_visc, filter=filter,
-              ducros=True, diffscale=diffscale, m=m, mv=mv, U=U, faces=(0, 0, 2, 2, 0, 0),
-              adiabatic=adiabatic, t_wall=T_inf)
+              ducros=True, diffscale=diffscale, m=m, mv=mv, U=U, faces=(0, 0, 5, 5, 0, 0) if slip else (0, 0, 2, 2, 0, 0),
+              adiabatic=adiabatic, t_wall=T_inf, wall2=slip)
     cs.wall_c3 = wall
     cs.dt = timestep(U, dmin, cfl)
     cs.info = dict(mach=mach, beta=beta, radius=radius, re=re, cfl=cfl, lengths=lengths, amp=amp,
-                   ratio=ratio, tanh_b=b, delta=delta, dmin=dmin)
+                   ratio=ratio, tanh_b=b, delta=delta, dmin=dmin, slip=slip)
     return cs
 
 
@@ -464,12 +470,15 @@ def channel(nx: int = 64, ny: int = 48, nz: int = 16, lengths=(4.0, 2.0, 1.0), b
             mach: float = 0.3, re: float = 2000.0, cfl: float = 0.5, chi: float = 0.2, filter: int = 1,
             iorder_visc: int = 4, adiabatic: bool = True, t_wall: float = 300.0, pulse: float = 1e-2,
             seed: int = 20261202, T_ref: float = 300.0, rho_ref: float = 1.177,
-            symmetric: bool = False) -> Case:
+            symmetric: bool = False, wall2: bool = False, slip: bool = False) -> Case:
     """Channel between no-slip walls at y = 0 and y = Ly (rows 1 and ny), periodic in
     x and z, y-nodes clustered at the walls.  Flow: a parabolic streamwise profile
     of Mach mach, 3-D velocity perturbations vanishing at the walls, and a Gaussian
     pressure pulse; temperature T_ref (the isothermal wall temperature t_wall).
-    symmetric: the flow is mirror-symmetric about y = Ly/2 (v antisymmetric)."""
+    symmetric: the flow is mirror-symmetric about y = Ly/2 (v antisymmetric).
+    wall2: advanced walls (type_wall_bc = 2); slip: with the flag 's' on both
+    walls (and wall2), and a uniform streamwise 0.3 mach added so that the
+    walls carry tangential flow."""
     Lx, Ly, Lz = lengths
     dx, dz = Lx / nx, Lz / nz
     ye = stretched_wall_coords(ny, Ly, beta)
@@ -487,6 +496,8 @@ def channel(nx: int = 64, ny: int = 48, nz: int = 16, lengths=(4.0, 2.0, 1.0), b
     ph = rng.uniform(0.0, 2.0 * math.pi, 6)
     kx, kz = 2.0 * math.pi / Lx, 2.0 * math.pi / Lz
     u = u_ref * (env + 0.05 * env * np.sin(kx * X + ph[0]) * np.cos(kz * Z + ph[1]))
+    if slip:
+        u = u + 0.3 * u_ref
     v = 0.05 * u_ref * env * env * np.sin(2.0 * kx * X + ph[2]) * np.sin(kz * Z + ph[3])
     if symmetric:
         v = v * (1.0 - 2.0 * eta)
@@ -516,8 +527,8 @@ def channel(nx: int = 64, ny: int = 48, nz: int = 16, lengths=(4.0, 2.0, 1.0), b
     ext2[2], ext2[3], ext3[2], ext3[3] = c2[0], c2[1], c3[0], c3[1]
     c = Case("channel", nx, ny, nz, curvilinear=False, iorder_visc=iorder_visc, filter=filter, ducros=False,
              diffscale=mu_ref / muw_ref, chi=chi, cart=[idx, ide, idz, idx.copy(), idv, idz.copy()], U=U,
-             faces=(0, 0, 2, 2, 0, 0), adiabatic=adiabatic, t_wall=t_wall,
-             cextrp2=tuple(ext2), cextrp3=tuple(ext3))
+             faces=(0, 0, 5, 5, 0, 0) if slip else (0, 0, 2, 2, 0, 0), adiabatic=adiabatic, t_wall=t_wall,
+             cextrp2=tuple(ext2), cextrp3=tuple(ext3), wall2=wall2 or slip)
     c.dt = timestep(U, min(dx, dz, float(np.min(np.diff(y)))), cfl)
     c.info = dict(mach=mach, re=re, cfl=cfl, beta=beta, lengths=lengths, y=y, u_ref=u_ref, p_ref=p_ref)
     return c

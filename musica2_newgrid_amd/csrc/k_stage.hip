@@ -360,7 +360,14 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
   bool hs_beyond[NS];
   bool own_beyond = false, own_beyond_x = false;
   double cw[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ci[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  // the own point's weight: nonzero on an advanced wall row (is_wall2), whose
+  // a02 / a20 derivative reads the row itself; with the pencil beyond the wall
+  // at 0 it is c0 f0 + 2 (f1 - f-1) - 1/2 (f2 - f-2), c0 = -3/2 at the min
+  // face and +3/2 at the max face (smod_flux_euler_11pts.f90:286-297, 584-595)
+  double cw0 = 0.0, ci0 = 0.0;
   int hw = 5, hwi = 5;
+  // a boundary row without an Euler increment (flux_euler skips it)
+  bool krow = false;
   if constexpr (WALL) {
 #pragma unroll
     for (int s = 0; s < NS; s++) {
@@ -371,10 +378,24 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
     own_beyond = (g.jbnd & 2) && j > g.ny;
     hw = jhalf(g, j);
     wall_coeffs(hw, cw);
+    const bool jadv = hw == 0 && (((g.jadv & 1) && j == 1) || ((g.jadv & 2) && j == g.ny));
+    if (jadv) {
+      cw[0] = A02_2;
+      cw[1] = A02_3;
+      cw0 = j == 1 ? A02_1 : -A02_1;
+    }
+    krow = hw == 0 && !jadv;
     if constexpr (IBND) {
       own_beyond_x = (g.ibnd & 2) && i > g.nx;
       hwi = ihalf(g, i);
       wall_coeffs(hwi, ci);
+      const bool iadv = hwi == 0 && (((g.iadv & 1) && i == 1) || ((g.iadv & 2) && i == g.nx));
+      if (iadv) {
+        ci[0] = A02_2;
+        ci[1] = A02_3;
+        ci0 = i == 1 ? A02_1 : -A02_1;
+      }
+      krow = krow || (hwi == 0 && !iadv);
     }
     P.kbc[0] = a.kbc[0];
     P.kbc[1] = a.kbc[1];
@@ -526,7 +547,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
         R dx;
         if constexpr (IBND) {
           const R *f = &sfx[v][ty][tx + 5];
-          dx = R(ci[0]) * (f[1] - f[-1]) + R(ci[1]) * (f[2] - f[-2]) + R(ci[2]) * (f[3] - f[-3]) +
+          dx = R(ci0) * f[0] + R(ci[0]) * (f[1] - f[-1]) + R(ci[1]) * (f[2] - f[-2]) + R(ci[2]) * (f[3] - f[-3]) +
                R(ci[3]) * (f[4] - f[-4]) + R(ci[4]) * (f[5] - f[-5]);
         } else {
           dx = d11c(&sfx[v][ty][tx + 5], 1);
@@ -534,8 +555,9 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
         R dy;
         if constexpr (WALL) {
           const R *f = &sfy[v][ty + 5][tx];
-          dy = R(cw[0]) * (f[TX] - f[-TX]) + R(cw[1]) * (f[2 * TX] - f[-2 * TX]) + R(cw[2]) * (f[3 * TX] - f[-3 * TX]) +
-               R(cw[3]) * (f[4 * TX] - f[-4 * TX]) + R(cw[4]) * (f[5 * TX] - f[-5 * TX]);
+          dy = R(cw0) * f[0] + R(cw[0]) * (f[TX] - f[-TX]) + R(cw[1]) * (f[2 * TX] - f[-2 * TX]) +
+               R(cw[2]) * (f[3 * TX] - f[-3 * TX]) + R(cw[3]) * (f[4 * TX] - f[-4 * TX]) +
+               R(cw[4]) * (f[5 * TX] - f[-5 * TX]);
         } else {
           dy = d11c(&sfy[v][ty + 5][tx], TX);
         }
@@ -557,10 +579,11 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
       // boundary rows: no Euler increment (flux_euler skips them); a wall row
       // keeps 0 (bc_wall zeroes it) plus the viscous increment, a
       // characteristic or inlet row takes its face slab, set by the boundary
-      // routines (no viscous increment there either: ndx_v = 2)
+      // routines (no viscous increment there either: ndx_v = 2).  An advanced
+      // wall row keeps its increment.
       bool kslab = false;
       if constexpr (WALL) {
-        if (hwi == 0 || hw == 0) {
+        if (krow) {
           const bool ia = hwi == 0;
           const double *kb = ia ? (i == 1 ? P.kbc[0] : P.kbc[1]) : (j == 1 ? P.kbc[2] : P.kbc[3]);
           if (kb) {

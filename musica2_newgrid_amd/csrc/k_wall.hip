@@ -29,6 +29,11 @@
 //               imposed, unlike the neighbour's interior point.  With tau/mu
 //               and the heat-flux gradient exchanged, this kernel rebuilds the
 //               flux of each wall-row point from its own u, v, w and mu(T).
+//               Likewise the pressure of the shock sensor and the spectral
+//               radii of the artificial viscosities, which primitives_visc
+//               computes on the ghosts from their own state
+//               (smod_primitive_pfg.f90:84-86), not exchanged: an advanced
+//               wall row (is_wall2) keeps the dissipation they weight.
 //
 // k_vdiv_wall   the viscous divergence on the same rows with one-sided
 //               a04/a13 (a31/a40) along each bounded axis
@@ -179,6 +184,98 @@ __device__ __forceinline__ double d5at(const double *f, idx_t p, idx_t st, int q
   return d;
 }
 
+// Advanced wall point (is_wall2) of the face (jsec ? j : i, side), row point
+// r of its section, after the stage's update: smod_bc_wall_2.f90:11-473 and
+// smod_bc_wall_2_c3.f90:363-712.  A slip face removes the normal momentum
+// (Cartesian: the normal component; c3: m becomes (m.t_ksi) e_ksi +
+// (m.t_phi) e_phi); a no-slip face zeroes the momentum and sets rhoE = rho cv
+// T_wall with the row's advanced rho, T_wall extrapolated from rows 2, 3 (and
+// on c3 walls the row's tangential neighbours) of the primitives of `src`
+// (adiabatic) or the isothermal wall's.  At an edge of i- and j-walls the
+// j-face's thread applies the i-face first, then its own (bc_wall's order,
+// mod_bc_apply.f90:249-255).  With dst = the stash only the extrapolated
+// temperatures are kept (j-face, then i-face of an edge); with src = the
+// stash they are read back.
+__device__ void wall_advanced(const WallArgs &a, bool jsec, int side, long r, int i, int k, idx_t p1, idx_t p2,
+                              idx_t p3, idx_t sidx) {
+  const Geom &g = a.g;
+  const bool from_src = a.src.v[0] || a.src32[0];
+  const bool to_stash = !a.dst.v[0] && !a.dst32[0];
+  auto prim = [&](idx_t q) {
+    return a.src.v[0] ? prim_at(a.src, q, a.gas.gam1, a.gas.rg) : prim_at32(a.src32, a.ref, q, a.gas.gam1, a.gas.rg);
+  };
+  // the i-face this j-wall point shares (an edge of walls on two axes), -1 if none
+  const int es = !jsec ? -1 : (((g.iwall & 1) && i == 1) ? 0 : (((g.iwall & 2) && i == g.nx) ? 1 : -1));
+  const bool slip = ((jsec ? a.jslip : a.islip) >> side) & 1;
+  const bool slip_e = es >= 0 && ((a.islip >> es) & 1);
+  double tw = a.t_wall, tw_e = a.t_wall;
+  if (a.adiabatic) {
+    if (from_src) {
+      if (!slip) {
+        const Pw q2 = prim(p2), q3 = prim(p3);
+        if (jsec && a.wc3[side]) {
+          const long nface = (long)g.nx * g.nz;
+          const Pw xm = prim(p1 - 1), xp = prim(p1 + 1), zm = prim(p1 - g.sy), zp = prim(p1 + g.sy);
+          const double gk = ldg(a.wc3[side], (idx_t)r), gp = ldg(a.wc3[side], (idx_t)(nface + r));
+          tw = (1.0 / 3.0) * (4.0 * q2.T - q3.T + (xp.T - xm.T) * gk + (zp.T - zm.T) * gp);
+        } else {
+          tw = (jsec ? a.c2[side] : a.ci2[side]) * q2.T - (jsec ? a.c3[side] : a.ci3[side]) * q3.T;
+        }
+      }
+      if (es >= 0 && !slip_e) {
+        const idx_t e2 = es ? p1 - 1 : p1 + 1, e3 = es ? p1 - 2 : p1 + 2;
+        tw_e = a.ci2[es] * prim(e2).T - a.ci3[es] * prim(e3).T;
+      }
+    } else {
+      tw = ldg(a.stash, sidx);
+      tw_e = ldg(a.stash, sidx + 1);
+    }
+  }
+  if (to_stash) {
+    stg(a.stash, sidx, tw);
+    stg(a.stash, sidx + 1, tw_e);
+    return;
+  }
+  double u[5];
+#pragma unroll
+  for (int v = 0; v < 5; v++) u[v] = a.dst.v[0] ? ldg(a.dst.v[v], p1) : a.ref[v] + (double)ldg(a.dst32[v], p1);
+  bool chg[5] = {false, false, false, false, false};
+  // one face's wall state on u: axis 0 (i) or 1 (j)
+  auto apply = [&](bool sl, int axis, double t) {
+    if (!sl) {
+      u[1] = u[2] = u[3] = 0.0;
+      u[4] = u[0] * (a.cv * t);
+      chg[1] = chg[2] = chg[3] = chg[4] = true;
+    } else if (axis == 1 && a.wt3[side]) {
+      // c3 slip wall: the tangential part of the momentum
+      const long nface = (long)g.nx * g.nz;
+      const double *wc = a.wc3[side], *wt = a.wt3[side];
+      double vk = 0.0, vp = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        vk = vk + u[1 + c] * ldg(wt, (idx_t)(c * nface + r));
+        vp = vp + u[1 + c] * ldg(wt, (idx_t)((3 + c) * nface + r));
+      }
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        u[1 + c] = vk * ldg(wc, (idx_t)((2 + c) * nface + r)) + vp * ldg(wc, (idx_t)((5 + c) * nface + r));
+        chg[1 + c] = true;
+      }
+    } else {
+      u[1 + axis] = 0.0;
+      chg[1 + axis] = true;
+    }
+  };
+  if (es >= 0) apply(slip_e, 0, tw_e);
+  apply(slip, jsec ? 1 : 0, tw);
+#pragma unroll
+  for (int v = 1; v < 5; v++) {
+    if (!chg[v]) continue;
+    if (a.dst.v[0]) stg(a.dst.v[v], p1, u[v]);
+    else stg(a.dst32[v], p1, (float)(u[v] - a.ref[v]));
+  }
+}
+
 __global__ __launch_bounds__(kWallThreads) void k_wall(WallArgs a) {
   const Geom &g = a.g;
   const long nface = (long)g.nx * g.nz, nfi = (long)g.ny * g.nz;
@@ -209,6 +306,10 @@ __global__ __launch_bounds__(kWallThreads) void k_wall(WallArgs a) {
   const idx_t p1 = lin(g, i, j, k);
   const idx_t p2 = side ? p1 - st : p1 + st, p3 = side ? p1 - 2 * st : p1 + 2 * st;
   const idx_t sidx = (idx_t)(2 * t);  // stash slot: the sections follow each other
+  if ((jsec ? g.jadv : g.iadv) & (1 << side)) {
+    wall_advanced(a, jsec, side, r, i, k, p1, p2, p3, sidx);
+    return;
+  }
   double rho, rhoe;
   if (a.src.v[0] || a.src32[0]) {
     auto prim = [&](idx_t q) {
@@ -422,6 +523,24 @@ __global__ __launch_bounds__(kWallThreads) void k_tau_wallrow(WallViscArgs a) {
   }
   const idx_t p = lin(g, i, j, k);
   const Pw o = prim_at(a.u0, p, a.gas.gam1, a.gas.rg);
+  if (a.want_sensor) stg(a.prs, p, o.p);
+  if (a.sr[0]) {
+    const double c = sqrt(a.gas.gam * a.gas.rg * o.T);
+    if (a.geo == kGeoC3) {
+      // the contravariant velocity and |grad xi| of the Euler metrics, as k_visc
+      const double ij = ldg(a.met.m[9], p);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const double mx = ldg(a.met.m[3 * d], p), my = ldg(a.met.m[3 * d + 1], p), mz = ldg(a.met.m[3 * d + 2], p);
+        const double vc = (o.u * mx + o.v * my + o.w * mz) * ij;
+        stg(a.sr[d], p, fabs(vc) + c * (sqrt(mx * mx + my * my + mz * mz) * fabs(ij)));
+      }
+    } else {
+      stg(a.sr[0], p, fabs(o.u) + c);
+      stg(a.sr[1], p, fabs(o.v) + c);
+      stg(a.sr[2], p, fabs(o.w) + c);
+    }
+  }
   const double mu = sutherland(a.gas, o.T);
   double tv[6];
 #pragma unroll

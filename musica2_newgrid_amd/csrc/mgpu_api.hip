@@ -123,9 +123,18 @@ DissipKind dissip_kind(const mgpu_case &c, bool wall) {
   return kDissipShock;
 }
 
+// Walls, with or without the slip flag.
+bool is_wall(int kind) { return kind == MGPU_FACE_WALL || kind == MGPU_FACE_SLIP; }
 // Boundary faces: no neighbour; the operators take their closures there.
-bool is_bound(int kind) {
-  return kind == MGPU_FACE_WALL || kind == MGPU_FACE_CHARAC || kind == MGPU_FACE_INLET;
+bool is_bound(int kind) { return is_wall(kind) || kind == MGPU_FACE_CHARAC || kind == MGPU_FACE_INLET; }
+// A slip wall: the bindings of assign_procedures.f90:415-470 -- with is_wall2,
+// every wall of an inviscid case and the walls with the slip flag of an
+// adiabatic one; an isothermal case binds no slip routine, and without
+// is_wall2 the flag selects nothing.
+bool is_slip(const mgpu_case &c, int kind) {
+  if (!c.is_wall2 || !is_wall(kind)) return false;
+  if (c.iorder_visc == 0) return true;
+  return c.is_adiab && kind == MGPU_FACE_SLIP;
 }
 bool is_charac(int kind) { return kind == MGPU_FACE_CHARAC || kind == MGPU_FACE_INLET; }
 
@@ -155,10 +164,14 @@ std::string check_charac(const mgpu_case &c) {
     }
     if (!any) continue;
     for (int f : {0, 1, 4, 5})
-      if (A.face_kind[f] == MGPU_FACE_WALL) {
+      if (is_wall(A.face_kind[f])) {
         snprintf(buf, sizeof buf, "block %d: walls with characteristic faces are supported on the j-faces only", b);
         return buf;
       }
+    if (c.is_wall2) {
+      snprintf(buf, sizeof buf, "block %d: characteristic faces with advanced walls (is_wall2) are not supported", b);
+      return buf;
+    }
     if (c.curvilinear) {
       snprintf(buf, sizeof buf, "block %d: characteristic faces need a Cartesian case", b);
       return buf;
@@ -191,7 +204,7 @@ std::string check_c3_walls(const mgpu_case &c) {
   if (c.curvilinear != MGPU_GEOM_C3) return std::string();
   for (int b = 0; b < c.nblocks; b++)
     for (int f : {0, 1, 4, 5})
-      if (c.blocks[b].face_kind[f] == MGPU_FACE_WALL) {
+      if (is_wall(c.blocks[b].face_kind[f])) {
         snprintf(buf, sizeof buf, "block %d face %d: 3-D curvilinear blocks take walls on their j-faces only", b, f);
         return buf;
       }
@@ -260,7 +273,7 @@ constexpr int kVarMap[3][5] = {{0, 1, 2, 3, 4}, {0, 2, 1, 3, 4}, {0, 1, 3, 2, 4}
 // i and j); -1: walls on the j- and k-faces, or on all three axes.
 int wall_perm(const mgpu_block &b) {
   bool w[3];
-  for (int a = 0; a < 3; a++) w[a] = b.face_kind[2 * a] == MGPU_FACE_WALL || b.face_kind[2 * a + 1] == MGPU_FACE_WALL;
+  for (int a = 0; a < 3; a++) w[a] = is_wall(b.face_kind[2 * a]) || is_wall(b.face_kind[2 * a + 1]);
   if (w[1] && w[2]) return -1;
   if (w[0] && !w[1] && !w[2]) return 1;
   if (w[2]) return 2;
@@ -398,9 +411,9 @@ std::string check_topology(const mgpu_case &c) {
         return buf;
       }
       if (is_charac(kind)) continue;  // check_charac
-      if (kind == MGPU_FACE_WALL) {
+      if (is_wall(kind)) {
         // the stored table: walls on the j-faces, or on the i- and j-faces
-        if (f > 3 || (f < 2 && !(A.face_kind[2] == MGPU_FACE_WALL || A.face_kind[3] == MGPU_FACE_WALL))) {
+        if (f > 3 || (f < 2 && !(is_wall(A.face_kind[2]) || is_wall(A.face_kind[3])))) {
           snprintf(buf, sizeof buf, "block %d face %d: walls are supported on the j-faces (2, 3), or on the "
                    "i- and j-faces together", b, f);
           return buf;
@@ -509,7 +522,7 @@ std::string check_wall_rows(const mgpu_case &c) {
   char buf[320];
   // wall bits of a block along axis ax (0: i, 1: j)
   auto wbits = [](const mgpu_block &X, int ax) {
-    return (X.face_kind[2 * ax] == MGPU_FACE_WALL ? 1 : 0) | (X.face_kind[2 * ax + 1] == MGPU_FACE_WALL ? 2 : 0);
+    return (is_wall(X.face_kind[2 * ax]) ? 1 : 0) | (is_wall(X.face_kind[2 * ax + 1]) ? 2 : 0);
   };
   for (int b = 0; b < c.nblocks; b++) {
     const mgpu_block &A = c.blocks[b];
@@ -811,7 +824,8 @@ void free_block(BlockData &bd) {
   if (bd.sens) (void)hipFree(bd.sens);
   if (bd.wstash) (void)hipFree(bd.wstash);
   bd.prs = bd.sens = bd.wstash = nullptr;
-  for (double **q : {&bd.kimax, &bd.kjmax, &bd.cst, &bd.uin, &bd.ppre, &bd.kf, &bd.wc3[0], &bd.wc3[1]}) {
+  for (double **q : {&bd.kimax, &bd.kjmax, &bd.cst, &bd.uin, &bd.ppre, &bd.kf, &bd.wc3[0], &bd.wc3[1], &bd.wt3[0],
+                     &bd.wt3[1]}) {
     if (*q) (void)hipFree(*q);
     *q = nullptr;
   }
@@ -1082,6 +1096,10 @@ int enqueue_walls(mgpu_handle *h, int src, int dst) {
     wa.adiabatic = bd.p.is_adiab;
     wa.wc3[0] = bd.wc3[0];
     wa.wc3[1] = bd.wc3[1];
+    wa.jslip = bd.jslip;
+    wa.islip = bd.islip;
+    wa.wt3[0] = bd.wt3[0];
+    wa.wt3[1] = bd.wt3[1];
     HIPCK(h, launch_wall(wa, h->stream));
   }
   return 0;
@@ -1098,6 +1116,7 @@ WallViscArgs wall_visc_args(const mgpu_handle *h, const BlockData &bd, int s0) {
   a.adiabatic = bd.p.is_adiab;
   a.prs = bd.prs;
   a.sens = bd.sens;
+  for (int q = 0; q < 3; q++) a.sr[q] = bd.sr[q];
   a.want_sensor = bd.prs != nullptr;
   a.ducros = h->c.is_ducros;
   a.geo = h->c.curvilinear;
@@ -1353,6 +1372,8 @@ std::string check_flags(const mgpu_case &c) {
     snprintf(buf, sizeof buf, "artificial viscosity: Cartesian or 2.5-D cases (artvisc_o9 has no c3 form)");
   else if (c.precision != MGPU_PREC_FP64 && c.precision != MGPU_PREC_MIXED)
     snprintf(buf, sizeof buf, "precision=%d", c.precision);
+  else if (c.is_wall2 != 0 && c.is_wall2 != 1)
+    snprintf(buf, sizeof buf, "is_wall2=%d: 0 or 1", c.is_wall2);
   else if (c.precision == MGPU_PREC_MIXED && c.nrk < 2)
     snprintf(buf, sizeof buf, "mixed precision needs nrk >= 2");
   else if ((c.launch_flags & MGPU_LAUNCH_STAGE_CLASSIC) && (c.launch_flags & MGPU_LAUNCH_STAGE_PIPELINED))
@@ -1422,8 +1443,12 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
   bd.g.sx = bl.nx + 2 * G;
   bd.g.sy = (idx_t)bd.g.sx * (idx_t)(bl.ny + 2 * G);
   bd.g.n = (long)bd.g.sy * (bl.nz + 2 * G);
-  bd.g.jwall = (bl.face_kind[2] == MGPU_FACE_WALL ? 1 : 0) | (bl.face_kind[3] == MGPU_FACE_WALL ? 2 : 0);
-  bd.g.iwall = (bl.face_kind[0] == MGPU_FACE_WALL ? 1 : 0) | (bl.face_kind[1] == MGPU_FACE_WALL ? 2 : 0);
+  bd.g.jwall = (is_wall(bl.face_kind[2]) ? 1 : 0) | (is_wall(bl.face_kind[3]) ? 2 : 0);
+  bd.g.iwall = (is_wall(bl.face_kind[0]) ? 1 : 0) | (is_wall(bl.face_kind[1]) ? 2 : 0);
+  bd.g.jadv = p.is_wall2 ? bd.g.jwall : 0;
+  bd.g.iadv = p.is_wall2 ? bd.g.iwall : 0;
+  bd.jslip = (is_slip(p, bl.face_kind[2]) ? 1 : 0) | (is_slip(p, bl.face_kind[3]) ? 2 : 0);
+  bd.islip = (is_slip(p, bl.face_kind[0]) ? 1 : 0) | (is_slip(p, bl.face_kind[1]) ? 2 : 0);
   bd.g.ibnd = (is_bound(bl.face_kind[0]) ? 1 : 0) | (is_bound(bl.face_kind[1]) ? 2 : 0);
   bd.g.jbnd = (is_bound(bl.face_kind[2]) ? 1 : 0) | (is_bound(bl.face_kind[3]) ? 2 : 0);
   bd.bytes = block_bytes(c, bl.nx, bl.ny, bl.nz);
@@ -1454,7 +1479,10 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
     rc = dalloc(h, &bd.wstash, (size_t)4 * bl.nx * bl.nz + (size_t)4 * bl.ny * bl.nz);
   if (c.curvilinear == MGPU_GEOM_C3)
     for (int s = 0; s < 2 && !rc; s++)
-      if ((bd.g.jwall >> s) & 1) rc = dalloc(h, &bd.wc3[s], (size_t)8 * bl.nx * bl.nz);
+      if ((bd.g.jwall >> s) & 1) {
+        rc = dalloc(h, &bd.wc3[s], (size_t)8 * bl.nx * bl.nz);
+        if (!rc && ((bd.jslip >> s) & 1)) rc = dalloc(h, &bd.wt3[s], (size_t)6 * bl.nx * bl.nz);
+      }
   {
     const size_t fi = (size_t)bl.ny * bl.nz, fj = (size_t)bl.nx * bl.nz;
     if (bl.face_kind[1] == MGPU_FACE_CHARAC) {
@@ -1492,6 +1520,7 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
     h->stage_family = 1;  // the classic stage kernel closes i-faces
   }
   if (bd.wc3[0] || bd.wc3[1]) h->stage_family = 1;  // and c3 walls
+  if (bd.g.jadv || bd.g.iadv) h->stage_family = 1;  // and advanced wall rows
   h->exchange_ready = false;
   h->tables_ready = false;
   return 0;
@@ -2039,6 +2068,16 @@ int mgpu_set_wall(mgpu_handle *h, int block, const double *cextrp2, const double
   return 0;
 }
 
+// A c3 block's walls are ready once every wall face has its coefficients and
+// every slip face its dual tangents.
+bool c3_walls_ready(const BlockData &bd) {
+  for (int q = 0; q < 2; q++) {
+    if (!((bd.g.jwall >> q) & 1)) continue;
+    if (!bd.wc3_set[q] || (((bd.jslip >> q) & 1) && !bd.wt3_set[q])) return false;
+  }
+  return true;
+}
+
 int mgpu_set_wall_c3(mgpu_handle *h, int block, int face, const double *const *coef) {
   if (!h) return MGPU_EINVAL;
   if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
@@ -2060,11 +2099,34 @@ int mgpu_set_wall_c3(mgpu_handle *h, int block, int face, const double *const *c
   }
   HIPCK(h, hipStreamSynchronize(h->stream));
   HIPCK(h, hipMemcpy(bd.wc3[s], tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
-  // a block is ready once every wall face has its coefficients
-  bool all = true;
-  for (int q = 0; q < 2; q++) all &= !((bd.g.jwall >> q) & 1) || q == s || bd.wc3_set[q];
   bd.wc3_set[s] = true;
-  bd.have_wall = all;
+  bd.have_wall = c3_walls_ready(bd);
+  return 0;
+}
+
+int mgpu_set_wall_c3_slip(mgpu_handle *h, int block, int face, const double *const *t) {
+  if (!h) return MGPU_EINVAL;
+  if (block < 0 || block >= (int)h->b.size()) return fail(h, MGPU_EINVAL, "block %d out of range", block);
+  if (!t) return fail(h, MGPU_EINVAL, "null argument");
+  BlockData &bd = h->b[block];
+  if (h->c.curvilinear != MGPU_GEOM_C3) return fail(h, MGPU_ESTATE, "case is not 3-D curvilinear");
+  if (face != 2 && face != 3) return fail(h, MGPU_EINVAL, "face %d: curvilinear walls are j-faces (2, 3)", face);
+  const int s = face - 2;
+  if (!bd.wt3[s]) return fail(h, MGPU_ESTATE, "block %d face %d is no slip wall", block, face);
+  const size_t nf = (size_t)bd.g.nx * bd.g.nz;
+  std::vector<double> tmp(6 * nf);
+  for (int q = 0; q < 6; q++) {
+    if (!t[q]) return fail(h, MGPU_EINVAL, "tangent array %d is null", q);
+    for (size_t e = 0; e < nf; e++) {
+      if (!std::isfinite(t[q][e]))
+        return fail(h, MGPU_EINVAL, "block %d face %d: tangent %d not finite at %zu", block, face, q, e);
+      tmp[q * nf + e] = t[q][e];
+    }
+  }
+  HIPCK(h, hipStreamSynchronize(h->stream));
+  HIPCK(h, hipMemcpy(bd.wt3[s], tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+  bd.wt3_set[s] = true;
+  bd.have_wall = c3_walls_ready(bd);
   return 0;
 }
 

@@ -35,6 +35,11 @@ struct BlockData {
   // c3 walls: per j-face, the 8 coefficient slabs of mgpu_set_wall_c3 (8*nx*nz doubles)
   double *wc3[2] = {nullptr, nullptr};
   bool wc3_set[2] = {false, false};
+  // slip walls (bits as g.jwall, g.iwall) and the dual tangents of a c3 slip
+  // face (6 slabs of nx*nz)
+  int jslip = 0, islip = 0;
+  double *wt3[2] = {nullptr, nullptr};
+  bool wt3_set[2] = {false, false};
   // characteristic outflow (imax, jmax) and supersonic inlet (imin): face
   // slabs (see slab_size) of what the reference keeps between the calls of
   // its boundary routines -- the increments of the characteristic rows, the

@@ -26,6 +26,9 @@ constexpr double D10_1 = -126.0, D10_2 = 84.0, D10_3 = -36.0, D10_4 = 9.0, D10_5
 constexpr double A3_1 = 0.5;
 constexpr double A7_1 = 3.0 / 4.0, A7_2 = -3.0 / 20.0, A7_3 = 1.0 / 60.0;
 constexpr double A9_1 = 4.0 / 5.0, A9_2 = -1.0 / 5.0, A9_3 = 4.0 / 105.0, A9_4 = -1.0 / 280.0;
+// 3-pt one-sided first derivative of an advanced wall row (is_wall2),
+// a20 = -a02 at a max face (mod_coeff_deriv.f90:113-117).
+constexpr double A02_1 = -3.0 / 2.0, A02_2 = 2.0, A02_3 = -1.0 / 2.0;
 // 5-pt one-sided first derivatives on the wall row (a04) and the next (a13);
 // a40 = -a04 and a31 = -a13 mirror them at a max face (mod_coeff_deriv.f90:114-124).
 constexpr double A04_1 = -25.0 / 12.0, A04_2 = 4.0, A04_3 = -3.0, A04_4 = 4.0 / 3.0, A04_5 = -1.0 / 4.0;
@@ -54,6 +57,10 @@ struct Geom {
   // supersonic inlet), at which the operators take their closures: bit 0 on
   // the min face, bit 1 on the max face of axis i (ibnd) and j (jbnd)
   int ibnd, jbnd;
+  // advanced walls (is_wall2): the wall rows of these faces take an Euler
+  // increment with the one-sided a02 / a20 derivative across the wall; bits
+  // as jwall (jadv) and iwall (iadv)
+  int iadv, jadv;
   long n;     // total padded points
 };
 
@@ -311,6 +318,10 @@ struct FilterArgs {
 // to `stash`: two values per wall point, jmin, jmax, then imin, imax).  On an
 // edge the j-face's state wins (bc_wall applies imin..jmax in order,
 // mod_bc_apply.f90:237-261, each face from the same primitives).
+// Advanced walls (g.jadv / g.iadv) keep the wall row of `dst` and change it
+// in place: slip faces remove its normal momentum, the others zero it and set
+// rhoE = rho cv T_wall, T_wall extrapolated from `src` (or the stash, which
+// then holds T_wall alone); on an edge the j-face acts last.
 struct WallArgs {
   Geom g;
   Gas gas;
@@ -332,6 +343,10 @@ struct WallArgs {
   // tangents x_ksi/|x_ksi| and x_phi/|x_phi| (x, y, z components); null on
   // Cartesian blocks, whose extrapolation uses c2, c3
   const double *wc3[2];
+  // slip walls among the advanced ones: bits as g.jwall (jslip), g.iwall
+  // (islip); a c3 slip face's dual tangents t_ksi, t_phi (6 slabs)
+  int jslip, islip;
+  const double *wt3[2];
 };
 
 // The viscous rows next to the bounded faces of one block (rows 1,2 and
@@ -351,6 +366,7 @@ struct WallViscArgs {
   int ducros;          // Ducros sensor (else 1)
   int geo;             // kGeoCart or kGeoC3 (viscous metrics mapping the index-space derivatives)
   const double *wc3[2];  // c3 walls: the coefficient slabs of WallArgs::wc3 (the unit tangents)
+  double *sr[3];       // artificial viscosities: spectral radii of U^n (null without)
 };
 
 // Characteristic (LODI) boundaries and the supersonic inlet of one block:

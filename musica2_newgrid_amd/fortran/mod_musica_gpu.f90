@@ -14,7 +14,7 @@ module mod_musica_gpu
   implicit none
 
   integer(c_int32_t), parameter :: MGPU_FACE_PERIODIC = 0, MGPU_FACE_INTERFACE = 1, MGPU_FACE_WALL = 2, &
-                                   MGPU_FACE_CHARAC = 3, MGPU_FACE_INLET = 4
+                                   MGPU_FACE_CHARAC = 3, MGPU_FACE_INLET = 4, MGPU_FACE_SLIP = 5
   integer(c_int32_t), parameter :: MGPU_FILTER_NONE = 0, MGPU_FILTER_SF = 1, MGPU_FILTER_SHOCK = 2
   integer(c_int32_t), parameter :: MGPU_FILTER_ARTVISC = 3, MGPU_FILTER_ARTVISC_SHOCK = 4
   integer(c_int32_t), parameter :: MGPU_PREC_FP64 = 0, MGPU_PREC_MIXED = 1
@@ -35,6 +35,7 @@ module mod_musica_gpu
      integer(c_int32_t) :: is_adiab = 1, is_sw_edoh = 0
      real(c_double) :: t_wall = 0.0_c_double
      real(c_double) :: p_ref = 0.0_c_double
+     integer(c_int32_t) :: is_wall2 = 0, reserved0 = 0
   end type mgpu_case_t
 
   ! Hot-path fields of param.ini (read_param, src/File_IO/read_param.f90:88-205)
@@ -89,6 +90,14 @@ module mod_musica_gpu
        integer(c_int), value :: block, face
        type(c_ptr), intent(in) :: coef(8)
      end function mgpu_set_wall_c3
+
+     ! t: c_loc of the 6 dual-tangent arrays (nx*nz each) of a c3 slip j-wall
+     integer(c_int) function mgpu_set_wall_c3_slip(h, block, face, t) bind(C, name='mgpu_set_wall_c3_slip')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int), value :: block, face
+       type(c_ptr), intent(in) :: t(6)
+     end function mgpu_set_wall_c3_slip
 
      integer(c_int) function mgpu_put_state(h, block, rho, rhou, rhov, rhow, rhoe) bind(C, name='mgpu_put_state')
        import :: c_int, c_ptr, c_double

@@ -21,8 +21,9 @@ def make_case_struct(case, precision: int = _lib.PREC_FP64, topology=None, launc
     multiblock.py); None means one block with periodic faces."""
     if topology is None:
         faces = tuple(getattr(case, "faces", (0,) * 6))
-        # the case's face codes: 0 periodic, 2 wall, 3 characteristic outflow, 4 supersonic inlet
-        code = {2: _lib.FACE_WALL, 3: _lib.FACE_CHARAC, 4: _lib.FACE_INLET}
+        # the case's face codes: 0 periodic, 2 wall, 3 characteristic outflow, 4 supersonic inlet,
+        # 5 wall with the slip flag
+        code = {2: _lib.FACE_WALL, 3: _lib.FACE_CHARAC, 4: _lib.FACE_INLET, 5: _lib.FACE_SLIP}
         kinds = [code.get(k, _lib.FACE_PERIODIC) for k in faces]
         topology = [{"nx": case.nx, "ny": case.ny, "nz": case.nz, "kind": kinds,
                      "nbr_block": [0] * 6, "nbr_face": [1, 0, 3, 2, 5, 4], "orient": [0] * 6}]
@@ -60,7 +61,19 @@ def make_case_struct(case, precision: int = _lib.PREC_FP64, topology=None, launc
     cs.is_sw_edoh = int(getattr(case, "sw_edoh", False))
     cs.t_wall = float(getattr(case, "t_wall", 0.0))
     cs.p_ref = float(getattr(case, "p_ref", 0.0))
+    cs.is_wall2 = int(getattr(case, "wall2", False))
     return cs, blocks
+
+
+def slip_face(case, kind: int) -> bool:
+    """A face of kind `kind` (MGPU_FACE_*) is a slip wall: with is_wall2, every
+    wall of an inviscid case and the flagged walls of an adiabatic one
+    (assign_procedures.f90:415-470)."""
+    if not getattr(case, "wall2", False) or kind not in (_lib.FACE_WALL, _lib.FACE_SLIP):
+        return False
+    if int(case.iorder_visc) == 0:
+        return True
+    return bool(getattr(case, "adiabatic", True)) and kind == _lib.FACE_SLIP
 
 
 class GpuSolver:
@@ -93,8 +106,12 @@ class GpuSolver:
                 # curvilinear walls: the per-point coefficients of each wall face
                 for f, slabs in bc.wall_c3.items():
                     arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in slabs]
-                    tab = (_lib._dp * 8)(*[dptr(a) for a in arrs])
+                    tab = (_lib._dp * 8)(*[dptr(a) for a in arrs[:8]])
                     check(L.mgpu_set_wall_c3(self.h, b, int(f), tab), self.h, "mgpu_set_wall_c3")
+                    if slip_face(case, self._blocks[b].face_kind[int(f)]):
+                        # a slip face: the dual tangents too
+                        tab = (_lib._dp * 6)(*[dptr(a) for a in arrs[8:14]])
+                        check(L.mgpu_set_wall_c3_slip(self.h, b, int(f), tab), self.h, "mgpu_set_wall_c3_slip")
             elif self._has_wall(b):
                 c2 = np.ascontiguousarray(getattr(bc, "cextrp2", (0.0,) * 6), dtype=np.float64)
                 c3 = np.ascontiguousarray(getattr(bc, "cextrp3", (0.0,) * 6), dtype=np.float64)
@@ -102,7 +119,7 @@ class GpuSolver:
             self.put_state(bc.U, b)
 
     def _has_wall(self, b: int) -> bool:
-        return any(self._blocks[b].face_kind[f] == _lib.FACE_WALL for f in range(6))
+        return any(self._blocks[b].face_kind[f] in (_lib.FACE_WALL, _lib.FACE_SLIP) for f in range(6))
 
     @property
     def nblocks(self) -> int:

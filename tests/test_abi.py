@@ -30,8 +30,9 @@ def test_struct_layout_matches_header(tmp_path):
     import subprocess
     from musica2_newgrid_amd._lib import MgpuBlock, MgpuCase
     assert ctypes.sizeof(MgpuBlock) == 4 * (3 + 24)
-    # 8 int32, one pointer, 8 crk, deltat and 9 gas/filter doubles, 2 int32, t_wall and p_ref
-    assert ctypes.sizeof(MgpuCase) == 32 + 8 + 8 * (8 + 1 + 9) + 8 + 8 + 8
+    # 8 int32, one pointer, 8 crk, deltat and 9 gas/filter doubles, 2 int32, t_wall and p_ref,
+    # is_wall2 and its reserved twin
+    assert ctypes.sizeof(MgpuCase) == 32 + 8 + 8 * (8 + 1 + 9) + 8 + 8 + 8 + 8
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{ROOT}/include/musica_gpu.h"', "int main(void) {"]
     for st, cname in ((MgpuCase, "mgpu_case"), (MgpuBlock, "mgpu_block")):
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

@@ -245,3 +245,26 @@ def test_curvilinear_walls_are_checked():
                    _lib.FACE_PERIODIC]
     rc, msg = _create_case(c, [blk])
     assert rc == _lib.EINVAL and "3-D curvilinear" in msg, msg
+
+
+def test_advanced_wall_flags_are_checked():
+    """is_wall2 is 0 or 1, and advanced walls do not meet characteristic or
+    inlet faces; refused before any device work."""
+    c, b = _shock_bl_table()
+    c.wall2 = True
+    rc, msg = _create_case(c, b)
+    assert rc == _lib.EINVAL and "is_wall2" in msg, msg
+    c = cases.channel(24, 20, 8, slip=True)
+    cs, keep = make_case_struct(c)
+    cs.is_wall2 = 2
+    h = C.c_void_p()
+    rc = _lib.lib().mgpu_create(C.byref(cs), C.byref(h))
+    msg = _lib.lib().mgpu_last_error(None).decode()
+    assert rc == _lib.EINVAL and "is_wall2=2" in msg, msg
+    # the slip flag is a face kind of its own, accepted where walls are
+    blk = {"nx": c.nx, "ny": c.ny, "nz": c.nz,
+           "kind": [_lib.FACE_PERIODIC, _lib.FACE_PERIODIC, _lib.FACE_SLIP, _lib.FACE_SLIP, _lib.FACE_SLIP,
+                    _lib.FACE_SLIP],
+           "nbr_block": [0] * 6, "nbr_face": [1, 0, 3, 2, 5, 4], "orient": [0] * 6}
+    rc, msg = _create_case(c, [blk])
+    assert rc == _lib.EINVAL and "walls" in msg, msg
