@@ -1,9 +1,12 @@
 """Benchmark of the MI355X time-advance hot path.
 
-Workload (default): the single curvilinear block 512x256x128, FP64, with
+Workload (default): the single curvilinear block 512x256x128, FP64, as the
+headline configuration states it: x uniform and periodic, slip walls on both
+y-faces (type_wall_bc = 2 with the slip flag) with tanh clustering (ratio
+1.05), a wavy lower wall y' = y + 0.05 sin(2 pi x/Lx)(1 - y/Ly), z periodic,
 3-D GCL metrics, an isentropic vortex (strength 5, radius 0.5) advected at
-Mach 0.5, 4-stage low-storage RK, 5-pt viscous terms in the last stage,
-shock-capturing filter with Ducros sensor (quiescent), periodic faces.
+Mach 0.5 in a uniform free stream, 4-stage low-storage RK, 5-pt viscous terms
+in the last stage, shock-capturing filter with Ducros sensor (quiescent).
 
 One "step" = one full RK time step (4 stages + filter) of that block, all
 inputs resident in HBM.  Metric: Mcell-RK-stages/s = cells * nrk * steps / t.
@@ -28,11 +31,12 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 # 64x8 tile, k-chunk 128), averaged over the 4 stages (3 Euler launches + the
 # viscous last stage): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch,
 # FETCH_SIZE scaled by 2.000 and WRITE_SIZE by 1.000 (their readings for a
-# 1 GiB device copy).  Summaries: profiles/step_kernels_c3_channel_512x256x128_fp64.md
-# (the walled channel), profiles/step_kernels_c3_512x256x128_fp64.md (the
-# periodic block).
+# 1 GiB device copy).  Summaries: profiles/step_kernels_c3_slip_channel_512x256x128_fp64.md
+# (the channel with slip walls), profiles/step_kernels_c3_channel_512x256x128_fp64.md
+# (no-slip walls), profiles/step_kernels_c3_512x256x128_fp64.md (the periodic block).
 STAGE_TRAFFIC = {
-    ("channel", 512, 256, 128): ((3 * 4.597e9 + 5.510e9) / 4, "profiles/step_kernels_c3_channel_512x256x128_fp64.md"),
+    ("channel_noslip", 512, 256, 128): ((3 * 4.597e9 + 5.510e9) / 4,
+                                        "profiles/step_kernels_c3_channel_512x256x128_fp64.md"),
     ("periodic", 512, 256, 128): ((3 * 4.589e9 + 5.338e9) / 4, "profiles/step_kernels_c3_512x256x128_fp64.md"),
 }
 
@@ -374,9 +378,9 @@ def main() -> int:
     ap.add_argument("--no-c2", action="store_true", help="skip the 2.5-D curvilinear leg")
     ap.add_argument("--no-shock-bl", action="store_true", help="skip the shock/boundary-layer leg")
     ap.add_argument("--no-periodic", action="store_true", help="skip the fully periodic curvilinear leg")
-    ap.add_argument("--headline", choices=["channel", "periodic"], default="channel",
-                    help="the curvilinear headline block: the walled channel as stated, or the fully periodic "
-                         "wavy block")
+    ap.add_argument("--headline", choices=["channel", "channel_noslip", "periodic"], default="channel",
+                    help="the curvilinear headline block: the channel with slip walls as stated, the same "
+                         "channel with no-slip walls (is_wall2 = F), or the fully periodic wavy block")
     ap.add_argument("--shock-bl-size", type=int, nargs=3, default=(1024, 512, 8), metavar=("NX", "NY", "NZ"))
     ap.add_argument("--shock-bl-error-steps", type=int, default=2000,
                     help="steps after which the shock/boundary-layer mixed density is compared with FP64")
@@ -401,6 +405,15 @@ def main() -> int:
     diss = {"shock": (2, "shock filter + Ducros"),
             "artvisc_shock": (4, "artificial viscosity with shock capturing + Ducros")}[args.dissipation]
     if args.headline == "channel":
+        case = cases.vortex_channel(args.nx, args.ny, args.nz, filter=diss[0], slip=True)
+        inf = case.info
+        workload = (f"curvilinear channel {args.nx}x{args.ny}x{args.nz}: x uniform and periodic, slip walls "
+                    f"(type_wall_bc 2, flag s, adiabatic) on both y-faces with tanh clustering (largest "
+                    f"neighbouring-spacing ratio {inf['ratio']}), wavy lower wall y' = y + {inf['amp']} "
+                    f"sin(2 pi x/Lx)(1 - y/Ly), z uniform and periodic, 3-D GCL metrics; isentropic vortex "
+                    f"(strength 5, radius 0.5) at Mach 0.5 in a uniform free stream; viscous, {diss[1]} "
+                    f"(quiescent), RK4, FP64, steps replayed from the captured HIP graph")
+    elif args.headline == "channel_noslip":
         case = cases.vortex_channel(args.nx, args.ny, args.nz, filter=diss[0])
         inf = case.info
         workload = (f"curvilinear channel {args.nx}x{args.ny}x{args.nz}: x uniform and periodic, no-slip adiabatic "
@@ -469,7 +482,7 @@ def main() -> int:
             "after_steps": mx["steps_done"],
             "kernel_ms": {k: {"launches": n, "ms": round(ms, 3)} for k, (n, ms) in mx["times"].items()},
         }
-    if args.headline == "channel" and not args.no_periodic:
+    if args.headline != "periodic" and not args.no_periodic:
         print("bench: periodic curvilinear leg", file=sys.stderr, flush=True)
         out["periodic_c3"] = periodic_leg(args.nx, args.ny, args.nz, args.steps, args.warmup, not args.no_tune, diss[0])
     if not args.no_c2:

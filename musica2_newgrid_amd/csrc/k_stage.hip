@@ -64,6 +64,15 @@ __device__ __forceinline__ void wall_coeffs(int hw, double c[5]) {
   c[4] = hw == 5 ? A11_5 : 0.0;
 }
 
+// Value a pencil cell holds for its flux f by code: 0 the flux, 1 zero (a
+// cell beyond a bounded face, by a select: its flux may be NaN, from the
+// unwritten ghosts of a stage set), 2 three quarters of it (one row beyond an
+// advanced face, where the cell's thread evaluated the wall row's flux).
+template <class R>
+__device__ __forceinline__ R pencil_value(unsigned code, R f) {
+  return code == 1u ? R(0) : (code == 2u ? R(-A02_1 / A02_2) * f : f);
+}
+
 // Raw inputs of one flux evaluation: the 5 conserved variables and the 3
 // metric terms of the flux direction (c3).
 struct Raw {
@@ -248,7 +257,18 @@ __device__ __forceinline__ void flux_halo(const Prim &q, int dir, const double m
   else flux_r<CURV, MIX>(q, 1, m, rf, F);
 }
 
-template <int TX, int TY, int GEO, bool VISC, int MODE, bool WALL>
+// WALL: 0 no bounded face; 1 the closures of bounded faces; 2 also advanced
+// wall rows (is_wall2), whose Euler increment takes the one-sided a02 / a20
+// derivative across the wall (smod_flux_euler_11pts.f90:42-54, 254-265,
+// 286-297, 584-595): -3/2 f0 + 2 f1 - 1/2 f2 at a min face.  The wall row
+// evaluates it with the centred form 2 (f1 - f-1) - 1/2 (f2 - f-2) of the
+// closure rows: the pencil holds 3/4 of the wall row's flux one row beyond
+// an advanced face and 0 further out, cells no other row reads.  The thread
+// that fills that cell loads the wall row's point instead of its own and
+// scales the flux by 3/4, so the k-march keeps the loop of the other wall
+// variants.  A variant of its own all the same, so that they keep their
+// registers.
+template <int TX, int TY, int GEO, bool VISC, int MODE, int WALL>
 __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__restrict__ tab, int nb, int ntiles) {
   // CURV: contravariant i/j fluxes (c3 or 2.5-D); C3: metrics at every point
   // (and a contravariant k flux); C2: 2-D metrics per column, Cartesian k
@@ -259,6 +279,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
   // bounded i-faces (characteristic outflow, inlet): Cartesian blocks only;
   // c3 blocks take walls on their j-faces
   constexpr bool IBND = WALL && !CURV;
+  constexpr bool ADV = WALL == 2;
   constexpr bool MIX = MODE != kFP64;
   constexpr bool IN32 = MODE == kMixMiddle || MODE == kMixLast;
   constexpr bool OUT32 = MODE == kMixFirst || MODE == kMixMiddle;
@@ -327,7 +348,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
       hs_off[s] = lin(g, gi, gj, 0);
     }
   }
-  const idx_t own_off = lin(g, i, j, 0);
+  idx_t own_off = lin(g, i, j, 0);
   const idx_t sz = g.sy;
   // 2.5-D: the metric rows of the halo cells and of the own column, and
   // 1/J, constant along k: loaded once
@@ -360,14 +381,11 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
   bool hs_beyond[NS];
   bool own_beyond = false, own_beyond_x = false;
   double cw[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ci[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  // the own point's weight: nonzero on an advanced wall row (is_wall2), whose
-  // a02 / a20 derivative reads the row itself; with the pencil beyond the wall
-  // at 0 it is c0 f0 + 2 (f1 - f-1) - 1/2 (f2 - f-2), c0 = -3/2 at the min
-  // face and +3/2 at the max face (smod_flux_euler_11pts.f90:286-297, 584-595)
-  double cw0 = 0.0, ci0 = 0.0;
   int hw = 5, hwi = 5;
-  // a boundary row without an Euler increment (flux_euler skips it)
-  bool krow = false;
+  // ADV: one code per pencil cell in a single word (2 bits each: halo cell s
+  // at bit 2s, the own point's j- and i-pencil cells at bits 28 and 30), see
+  // pencil_value; cells one row beyond an advanced face take code 2
+  unsigned acode = 0u;
   if constexpr (WALL) {
 #pragma unroll
     for (int s = 0; s < NS; s++) {
@@ -378,24 +396,62 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
     own_beyond = (g.jbnd & 2) && j > g.ny;
     hw = jhalf(g, j);
     wall_coeffs(hw, cw);
-    const bool jadv = hw == 0 && (((g.jadv & 1) && j == 1) || ((g.jadv & 2) && j == g.ny));
-    if (jadv) {
-      cw[0] = A02_2;
-      cw[1] = A02_3;
-      cw0 = j == 1 ? A02_1 : -A02_1;
+    if constexpr (ADV) {
+      if (hw == 0 && (((g.jadv & 1) && j == 1) || ((g.jadv & 2) && j == g.ny))) {
+        // the wall row takes an increment: not a row without one (hw 6)
+        cw[0] = A02_2;
+        cw[1] = A02_3;
+        hw = 6;
+      }
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        const int gj = j0 - 5 + hs_r[s], gi = i0 - 5 + hs_c[s];
+        unsigned c = hs_beyond[s] ? 1u : 0u;
+        if (hs_dir[s] == 1) {
+          if ((g.jadv & 1) && gj == 0) {
+            hs_off[s] += (idx_t)g.sx;
+            c = 2u;
+          } else if ((g.jadv & 2) && gj == g.ny + 1) {
+            hs_off[s] -= (idx_t)g.sx;
+            c = 2u;
+          }
+        } else if (IBND) {
+          if ((g.iadv & 1) && gi == 0) {
+            hs_off[s] += 1;
+            c = 2u;
+          } else if ((g.iadv & 2) && gi == g.nx + 1) {
+            hs_off[s] -= 1;
+            c = 2u;
+          }
+        }
+        acode |= c << (2 * s);
+      }
+      unsigned cy = own_beyond ? 1u : 0u;
+      if ((g.jadv & 2) && j == g.ny + 1) {
+        // serves the j-pencil only (needx needs j <= ny)
+        own_off -= (idx_t)g.sx;
+        cy = 2u;
+      }
+      acode |= cy << 28;
     }
-    krow = hw == 0 && !jadv;
     if constexpr (IBND) {
       own_beyond_x = (g.ibnd & 2) && i > g.nx;
       hwi = ihalf(g, i);
       wall_coeffs(hwi, ci);
-      const bool iadv = hwi == 0 && (((g.iadv & 1) && i == 1) || ((g.iadv & 2) && i == g.nx));
-      if (iadv) {
-        ci[0] = A02_2;
-        ci[1] = A02_3;
-        ci0 = i == 1 ? A02_1 : -A02_1;
+      if constexpr (ADV) {
+        if (hwi == 0 && (((g.iadv & 1) && i == 1) || ((g.iadv & 2) && i == g.nx))) {
+          ci[0] = A02_2;
+          ci[1] = A02_3;
+          hwi = 6;
+        }
+        unsigned cx = own_beyond_x ? 1u : 0u;
+        if ((g.iadv & 2) && i == g.nx + 1) {
+          // serves the i-pencil only (needy needs i <= nx)
+          own_off -= 1;
+          cx = 2u;
+        }
+        acode |= cx << 30;
       }
-      krow = krow || (hwi == 0 && !iadv);
     }
     P.kbc[0] = a.kbc[0];
     P.kbc[1] = a.kbc[1];
@@ -487,12 +543,14 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
           flux_r<CURV, MIX>(pq, 0, rxm, rf, F);
         }
 #pragma unroll
-        for (int v = 0; v < 5; v++) sfx[v][ty][tx + 5] = (IBND && own_beyond_x) ? R(0) : F[v];
+        for (int v = 0; v < 5; v++)
+          sfx[v][ty][tx + 5] = ADV ? pencil_value(acode >> 30, F[v]) : ((IBND && own_beyond_x) ? R(0) : F[v]);
       }
       if (needy) {
         flux_r<CURV, MIX>(pq, 1, rym, rf, F);
 #pragma unroll
-        for (int v = 0; v < 5; v++) sfy[v][ty + 5][tx] = (WALL && own_beyond) ? R(0) : F[v];
+        for (int v = 0; v < 5; v++)
+          sfy[v][ty + 5][tx] = ADV ? pencil_value((acode >> 28) & 3u, F[v]) : ((WALL && own_beyond) ? R(0) : F[v]);
       }
     }
     if (hs_on[0]) {
@@ -500,10 +558,12 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
       flux_halo<CURV, MIX>(prim_raw(rh0.u, gam1), hs_dir[0], rh0.m, rf, F);
       if (hs_dir[0] == 0) {
 #pragma unroll
-        for (int v = 0; v < 5; v++) sfx[v][hs_r[0]][hs_c[0]] = (WALL && hs_beyond[0]) ? R(0) : F[v];
+        for (int v = 0; v < 5; v++)
+          sfx[v][hs_r[0]][hs_c[0]] = ADV ? pencil_value(acode & 3u, F[v]) : ((WALL && hs_beyond[0]) ? R(0) : F[v]);
       } else {
 #pragma unroll
-        for (int v = 0; v < 5; v++) sfy[v][hs_r[0]][hs_c[0]] = (WALL && hs_beyond[0]) ? R(0) : F[v];
+        for (int v = 0; v < 5; v++)
+          sfy[v][hs_r[0]][hs_c[0]] = ADV ? pencil_value(acode & 3u, F[v]) : ((WALL && hs_beyond[0]) ? R(0) : F[v]);
       }
     }
     __builtin_amdgcn_sched_barrier(0);
@@ -527,10 +587,14 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
           flux_halo<CURV, MIX>(prim_raw(rh[s].u, gam1), hs_dir[s], rh[s].m, rf, F);
           if (hs_dir[s] == 0) {
 #pragma unroll
-            for (int v = 0; v < 5; v++) sfx[v][hs_r[s]][hs_c[s]] = (WALL && hs_beyond[s]) ? R(0) : F[v];
+            for (int v = 0; v < 5; v++)
+              sfx[v][hs_r[s]][hs_c[s]] =
+                  ADV ? pencil_value((acode >> (2 * s)) & 3u, F[v]) : ((WALL && hs_beyond[s]) ? R(0) : F[v]);
           } else {
 #pragma unroll
-            for (int v = 0; v < 5; v++) sfy[v][hs_r[s]][hs_c[s]] = (WALL && hs_beyond[s]) ? R(0) : F[v];
+            for (int v = 0; v < 5; v++)
+              sfy[v][hs_r[s]][hs_c[s]] =
+                  ADV ? pencil_value((acode >> (2 * s)) & 3u, F[v]) : ((WALL && hs_beyond[s]) ? R(0) : F[v]);
           }
         }
       }
@@ -547,7 +611,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
         R dx;
         if constexpr (IBND) {
           const R *f = &sfx[v][ty][tx + 5];
-          dx = R(ci0) * f[0] + R(ci[0]) * (f[1] - f[-1]) + R(ci[1]) * (f[2] - f[-2]) + R(ci[2]) * (f[3] - f[-3]) +
+          dx = R(ci[0]) * (f[1] - f[-1]) + R(ci[1]) * (f[2] - f[-2]) + R(ci[2]) * (f[3] - f[-3]) +
                R(ci[3]) * (f[4] - f[-4]) + R(ci[4]) * (f[5] - f[-5]);
         } else {
           dx = d11c(&sfx[v][ty][tx + 5], 1);
@@ -555,9 +619,8 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
         R dy;
         if constexpr (WALL) {
           const R *f = &sfy[v][ty + 5][tx];
-          dy = R(cw0) * f[0] + R(cw[0]) * (f[TX] - f[-TX]) + R(cw[1]) * (f[2 * TX] - f[-2 * TX]) +
-               R(cw[2]) * (f[3 * TX] - f[-3 * TX]) + R(cw[3]) * (f[4 * TX] - f[-4 * TX]) +
-               R(cw[4]) * (f[5 * TX] - f[-5 * TX]);
+          dy = R(cw[0]) * (f[TX] - f[-TX]) + R(cw[1]) * (f[2 * TX] - f[-2 * TX]) + R(cw[2]) * (f[3 * TX] - f[-3 * TX]) +
+               R(cw[3]) * (f[4 * TX] - f[-4 * TX]) + R(cw[4]) * (f[5 * TX] - f[-5 * TX]);
         } else {
           dy = d11c(&sfy[v][ty + 5][tx], TX);
         }
@@ -580,10 +643,10 @@ __global__ __launch_bounds__(TX *TY, 2) void k_stage(const Batch<StageArgs> *__r
       // keeps 0 (bc_wall zeroes it) plus the viscous increment, a
       // characteristic or inlet row takes its face slab, set by the boundary
       // routines (no viscous increment there either: ndx_v = 2).  An advanced
-      // wall row keeps its increment.
+      // wall row keeps its increment (its half-width reads 6).
       bool kslab = false;
       if constexpr (WALL) {
-        if (krow) {
+        if (hwi == 0 || hw == 0) {
           const bool ia = hwi == 0;
           const double *kb = ia ? (i == 1 ? P.kbc[0] : P.kbc[1]) : (j == 1 ? P.kbc[2] : P.kbc[3]);
           if (kb) {
@@ -1170,56 +1233,62 @@ hipError_t launch_g(const Batch<StageArgs> *tab, int nb, int nt, bool visc, hipS
   dim3 blk(TX * TY), grd(nt);
   if constexpr (MODE == kFP64 || MODE == kMixLast) {
     if (visc) {
-      hipLaunchKernelGGL((k_stage<TX, TY, GEO, true, MODE, false>), grd, blk, 0, s, tab, nb, nt);
+      hipLaunchKernelGGL((k_stage<TX, TY, GEO, true, MODE, 0>), grd, blk, 0, s, tab, nb, nt);
       return hipGetLastError();
     }
   }
-  hipLaunchKernelGGL((k_stage<TX, TY, GEO, false, MODE, false>), grd, blk, 0, s, tab, nb, nt);
+  hipLaunchKernelGGL((k_stage<TX, TY, GEO, false, MODE, 0>), grd, blk, 0, s, tab, nb, nt);
   return hipGetLastError();
 }
 
-template <int TX, int TY, int MODE>
-hipError_t launch_t(const Batch<StageArgs> *tab, int nb, int nt, int geo, bool visc, bool wall, hipStream_t s) {
+template <int TX, int TY, int MODE, int W>
+hipError_t launch_w(const Batch<StageArgs> *tab, int nb, int nt, int geo, bool visc, hipStream_t s) {
   dim3 blk(TX * TY), grd(nt);
-  // the viscous increment is added in the last stage only
   constexpr bool VOK = MODE == kFP64 || MODE == kMixLast;
-  if (visc && !VOK) return hipErrorInvalidValue;
-  if (wall) {
-    if (geo == kGeoC2) return hipErrorInvalidValue;
-    if (geo == kGeoC3) {
-      if constexpr (VOK) {
-        if (visc) {
-          hipLaunchKernelGGL((k_stage<TX, TY, kGeoC3, true, MODE, true>), grd, blk, 0, s, tab, nb, nt);
-          return hipGetLastError();
-        }
-      }
-      hipLaunchKernelGGL((k_stage<TX, TY, kGeoC3, false, MODE, true>), grd, blk, 0, s, tab, nb, nt);
-      return hipGetLastError();
-    }
+  if (geo == kGeoC2) return hipErrorInvalidValue;
+  if (geo == kGeoC3) {
     if constexpr (VOK) {
       if (visc) {
-        hipLaunchKernelGGL((k_stage<TX, TY, kGeoCart, true, MODE, true>), grd, blk, 0, s, tab, nb, nt);
+        hipLaunchKernelGGL((k_stage<TX, TY, kGeoC3, true, MODE, W>), grd, blk, 0, s, tab, nb, nt);
         return hipGetLastError();
       }
     }
-    hipLaunchKernelGGL((k_stage<TX, TY, kGeoCart, false, MODE, true>), grd, blk, 0, s, tab, nb, nt);
+    hipLaunchKernelGGL((k_stage<TX, TY, kGeoC3, false, MODE, W>), grd, blk, 0, s, tab, nb, nt);
     return hipGetLastError();
   }
+  if constexpr (VOK) {
+    if (visc) {
+      hipLaunchKernelGGL((k_stage<TX, TY, kGeoCart, true, MODE, W>), grd, blk, 0, s, tab, nb, nt);
+      return hipGetLastError();
+    }
+  }
+  hipLaunchKernelGGL((k_stage<TX, TY, kGeoCart, false, MODE, W>), grd, blk, 0, s, tab, nb, nt);
+  return hipGetLastError();
+}
+
+// wall: 0 no bounded face, 1 bounded faces, 2 also advanced wall rows
+template <int TX, int TY, int MODE>
+hipError_t launch_t(const Batch<StageArgs> *tab, int nb, int nt, int geo, bool visc, int wall, hipStream_t s) {
+  // the viscous increment is added in the last stage only
+  constexpr bool VOK = MODE == kFP64 || MODE == kMixLast;
+  if (visc && !VOK) return hipErrorInvalidValue;
+  if (wall == 2) return launch_w<TX, TY, MODE, 2>(tab, nb, nt, geo, visc, s);
+  if (wall) return launch_w<TX, TY, MODE, 1>(tab, nb, nt, geo, visc, s);
   if (geo == kGeoC3) return launch_g<TX, TY, kGeoC3, MODE>(tab, nb, nt, visc, s);
   if (geo == kGeoC2) return launch_g<TX, TY, kGeoC2, MODE>(tab, nb, nt, visc, s);
   return launch_g<TX, TY, kGeoCart, MODE>(tab, nb, nt, visc, s);
 }
 
 template <int MODE>
-hipError_t launch_shape(const Batch<StageArgs> *tab, int nb, int nt, int geo, bool visc, int shape, bool wall,
+hipError_t launch_shape(const Batch<StageArgs> *tab, int nb, int nt, int geo, bool visc, int shape, int wall,
                         hipStream_t s) {
   switch (shape) {
     case 1: return launch_t<32, 8, MODE>(tab, nb, nt, geo, visc, wall, s);
     case 2: return launch_t<64, 8, MODE>(tab, nb, nt, geo, visc, wall, s);
     case 3: return launch_t<16, 16, MODE>(tab, nb, nt, geo, visc, wall, s);
     case 4: return launch_t<32, 16, MODE>(tab, nb, nt, geo, visc, wall, s);
-    case 5: return launch_tp<64, 8, MODE>(tab, nb, nt, geo, visc, wall, s);
-    case 6: return launch_tp<32, 16, MODE>(tab, nb, nt, geo, visc, wall, s);
+    case 5: return wall == 2 ? hipErrorInvalidValue : launch_tp<64, 8, MODE>(tab, nb, nt, geo, visc, wall != 0, s);
+    case 6: return wall == 2 ? hipErrorInvalidValue : launch_tp<32, 16, MODE>(tab, nb, nt, geo, visc, wall != 0, s);
     default: return launch_t<64, 4, MODE>(tab, nb, nt, geo, visc, wall, s);
   }
 }
@@ -1242,7 +1311,7 @@ TileRange stage_tiles(const Geom &g, int shape, int kchunk) {
 }
 
 hipError_t launch_stage(const Batch<StageArgs> *tab, int nb, int nt, int geo, bool visc, int shape, int mode,
-                        bool wall, hipStream_t s) {
+                        int wall, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
   switch (mode) {
     case kMixFirst: return launch_shape<kMixFirst>(tab, nb, nt, geo, visc, shape, wall, s);

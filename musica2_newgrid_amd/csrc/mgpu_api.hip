@@ -1067,6 +1067,10 @@ int build_tables(mgpu_handle *h) {
 // `src` (-1: from the block's stash) into the wall rows of set `dst` (-1:
 // into the stash).  A set >= 0 is an FP64 state set, a set <= -2 the
 // mixed-precision FP32 stage set -2 - set.
+// The stage kernel's wall variant: 0 no bounded face, 1 bounded faces, 2
+// also advanced wall rows.
+int stage_wall(const mgpu_handle *h) { return h->bnd ? (h->adv ? 2 : 1) : 0; }
+
 int enqueue_walls(mgpu_handle *h, int src, int dst) {
   for (const auto &bd : h->b) {
     if (!bd.g.jwall && !bd.g.iwall) continue;
@@ -1257,7 +1261,7 @@ int enqueue_step(mgpu_handle *h, int s0) {
       ProfScope ps(h, MGPU_K_STAGE);
       const int v = (last && visc) ? 1 : 0;
       HIPCK(h, launch_stage(h->d_stage + ((size_t)s0 * nrk + (irk - 1)) * nb, nb, h->nt_stage[v], geo,
-                            last && visc, h->shape[v], stage_mode(h, irk), h->bnd, h->stream));
+                            last && visc, h->shape[v], stage_mode(h, irk), stage_wall(h), h->stream));
     }
     if (h->mixed && !last) {
       ProfScope ps(h, MGPU_K_HALO);
@@ -1520,7 +1524,10 @@ int handle_add_block(mgpu_handle *h, const mgpu_block &bl, const mgpu_case &p) {
     h->stage_family = 1;  // the classic stage kernel closes i-faces
   }
   if (bd.wc3[0] || bd.wc3[1]) h->stage_family = 1;  // and c3 walls
-  if (bd.g.jadv || bd.g.iadv) h->stage_family = 1;  // and advanced wall rows
+  if (bd.g.jadv || bd.g.iadv) {
+    h->adv = true;
+    h->stage_family = 1;  // and advanced wall rows
+  }
   h->exchange_ready = false;
   h->tables_ready = false;
   return 0;
@@ -2206,10 +2213,10 @@ int mgpu_autotune(mgpu_handle *h, int flags) {
         const int mode = stage_mode(h, irk);
         hipError_t e = hipMemcpy(h->d_tune, tab.data(), nb * sizeof(tab[0]), hipMemcpyHostToDevice);
         // one warm-up launch, then three timed ones
-        if (e == hipSuccess) e = launch_stage(h->d_tune, nb, nt, geo, v == 1, sh, mode, h->bnd, h->stream);
+        if (e == hipSuccess) e = launch_stage(h->d_tune, nb, nt, geo, v == 1, sh, mode, stage_wall(h), h->stream);
         if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
         for (int r = 0; r < 3 && e == hipSuccess; r++)
-          e = launch_stage(h->d_tune, nb, nt, geo, v == 1, sh, mode, h->bnd, h->stream);
+          e = launch_stage(h->d_tune, nb, nt, geo, v == 1, sh, mode, stage_wall(h), h->stream);
         if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.f;

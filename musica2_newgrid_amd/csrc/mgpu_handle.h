@@ -117,6 +117,7 @@ struct mgpu_handle {
   bool mixed = false;  // FP32 flux/derivative/filter arithmetic
   bool wall = false;   // some block has no-slip walls
   bool bnd = false;    // some block has boundary faces (closure variants of the stage and filter kernels)
+  bool adv = false;    // some block has advanced wall rows (is_wall2: their stage-kernel variant)
   bool charac = false; // some block has characteristic or inlet faces
   bool shared_ref = false;  // one mixed-precision reference state for all blocks (set from block 0)
   double *diag_buf = nullptr;  // per-workgroup partial sums of mgpu_diagnostics

@@ -461,8 +461,10 @@ extern const int kStageShapes[][2];
 extern const int kNumStageShapes;
 extern const int kFirstPipelinedShape;  // shapes from here on: the pipelined stage kernel
 TileRange stage_tiles(const Geom &g, int shape, int kchunk);
+// wall: 0 no bounded face, 1 bounded faces (their closures), 2 also advanced
+// wall rows (is_wall2; classic shapes only)
 hipError_t launch_stage(const Batch<StageArgs> *tab, int nb, int ntiles, int geo, bool visc, int shape,
-                        int mode, bool wall, hipStream_t s);
+                        int mode, int wall, hipStream_t s);
 TileRange visc_tiles(const Geom &g, int kchunk);  // kchunk 0: default
 hipError_t launch_visc(const Batch<ViscArgs> *tab, int nb, int ntiles, int geo, hipStream_t s);
 TileRange vdiv_tiles(const Geom &g, int kchunk);
