@@ -28,13 +28,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
 # HBM bytes per k_stage launch on the 512x256x128 curvilinear blocks (tuned
-# 64x8 tile, k-chunk 128), averaged over the 4 stages (3 Euler launches + the
+# tiles, k-chunk 128), averaged over the 4 stages (3 Euler launches + the
 # viscous last stage): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch,
 # FETCH_SIZE scaled by 2.000 and WRITE_SIZE by 1.000 (their readings for a
 # 1 GiB device copy).  Summaries: profiles/step_kernels_c3_slip_channel_512x256x128_fp64.md
 # (the channel with slip walls), profiles/step_kernels_c3_channel_512x256x128_fp64.md
 # (no-slip walls), profiles/step_kernels_c3_512x256x128_fp64.md (the periodic block).
 STAGE_TRAFFIC = {
+    ("channel", 512, 256, 128): ((3 * 4.598e9 + 5.484e9) / 4,
+                                 "profiles/step_kernels_c3_slip_channel_512x256x128_fp64.md"),
     ("channel_noslip", 512, 256, 128): ((3 * 4.597e9 + 5.510e9) / 4,
                                         "profiles/step_kernels_c3_channel_512x256x128_fp64.md"),
     ("periodic", 512, 256, 128): ((3 * 4.589e9 + 5.338e9) / 4, "profiles/step_kernels_c3_512x256x128_fp64.md"),
