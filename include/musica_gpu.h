@@ -120,8 +120,9 @@ extern "C" {
                                      * faces times the spectral radius, U += deltat*K; every geometry,
                                      * iorder_visc = 4 */
 
-/* Arithmetic: FP64 everywhere, or FP32 flux/derivative/filter arithmetic with
- * FP64 RK accumulation, metrics and shock sensor. */
+/* Arithmetic: FP64 everywhere, or FP32 flux and derivative arithmetic and
+ * FP32 intermediate stage states, with FP64 RK accumulation, metrics, viscous
+ * terms, shock sensor and filters (the filters read and write FP64 states). */
 #define MGPU_PREC_FP64 0
 #define MGPU_PREC_MIXED 1
 

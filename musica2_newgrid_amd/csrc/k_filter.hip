@@ -92,19 +92,15 @@ __device__ __forceinline__ void ld5(const CState &U, idx_t p, double o[5]) {
   for (int v = 0; v < 5; v++) o[v] = ldg(U.v[v], p);
 }
 
-// State at element p as the filter arithmetic sees it: FP64, or (mixed
-// precision) the FP32 difference with the reference state.  The filter
-// stencils sum to zero, so the constant drops out of K.
-template <bool MIX, class R>
-__device__ __forceinline__ void ldr(const CState &U, idx_t p, const RefState &rf, R o[5]) {
-  if constexpr (MIX) {
-    const double ref[5] = {rf.r, rf.ru, rf.rv, rf.rw, rf.re};
+// State at element p.  The filters run in FP64 in both precisions: they read
+// and write FP64 states (the last stage's output and U^{n+1}), so FP32
+// arithmetic saved no time (0.93 ms per launch either way on the 512x256x128
+// channel), while rounding the state's difference with the reference state
+// to FP32 before the stencil was the largest error of the mixed path
+// (density 2.4e-6 after 1000 steps there, 4.7e-8 with FP64 filters).
+__device__ __forceinline__ void ldr(const CState &U, idx_t p, double o[5]) {
 #pragma unroll
-    for (int v = 0; v < 5; v++) o[v] = (R)(ldg(U.v[v], p) - ref[v]);
-  } else {
-#pragma unroll
-    for (int v = 0; v < 5; v++) o[v] = ldg(U.v[v], p);
-  }
+  for (int v = 0; v < 5; v++) o[v] = ldg(U.v[v], p);
 }
 
 // Filter increment along j of rows next to a wall: nothing on the first two,
@@ -125,10 +121,10 @@ __device__ __forceinline__ R fwall(const R *y, int s, int hw, double chi) {
   }
 }
 
-template <bool MIX, bool WALL>
+template <bool WALL>
 __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__restrict__ tab, int nb, int ntiles) {
 
-  using R = typename std::conditional<MIX, float, double>::type;
+  using R = double;
   __shared__ R sux[5][FY][FX + 10];
   __shared__ R suy[5][FY + 10][FX];
   int i0, j0, kbeg, kend;
@@ -144,7 +140,6 @@ __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__
   const idx_t sz = g.sy, own_off = lin(g, i, j, 0);
   Halo hl;
   halo_setup(g, i0, j0, hl);
-  const RefState &rf = a.ref;
   R dd[6];
 #pragma unroll
   for (int l = 0; l < 6; l++) dd[l] = (R)a.dd[l];
@@ -152,27 +147,25 @@ __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__
   R q[11][5];
   if (own) {
 #pragma unroll
-    for (int l = 0; l < 10; l++) ldr<MIX>(uinp, own_off + (idx_t)(kbeg - 5 + l) * sz, rf, q[l]);
+    for (int l = 0; l < 10; l++) ldr(uinp, own_off + (idx_t)(kbeg - 5 + l) * sz, q[l]);
   }
   // the state loads of a plane (queue point k+5, pencil cell, halo cells)
   // are issued one plane ahead, during the previous plane's stencils
   R qn[5], uc[5], uh[NS][5];
   {
     const idx_t pk = own_off + (idx_t)kbeg * sz;
-    if (own) ldr<MIX>(uinp, pk + 5 * sz, rf, qn);
-    if (!own && (needx || needy)) ldr<MIX>(uinp, pk, rf, uc);
+    if (own) ldr(uinp, pk + 5 * sz, qn);
+    if (!own && (needx || needy)) ldr(uinp, pk, uc);
 #pragma unroll
     for (int s = 0; s < NS; s++)
-      if (hl.on[s]) ldr<MIX>(uinp, hl.off[s] + (idx_t)kbeg * sz, rf, uh[s]);
+      if (hl.on[s]) ldr(uinp, hl.off[s] + (idx_t)kbeg * sz, uh[s]);
   }
   for (int k = kbeg; k <= kend; k++) {
     const idx_t pk = own_off + (idx_t)k * sz;
-    double u64[5];  // mixed: the FP64 state of the own point, to which K is added
     if (own) {
 #pragma unroll
       for (int v = 0; v < 5; v++) q[10][v] = qn[v];
     }
-    if (MIX && own) ld5(uinp, pk, u64);
     if (needx || needy) {
 #pragma unroll
       for (int v = 0; v < 5; v++) {
@@ -194,11 +187,11 @@ __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__
     __syncthreads();
     if (k < kend) {
       const idx_t pn = pk + sz;
-      if (own) ldr<MIX>(uinp, pn + 5 * sz, rf, qn);
-      if (!own && (needx || needy)) ldr<MIX>(uinp, pn, rf, uc);
+      if (own) ldr(uinp, pn + 5 * sz, qn);
+      if (!own && (needx || needy)) ldr(uinp, pn, uc);
 #pragma unroll
       for (int s = 0; s < NS; s++)
-        if (hl.on[s]) ldr<MIX>(uinp, hl.off[s] + (idx_t)(k + 1) * sz, rf, uh[s]);
+        if (hl.on[s]) ldr(uinp, hl.off[s] + (idx_t)(k + 1) * sz, uh[s]);
     }
     if (own) {
       const int hw = WALL ? jhalf(g, j) : 5;
@@ -224,8 +217,7 @@ __global__ __launch_bounds__(FT, 2) void k_filter_sf(const Batch<FilterArgs> *__
         }
         K = K + dd[0] * q[5][v] + dd[1] * (q[6][v] + q[4][v]) + dd[2] * (q[7][v] + q[3][v]) +
             dd[3] * (q[8][v] + q[2][v]) + dd[4] * (q[9][v] + q[1][v]) + dd[5] * (q[10][v] + q[0][v]);
-        if constexpr (MIX) stg_nt(uoutp.v[v], pk, u64[v] + (double)K);
-        else stg_nt(uoutp.v[v], pk, q[5][v] + K);
+        stg_nt(uoutp.v[v], pk, q[5][v] + K);
       }
     }
     __syncthreads();
@@ -302,13 +294,13 @@ __device__ __forceinline__ double edoh_mul(const FilterArgs &a, int d, idx_t p, 
 // EDOH: the shock filter's Edoh switch, each direction's increment times
 // min(1, CFL_d / chi) with the local CFL_d = sr_d deltat (idx_d on Cartesian
 // blocks) (mod_filtering_shock.f90:620-700, :1059-1140, :1450-1496).
-template <bool MIX, bool ARTV, bool WALL, bool EDOH, bool AVS>
+template <bool ARTV, bool WALL, bool EDOH, bool AVS>
 __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> *__restrict__ tab, int nb,
                                                         int ntiles) {
   static_assert(!(WALL && ARTV), "walls: not with artvisc_o9");
   static_assert(!(EDOH && (ARTV || AVS)), "the Edoh switch belongs to the shock filter");
   static_assert(!(AVS && ARTV), "one artificial viscosity at a time");
-  using R = typename std::conditional<MIX, float, double>::type;
+  using R = double;
   __shared__ R sux[5][FY][FX + 10];
   __shared__ R suy[5][FY + 10][FX];
   __shared__ double spx[FY][FX + 4];    // p at i0-2 .. i0+FX+1
@@ -372,13 +364,12 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
     }
   }
 
-  const RefState &rf = a.ref;
-  R q[11][5];        // U at k-5..k+5 (mixed: minus the reference state)
+  R q[11][5];        // U at k-5..k+5
   double qp[5];      // p at k-2..k+2 (ARTV: |w|+c at k-1..k+1 in qp[0..2])
   double qs[3];      // sensor at k-1..k+1
   if (own) {
 #pragma unroll
-    for (int l = 0; l < 10; l++) ldr<MIX>(uinp, own_off + (idx_t)(kbeg - 5 + l) * sz, rf, q[l]);
+    for (int l = 0; l < 10; l++) ldr(uinp, own_off + (idx_t)(kbeg - 5 + l) * sz, q[l]);
     if constexpr (ARTV) {
 #pragma unroll
       for (int l = 0; l < 2; l++) qp[l] = ldg(prsp, own_off + (idx_t)(kbeg - 1 + l) * sz);
@@ -397,14 +388,14 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
   auto load_plane = [&](int kk) {
     const idx_t pkk = own_off + (idx_t)kk * sz;
     if (own) {
-      ldr<MIX>(uinp, pkk + 5 * sz, rf, qn);
+      ldr(uinp, pkk + 5 * sz, qn);
       qpn = ARTV ? ldg(prsp, pkk + sz) : ldg(prsp, pkk + 2 * sz);
       qsn = duc ? ldg(sensp, pkk + sz) : 1.0;
     }
-    if (!own && (needx || needy)) ldr<MIX>(uinp, pkk, rf, uc);
+    if (!own && (needx || needy)) ldr(uinp, pkk, uc);
 #pragma unroll
     for (int s = 0; s < NS; s++)
-      if (hl.on[s]) ldr<MIX>(uinp, hl.off[s] + (idx_t)kk * sz, rf, uh[s]);
+      if (hl.on[s]) ldr(uinp, hl.off[s] + (idx_t)kk * sz, uh[s]);
 #pragma unroll
     for (int s = 0; s < NSP; s++) {
       if (pp_on[s]) {
@@ -417,11 +408,9 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
 
   for (int k = kbeg; k <= kend; k++) {
     const idx_t pk = own_off + (idx_t)k * sz;
-    double u64[5];  // mixed: the FP64 state of the own point, to which K is added
     if (own) {
 #pragma unroll
       for (int v = 0; v < 5; v++) q[10][v] = qn[v];
-      if constexpr (MIX) ld5(uinp, pk, u64);
       if constexpr (ARTV) qp[2] = qpn;
       else qp[4] = qpn;
       qs[2] = qsn;
@@ -544,8 +533,7 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
       }
 #pragma unroll
       for (int v = 0; v < 5; v++) {
-        if constexpr (MIX) stg_nt(uoutp.v[v], pk, u64[v] + a.dt * K[v]);
-        else stg_nt(uoutp.v[v], pk, q[5][v] + a.dt * K[v]);
+        stg_nt(uoutp.v[v], pk, q[5][v] + a.dt * K[v]);
       }
     }
     if (!ARTV && own) {
@@ -629,9 +617,7 @@ __global__ __launch_bounds__(FT, 2) void k_filter_shock(const Batch<FilterArgs> 
       }
 #pragma unroll
       for (int v = 0; v < 5; v++) {
-        if constexpr (AVS && MIX) stg_nt(uoutp.v[v], pk, u64[v] + a.dt * (double)K[v]);
-        else if constexpr (AVS) stg_nt(uoutp.v[v], pk, q[5][v] + a.dt * K[v]);
-        else if constexpr (MIX) stg_nt(uoutp.v[v], pk, u64[v] + (double)K[v]);
+        if constexpr (AVS) stg_nt(uoutp.v[v], pk, q[5][v] + a.dt * K[v]);
         else stg_nt(uoutp.v[v], pk, q[5][v] + K[v]);
       }
     }
@@ -763,53 +749,29 @@ TileRange filter_tiles(const Geom &g, int kc_req) {
   return t;
 }
 
-hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int nt, bool mixed, bool wall, hipStream_t s) {
+hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int nt, bool wall, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
-  if (wall) {
-    if (mixed) hipLaunchKernelGGL((k_filter_sf<true, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
-    else hipLaunchKernelGGL((k_filter_sf<false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
-  } else if (mixed) {
-    hipLaunchKernelGGL((k_filter_sf<true, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
-  } else {
-    hipLaunchKernelGGL((k_filter_sf<false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
-  }
+  if (wall) hipLaunchKernelGGL((k_filter_sf<true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  else hipLaunchKernelGGL((k_filter_sf<false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   return hipGetLastError();
 }
 
-template <bool EDOH>
-static void launch_shock_variant(const Batch<FilterArgs> *tab, int nb, int nt, bool mixed, bool wall, hipStream_t s) {
-  if (wall && mixed)
-    hipLaunchKernelGGL((k_filter_shock<true, false, true, EDOH, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
-  else if (wall)
-    hipLaunchKernelGGL((k_filter_shock<false, false, true, EDOH, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
-  else if (mixed)
-    hipLaunchKernelGGL((k_filter_shock<true, false, false, EDOH, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
-  else
-    hipLaunchKernelGGL((k_filter_shock<false, false, false, EDOH, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
-}
-
-hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int nt, bool mixed, DissipKind kind, bool wall,
-                               bool edoh, hipStream_t s) {
+hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int nt, DissipKind kind, bool wall, bool edoh,
+                               hipStream_t s) {
   if (nt <= 0) return hipSuccess;
   if (kind == kDissipArtvisc) {
     if (wall || edoh) return hipErrorInvalidValue;
-    if (mixed)
-      hipLaunchKernelGGL((k_filter_shock<true, true, false, false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
-    else
-      hipLaunchKernelGGL((k_filter_shock<false, true, false, false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    hipLaunchKernelGGL((k_filter_shock<true, false, false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   } else if (kind == kDissipArtviscShock) {
     if (edoh) return hipErrorInvalidValue;
-    if (wall && mixed)
-      hipLaunchKernelGGL((k_filter_shock<true, false, true, false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
-    else if (wall)
-      hipLaunchKernelGGL((k_filter_shock<false, false, true, false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
-    else if (mixed)
-      hipLaunchKernelGGL((k_filter_shock<true, false, false, false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
-    else
-      hipLaunchKernelGGL((k_filter_shock<false, false, false, false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    if (wall) hipLaunchKernelGGL((k_filter_shock<false, true, false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    else hipLaunchKernelGGL((k_filter_shock<false, false, false, true>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+  } else if (edoh) {
+    if (wall) hipLaunchKernelGGL((k_filter_shock<false, true, true, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    else hipLaunchKernelGGL((k_filter_shock<false, false, true, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   } else {
-    if (edoh) launch_shock_variant<true>(tab, nb, nt, mixed, wall, s);
-    else launch_shock_variant<false>(tab, nb, nt, mixed, wall, s);
+    if (wall) hipLaunchKernelGGL((k_filter_shock<false, true, false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
+    else hipLaunchKernelGGL((k_filter_shock<false, false, false, false>), dim3(nt), dim3(FT), 0, s, tab, nb, nt);
   }
   return hipGetLastError();
 }

@@ -960,7 +960,6 @@ FilterArgs filter_args(const mgpu_handle *h, const BlockData &bd, int in, int ou
   fa.xnu0 = h->c.filter == MGPU_FILTER_ARTVISC ? 0.0 : -bd.p.dissip_shock;
   fa.xnu10 = bd.p.dissip_coeff / 1260.0;
   fa.csens = bd.p.csens;
-  fa.ref = bd.ref;
   for (int q = 0; q < 3; q++) fa.sr[q] = bd.sr[q];
   // Cartesian metrics of the Edoh factor and the artificial viscosity (on a
   // 2.5-D block the i, j spectral radii carry their metric already)
@@ -1287,9 +1286,9 @@ int enqueue_step(mgpu_handle *h, int s0) {
     {
       ProfScope ps(h, MGPU_K_FILTER);
       if (c.filter == MGPU_FILTER_SF)
-        HIPCK(h, launch_filter_sf(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, h->bnd, h->stream));
+        HIPCK(h, launch_filter_sf(h->d_filt + s0 * nb, nb, h->nt_filt, h->bnd, h->stream));
       else
-        HIPCK(h, launch_filter_shock(h->d_filt + s0 * nb, nb, h->nt_filt, h->mixed, dissip_kind(c, h->wall), h->bnd,
+        HIPCK(h, launch_filter_shock(h->d_filt + s0 * nb, nb, h->nt_filt, dissip_kind(c, h->wall), h->bnd,
                                      edoh_on(c), h->stream));
       if (h->bnd && has_sensor(c))
         for (const auto &bd : h->b) {
@@ -1940,8 +1939,8 @@ int tune_aux(mgpu_handle *h, bool force, hipEvent_t e0, hipEvent_t e1) {
         if (kind == 0) return launch_visc(h->d_visc + s0 * nb, nb, h->nt_visc, geo, h->stream);
         if (kind == 1) return launch_vdiv(h->d_vdiv, nb, h->nt_vdiv, geo, h->stream);
         if (h->c.filter == MGPU_FILTER_SF)
-          return launch_filter_sf(h->d_tune_f, nb, h->nt_filt, h->mixed, h->bnd, h->stream);
-        return launch_filter_shock(h->d_tune_f, nb, h->nt_filt, h->mixed, dissip_kind(h->c, h->wall), h->bnd,
+          return launch_filter_sf(h->d_tune_f, nb, h->nt_filt, h->bnd, h->stream);
+        return launch_filter_shock(h->d_tune_f, nb, h->nt_filt, dissip_kind(h->c, h->wall), h->bnd,
                                    edoh_on(h->c), h->stream);
       };
       hipError_t e = launch();

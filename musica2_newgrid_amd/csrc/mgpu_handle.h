@@ -114,7 +114,7 @@ struct mgpu_handle {
   // one captured step per U^n set (a case without filter rotates the sets)
   bool use_graph = true;
   hipGraphExec_t gexec[3] = {};
-  bool mixed = false;  // FP32 flux/derivative/filter arithmetic
+  bool mixed = false;  // FP32 flux/derivative arithmetic and intermediate stage states
   bool wall = false;   // some block has no-slip walls
   bool bnd = false;    // some block has boundary faces (closure variants of the stage and filter kernels)
   bool adv = false;    // some block has advanced wall rows (is_wall2: their stage-kernel variant)

@@ -309,7 +309,6 @@ struct FilterArgs {
   double dt;            // artificial viscosity (U += deltat*K), Edoh switch (local CFL)
   int edoh;             // shock filter: Edoh switch
   int avs;              // artificial viscosity with shock capturing (k_shock_wall)
-  RefState ref;        // mixed precision only
 };
 
 // No-slip wall state (bc_wall with dp/dn = 0) of the j-faces (and i-faces)
@@ -470,18 +469,19 @@ hipError_t launch_visc(const Batch<ViscArgs> *tab, int nb, int ntiles, int geo, 
 TileRange vdiv_tiles(const Geom &g, int kchunk);
 hipError_t launch_vdiv(const Batch<VdivArgs> *tab, int nb, int ntiles, int geo, hipStream_t s);
 TileRange filter_tiles(const Geom &g, int kchunk);
-hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, bool wall,
-                            hipStream_t s);
+// The filters run in FP64 in both precisions (they read and write FP64
+// states); `wall`: the closures of blocks with bounded faces.
+hipError_t launch_filter_sf(const Batch<FilterArgs> *tab, int nb, int ntiles, bool wall, hipStream_t s);
 // The operators sharing the shock filter's face-dissipation structure.
 enum DissipKind {
   kDissipShock = 0,         // filter_o10_shock: U_n += K
   kDissipArtvisc = 1,       // artvisc_o9: U_n += deltat*K, spectral-radius weighted d10 faces
   kDissipArtviscShock = 2,  // artvisc_o9_shock: U_n += deltat*K, spectral-radius weighted shock faces
 };
-// `wall`: the closures of blocks with j-walls (not artvisc_o9); `edoh`: the
-// shock filter's Edoh switch; `mixed`: FP32 faces of U - U_ref, FP64 update.
-hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int ntiles, bool mixed, DissipKind kind,
-                               bool wall, bool edoh, hipStream_t s);
+// `wall`: the closures of blocks with bounded faces (not artvisc_o9); `edoh`:
+// the shock filter's Edoh switch.
+hipError_t launch_filter_shock(const Batch<FilterArgs> *tab, int nb, int ntiles, DissipKind kind, bool wall,
+                               bool edoh, hipStream_t s);
 // The increment along axis (0: i, 1: j) of the rows next to the bounded
 // faces of that axis of one block, after launch_filter_shock(.., wall), in
 // FP64 (the filter's input is the last stage's FP64 output): the shock

@@ -102,7 +102,7 @@ def test_graph_matches_eager_and_tuned(which):
 
 @pytest.mark.parametrize("which", ["tgv_artvisc", "tgv_ducros", "vortex_c3", "vortex_c2", "channel"])
 def test_mixed_precision_error(which):
-    """Mixed path (FP32 faces of U - U_ref, FP64 update): density error
+    """Mixed path (FP32 stage arithmetic and states, FP64 dissipation): density error
     against the FP64 oracle after 200 steps (100 for the channel)."""
     from musica2_newgrid_amd import _lib
     from musica2_newgrid_amd.solver import GpuSolver

@@ -1,13 +1,13 @@
-"""Mixed precision (MGPU_PREC_MIXED): FP32 flux, derivative and filter
-arithmetic about a reference state, FP64 state accumulation, metrics,
-Jacobian, viscous terms and shock sensor.  Density L-infinity error
-(relative to max rho) against the FP64 oracle after 1000 steps: at most 1e-6
-on the TGV (M=0.1); the strong isentropic vortex (beta=5, M=0.5, density
-dip of 8 %) on a 32x24x16 curvilinear block measures 5.1e-6, so its bound
-here is 1e-5 -- FP32 rounding of fluxes and of the stored intermediate
-stage states, whose perturbation is O(1) of the mean, accumulates over the
-4000 stages.  The errors are printed with each
-run."""
+"""Mixed precision (MGPU_PREC_MIXED): FP32 flux and derivative arithmetic
+about a reference state and FP32 intermediate stage states; FP64 state
+accumulation, metrics, Jacobian, viscous terms, shock sensor and filters.
+Density L-infinity error (relative to max rho) against the FP64 oracle after
+1000 steps: at most 1e-6 on the TGV (M=0.1); the strong isentropic vortex
+(beta=5, M=0.5, density dip of 8 %) crammed into a 32x24x16 curvilinear block
+measures 4.7e-6, so its bound here is 1e-5 -- FP32 rounding of the fluxes and
+of the stored intermediate stage states, whose perturbation is O(1) of the
+mean, accumulates over the 4000 stages (the same vortex on the 512x256x128
+channel stays below 1e-7).  The errors are printed with each run."""
 import numpy as np
 import pytest
 

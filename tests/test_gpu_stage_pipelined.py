@@ -71,7 +71,8 @@ def test_pipelined_mixed_equals_classic(which):
     Uc = _run(c, 3, "classic", precision=_lib.PREC_MIXED)
     d = relerr(Up, Uc)
     print("relative difference pipelined vs classic, mixed", d, flush=True)
-    assert d < 1e-7
+    # the two kernels round their FP32 sums in different orders
+    assert d < 1e-6
 
 
 @pytest.mark.parametrize("which", ["c3", "channel"])
