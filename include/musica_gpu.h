@@ -66,8 +66,10 @@ extern "C" {
  * momentum, rhoE = rho cv T_wall with the advanced rho), or slip for every
  * wall of an inviscid case (see MGPU_FACE_SLIP).  On the faces of one axis
  * per block (i, j or k; a block with walls on its i- or k-faces is stored
- * with that axis exchanged with j, invisibly to the caller) or on the i- and
- * j-faces together of Cartesian cases, or on the j-faces of 3-D curvilinear
+ * with that axis exchanged with j, invisibly to the caller) or of two axes
+ * of Cartesian blocks (walls on the j- and k-faces: the block stored with its
+ * j, k, i as i, j, k, the k-faces' state winning on the edges as in bc_wall's
+ * face order), or on the j-faces of 3-D curvilinear
  * cases (FP64 or mixed, any filter), at least 10 cells between the walls;
  * the block's extrapolation coefficients come from mgpu_set_wall
  * (Cartesian) or mgpu_set_wall_c3 (curvilinear).
@@ -358,7 +360,8 @@ void mgpu_destroy(mgpu_handle *h);
  * (communication_3dv :376-402): 0..5 the stress components xx,xy,xz,yy,yz,zz,
  * 6..8 the energy fluxes along x,y,z, 9 p, 10 the Ducros sensor, 11..13 the
  * spectral radii along i,j,k.  Blocks with walls on their i- or k-faces are
- * stored with that axis exchanged with j; the plan undoes that, so it can be
+ * stored with that axis exchanged with j, blocks with walls on their j- and
+ * k-faces with their axes cycled; the plan undoes that, so it can be
  * checked against the reference's face types (src/Parallel/
  * mod_mpi_types_two_sided.f90:47-197) on the caller's arrays. */
 int mgpu_exchange_plan(const mgpu_case *c, int set, int64_t *map, int64_t cap, int64_t *count);

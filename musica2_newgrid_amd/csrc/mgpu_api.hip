@@ -261,29 +261,54 @@ void face_axes(int f, int &a, int &t1, int &t2) {
 // orientation, and every interface is declared from both sides with the
 // inverse orientation.  Returns an error message or an empty string.
 // ---- axis relabeling of blocks with walls on the i- or k-faces ----
-// face f of the caller's block is face kFaceMap[perm][f] of the stored block
-// (and back: the map is its own inverse); the caller's axis a is the stored
-// axis kAxisMap[perm][a]; momentum v of the caller is stored as kVarMap[perm][v].
-constexpr int kFaceMap[3][6] = {{0, 1, 2, 3, 4, 5}, {2, 3, 0, 1, 4, 5}, {0, 1, 4, 5, 2, 3}};
-constexpr int kAxisMap[3][3] = {{0, 1, 2}, {1, 0, 2}, {0, 2, 1}};
-constexpr int kVarMap[3][5] = {{0, 1, 2, 3, 4}, {0, 2, 1, 3, 4}, {0, 1, 3, 2, 4}};
+// face f of the caller's block is face kFaceMap[perm][f] of the stored block;
+// the caller's axis a is the stored axis kAxisMap[perm][a]; momentum v of the
+// caller is stored as kVarMap[perm][v].  Perms 1 and 2 exchange two axes (each
+// its own inverse); perm 3 cycles them (the caller's j, k, i are stored as i,
+// j, k), so the stored i-face before the stored j-face keeps bc_wall's face
+// order at the edges; caller_axis / caller_face / caller_var invert the maps.
+constexpr int kFaceMap[4][6] = {{0, 1, 2, 3, 4, 5}, {2, 3, 0, 1, 4, 5}, {0, 1, 4, 5, 2, 3}, {4, 5, 0, 1, 2, 3}};
+constexpr int kAxisMap[4][3] = {{0, 1, 2}, {1, 0, 2}, {0, 2, 1}, {2, 0, 1}};
+constexpr int kVarMap[4][5] = {{0, 1, 2, 3, 4}, {0, 2, 1, 3, 4}, {0, 1, 3, 2, 4}, {0, 3, 1, 2, 4}};
+
+// The caller's axis / face / variable stored as s.
+int caller_axis(int perm, int s) {
+  for (int a = 0; a < 3; a++)
+    if (kAxisMap[perm][a] == s) return a;
+  return -1;
+}
+int caller_face(int perm, int s) {
+  for (int f = 0; f < 6; f++)
+    if (kFaceMap[perm][f] == s) return f;
+  return -1;
+}
+int caller_var(int perm, int s) {
+  for (int v = 0; v < 5; v++)
+    if (kVarMap[perm][v] == s) return v;
+  return -1;
+}
 
 // 0: walls (if any) on the j-faces, or on the i- and j-faces; 1: on the
 // i-faces alone; 2: on the k-faces alone, or on the i- and k-faces (stored as
-// i and j); -1: walls on the j- and k-faces, or on all three axes.
+// i and j); 3: on the j- and k-faces (stored as i and j); -1: walls on all
+// three axes.
 int wall_perm(const mgpu_block &b) {
   bool w[3];
   for (int a = 0; a < 3; a++) w[a] = is_wall(b.face_kind[2 * a]) || is_wall(b.face_kind[2 * a + 1]);
-  if (w[1] && w[2]) return -1;
+  if (w[0] && w[1] && w[2]) return -1;
+  if (w[1] && w[2]) return 3;
   if (w[0] && !w[1] && !w[2]) return 1;
   if (w[2]) return 2;
   return 0;
 }
 
 // The stored block's tangential axes of a face (the two other axes in
-// increasing order) come out in the caller's reverse order for the caller's
-// k-faces under perm 1 and i-faces under perm 2.
-bool tangential_flipped(int perm, int f) { return (perm == 1 && f / 2 == 2) || (perm == 2 && f / 2 == 0); }
+// increasing order) come out in the caller's reverse order: the caller's
+// k-faces under perm 1, i-faces under perm 2, j- and k-faces under perm 3.
+bool tangential_flipped(int perm, int f) {
+  const int a = f / 2, p = a == 0 ? 1 : 0, q = a == 2 ? 1 : 2;
+  return kAxisMap[perm][p] > kAxisMap[perm][q];
+}
 
 // Orientation code of an interface after relabeling: the caller's code maps
 // this face's tangential coordinates (p, q) to the neighbour's (s1, s2) by a
@@ -316,7 +341,7 @@ std::string relabel_blocks(const mgpu_case &c, std::vector<mgpu_block> &out, std
     perm[b] = wall_perm(c.blocks[b]);
     if (perm[b] < 0) {
       snprintf(buf, sizeof buf,
-               "block %d: walls on the j- and k-faces together (or on all three axes) are not supported", b);
+               "block %d: walls on all three axes are not supported", b);
       return buf;
     }
   }
@@ -374,26 +399,14 @@ std::string check_cartesian_frames(const mgpu_case &c) {
 // A caller's padded array of block extents (as the caller sees them) to the
 // stored layout (to_stored) or back.
 void relabel_array(const Geom &g, int perm, const double *src, double *dst, bool to_stored) {
-  const long Q0 = g.nx + 2 * G, Q1 = g.ny + 2 * G, Q2 = g.nz + 2 * G;
-  long P0 = Q0, P1 = Q1;  // the caller's padded extents along its i and j
-  if (perm == 1) {
-    P0 = Q1;
-    P1 = Q0;
-  } else if (perm == 2) {
-    P1 = Q2;
-  }
-  for (long c = 0; c < Q2; c++)
-    for (long b = 0; b < Q1; b++)
-      for (long a = 0; a < Q0; a++) {
-        long ui = a, uj = b, uk = c;
-        if (perm == 1) {
-          ui = b;
-          uj = a;
-        } else if (perm == 2) {
-          uj = c;
-          uk = b;
-        }
-        const long s = a + Q0 * (b + Q1 * c), u = ui + P0 * (uj + P1 * uk);
+  const long Q[3] = {g.nx + 2 * G, g.ny + 2 * G, g.nz + 2 * G};
+  const int *m = kAxisMap[perm];
+  const long P0 = Q[m[0]], P1 = Q[m[1]];  // the caller's padded extents along its i and j
+  long x[3];
+  for (x[2] = 0; x[2] < Q[2]; x[2]++)
+    for (x[1] = 0; x[1] < Q[1]; x[1]++)
+      for (x[0] = 0; x[0] < Q[0]; x[0]++) {
+        const long s = x[0] + Q[0] * (x[1] + Q[1] * x[2]), u = x[m[0]] + P0 * (x[m[1]] + P1 * x[m[2]]);
         if (to_stored) dst[s] = src[u];
         else dst[u] = src[s];
       }
@@ -585,7 +598,7 @@ struct XSet {
 };
 
 // Stored axis of block n holding the caller's axis that block b stores as s.
-int axis_between(int pb, int pn, int s) { return kAxisMap[pn][kAxisMap[pb][s]]; }
+int axis_between(int pb, int pn, int s) { return kAxisMap[pn][caller_axis(pb, s)]; }
 
 // Index of the symmetric tensor component (s, t) in tau[0..5] (xx, xy, xz, yy, yz, zz).
 int tau_index(int s, int t) {
@@ -603,7 +616,7 @@ const int kTauPair[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
 void face_vmap(const XSet &xs, int pb, int pn, const int dst_ax[3], const int src_ax[3], int8_t vmap[kIfaceMaxVar]) {
   for (int v = 0; v < kIfaceMaxVar; v++) vmap[v] = (int8_t)v;
   if (!xs.visc) {
-    for (int v = 0; v < 5; v++) vmap[v] = (int8_t)kVarMap[pn][kVarMap[pb][v]];
+    for (int v = 0; v < 5; v++) vmap[v] = (int8_t)kVarMap[pn][caller_var(pb, v)];
     return;
   }
   for (int q = 0; q < 6; q++)
@@ -1640,7 +1653,7 @@ int put_state(mgpu_handle *h, int block, const double *const *src_in) {
   std::vector<double> tmp[5];
   const double *sp[5];
   for (int v = 0; v < 5; v++) {
-    const double *u = src_in[kVarMap[bd.perm][v]];
+    const double *u = src_in[caller_var(bd.perm, v)];
     if (bd.perm) {
       tmp[v].resize((size_t)bd.g.n);
       relabel_array(bd.g, bd.perm, u, tmp[v].data(), true);
@@ -1671,14 +1684,14 @@ int put_state(mgpu_handle *h, int block, const double *const *src_in) {
     }
     // the mean in the caller's frame, then in each block's stored axis order
     double mc[5];
-    for (int v = 0; v < 5; v++) mc[kVarMap[bd.perm][v]] = mean[v];
+    for (int v = 0; v < 5; v++) mc[caller_var(bd.perm, v)] = mean[v];
     auto ref_of = [&](int perm) {
       RefState r{};
-      r.r = mc[kVarMap[perm][0]];
-      r.ru = mc[kVarMap[perm][1]];
-      r.rv = mc[kVarMap[perm][2]];
-      r.rw = mc[kVarMap[perm][3]];
-      r.re = mc[kVarMap[perm][4]];
+      r.r = mc[caller_var(perm, 0)];
+      r.ru = mc[caller_var(perm, 1)];
+      r.rv = mc[caller_var(perm, 2)];
+      r.rw = mc[caller_var(perm, 3)];
+      r.re = mc[caller_var(perm, 4)];
       r.u = r.ru / r.r;
       r.v = r.rv / r.r;
       r.w = r.rw / r.r;
@@ -1711,7 +1724,7 @@ int get_state(mgpu_handle *h, int block, double *const *dst) {
   std::vector<double> tmp((size_t)bd.g.n);
   for (int v = 0; v < 5; v++) {
     HIPCK(h, hipMemcpy(tmp.data(), bd.st[h->cur][v], bytes, hipMemcpyDeviceToHost));
-    relabel_array(bd.g, bd.perm, tmp.data(), dst[kVarMap[bd.perm][v]], false);
+    relabel_array(bd.g, bd.perm, tmp.data(), dst[caller_var(bd.perm, v)], false);
   }
   return 0;
 }
@@ -2052,7 +2065,7 @@ int mgpu_set_wall(mgpu_handle *h, int block, const double *cextrp2, const double
     return fail(h, MGPU_ESTATE, "block %d: curvilinear walls take their coefficients from mgpu_set_wall_c3", block);
   for (int s = 0; s < 2; s++) {
     if (!((bd.g.jwall >> s) & 1)) continue;
-    const int fu = kFaceMap[bd.perm][2 + s];  // the caller's face stored as j-face 2 + s
+    const int fu = caller_face(bd.perm, 2 + s);  // the caller's face stored as j-face 2 + s
     const double a = cextrp2[fu], b = cextrp3[fu];
     if (!std::isfinite(a) || !std::isfinite(b))
       return fail(h, MGPU_EINVAL, "block %d face %d: extrapolation coefficients not finite", block, fu);
@@ -2061,7 +2074,7 @@ int mgpu_set_wall(mgpu_handle *h, int block, const double *cextrp2, const double
   }
   for (int s = 0; s < 2; s++) {
     if (!((bd.g.iwall >> s) & 1)) continue;
-    const int fu = kFaceMap[bd.perm][s];  // the caller's face stored as i-face s
+    const int fu = caller_face(bd.perm, s);  // the caller's face stored as i-face s
     const double a = cextrp2[fu], b = cextrp3[fu];
     if (!std::isfinite(a) || !std::isfinite(b))
       return fail(h, MGPU_EINVAL, "block %d face %d: extrapolation coefficients not finite", block, fu);
@@ -2352,27 +2365,17 @@ int mgpu_exchange_plan(const mgpu_case *c, int set, int64_t *map, int64_t cap, i
   auto caller_elem = [&](int b, long off) -> long {
     const Geom &g = bds[b].g;
     const long Q0 = g.sx, Q1 = g.ny + 2 * G, Q2 = g.nz + 2 * G;
-    const long a0 = off % Q0, a1 = (off / Q0) % Q1, a2 = off / (Q0 * Q1);
-    long ui = a0, uj = a1, uk = a2, P0 = Q0, P1 = Q1;
-    if (perm[b] == 1) {
-      ui = a1;
-      uj = a0;
-      P0 = Q1;
-      P1 = Q0;
-    } else if (perm[b] == 2) {
-      uj = a2;
-      uk = a1;
-      P1 = Q2;
-    }
-    return ui + P0 * (uj + P1 * uk);
+    const long x[3] = {off % Q0, (off / Q0) % Q1, off / (Q0 * Q1)}, Q[3] = {Q0, Q1, Q2};
+    const int *m = kAxisMap[perm[b]];
+    return x[m[0]] + Q[m[0]] * (x[m[1]] + Q[m[1]] * x[m[2]]);
   };
-  auto caller_var = [&](int b, int v) -> int {
+  auto plan_var = [&](int b, int v) -> int {
     const int p = perm[b];
-    if (set == 0) return kVarMap[p][v];
-    if (v < 6) return tau_index(kAxisMap[p][kTauPair[v][0]], kAxisMap[p][kTauPair[v][1]]);
-    if (v < NTAU) return 6 + kAxisMap[p][v - 6];
+    if (set == 0) return caller_var(p, v);
+    if (v < 6) return tau_index(caller_axis(p, kTauPair[v][0]), caller_axis(p, kTauPair[v][1]));
+    if (v < NTAU) return 6 + caller_axis(p, v - 6);
     if (v < NTAU + 2) return v;
-    return NTAU + 2 + kAxisMap[p][v - NTAU - 2];
+    return NTAU + 2 + caller_axis(p, v - NTAU - 2);
   };
   const int nv = xs.nvar();
   int64_t n = 0;
@@ -2387,10 +2390,10 @@ int mgpu_exchange_plan(const mgpu_case *c, int set, int64_t *map, int64_t cap, i
             int64_t *e = map + 6 * n;
             e[0] = t.dst;
             e[1] = caller_elem(t.dst, pd);
-            e[2] = caller_var(t.dst, v);
+            e[2] = plan_var(t.dst, v);
             e[3] = t.src;
             e[4] = caller_elem(t.src, ps);
-            e[5] = caller_var(t.src, t.vmap[v]);
+            e[5] = plan_var(t.src, t.vmap[v]);
           }
         }
   *count = n;

@@ -55,7 +55,8 @@ struct BlockData {
   bool have_wall = false;  // wall coefficients set (blocks with walls)
   // Axis relabeling: the kernels close walls on the j-faces only, so a block
   // with walls on its i-faces (perm 1) is stored with i and j exchanged, one
-  // with walls on its k-faces (perm 2) with j and k exchanged; the state,
+  // with walls on its k-faces (perm 2) with j and k exchanged, one with walls
+  // on its j- and k-faces (perm 3) with its j, k, i stored as i, j, k; the state,
   // Cartesian metrics and wall coefficients are exchanged with it at the API.
   int perm = 0;
 };

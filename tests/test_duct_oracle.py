@@ -9,7 +9,11 @@ stage of crk = 1 without viscosity, the wall state after the step from U^n.
 With cextrp2 = 4/3 and cextrp3 = 1/3 (uniform rows) a face's extrapolation of
 a linear field is the wall value plus 2/3 of one row's increment along its
 normal; the edges take the j-faces' value, the rest of each wall row its own
-face's."""
+face's.
+
+Walls on the j- and k-faces: the duct with its axes cycled (x, y, z become
+y, z, x) is the same flow, its edges taking the k-faces' state (the duct's
+j-faces'), within rounding of the reordered sums."""
 import numpy as np
 
 from musica2_newgrid_amd import cases
@@ -83,3 +87,32 @@ def test_duct_runs_and_keeps_its_walls():
         w = U1[v][G:-G, G:-G, G:-G]
         assert np.all(w[:, 0, :] == 0.0) and np.all(w[:, -1, :] == 0.0)
         assert np.all(w[:, :, 0] == 0.0) and np.all(w[:, :, -1] == 0.0)
+
+
+def cycle_axes(c):
+    """The same duct with its axes cycled (x, y, z become y, z, x): walls on
+    the j- and k-faces, periodic along i."""
+    t = cases.Case(c.name + "_jk", c.nz, c.nx, c.ny, curvilinear=False, iorder_visc=c.iorder_visc,
+                   filter=c.filter, ducros=c.ducros, diffscale=c.diffscale, chi=c.chi, dt=c.dt, nrk=c.nrk,
+                   crk=c.crk, cart=[c.cart[2], c.cart[0], c.cart[1], c.cart[5], c.cart[3], c.cart[4]],
+                   faces=tuple(c.faces[f] for f in (4, 5, 0, 1, 2, 3)), adiabatic=c.adiabatic,
+                   t_wall=c.t_wall, cextrp2=tuple(c.cextrp2[f] for f in (4, 5, 0, 1, 2, 3)),
+                   cextrp3=tuple(c.cextrp3[f] for f in (4, 5, 0, 1, 2, 3)), wall2=c.wall2)
+    U = [np.ascontiguousarray(np.transpose(a, (1, 2, 0))) for a in c.U]
+    t.U = [U[0], U[3], U[1], U[2], U[4]]
+    return t
+
+
+def test_walls_on_the_j_and_k_faces_are_the_duct_cycled():
+    from oracle.oracle import Oracle
+    for adiabatic in (True, False):
+        c = cases.duct(36, 32, 12, filter=2, adiabatic=adiabatic)
+        c.ducros = True
+        t = cycle_axes(c)
+        Ud, Ut = c.copy_state(), t.copy_state()
+        Oracle(c).step(Ud, 10)
+        Oracle(t).step(Ut, 10)
+        for v, w in zip((0, 1, 2, 3, 4), (0, 3, 1, 2, 4)):
+            a = gr.interior(Ut[v])
+            b = np.transpose(gr.interior(Ud[w]), (1, 2, 0))
+            assert np.max(np.abs(a - b)) <= 1e-13 * (np.max(np.abs(b)) + 1.0), (adiabatic, v)

@@ -3,13 +3,15 @@ face copies (the reference's face types, mod_mpi_types_two_sided.f90:47-197,
 restated by orc_fill_interfaces) on the caller's arrays.
 
 A block with walls on its i- or k-faces is stored with that axis exchanged
-with j, and its momentum, stress and spectral-radius components with it.
+with j, one with walls on its j- and k-faces with its axes cycled (j, k, i
+stored as i, j, k), and its momentum, stress and spectral-radius components
+with them.
 mgpu_exchange_plan returns what the halo copies write, mapped back to the
 caller's frame: every ghost value of every array must come from the element
 and the array the oracle's brute-force face copy names (the same quantity in
 the caller's frame: the caller's axes are the physical axes on a Cartesian
-case).  Covered: two-block rings with walls on any axis of either block, all
-reversal codes; a c3 ring whose blocks' axes are signed permutations of each
+case).  Covered: two-block rings with walls on any axis, or on two axes, of either
+block, all reversal codes; a c3 ring whose blocks' axes are signed permutations of each
 other (all eight orientation codes), where the spectral radius along an
 index axis comes from the neighbour's index axis it meets.
 """
@@ -96,7 +98,8 @@ def check_plan(case, blocks, set_, radius_axes=None):
 def ring(wa, wb, x, o):
     """Blocks A (10x11x12) and B joined along axis x in a ring (A's max face to
     B's min face and back, orientation code o); A walled on axis wa, B on wb
-    (None: periodic), the other axes periodic."""
+    (None: periodic; a tuple: walls on each axis in it), the other axes
+    periodic."""
     ext_a = [10, 11, 12]
     ext_b = list(ext_a)
     ext_b[x] = 13
@@ -104,8 +107,8 @@ def ring(wa, wb, x, o):
     for b, (ext, w) in enumerate(((ext_a, wa), (ext_b, wb))):
         kind = [_lib.FACE_PERIODIC] * 6
         nbr_block, nbr_face, orient = [b] * 6, [f ^ 1 for f in range(6)], [0] * 6
-        if w is not None:
-            kind[2 * w] = kind[2 * w + 1] = _lib.FACE_WALL
+        for a in (() if w is None else w if isinstance(w, tuple) else (w,)):
+            kind[2 * a] = kind[2 * a + 1] = _lib.FACE_WALL
         for s in range(2):
             f = 2 * x + s
             kind[f] = _lib.FACE_INTERFACE
@@ -117,6 +120,10 @@ def ring(wa, wb, x, o):
 
 RINGS = [(wa, wb, x) for wa in (None, 0, 1, 2) for wb in (None, 0, 1, 2) for x in range(3)
          if x != wa and x != wb]
+# walls on two axes: j and k (axes cycled) against every other relabeling, i
+# and j (none), i and k (j and k exchanged) against i alone (i and j exchanged)
+RINGS += [((1, 2), wb, 0) for wb in (None, 1, 2, (1, 2))]
+RINGS += [((0, 1), 0, 2), ((0, 2), 0, 1), ((0, 2), (0, 2), 1)]
 
 
 @pytest.mark.parametrize("wa,wb,x", RINGS)

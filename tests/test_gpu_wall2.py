@@ -114,6 +114,20 @@ def test_relabeled_advanced_walls_match_oracle():
     assert e < 1e-12
 
 
+def test_advanced_walls_on_the_j_and_k_faces_match_oracle():
+    """No slip on the j-faces, slip on the k-faces (the block stored with its
+    j, k, i as i, j, k): the edges take the k-faces' slip last."""
+    from tests.test_duct_oracle import cycle_axes
+    t = cycle_axes(_duct2())
+    n = 10
+    print(f"advanced walls on the j- and k-faces {t.nx}x{t.ny}x{t.nz}, {n} steps", flush=True)
+    Ug = _gpu(t, n)
+    Uo = _oracle(t, n)
+    e = relerr(Ug, Uo)
+    print(f"relerr {e:.3e}", flush=True)
+    assert e < 1e-12
+
+
 @pytest.mark.parametrize("filt", [2, 1, 0])
 def test_slip_curvilinear_channel_matches_oracle(filt):
     c = cases.vortex_channel(48, 40, 12, filter=filt, slip=True)

@@ -4,14 +4,16 @@ every wall, periodic z) with no filter, the selective filter and the shock
 filter (Ducros), adiabatic and isothermal walls, FP64 within 1e-12 relative;
 the edges taking the j-faces' wall state (the one-stage linear-pressure case
 of the oracle pin); walls on the i- and k-faces (the block stored with j and
-k exchanged); graph replays bit-identical to eager launches; the mixed path
-against FP64."""
+k exchanged); walls on the j- and k-faces (the block stored with its j, k, i
+as i, j, k, so the edges take the k-faces' state, last in bc_wall's order);
+graph replays bit-identical to eager launches; the mixed path against FP64."""
 import numpy as np
 import pytest
 
 from musica2_newgrid_amd import _lib, cases
 from musica2_newgrid_amd import grid as gr
 from tests._util import relerr
+from tests.test_duct_oracle import cycle_axes as _cycle
 
 pytestmark = pytest.mark.gpu
 
@@ -72,6 +74,45 @@ def _swap_jk(c):
     U[2], U[3] = U[3], U[2]
     t.U = U
     return t
+
+
+@pytest.mark.parametrize("filt,ducros,adiabatic", [(2, True, True), (0, False, True), (2, False, False)])
+def test_walls_on_the_j_and_k_faces_match_oracle(filt, ducros, adiabatic):
+    c = cases.duct(36, 32, 12, filter=filt, adiabatic=adiabatic)
+    c.ducros = ducros
+    t = _cycle(c)
+    n = 10
+    print(f"duct with walls on the j- and k-faces {t.nx}x{t.ny}x{t.nz} filter={filt} ducros={ducros} "
+          f"adiabatic={adiabatic}, {n} steps", flush=True)
+    Ug = _gpu(t, n)
+    Uo = _oracle(t, n)
+    e = relerr(Ug, Uo)
+    print(f"relerr {e:.3e}", flush=True)
+    assert e < 1e-12
+    # the same flow as the duct itself, axes cycled
+    Ud = _oracle(c, n)
+    for v, w in zip((0, 1, 2, 3, 4), (0, 3, 1, 2, 4)):
+        a = gr.interior(Ug[v])
+        b = np.transpose(gr.interior(Ud[w]), (1, 2, 0))
+        assert np.allclose(a, b, rtol=1e-10, atol=1e-10 * float(np.max(np.abs(b)) + 1.0))
+
+
+def test_walls_on_the_j_and_k_faces_graph_and_mixed():
+    c = cases.duct(40, 36, 12, filter=2)
+    c.ducros = True
+    t = _cycle(c)
+    print(f"duct with walls on the j- and k-faces {t.nx}x{t.ny}x{t.nz}: graph vs eager (6 steps), "
+          f"mixed vs FP64 (30 steps)", flush=True)
+    Ua = _gpu(t, 6)
+    Ub = _gpu(t, 6, eager=True)
+    for a, b in zip(Ua, Ub):
+        assert np.array_equal(gr.interior(a), gr.interior(b))
+    Um = _gpu(t, 30, precision=_lib.PREC_MIXED)
+    Ud = _gpu(t, 30)
+    a, b = gr.interior(Um[0]), gr.interior(Ud[0])
+    e = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    print(f"mixed vs FP64 rho Linf {e:.3e}", flush=True)
+    assert np.all(np.isfinite(a)) and e < 1e-6
 
 
 def test_walls_on_the_i_and_k_faces_match_oracle():
