@@ -5,7 +5,8 @@ filter (Ducros), adiabatic and isothermal walls, FP64 within 1e-12 relative;
 the edges taking the j-faces' wall state (the one-stage linear-pressure case
 of the oracle pin); walls on the i- and k-faces (the block stored with j and
 k exchanged); walls on the j- and k-faces (the block stored with its j, k, i
-as i, j, k, so the edges take the k-faces' state, last in bc_wall's order);
+as i, j, k, so the edges take the k-faces' state, last in bc_wall's order),
+also cut in two blocks along i (no-slip or advanced walls);
 graph replays bit-identical to eager launches; the mixed path against FP64."""
 import numpy as np
 import pytest
@@ -113,6 +114,47 @@ def test_walls_on_the_j_and_k_faces_graph_and_mixed():
     e = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
     print(f"mixed vs FP64 rho Linf {e:.3e}", flush=True)
     assert np.all(np.isfinite(a)) and e < 1e-6
+
+
+@pytest.mark.parametrize("wall2", [False, True])
+def test_j_and_k_walled_duct_in_two_blocks(wall2):
+    """The axis-cycled duct cut in two blocks along i: the interfaces sit on
+    i-faces, stored as k-faces with their tangential axes reversed; advanced
+    walls (no slip on the j-faces, slip on the k-faces) or no-slip walls."""
+    from oracle import oracle as orc
+    c = cases.duct(40, 36, 16, filter=2)
+    c.ducros = True
+    if wall2:
+        c.wall2 = True
+        c.faces = (2, 2, 5, 5, 0, 0)
+    t = _cycle(c)
+    h = t.nx // 2
+    blocks, bcs = [], []
+    for b in range(2):
+        lo, hi = b * h, b * h + h + 2 * G
+        sub = cases.Case(f"half{b}", h, t.ny, t.nz, curvilinear=False, iorder_visc=t.iorder_visc,
+                         filter=t.filter, ducros=t.ducros, diffscale=t.diffscale, chi=t.chi, dt=t.dt,
+                         cart=[t.cart[0][lo:hi].copy(), t.cart[1], t.cart[2], t.cart[3][lo:hi].copy(), t.cart[4],
+                               t.cart[5]],
+                         faces=(1, 1) + tuple(t.faces[2:]), adiabatic=t.adiabatic, t_wall=t.t_wall,
+                         cextrp2=t.cextrp2, cextrp3=t.cextrp3, wall2=t.wall2)
+        sub.U = [np.ascontiguousarray(a[:, :, lo:hi]) for a in t.U]
+        bcs.append(sub)
+        kinds = [_lib.FACE_INTERFACE] * 2 + [int(k) for k in t.faces[2:]]
+        blocks.append(dict(nx=h, ny=t.ny, nz=t.nz, kind=kinds, nbr_block=[1 - b, 1 - b, b, b, b, b],
+                           nbr_face=[1, 0, 3, 2, 5, 4], orient=[0] * 6,
+                           wall=[0, 0] + [max(0, int(k) - 1) for k in t.faces[2:]]))
+    n = 6
+    print(f"j- and k-walled duct in 2 blocks along i, {t.nx}x{t.ny}x{t.nz}, wall2={wall2}, {n} steps", flush=True)
+    from musica2_newgrid_amd.solver import GpuSolver
+    with GpuSolver(t, topology=blocks, block_cases=bcs) as s:
+        s.step(n)
+        Ug = s.get_states()
+    Uc = [bc.copy_state() for bc in bcs]
+    orc.step_multi(t, blocks, bcs, Uc, n)
+    e = max(relerr(a, b) for a, b in zip(Ug, Uc))
+    print(f"relerr {e:.3e}", flush=True)
+    assert e < 1e-12
 
 
 def test_walls_on_the_i_and_k_faces_match_oracle():
