@@ -69,8 +69,10 @@ extern "C" {
  * with that axis exchanged with j, invisibly to the caller) or of two axes
  * of Cartesian blocks (walls on the j- and k-faces: the block stored with its
  * j, k, i as i, j, k, the k-faces' state winning on the edges as in bc_wall's
- * face order), or on the j-faces of 3-D curvilinear
- * cases (FP64 or mixed, any filter), at least 10 cells between the walls;
+ * face order), or on the faces of one axis of 3-D curvilinear blocks (walls
+ * on their i- or k-faces: the block stored with that index axis exchanged
+ * with j, its metric rows with it; momentum and tau stay in x, y, z)
+ * (FP64 or mixed, any filter), at least 10 cells between the walls;
  * the block's extrapolation coefficients come from mgpu_set_wall
  * (Cartesian) or mgpu_set_wall_c3 (curvilinear).
  * With a shock sensor (filters 2, 4) the ghost row beyond a wall must hold a
@@ -233,16 +235,19 @@ int mgpu_set_metrics_c2(mgpu_handle *h, int block, const double *const *m2, cons
  * row (is_wall2) uses (:245-256). */
 int mgpu_set_wall(mgpu_handle *h, int block, const double *cextrp2, const double *cextrp3);
 
-/* Wall coefficients of a j-face (face 2 or 3) of a 3-D curvilinear block
- * (MGPU_GEOM_C3, where walls sit on j-faces only), as grid_normals.f90:
- * 232-300 computes them for bc_wall_jmin/jmax_*_dpdn_c3
- * (src/Boundary/smod_bc_wall_c3.f90:842-1671) and the adiabatic wall's
- * tangential temperature gradient (src/Compute_flow/smod_grad_T_c3.f90:
- * 249-275): coef[0..7] are arrays of nx*nz doubles (i fastest) over the wall
- * points: gksigeta3 = -grad(ksi).grad(eta)/|grad(eta)|^2, gphigeta3 (phi for
- * ksi), the unit tangent x_ksi/|x_ksi| (x, y, z: txn_ksi, tyn_ksi, tzn_ksi)
- * and x_phi/|x_phi| (txn_phi, tyn_phi, tzn_phi).  Required for every wall
- * face of such a block before its first step. */
+/* Wall coefficients of a wall face of a 3-D curvilinear block
+ * (MGPU_GEOM_C3), as grid_normals.f90:232-300 computes them for
+ * bc_wall_jmin/jmax_*_dpdn_c3 (src/Boundary/smod_bc_wall_c3.f90:842-1671)
+ * and the adiabatic wall's tangential temperature gradient
+ * (src/Compute_flow/smod_grad_T_c3.f90:249-275): for a j-face coef[0..7] are
+ * arrays of nx*nz doubles (i fastest) over the wall points: gksigeta3 =
+ * -grad(ksi).grad(eta)/|grad(eta)|^2, gphigeta3 (phi for ksi), the unit
+ * tangent x_ksi/|x_ksi| (x, y, z: txn_ksi, tyn_ksi, tzn_ksi) and x_phi/|x_phi|
+ * (txn_phi, tyn_phi, tzn_phi).  An i- or k-face takes the same with its own
+ * two tangential axes in order (i-face: eta, phi over ny*nz points, j
+ * fastest, getagksi3, gphigksi3, :10-839; k-face: ksi, eta over nx*ny, i
+ * fastest, :1674-2505).  Required for every wall face of such a block before
+ * its first step. */
 int mgpu_set_wall_c3(mgpu_handle *h, int block, int face, const double *const *coef);
 
 /* Dual tangents of a slip j-face (face 2 or 3) of a 3-D curvilinear block,

@@ -202,12 +202,15 @@ std::string check_charac(const mgpu_case &c) {
 std::string check_c3_walls(const mgpu_case &c) {
   char buf[256];
   if (c.curvilinear != MGPU_GEOM_C3) return std::string();
-  for (int b = 0; b < c.nblocks; b++)
-    for (int f : {0, 1, 4, 5})
-      if (is_wall(c.blocks[b].face_kind[f])) {
-        snprintf(buf, sizeof buf, "block %d face %d: 3-D curvilinear blocks take walls on their j-faces only", b, f);
-        return buf;
-      }
+  for (int b = 0; b < c.nblocks; b++) {
+    int axes = 0;
+    for (int a = 0; a < 3; a++)
+      if (is_wall(c.blocks[b].face_kind[2 * a]) || is_wall(c.blocks[b].face_kind[2 * a + 1])) axes++;
+    if (axes > 1) {
+      snprintf(buf, sizeof buf, "block %d: 3-D curvilinear blocks take walls on the faces of one axis", b);
+      return buf;
+    }
+  }
   return std::string();
 }
 
@@ -613,15 +616,18 @@ const int kTauPair[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
 // frame through both blocks' axis relabelings; the spectral radius along an
 // index axis is that of the neighbour's index axis it meets (on c3 and 2.5-D
 // blocks a swapping interface exchanges them; the radius has no sign).
-void face_vmap(const XSet &xs, int pb, int pn, const int dst_ax[3], const int src_ax[3], int8_t vmap[kIfaceMaxVar]) {
+// vb, vn: the blocks' relabelings of the physical axes (their perms on a
+// Cartesian case, none on a curvilinear one, whose momentum and tau stay in
+// the caller's x, y, z while only the index axes move).
+void face_vmap(const XSet &xs, int vb, int vn, const int dst_ax[3], const int src_ax[3], int8_t vmap[kIfaceMaxVar]) {
   for (int v = 0; v < kIfaceMaxVar; v++) vmap[v] = (int8_t)v;
   if (!xs.visc) {
-    for (int v = 0; v < 5; v++) vmap[v] = (int8_t)kVarMap[pn][caller_var(pb, v)];
+    for (int v = 0; v < 5; v++) vmap[v] = (int8_t)kVarMap[vn][caller_var(vb, v)];
     return;
   }
   for (int q = 0; q < 6; q++)
-    vmap[q] = (int8_t)tau_index(axis_between(pb, pn, kTauPair[q][0]), axis_between(pb, pn, kTauPair[q][1]));
-  for (int d = 0; d < 3; d++) vmap[6 + d] = (int8_t)(6 + axis_between(pb, pn, d));
+    vmap[q] = (int8_t)tau_index(axis_between(vb, vn, kTauPair[q][0]), axis_between(vb, vn, kTauPair[q][1]));
+  for (int d = 0; d < 3; d++) vmap[6 + d] = (int8_t)(6 + axis_between(vb, vn, d));
   if (xs.radii) {
     const int off = NTAU + (xs.sensor ? 2 : 0);
     int meet[3];
@@ -682,7 +688,7 @@ bool build_tasks(const mgpu_case &c, const std::vector<BlockData> &bds, int ng, 
       if (!c.curvilinear)
         for (int x = 0; x < 3; x++)
           if (src_ax[x] != axis_between(bds[b].perm, bds[nb].perm, dst_ax[x])) return false;
-      face_vmap(xs, bds[b].perm, bds[nb].perm, dst_ax, src_ax, t.vmap);
+      face_vmap(xs, bds[b].vperm, bds[nb].vperm, dst_ax, src_ax, t.vmap);
       for (int x = 0; x < 3; x++) {
         t.n[x] = ext[ord[x]];
         t.dst_s[x] = (int)dv[ord[x]];
@@ -1586,24 +1592,33 @@ int put_metrics_c3(mgpu_handle *h, int block, const double *const *m, const doub
   if (!m) return fail(h, MGPU_EINVAL, "null metric table");
   BlockData &bd = h->b[block];
   const size_t bytes = sizeof(double) * (size_t)bd.g.n;
-  for (int q = 0; q < 10; q++) {
+  // a relabeled block (walls on its i- or k-faces): the rows of its stored
+  // index axes, each array in the stored axis order; the physical components
+  // x, y, z and 1/J keep their places
+  std::vector<double> tmp(bd.perm ? (size_t)bd.g.n : 0);
+  auto stored = [&](const double *const *src, int q) -> const double * {
+    const double *a = q < 9 ? src[3 * caller_axis(bd.perm, q / 3) + q % 3] : src[q];
+    if (!bd.perm) return a;
+    relabel_array(bd.g, bd.perm, a, tmp.data(), true);
+    return tmp.data();
+  };
+  for (int q = 0; q < 10; q++)
     if (!m[q]) return fail(h, MGPU_EINVAL, "metric %d is null", q);
-    HIPCK(h, hipMemcpy(bd.m[q], m[q], bytes, hipMemcpyHostToDevice));
-  }
-  if (bd.m32[0]) {
-    // the mixed path's FP32 copies of the flux metric terms
-    std::vector<float> f((size_t)bd.g.n);
-    for (int q = 0; q < 9; q++) {
-      for (size_t e = 0; e < f.size(); e++) f[e] = (float)m[q][e];
+  std::vector<float> f(bd.m32[0] ? (size_t)bd.g.n : 0);
+  for (int q = 0; q < 10; q++) {
+    const double *a = stored(m, q);
+    HIPCK(h, hipMemcpy(bd.m[q], a, bytes, hipMemcpyHostToDevice));
+    if (bd.m32[0] && q < 9) {
+      // the mixed path's FP32 copies of the flux metric terms
+      for (size_t e = 0; e < f.size(); e++) f[e] = (float)a[e];
       HIPCK(h, hipMemcpy(bd.m32[q], f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
     }
   }
   if (h->c.iorder_visc) {
     if (!mv) return fail(h, MGPU_EINVAL, "viscous metric table is null");
-    for (int q = 0; q < 10; q++) {
+    for (int q = 0; q < 10; q++)
       if (!mv[q]) return fail(h, MGPU_EINVAL, "viscous metric %d is null", q);
-      HIPCK(h, hipMemcpy(bd.mv[q], mv[q], bytes, hipMemcpyHostToDevice));
-    }
+    for (int q = 0; q < 10; q++) HIPCK(h, hipMemcpy(bd.mv[q], stored(mv, q), bytes, hipMemcpyHostToDevice));
   }
   bd.have_metrics = true;
   return 0;
@@ -1653,7 +1668,7 @@ int put_state(mgpu_handle *h, int block, const double *const *src_in) {
   std::vector<double> tmp[5];
   const double *sp[5];
   for (int v = 0; v < 5; v++) {
-    const double *u = src_in[caller_var(bd.perm, v)];
+    const double *u = src_in[caller_var(bd.vperm, v)];
     if (bd.perm) {
       tmp[v].resize((size_t)bd.g.n);
       relabel_array(bd.g, bd.perm, u, tmp[v].data(), true);
@@ -1684,7 +1699,7 @@ int put_state(mgpu_handle *h, int block, const double *const *src_in) {
     }
     // the mean in the caller's frame, then in each block's stored axis order
     double mc[5];
-    for (int v = 0; v < 5; v++) mc[caller_var(bd.perm, v)] = mean[v];
+    for (int v = 0; v < 5; v++) mc[caller_var(bd.vperm, v)] = mean[v];
     auto ref_of = [&](int perm) {
       RefState r{};
       r.r = mc[caller_var(perm, 0)];
@@ -1700,9 +1715,9 @@ int put_state(mgpu_handle *h, int block, const double *const *src_in) {
       return r;
     };
     if (h->shared_ref)
-      for (auto &b : h->b) b.ref = ref_of(b.perm);
+      for (auto &b : h->b) b.ref = ref_of(b.vperm);
     else
-      bd.ref = ref_of(bd.perm);
+      bd.ref = ref_of(bd.vperm);
     h->tables_ready = false;
   }
   return 0;
@@ -1724,7 +1739,7 @@ int get_state(mgpu_handle *h, int block, double *const *dst) {
   std::vector<double> tmp((size_t)bd.g.n);
   for (int v = 0; v < 5; v++) {
     HIPCK(h, hipMemcpy(tmp.data(), bd.st[h->cur][v], bytes, hipMemcpyDeviceToHost));
-    relabel_array(bd.g, bd.perm, tmp.data(), dst[caller_var(bd.perm, v)], false);
+    relabel_array(bd.g, bd.perm, tmp.data(), dst[caller_var(bd.vperm, v)], false);
   }
   return 0;
 }
@@ -2023,7 +2038,10 @@ int mgpu_create(const mgpu_case *c, mgpu_handle **out) {
   if (rc) return rc;
   for (int bI = 0; bI < c->nblocks && !rc; bI++) {
     rc = handle_add_block(h, stored[bI], *c);
-    if (!rc) h->b[bI].perm = perm[bI];
+    if (!rc) {
+      h->b[bI].perm = perm[bI];
+      h->b[bI].vperm = c->curvilinear ? 0 : perm[bI];
+    }
   }
   if (!rc) rc = setup_exchange(h);
   if (rc) {
@@ -2103,8 +2121,12 @@ int mgpu_set_wall_c3(mgpu_handle *h, int block, int face, const double *const *c
   if (!coef) return fail(h, MGPU_EINVAL, "null argument");
   BlockData &bd = h->b[block];
   if (h->c.curvilinear != MGPU_GEOM_C3) return fail(h, MGPU_ESTATE, "case is not 3-D curvilinear: use mgpu_set_wall");
-  if (face != 2 && face != 3) return fail(h, MGPU_EINVAL, "face %d: curvilinear walls are j-faces (2, 3)", face);
-  const int s = face - 2;
+  if (face < 0 || face > 5) return fail(h, MGPU_EINVAL, "face %d out of range", face);
+  // the face as stored (a j-face: walls on the i- or k-faces are stored
+  // there, with the face's two tangential axes in the same order)
+  const int fs = kFaceMap[bd.perm][face];
+  if (fs != 2 && fs != 3) return fail(h, MGPU_ESTATE, "block %d face %d is no wall", block, face);
+  const int s = fs - 2;
   if (!bd.wc3[s]) return fail(h, MGPU_ESTATE, "block %d face %d is no wall", block, face);
   const size_t nf = (size_t)bd.g.nx * bd.g.nz;
   std::vector<double> tmp(8 * nf);
@@ -2129,8 +2151,10 @@ int mgpu_set_wall_c3_slip(mgpu_handle *h, int block, int face, const double *con
   if (!t) return fail(h, MGPU_EINVAL, "null argument");
   BlockData &bd = h->b[block];
   if (h->c.curvilinear != MGPU_GEOM_C3) return fail(h, MGPU_ESTATE, "case is not 3-D curvilinear");
-  if (face != 2 && face != 3) return fail(h, MGPU_EINVAL, "face %d: curvilinear walls are j-faces (2, 3)", face);
-  const int s = face - 2;
+  if (face < 0 || face > 5) return fail(h, MGPU_EINVAL, "face %d out of range", face);
+  const int fs = kFaceMap[bd.perm][face];
+  if (fs != 2 && fs != 3) return fail(h, MGPU_ESTATE, "block %d face %d is no slip wall", block, face);
+  const int s = fs - 2;
   if (!bd.wt3[s]) return fail(h, MGPU_ESTATE, "block %d face %d is no slip wall", block, face);
   const size_t nf = (size_t)bd.g.nx * bd.g.nz;
   std::vector<double> tmp(6 * nf);
@@ -2351,6 +2375,7 @@ int mgpu_exchange_plan(const mgpu_case *c, int set, int64_t *map, int64_t cap, i
     g.sy = (idx_t)g.sx * (idx_t)(bl.ny + 2 * G);
     g.n = (long)g.sy * (bl.nz + 2 * G);
     bds[b].perm = perm[b];
+    bds[b].vperm = c->curvilinear ? 0 : perm[b];
   }
   XSet xs;
   xs.visc = set == 1;
@@ -2370,10 +2395,10 @@ int mgpu_exchange_plan(const mgpu_case *c, int set, int64_t *map, int64_t cap, i
     return x[m[0]] + Q[m[0]] * (x[m[1]] + Q[m[1]] * x[m[2]]);
   };
   auto plan_var = [&](int b, int v) -> int {
-    const int p = perm[b];
-    if (set == 0) return caller_var(p, v);
-    if (v < 6) return tau_index(caller_axis(p, kTauPair[v][0]), caller_axis(p, kTauPair[v][1]));
-    if (v < NTAU) return 6 + caller_axis(p, v - 6);
+    const int p = perm[b], vp = bds[b].vperm;
+    if (set == 0) return caller_var(vp, v);
+    if (v < 6) return tau_index(caller_axis(vp, kTauPair[v][0]), caller_axis(vp, kTauPair[v][1]));
+    if (v < NTAU) return 6 + caller_axis(vp, v - 6);
     if (v < NTAU + 2) return v;
     return NTAU + 2 + caller_axis(p, v - NTAU - 2);
   };

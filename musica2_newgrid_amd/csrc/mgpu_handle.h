@@ -59,6 +59,9 @@ struct BlockData {
   // on its j- and k-faces (perm 3) with its j, k, i stored as i, j, k; the state,
   // Cartesian metrics and wall coefficients are exchanged with it at the API.
   int perm = 0;
+  // the relabeling of the physical axes (momentum, tau): perm on a Cartesian
+  // case, none on a curvilinear one (only its index axes and metric rows move)
+  int vperm = 0;
 };
 
 struct ProfRec {

@@ -226,16 +226,16 @@ def test_characteristic_faces_are_checked():
 
 
 def test_curvilinear_walls_are_checked():
-    """Walls on 3-D curvilinear blocks sit on their j-faces (no axis
-    relabeling of c3 metrics); 2.5-D blocks take no walls; refused before any
-    device work."""
+    """Walls on 3-D curvilinear blocks sit on the faces of one axis (the
+    kernels close the stored j-faces only, where i- or k-walls are relabeled);
+    2.5-D blocks take no walls; refused before any device work."""
     c = cases.vortex_channel(24, 20, 8)
     blk = {"nx": c.nx, "ny": c.ny, "nz": c.nz,
-           "kind": [_lib.FACE_WALL, _lib.FACE_WALL, _lib.FACE_PERIODIC, _lib.FACE_PERIODIC, _lib.FACE_PERIODIC,
+           "kind": [_lib.FACE_WALL, _lib.FACE_WALL, _lib.FACE_WALL, _lib.FACE_WALL, _lib.FACE_PERIODIC,
                     _lib.FACE_PERIODIC],
            "nbr_block": [0] * 6, "nbr_face": [1, 0, 3, 2, 5, 4], "orient": [0] * 6}
     rc, msg = _create_case(c, [blk])
-    assert rc == _lib.EINVAL and "j-faces only" in msg, msg
+    assert rc == _lib.EINVAL and "faces of one axis" in msg, msg
     c.curvilinear = 2
     blk["kind"] = [_lib.FACE_PERIODIC, _lib.FACE_PERIODIC, _lib.FACE_WALL, _lib.FACE_WALL, _lib.FACE_PERIODIC,
                    _lib.FACE_PERIODIC]
