@@ -13,7 +13,8 @@ the caller's frame: the caller's axes are the physical axes on a Cartesian
 case).  Covered: two-block rings with walls on any axis, or on two axes, of either
 block, all reversal codes; a c3 ring whose blocks' axes are signed permutations of each
 other (all eight orientation codes), where the spectral radius along an
-index axis comes from the neighbour's index axis it meets.
+index axis comes from the neighbour's index axis it meets; c3 rings of blocks
+walled on their i- or k-faces.
 """
 import ctypes as C
 import itertools
@@ -132,6 +133,17 @@ def test_walled_rings_match_the_oracle_face_copies(wa, wb, x, set_):
     c = cases.tgv(12)
     for o in (0, 2, 4, 6):
         check_plan(c, ring(wa, wb, x, o), set_)
+
+
+@pytest.mark.parametrize("w,x", [(0, 1), (0, 2), (2, 0), (2, 1)])
+@pytest.mark.parametrize("set_", [0, 1])
+def test_c3_walled_rings_match_the_oracle_face_copies(w, x, set_):
+    """Curvilinear blocks walled on their i- or k-faces (stored with that axis
+    exchanged with j): momentum and tau keep the caller's x, y, z, only the
+    index axes (and the spectral radii along them) move."""
+    c = cases.vortex_wavy(12, 12, 12)
+    for o in (0, 2, 4, 6):
+        check_plan(c, ring(w, w, x, o), set_)
 
 
 def test_c3_permuted_ring_matches_the_oracle_face_copies():
