@@ -13,7 +13,7 @@ the curvilinear channel turned -- two index axes exchanged and the two
 physical axes with them -- is the same flow turned: the GPU on the turned
 channel against the oracle on the channel, turned back, FP64 within 1e-12
 relative (every filter, slip and no-slip walls), and the mixed path against
-FP64 on the turned channel."""
+FP64 on the turned channel; graph replay and autotuned tiles."""
 import numpy as np
 import pytest
 
@@ -69,7 +69,7 @@ def _turn(c, a, b):
     return t
 
 
-def _untern(U, a, b):
+def _unturn(U, a, b):
     """A state of the turned case back in the frame of the original."""
     sw = [0, 1, 2]
     sw[a], sw[b] = b, a
@@ -101,7 +101,7 @@ def test_turned_curvilinear_channel_matches_oracle(axes, filt, slip):
     wf = [f for f in range(6) if t.faces[f] in (2, 5)]
     print(f"c3 channel turned {axes} ({t.nx}x{t.ny}x{t.nz}, walls on faces {wf}) filter={filt} slip={slip}, "
           f"{n} steps", flush=True)
-    Ug = _untern(_gpu(t, n), *axes)
+    Ug = _unturn(_gpu(t, n), *axes)
     Uo = _oracle(c, n)
     e = relerr(Ug, Uo)
     print(f"relerr {e:.3e}", flush=True)
@@ -118,3 +118,26 @@ def test_turned_curvilinear_channel_mixed():
     e = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
     print(f"mixed vs FP64 rho Linf {e:.3e}", flush=True)
     assert np.all(np.isfinite(a)) and e < 1e-6
+
+
+def test_turned_curvilinear_channel_graph_and_tuned():
+    """Walls on the i-faces: graph replay equal to eager launches bit for bit,
+    autotuned tiles against the oracle."""
+    from musica2_newgrid_amd.solver import GpuSolver
+    c = cases.vortex_channel(40, 32, 12, slip=True)
+    t = _turn(c, 0, 1)
+    n = 8
+    print(f"c3 channel with walls on the i-faces {t.nx}x{t.ny}x{t.nz}: graph vs eager, tuned vs oracle, {n} steps",
+          flush=True)
+    out = []
+    for eager, tune in ((False, False), (True, False), (False, True)):
+        with GpuSolver(t, eager=eager) as s:
+            if tune:
+                s.autotune()
+            s.step(n)
+            out.append(s.get_state())
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(gr.interior(a), gr.interior(b))
+    e = relerr(_unturn(out[2], 0, 1), _oracle(c, n))
+    print(f"tuned relerr {e:.3e}", flush=True)
+    assert e < 1e-12
