@@ -13,7 +13,8 @@ the curvilinear channel turned -- two index axes exchanged and the two
 physical axes with them -- is the same flow turned: the GPU on the turned
 channel against the oracle on the channel, turned back, FP64 within 1e-12
 relative (every filter, slip and no-slip walls), and the mixed path against
-FP64 on the turned channel; graph replay and autotuned tiles."""
+FP64 on the turned channel; graph replay and autotuned tiles; the turned
+channel in two blocks joined along k against the oracle's two blocks."""
 import numpy as np
 import pytest
 
@@ -140,4 +141,52 @@ def test_turned_curvilinear_channel_graph_and_tuned():
         assert np.array_equal(gr.interior(a), gr.interior(b))
     e = relerr(_unturn(out[2], 0, 1), _oracle(c, n))
     print(f"tuned relerr {e:.3e}", flush=True)
+    assert e < 1e-12
+
+
+def _halves(c):
+    """c cut in two blocks along k (its periodic k-faces become interfaces)."""
+    G = gr.G
+    h = c.nz // 2
+    subs, blocks = [], []
+    for b in range(2):
+        lo, hi = b * h, b * h + h + 2 * G
+        sub = cases.Case(f"{c.name}_k{b}", c.nx, c.ny, h, curvilinear=True, iorder_visc=c.iorder_visc,
+                         filter=c.filter, ducros=c.ducros, diffscale=c.diffscale, chi=c.chi, dt=c.dt, nrk=c.nrk,
+                         crk=c.crk, m=[np.ascontiguousarray(x[lo:hi]) for x in c.m],
+                         mv=[np.ascontiguousarray(x[lo:hi]) for x in c.mv] if c.mv is not None else None,
+                         faces=tuple(c.faces[:4]) + (1, 1), adiabatic=c.adiabatic, t_wall=c.t_wall, wall2=c.wall2)
+        sub.U = [np.ascontiguousarray(x[lo:hi]) for x in c.U]
+        sub.wall_c3 = {f: [np.ascontiguousarray(x[b * h:(b + 1) * h]) for x in s] for f, s in c.wall_c3.items()}
+        subs.append(sub)
+        kinds = [int(k) for k in c.faces[:4]] + [_lib.FACE_INTERFACE] * 2
+        blocks.append(dict(nx=c.nx, ny=c.ny, nz=h, kind=kinds, nbr_block=[b, b, b, b, 1 - b, 1 - b],
+                           nbr_face=[1, 0, 3, 2, 5, 4], orient=[0] * 6,
+                           wall=[max(0, k - 1) for k in kinds[:4]] + [0, 0]))
+    return subs, blocks
+
+
+def test_turned_curvilinear_channel_in_two_blocks():
+    """Walls on the i-faces of two blocks joined along k, against the oracle's
+    two blocks of the channel, turned back."""
+    from musica2_newgrid_amd.solver import GpuSolver
+    from oracle import oracle as orc
+    c = cases.vortex_channel(40, 32, 12, slip=True)
+    subs, blocks = _halves(c)
+    tsubs = [_turn(s, 0, 1) for s in subs]
+    tblocks = []
+    for blk, ts in zip(blocks, tsubs):
+        kinds = list(blk["kind"])
+        tblocks.append(dict(nx=ts.nx, ny=ts.ny, nz=ts.nz, kind=[kinds[2], kinds[3], kinds[0], kinds[1]] + kinds[4:],
+                            nbr_block=blk["nbr_block"], nbr_face=blk["nbr_face"], orient=blk["orient"]))
+    t = _turn(c, 0, 1)
+    n = 8
+    print(f"c3 channel with walls on the i-faces in 2 blocks along k ({t.nx}x{t.ny}x{t.nz}), {n} steps", flush=True)
+    with GpuSolver(t, topology=tblocks, block_cases=tsubs) as s:
+        s.step(n)
+        Ug = [_unturn(U, 0, 1) for U in s.get_states()]
+    Uo = [bc.copy_state() for bc in subs]
+    orc.step_multi(c, blocks, subs, Uo, n)
+    e = max(relerr(a, b) for a, b in zip(Ug, Uo))
+    print(f"relerr {e:.3e}", flush=True)
     assert e < 1e-12
